@@ -1,0 +1,215 @@
+"""Benchmark: cloth vertex-substeps/s of the XPBD substep path on one MI355X.
+
+A step is one frame: every cloth of the workload advances by one
+ssbl_step_solver_ex(substeps, iterations) worth of work.  Two forms run on the
+same GPU and the same inputs:
+  plain   -- one ABI41 handle per cloth, stepped one after another (the
+             reference's per-slot loop, session_manager.py:2528-2591);
+  batched -- the same handles stepped together through ssbl_ext_step_batch
+             (one launch per solver phase covers every cloth).
+`value` is the batched form's whole-job vertex-substeps/s with all state
+resident in HBM.  Per-frame latency percentiles come from a separate pass that
+synchronises after every frame.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+import __graft_entry__  # noqa: E402
+
+__graft_entry__.load_package()
+from ssbl_xpbd_amd import native_backend as nb, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pass_bytes_per_vertex(scene) -> float:
+    """Algorithmic bytes of one fused constraint/contact pass per vertex.
+
+    Unique compulsory traffic: read position (12) + inverse mass (4) + previous
+    position (12) + velocity (12) + the incident edges' records (12 bytes per
+    edge: two indices and a rest length, E/V edges per vertex), write position,
+    previous position and velocity (36).
+    """
+    v = scene.vertex_count
+    e = sum(len(c.edges) for c in scene.cloths)
+    return 12 + 4 + 12 + 12 + 12.0 * e / v + 36
+
+
+def make_workload(name: str, seed: int):
+    if name == "plain_32x32":
+        return scenes.config1(seed=seed)
+    if name == "batched_64_uneven":
+        return scenes.config3(seed=seed)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def create(scene, lib=None):
+    return [nb.NativeXpbdSolver(c, o, scene.static_triangles, lib=lib) for c, o in zip(scene.cloths, scene.options)]
+
+
+def run_plain(solvers, scene, frames):
+    for _ in range(frames):
+        for s in solvers:
+            s.step(scene.substeps, scene.iterations, diagnostics=False, synchronize=False)
+
+
+def run_batched(batch, scene, frames):
+    for _ in range(frames):
+        batch.step(scene.substeps, scene.iterations)
+
+
+def timed(fn, lib):
+    nb.synchronize(lib)
+    t0 = time.perf_counter()
+    fn()
+    nb.synchronize(lib)
+    return time.perf_counter() - t0
+
+
+def latencies(step_frame, lib, frames):
+    out = []
+    for _ in range(frames):
+        nb.synchronize(lib)
+        t0 = time.perf_counter()
+        step_frame()
+        nb.synchronize(lib)
+        out.append((time.perf_counter() - t0) * 1e3)
+    return np.asarray(out)
+
+
+def kernel_stats(lib, fn):
+    lib.ssbl_ext_kernel_timing(1)
+    fn()
+    nb.synchronize(lib)
+    lib.ssbl_ext_kernel_timing(0)
+    buf = __import__("ctypes").create_string_buffer(1 << 16)
+    if not lib.ssbl_ext_kernel_stats(buf, len(buf)):
+        return {}
+    return json.loads(buf.value.decode())
+
+
+def cpu_baseline(scene, budget_s: float):
+    """The CPU oracle (FP32, single thread) on a bounded sample of the workload."""
+    lib = nb.load_library(str(ROOT / "oracle" / "liboracle_abi41.so"))
+    order = np.argsort([len(c.positions_world) for c in scene.cloths])
+    picked, work = [], 0
+    for k in order[::max(1, len(order) // 8)]:
+        picked.append(int(k))
+    solvers = [nb.NativeXpbdSolver(scene.cloths[k], scene.options[k], scene.static_triangles, lib=lib) for k in picked]
+    verts = sum(s.vertex_count for s in solvers)
+    frames = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and frames < scene.frames:
+        for s in solvers:
+            s.step(scene.substeps, scene.iterations, diagnostics=False, synchronize=False)
+        frames += 1
+    dt = time.perf_counter() - t0
+    work = verts * scene.substeps * frames
+    return {"value": work / dt, "unit": "vertex-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP32 single thread, {len(solvers)} of {len(scene.cloths)} cloths ({verts} vertices) x {frames} frames"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30, help="timed frames")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=os.environ.get("SSBL_BENCH_WORKLOAD", "batched_64_uneven"))
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--latency-frames", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        raise SystemExit(2)
+
+    scene = make_workload(args.workload, args.seed)
+    lib = nb.load_library()
+    print(f"# workload {scene.name}: {len(scene.cloths)} cloths, {scene.vertex_count} vertices, "
+          f"{scene.substeps}x{scene.iterations}; device {lib.ssbl_ext_backend_name().decode()}", file=sys.stderr, flush=True)
+    plain = create(scene)
+    members = create(scene)
+    batch = nb.SolverBatch(members) if len(members) > 1 else None
+    work_per_frame = scene.vertex_count * scene.substeps
+
+    run_plain(plain, scene, args.warmup)
+    t_plain = timed(lambda: run_plain(plain, scene, args.steps), lib)
+    print(f"# plain: {args.steps} frames in {t_plain:.3f} s", file=sys.stderr, flush=True)
+    if batch is not None:
+        run_batched(batch, scene, args.warmup)
+        t_batch = timed(lambda: run_batched(batch, scene, args.steps), lib)
+        step_frame = lambda: run_batched(batch, scene, 1)  # noqa: E731
+    else:
+        t_batch = t_plain
+        step_frame = lambda: run_plain(plain, scene, 1)  # noqa: E731
+    print(f"# batched: {args.steps} frames in {t_batch:.3f} s", file=sys.stderr, flush=True)
+    lat_b = latencies(step_frame, lib, args.latency_frames)
+    lat_p = latencies(lambda: run_plain(plain, scene, 1), lib, args.latency_frames)
+
+    stats = kernel_stats(lib, step_frame)
+    dominant = max(stats.items(), key=lambda kv: kv[1]["total_ms"]) if stats else ("none", {"launches": 1, "total_ms": 0.0})
+    avg_ms = dominant[1]["total_ms"] / max(dominant[1]["launches"], 1)
+    bytes_per_launch = pass_bytes_per_vertex(scene) * scene.vertex_count
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    value = args.steps * work_per_frame / t_batch
+    plain_value = args.steps * work_per_frame / t_plain
+    line = {
+        "metric": "cloth vertex-substeps/s, batched vs plain on same GPU; p50/p99 step latency",
+        "value": value,
+        "unit": "vertex-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_batch / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded grid cloths)",
+        "config": {
+            "workload": scene.name,
+            "cloths": len(scene.cloths),
+            "vertices": scene.vertex_count,
+            "substeps": scene.substeps,
+            "iterations": scene.iterations,
+            "cross_cloth_collision": False,
+            "plain_vertex_substeps_per_s": plain_value,
+            "batched_over_plain": value / plain_value,
+            "frame_ms_p50_batched": float(np.percentile(lat_b, 50)),
+            "frame_ms_p99_batched": float(np.percentile(lat_b, 99)),
+            "frame_ms_p50_plain": float(np.percentile(lat_p, 50)),
+            "frame_ms_p99_plain": float(np.percentile(lat_p, 99)),
+            "kernels": stats,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant[0],
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": avg_ms,
+        },
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,68 @@
+/*
+ * ssbl_xpbd_hip_ext.h -- entry points this library adds beside the ABI41
+ * contract.  None of them has a counterpart in the reference; they serve
+ * callers the reference has no interface for:
+ *   - parity tests, which need more state than ssbl_download_positions gives
+ *     (prev/vel, self-collision hash cells and candidate pairs, the static SDF);
+ *   - batched and served stepping, which advances many handles with one
+ *     launch per solver phase instead of one ssbl_step_solver_ex per handle;
+ *   - measurement, which needs device-side kernel durations.
+ * The same symbols are exported by the CPU oracle library so one host harness
+ * drives both.  Conventions follow the ABI41 functions: 1 = ok, 0 = error with
+ * ssbl_last_error() set.
+ */
+#ifndef SSBL_XPBD_HIP_EXT_H
+#define SSBL_XPBD_HIP_EXT_H
+
+#include "ssbl_xpbd_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Direct-stretch semantics.  0 (default): the reference's single uncoloured
+ * pass (ssbl_xpbd_cuda_abi41.cu:1927-1980) made deterministic as a Jacobi
+ * sweep -- every edge reads the positions from before the pass and each vertex
+ * sums its corrections in ascending edge order.  1: graph-coloured
+ * Gauss-Seidel, one pass per edge colour in colour order. */
+enum { SSBL_EXT_STRETCH_JACOBI = 0, SSBL_EXT_STRETCH_COLORED = 1 };
+SSBL_API int ssbl_ext_set_stretch_mode(void* handle, int mode);
+
+/* Copies pos, prev and vel (each float[3V], any pointer may be NULL). */
+SSBL_API int ssbl_ext_download_state(void* handle, float* pos, float* prev, float* vel, int max_floats);
+
+/* Self-collision introspection, valid after a step that ran self collision.
+ * cells: int[3V] vertex cell coordinates of the last vertex-hash build.
+ * counts: uint[V], indices: uint[V*32] -- the last vertex-vertex candidate table. */
+SSBL_API int ssbl_ext_self_hash_cells(void* handle, int* cells, int max_ints);
+SSBL_API int ssbl_ext_self_neighbors(void* handle, unsigned int* counts, unsigned int* indices, int max_vertices);
+
+/* Static SDF grid: dims int[3], origin_and_voxel float[4]; values float[dx*dy*dz]
+ * (values may be NULL to query the shape).  Returns 0 when no SDF is built. */
+SSBL_API int ssbl_ext_static_sdf(void* handle, int* dims, float* origin_and_voxel, float* values, int max_values);
+
+/* Batched stepping: every handle advances by the same substeps/iterations; the
+ * result of each handle equals what ssbl_step_solver_ex on that handle alone
+ * produces.  The batch refers to the handles; it does not own them. */
+SSBL_API void* ssbl_ext_create_batch(void* const* handles, int count);
+SSBL_API int ssbl_ext_destroy_batch(void* batch);
+SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int force_sync);
+
+/* Waits for all work the library queued. */
+SSBL_API int ssbl_ext_synchronize(void);
+
+/* Device-side kernel timing.  While enabled, every launch is bracketed by a
+ * pair of device events on the stream it runs on.  ssbl_ext_kernel_stats
+ * writes a JSON object {"kernel": {"launches": n, "total_ms": t}, ...} and
+ * resets the counters. */
+SSBL_API int ssbl_ext_kernel_timing(int enable);
+SSBL_API int ssbl_ext_kernel_stats(char* json_out, int capacity);
+
+/* Implementation name and device description, e.g. "hip gfx950 ...". */
+SSBL_API const char* ssbl_ext_backend_name(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* SSBL_XPBD_HIP_EXT_H */

@@ -1,0 +1,135 @@
+// xpbd_device.hpp -- device-side data views and kernel launch interface of the
+// gfx950 XPBD cloth library.
+//
+// HBM layout of one cloth (all arrays 256-byte aligned, vertex arrays padded to
+// a multiple of the 256-vertex tile):
+//   pos[2]  float4 x V  ping-pong positions; .w = effective inverse mass
+//                       (0 for pinned/kinematic vertices)
+//   prev    float4 x V  previous positions
+//   vel     float4 x V  velocities
+//   rest    float4 x V  rest positions; .w = inverse mass as uploaded
+//   adj     int2 x 2E   vertex -> incident stretch edges, ascending edge index:
+//                       {other vertex | first-endpoint bit 31, rest length bits}
+// Per-vertex passes (integrate, the fused constraint/contact pass, velocity)
+// read one float4 per array per vertex -- 16-byte coalesced loads -- and gather
+// neighbour positions through L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "ssbl_xpbd_hip.h"
+
+namespace ssbl {
+
+constexpr int kTile = 256;              // vertices per workgroup tile
+constexpr int kCountSlots = 38;         // diagnostic counters per cloth
+constexpr int kNeighborSlots = 32;      // vertex-vertex candidates per vertex
+constexpr int kBucketSlots = 32;        // entries a query sees per hash bucket
+constexpr int kMaxForceFields = 64;
+constexpr double kFixedScale = 4294967296.0;  // 2^32: self-collision accumulators
+
+enum CountSlot {
+    kCountSoft = 0, kCountExactImpulse = 1, kCountEdgeEdge = 2, kCountHardFallbacks = 3,
+    kCountSelfCandidates = 9, kCountSelfOverflow = 10, kCountStaticSdf = 11, kCountLraTacks = 12,
+    kCountBendingWings = 13, kCountTackGuards = 14, kCountBendingGuards = 15, kCountFriction = 33
+};
+
+// Work of the fused per-vertex pass, as a bit mask.
+enum PassOps : unsigned {
+    kOpStretch = 1u << 0,     // Jacobi stretch gather (src -> dst)
+    kOpPin = 1u << 1,         // pin projection
+    kOpAnalytic = 1u << 2,    // ground / sphere / wall contacts
+    kOpSdf = 1u << 3,         // static SDF contact
+    kOpPinPolish = 1u << 4,   // second pin projection after contacts
+    kOpVelocity = 1u << 5,    // velocity update (end of substep)
+    kOpLra = 1u << 6,         // LRA tether gather (src -> dst), instead of stretch
+    kOpCount = 1u << 7,       // count diagnostics
+    kOpLastIter = 1u << 8,    // last iteration of the substep (pin polish always runs)
+};
+
+// Everything a kernel needs to know about one cloth.  Lives in device memory
+// (a table of these for a batch); host keeps a mirror.
+struct ClothView {
+    int nv, ne, nt, nbend, nlra, npin, nfield;
+    int pad0;
+    float4* pos0;
+    float4* pos1;
+    float4* prev;
+    float4* vel;
+    float4* rest;
+    const int* adj_off;        // V+1
+    const int2* adj;           // stretch incidences
+    const int* lra_off;        // V+1
+    const int4* lra_adj;       // {other | first bit, rest bits, lra index, 0}
+    const int* tri_off;        // V+1 incident triangles (surface normal)
+    const int* tri_inc;
+    const int4* tris;          // {a, b, c, 0}
+    const int2* bends;         // bend pairs
+    const float* bend_rest;
+    const int* pin_off;        // V+1 -> pins of that vertex (ascending pin index)
+    const int* pin_ids;
+    const float4* pins;        // {target xyz, relaxation}; relaxation 2 = hard, <=0 ignored
+    const float* area_share;
+    const float* rest_scale;
+    const SsblXpbdForceField* fields;
+    const float* sdf;
+    const float* static_tris;     // 9 floats per static triangle
+    int nstatic;
+    int pad2;
+    unsigned long long* counts;   // kCountSlots
+    float* max_smoothed;          // self-collision max averaged delta
+    // parameters
+    float dt_sub;
+    float damping;
+    float gravity[3];
+    float stretch_alpha;          // compliance / max(dt^2, eps), per substep
+    float bend_alpha;
+    float lra_compliance;
+    float collision_margin;
+    float thickness;
+    float friction;
+    float tangent_damping;
+    int use_pressure;
+    float pressure_strength;
+    float pressure_accel_scale;
+    float pressure_impulse_rest_scale;
+    float pressure_velocity_rest_scale;
+    float pressure_velocity_length_scale;
+    SsblXpbdRuntimeColliders colliders;
+    int sdf_ready;
+    int sdf_dim[3];
+    float sdf_origin[3];
+    float sdf_voxel;
+    int need_normal;
+    int contacts_on;              // analytic colliders or a static SDF present
+};
+
+struct Tile { int cloth, begin, end, pad; };
+
+// Kernel launch wrappers (xpbd_kernels.hip).  All launches go to `stream`.
+struct LaunchCtx {
+    hipStream_t stream;
+    const ClothView* views;     // device table
+    const Tile* tiles;          // device tile table
+    int ntiles;
+};
+
+void launch_integrate(const LaunchCtx& c, int parity, unsigned ops);
+void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops);
+void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops);
+void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float ox, float oy, float oz, float voxel,
+                      const float* tris, int ntris, int signed_sdf);
+void launch_fill_u32(hipStream_t s, unsigned* p, unsigned value, size_t n);
+
+// Self collision (xpbd_self.hip).
+struct SelfHashDev {
+    int buckets;
+    int* counts;      // buckets
+    int* offsets;     // buckets + 1
+    int* cursor;      // buckets
+    int* entries;     // capacity
+    int capacity;
+};
+
+}  // namespace ssbl

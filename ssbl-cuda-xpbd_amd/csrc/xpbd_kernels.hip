@@ -1,0 +1,721 @@
+// xpbd_kernels.hip -- per-vertex and per-colour kernels of the XPBD substep
+// for gfx950 (wave64, 256-thread workgroups, one 256-vertex tile per group).
+//
+// Reference path (native/src/ssbl_xpbd_cuda_abi41.cu, cited A41:<line>):
+// integrate A41:1869-1925, pins A41:3466-3509, direct stretch A41:1927-1980,
+// bend A41:2886-2970, LRA A41:2710-2803, analytic contacts A41:3511-3590 with
+// the response of A41:910-949, static SDF contact A41:1831-1867, velocity
+// update A41:5846-5884, SDF build A41:1787-1829.
+//
+// The reference runs each of these as its own launch over all vertices (or
+// edges, with float atomics).  Here every vertex-local step of one solver
+// iteration is fused into a single pass: the stretch projection is gathered per
+// vertex from an incidence list (a Jacobi sweep: neighbours are read from the
+// ping-pong source buffer), then pins, analytic and SDF contacts, the second
+// pin projection and, on the last iteration, the velocity update run on the
+// vertex's registers before one coalesced write.  A batch of cloths is a table
+// of ClothView + a tile table; every launch covers all tiles of all cloths.
+#include "xpbd_device.hpp"
+#include "xpbd_math.hpp"
+#include "xpbd_timing.hpp"
+
+namespace ssbl {
+
+// Tile index for this workgroup: consecutive tiles go to the same XCD (blocks
+// are dealt round-robin over the 8 XCDs), so a cloth's neighbouring tiles
+// share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+    const int b = static_cast<int>(blockIdx.x);
+    const int xcd = b & 7, local = b >> 3;
+    const int per = ntiles >> 3, extra = ntiles & 7;
+    return xcd * per + min(xcd, extra) + local;
+}
+
+// ------------------------------------------------------------ contacts
+// A41:910-949 velocity response of a static contact.
+__device__ inline void contact_response(const ClothView& c, F3 n, float proxy, F3& v, bool count) {
+    if (!finite3(n) || !finite3(v)) return;
+    const float vn = dot(v, n);
+    if (vn < 0.0f) v = v - mul(n, vn);
+    bool corrected = false;
+    F3 tangent = v - mul(n, dot(v, n));
+    float tl = len3(tangent);
+    const float damping = clamp01f(c.tangent_damping);
+    if (damping > 0.0f && tl > 1.0e-6f) { v = v - mul(tangent, damping); corrected = true; }
+    tangent = v - mul(n, dot(v, n));
+    tl = len3(tangent);
+    const float friction = fmaxf(c.friction, 0.0f);
+    if (friction > 0.0f && proxy > 0.0f && tl > 1.0e-6f) {
+        const float drop = fminf(tl, friction * proxy);
+        if (drop > 1.0e-7f) { v = v - mul(tangent, drop / fmaxf(tl, kEps)); corrected = true; }
+    }
+    if (count && corrected) atomicAdd(&c.counts[kCountFriction], 1ull);
+}
+
+// A41:3466-3509: every pin of vertex v in ascending pin order.
+__device__ inline void apply_pins(const ClothView& c, int v, bool pinned, F3& p, F3& q, F3& vel, float dt) {
+    const int b = c.pin_off[v], e = c.pin_off[v + 1];
+    for (int k = b; k < e; ++k) {
+        const float4 pin = c.pins[c.pin_ids[k]];
+        const float relax = pin.w;
+        if (relax < 0.0f) continue;                       // weight < 0.05 or not finite
+        const F3 target = f3(pin.x, pin.y, pin.z);
+        if (relax >= 2.0f || pinned) {                    // hard pin
+            p = target; q = target; vel = f3(0.0f, 0.0f, 0.0f);
+            continue;
+        }
+        if (relax <= 0.0f) continue;
+        const F3 corrected = p + mul(target - p, relax);
+        const F3 damped = mul(vel, fmaxf(1.0f - relax, 0.0f));
+        p = corrected;
+        q = corrected - mul(damped, fmaxf(dt, kEps));
+        vel = damped;
+    }
+}
+
+// A41:3511-3590
+__device__ inline void analytic_contacts(const ClothView& c, F3& p, F3& q, F3& vel, float dt, bool count) {
+    const float margin = fmaxf(c.collision_margin, 0.0f);
+    const float sdt = fmaxf(dt, kEps);
+    const SsblXpbdRuntimeColliders& col = c.colliders;
+    F3 v = vel;
+    bool hit = false;
+    if (col.use_ground && p.z < col.ground_height + margin) {
+        const float tz = col.ground_height + margin;
+        const float pen = tz - p.z;
+        const F3 n = f3(0.0f, 0.0f, 1.0f);
+        const float inward = fmaxf(-dot(v, n), 0.0f);
+        p.z = tz;
+        contact_response(c, n, inward + pen / sdt, v, count);
+        hit = true;
+    }
+    if (col.use_sphere && col.sphere_radius > 0.0f) {
+        const F3 center = f3(col.sphere_center[0], col.sphere_center[1], col.sphere_center[2]);
+        const F3 d = p - center;
+        const float dist = sqrtf(fmaxf(dot(d, d), kEps));
+        const float target = col.sphere_radius + margin;
+        if (dist < target) {
+            const F3 n = dist > kEps ? mul(d, 1.0f / dist) : f3(0.0f, 0.0f, 1.0f);
+            const float pen = target - dist;
+            const float inward = fmaxf(-dot(v, n), 0.0f);
+            p = center + mul(n, target);
+            contact_response(c, n, inward + pen / sdt, v, count);
+            hit = true;
+        }
+    }
+    if (col.use_wall) {
+        const F3 o = f3(col.wall_origin[0], col.wall_origin[1], col.wall_origin[2]);
+        F3 n = f3(col.wall_normal[0], col.wall_normal[1], col.wall_normal[2]);
+        const float nl = sqrtf(fmaxf(dot(n, n), kEps));
+        n = mul(n, 1.0f / nl);
+        const float gap = dot(p - o, n);
+        if (gap < margin) {
+            const float pen = margin - gap;
+            const float inward = fmaxf(-dot(v, n), 0.0f);
+            p = p + mul(n, margin - gap);
+            contact_response(c, n, inward + pen / sdt, v, count);
+            hit = true;
+        }
+    }
+    if (hit) { vel = v; q = p - mul(v, sdt); }
+}
+
+// A41:1645-1691 trilinear sample; false outside the grid.
+__device__ inline bool sdf_sample(const ClothView& c, F3 p, float* out) {
+    const int dx = c.sdf_dim[0], dy = c.sdf_dim[1], dz = c.sdf_dim[2];
+    if (dx < 2 || dy < 2 || dz < 2 || c.sdf_voxel <= 0.0f) return false;
+    const float inv = 1.0f / c.sdf_voxel;
+    const float gx = (p.x - c.sdf_origin[0]) * inv, gy = (p.y - c.sdf_origin[1]) * inv, gz = (p.z - c.sdf_origin[2]) * inv;
+    if (gx < 0.0f || gy < 0.0f || gz < 0.0f || gx > static_cast<float>(dx - 1) || gy > static_cast<float>(dy - 1) ||
+        gz > static_cast<float>(dz - 1))
+        return false;
+    const int x0 = min(max(static_cast<int>(floorf(gx)), 0), dx - 1), y0 = min(max(static_cast<int>(floorf(gy)), 0), dy - 1),
+              z0 = min(max(static_cast<int>(floorf(gz)), 0), dz - 1);
+    const int x1 = min(x0 + 1, dx - 1), y1 = min(y0 + 1, dy - 1), z1 = min(z0 + 1, dz - 1);
+    const float tx = gx - static_cast<float>(x0), ty = gy - static_cast<float>(y0), tz = gz - static_cast<float>(z0);
+    const float* s = c.sdf;
+    auto at = [&](int x, int y, int z) { return s[(static_cast<size_t>(z) * dy + y) * dx + x]; };
+    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+    const float c00 = lerp(at(x0, y0, z0), at(x1, y0, z0), tx), c10 = lerp(at(x0, y1, z0), at(x1, y1, z0), tx);
+    const float c01 = lerp(at(x0, y0, z1), at(x1, y0, z1), tx), c11 = lerp(at(x0, y1, z1), at(x1, y1, z1), tx);
+    *out = lerp(lerp(c00, c10, ty), lerp(c01, c11, ty), tz);
+    return isfinite(*out);
+}
+
+// A41:1693-1723
+__device__ inline float sdf_sample_clamped(const ClothView& c, F3 p) {
+    const int dx = c.sdf_dim[0], dy = c.sdf_dim[1], dz = c.sdf_dim[2];
+    const float inv = 1.0f / fmaxf(c.sdf_voxel, kEps);
+    const float gx = fminf(fmaxf((p.x - c.sdf_origin[0]) * inv, 0.0f), static_cast<float>(dx - 1));
+    const float gy = fminf(fmaxf((p.y - c.sdf_origin[1]) * inv, 0.0f), static_cast<float>(dy - 1));
+    const float gz = fminf(fmaxf((p.z - c.sdf_origin[2]) * inv, 0.0f), static_cast<float>(dz - 1));
+    const int x0 = min(max(static_cast<int>(floorf(gx)), 0), dx - 1), y0 = min(max(static_cast<int>(floorf(gy)), 0), dy - 1),
+              z0 = min(max(static_cast<int>(floorf(gz)), 0), dz - 1);
+    const int x1 = min(x0 + 1, dx - 1), y1 = min(y0 + 1, dy - 1), z1 = min(z0 + 1, dz - 1);
+    const float tx = gx - static_cast<float>(x0), ty = gy - static_cast<float>(y0), tz = gz - static_cast<float>(z0);
+    const float* s = c.sdf;
+    auto at = [&](int x, int y, int z) { return s[(static_cast<size_t>(z) * dy + y) * dx + x]; };
+    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+    return lerp(lerp(lerp(at(x0, y0, z0), at(x1, y0, z0), tx), lerp(at(x0, y1, z0), at(x1, y1, z0), tx), ty),
+                lerp(lerp(at(x0, y0, z1), at(x1, y0, z1), tx), lerp(at(x0, y1, z1), at(x1, y1, z1), tx), ty), tz);
+}
+
+// A41:1771-1785, falling back to the nearest static triangle (A41:1749-1769).
+__device__ inline F3 sdf_normal(const ClothView& c, F3 p) {
+    const float h = fmaxf(c.sdf_voxel, 1.0e-5f);
+    const float dx = sdf_sample_clamped(c, p + f3(h, 0.0f, 0.0f)) - sdf_sample_clamped(c, p + f3(-h, 0.0f, 0.0f));
+    const float dy = sdf_sample_clamped(c, p + f3(0.0f, h, 0.0f)) - sdf_sample_clamped(c, p + f3(0.0f, -h, 0.0f));
+    const float dz = sdf_sample_clamped(c, p + f3(0.0f, 0.0f, h)) - sdf_sample_clamped(c, p + f3(0.0f, 0.0f, -h));
+    const F3 n = f3(dx, dy, dz);
+    const float lsq = dot(n, n);
+    if (isfinite(lsq) && lsq > 1.0e-12f) return mul(n, rsqrt_exact(lsq));
+    F3 best_n = f3(0.0f, 0.0f, 1.0f);
+    float best = 3.402823466e+38f;
+    for (int t = 0; t < c.nstatic; ++t) {
+        const float* T = c.static_tris + 9 * t;
+        const F3 a = f3(T[0], T[1], T[2]), b = f3(T[3], T[4], T[5]), cc = f3(T[6], T[7], T[8]);
+        const F3 d = p - closest_point_on_triangle(p, a, b, cc);
+        const float dsq = dot(d, d);
+        if (dsq < best) {
+            best = dsq;
+            best_n = dsq > kEps * kEps ? normalize_or(d, best_n) : normalize_or(cross(b - a, cc - a), best_n);
+        }
+    }
+    return normalize_or(best_n, f3(0.0f, 0.0f, 1.0f));
+}
+
+// A41:1831-1867 (the caller skips pinned vertices).
+__device__ inline void sdf_contact(const ClothView& c, F3& p, F3& q, F3& vel, float dt, bool count) {
+    const float margin = fmaxf(c.collision_margin, 0.0f);
+    if (margin <= 0.0f) return;
+    float d = 0.0f;
+    if (!sdf_sample(c, p, &d) || d >= margin) return;
+    const F3 n = sdf_normal(c, p);
+    if (!finite3(n)) return;
+    const float pen = margin - d;
+    if (!isfinite(pen) || pen <= 0.0f) return;
+    const F3 corrected = p + mul(n, pen);
+    F3 v = vel;
+    const float inward = fmaxf(-dot(v, n), 0.0f);
+    contact_response(c, n, inward + pen / fmaxf(dt, kEps), v, count);
+    p = corrected;
+    vel = v;
+    q = corrected - mul(v, fmaxf(dt, kEps));
+    if (count) atomicAdd(&c.counts[kCountStaticSdf], 1ull);
+}
+
+// A41:5846-5884
+__device__ inline void velocity_update(const ClothView& c, F3 p, F3 q, F3& vel, float dt) {
+    const float sdt = fmaxf(dt, kEps);
+    const F3 meas = mul(p - q, 1.0f / sdt);
+    const F3 base = vel;
+    if (!finite3(meas) || !finite3(base)) { vel = f3(0.0f, 0.0f, 0.0f); return; }
+    const F3 pv = meas - base;
+    const float pdb = dot(pv, base);
+    const float fs = (isfinite(pdb) && pdb > 0.0f) ? 0.9f : 1.0f;
+    const float max_fd = fmaxf(2.0e-4f, fmaxf(c.thickness, 0.0f) * 0.35f);
+    const float max_fs = max_fd / sdt;
+    F3 fb = mul(pv, fs);
+    const float speed = len3(fb);
+    if (fs < 1.0f && isfinite(speed) && speed > max_fs && speed > kEps) fb = mul(fb, max_fs / speed);
+    if (!finite3(fb)) fb = f3(0.0f, 0.0f, 0.0f);
+    vel = base + fb;
+}
+
+// ---------------------------------------------------------- force fields
+// A41:1263-1531 (Blender-compatible improved Perlin noise for turbulence).
+__constant__ unsigned char kNoisePerm[256] = {
+    0xA2,0xA0,0x19,0x3B,0xF8,0xEB,0xAA,0xEE,0xF3,0x1C,0x67,0x28,0x1D,0xED,0x0,0xDE,0x95,0x2E,0xDC,0x3F,0x3A,0x82,0x35,0x4D,0x6C,0xBA,0x36,0xD0,0xF6,0xC,0x79,0x32,
+    0xD1,0x59,0xF4,0x8,0x8B,0x63,0x89,0x2F,0xB8,0xB4,0x97,0x83,0xF2,0x8F,0x18,0xC7,0x51,0x14,0x65,0x87,0x48,0x20,0x42,0xA8,0x80,0xB5,0x40,0x13,0xB2,0x22,0x7E,0x57,
+    0xBC,0x7F,0x6B,0x9D,0x86,0x4C,0xC8,0xDB,0x7C,0xD5,0x25,0x4E,0x5A,0x55,0x74,0x50,0xCD,0xB3,0x7A,0xBB,0xC3,0xCB,0xB6,0xE2,0xE4,0xEC,0xFD,0x98,0xB,0x96,0xD3,0x9E,
+    0x5C,0xA1,0x64,0xF1,0x81,0x61,0xE1,0xC4,0x24,0x72,0x49,0x8C,0x90,0x4B,0x84,0x34,0x38,0xAB,0x78,0xCA,0x1F,0x1,0xD7,0x93,0x11,0xC1,0x58,0xA9,0x31,0xF9,0x44,0x6D,
+    0xBF,0x33,0x9C,0x5F,0x9,0x94,0xA3,0x85,0x6,0xC6,0x9A,0x1E,0x7B,0x46,0x15,0x30,0x27,0x2B,0x1B,0x71,0x3C,0x5B,0xD6,0x6F,0x62,0xAC,0x4F,0xC2,0xC0,0xE,0xB1,0x23,
+    0xA7,0xDF,0x47,0xB0,0x77,0x69,0x5,0xE9,0xE6,0xE7,0x76,0x73,0xF,0xFE,0x6E,0x9B,0x56,0xEF,0x12,0xA5,0x37,0xFC,0xAE,0xD9,0x3,0x8E,0xDD,0x10,0xB9,0xCE,0xC9,0x8D,
+    0xDA,0x2A,0xBD,0x68,0x17,0x9F,0xBE,0xD4,0xA,0xCC,0xD2,0xE8,0x43,0x3D,0x70,0xB7,0x2,0x7D,0x99,0xD8,0xD,0x60,0x8A,0x4,0x2C,0x3E,0x92,0xE5,0xAF,0x53,0x7,0xE0,
+    0x29,0xA6,0xC5,0xE3,0xF5,0xF7,0x4A,0x41,0x26,0x6A,0x16,0x5E,0x52,0x2D,0x21,0xAD,0xF0,0x91,0xFF,0xEA,0x54,0xFA,0x66,0x1A,0x45,0x39,0xCF,0x75,0xA4,0x88,0xFB,0x5D};
+
+__device__ __forceinline__ int perm(int i) { return kNoisePerm[i & 255]; }
+__device__ __forceinline__ float fade(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+__device__ __forceinline__ float nlerp(float t, float a, float b) { return a + t * (b - a); }
+__device__ __forceinline__ float grad(int h, float x, float y, float z) {
+    h &= 15;
+    const float u = h < 8 ? x : y;
+    const float v = h < 4 ? y : (h == 12 || h == 14 ? x : z);
+    return ((h & 1) == 0 ? u : -u) + ((h & 2) == 0 ? v : -v);
+}
+__device__ inline float perlin(float x, float y, float z) {
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    const int X = static_cast<int>(fx) & 255, Y = static_cast<int>(fy) & 255, Z = static_cast<int>(fz) & 255;
+    x -= fx; y -= fy; z -= fz;
+    const float u = fade(x), v = fade(y), w = fade(z);
+    // the 512-entry table of the reference is the 256-entry permutation twice
+    const int A = perm(X) + Y, AA = perm(A) + Z, AB = perm(A + 1) + Z;
+    const int B = perm(X + 1) + Y, BA = perm(B) + Z, BB = perm(B + 1) + Z;
+    return nlerp(w,
+                 nlerp(v, nlerp(u, grad(perm(AA), x, y, z), grad(perm(BA), x - 1.0f, y, z)),
+                       nlerp(u, grad(perm(AB), x, y - 1.0f, z), grad(perm(BB), x - 1.0f, y - 1.0f, z))),
+                 nlerp(v, nlerp(u, grad(perm(AA + 1), x, y, z - 1.0f), grad(perm(BA + 1), x - 1.0f, y, z - 1.0f)),
+                       nlerp(u, grad(perm(AB + 1), x, y - 1.0f, z - 1.0f), grad(perm(BB + 1), x - 1.0f, y - 1.0f, z - 1.0f))));
+}
+__device__ inline float turbulence(float size, float x, float y, float z) {
+    if (size != 0.0f && isfinite(size)) { const float inv = 1.0f / size; x *= inv; y *= inv; z *= inv; }
+    float sum = 0.0f, amp = 1.0f, fs = 1.0f;
+    for (int i = 0; i <= 2; ++i, amp *= 0.5f, fs *= 2.0f) sum += (0.5f + 0.5f * perlin(fs * x, fs * y, fs * z)) * amp;
+    return sum * (4.0f / 7.0f);
+}
+__device__ inline float field_noise(F3 p, int seed, float salt) {
+    const float value = sinf(p.x * 12.9898f + p.y * 78.233f + p.z * 37.719f + (static_cast<float>(seed) + salt) * 19.191f) * 43758.5453f;
+    return (value - floorf(value)) * 2.0f - 1.0f;
+}
+__device__ inline float falloff(const SsblXpbdForceField& f, float distance, float radial) {
+    if (!isfinite(distance)) return 0.0f;
+    const float mn = fmaxf(f.distance_min, 0.0f), mx = fmaxf(f.distance_max, 0.0f);
+    const float rmn = fmaxf(f.radial_min, 0.0f), rmx = fmaxf(f.radial_max, 0.0f);
+    if (f.use_max_distance && mx > 0.0f && distance >= mx) return 0.0f;
+    if (f.use_radial_max && rmx > 0.0f && radial >= rmx) return 0.0f;
+    float att = 1.0f;
+    if (f.use_max_distance && mx > mn && distance > mn) att *= clamp01f((mx - distance) / fmaxf(mx - mn, kEps));
+    if (f.use_min_distance && f.falloff_power > 0.0f && distance > mn) {
+        const float ref = fmaxf(mn, 1.0e-4f);
+        att *= powf(ref / fmaxf(distance, ref), f.falloff_power);
+    }
+    if (f.use_radial_max && rmx > rmn && radial > rmn) att *= clamp01f((rmx - radial) / fmaxf(rmx - rmn, kEps));
+    if (f.use_radial_min && f.radial_falloff > 0.0f && radial > rmn) {
+        const float ref = fmaxf(rmn, 1.0e-4f);
+        att *= powf(ref / fmaxf(radial, ref), f.radial_falloff);
+    }
+    return isfinite(att) ? att : 0.0f;
+}
+__device__ inline F3 limit_accel(F3 a) {
+    const float len = len3(a);
+    if (!isfinite(len) || len <= 5000.0f) return isfinite(len) ? a : f3(0.0f, 0.0f, 0.0f);
+    return mul(a, 5000.0f / fmaxf(len, kEps));
+}
+__device__ inline F3 with_flow(const SsblXpbdForceField& f, F3 force, F3 vel, float fo) {
+    if (f.type != 6 && f.type != 9) {
+        const float flow = isfinite(f.flow) ? f.flow : 0.0f;
+        if (fabsf(flow) > 1.0e-6f && fo != 0.0f) force = force + mul(vel, -flow * fo);
+    }
+    return limit_accel(force);
+}
+__device__ inline F3 eval_field(const SsblXpbdForceField& f, F3 p, F3 vel, F3 n, bool has_n) {
+    const F3 zero = f3(0.0f, 0.0f, 0.0f);
+    if (!isfinite(f.strength) || f.strength == 0.0f || f.apply_to_location == 0) return zero;
+    const F3 delta = p - f3(f.origin[0], f.origin[1], f.origin[2]);
+    const F3 axis = normalize_or(f3(f.axis[0], f.axis[1], f.axis[2]), f3(0.0f, 0.0f, 1.0f));
+    const F3 radial_delta = f.use_2d_force ? delta - mul(axis, dot(delta, axis)) : delta;
+    const float distance = len3(delta), radial_distance = len3(radial_delta);
+    const float fo = falloff(f, distance, radial_distance);
+    const float strength = f.strength * fo;
+    if (strength == 0.0f || !isfinite(strength)) return zero;
+    switch (f.type) {
+    case 1: {
+        if (!has_n) return zero;
+        const F3 dir = normalize_or(f3(f.direction[0], f.direction[1], f.direction[2]), zero);
+        const float proj = dot(dir, n);
+        if (!isfinite(proj) || proj <= 0.0f) return zero;
+        float mag = strength * proj;
+        if (f.noise > 0.0f) mag *= fmaxf(0.0f, 1.0f + f.noise * field_noise(p, f.seed, 0.0f));
+        return with_flow(f, mul(dir, mag), vel, fo);
+    }
+    case 2: case 5: {
+        const F3 sd = f.use_2d_force ? radial_delta : delta;
+        const float sdist = f.use_2d_force ? radial_distance : distance;
+        if (sdist <= 1.0e-6f) return zero;
+        float scale = strength / fmaxf(sdist, kEps);
+        if (f.type == 5) scale /= fmaxf(sdist * sdist, 0.01f);
+        return with_flow(f, mul(sd, scale), vel, fo);
+    }
+    case 3: {
+        const F3 radial = delta - mul(axis, dot(delta, axis));
+        if (dot(radial, radial) <= 1.0e-10f) return zero;
+        return with_flow(f, mul(normalize_or(cross(axis, radial), zero), strength), vel, fo);
+    }
+    case 6: {
+        const float sdist = f.use_2d_force ? radial_distance : distance;
+        const F3 sd = f.use_2d_force ? radial_delta : delta;
+        if (sdist <= 1.0e-6f) return zero;
+        const F3 dir = mul(sd, 1.0f / fmaxf(sdist, kEps));
+        const float spring = -strength * (sdist - fmaxf(f.rest_length, 0.0f));
+        const float damping = -dot(vel, dir) * fmaxf(f.harmonic_damping, 0.0f) * fabsf(strength);
+        return with_flow(f, mul(dir, spring + damping), vel, fo);
+    }
+    case 7: {
+        if (distance <= 1.0e-6f) return zero;
+        const F3 dir = mul(delta, 1.0f / fmaxf(distance, kEps));
+        const float radius = fmaxf(f.rest_length, fmaxf(f.size, 0.1f));
+        const float ratio = fminf(radius / fmaxf(distance, 1.0e-3f), 6.0f);
+        const float r2 = ratio * ratio, r6 = r2 * r2 * r2;
+        return with_flow(f, mul(dir, strength * (r6 * r6 - r6)), vel, fo);
+    }
+    case 8: {
+        const F3 ax = normalize_or(f3(f.direction[0], f.direction[1], f.direction[2]), zero);
+        return with_flow(f, mul(cross(vel, ax), strength), vel, fo);
+    }
+    case 9: {
+        const float speed = len3(vel);
+        if (speed <= 1.0e-6f) return zero;
+        float lin = fmaxf(f.linear_drag, 0.0f);
+        const float quad = fmaxf(f.quadratic_drag, 0.0f);
+        if (lin <= 0.0f && quad <= 0.0f) lin = fabsf(strength);
+        return with_flow(f, mul(vel, -(lin + quad * speed)), vel, fo);
+    }
+    case 4: {
+        const F3 q = f.use_global_coords ? p : delta + axis;
+        const F3 nv = f3(-1.0f + 2.0f * turbulence(f.size, q.x, q.y, q.z), -1.0f + 2.0f * turbulence(f.size, q.y, q.z, q.x),
+                         -1.0f + 2.0f * turbulence(f.size, q.z, q.x, q.y));
+        return with_flow(f, mul(nv, strength), vel, fo);
+    }
+    case 10: {
+        float freq = fmaxf(fmaxf(f.noise, f.texture_nabla), 0.25f);
+        if (f.size > 1.0e-6f) freq = fmaxf(freq, 1.0f / f.size);
+        const F3 q = mul(delta, freq);
+        const F3 nv = f3(field_noise(q, f.seed, 0.0f), field_noise(q, f.seed, 7.0f), field_noise(q, f.seed, 13.0f));
+        return with_flow(f, mul(nv, strength), vel, fo);
+    }
+    default:
+        return zero;
+    }
+}
+
+// ------------------------------------------------------------- integrate
+// A41:1213-1261 area-free vertex normal over the incident triangles, read
+// from the pass's source positions.
+__device__ inline F3 vertex_normal(const ClothView& c, const float4* src, int v, bool* has) {
+    *has = false;
+    const int s = c.tri_off[v], e = c.tri_off[v + 1];
+    if (e <= s) return f3(0.0f, 0.0f, 0.0f);
+    F3 acc = f3(0.0f, 0.0f, 0.0f);
+    for (int k = s; k < e; ++k) {
+        const int4 t = c.tris[c.tri_inc[k]];
+        const F3 a = f3(src[t.x]), b = f3(src[t.y]), cc = f3(src[t.z]);
+        const F3 n = cross(b - a, cc - a);
+        const float lsq = dot(n, n);
+        if (!isfinite(lsq) || lsq <= 1.0e-10f) continue;
+        acc = acc + mul(n, rsqrt_exact(lsq));
+    }
+    const float asq = dot(acc, acc);
+    if (!isfinite(asq) || asq <= 1.0e-10f) return f3(0.0f, 0.0f, 0.0f);
+    *has = true;
+    return mul(acc, rsqrt_exact(asq));
+}
+
+__global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
+                                                     int ntiles, int parity, unsigned ops) {
+    const Tile tile = tiles[xcd_tile(ntiles)];
+    const int v = tile.begin + static_cast<int>(threadIdx.x);
+    if (v >= tile.end) return;
+    const ClothView& c = views[tile.cloth];
+    const float4* src = parity ? c.pos1 : c.pos0;
+    float4* dst = parity ? c.pos0 : c.pos1;
+    const float dt = c.dt_sub;
+    const float4 p4 = src[v];
+    const bool pinned = p4.w == 0.0f;
+    F3 p = f3(p4), q = p, vel;
+    if (pinned) {
+        vel = f3(0.0f, 0.0f, 0.0f);
+        p = f3(c.rest[v]);
+    } else {
+        vel = f3(c.vel[v]);
+        bool has = false;
+        const F3 n = c.need_normal ? vertex_normal(c, src, v, &has) : f3(0.0f, 0.0f, 0.0f);
+        F3 acc = f3(c.gravity[0], c.gravity[1], c.gravity[2]);
+        const float im = p4.w;
+        const bool pressure = c.use_pressure && has;
+        if (pressure) {
+            const float pf = fmaxf(c.pressure_strength, 0.0f);
+            const float share = c.area_share[v];
+            if (isfinite(share) && share > 0.0f) acc = acc + mul(n, pf * c.pressure_accel_scale * share * im);
+        }
+        if (c.nfield > 0) {
+            F3 fa = f3(0.0f, 0.0f, 0.0f);
+            const int cnt = min(c.nfield, kMaxForceFields);
+            for (int k = 0; k < cnt; ++k) fa = fa + eval_field(c.fields[k], p, vel, n, has);
+            acc = acc + limit_accel(fa);
+        }
+        F3 nv = vel + mul(acc, dt);
+        nv = mul(nv, c.damping);
+        if (pressure) {
+            const float local_rest = c.rest_scale[v];
+            if (isfinite(local_rest) && local_rest > kEps && isfinite(dt) && dt > kEps) {
+                const float pa = fmaxf(c.pressure_strength, 0.0f) * fmaxf(c.pressure_accel_scale, 0.0f) *
+                                 fmaxf(c.area_share[v], 0.0f) * fmaxf(im, 0.0f);
+                const float max_pd = fmaxf(local_rest * fmaxf(c.pressure_impulse_rest_scale, 0.0f), 1.0e-7f);
+                const float pd = pa * dt * dt;
+                if (isfinite(pd) && pd > max_pd) nv = nv - mul(n, (pd - max_pd) / dt);
+                const float vr = fmaxf(local_rest, fmaxf(c.pressure_velocity_length_scale, 0.0f));
+                const float max_out = fmaxf(vr * fmaxf(c.pressure_velocity_rest_scale, 0.0f) / dt, 1.0e-6f);
+                const float out = dot(nv, n);
+                if (isfinite(out) && out > max_out) nv = nv - mul(n, out - max_out);
+            }
+        }
+        vel = nv;
+        p = p + mul(nv, dt);
+    }
+    if (ops & kOpPin) apply_pins(c, v, pinned, p, q, vel, dt);
+    dst[v] = with_w(p, p4.w);
+    c.prev[v] = with_w(q, 0.0f);
+    c.vel[v] = with_w(vel, 0.0f);
+}
+
+// ---------------------------------------------------------- vertex pass
+__global__ __launch_bounds__(kTile) void k_vertex_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
+                                                       int ntiles, int src_parity, int dst_parity, unsigned ops) {
+    const Tile tile = tiles[xcd_tile(ntiles)];
+    const int v = tile.begin + static_cast<int>(threadIdx.x);
+    if (v >= tile.end) return;
+    const ClothView& c = views[tile.cloth];
+    const float4* src = src_parity ? c.pos1 : c.pos0;
+    float4* dst = dst_parity ? c.pos1 : c.pos0;
+    const float dt = c.dt_sub;
+    const bool count = (ops & kOpCount) != 0u;
+    const float4 p4 = src[v];
+    const float w = p4.w;
+    // .w holds 0 exactly for the vertices the reference flags as pinned
+    // (inverse mass <= 0, A41:8861-8865)
+    const bool pinned = w == 0.0f;
+    F3 p = f3(p4), q = f3(c.prev[v]), vel = f3(c.vel[v]);
+    if (ops & kOpStretch) {
+        // A41:1927-1980 gathered: each incident edge is evaluated exactly as
+        // the edge thread would (the difference vector is negated for the
+        // second endpoint, which flips the correction's sign bit-exactly).
+        const float alpha = c.stretch_alpha;
+        const int b = c.adj_off[v], e = c.adj_off[v + 1];
+        const F3 p0 = p;
+        unsigned clamps = 0;
+        for (int k = b; k < e; ++k) {
+            const int2 a = c.adj[k];
+            const int other = a.x & 0x7fffffff;
+            const float restl = __int_as_float(a.y);
+            const float4 o4 = src[other];
+            const float wsum = w + o4.w;
+            if (wsum <= 0.0f) continue;
+            const F3 delta = f3(o4) - p0;
+            const float len = sqrtf(fmaxf(dot(delta, delta), kEps));
+            const float cval = len - restl;
+            const float dl = -cval / (wsum + alpha);
+            F3 corr = mul(delta, 0.18f * dl / fmaxf(len, kEps));
+            const float cl = sqrtf(fmaxf(dot(corr, corr), 0.0f));
+            const float mc = fmaxf(0.0025f, fminf(fmaxf(restl, 0.0f) * 0.10f, 0.025f));
+            if (cl > mc) {
+                corr = mul(corr, mc / fmaxf(cl, kEps));
+                if (a.x < 0) ++clamps;
+            }
+            if (w > 0.0f) {
+                const F3 d = mul(corr, -w);
+                p = p + d;
+                if (cval > 0.0f) q = q + mul(d, 0.08f);
+            }
+        }
+        if (count && clamps) atomicAdd(&c.counts[kCountHardFallbacks], static_cast<unsigned long long>(clamps));
+    } else if (ops & kOpLra) {
+        // A41:2710-2803 gathered the same way (tethers are one-sided: only a
+        // stretched tether acts).
+        const int b = c.lra_off[v], e = c.lra_off[v + 1];
+        const F3 p0 = p;
+        unsigned tacks = 0, guards = 0;
+        for (int k = b; k < e; ++k) {
+            const int4 a = c.lra_adj[k];
+            const int other = a.x & 0x7fffffff;
+            const bool first = a.x < 0;
+            const float restl = __int_as_float(a.y);
+            const float4 o4 = src[other];
+            const float wsum = w + o4.w;
+            if (!isfinite(wsum) || wsum <= 0.0f) { guards += first; continue; }
+            const F3 po = f3(o4);
+            if (!finite3(p0) || !finite3(po)) { guards += first; continue; }
+            F3 dx = po - p0;
+            const float lsq = __builtin_fmaf(dx.x, dx.x, __builtin_fmaf(dx.y, dx.y, dx.z * dx.z));
+            if (!isfinite(lsq) || lsq <= 1.0e-10f || !isfinite(restl) || restl <= kEps) { guards += first; continue; }
+            const float inv_len = rsqrt_exact(lsq);
+            const float len = lsq * inv_len;
+            const float stretch = len - restl;
+            if (!isfinite(stretch) || stretch <= 0.0f) continue;
+            dx = mul(dx, inv_len);
+            const float bc = fmaxf(c.lra_compliance, 1.0e-12f);
+            const float stiff = (1.0f / bc) * 500000.0f;
+            const float ealpha = (bc / 500000.0f) / fmaxf(dt * dt, kEps);
+            const float sw = stiff / fmaxf(stiff + 500000.0f, 1.0f);
+            const float dl = -stretch / fmaxf(wsum + ealpha, kEps);
+            F3 corr = mul(dx, 0.35f * sw * dl);
+            if (!finite3(corr)) { guards += first; continue; }
+            const float cl = len3(corr);
+            const float mc = fmaxf(5.0e-4f, fminf(fmaxf(restl, c.thickness) * 0.20f, 0.035f));
+            if (!isfinite(cl) || cl <= 0.0f) continue;
+            if (cl > mc) { corr = mul(corr, mc / fmaxf(cl, kEps)); guards += first; }
+            const bool th = c.thickness > 0.0f;
+            const float cap = th ? 0.0025f : 0.012f;
+            const float rl = fmaxf(restl, 0.0f) * (th ? 0.02f : 0.08f);
+            const float tl = th ? c.thickness * 0.12f : cap;
+            const float al = fmaxf(5.0e-4f, fminf(fminf(rl, tl), cap));
+            if (w > 0.0f) {
+                const F3 d = limit_delta(mul(corr, -w / fmaxf(wsum, kEps)), al);
+                p = p + d;
+                q = q + mul(d, 0.32f);
+            }
+            tacks += first;
+        }
+        if (count && tacks) atomicAdd(&c.counts[kCountLraTacks], static_cast<unsigned long long>(tacks));
+        if (count && guards) atomicAdd(&c.counts[kCountTackGuards], static_cast<unsigned long long>(guards));
+    }
+    if (ops & kOpPin) apply_pins(c, v, pinned, p, q, vel, dt);
+    if ((ops & kOpAnalytic) && !pinned) analytic_contacts(c, p, q, vel, dt, count);
+    if ((ops & kOpSdf) && c.sdf_ready && !pinned) sdf_contact(c, p, q, vel, dt, count);
+    // A41:9360-9369: the second pin projection runs after any contact work,
+    // and on every last iteration (dynamic collision is scheduled there).
+    if ((ops & kOpPinPolish) && ((ops & kOpLastIter) || c.contacts_on)) apply_pins(c, v, pinned, p, q, vel, dt);
+    if (ops & kOpVelocity) {
+        if (pinned) vel = f3(0.0f, 0.0f, 0.0f);
+        else velocity_update(c, p, q, vel, dt);
+    }
+    dst[v] = with_w(p, w);
+    c.prev[v] = with_w(q, 0.0f);
+    c.vel[v] = with_w(vel, 0.0f);
+}
+
+// -------------------------------------------------------------- bending
+// A41:2886-2970, one launch per colour; a segment is {cloth, begin, end}.
+__global__ __launch_bounds__(kTile) void k_bend_color(const ClothView* __restrict__ views, const int4* __restrict__ segs,
+                                                      int parity, unsigned ops) {
+    const int4 seg = segs[blockIdx.x];
+    const int k = seg.y + static_cast<int>(threadIdx.x);
+    if (k >= seg.z) return;
+    const ClothView& c = views[seg.x];
+    float4* pos = parity ? c.pos1 : c.pos0;
+    const bool count = (ops & kOpCount) != 0u;
+    const int2 pr = c.bends[k];
+    const int i = pr.x, j = pr.y;
+    if (i < 0 || j < 0 || i >= c.nv || j >= c.nv || i == j) { if (count) atomicAdd(&c.counts[kCountBendingGuards], 1ull); return; }
+    const float4 pi4 = pos[i], pj4 = pos[j];
+    const float wi = pi4.w, wj = pj4.w;
+    const float w = wi + wj;
+    if (!isfinite(w) || w <= 0.0f) return;
+    const float restl = c.bend_rest[k];
+    if (!isfinite(restl) || restl <= kEps) { if (count) atomicAdd(&c.counts[kCountBendingGuards], 1ull); return; }
+    const F3 pi = f3(pi4), pj = f3(pj4);
+    if (!finite3(pi) || !finite3(pj)) { if (count) atomicAdd(&c.counts[kCountBendingGuards], 1ull); return; }
+    const F3 delta = pj - pi;
+    const float lsq = __builtin_fmaf(delta.x, delta.x, __builtin_fmaf(delta.y, delta.y, delta.z * delta.z));
+    if (!isfinite(lsq) || lsq <= 1.0e-10f) { if (count) atomicAdd(&c.counts[kCountBendingGuards], 1ull); return; }
+    const float inv_len = rsqrt_exact(lsq);
+    const float len = lsq * inv_len;
+    const float cval = len - restl;
+    if (!isfinite(cval) || fabsf(cval) <= 1.0e-6f) return;
+    const float dl = -cval / fmaxf(w + c.bend_alpha, kEps);
+    F3 corr = mul(delta, 0.10f * dl * inv_len);
+    if (!finite3(corr)) { if (count) atomicAdd(&c.counts[kCountBendingGuards], 1ull); return; }
+    const float cl = len3(corr);
+    const float mc = fmaxf(2.5e-4f, fminf(fmaxf(restl, c.thickness) * 0.08f, 0.020f));
+    if (!isfinite(cl) || cl <= 0.0f) return;
+    if (cl > mc) { corr = mul(corr, mc / fmaxf(cl, kEps)); if (count) atomicAdd(&c.counts[kCountBendingGuards], 1ull); }
+    if (wi > 0.0f) {
+        const F3 d = mul(corr, -wi);
+        pos[i] = with_w(pi + d, wi);
+        const float4 q = c.prev[i];
+        c.prev[i] = with_w(f3(q) + mul(d, 0.06f), 0.0f);
+    }
+    if (wj > 0.0f) {
+        const F3 d = mul(corr, wj);
+        pos[j] = with_w(pj + d, wj);
+        const float4 q = c.prev[j];
+        c.prev[j] = with_w(f3(q) + mul(d, 0.06f), 0.0f);
+    }
+    if (count) wave_count(c.counts, kCountBendingWings, true);
+}
+
+// ---------------------------------------------------------- SDF build
+// A41:1725-1747 +x ray against a triangle (Moller-Trumbore), for the sign.
+__device__ inline bool ray_hits(F3 p, F3 a, F3 b, F3 c) {
+    const F3 dir = f3(1.0f, 0.0f, 0.0f);
+    const F3 e1 = b - a, e2 = c - a;
+    const F3 h = cross(dir, e2);
+    const float det = dot(e1, h);
+    if (fabsf(det) <= 1.0e-8f) return false;
+    const float inv = 1.0f / det;
+    const F3 s = p - a;
+    const float u = inv * dot(s, h);
+    if (u < -1.0e-5f || u > 1.0f + 1.0e-5f) return false;
+    const F3 q = cross(s, e1);
+    const float vv = inv * dot(dir, q);
+    if (vv < -1.0e-5f || u + vv > 1.0f + 1.0e-5f) return false;
+    return inv * dot(e2, q) > 1.0e-6f;
+}
+
+// A41:1787-1829: brute-force distance of every voxel to the static triangles,
+// triangles staged through LDS in chunks.
+__global__ __launch_bounds__(kTile) void k_sdf_build(float* __restrict__ sdf, int dx, int dy, int dz, float ox, float oy,
+                                                     float oz, float voxel, const float* __restrict__ tris, int ntris,
+                                                     int signed_sdf) {
+    __shared__ float chunk[kTile * 9];
+    const long long vox = static_cast<long long>(blockIdx.x) * kTile + threadIdx.x;
+    const long long total = static_cast<long long>(dx) * dy * dz;
+    const bool active = vox < total;
+    F3 p = f3(0.0f, 0.0f, 0.0f), rp = p;
+    if (active) {
+        const int x = static_cast<int>(vox % dx), y = static_cast<int>((vox / dx) % dy), z = static_cast<int>(vox / (1LL * dx * dy));
+        p = f3(ox, oy, oz) + f3(static_cast<float>(x) * voxel, static_cast<float>(y) * voxel, static_cast<float>(z) * voxel);
+        rp = p + f3(0.0f, voxel * 1.37e-4f, voxel * 2.11e-4f);
+    }
+    float best = 3.402823466e+38f;
+    int parity = 0;
+    for (int base = 0; base < ntris; base += kTile) {
+        const int n = min(kTile, ntris - base);
+        __syncthreads();
+        for (int k = threadIdx.x; k < n * 9; k += kTile) chunk[k] = tris[static_cast<size_t>(base) * 9 + k];
+        __syncthreads();
+        if (active) {
+            for (int t = 0; t < n; ++t) {
+                const float* T = chunk + 9 * t;
+                const F3 a = f3(T[0], T[1], T[2]), b = f3(T[3], T[4], T[5]), c = f3(T[6], T[7], T[8]);
+                const F3 d = p - closest_point_on_triangle(p, a, b, c);
+                best = fminf(best, dot(d, d));
+                if (signed_sdf && ray_hits(rp, a, b, c)) parity ^= 1;
+            }
+        }
+    }
+    if (active) {
+        float dist = sqrtf(fmaxf(best, 0.0f));
+        if (signed_sdf && parity) dist = -dist;
+        sdf[vox] = dist;
+    }
+}
+
+__global__ void k_fill_u32(unsigned* p, unsigned value, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+// ------------------------------------------------------------ launches
+void launch_integrate(const LaunchCtx& c, int parity, unsigned ops) {
+    TimedLaunch t("integrate", c.stream);
+    hipLaunchKernelGGL(k_integrate, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
+}
+
+void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops) {
+    const char* name = (ops & kOpStretch) ? "vertex_pass_stretch" : ((ops & kOpLra) ? "vertex_pass_lra" : "vertex_pass_local");
+    TimedLaunch t(name, c.stream);
+    hipLaunchKernelGGL(k_vertex_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, src_parity,
+                       dst_parity, ops);
+}
+
+void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops) {
+    if (nseg_blocks <= 0) return;
+    TimedLaunch t("bend_color", c.stream);
+    hipLaunchKernelGGL(k_bend_color, dim3(nseg_blocks), dim3(kTile), 0, c.stream, c.views, segments, parity, ops);
+}
+
+void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float ox, float oy, float oz, float voxel,
+                      const float* tris, int ntris, int signed_sdf) {
+    const long long total = static_cast<long long>(dx) * dy * dz;
+    const unsigned blocks = static_cast<unsigned>((total + kTile - 1) / kTile);
+    TimedLaunch t("sdf_build", s);
+    hipLaunchKernelGGL(k_sdf_build, dim3(blocks), dim3(kTile), 0, s, sdf, dx, dy, dz, ox, oy, oz, voxel, tris, ntris, signed_sdf);
+}
+
+void launch_fill_u32(hipStream_t s, unsigned* p, unsigned value, size_t n) {
+    if (n == 0) return;
+    TimedLaunch t("fill", s);
+    hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, p, value, n);
+}
+
+}  // namespace ssbl

@@ -1,0 +1,96 @@
+// xpbd_math.hpp -- float3 arithmetic for the gfx950 kernels.
+//
+// The operation order of every helper is fixed so results are reproducible:
+// dot products are fused multiply-adds in x, y, z order (the reference's
+// fma_rn helpers, ssbl_xpbd_cuda_abi41.cu:622-636), 1/sqrt is computed as a
+// correctly rounded division by a correctly rounded square root, and the
+// library is built with -ffp-contract=off so no other multiply-add is fused.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ssbl {
+
+constexpr float kEps = 1.0e-7f;
+
+struct F3 { float x, y, z; };
+
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 f3(float4 v) { return F3{v.x, v.y, v.z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 mul(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot(F3 a, F3 b) {
+    return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, __builtin_fmaf(a.x, b.x, 0.0f)));
+}
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+    return f3(__builtin_fmaf(a.y, b.z, -(a.z * b.y)), __builtin_fmaf(a.z, b.x, -(a.x * b.z)),
+              __builtin_fmaf(a.x, b.y, -(a.y * b.x)));
+}
+__device__ __forceinline__ float len3(F3 a) { return sqrtf(fmaxf(dot(a, a), 0.0f)); }
+__device__ __forceinline__ bool finite3(F3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+__device__ __forceinline__ float clamp01f(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+__device__ __forceinline__ float rsqrt_exact(float v) { return 1.0f / sqrtf(v); }
+__device__ __forceinline__ float4 with_w(F3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+__device__ __forceinline__ F3 limit_delta(F3 d, float max_len) {
+    if (!finite3(d) || !isfinite(max_len) || max_len <= 0.0f) return f3(0.0f, 0.0f, 0.0f);
+    const float len = len3(d);
+    if (!isfinite(len) || len <= kEps) return f3(0.0f, 0.0f, 0.0f);
+    return len > max_len ? mul(d, max_len / len) : d;
+}
+
+__device__ __forceinline__ F3 normalize_or(F3 v, F3 fb) {
+    const float len = len3(v);
+    if (!isfinite(len) || len <= kEps) return fb;
+    return mul(v, 1.0f / len);
+}
+
+// Closest point on triangle abc to p (Ericson, Real-Time Collision Detection
+// 5.1.5; the region order of ssbl_xpbd_cuda_abi41.cu:1589-1635).
+__device__ inline F3 closest_point_on_triangle(F3 p, F3 a, F3 b, F3 c) {
+    const F3 ab = b - a, ac = c - a, ap = p - a;
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) return a;
+    const F3 bp = p - b;
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) return b;
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return a + mul(ab, d1 / fmaxf(d1 - d3, kEps));
+    const F3 cp = p - c;
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) return c;
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return a + mul(ac, d2 / fmaxf(d2 - d6, kEps));
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float w = (d4 - d3) / fmaxf((d4 - d3) + (d5 - d6), kEps);
+        return b + mul(c - b, w);
+    }
+    const float denom = 1.0f / fmaxf(va + vb + vc, kEps);
+    const float v = vb * denom, w = vc * denom;
+    return a + (mul(ab, v) + mul(ac, w));
+}
+
+__device__ __forceinline__ int cell_coord(float value, float cell) {
+    return static_cast<int>(floorf(value / fmaxf(cell, kEps)));
+}
+
+// Spatial hash of a cell (ssbl_xpbd_cuda_abi41.cu:658-664), bit-exact.
+__device__ __forceinline__ unsigned hash_cell(int x, int y, int z, int buckets) {
+    const unsigned h = static_cast<unsigned>(x) * 73856093u ^ static_cast<unsigned>(y) * 19349663u ^
+                       static_cast<unsigned>(z) * 83492791u;
+    return h & static_cast<unsigned>(buckets - 1);
+}
+
+// Wave64-aggregated counter increment: one atomic per wavefront.
+__device__ __forceinline__ void wave_count(unsigned long long* counts, int slot, bool cond) {
+    const unsigned long long active = __ballot(1);
+    const unsigned long long hits = __ballot(cond);
+    const int leader = __ffsll(static_cast<long long>(active)) - 1;
+    if (static_cast<int>(threadIdx.x & 63u) == leader && hits != 0ull) {
+        atomicAdd(&counts[slot], static_cast<unsigned long long>(__popcll(hits)));
+    }
+}
+
+}  // namespace ssbl

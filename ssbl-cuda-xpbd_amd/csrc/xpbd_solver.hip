@@ -1,0 +1,1183 @@
+// xpbd_solver.hip -- host side of libssbl_xpbd_hip_abi41.so: the ABI41 entry
+// points (include/ssbl_xpbd_hip.h), the ssbl_ext_* extensions
+// (include/ssbl_xpbd_hip_ext.h), device memory set-up and the substep
+// schedule.
+//
+// Reference: native/src/ssbl_xpbd_cuda_abi41.cu (A41:<line>); create
+// A41:8791-8967, step A41:9087-9398, uploads A41:5886-7109, 8282-8317.
+//
+// Ordering model: the library owns one HIP stream; every entry point that
+// changes device data first waits for that stream (the reference's cudaMemcpy
+// on the legacy default stream waits for all earlier work the same way), so the
+// caller may free its arrays on return.  ssbl_step_solver_ex only enqueues
+// work unless force_sync / fetch_diagnostics is set.
+#include "ssbl_xpbd_hip.h"
+#include "ssbl_xpbd_hip_ext.h"
+#include "xpbd_device.hpp"
+#include "xpbd_self.hpp"
+#include "xpbd_timing.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace ssbl {
+
+thread_local std::string g_error;
+
+bool fail(const std::string& message) {
+    g_error = message;
+    return false;
+}
+
+bool hip_ok(hipError_t err, const char* what) {
+    if (err == hipSuccess) return true;
+    g_error = std::string(what) + ": " + hipGetErrorString(err);
+    return false;
+}
+
+hipStream_t lib_stream() {
+    static hipStream_t stream = [] {
+        hipStream_t s = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+        return s;
+    }();
+    return stream;
+}
+
+// ---------------------------------------------------------------- timing
+namespace {
+std::atomic<bool> g_timing{false};
+std::mutex g_timing_mutex;
+struct Pending { const char* name; hipEvent_t begin, end; };
+std::vector<Pending> g_pending;
+std::vector<hipEvent_t> g_event_pool;
+struct KStat { long long launches = 0; double total_ms = 0.0; };
+std::map<std::string, KStat> g_kstats;
+
+hipEvent_t take_event() {
+    if (!g_event_pool.empty()) { hipEvent_t e = g_event_pool.back(); g_event_pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+void resolve_pending() {
+    for (const Pending& p : g_pending) {
+        float ms = 0.0f;
+        hipEventSynchronize(p.end);
+        hipEventElapsedTime(&ms, p.begin, p.end);
+        KStat& k = g_kstats[p.name];
+        k.launches += 1;
+        k.total_ms += ms;
+        g_event_pool.push_back(p.begin);
+        g_event_pool.push_back(p.end);
+    }
+    g_pending.clear();
+}
+}  // namespace
+
+bool timing_enabled() { return g_timing.load(std::memory_order_relaxed); }
+void timing_record(const char*, hipStream_t stream, hipEvent_t* begin_out) {
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    *begin_out = take_event();
+    hipEventRecord(*begin_out, stream);
+}
+void timing_finish(const char* name, hipStream_t stream, hipEvent_t begin) {
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    hipEvent_t end = take_event();
+    hipEventRecord(end, stream);
+    g_pending.push_back({name, begin, end});
+    if (g_pending.size() > 4096) resolve_pending();
+}
+
+// ------------------------------------------------------- device buffers
+template <class T> struct Dev {
+    T* p = nullptr;
+    size_t n = 0;
+    Dev() = default;
+    Dev(const Dev&) = delete;
+    Dev& operator=(const Dev&) = delete;
+    ~Dev() { release(); }
+    void release() { if (p) hipFree(p); p = nullptr; n = 0; }
+    bool alloc(size_t count, const char* what) {
+        release();
+        if (count == 0) return true;
+        if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count), what)) { p = nullptr; return false; }
+        n = count;
+        return true;
+    }
+    bool upload(const std::vector<T>& h, const char* what) {
+        if (h.size() != n && !alloc(h.size(), what)) return false;
+        if (h.empty()) return true;
+        return hip_ok(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice), what);
+    }
+};
+
+// Pinned staging ring for per-step parameter uploads that are ordered on the
+// stream (the previous step may still be reading the device copy).
+struct Staging {
+    static constexpr int kRing = 4;
+    void* host[kRing] = {};
+    hipEvent_t done[kRing] = {};
+    size_t bytes = 0;
+    int next = 0;
+    ~Staging() { release(); }
+    void release() {
+        for (int i = 0; i < kRing; ++i) {
+            if (host[i]) hipHostFree(host[i]);
+            if (done[i]) hipEventDestroy(done[i]);
+            host[i] = nullptr;
+            done[i] = nullptr;
+        }
+        bytes = 0;
+    }
+    bool upload(void* dst, const void* src, size_t n, hipStream_t s) {
+        if (n > bytes) {
+            release();
+            for (int i = 0; i < kRing; ++i) {
+                if (!hip_ok(hipHostMalloc(&host[i], n), "staging allocation")) return false;
+                if (!hip_ok(hipEventCreateWithFlags(&done[i], hipEventDisableTiming), "staging event")) return false;
+            }
+            bytes = n;
+        }
+        const int slot = next;
+        next = (next + 1) % kRing;
+        if (!hip_ok(hipEventSynchronize(done[slot]), "staging wait")) return false;
+        std::memcpy(host[slot], src, n);
+        if (!hip_ok(hipMemcpyAsync(dst, host[slot], n, hipMemcpyHostToDevice, s), "parameter upload")) return false;
+        return hip_ok(hipEventRecord(done[slot], s), "staging record");
+    }
+};
+
+int next_pow2(int v) {
+    int r = 1;
+    v = std::max(v, 1);
+    while (r < v && r < (1 << 30)) r <<= 1;
+    return r;
+}
+
+// ----------------------------------------------------------------- solver
+struct HostPin { int index; float target[3]; float weight; };
+
+struct Solver {
+    SsblXpbdConfig cfg{};
+    SsblXpbdRuntimeColliders colliders{};
+    int stretch_mode = SSBL_EXT_STRETCH_JACOBI;
+    int nv = 0;
+    // host mirrors needed after create
+    std::vector<float4> h_rest;          // w = inverse mass as uploaded
+    std::vector<float> h_weff;           // effective inverse mass
+    std::vector<int> edge_colors, bend_colors;
+    bool bend_coloring_valid = false;
+    int lra_invalid = 0;
+    std::vector<HostPin> pins;
+    int pins_iterations = -1;
+    std::vector<SsblXpbdForceField> fields;
+    int unsupported_fields = 0;
+    float pressure_accel_scale = 1792.0f, pressure_impulse_rest_scale = 0.85f;
+    float pressure_velocity_rest_scale = 2.25f, pressure_velocity_length_scale = 1.0f;
+    // static SDF
+    int sdf_ready = 0, sdf_dim[3] = {0, 0, 0}, sdf_signed = 0, nstatic = 0;
+    float sdf_origin[3] = {0, 0, 0}, sdf_voxel = 0.0f;
+    long long sdf_rebuilds = 0, sdf_unsigned = 0;
+    float sdf_build_ms = 0.0f;
+    // device state
+    Dev<float4> pos0, pos1, prev, vel, rest;
+    Dev<int> adj_off, lra_off, tri_off, tri_inc, pin_off, pin_ids;
+    Dev<int2> adj, bends;
+    Dev<int4> lra_adj, tris, bend_segs;
+    Dev<float> bend_rest, area_share, rest_scale, sdf, static_tris;
+    Dev<float4> d_pins;
+    Dev<SsblXpbdForceField> d_fields;
+    Dev<unsigned long long> counts;
+    Dev<float> max_smoothed;
+    Dev<ClothView> d_view;
+    Dev<Tile> d_tiles;
+    std::vector<int> bend_seg_first, bend_seg_count;   // per colour, into bend_segs
+    std::vector<int4> h_bend_segs;
+    int ntiles = 0;
+    Staging staging;
+    SelfCollision self;
+    ClothView view{};
+    SsblXpbdDiagnostics diag{};
+    bool counts_pending = false;
+
+    bool create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh);
+    bool build_sdf(const float* triangles, int count);
+    bool upload_pins(const int* idx, const float* p, const float* w, int count);
+    bool refresh_pins(int iterations);
+    bool upload_fields(const SsblXpbdForceField* f, int count, int unsupported);
+    void fill_view(int substeps);
+    float self_radius() const {
+        const float req = std::max(cfg.self_collision_distance, 0.0f);
+        if (req > 0.0f) return req;
+        return std::max(std::max(cfg.cloth_thickness, cfg.collision_margin), 0.0f);
+    }
+    bool analytic_on() const { return colliders.use_ground || colliders.use_wall || colliders.use_sphere; }
+};
+
+bool sync_stream() { return hip_ok(hipStreamSynchronize(lib_stream()), "stream synchronize"); }
+
+// A41:7413-7584 per-vertex pressure area shares and rest scales (host set-up).
+static void pressure_setup(Solver& s, const float* rest_f, const std::vector<int>& tri, const std::vector<int>& edges,
+                           const std::vector<float>& edge_rest, std::vector<float>& share, std::vector<float>& scale) {
+    const int n = s.nv;
+    auto P = [&](int i, int k) { return rest_f[3 * i + k]; };
+    auto cross_dot = [&](int a, int b, int c) {
+        const float ux = P(b, 0) - P(a, 0), uy = P(b, 1) - P(a, 1), uz = P(b, 2) - P(a, 2);
+        const float vx = P(c, 0) - P(a, 0), vy = P(c, 1) - P(a, 1), vz = P(c, 2) - P(a, 2);
+        const float cx = std::fma(uy, vz, -(uz * vy)), cy = std::fma(uz, vx, -(ux * vz)), cz = std::fma(ux, vy, -(uy * vx));
+        return std::fma(cz, cz, std::fma(cy, cy, std::fma(cx, cx, 0.0f)));
+    };
+    share.assign(n, 0.0f);
+    double total_area = 0.0;
+    int valid = 0;
+    const int nt = static_cast<int>(tri.size() / 3);
+    for (int t = 0; t < nt; ++t) {
+        const int a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+        if (a >= n || b >= n || c >= n) continue;
+        const float sq = cross_dot(a, b, c);
+        if (!std::isfinite(sq) || sq <= 0.0f) continue;
+        const float area = 0.5f * std::sqrt(sq);
+        if (!std::isfinite(area) || area <= 0.0f) continue;
+        const float third = area / 3.0f;
+        share[a] += third; share[b] += third; share[c] += third;
+        total_area += static_cast<double>(area);
+        ++valid;
+    }
+    if (valid <= 0 || total_area <= 0.0 || !std::isfinite(total_area)) {
+        std::fill(share.begin(), share.end(), 0.0f);
+    } else {
+        const float mean = static_cast<float>(total_area / static_cast<double>(std::max(n, 1)));
+        const float cap = std::max(mean * 16.0f, 1.0e-12f);
+        for (float& x : share) x = (!std::isfinite(x) || x <= 0.0f) ? 0.0f : std::min(x, cap);
+    }
+    scale.assign(n, 0.0f);
+    std::vector<double> sums(n, 0.0);
+    std::vector<int> cnt(n, 0);
+    const int ne = static_cast<int>(edge_rest.size());
+    for (int e = 0; e < ne; ++e) {
+        const int i = edges[2 * e], j = edges[2 * e + 1];
+        const float r = edge_rest[e];
+        if (i >= n || j >= n || !std::isfinite(r) || r <= 0.0f) continue;
+        sums[i] += r; sums[j] += r; cnt[i]++; cnt[j]++;
+    }
+    double total = 0.0;
+    int vc = 0;
+    for (int i = 0; i < n; ++i) {
+        float x = 0.0f;
+        if (cnt[i] > 0) x = static_cast<float>(sums[i] / static_cast<double>(cnt[i]));
+        else if (std::isfinite(share[i]) && share[i] > 0.0f) x = std::sqrt(share[i]);
+        if (std::isfinite(x) && x > 0.0f) { scale[i] = x; total += x; ++vc; }
+    }
+    const float mean_scale = vc > 0 ? static_cast<float>(total / static_cast<double>(vc)) : 1.0f;
+    const float lo = std::max(mean_scale * 0.02f, 1.0e-7f), hi = std::max(mean_scale * 16.0f, lo);
+    for (float& x : scale) {
+        if (!std::isfinite(x) || x <= 0.0f) x = mean_scale;
+        x = std::max(lo, std::min(x, hi));
+    }
+    float obj = 0.08f;
+    if (const char* raw = std::getenv("SSBL_ABI41_PRESSURE_VELOCITY_OBJECT_SCALE")) {
+        char* end = nullptr;
+        const float v = std::strtof(raw, &end);
+        if (raw[0] && end != raw && std::isfinite(v)) obj = v;
+    }
+    obj = std::max(0.005f, std::min(obj, 1.0f));
+    float ms = 0.0f;
+    int mc = 0;
+    for (float x : scale) if (std::isfinite(x) && x > 0.0f) { ms += x; ++mc; }
+    ms = mc > 0 ? ms / static_cast<float>(mc) : 1.0f;
+    s.pressure_velocity_length_scale = std::max(ms, 1.0e-6f);
+    bool any = false;
+    float lo3[3] = {0, 0, 0}, hi3[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(P(i, 0)) || !std::isfinite(P(i, 1)) || !std::isfinite(P(i, 2))) continue;
+        for (int k = 0; k < 3; ++k) {
+            if (!any) { lo3[k] = hi3[k] = P(i, k); }
+            else { lo3[k] = std::min(lo3[k], P(i, k)); hi3[k] = std::max(hi3[k], P(i, k)); }
+        }
+        any = true;
+    }
+    if (any) {
+        const float sx = hi3[0] - lo3[0], sy = hi3[1] - lo3[1], sz = hi3[2] - lo3[2];
+        const float d = std::sqrt(std::max(std::fma(sz, sz, std::fma(sy, sy, std::fma(sx, sx, 0.0f))), 0.0f));
+        if (std::isfinite(d) && d > 0.0f) s.pressure_velocity_length_scale = std::max(d * obj, ms * 4.0f);
+    }
+}
+
+static float env_float(const char* name, float def) {
+    const char* raw = std::getenv(name);
+    if (!raw || !raw[0]) return def;
+    char* end = nullptr;
+    const float v = std::strtof(raw, &end);
+    return (end == raw || !std::isfinite(v)) ? def : v;
+}
+
+bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
+    cfg = *config;  // A41:8797-8848
+    cfg.vertex_count = std::max(cfg.vertex_count, 0);
+    cfg.edge_count = std::max(cfg.edge_count, 0);
+    cfg.bend_count = std::max(cfg.bend_count, 0);
+    cfg.lra_count = std::max(cfg.lra_count, 0);
+    cfg.triangle_count = std::max(cfg.triangle_count, 0);
+    cfg.damping = std::isfinite(cfg.damping) ? cfg.damping : 1.0f;
+    cfg.damping = std::max(0.0f, std::min(cfg.damping, 1.0f));
+    if (!std::isfinite(cfg.bend_compliance) || cfg.bend_compliance < 0.0f) cfg.bend_compliance = 0.0f;
+    if (!std::isfinite(cfg.lra_compliance) || cfg.lra_compliance < 0.0f) cfg.lra_compliance = 0.0f;
+    if (!std::isfinite(cfg.stretch_optimization_strength)) cfg.stretch_optimization_strength = 0.0f;
+    cfg.stretch_optimization_strength = std::min(std::max(cfg.stretch_optimization_strength, 0.0f), 1.0f);
+    cfg.stretch_optimization_enabled =
+        (cfg.edge_count > 0 && cfg.stretch_optimization_enabled && cfg.stretch_optimization_strength > 0.0f) ? 1 : 0;
+    if (cfg.stretch_optimization_enabled) return fail("stretch optimization (PCG) is not available in this build yet");
+    if (cfg.self_collision && cfg.self_collision_mode == 1) return fail("fast self-collision mode is not available in this build yet");
+    pressure_accel_scale = std::max(1.0f, std::min(env_float("SSBL_ABI41_PRESSURE_UI_ACCEL_SCALE", 1792.0f), 8192.0f));
+    pressure_impulse_rest_scale = std::max(0.05f, std::min(env_float("SSBL_ABI41_PRESSURE_IMPULSE_REST_SCALE", 0.85f), 8.0f));
+    pressure_velocity_rest_scale = std::max(0.10f, std::min(env_float("SSBL_ABI41_PRESSURE_VELOCITY_REST_SCALE", 2.25f), 16.0f));
+    colliders.use_ground = cfg.use_ground;
+    colliders.ground_height = cfg.ground_height;
+    colliders.use_wall = cfg.use_wall;
+    std::memcpy(colliders.wall_origin, cfg.wall_origin, sizeof(float) * 3);
+    std::memcpy(colliders.wall_normal, cfg.wall_normal, sizeof(float) * 3);
+    colliders.use_sphere = cfg.use_sphere;
+    std::memcpy(colliders.sphere_center, cfg.sphere_center, sizeof(float) * 3);
+    colliders.sphere_radius = cfg.sphere_radius;
+    if (!lib_stream()) return fail("cannot create the HIP stream");
+
+    const int n = nv = cfg.vertex_count;
+    const float* P = mesh->positions;
+    h_rest.resize(n);
+    h_weff.resize(n);
+    std::vector<float4> h_pos(n), h_zero(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f)), h_prev(n);
+    for (int i = 0; i < n; ++i) {
+        const float im = mesh->inv_mass[i];
+        h_weff[i] = im <= 0.0f ? 0.0f : im;   // A41:8861-8865 pinned flag
+        h_rest[i] = make_float4(P[3 * i], P[3 * i + 1], P[3 * i + 2], im);
+        h_pos[i] = make_float4(P[3 * i], P[3 * i + 1], P[3 * i + 2], h_weff[i]);
+        h_prev[i] = make_float4(P[3 * i], P[3 * i + 1], P[3 * i + 2], 0.0f);
+    }
+    // stretch edges, A41:8866-8873; incidence lists in ascending edge order
+    std::vector<int> edges(2 * static_cast<size_t>(cfg.edge_count));
+    std::vector<float> edge_rest(cfg.edge_count);
+    for (int e = 0; e < cfg.edge_count; ++e) {
+        edges[2 * e] = std::max(mesh->edges ? mesh->edges[2 * e] : 0, 0);
+        edges[2 * e + 1] = std::max(mesh->edges ? mesh->edges[2 * e + 1] : 0, 0);
+        edge_rest[e] = mesh->edge_rest_lengths ? mesh->edge_rest_lengths[e] : 0.0f;
+    }
+    std::vector<int> h_adj_off(n + 1, 0);
+    for (int e = 0; e < cfg.edge_count; ++e) {
+        const int i = edges[2 * e], j = edges[2 * e + 1];
+        if (i >= n || j >= n || i == j) continue;
+        h_adj_off[i + 1]++; h_adj_off[j + 1]++;
+    }
+    for (int i = 0; i < n; ++i) h_adj_off[i + 1] += h_adj_off[i];
+    std::vector<int2> h_adj(h_adj_off[n]);
+    {
+        std::vector<int> cur(h_adj_off.begin(), h_adj_off.end() - 1);
+        for (int e = 0; e < cfg.edge_count; ++e) {
+            const int i = edges[2 * e], j = edges[2 * e + 1];
+            if (i >= n || j >= n || i == j) continue;
+            int rb;
+            std::memcpy(&rb, &edge_rest[e], 4);
+            h_adj[cur[i]++] = make_int2(static_cast<int>(static_cast<unsigned>(j) | 0x80000000u), rb);
+            h_adj[cur[j]++] = make_int2(i, rb);
+        }
+    }
+    edge_colors.clear();
+    if (cfg.edge_color_count > 0) {
+        if (!mesh->edge_color_offsets) return fail("ABI41 edge color offsets are required");
+        edge_colors.assign(mesh->edge_color_offsets, mesh->edge_color_offsets + cfg.edge_color_count + 1);
+    }
+    if (cfg.bend_count > 0 && (!mesh->bends || !mesh->bend_rest_lengths))
+        return fail("ABI41 bend constraints require bend pairs and rest lengths");
+    if (cfg.lra_count > 0 && (!mesh->lra_edges || !mesh->lra_rest_lengths))
+        return fail("ABI41 LRA constraints require edge pairs and rest lengths");
+    bend_colors.clear();
+    if (cfg.bend_color_count > 0) {
+        if (!mesh->bend_color_offsets) return fail("ABI41 bend color offsets are required");
+        bend_colors.assign(mesh->bend_color_offsets, mesh->bend_color_offsets + cfg.bend_color_count + 1);
+    }
+    std::vector<int2> h_bends(cfg.bend_count);
+    std::vector<float> h_bend_rest(cfg.bend_count);
+    for (int b = 0; b < cfg.bend_count; ++b) {
+        h_bends[b] = make_int2(mesh->bends[2 * b], mesh->bends[2 * b + 1]);
+        h_bend_rest[b] = mesh->bend_rest_lengths[b];
+    }
+    // The coloured bend pass is a Gauss-Seidel sweep only when no colour
+    // touches a vertex twice; otherwise every colour range is treated as one.
+    bend_coloring_valid = !bend_colors.empty();
+    if (bend_coloring_valid) {
+        std::vector<int> stamp(n, -1);
+        for (int c = 0; c < cfg.bend_color_count && bend_coloring_valid; ++c) {
+            const int lo = std::max(bend_colors[c], 0), hi = std::min(bend_colors[c + 1], cfg.bend_count);
+            for (int b = lo; b < hi; ++b) {
+                const int i = h_bends[b].x, j = h_bends[b].y;
+                if (i < 0 || j < 0 || i >= n || j >= n) continue;
+                if (stamp[i] == c || stamp[j] == c) { bend_coloring_valid = false; break; }
+                stamp[i] = c; stamp[j] = c;
+            }
+        }
+    }
+    if (cfg.bend_count > 0 && !bend_coloring_valid) return fail("bend constraints need a valid colouring in this build");
+    h_bend_segs.clear();
+    bend_seg_first.clear();
+    bend_seg_count.clear();
+    for (int c = 0; c < cfg.bend_color_count; ++c) {
+        const int lo = std::max(bend_colors[c], 0), hi = std::min(bend_colors[c + 1], cfg.bend_count);
+        bend_seg_first.push_back(static_cast<int>(h_bend_segs.size()));
+        for (int b = lo; b < hi; b += kTile) h_bend_segs.push_back(make_int4(0, b, std::min(hi, b + kTile), 0));
+        bend_seg_count.push_back(static_cast<int>(h_bend_segs.size()) - bend_seg_first.back());
+    }
+    // LRA incidences, ascending constraint order
+    std::vector<int> h_lra_off(n + 1, 0);
+    std::vector<int4> h_lra;
+    lra_invalid = 0;
+    for (int k = 0; k < cfg.lra_count; ++k) {
+        const int i = mesh->lra_edges[2 * k], j = mesh->lra_edges[2 * k + 1];
+        if (i < 0 || j < 0 || i >= n || j >= n || i == j) { lra_invalid++; continue; }
+        h_lra_off[i + 1]++; h_lra_off[j + 1]++;
+    }
+    for (int i = 0; i < n; ++i) h_lra_off[i + 1] += h_lra_off[i];
+    h_lra.resize(h_lra_off[n]);
+    {
+        std::vector<int> cur(h_lra_off.begin(), h_lra_off.end() - 1);
+        for (int k = 0; k < cfg.lra_count; ++k) {
+            const int i = mesh->lra_edges[2 * k], j = mesh->lra_edges[2 * k + 1];
+            if (i < 0 || j < 0 || i >= n || j >= n || i == j) continue;
+            int rb;
+            std::memcpy(&rb, &mesh->lra_rest_lengths[k], 4);
+            h_lra[cur[i]++] = make_int4(static_cast<int>(static_cast<unsigned>(j) | 0x80000000u), rb, k, 0);
+            h_lra[cur[j]++] = make_int4(i, rb, k, 0);
+        }
+    }
+    // triangles and the incident-triangle lists (A41:7051-7109)
+    std::vector<int> tri(3 * static_cast<size_t>(cfg.triangle_count));
+    for (int t = 0; t < 3 * cfg.triangle_count; ++t) tri[t] = mesh->triangles ? std::max(mesh->triangles[t], 0) : 0;
+    std::vector<int4> h_tris(cfg.triangle_count);
+    for (int t = 0; t < cfg.triangle_count; ++t) h_tris[t] = make_int4(tri[3 * t], tri[3 * t + 1], tri[3 * t + 2], 0);
+    std::vector<int> h_tri_off(n + 1, 0);
+    auto tri_ok = [&](int t) { return tri[3 * t] < n && tri[3 * t + 1] < n && tri[3 * t + 2] < n; };
+    for (int t = 0; t < cfg.triangle_count; ++t)
+        if (tri_ok(t)) for (int k = 0; k < 3; ++k) h_tri_off[tri[3 * t + k] + 1]++;
+    for (int i = 0; i < n; ++i) h_tri_off[i + 1] += h_tri_off[i];
+    std::vector<int> h_tri_inc(h_tri_off[n]);
+    {
+        std::vector<int> cur(h_tri_off.begin(), h_tri_off.end() - 1);
+        for (int t = 0; t < cfg.triangle_count; ++t)
+            if (tri_ok(t)) for (int k = 0; k < 3; ++k) h_tri_inc[cur[tri[3 * t + k]]++] = t;
+    }
+    std::vector<float> h_share, h_scale;
+    pressure_setup(*this, P, tri, edges, edge_rest, h_share, h_scale);
+
+    ntiles = (n + kTile - 1) / kTile;
+    std::vector<Tile> h_tiles(ntiles);
+    for (int t = 0; t < ntiles; ++t) h_tiles[t] = Tile{0, t * kTile, std::min(n, (t + 1) * kTile), 0};
+    std::vector<int> h_pin_off(n + 1, 0);
+
+    if (!sync_stream()) return false;
+    bool ok = pos0.upload(h_pos, "position allocation") && pos1.upload(h_pos, "position allocation") &&
+              prev.upload(h_prev, "previous position allocation") && vel.upload(h_zero, "velocity allocation") &&
+              rest.upload(h_rest, "rest position allocation") && adj_off.upload(h_adj_off, "edge incidence allocation") &&
+              adj.upload(h_adj, "edge incidence allocation") && lra_off.upload(h_lra_off, "LRA incidence allocation") &&
+              lra_adj.upload(h_lra, "LRA incidence allocation") && tri_off.upload(h_tri_off, "triangle incidence allocation") &&
+              tri_inc.upload(h_tri_inc, "triangle incidence allocation") && tris.upload(h_tris, "triangle allocation") &&
+              bends.upload(h_bends, "bend allocation") && bend_rest.upload(h_bend_rest, "bend rest allocation") &&
+              bend_segs.upload(h_bend_segs, "bend segment allocation") && area_share.upload(h_share, "pressure share allocation") &&
+              rest_scale.upload(h_scale, "pressure scale allocation") && pin_off.upload(h_pin_off, "pin allocation") &&
+              counts.alloc(kCountSlots, "diagnostic allocation") && max_smoothed.alloc(1, "diagnostic allocation") &&
+              d_view.alloc(1, "view allocation") && d_tiles.upload(h_tiles, "tile allocation");
+    if (!ok) return false;
+    if (!hip_ok(hipMemset(counts.p, 0, sizeof(unsigned long long) * kCountSlots), "diagnostic reset")) return false;
+    if (!hip_ok(hipMemset(max_smoothed.p, 0, sizeof(float)), "diagnostic reset")) return false;
+    if (cfg.self_collision && !self.prepare(cfg, edges, tri, n)) return false;
+    diag.finite_flag = 1;
+    const int stc = std::max(cfg.static_triangle_count, 0);
+    if (stc > 0 && !mesh->static_triangles) return fail("static triangle data is required when triangle count is nonzero");
+    return build_sdf(mesh->static_triangles, stc);
+}
+
+// A41:6026-6063 closed-soup test on a 1e-6 welding grid.
+struct QKey { long long x, y, z; bool operator==(const QKey& o) const { return x == o.x && y == o.y && z == o.z; } };
+struct QKeyHash {
+    size_t operator()(const QKey& k) const {
+        uint64_t h = static_cast<uint64_t>(k.x) * 73856093ull ^ static_cast<uint64_t>(k.y) * 19349663ull ^ static_cast<uint64_t>(k.z) * 83492791ull;
+        h ^= h >> 33; h *= 0xff51afd7ed558ccdull; h ^= h >> 33;
+        return static_cast<size_t>(h);
+    }
+};
+static bool closed_soup(const float* t, int count) {
+    if (!t || count <= 0) return false;
+    std::unordered_map<QKey, int, QKeyHash> ids;
+    std::unordered_map<uint64_t, int> edge_uses;
+    auto vid = [&](const float* v) {
+        const QKey k{std::llround(double(v[0]) * 1e6), std::llround(double(v[1]) * 1e6), std::llround(double(v[2]) * 1e6)};
+        auto f = ids.find(k);
+        if (f != ids.end()) return f->second;
+        const int id = static_cast<int>(ids.size());
+        ids.emplace(k, id);
+        return id;
+    };
+    auto key = [](int a, int b) { return (uint64_t(unsigned(std::min(a, b))) << 32) | uint64_t(unsigned(std::max(a, b))); };
+    for (int i = 0; i < count; ++i) {
+        const float* a = t + 9 * i;
+        for (int k = 0; k < 9; ++k) if (!std::isfinite(a[k])) return false;
+        const int ia = vid(a), ib = vid(a + 3), ic = vid(a + 6);
+        if (ia == ib || ib == ic || ic == ia) continue;
+        edge_uses[key(ia, ib)]++; edge_uses[key(ib, ic)]++; edge_uses[key(ic, ia)]++;
+    }
+    if (edge_uses.empty()) return false;
+    for (auto& kv : edge_uses) if (kv.second != 2) return false;
+    return true;
+}
+
+// A41:6123-6218 grid set-up on the host, distances on the device.
+bool Solver::build_sdf(const float* triangles, int count) {
+    sdf_ready = 0;
+    sdf_dim[0] = sdf_dim[1] = sdf_dim[2] = 0;
+    sdf_signed = 0;
+    nstatic = 0;
+    cfg.static_triangle_count = count;
+    if (count <= 0) return true;
+    const auto t0 = std::chrono::steady_clock::now();
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = 0; i < 3 * count; ++i) {
+        const float* v = triangles + 3 * i;
+        if (!std::isfinite(v[0]) || !std::isfinite(v[1]) || !std::isfinite(v[2]))
+            return fail("static triangle data contains non-finite coordinates");
+        for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], v[k]); hi[k] = std::max(hi[k], v[k]); }
+    }
+    float voxel = (std::isfinite(cfg.static_sdf_voxel_size) && cfg.static_sdf_voxel_size > 0.0f)
+                      ? cfg.static_sdf_voxel_size : std::max(std::max(cfg.collision_margin, 0.0f) * 0.5f, 0.002f);
+    const int band = std::max(cfg.static_sdf_band_voxels > 0 ? cfg.static_sdf_band_voxels : 4, 1);
+    const int max_res = std::max(cfg.static_sdf_max_resolution > 0 ? cfg.static_sdf_max_resolution : 160, 16);
+    const float expand = std::max(cfg.collision_margin, 0.0f) + static_cast<float>(band) * voxel + voxel * 2.0f;
+    float ext[3];
+    for (int k = 0; k < 3; ++k) { lo[k] -= expand; hi[k] += expand; ext[k] = std::max(hi[k] - lo[k], voxel); }
+    const float largest = std::max(ext[0], std::max(ext[1], ext[2]));
+    if (largest / voxel + 1.0f > static_cast<float>(max_res)) voxel = largest / static_cast<float>(std::max(max_res - 1, 1));
+    voxel = std::max(voxel, 1.0e-5f);
+    int dim[3];
+    for (int k = 0; k < 3; ++k) dim[k] = std::min(std::max(static_cast<int>(std::ceil(ext[k] / voxel)) + 1, 2), max_res);
+    const long long voxels = 1LL * dim[0] * dim[1] * dim[2];
+    if (voxels <= 0 || voxels > std::numeric_limits<int>::max()) return fail("static SDF grid is too large");
+    const int sgn = closed_soup(triangles, count) ? 1 : 0;
+    if (!sgn) sdf_unsigned++;
+    if (!sync_stream()) return false;
+    std::vector<float> h_tris(triangles, triangles + 9 * static_cast<size_t>(count));
+    if (!static_tris.upload(h_tris, "static triangle upload")) return false;
+    if (sdf.n < static_cast<size_t>(voxels) && !sdf.alloc(static_cast<size_t>(voxels), "static SDF allocation")) return false;
+    launch_sdf_build(lib_stream(), sdf.p, dim[0], dim[1], dim[2], lo[0], lo[1], lo[2], voxel, static_tris.p, count, sgn);
+    if (!hip_ok(hipGetLastError(), "build static SDF") || !sync_stream()) return false;
+    for (int k = 0; k < 3; ++k) { sdf_dim[k] = dim[k]; sdf_origin[k] = lo[k]; }
+    sdf_voxel = voxel;
+    sdf_signed = sgn;
+    nstatic = count;
+    sdf_ready = 1;
+    sdf_rebuilds++;
+    sdf_build_ms = static_cast<float>(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    return true;
+}
+
+// A41:6996-7049.  Pins are kept on the host and turned into the per-vertex
+// device lists with their relaxation factor (A41:3466-3477), which depends on
+// the iteration count of the step.
+bool Solver::upload_pins(const int* idx, const float* p, const float* w, int count) {
+    if (count < 0) return fail("invalid pin count");
+    if (count > 0 && (!idx || !p)) return fail("pin arrays are required when pin count is nonzero");
+    pins.clear();
+    for (int k = 0; k < count; ++k) pins.push_back(HostPin{idx[k], {p[3 * k], p[3 * k + 1], p[3 * k + 2]}, w ? w[k] : 1.0f});
+    pins_iterations = -1;
+    return true;
+}
+
+bool Solver::refresh_pins(int iterations) {
+    if (pins_iterations == iterations) return true;
+    const float exponent = 1.0f / static_cast<float>(std::max(iterations + 1, 1));
+    std::vector<int> off(nv + 1, 0), ids;
+    std::vector<float4> h(pins.size());
+    for (size_t k = 0; k < pins.size(); ++k) {
+        const HostPin& pin = pins[k];
+        const float w = pin.weight;
+        float relax;
+        if (!std::isfinite(w) || w < 0.05f) relax = -1.0f;
+        else if (w >= 0.75f) relax = 2.0f;
+        else {
+            const float total = std::min(std::max(w / 0.75f, 0.0f), 1.0f);
+            if (total >= 1.0f) relax = 1.0f;
+            else {
+                const float remaining = std::max(1.0f - total, 0.0f);
+                const float r = 1.0f - std::pow(remaining, std::max(exponent, 1.0e-4f));
+                relax = std::isfinite(r) ? std::min(std::max(r, 0.0f), 1.0f) : total;
+            }
+        }
+        h[k] = make_float4(pin.target[0], pin.target[1], pin.target[2], relax);
+        if (pin.index >= 0 && pin.index < nv) off[pin.index + 1]++;
+    }
+    for (int i = 0; i < nv; ++i) off[i + 1] += off[i];
+    ids.assign(off[nv], 0);
+    std::vector<int> cur(off.begin(), off.end() - 1);
+    for (size_t k = 0; k < pins.size(); ++k)
+        if (pins[k].index >= 0 && pins[k].index < nv) ids[cur[pins[k].index]++] = static_cast<int>(k);
+    if (!sync_stream()) return false;
+    if (!pin_off.upload(off, "pin upload") || !pin_ids.upload(ids, "pin upload") || !d_pins.upload(h, "pin upload")) return false;
+    pins_iterations = iterations;
+    return true;
+}
+
+bool Solver::upload_fields(const SsblXpbdForceField* f, int count, int unsupported) {  // A41:5920-5964
+    if (count < 0 || count > kMaxForceFields) return fail("force field count exceeds SSBL maximum");
+    unsupported_fields = std::max(unsupported, 0);
+    if (count > 0 && !f) return fail("missing force field data");
+    fields.assign(f, f + count);
+    if (!sync_stream()) return false;
+    return d_fields.upload(fields, "force field upload");
+}
+
+void Solver::fill_view(int substeps) {
+    ClothView& v = view;
+    v = ClothView{};
+    v.nv = nv; v.ne = cfg.edge_count; v.nt = cfg.triangle_count; v.nbend = cfg.bend_count; v.nlra = cfg.lra_count;
+    v.npin = static_cast<int>(pins.size()); v.nfield = static_cast<int>(fields.size());
+    v.pos0 = pos0.p; v.pos1 = pos1.p; v.prev = prev.p; v.vel = vel.p; v.rest = rest.p;
+    v.adj_off = adj_off.p; v.adj = adj.p; v.lra_off = lra_off.p; v.lra_adj = lra_adj.p;
+    v.tri_off = tri_off.p; v.tri_inc = tri_inc.p; v.tris = tris.p; v.bends = bends.p; v.bend_rest = bend_rest.p;
+    v.pin_off = pin_off.p; v.pin_ids = pin_ids.p; v.pins = d_pins.p;
+    v.area_share = area_share.p; v.rest_scale = rest_scale.p; v.fields = d_fields.p;
+    v.sdf = sdf.p; v.static_tris = static_tris.p; v.nstatic = nstatic;
+    v.counts = counts.p; v.max_smoothed = max_smoothed.p;
+    const float dt = cfg.dt / static_cast<float>(substeps);
+    v.dt_sub = dt;
+    v.damping = cfg.damping;
+    for (int k = 0; k < 3; ++k) v.gravity[k] = cfg.gravity[k];
+    v.stretch_alpha = std::max(cfg.stretch_compliance, 0.0f) / std::max(dt * dt, 1.0e-7f);
+    v.bend_alpha = std::max(cfg.bend_compliance, 0.0f) / std::max(dt * dt, 1.0e-7f);
+    v.lra_compliance = cfg.lra_compliance;
+    v.collision_margin = cfg.collision_margin;
+    v.thickness = cfg.cloth_thickness;
+    v.friction = cfg.contact_friction;
+    v.tangent_damping = cfg.contact_tangent_damping;
+    v.use_pressure = (cfg.use_volume_pressure && cfg.pressure_strength > 0.0f) ? 1 : 0;
+    v.pressure_strength = cfg.pressure_strength;
+    v.pressure_accel_scale = pressure_accel_scale;
+    v.pressure_impulse_rest_scale = pressure_impulse_rest_scale;
+    v.pressure_velocity_rest_scale = pressure_velocity_rest_scale;
+    v.pressure_velocity_length_scale = pressure_velocity_length_scale;
+    v.colliders = colliders;
+    v.sdf_ready = sdf_ready;
+    for (int k = 0; k < 3; ++k) { v.sdf_dim[k] = sdf_dim[k]; v.sdf_origin[k] = sdf_origin[k]; }
+    v.sdf_voxel = sdf_voxel;
+    v.need_normal = (v.use_pressure || !fields.empty()) ? 1 : 0;
+    v.contacts_on = (analytic_on() || sdf_ready) ? 1 : 0;
+}
+
+// ------------------------------------------------------------ schedule
+// One step (A41:9165-9378) of every member of `members` (one launch per phase
+// covers all of them).  `ctx` addresses their views and tiles.
+struct StepPlan {
+    bool any_pins = false, any_edges = false, any_lra = false, any_bend = false, any_self = false;
+    int max_bend_colors = 0;
+};
+
+static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const std::vector<int4>* bend_segs_host,
+                         const int4* bend_segs_dev, const std::vector<int>& seg_first, const std::vector<int>& seg_count,
+                         SelfBatch* self_batch, int substeps, int iterations, bool count) {
+    StepPlan plan;
+    for (Solver* s : members) {
+        plan.any_pins |= !s->pins.empty();
+        plan.any_edges |= s->cfg.edge_count > 0;
+        plan.any_lra |= s->cfg.lra_count > 0;
+        plan.any_bend |= s->cfg.bend_count > 0;
+        plan.any_self |= s->cfg.self_collision && s->nv > 0 && s->self_radius() > 0.0f;
+    }
+    (void)bend_segs_host;
+    const unsigned cnt = count ? kOpCount : 0u;
+    const unsigned pin_ops = plan.any_pins ? (kOpPin | kOpPinPolish) : 0u;
+    int parity = 0;
+    for (int s = 0; s < substeps; ++s) {
+        launch_integrate(ctx, parity, plan.any_pins ? kOpPin : 0u);
+        parity ^= 1;
+        for (int it = 0; it < iterations; ++it) {
+            const bool last = it == iterations - 1;
+            const bool final_c = last && s == substeps - 1;
+            const unsigned last_op = last ? kOpLastIter : 0u;
+            const unsigned stretch = plan.any_edges ? kOpStretch : 0u;
+            bool self_now = false;
+            if (plan.any_self && last) {
+                for (Solver* m : members) {
+                    const int interval = std::max(m->cfg.self_collision_interval, 1);
+                    if (m->cfg.self_collision && (((s + 1) % interval) == 0 || s == substeps - 1)) self_now = true;
+                }
+            }
+            const bool split_final = final_c && (plan.any_lra || plan.any_bend);
+            if (!split_final && !self_now) {
+                launch_vertex_pass(ctx, parity, parity ^ 1,
+                                   stretch | pin_ops | kOpAnalytic | kOpSdf | last_op | (last ? kOpVelocity : 0u) | cnt);
+                parity ^= 1;
+                continue;
+            }
+            if (split_final) {
+                // stretch, then LRA and the coloured bend sweep, then the
+                // vertex-local contact work (A41:9178-9229 order)
+                launch_vertex_pass(ctx, parity, parity ^ 1, stretch | cnt);
+                parity ^= 1;
+                if (plan.any_lra) {
+                    launch_vertex_pass(ctx, parity, parity ^ 1, kOpLra | cnt);
+                    parity ^= 1;
+                }
+                if (plan.any_bend) {
+                    for (size_t c = 0; c < seg_first.size(); ++c)
+                        launch_bend_color(ctx, bend_segs_dev + seg_first[c], seg_count[c], parity, cnt);
+                }
+                launch_vertex_pass(ctx, parity, parity,
+                                   pin_ops | kOpAnalytic | kOpSdf | last_op | (self_now ? 0u : kOpVelocity) | cnt);
+            } else {
+                launch_vertex_pass(ctx, parity, parity ^ 1, stretch | (pin_ops & ~kOpPinPolish) | kOpAnalytic | kOpSdf | cnt);
+                parity ^= 1;
+            }
+            if (self_now) {
+                if (!self_batch || !self_batch->run(ctx, parity, s, substeps, count)) return false;
+                launch_vertex_pass(ctx, parity, parity, (pin_ops & kOpPinPolish) | last_op | kOpVelocity | cnt);
+            }
+        }
+    }
+    if (parity != 0) launch_vertex_pass(ctx, 1, 0, 0u);
+    return hip_ok(hipGetLastError(), "launch ABI41 substep");
+}
+
+static void reset_diag(Solver& s) {
+    s.diag = SsblXpbdDiagnostics{};
+    s.diag.finite_flag = 1;
+    s.diag.static_triangle_count = s.cfg.static_triangle_count;
+    s.diag.force_field_count = static_cast<long long>(s.fields.size());
+    s.diag.unsupported_force_field_count = s.unsupported_fields;
+    s.diag.static_sdf_rebuild_count = s.sdf_rebuilds;
+    s.diag.static_sdf_voxel_count = s.sdf_ready ? 1LL * s.sdf_dim[0] * s.sdf_dim[1] * s.sdf_dim[2] : 0;
+    s.diag.static_sdf_grid_x = s.sdf_dim[0];
+    s.diag.static_sdf_grid_y = s.sdf_dim[1];
+    s.diag.static_sdf_grid_z = s.sdf_dim[2];
+    s.diag.static_sdf_build_ms = s.sdf_build_ms;
+    s.diag.static_sdf_unsigned_fallback_count = s.sdf_unsigned;
+}
+
+static bool fetch_counts(Solver& s) {  // A41:8059-8179
+    unsigned long long c[kCountSlots] = {};
+    float ms = 0.0f;
+    if (!hip_ok(hipMemcpy(c, s.counts.p, sizeof(c), hipMemcpyDeviceToHost), "fetch ABI41 diagnostics")) return false;
+    if (!hip_ok(hipMemcpy(&ms, s.max_smoothed.p, sizeof(float), hipMemcpyDeviceToHost), "fetch ABI41 diagnostics")) return false;
+    SsblXpbdDiagnostics& d = s.diag;
+    d.abi41_soft_contact_count = static_cast<long long>(c[kCountSoft]);
+    d.abi41_exact_impulse_contact_count = static_cast<long long>(c[kCountExactImpulse]);
+    d.abi41_edge_edge_contact_count = static_cast<long long>(c[kCountEdgeEdge]);
+    d.abi41_hard_projection_fallbacks = static_cast<long long>(c[kCountHardFallbacks]);
+    d.abi41_max_smoothed_delta = ms;
+    d.fast_soft_repulsion_candidates = static_cast<long long>(c[kCountSelfCandidates]);
+    d.fast_soft_repulsion_applied = static_cast<long long>(c[kCountSoft]);
+    d.external_friction_corrections = static_cast<long long>(c[kCountFriction]);
+    d.self_candidate_count = static_cast<long long>(c[kCountSelfCandidates]);
+    d.static_sdf_contact_count = static_cast<long long>(c[kCountStaticSdf]);
+    d.candidate_count = static_cast<long long>(c[kCountSelfCandidates] + c[kCountStaticSdf]);
+    d.abi41_lra_tack_count = static_cast<long long>(c[kCountLraTacks]);
+    d.abi41_bending_wing_count = static_cast<long long>(c[kCountBendingWings]);
+    d.abi41_tack_jitter_guarded = static_cast<long long>(c[kCountTackGuards]) + (s.cfg.lra_count > 0 ? s.lra_invalid : 0);
+    d.abi41_bending_guarded = static_cast<long long>(c[kCountBendingGuards]);
+    return true;
+}
+
+static bool prepare_member(Solver& s, int substeps, int iterations, bool count) {
+    if (!s.pins.empty() && !s.refresh_pins(iterations)) return false;
+    s.fill_view(substeps);
+    if (count) {
+        launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.counts.p), 0u, 2 * static_cast<size_t>(kCountSlots));
+        launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.max_smoothed.p), 0u, 1);
+    }
+    return true;
+}
+
+static bool step_one(Solver& s, int substeps, int iterations, int fetch, int force_sync) {
+    const auto t0 = std::chrono::steady_clock::now();
+    reset_diag(s);
+    substeps = std::max(substeps, 1);
+    iterations = std::max(iterations, 1);
+    const bool count = fetch != 0;
+    if (!prepare_member(s, substeps, iterations, count)) return false;
+    if (!s.staging.upload(s.d_view.p, &s.view, sizeof(ClothView), lib_stream())) return false;
+    if (s.nv > 0) {
+        LaunchCtx ctx{lib_stream(), s.d_view.p, s.d_tiles.p, s.ntiles};
+        std::vector<Solver*> one{&s};
+        SelfBatch* sb = s.cfg.self_collision ? s.self.single_batch(s.view, s.d_view.p) : nullptr;
+        if (!run_schedule(one, ctx, &s.h_bend_segs, s.bend_segs.p, s.bend_seg_first, s.bend_seg_count, sb, substeps,
+                          iterations, count))
+            return false;
+    }
+    if (force_sync || fetch) {
+        const auto ts = std::chrono::steady_clock::now();
+        if (!sync_stream()) return false;
+        s.diag.sync_ms = static_cast<float>(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count());
+    }
+    if (fetch && !fetch_counts(s)) return false;
+    s.diag.step_ms = static_cast<float>(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    s.diag.constraints_ms = s.diag.step_ms;
+    s.diag.resolved_contacts = s.diag.abi41_soft_contact_count + s.diag.abi41_exact_impulse_contact_count +
+                               s.diag.abi41_edge_edge_contact_count + s.diag.static_sdf_contact_count;
+    return true;
+}
+
+// ----------------------------------------------------------------- batch
+struct Batch {
+    std::vector<Solver*> members;
+    Dev<ClothView> d_views;
+    Dev<Tile> d_tiles;
+    Dev<int4> d_bend_segs;
+    std::vector<int4> h_bend_segs;
+    std::vector<int> seg_first, seg_count;
+    std::vector<ClothView> h_views;
+    Staging staging;
+    int ntiles = 0;
+    SelfBatchHolder self;
+};
+
+static bool build_batch(Batch& b) {
+    std::vector<Tile> tiles;
+    int max_colors = 0;
+    for (size_t m = 0; m < b.members.size(); ++m) {
+        Solver* s = b.members[m];
+        for (int t = 0; t < s->ntiles; ++t) tiles.push_back(Tile{static_cast<int>(m), t * kTile, std::min(s->nv, (t + 1) * kTile), 0});
+        max_colors = std::max(max_colors, static_cast<int>(s->bend_seg_first.size()));
+        if (s->stretch_mode != b.members[0]->stretch_mode) return fail("batch members must share the stretch mode");
+    }
+    b.h_bend_segs.clear();
+    b.seg_first.clear();
+    b.seg_count.clear();
+    for (int c = 0; c < max_colors; ++c) {
+        b.seg_first.push_back(static_cast<int>(b.h_bend_segs.size()));
+        for (size_t m = 0; m < b.members.size(); ++m) {
+            Solver* s = b.members[m];
+            if (c >= static_cast<int>(s->bend_seg_first.size())) continue;
+            for (int k = 0; k < s->bend_seg_count[c]; ++k) {
+                int4 seg = s->h_bend_segs[s->bend_seg_first[c] + k];
+                seg.x = static_cast<int>(m);
+                b.h_bend_segs.push_back(seg);
+            }
+        }
+        b.seg_count.push_back(static_cast<int>(b.h_bend_segs.size()) - b.seg_first.back());
+    }
+    b.ntiles = static_cast<int>(tiles.size());
+    b.h_views.resize(b.members.size());
+    return b.d_views.alloc(b.members.size(), "batch view allocation") && b.d_tiles.upload(tiles, "batch tile allocation") &&
+           b.d_bend_segs.upload(b.h_bend_segs, "batch bend allocation");
+}
+
+static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
+    substeps = std::max(substeps, 1);
+    iterations = std::max(iterations, 1);
+    for (size_t m = 0; m < b.members.size(); ++m) {
+        Solver& s = *b.members[m];
+        reset_diag(s);
+        if (!prepare_member(s, substeps, iterations, false)) return false;
+        b.h_views[m] = s.view;
+    }
+    if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
+    LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles};
+    SelfBatch* sb = b.self.get(b.members, b.h_views, b.d_views.p);
+    if (!run_schedule(b.members, ctx, &b.h_bend_segs, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false))
+        return false;
+    return force_sync ? sync_stream() : true;
+}
+
+}  // namespace ssbl
+
+using namespace ssbl;
+
+// =================================================================== ABI41
+extern "C" {
+
+SSBL_API void* ssbl_create_solver(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
+    g_error.clear();
+    if (!config || config->vertex_count < 0 || config->edge_count < 0 || config->triangle_count < 0 || !std::isfinite(config->dt) ||
+        config->dt <= 0.0f || !mesh || !mesh->positions || !mesh->inv_mass) {
+        fail("invalid ABI41 solver create request");
+        return nullptr;
+    }
+    Solver* s = new Solver();
+    if (!s->create(config, mesh)) {
+        const std::string err = g_error;
+        delete s;
+        g_error = err;
+        return nullptr;
+    }
+    return s;
+}
+
+SSBL_API int ssbl_destroy_solver(void* handle) {
+    g_error.clear();
+    if (handle) { sync_stream(); delete static_cast<Solver*>(handle); }
+    return 1;
+}
+
+SSBL_API int ssbl_reset_solver(void* handle) {  // A41:8975-8990
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail("invalid solver handle") ? 1 : 0;
+    if (!sync_stream()) return 0;
+    std::vector<float4> p(s->nv), q(s->nv), z(s->nv, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (int i = 0; i < s->nv; ++i) {
+        p[i] = make_float4(s->h_rest[i].x, s->h_rest[i].y, s->h_rest[i].z, s->h_weff[i]);
+        q[i] = make_float4(s->h_rest[i].x, s->h_rest[i].y, s->h_rest[i].z, 0.0f);
+    }
+    if (!s->pos0.upload(p, "reset positions") || !s->pos1.upload(p, "reset positions") || !s->prev.upload(q, "reset previous positions") ||
+        !s->vel.upload(z, "reset velocities"))
+        return 0;
+    s->diag = SsblXpbdDiagnostics{};
+    s->diag.finite_flag = 1;
+    return 1;
+}
+
+SSBL_API int ssbl_update_pin_targets(void* handle, const int* indices, const float* positions, const float* weights, int count) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail("invalid solver handle") ? 1 : 0;
+    return s->upload_pins(indices, positions, weights, count) ? 1 : 0;
+}
+
+SSBL_API int ssbl_update_runtime_colliders(void* handle, const SsblXpbdRuntimeColliders* in) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !in) return fail("invalid runtime collider update") ? 1 : 0;
+    s->colliders = *in;
+    return 1;
+}
+
+SSBL_API int ssbl_update_positions(void* handle, const float* positions, int max_floats) {  // A41:9007-9019
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !positions || max_floats < s->nv * 3) return fail("invalid position upload") ? 1 : 0;
+    if (!sync_stream()) return 0;
+    std::vector<float4> p(s->nv), q(s->nv);
+    for (int i = 0; i < s->nv; ++i) {
+        p[i] = make_float4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], s->h_weff[i]);
+        q[i] = make_float4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 0.0f);
+    }
+    return (s->pos0.upload(p, "upload positions") && s->prev.upload(q, "upload previous positions")) ? 1 : 0;
+}
+
+SSBL_API int ssbl_update_static_triangles(void* handle, const float* triangles, int triangle_count) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail("invalid static triangle update") ? 1 : 0;
+    triangle_count = std::max(triangle_count, 0);
+    if (triangle_count > 0 && !triangles) return fail("static triangle data is required when triangle count is nonzero") ? 1 : 0;
+    return s->build_sdf(triangles, triangle_count) ? 1 : 0;
+}
+
+SSBL_API int ssbl_update_dynamic_triangles(void* handle, const float* triangles, int triangle_count) {
+    g_error.clear();
+    (void)triangles;
+    if (!handle) return fail("invalid dynamic triangle update") ? 1 : 0;
+    if (triangle_count > 0) return fail("dynamic triangle colliders are not available in this build yet") ? 1 : 0;
+    return 1;
+}
+
+SSBL_API int ssbl_update_frame_inputs(void* handle, const SsblXpbdFrameInputs* in) {  // A41:9035-9085
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !in) return fail("invalid frame input update") ? 1 : 0;
+    if (in->update_pin_targets && !s->upload_pins(in->pin_indices, in->pin_positions, in->pin_weights, in->pin_count)) return 0;
+    if (in->update_runtime_colliders) s->colliders = in->runtime_colliders;
+    if ((in->update_dynamic_triangles && in->dynamic_triangle_count > 0) ||
+        (in->update_dynamic_indexed_triangles && in->dynamic_indexed_triangle_count > 0) ||
+        (in->update_dynamic_particles && in->dynamic_particle_count > 0))
+        return fail("dynamic colliders are not available in this build yet") ? 1 : 0;
+    if (in->update_static_triangles) {
+        const int c = std::max(in->static_triangle_count, 0);
+        if (c > 0 && !in->static_triangles) return fail("static triangle data is required when triangle count is nonzero") ? 1 : 0;
+        if (!s->build_sdf(in->static_triangles, c)) return 0;
+    }
+    if (in->update_force_fields && !s->upload_fields(in->force_fields, in->force_field_count, in->unsupported_force_field_count)) return 0;
+    s->diag.force_field_count = static_cast<long long>(s->fields.size());
+    s->diag.unsupported_force_field_count = s->unsupported_fields;
+    return 1;
+}
+
+SSBL_API int ssbl_step_solver_ex(void* handle, int substeps, int iterations, int fetch_diagnostics, int force_sync) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail("invalid solver handle") ? 1 : 0;
+    return step_one(*s, substeps, iterations, fetch_diagnostics, force_sync) ? 1 : 0;
+}
+
+SSBL_API int ssbl_step_solver(void* handle, int substeps, int iterations) {
+    return ssbl_step_solver_ex(handle, substeps, iterations, 1, 1);
+}
+
+static bool download_float4(const Dev<float4>& src, int n, float* out) {
+    std::vector<float4> h(n);
+    if (n > 0 && !hip_ok(hipMemcpy(h.data(), src.p, sizeof(float4) * n, hipMemcpyDeviceToHost), "download ABI41 state")) return false;
+    for (int i = 0; i < n; ++i) { out[3 * i] = h[i].x; out[3 * i + 1] = h[i].y; out[3 * i + 2] = h[i].z; }
+    return true;
+}
+
+SSBL_API int ssbl_download_positions(void* handle, float* out, int max_floats) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !out || max_floats < s->nv * 3) return fail("invalid position download") ? 1 : 0;
+    if (!sync_stream()) return 0;
+    return download_float4(s->pos0, s->nv, out) ? 1 : 0;
+}
+
+SSBL_API int ssbl_get_diagnostics(void* handle, SsblXpbdDiagnostics* out) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !out) return fail("invalid diagnostics request") ? 1 : 0;
+    *out = s->diag;
+    return 1;
+}
+
+SSBL_API unsigned int ssbl_capabilities(void) { return SSBL_CAP_PIN_WEIGHTS | SSBL_CAP_HIP_BATCHED_STEP; }
+
+SSBL_API void* ssbl_create_global_dynamic_scene(void) { g_error.clear(); return new int(0); }
+SSBL_API int ssbl_destroy_global_dynamic_scene(void* scene) { g_error.clear(); delete static_cast<int*>(scene); return 1; }
+SSBL_API int ssbl_update_global_dynamic_scene(void* scene, const SsblXpbdGlobalDynamicSceneInputs* in) {
+    g_error.clear();
+    if (!scene || !in) return fail("invalid global dynamic scene update") ? 1 : 0;
+    if (in->particle_count > 0 || in->triangle_count > 0) return fail("the global dynamic scene is not available in this build yet") ? 1 : 0;
+    return 1;
+}
+SSBL_API int ssbl_attach_global_dynamic_scene(void* handle, void* scene, int, int, int) {
+    g_error.clear();
+    if (!handle || !scene) return fail("invalid global dynamic scene attach") ? 1 : 0;
+    return fail("the global dynamic scene is not available in this build yet") ? 1 : 0;
+}
+SSBL_API int ssbl_clear_global_dynamic_scene(void* handle) {
+    g_error.clear();
+    if (!handle) return fail("invalid global dynamic scene clear") ? 1 : 0;
+    return 1;
+}
+SSBL_API const char* ssbl_last_error(void) { return g_error.c_str(); }
+
+// ============================================================== extensions
+SSBL_API int ssbl_ext_set_stretch_mode(void* handle, int mode) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || (mode != SSBL_EXT_STRETCH_JACOBI && mode != SSBL_EXT_STRETCH_COLORED)) return fail("invalid stretch mode") ? 1 : 0;
+    if (mode == SSBL_EXT_STRETCH_COLORED) return fail("coloured stretch mode is not available in this build yet") ? 1 : 0;
+    s->stretch_mode = mode;
+    return 1;
+}
+
+SSBL_API int ssbl_ext_download_state(void* handle, float* p, float* q, float* v, int max_floats) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || max_floats < s->nv * 3) return fail("invalid state download") ? 1 : 0;
+    if (!sync_stream()) return 0;
+    if (p && !download_float4(s->pos0, s->nv, p)) return 0;
+    if (q && !download_float4(s->prev, s->nv, q)) return 0;
+    if (v && !download_float4(s->vel, s->nv, v)) return 0;
+    return 1;
+}
+
+SSBL_API int ssbl_ext_self_hash_cells(void* handle, int* cells, int max_ints) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !cells || max_ints < 3 * s->nv) return fail("invalid hash cell request") ? 1 : 0;
+    if (!sync_stream()) return 0;
+    return s->self.download_cells(cells, s->nv) ? 1 : 0;
+}
+
+SSBL_API int ssbl_ext_self_neighbors(void* handle, unsigned* counts, unsigned* indices, int max_vertices) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !counts || !indices || max_vertices < s->nv) return fail("invalid neighbour request") ? 1 : 0;
+    if (!sync_stream()) return 0;
+    return s->self.download_neighbors(counts, indices, s->nv) ? 1 : 0;
+}
+
+SSBL_API int ssbl_ext_static_sdf(void* handle, int* dims, float* ov, float* values, int max_values) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !dims || !ov) return fail("invalid SDF request") ? 1 : 0;
+    if (!s->sdf_ready) return fail("no static SDF") ? 1 : 0;
+    for (int k = 0; k < 3; ++k) dims[k] = s->sdf_dim[k];
+    ov[0] = s->sdf_origin[0]; ov[1] = s->sdf_origin[1]; ov[2] = s->sdf_origin[2]; ov[3] = s->sdf_voxel;
+    if (values) {
+        const size_t n = static_cast<size_t>(s->sdf_dim[0]) * s->sdf_dim[1] * s->sdf_dim[2];
+        if (max_values < static_cast<long long>(n)) return fail("SDF buffer too small") ? 1 : 0;
+        if (!sync_stream()) return 0;
+        if (!hip_ok(hipMemcpy(values, s->sdf.p, sizeof(float) * n, hipMemcpyDeviceToHost), "download SDF")) return 0;
+    }
+    return 1;
+}
+
+SSBL_API void* ssbl_ext_create_batch(void* const* handles, int count) {
+    g_error.clear();
+    if (!handles || count <= 0) { fail("invalid batch"); return nullptr; }
+    Batch* b = new Batch();
+    for (int k = 0; k < count; ++k) {
+        if (!handles[k]) { delete b; fail("invalid batch member"); return nullptr; }
+        b->members.push_back(static_cast<Solver*>(handles[k]));
+    }
+    if (!build_batch(*b)) { const std::string e = g_error; delete b; g_error = e; return nullptr; }
+    return b;
+}
+SSBL_API int ssbl_ext_destroy_batch(void* batch) {
+    g_error.clear();
+    if (batch) { sync_stream(); delete static_cast<Batch*>(batch); }
+    return 1;
+}
+SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int force_sync) {
+    g_error.clear();
+    if (!batch) return fail("invalid batch") ? 1 : 0;
+    return step_batch(*static_cast<Batch*>(batch), substeps, iterations, force_sync) ? 1 : 0;
+}
+SSBL_API int ssbl_ext_synchronize(void) { g_error.clear(); return sync_stream() ? 1 : 0; }
+SSBL_API int ssbl_ext_kernel_timing(int enable) {
+    g_error.clear();
+    if (!enable) {
+        sync_stream();
+        std::lock_guard<std::mutex> lock(g_timing_mutex);
+        resolve_pending();
+    }
+    g_timing.store(enable != 0);
+    return 1;
+}
+SSBL_API int ssbl_ext_kernel_stats(char* out, int cap) {
+    g_error.clear();
+    if (!out || cap <= 2) return fail("invalid stats buffer") ? 1 : 0;
+    sync_stream();
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    resolve_pending();
+    std::string js = "{";
+    bool first = true;
+    for (auto& kv : g_kstats) {
+        char buf[256];
+        std::snprintf(buf, sizeof(buf), "%s\"%s\": {\"launches\": %lld, \"total_ms\": %.6f}", first ? "" : ", ", kv.first.c_str(),
+                      kv.second.launches, kv.second.total_ms);
+        js += buf;
+        first = false;
+    }
+    js += "}";
+    g_kstats.clear();
+    if (static_cast<int>(js.size()) + 1 > cap) return fail("stats buffer too small") ? 1 : 0;
+    std::memcpy(out, js.c_str(), js.size() + 1);
+    return 1;
+}
+SSBL_API const char* ssbl_ext_backend_name(void) {
+    static std::string name = [] {
+        hipDeviceProp_t prop{};
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return std::string("hip (no device)");
+        return std::string("hip ") + prop.gcnArchName + " " + prop.name;
+    }();
+    return name.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,293 @@
+"""ctypes bridge to the ABI41 library -- the host-side mirror of the reference's
+native_backend.py (NativeXpbdSolver, native_backend.py:902-1463).
+
+The default library is the gfx950 HIP build next to this file,
+libssbl_xpbd_hip_abi41.so; SSBL_NATIVE_DLL_PATH overrides the path exactly as
+in the reference (native_backend.py:511-514).  A missing or unloadable library
+raises NativeBackendUnavailable -- there is no fallback implementation.
+
+Any library exporting the ABI41 symbols can be passed explicitly (`lib=`); the
+tests use that to drive the CPU oracle through the same class.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+from .scenes import ClothBuildData, SolverOptions
+
+LIBRARY_NAME = "libssbl_xpbd_hip_abi41.so"
+_CACHE: dict[str, ctypes.CDLL] = {}
+
+
+class NativeBackendUnavailable(RuntimeError):
+    pass
+
+
+class NativeSolverError(RuntimeError):
+    pass
+
+
+def dll_path() -> str:
+    override = os.environ.get("SSBL_NATIVE_DLL_PATH")
+    if override:
+        return override
+    return str(Path(__file__).resolve().parent / LIBRARY_NAME)
+
+
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    path = str(path or dll_path())
+    if path in _CACHE:
+        return _CACHE[path]
+    if not os.path.exists(path):
+        raise NativeBackendUnavailable(f"missing ABI41 library {path}; build it with __graft_entry__.build()")
+    try:
+        lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    except OSError as exc:
+        raise NativeBackendUnavailable(f"unable to load ABI41 library {path}: {exc}") from exc
+    abi.bind(lib, with_ext=True)
+    _CACHE[path] = lib
+    return lib
+
+
+def last_error(lib) -> str:
+    raw = lib.ssbl_last_error()
+    return raw.decode("utf-8", "replace") if raw else ""
+
+
+def _fptr(arr):
+    if arr is None:
+        return None
+    return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _iptr(arr):
+    if arr is None:
+        return None
+    return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def config_from_options(cloth: ClothBuildData, options: SolverOptions, static_triangles: np.ndarray) -> abi.Config:
+    """SolverOptions -> SsblXpbdConfig (native_backend.py:712-766)."""
+    cfg = abi.Config()
+    cfg.vertex_count = len(cloth.positions_world)
+    cfg.edge_count = len(cloth.edges)
+    cfg.bend_count = len(cloth.bends)
+    cfg.lra_count = len(cloth.lra_edges)
+    cfg.triangle_count = len(cloth.triangles)
+    cfg.static_triangle_count = len(static_triangles)
+    cfg.edge_color_count = max(len(cloth.edge_color_offsets) - 1, 0)
+    cfg.bend_color_count = max(len(cloth.bend_color_offsets) - 1, 0)
+    cfg.lra_color_count = max(len(cloth.lra_color_offsets) - 1, 0)
+    for name in ("dt", "damping", "stretch_compliance", "bend_compliance", "lra_compliance", "collision_margin",
+                 "ground_height", "sphere_radius", "cloth_thickness", "self_collision_distance", "rest_volume",
+                 "volume_compliance", "pressure_strength", "volume_target_scale", "contact_friction",
+                 "contact_tangent_damping", "contact_compliance", "stretch_optimization_strength", "static_sdf_voxel_size"):
+        setattr(cfg, name, float(getattr(options, name)))
+    for name in ("use_ground", "use_wall", "use_sphere", "self_collision", "self_collision_mode", "self_collision_interval",
+                 "max_self_collision_neighbors", "use_volume_pressure", "volume_solve_interval", "self_probe_interval",
+                 "self_surface_pair_interval", "jitter_stabilizer_enabled", "fast_self_collision_passes",
+                 "stretch_optimization_enabled", "static_sdf_band_voxels", "static_sdf_max_resolution"):
+        setattr(cfg, name, int(getattr(options, name)))
+    cfg.gravity = (ctypes.c_float * 3)(*map(float, options.gravity))
+    cfg.wall_origin = (ctypes.c_float * 3)(*map(float, options.wall_origin))
+    cfg.wall_normal = (ctypes.c_float * 3)(*map(float, options.wall_normal))
+    cfg.sphere_center = (ctypes.c_float * 3)(*map(float, options.sphere_center))
+    return cfg
+
+
+def runtime_colliders_from_options(options: SolverOptions) -> abi.RuntimeColliders:
+    rc = abi.RuntimeColliders()
+    rc.use_ground = int(options.use_ground)
+    rc.ground_height = float(options.ground_height)
+    rc.use_wall = int(options.use_wall)
+    rc.wall_origin = (ctypes.c_float * 3)(*map(float, options.wall_origin))
+    rc.wall_normal = (ctypes.c_float * 3)(*map(float, options.wall_normal))
+    rc.use_sphere = int(options.use_sphere)
+    rc.sphere_center = (ctypes.c_float * 3)(*map(float, options.sphere_center))
+    rc.sphere_radius = float(options.sphere_radius)
+    return rc
+
+
+class NativeXpbdSolver:
+    """One ABI41 solver handle (native_backend.py:902-1463)."""
+
+    def __init__(self, cloth: ClothBuildData, options: SolverOptions, static_triangles: np.ndarray | None = None,
+                 *, lib: ctypes.CDLL | None = None):
+        self._lib = lib if lib is not None else load_library()
+        caps = self._lib.ssbl_capabilities()
+        if options.stretch_optimization_enabled and not caps & abi.CAP_STRETCH_OPTIMIZATION:
+            raise NativeSolverError("loaded ABI41 library does not support hard stretch optimization")
+        if not caps & abi.CAP_PIN_WEIGHTS:
+            raise NativeSolverError("loaded ABI41 library does not support vertex-group pin weights")
+        static = _f32(np.zeros((0, 9)) if static_triangles is None else static_triangles, (-1, 9))
+        self.vertex_count = len(cloth.positions_world)
+        self._keep = dict(
+            positions=_f32(cloth.positions_world, (-1, 3)), inv_mass=_f32(cloth.inv_mass), edges=_i32(cloth.edges),
+            edge_rest=_f32(cloth.edge_rest_lengths), edge_off=_i32(cloth.edge_color_offsets), bends=_i32(cloth.bends),
+            bend_rest=_f32(cloth.bend_rest_lengths), bend_off=_i32(cloth.bend_color_offsets), lra=_i32(cloth.lra_edges),
+            lra_rest=_f32(cloth.lra_rest_lengths), lra_off=_i32(cloth.lra_color_offsets), tris=_i32(cloth.triangles),
+            static=static)
+        k = self._keep
+        cfg = config_from_options(cloth, options, static)
+        mesh = abi.Mesh(
+            positions=_fptr(k["positions"]), inv_mass=_fptr(k["inv_mass"]), edges=_iptr(k["edges"]),
+            edge_rest_lengths=_fptr(k["edge_rest"]), edge_color_offsets=_iptr(k["edge_off"]), bends=_iptr(k["bends"]),
+            bend_rest_lengths=_fptr(k["bend_rest"]), bend_color_offsets=_iptr(k["bend_off"]), lra_edges=_iptr(k["lra"]),
+            lra_rest_lengths=_fptr(k["lra_rest"]), lra_color_offsets=_iptr(k["lra_off"]), triangles=_iptr(k["tris"]),
+            static_triangles=_fptr(k["static"]))
+        self._handle = self._lib.ssbl_create_solver(ctypes.byref(cfg), ctypes.byref(mesh))
+        if not self._handle:
+            raise NativeSolverError(last_error(self._lib))
+        self._keep = None
+        if len(cloth.pin_indices) > 0:
+            self.update_pin_targets(cloth.pin_indices, cloth.pin_targets_world, cloth.pin_weights)
+        if cloth.start_positions is not None:
+            self.update_positions(cloth.start_positions)
+
+    @property
+    def handle(self):
+        return self._handle
+
+    @property
+    def lib(self):
+        return self._lib
+
+    def _check(self, ok) -> None:
+        if not ok:
+            raise NativeSolverError(last_error(self._lib))
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            self._lib.ssbl_destroy_solver(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self) -> None:
+        self._check(self._lib.ssbl_reset_solver(self._handle))
+
+    def update_pin_targets(self, indices, positions, weights=None) -> None:
+        idx = _i32(indices)
+        pos = _f32(positions, (-1, 3))
+        w = None if weights is None or len(weights) == 0 else _f32(weights)
+        self._check(self._lib.ssbl_update_pin_targets(self._handle, _iptr(idx), _fptr(pos), _fptr(w), len(idx)))
+
+    def update_runtime_colliders(self, options: SolverOptions) -> None:
+        rc = runtime_colliders_from_options(options)
+        self._check(self._lib.ssbl_update_runtime_colliders(self._handle, ctypes.byref(rc)))
+
+    def update_positions(self, positions) -> None:
+        p = _f32(positions, (-1, 3))
+        self._check(self._lib.ssbl_update_positions(self._handle, _fptr(p), p.size))
+
+    def update_static_triangles(self, static_triangles) -> None:
+        t = _f32(static_triangles, (-1, 9))
+        self._check(self._lib.ssbl_update_static_triangles(self._handle, _fptr(t), len(t)))
+
+    def update_force_fields(self, fields: list[dict]) -> None:
+        arr = (abi.ForceField * max(len(fields), 1))()
+        for k, f in enumerate(fields):
+            for key, value in f.items():
+                cur = getattr(arr[k], key)
+                setattr(arr[k], key, (ctypes.c_float * 3)(*value) if hasattr(cur, "__len__") else value)
+        fi = abi.FrameInputs()
+        fi.update_force_fields = 1
+        fi.force_fields = ctypes.cast(arr, ctypes.POINTER(abi.ForceField))
+        fi.force_field_count = len(fields)
+        self._check(self._lib.ssbl_update_frame_inputs(self._handle, ctypes.byref(fi)))
+
+    def step(self, substeps: int, iterations: int, diagnostics: bool = True, synchronize: bool = True) -> None:
+        self._check(self._lib.ssbl_step_solver_ex(self._handle, int(substeps), int(iterations), int(bool(diagnostics)),
+                                                   int(bool(synchronize or diagnostics))))
+
+    def download_positions(self) -> np.ndarray:
+        out = np.empty((self.vertex_count, 3), np.float32)
+        self._check(self._lib.ssbl_download_positions(self._handle, _fptr(out), out.size))
+        return out
+
+    def download_state(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        p, q, v = (np.empty((self.vertex_count, 3), np.float32) for _ in range(3))
+        self._check(self._lib.ssbl_ext_download_state(self._handle, _fptr(p), _fptr(q), _fptr(v), p.size))
+        return p, q, v
+
+    def diagnostics(self) -> abi.Diagnostics:
+        d = abi.Diagnostics()
+        self._check(self._lib.ssbl_get_diagnostics(self._handle, ctypes.byref(d)))
+        return d
+
+    def set_stretch_mode(self, mode: int) -> None:
+        self._check(self._lib.ssbl_ext_set_stretch_mode(self._handle, int(mode)))
+
+    def self_hash_cells(self) -> np.ndarray:
+        out = np.empty((self.vertex_count, 3), np.int32)
+        self._check(self._lib.ssbl_ext_self_hash_cells(self._handle, _iptr(out), out.size))
+        return out
+
+    def self_neighbors(self) -> tuple[np.ndarray, np.ndarray]:
+        counts = np.empty(self.vertex_count, np.uint32)
+        idx = np.empty((self.vertex_count, 32), np.uint32)
+        self._check(self._lib.ssbl_ext_self_neighbors(
+            self._handle, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+            idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), self.vertex_count))
+        return counts, idx
+
+    def static_sdf(self) -> tuple[np.ndarray, np.ndarray]:
+        dims = np.zeros(3, np.int32)
+        ov = np.zeros(4, np.float32)
+        self._check(self._lib.ssbl_ext_static_sdf(self._handle, _iptr(dims), _fptr(ov), None, 0))
+        vals = np.empty(int(np.prod(dims)), np.float32)
+        self._check(self._lib.ssbl_ext_static_sdf(self._handle, _iptr(dims), _fptr(ov), _fptr(vals), vals.size))
+        return vals.reshape(dims[2], dims[1], dims[0]), ov
+
+
+class SolverBatch:
+    """Many handles stepped together with one launch per solver phase."""
+
+    def __init__(self, solvers: list[NativeXpbdSolver]):
+        if not solvers:
+            raise ValueError("empty batch")
+        self._lib = solvers[0].lib
+        self.solvers = list(solvers)
+        handles = (ctypes.c_void_p * len(solvers))(*[s.handle for s in solvers])
+        self._handle = self._lib.ssbl_ext_create_batch(handles, len(solvers))
+        if not self._handle:
+            raise NativeSolverError(last_error(self._lib))
+
+    def step(self, substeps: int, iterations: int, synchronize: bool = False) -> None:
+        if not self._lib.ssbl_ext_step_batch(self._handle, int(substeps), int(iterations), int(bool(synchronize))):
+            raise NativeSolverError(last_error(self._lib))
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            self._lib.ssbl_ext_destroy_batch(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize(lib=None) -> None:
+    lib = lib or load_library()
+    if not lib.ssbl_ext_synchronize():
+        raise NativeSolverError(last_error(lib))
