@@ -231,8 +231,12 @@ template <class R> R turbulence(R size, R x, R y, R z) {  // A41:1323-1337
     for (int i = 0; i <= 2; ++i, amp *= R(0.5), fs *= R(2)) sum += (R(0.5) + R(0.5) * perlin(fs * x, fs * y, fs * z)) * amp;
     return sum * (R(4) / R(7));
 }
+// sin and pow of the force fields are evaluated in double and rounded once,
+// so host and device maths libraries agree on the float result.
+template <class R> R sin_r(R x) { return static_cast<R>(std::sin(static_cast<double>(x))); }
+template <class R> R pow_r(R x, R y) { return static_cast<R>(std::pow(static_cast<double>(x), static_cast<double>(y))); }
 template <class R> R field_noise(V3<R> p, int seed, R salt) {  // A41:1339-1347
-    R value = std::sin(p.x * R(12.9898) + p.y * R(78.233) + p.z * R(37.719) + (R(seed) + salt) * R(19.191)) * R(43758.5453);
+    R value = sin_r(p.x * R(12.9898) + p.y * R(78.233) + p.z * R(37.719) + (R(seed) + salt) * R(19.191)) * R(43758.5453);
     return (value - std::floor(value)) * R(2) - R(1);
 }
 template <class R> R falloff(const SsblXpbdForceField& f, R distance, R radial) {  // A41:1355-1386
@@ -245,12 +249,12 @@ template <class R> R falloff(const SsblXpbdForceField& f, R distance, R radial) 
     if (f.use_max_distance && mx > mn && distance > mn) att *= clamp01((mx - distance) / std::max(mx - mn, R(kEpsF)));
     if (f.use_min_distance && f.falloff_power > 0.0f && distance > mn) {
         R ref = std::max(mn, R(1.0e-4));
-        att *= std::pow(ref / std::max(distance, ref), R(f.falloff_power));
+        att *= pow_r(ref / std::max(distance, ref), R(f.falloff_power));
     }
     if (f.use_radial_max && rmx > rmn && radial > rmn) att *= clamp01((rmx - radial) / std::max(rmx - rmn, R(kEpsF)));
     if (f.use_radial_min && f.radial_falloff > 0.0f && radial > rmn) {
         R ref = std::max(rmn, R(1.0e-4));
-        att *= std::pow(ref / std::max(radial, ref), R(f.radial_falloff));
+        att *= pow_r(ref / std::max(radial, ref), R(f.radial_falloff));
     }
     return std::isfinite(att) ? att : R(0);
 }

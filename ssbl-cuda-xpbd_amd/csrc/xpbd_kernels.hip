@@ -263,8 +263,15 @@ __device__ inline float turbulence(float size, float x, float y, float z) {
     for (int i = 0; i <= 2; ++i, amp *= 0.5f, fs *= 2.0f) sum += (0.5f + 0.5f * perlin(fs * x, fs * y, fs * z)) * amp;
     return sum * (4.0f / 7.0f);
 }
+// sin and pow are evaluated in double and rounded once: the hash noise
+// multiplies sin by 43758.5, so a one-ulp difference between maths libraries
+// would change the noise value entirely.
+__device__ __forceinline__ float sin_r(float x) { return static_cast<float>(sin(static_cast<double>(x))); }
+__device__ __forceinline__ float pow_r(float x, float y) {
+    return static_cast<float>(pow(static_cast<double>(x), static_cast<double>(y)));
+}
 __device__ inline float field_noise(F3 p, int seed, float salt) {
-    const float value = sinf(p.x * 12.9898f + p.y * 78.233f + p.z * 37.719f + (static_cast<float>(seed) + salt) * 19.191f) * 43758.5453f;
+    const float value = sin_r(p.x * 12.9898f + p.y * 78.233f + p.z * 37.719f + (static_cast<float>(seed) + salt) * 19.191f) * 43758.5453f;
     return (value - floorf(value)) * 2.0f - 1.0f;
 }
 __device__ inline float falloff(const SsblXpbdForceField& f, float distance, float radial) {
@@ -277,12 +284,12 @@ __device__ inline float falloff(const SsblXpbdForceField& f, float distance, flo
     if (f.use_max_distance && mx > mn && distance > mn) att *= clamp01f((mx - distance) / fmaxf(mx - mn, kEps));
     if (f.use_min_distance && f.falloff_power > 0.0f && distance > mn) {
         const float ref = fmaxf(mn, 1.0e-4f);
-        att *= powf(ref / fmaxf(distance, ref), f.falloff_power);
+        att *= pow_r(ref / fmaxf(distance, ref), f.falloff_power);
     }
     if (f.use_radial_max && rmx > rmn && radial > rmn) att *= clamp01f((rmx - radial) / fmaxf(rmx - rmn, kEps));
     if (f.use_radial_min && f.radial_falloff > 0.0f && radial > rmn) {
         const float ref = fmaxf(rmn, 1.0e-4f);
-        att *= powf(ref / fmaxf(radial, ref), f.radial_falloff);
+        att *= pow_r(ref / fmaxf(radial, ref), f.radial_falloff);
     }
     return isfinite(att) ? att : 0.0f;
 }

@@ -75,9 +75,31 @@ def test_pressure_and_wind(nb, hip, oracle, scenes):
     r.update_force_fields([wind])
     worst = run_and_compare(g, r, 30, 5, 5, every=5)
     print(f"pressure+wind: max |hip - oracle| = {worst:.3e}")
-    # force-field noise uses sinf, whose last bit differs between the device
-    # and host maths libraries; positions stay within the stated tolerance
-    assert worst <= 1.0e-4
+    assert worst <= TOL
+
+
+def test_all_force_field_types(nb, hip, oracle, scenes):
+    cloth = scenes.grid_cloth(20, 0.8, 1.0, seed=13)
+    opts = scenes.SolverOptions(use_ground=True)
+    g, r = make_pair(nb, hip, oracle, cloth, opts, None)
+    base = dict(apply_to_location=1, axis=(0.0, 0.0, 1.0))
+    fields = [
+        dict(base, type=2, strength=-3.0, origin=(0.0, 0.0, 0.2), use_max_distance=1, distance_max=2.0, use_min_distance=1,
+             distance_min=0.1, falloff_power=1.5),
+        dict(base, type=3, strength=2.0, origin=(0.1, 0.0, 0.0), flow=0.5),
+        dict(base, type=4, strength=1.5, size=0.3, origin=(0.0, 0.2, 0.0)),
+        dict(base, type=5, strength=0.5, origin=(0.3, 0.3, 0.5)),
+        dict(base, type=6, strength=1.0, rest_length=0.5, harmonic_damping=0.2, origin=(0.0, 0.0, 1.0)),
+        dict(base, type=7, strength=0.2, size=0.2, origin=(0.0, 0.0, 0.5)),
+        dict(base, type=8, strength=0.7, direction=(0.0, 1.0, 0.0)),
+        dict(base, type=9, strength=0.3, linear_drag=0.2, quadratic_drag=0.1),
+        dict(base, type=10, strength=0.8, noise=1.5, seed=7, use_radial_max=1, radial_max=3.0, use_2d_force=1),
+    ]
+    g.update_force_fields(fields)
+    r.update_force_fields(fields)
+    worst = run_and_compare(g, r, 20, 4, 4, every=5)
+    print(f"force fields: max |hip - oracle| = {worst:.3e}")
+    assert worst <= TOL
 
 
 def test_lra_tethers(nb, hip, oracle, scenes):
