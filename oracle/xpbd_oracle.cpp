@@ -387,6 +387,7 @@ template <class R> struct Solver {
     // self collision
     std::vector<int> self_cells;              // 3V from the last vertex-hash build
     std::vector<unsigned> nb_counts, nb_index;  // V, V*32
+    std::vector<Vec> snap_pos, snap_prev;     // positions the last candidate table was built from
     std::vector<int64_t> acc_q;               // 3V fixed point
     std::vector<int> acc_w;
     float max_smoothed = 0.0f;
@@ -1269,6 +1270,8 @@ template <class R> void Solver<R>::self_collision() {
     for (int pass = 0; pass < passes; ++pass) {
         std::fill(acc_q.begin(), acc_q.end(), 0);
         std::fill(acc_w.begin(), acc_w.end(), 0);
+        snap_pos = pos;
+        snap_prev = prev;
         // neighbour table, A41:4738-4788 (hash) / 4790-4814 (brute force)
         for (int s = 0; s < n; ++s) {
             unsigned acc = 0;
@@ -1800,6 +1803,19 @@ SSBL_API int ssbl_ext_self_neighbors(void* handle, unsigned* counts, unsigned* i
         if (max_vertices < s.cfg.vertex_count) return fail("invalid neighbour request") ? 1 : 0;
         std::copy(s.nb_counts.begin(), s.nb_counts.end(), counts);
         std::copy(s.nb_index.begin(), s.nb_index.end(), indices);
+        return 1;
+    });
+}
+
+// Oracle-only: the positions the last self-collision candidate table was
+// built from (tests enumerate the same pairs by brute force).
+SSBL_API int ssbl_oracle_self_snapshot(void* handle, float* p, float* q, int max_floats) {
+    g_error.clear();
+    if (!handle || !p || !q) return fail("invalid snapshot request") ? 1 : 0;
+    return with(handle, [&](auto& s) {
+        if (max_floats < 3 * s.cfg.vertex_count || s.snap_pos.empty()) return fail("no self-collision snapshot") ? 1 : 0;
+        copy_out(s, s.snap_pos, p);
+        copy_out(s, s.snap_prev, q);
         return 1;
     });
 }

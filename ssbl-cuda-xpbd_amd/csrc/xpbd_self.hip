@@ -1,3 +1,758 @@
-// xpbd_self.hip -- self-collision kernels (spatial hashes, candidate pairs,
-// vertex-vertex / vertex-triangle / edge-edge responses).
+// xpbd_self.hip -- self collision on gfx950: spatial hashes built by counting
+// sort, the vertex-vertex candidate table, and the vertex-vertex,
+// vertex-triangle and edge-edge responses (strict mode).
+//
+// Reference (native/src/ssbl_xpbd_cuda_abi41.cu, A41:<line>): cell and hash
+// A41:654-664, hash builds A41:4550-4575 / 4632-4694 / 5532-5593, candidate
+// table A41:4738-4814 with validity A41:4696-4717, vertex-vertex response
+// A41:4816-4951, vertex-triangle A41:4990-5224, edge-edge A41:5595-5844,
+// averaging A41:4953-4988, sizing A41:6784-6994.
+//
+// The reference's buckets are fixed 32-slot arrays filled by atomics, so their
+// order (and which entries overflow) depends on scheduling.  Here a bucket is
+// a counting-sorted segment whose first 32 entries are put in ascending key
+// order -- the order a query visits -- so a build is deterministic and a query
+// sees exactly the entries the CPU oracle sees.  Responses accumulate into
+// per-vertex 64-bit fixed-point sums (2^-32 m), which makes the sum independent
+// of the order the atomics land in.
 #include "xpbd_self.hpp"
+
+#include "xpbd_math.hpp"
+#include "xpbd_timing.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace ssbl {
+
+namespace {
+
+constexpr int kScanBlock = 1024;          // threads of the scan kernels
+constexpr int kScanItems = 4;             // items per thread per chunk
+constexpr int kScanChunk = kScanBlock * kScanItems;
+constexpr int kMaxPrimCells = 64;         // A41:34
+
+int next_pow2(int v) {
+    int r = 1;
+    v = std::max(v, 1);
+    while (r < v && r < (1 << 30)) r <<= 1;
+    return r;
+}
+
+unsigned grid_for(long long n, int block) { return static_cast<unsigned>(std::max<long long>((n + block - 1) / block, 1)); }
+
+// ----------------------------------------------------------- exclusive scan
+// Inclusive wave64 scan through DPP-free lane shuffles.
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix and writes the block total.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
+    __shared__ int wave_sums[kScanBlock / 64];
+    const int lane = static_cast<int>(threadIdx.x & 63u), wave = static_cast<int>(threadIdx.x >> 6);
+    const int inc = wave_inclusive_scan(v);
+    if (lane == 63) wave_sums[wave] = inc;
+    __syncthreads();
+    if (wave == 0) {
+        const int s = lane < kScanBlock / 64 ? wave_sums[lane] : 0;
+        const int si = wave_inclusive_scan(s);
+        if (lane < kScanBlock / 64) wave_sums[lane] = si - s;
+        if (lane == kScanBlock / 64 - 1) *total = si;
+    }
+    __syncthreads();
+    const int r = wave_sums[wave] + inc - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ block_sums) {
+    const long long base = static_cast<long long>(blockIdx.x) * kScanChunk;
+    int s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long i = base + static_cast<long long>(k) * kScanBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    __shared__ int total;
+    block_exclusive_scan(s, &total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_block_sums(int* __restrict__ sums, int nblocks) {
+    __shared__ int total;
+    int carry = 0;
+    for (int base = 0; base < nblocks; base += kScanBlock) {
+        const int i = base + static_cast<int>(threadIdx.x);
+        const int v = i < nblocks ? sums[i] : 0;
+        const int ex = block_exclusive_scan(v, &total);
+        if (i < nblocks) sums[i] = carry + ex;
+        carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_final(const int* __restrict__ in, int n, const int* __restrict__ block_sums,
+                                                           int* __restrict__ out) {
+    const long long base = static_cast<long long>(blockIdx.x) * kScanChunk;
+    // each thread owns kScanItems consecutive items
+    const long long first = base + static_cast<long long>(threadIdx.x) * kScanItems;
+    int vals[kScanItems];
+    int s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long i = first + k;
+        vals[k] = i < n ? in[i] : 0;
+        s += vals[k];
+    }
+    __shared__ int total;
+    int run = block_exclusive_scan(s, &total) + block_sums[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long i = first + k;
+        if (i < n) out[i] = run;
+        run += vals[k];
+    }
+    if (first <= n - 1 && first + kScanItems > n - 1) out[n] = run;   // grand total after the last item
+}
+
+void exclusive_scan(hipStream_t s, const int* in, int n, int* out, int* tmp) {
+    const int nb = (n + kScanChunk - 1) / kScanChunk;
+    TimedLaunch t("self_hash_scan", s);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp, out);
+}
+
+// ------------------------------------------------------------ hash builds
+__device__ __forceinline__ const float4* cur_pos(const ClothView& c, int parity) { return parity ? c.pos1 : c.pos0; }
+
+__global__ void k_fill_i32(int* p, int value, int n) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = value;
+}
+
+// A41:4550-4575: vertex cell and bucket.
+__global__ __launch_bounds__(256) void k_vhash_count(const ClothView* __restrict__ cv, SelfView sv, int parity) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= sv.nv) return;
+    const float4 p = cur_pos(*cv, parity)[i];
+    const int cx = cell_coord(p.x, sv.cell), cy = cell_coord(p.y, sv.cell), cz = cell_coord(p.z, sv.cell);
+    const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.vh.buckets));
+    sv.vcell[i] = make_int4(cx, cy, cz, b);
+    atomicAdd(&sv.vh.count[b], 1);
+}
+
+__global__ __launch_bounds__(256) void k_vhash_scatter(SelfView sv) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= sv.nv) return;
+    const int b = sv.vcell[i].w;
+    const int slot = atomicAdd(&sv.vh.cursor[b], 1);
+    sv.vh.entries[sv.vh.off[b] + slot] = i;
+}
+
+// Primitive box in cells (A41:4632-4694, 5532-5593) with the >64-cell guard
+// (A41:871-900).  Returns the number of cells.
+__device__ inline int prim_box(F3 lo, F3 hi, F3 fb, float cs, int* b0, int* b1, bool* guarded) {
+    int x0 = cell_coord(lo.x, cs), y0 = cell_coord(lo.y, cs), z0 = cell_coord(lo.z, cs);
+    int x1 = cell_coord(hi.x, cs), y1 = cell_coord(hi.y, cs), z1 = cell_coord(hi.z, cs);
+    const long long sx = static_cast<long long>(x1) - x0 + 1, sy = static_cast<long long>(y1) - y0 + 1,
+                    sz = static_cast<long long>(z1) - z0 + 1;
+    *guarded = false;
+    if (sx > 0 && sy > 0 && sz > 0 && sx * sy * sz > kMaxPrimCells) {
+        x0 = x1 = cell_coord(fb.x, cs);
+        y0 = y1 = cell_coord(fb.y, cs);
+        z0 = z1 = cell_coord(fb.z, cs);
+        *guarded = true;
+    }
+    b0[0] = x0; b0[1] = y0; b0[2] = z0;
+    b1[0] = x1; b1[1] = y1; b1[2] = z1;
+    if (x1 < x0 || y1 < y0 || z1 < z0) return 0;
+    return (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+}
+
+__device__ inline bool tri_valid(const SelfView& sv, int4 t) {
+    return t.x < sv.nv && t.y < sv.nv && t.z < sv.nv && t.x != t.y && t.y != t.z && t.x != t.z;
+}
+__device__ inline bool edge_valid(const SelfView& sv, int2 e) { return e.x < sv.nv && e.y < sv.nv && e.x != e.y; }
+
+__device__ inline bool tri_box(const ClothView& c, const SelfView& sv, int parity, int t, int* b0, int* b1, bool* g) {
+    const int4 tr = sv.tris[t];
+    if (!tri_valid(sv, tr)) return false;
+    const float4* P = cur_pos(c, parity);
+    const F3 a = f3(P[tr.x]), b = f3(P[tr.y]), cc = f3(P[tr.z]);
+    const float m = sv.prim_margin;
+    const F3 lo = f3(fminf(a.x, fminf(b.x, cc.x)) - m, fminf(a.y, fminf(b.y, cc.y)) - m, fminf(a.z, fminf(b.z, cc.z)) - m);
+    const F3 hi = f3(fmaxf(a.x, fmaxf(b.x, cc.x)) + m, fmaxf(a.y, fmaxf(b.y, cc.y)) + m, fmaxf(a.z, fmaxf(b.z, cc.z)) + m);
+    prim_box(lo, hi, mul((a + b) + cc, 1.0f / 3.0f), sv.cell, b0, b1, g);
+    return true;
+}
+
+__device__ inline bool edge_box(const ClothView& c, const SelfView& sv, int parity, int e, int* b0, int* b1, bool* g) {
+    const int2 ed = sv.edges[e];
+    if (!edge_valid(sv, ed)) return false;
+    const float4* P = cur_pos(c, parity);
+    const F3 a = f3(P[ed.x]), b = f3(P[ed.y]);
+    const float m = sv.prim_margin;
+    const F3 lo = f3(fminf(a.x, b.x) - m, fminf(a.y, b.y) - m, fminf(a.z, b.z) - m);
+    const F3 hi = f3(fmaxf(a.x, b.x) + m, fmaxf(a.y, b.y) + m, fmaxf(a.z, b.z) + m);
+    prim_box(lo, hi, mul(a + b, 0.5f), sv.cell, b0, b1, g);
+    return true;
+}
+
+// kind 0 = triangles, 1 = edges; phase 0 counts, phase 1 scatters keys.
+__global__ __launch_bounds__(256) void k_prim_hash(const ClothView* __restrict__ cv, SelfView sv, int parity, int kind,
+                                                   int phase, int count_diag) {
+    const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int n = kind == 0 ? sv.nt : sv.ne;
+    if (k >= n) return;
+    int b0[3], b1[3];
+    bool guarded = false;
+    const bool ok = kind == 0 ? tri_box(*cv, sv, parity, k, b0, b1, &guarded) : edge_box(*cv, sv, parity, k, b0, b1, &guarded);
+    if (!ok) return;
+    const HashView& h = kind == 0 ? sv.th : sv.eh;
+    if (phase == 0 && guarded && count_diag) atomicAdd(&cv->counts[kCountSelfOverflow], 1ull);
+    int idx = 0;
+    for (int z = b0[2]; z <= b1[2]; ++z)
+        for (int y = b0[1]; y <= b1[1]; ++y)
+            for (int x = b0[0]; x <= b1[0]; ++x, ++idx) {
+                const int b = static_cast<int>(hash_cell(x, y, z, h.buckets));
+                if (phase == 0) {
+                    atomicAdd(&h.count[b], 1);
+                } else {
+                    const int slot = atomicAdd(&h.cursor[b], 1);
+                    h.entries[h.off[b] + slot] = k * kMaxPrimCells + idx;
+                }
+            }
+}
+
+// Puts the smallest min(count, 32) keys of every bucket first, ascending.
+__global__ __launch_bounds__(256) void k_bucket_order(HashView h, unsigned long long* counts, int count_diag) {
+    const int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= h.buckets) return;
+    const int s = h.off[b], n = h.count[b];
+    if (n <= 1) return;
+    int* e = h.entries + s;
+    if (n > kBucketSlots && count_diag) atomicAdd(&counts[kCountSelfOverflow], static_cast<unsigned long long>(n - kBucketSlots));
+    if (n <= 2 * kBucketSlots) {
+        for (int i = 1; i < n; ++i) {
+            const int key = e[i];
+            int j = i - 1;
+            while (j >= 0 && e[j] > key) { e[j + 1] = e[j]; --j; }
+            e[j + 1] = key;
+        }
+        return;
+    }
+    for (int i = 0; i < kBucketSlots; ++i) {   // partial selection sort
+        int m = i;
+        for (int j = i + 1; j < n; ++j) if (e[j] < e[m]) m = j;
+        const int t = e[i]; e[i] = e[m]; e[m] = t;
+    }
+}
+
+// ------------------------------------------------------------- responses
+__device__ __forceinline__ void accumulate(const SelfView& sv, const float4* P, int v, F3 d) {
+    if (v < 0 || v >= sv.nv || P[v].w == 0.0f) return;   // pinned vertices take no self correction
+    atomicAdd(&sv.acc[3 * v + 0], static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.x) * kFixedScale)));
+    atomicAdd(&sv.acc[3 * v + 1], static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.y) * kFixedScale)));
+    atomicAdd(&sv.acc[3 * v + 2], static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(d.z) * kFixedScale)));
+    atomicAdd(&sv.accw[v], 1);
+}
+
+// A41:4594-4617
+__device__ inline void triangle_barycentric(F3 p, F3 a, F3 b, F3 c, float* wa, float* wb, float* wc) {
+    const F3 v0 = b - a, v1 = c - a, v2 = p - a;
+    const float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
+    const float denom = d00 * d11 - d01 * d01;
+    if (fabsf(denom) <= kEps) { *wa = *wb = *wc = 1.0f / 3.0f; return; }
+    float v = (d11 * d20 - d01 * d21) / denom;
+    float w = (d00 * d21 - d01 * d20) / denom;
+    v = fminf(fmaxf(v, 0.0f), 1.0f);
+    w = fminf(fmaxf(w, 0.0f), 1.0f - v);
+    const float u = fmaxf(1.0f - v - w, 0.0f);
+    const float sum = fmaxf(u + v + w, kEps);
+    *wa = u / sum; *wb = v / sum; *wc = w / sum;
+}
+
+// A41:5458-5505
+__device__ inline void closest_segment_parameters(F3 p1, F3 q1, F3 p2, F3 q2, float* s, float* t) {
+    const F3 d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
+    const float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    if (a <= kEps && e <= kEps) { *s = 0.0f; *t = 0.0f; return; }
+    if (a <= kEps) { *s = 0.0f; *t = clamp01f(f / fmaxf(e, kEps)); return; }
+    const float c = dot(d1, r);
+    if (e <= kEps) { *t = 0.0f; *s = clamp01f(-c / fmaxf(a, kEps)); return; }
+    const float b = dot(d1, d2);
+    const float denom = a * e - b * b;
+    *s = fabsf(denom) > kEps ? clamp01f((b * f - c * e) / denom) : 0.0f;
+    const float tnom = b * (*s) + f;
+    if (tnom < 0.0f) { *t = 0.0f; *s = clamp01f(-c / fmaxf(a, kEps)); }
+    else if (tnom > e) { *t = 1.0f; *s = clamp01f((b - c) / fmaxf(a, kEps)); }
+    else { *t = tnom / e; }
+}
+
+__device__ inline bool vv_valid(const ClothView& c, const SelfView& sv, const float4* P, int s, int o) {  // A41:4696-4717
+    if (s == o || sv.radius <= 0.0f || sv.onset <= sv.radius || P[s].w == 0.0f) return false;
+    const F3 rd = f3(c.rest[s]) - f3(c.rest[o]);
+    if (dot(rd, rd) < sv.rest_skip * sv.rest_skip) return false;
+    const F3 cd = f3(P[s]) - f3(P[o]), pd = f3(c.prev[s]) - f3(c.prev[o]);
+    return dot(cd, cd) < sv.onset * sv.onset || dot(pd, pd) < sv.onset * sv.onset;
+}
+
+// Vertex part: candidate table (A41:4738-4814) + vertex-vertex response
+// (A41:4816-4951) + vertex-triangle repulsion (A41:5109-5224).
+__device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const float4* P, int s, bool count) {
+    const float cs = sv.cell;
+    const F3 ps = f3(P[s]);
+    unsigned acc = 0;
+    unsigned* out = sv.nb_idx + static_cast<size_t>(s) * kNeighborSlots;
+    if (sv.use_vhash) {
+        const int cx = cell_coord(ps.x, cs), cy = cell_coord(ps.y, cs), cz = cell_coord(ps.z, cs);
+        for (int dz = -1; dz <= 1 && acc < kNeighborSlots; ++dz)
+            for (int dy = -1; dy <= 1 && acc < kNeighborSlots; ++dy)
+                for (int dx = -1; dx <= 1 && acc < kNeighborSlots; ++dx) {
+                    const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
+                    const int b = static_cast<int>(hash_cell(qx, qy, qz, sv.vh.buckets));
+                    const int base = sv.vh.off[b], lim = min(sv.vh.count[b], kBucketSlots);
+                    for (int k = 0; k < lim && acc < kNeighborSlots; ++k) {
+                        const int o = sv.vh.entries[base + k];
+                        const int4 oc = sv.vcell[o];
+                        if (oc.x != qx || oc.y != qy || oc.z != qz || !vv_valid(c, sv, P, s, o)) continue;
+                        out[acc++] = static_cast<unsigned>(o);
+                    }
+                }
+    } else {
+        for (int o = 0; o < sv.nv && acc < kNeighborSlots; ++o)
+            if (vv_valid(c, sv, P, s, o)) out[acc++] = static_cast<unsigned>(o);
+    }
+    if (acc >= kNeighborSlots && count) atomicAdd(&c.counts[kCountSelfOverflow], 1ull);
+    sv.nb_count[s] = acc;
+
+    const float r = sv.radius, onset = sv.onset;
+    const float thickness = r * 0.5f + r * 0.5f;
+    unsigned cand = 0, hits = 0, soft = 0;
+    if (P[s].w != 0.0f && acc > 0) {
+        const F3 pc = ps, pp = f3(c.prev[s]);
+        for (unsigned k = 0; k < acc; ++k) {
+            const int o = static_cast<int>(out[k]);
+            ++cand;
+            const F3 qc = f3(P[o]), qp = f3(c.prev[o]);
+            if (thickness <= 0.0f) continue;
+            F3 resp = f3(0.0f, 0.0f, 0.0f);
+            bool hit = false;
+            const F3 dref = pp - qp;
+            const float cco = dot(dref, dref) - thickness * thickness;
+            const F3 rel = (pc - pp) - (qc - qp);
+            const float aco = dot(rel, rel), bco = dot(dref, rel);
+            const bool other_pinned = P[o].w == 0.0f;
+            const float scale = other_pinned ? 1.0f : 0.5f;
+            if (cco >= 0.0f && bco < 0.0f && aco > 1.0e-5f) {
+                const float disc = bco * bco - aco * cco;
+                if (disc >= 0.0f) {
+                    const float t = (-bco - sqrtf(disc)) / aco;
+                    if (t > 0.0f && t <= 1.0f) {
+                        const float sv1 = 1.0f - t;
+                        const F3 ph = mul(pp, sv1) + mul(pc, t), qh = mul(qp, sv1) + mul(qc, t);
+                        F3 nn = ph - qh;
+                        const float nsq = dot(nn, nn);
+                        if (nsq > kEps) {
+                            nn = mul(nn, rsqrt_exact(nsq));
+                            const float vn = dot((pc - ph) - (qc - qh), nn);
+                            if (vn > 0.0f) { resp = mul(nn, -vn * scale); hit = true; }
+                        }
+                    }
+                }
+            }
+            if (!hit) {
+                const F3 d = pc - qc;
+                const float dsq = dot(d, d);
+                if (dsq >= thickness * thickness) continue;
+                F3 nn = f3(0.0f, 0.0f, 1.0f);
+                float dist = sqrtf(fmaxf(dsq, kEps));
+                if (dsq > kEps) nn = mul(d, 1.0f / dist);
+                else {
+                    const F3 rd = f3(c.rest[s]) - f3(c.rest[o]);
+                    const float rsq = dot(rd, rd);
+                    if (rsq > kEps) nn = mul(rd, rsqrt_exact(rsq));
+                    dist = 0.0f;
+                }
+                resp = mul(nn, fminf((thickness - dist) * scale, thickness * 0.35f));
+                hit = true;
+            }
+            if (hit && dot(resp, resp) > kEps) {
+                const float ml = fmaxf(thickness * 0.35f, 1.0e-4f);
+                const float rl = len3(resp);
+                if (rl > ml) resp = mul(resp, ml / rl);
+                accumulate(sv, P, s, resp);
+                ++hits;
+            }
+        }
+    }
+    // vertex-triangle (A41:4990-5107 per pair)
+    if (sv.nt > 0) {
+        auto pair = [&](int v, int t) -> bool {
+            if (t < 0 || t >= sv.nt || r <= 0.0f || onset <= r) return false;
+            const int4 tr = sv.tris[t];
+            if (!tri_valid(sv, tr)) return false;
+            const int ia = tr.x, ib = tr.y, ic = tr.z;
+            if (v == ia || v == ib || v == ic) return false;
+            {
+                const F3 rv = f3(c.rest[v]);
+                const F3 rq = closest_point_on_triangle(rv, f3(c.rest[ia]), f3(c.rest[ib]), f3(c.rest[ic]));
+                const F3 rdd = rv - rq;
+                if (dot(rdd, rdd) < sv.rest_skip * sv.rest_skip) return false;
+            }
+            const F3 p = f3(P[v]), a = f3(P[ia]), b = f3(P[ib]), cc = f3(P[ic]);
+            const F3 q = closest_point_on_triangle(p, a, b, cc);
+            const F3 d = p - q;
+            const float dsq = dot(d, d);
+            float dist = sqrtf(fmaxf(dsq, kEps));
+            if (dist >= onset) return false;
+            F3 nn = f3(0.0f, 0.0f, 1.0f);
+            if (dsq > kEps) nn = mul(d, 1.0f / dist);
+            else {
+                nn = cross(b - a, cc - a);
+                const float nsq = dot(nn, nn);
+                if (nsq > kEps) nn = mul(nn, rsqrt_exact(nsq));
+                else {
+                    const F3 rv = f3(c.rest[v]);
+                    const F3 rdl = rv - closest_point_on_triangle(rv, f3(c.rest[ia]), f3(c.rest[ib]), f3(c.rest[ic]));
+                    const float rsq = dot(rdl, rdl);
+                    if (rsq > kEps) nn = mul(rdl, rsqrt_exact(rsq));
+                }
+                dist = 0.0f;
+            }
+            ++cand;
+            float wa = 1.0f / 3.0f, wb = wa, wc = wa;
+            triangle_barycentric(q, a, b, cc, &wa, &wb, &wc);
+            const float pen = r - dist;
+            float push;
+            if (pen > 0.0f) push = fminf(pen * 0.35f + r * 0.01f, r * 0.25f);
+            else {
+                const float x = 1.0f - fminf(dist / onset, 1.0f);
+                push = fminf(x * x * r * 0.06f, r * 0.04f);
+            }
+            const float wv = c.rest[v].w, wta = c.rest[ia].w * wa, wtb = c.rest[ib].w * wb, wtc = c.rest[ic].w * wc;
+            const float total = wv + wta + wtb + wtc;
+            if (push <= 0.0f || total <= 0.0f) return true;
+            const F3 delta = mul(nn, push / total);
+            if (wv > 0.0f) accumulate(sv, P, v, mul(delta, wv));
+            if (wta > 0.0f) accumulate(sv, P, ia, mul(delta, -wta));
+            if (wtb > 0.0f) accumulate(sv, P, ib, mul(delta, -wtb));
+            if (wtc > 0.0f) accumulate(sv, P, ic, mul(delta, -wtc));
+            ++soft;
+            return true;
+        };
+        int accepted = 0;
+        if (sv.use_thash) {
+            const int cx = cell_coord(ps.x, cs), cy = cell_coord(ps.y, cs), cz = cell_coord(ps.z, cs);
+            const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.th.buckets));
+            const int base = sv.th.off[b], lim = min(sv.th.count[b], kBucketSlots);
+            for (int k = 0; k < lim && accepted < sv.max_nb; ++k)
+                if (pair(s, sv.th.entries[base + k] / kMaxPrimCells)) ++accepted;
+        } else {
+            for (int t = 0; t < sv.nt && accepted < sv.max_nb; ++t)
+                if (pair(s, t)) ++accepted;
+        }
+    }
+    if (count) {
+        if (cand) atomicAdd(&c.counts[kCountSelfCandidates], static_cast<unsigned long long>(cand));
+        if (hits) atomicAdd(&c.counts[kCountExactImpulse], static_cast<unsigned long long>(hits));
+        if (soft) atomicAdd(&c.counts[kCountSoft], static_cast<unsigned long long>(soft));
+    }
+}
+
+// Edge part: A41:5595-5844.
+__device__ inline void self_edge(const ClothView& c, const SelfView& sv, const float4* P, int e, bool count) {
+    const float r = sv.radius, onset = sv.onset, cs = sv.cell;
+    unsigned cand = 0, hits = 0;
+    auto pair = [&](int ea, int eb) -> bool {
+        if (eb <= ea || ea < 0 || eb >= sv.ne || r <= 0.0f || onset <= r) return false;
+        const int2 A = sv.edges[ea], B = sv.edges[eb];
+        if (!edge_valid(sv, A) || !edge_valid(sv, B)) return false;
+        const int a0 = A.x, a1 = A.y, b0 = B.x, b1 = B.y;
+        if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
+        {
+            const F3 r0 = f3(c.rest[a0]), r1 = f3(c.rest[a1]), r2 = f3(c.rest[b0]), r3 = f3(c.rest[b1]);
+            float s = 0.0f, t = 0.0f;
+            closest_segment_parameters(r0, r1, r2, r3, &s, &t);
+            const F3 dd = (r0 + mul(r1 - r0, s)) - (r2 + mul(r3 - r2, t));
+            if (dot(dd, dd) < sv.rest_skip * sv.rest_skip) return false;
+        }
+        const F3 p0 = f3(P[a0]), p1 = f3(P[a1]), p2 = f3(P[b0]), p3 = f3(P[b1]);
+        float s = 0.0f, t = 0.0f;
+        closest_segment_parameters(p0, p1, p2, p3, &s, &t);
+        const F3 pa = p0 + mul(p1 - p0, s), pb = p2 + mul(p3 - p2, t);
+        const F3 d = pa - pb;
+        const float dsq = dot(d, d);
+        float dist = sqrtf(fmaxf(dsq, kEps));
+        if (dist >= onset) return false;
+        F3 nn = f3(0.0f, 0.0f, 1.0f);
+        if (dsq > kEps) nn = mul(d, 1.0f / dist);
+        else {
+            nn = cross(p1 - p0, p3 - p2);
+            const float nsq = dot(nn, nn);
+            if (nsq > kEps) nn = mul(nn, rsqrt_exact(nsq));
+            else {
+                const F3 r0 = f3(c.rest[a0]), r1 = f3(c.rest[a1]), r2 = f3(c.rest[b0]), r3 = f3(c.rest[b1]);
+                float rs = 0.0f, rt = 0.0f;
+                closest_segment_parameters(r0, r1, r2, r3, &rs, &rt);
+                const F3 rd = (r0 + mul(r1 - r0, rs)) - (r2 + mul(r3 - r2, rt));
+                const float rsq = dot(rd, rd);
+                if (rsq > kEps) nn = mul(rd, rsqrt_exact(rsq));
+            }
+            dist = 0.0f;
+        }
+        ++cand;
+        const float pen = r - dist;
+        float push;
+        if (pen > 0.0f) push = fminf(pen * 0.35f + r * 0.01f, r * 0.25f);
+        else {
+            const float x = 1.0f - fminf(dist / onset, 1.0f);
+            push = fminf(x * x * r * 0.06f, r * 0.04f);
+        }
+        const float w0 = 1.0f - s, w1 = s, w2 = 1.0f - t, w3 = t;
+        const float ma0 = c.rest[a0].w, ma1 = c.rest[a1].w, mb0 = c.rest[b0].w, mb1 = c.rest[b1].w;
+        const float wa0 = ma0 * w0, wa1 = ma1 * w1, wb0 = mb0 * w2, wb1 = mb1 * w3;
+        const float total = ma0 * w0 * w0 + ma1 * w1 * w1 + mb0 * w2 * w2 + mb1 * w3 * w3;
+        if (push <= 0.0f || total <= 0.0f) return true;
+        const F3 delta = mul(nn, push / total);
+        if (wa0 > 0.0f) accumulate(sv, P, a0, mul(delta, wa0));
+        if (wa1 > 0.0f) accumulate(sv, P, a1, mul(delta, wa1));
+        if (wb0 > 0.0f) accumulate(sv, P, b0, mul(delta, -wb0));
+        if (wb1 > 0.0f) accumulate(sv, P, b1, mul(delta, -wb1));
+        ++hits;
+        return true;
+    };
+    int accepted = 0;
+    if (sv.use_ehash) {
+        const int2 E = sv.edges[e];
+        if (!edge_valid(sv, E)) return;
+        const F3 mid = mul(f3(P[E.x]) + f3(P[E.y]), 0.5f);
+        const int cx = cell_coord(mid.x, cs), cy = cell_coord(mid.y, cs), cz = cell_coord(mid.z, cs);
+        const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.eh.buckets));
+        const int base = sv.eh.off[b], lim = min(sv.eh.count[b], kBucketSlots);
+        for (int k = 0; k < lim && accepted < sv.max_nb; ++k)
+            if (pair(e, sv.eh.entries[base + k] / kMaxPrimCells)) ++accepted;
+    } else {
+        for (int o = e + 1; o < sv.ne && accepted < sv.max_nb; ++o)
+            if (pair(e, o)) ++accepted;
+    }
+    if (count) {
+        if (cand) atomicAdd(&c.counts[kCountSelfCandidates], static_cast<unsigned long long>(cand));
+        if (hits) atomicAdd(&c.counts[kCountEdgeEdge], static_cast<unsigned long long>(hits));
+    }
+}
+
+// Threads [0, V) do the vertex part, [V, V + E) the edge part; every read is
+// of positions that stay fixed until the apply kernel.
+__global__ __launch_bounds__(256) void k_self_contacts(const ClothView* __restrict__ cv, SelfView sv, int parity, int count) {
+    const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const ClothView& c = *cv;
+    const float4* P = cur_pos(c, parity);
+    if (k < sv.nv) self_vertex(c, sv, P, k, count != 0);
+    else if (k < sv.nv + sv.ne) self_edge(c, sv, P, k - sv.nv, count != 0);
+}
+
+// A41:4953-4988: average, clamp, apply; the accumulators are cleared for the
+// next pass.
+__global__ __launch_bounds__(256) void k_self_apply(const ClothView* __restrict__ cv, SelfView sv, int parity) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= sv.nv) return;
+    const ClothView& c = *cv;
+    const int w = sv.accw[i];
+    if (w <= 0) return;
+    float4* P = parity ? c.pos1 : c.pos0;
+    const double inv = 1.0 / kFixedScale;
+    const F3 sum = f3(static_cast<float>(static_cast<double>(static_cast<long long>(sv.acc[3 * i + 0])) * inv),
+                      static_cast<float>(static_cast<double>(static_cast<long long>(sv.acc[3 * i + 1])) * inv),
+                      static_cast<float>(static_cast<double>(static_cast<long long>(sv.acc[3 * i + 2])) * inv));
+    sv.acc[3 * i + 0] = 0ull;
+    sv.acc[3 * i + 1] = 0ull;
+    sv.acc[3 * i + 2] = 0ull;
+    sv.accw[i] = 0;
+    F3 avg = mul(sum, 1.0f / static_cast<float>(w));
+    const float max_delta = fmaxf(sv.radius * 0.35f, 1.0e-4f);
+    const float dl = len3(avg);
+    if (dl > max_delta) avg = mul(avg, max_delta / dl);
+    const float alen = len3(avg);
+    if (alen > 0.0f) atomicMax(reinterpret_cast<int*>(c.max_smoothed), __float_as_int(alen));
+    const float4 p = P[i];
+    P[i] = with_w(f3(p) + avg, p.w);
+    const float4 q = c.prev[i];
+    c.prev[i] = with_w(f3(q) + mul(avg, 0.55f), 0.0f);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+SelfCollision::~SelfCollision() {
+    for (void* p : allocations_) (void)hipFree(p);
+}
+
+void* SelfCollision::alloc_bytes(size_t bytes) {
+    void* p = nullptr;
+    if (!hip_ok(hipMalloc(&p, std::max<size_t>(bytes, 256)), "self collision allocation")) return nullptr;
+    allocations_.push_back(p);
+    return p;
+}
+
+// A41:6784-6994 sizing: cell = max(1.8 r, 1 mm); buckets = next_pow2(max(1024, 4n)).
+bool SelfCollision::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, const std::vector<int>& tris, int nv,
+                            unsigned long long* counts) {
+    ready_ = false;
+    const float req = std::max(cfg.self_collision_distance, 0.0f);
+    const float r = req > 0.0f ? req : std::max(std::max(cfg.cloth_thickness, cfg.collision_margin), 0.0f);
+    if (!cfg.self_collision || nv <= 0 || r <= 0.0f) return true;
+    SelfView& s = sv_;
+    s = SelfView{};
+    s.nv = nv;
+    s.nt = cfg.triangle_count;
+    s.ne = cfg.edge_count;
+    s.radius = r;
+    s.onset = std::max(r * 1.8f, r + 1.0e-7f);
+    s.rest_skip = std::max(s.onset * 1.25f, r * 2.5f);
+    s.cell = std::max(r * 1.8f, 1.0e-3f);
+    s.prim_margin = r * 1.8f;
+    s.max_nb = std::max(1, std::min(cfg.max_self_collision_neighbors, kBucketSlots));
+    s.counts = counts;
+    s.use_vhash = nv >= 128 ? 1 : 0;
+    s.use_thash = s.nt >= 64 ? 1 : 0;
+    s.use_ehash = s.ne >= 64 ? 1 : 0;
+    passes_ = std::max(1, std::min(cfg.fast_self_collision_passes, 8));
+    interval_ = std::max(cfg.self_collision_interval, 1);
+    auto make_hash = [&](HashView& h, int n, int capacity) -> bool {
+        h.buckets = next_pow2(std::max(1024, n * 4));
+        h.capacity = capacity;
+        h.count = static_cast<int*>(alloc_bytes(sizeof(int) * h.buckets));
+        h.off = static_cast<int*>(alloc_bytes(sizeof(int) * (h.buckets + 1)));
+        h.cursor = static_cast<int*>(alloc_bytes(sizeof(int) * h.buckets));
+        h.entries = static_cast<int*>(alloc_bytes(sizeof(int) * static_cast<size_t>(std::max(capacity, 1))));
+        return h.count && h.off && h.cursor && h.entries;
+    };
+    if (s.use_vhash && !make_hash(s.vh, nv, nv)) return false;
+    if (s.use_thash && !make_hash(s.th, s.nt, s.nt * kMaxPrimCells)) return false;
+    if (s.use_ehash && !make_hash(s.eh, s.ne, s.ne * kMaxPrimCells)) return false;
+    int max_buckets = std::max(s.vh.buckets, std::max(s.th.buckets, s.eh.buckets));
+    s.scan_tmp = static_cast<int*>(alloc_bytes(sizeof(int) * (max_buckets / kScanChunk + 2)));
+    s.vcell = static_cast<int4*>(alloc_bytes(sizeof(int4) * nv));
+    s.nb_count = static_cast<unsigned*>(alloc_bytes(sizeof(unsigned) * nv));
+    s.nb_idx = static_cast<unsigned*>(alloc_bytes(sizeof(unsigned) * static_cast<size_t>(nv) * kNeighborSlots));
+    s.acc = static_cast<unsigned long long*>(alloc_bytes(sizeof(unsigned long long) * 3 * static_cast<size_t>(nv)));
+    s.accw = static_cast<int*>(alloc_bytes(sizeof(int) * nv));
+    int4* d_tris = static_cast<int4*>(alloc_bytes(sizeof(int4) * std::max(s.nt, 1)));
+    int2* d_edges = static_cast<int2*>(alloc_bytes(sizeof(int2) * std::max(s.ne, 1)));
+    if (!s.scan_tmp || !s.vcell || !s.nb_count || !s.nb_idx || !s.acc || !s.accw || !d_tris || !d_edges) return false;
+    std::vector<int4> ht(s.nt);
+    for (int t = 0; t < s.nt; ++t) ht[t] = make_int4(tris[3 * t], tris[3 * t + 1], tris[3 * t + 2], 0);
+    std::vector<int2> he(s.ne);
+    for (int e = 0; e < s.ne; ++e) he[e] = make_int2(edges[2 * e], edges[2 * e + 1]);
+    if (s.nt > 0 && !hip_ok(hipMemcpy(d_tris, ht.data(), sizeof(int4) * s.nt, hipMemcpyHostToDevice), "self triangle upload")) return false;
+    if (s.ne > 0 && !hip_ok(hipMemcpy(d_edges, he.data(), sizeof(int2) * s.ne, hipMemcpyHostToDevice), "self edge upload")) return false;
+    s.tris = d_tris;
+    s.edges = d_edges;
+    if (!hip_ok(hipMemset(s.acc, 0, sizeof(unsigned long long) * 3 * static_cast<size_t>(nv)), "self reset") ||
+        !hip_ok(hipMemset(s.accw, 0, sizeof(int) * nv), "self reset") ||
+        !hip_ok(hipMemset(s.nb_count, 0, sizeof(unsigned) * nv), "self reset") ||
+        !hip_ok(hipMemset(s.vcell, 0, sizeof(int4) * nv), "self reset"))
+        return false;
+    ready_ = true;
+    return true;
+}
+
+SelfBatch* SelfCollision::single_batch(const ClothView&, const ClothView* dview) {
+    dview_ = dview;
+    return ready_ ? this : nullptr;
+}
+
+bool SelfCollision::run(const LaunchCtx& ctx, int parity, int substep, int substeps, bool count) {
+    if (!(((substep + 1) % interval_) == 0 || substep == substeps - 1)) return true;
+    return run_for(ctx.stream, dview_, parity, count);
+}
+
+static void build_prim_hash(hipStream_t st, const ClothView* cv, const SelfView& s, const HashView& h, int kind, int n,
+                            int parity, bool count) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(h.buckets, 256)), dim3(256), 0, st, h.count, 0, h.buckets);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(h.buckets, 256)), dim3(256), 0, st, h.cursor, 0, h.buckets);
+    hipLaunchKernelGGL(k_prim_hash, dim3(grid_for(n, 256)), dim3(256), 0, st, cv, s, parity, kind, 0, count ? 1 : 0);
+    exclusive_scan(st, h.count, h.buckets, h.off, s.scan_tmp);
+    hipLaunchKernelGGL(k_prim_hash, dim3(grid_for(n, 256)), dim3(256), 0, st, cv, s, parity, kind, 1, 0);
+}
+
+bool SelfCollision::run_for(hipStream_t st, const ClothView* cv, int parity, bool count) {
+    if (!ready_) return true;
+    const SelfView& s = sv_;
+    {
+        TimedLaunch t("self_hash", st);
+        if (s.use_vhash) {
+            hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(s.vh.buckets, 256)), dim3(256), 0, st, s.vh.count, 0, s.vh.buckets);
+            hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(s.vh.buckets, 256)), dim3(256), 0, st, s.vh.cursor, 0, s.vh.buckets);
+            hipLaunchKernelGGL(k_vhash_count, dim3(grid_for(s.nv, 256)), dim3(256), 0, st, cv, s, parity);
+            exclusive_scan(st, s.vh.count, s.vh.buckets, s.vh.off, s.scan_tmp);
+            hipLaunchKernelGGL(k_vhash_scatter, dim3(grid_for(s.nv, 256)), dim3(256), 0, st, s);
+        }
+        if (s.use_thash) build_prim_hash(st, cv, s, s.th, 0, s.nt, parity, count);
+        if (s.use_ehash) build_prim_hash(st, cv, s, s.eh, 1, s.ne, parity, count);
+    }
+    {
+        TimedLaunch t("self_bucket_order", st);
+        const int cd = count ? 1 : 0;
+        if (s.use_vhash) hipLaunchKernelGGL(k_bucket_order, dim3(grid_for(s.vh.buckets, 256)), dim3(256), 0, st, s.vh, s.counts, cd);
+        if (s.use_thash) hipLaunchKernelGGL(k_bucket_order, dim3(grid_for(s.th.buckets, 256)), dim3(256), 0, st, s.th, s.counts, cd);
+        if (s.use_ehash) hipLaunchKernelGGL(k_bucket_order, dim3(grid_for(s.eh.buckets, 256)), dim3(256), 0, st, s.eh, s.counts, cd);
+    }
+    for (int pass = 0; pass < passes_; ++pass) {
+        {
+            TimedLaunch t("self_contacts", st);
+            hipLaunchKernelGGL(k_self_contacts, dim3(grid_for(static_cast<long long>(s.nv) + s.ne, 256)), dim3(256), 0, st, cv, s,
+                               parity, count ? 1 : 0);
+        }
+        {
+            TimedLaunch t("self_apply", st);
+            hipLaunchKernelGGL(k_self_apply, dim3(grid_for(s.nv, 256)), dim3(256), 0, st, cv, s, parity);
+        }
+    }
+    return hip_ok(hipGetLastError(), "launch self collision");
+}
+
+bool SelfCollision::download_cells(int* cells, int nv) {
+    if (!ready_ || !sv_.use_vhash) return fail("no self-collision vertex hash");
+    std::vector<int4> h(nv);
+    if (!hip_ok(hipMemcpy(h.data(), sv_.vcell, sizeof(int4) * nv, hipMemcpyDeviceToHost), "download hash cells")) return false;
+    for (int i = 0; i < nv; ++i) { cells[3 * i] = h[i].x; cells[3 * i + 1] = h[i].y; cells[3 * i + 2] = h[i].z; }
+    return true;
+}
+
+bool SelfCollision::download_neighbors(unsigned* counts, unsigned* indices, int nv) {
+    if (!ready_) return fail("no self-collision table");
+    return hip_ok(hipMemcpy(counts, sv_.nb_count, sizeof(unsigned) * nv, hipMemcpyDeviceToHost), "download neighbours") &&
+           hip_ok(hipMemcpy(indices, sv_.nb_idx, sizeof(unsigned) * static_cast<size_t>(nv) * kNeighborSlots, hipMemcpyDeviceToHost),
+                  "download neighbours");
+}
+
+}  // namespace ssbl
+
+namespace ssbl {
+
+SelfBatch* SelfBatchHolder::get(const std::vector<Solver*>& members, const std::vector<ClothView>&, const ClothView* dviews) {
+    runner_.entries.clear();
+    for (size_t m = 0; m < members.size(); ++m) {
+        SelfCollision* sc = solver_self(members[m]);
+        if (sc && sc->enabled()) runner_.entries.push_back(Entry{sc, dviews + m, sc->interval()});
+    }
+    return runner_.entries.empty() ? nullptr : &runner_;
+}
+
+bool SelfBatchHolder::Runner::run(const LaunchCtx& ctx, int parity, int substep, int substeps, bool count) {
+    for (const Entry& e : entries) {
+        if (!(((substep + 1) % e.substep_interval) == 0 || substep == substeps - 1)) continue;
+        if (!e.self->run_for(ctx.stream, e.dview, parity, count)) return false;
+    }
+    return true;
+}
+
+}  // namespace ssbl

@@ -1,4 +1,8 @@
 // xpbd_self.hpp -- self-collision state of one cloth and its per-step driver.
+//
+// Reference: A41:4542-5844 (hash builds, candidate table, vertex-vertex,
+// vertex-triangle and edge-edge responses, averaging), host set-up
+// A41:6784-6994, schedule A41:9255-9339 (strict mode).
 #pragma once
 
 #include <string>
@@ -9,8 +13,39 @@
 namespace ssbl {
 
 bool fail(const std::string& message);
+bool hip_ok(hipError_t err, const char* what);
 
 struct Solver;
+
+// One spatial hash as a counting-sorted bucket table: bucket b holds
+// entries[off[b] .. off[b+1]), the first min(count, 32) of them in ascending
+// key order (key = vertex, or primitive * 64 + cell index in the primitive's
+// box loop), which is the order queries visit.
+struct HashView {
+    int buckets;
+    int capacity;
+    int* count;      // buckets
+    int* off;        // buckets + 1
+    int* cursor;     // buckets
+    int* entries;    // capacity
+};
+
+struct SelfView {
+    int nv, nt, ne;
+    int use_vhash, use_thash, use_ehash;
+    float cell, radius, onset, rest_skip, prim_margin;
+    int max_nb;
+    int4* vcell;                 // V: cell x, y, z, bucket (vertex hash)
+    HashView vh, th, eh;
+    unsigned* nb_count;          // V
+    unsigned* nb_idx;            // V * 32
+    unsigned long long* acc;     // 3V fixed-point 2^-32 m
+    int* accw;                   // V
+    const int4* tris;
+    const int2* edges;           // mesh order, endpoints clamped >= 0
+    int* scan_tmp;               // block sums for the bucket scans
+    unsigned long long* counts;  // the cloth's diagnostic counters
+};
 
 // Runs the self-collision passes of one step for a set of cloths.
 class SelfBatch {
@@ -19,19 +54,50 @@ public:
     virtual bool run(const LaunchCtx& ctx, int parity, int substep, int substeps, bool count) = 0;
 };
 
-class SelfCollision {
+class SelfCollision : public SelfBatch {
 public:
-    bool prepare(const SsblXpbdConfig&, const std::vector<int>&, const std::vector<int>&, int) {
-        return fail("self collision is not available in this build yet");
-    }
-    SelfBatch* single_batch(const ClothView&, const ClothView*) { return nullptr; }
-    bool download_cells(int*, int) { return fail("no self-collision hash"); }
-    bool download_neighbors(unsigned*, unsigned*, int) { return fail("no self-collision table"); }
+    SelfCollision() = default;
+    SelfCollision(const SelfCollision&) = delete;
+    SelfCollision& operator=(const SelfCollision&) = delete;
+    ~SelfCollision() override;
+    bool prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, const std::vector<int>& tris, int nv,
+                 unsigned long long* counts);
+    // The driver for a lone cloth: `view` is the cloth's host view mirror and
+    // `dview` its device copy.
+    SelfBatch* single_batch(const ClothView& view, const ClothView* dview);
+    bool run(const LaunchCtx& ctx, int parity, int substep, int substeps, bool count) override;
+    bool run_for(hipStream_t stream, const ClothView* dview, int parity, bool count);
+    bool download_cells(int* cells, int nv);
+    bool download_neighbors(unsigned* counts, unsigned* indices, int nv);
+    bool enabled() const { return ready_; }
+    int interval() const { return interval_; }
+
+private:
+    bool ready_ = false;
+    int passes_ = 1;
+    int interval_ = 1;
+    SelfView sv_{};
+    std::vector<void*> allocations_;
+    const ClothView* dview_ = nullptr;
+    void* alloc_bytes(size_t bytes);
 };
+
+// Self collision of every member of a batch (members run one after another).
+// The self-collision state of a solver (defined with the solver).
+SelfCollision* solver_self(Solver* s);
 
 class SelfBatchHolder {
 public:
-    SelfBatch* get(const std::vector<Solver*>&, const std::vector<ClothView>&, const ClothView*) { return nullptr; }
+    SelfBatch* get(const std::vector<Solver*>& members, const std::vector<ClothView>& views, const ClothView* dviews);
+
+private:
+    struct Entry { SelfCollision* self; const ClothView* dview; int substep_interval; };
+    class Runner : public SelfBatch {
+    public:
+        std::vector<Entry> entries;
+        bool run(const LaunchCtx& ctx, int parity, int substep, int substeps, bool count) override;
+    };
+    Runner runner_;
 };
 
 }  // namespace ssbl

@@ -227,6 +227,8 @@ struct Solver {
     bool analytic_on() const { return colliders.use_ground || colliders.use_wall || colliders.use_sphere; }
 };
 
+SelfCollision* solver_self(Solver* s) { return &s->self; }
+
 bool sync_stream() { return hip_ok(hipStreamSynchronize(lib_stream()), "stream synchronize"); }
 
 // A41:7413-7584 per-vertex pressure area shares and rest scales (host set-up).
@@ -499,7 +501,7 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     if (!ok) return false;
     if (!hip_ok(hipMemset(counts.p, 0, sizeof(unsigned long long) * kCountSlots), "diagnostic reset")) return false;
     if (!hip_ok(hipMemset(max_smoothed.p, 0, sizeof(float)), "diagnostic reset")) return false;
-    if (cfg.self_collision && !self.prepare(cfg, edges, tri, n)) return false;
+    if (cfg.self_collision && !self.prepare(cfg, edges, tri, n, counts.p)) return false;
     diag.finite_flag = 1;
     const int stc = std::max(cfg.static_triangle_count, 0);
     if (stc > 0 && !mesh->static_triangles) return fail("static triangle data is required when triangle count is nonzero");

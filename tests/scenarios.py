@@ -93,3 +93,96 @@ def bend_pair(lib, compliance=0.0):
     r.set_positions(folded)
     r.step(1, 1)
     return folded, r.positions(), rest
+
+
+# --- self collision: the reference smoke scenes (abi41_smoke.cpp:227-597) in
+# strict mode (self_collision_mode 2), with the smoke's own pass criteria.
+SELF_CFG = dict(self_collision=1, self_collision_mode=2, cloth_thickness=0.10, collision_margin=0.0,
+                max_self_collision_neighbors=32, fast_self_collision_passes=4)
+
+
+def self_vv(lib):
+    n = 600
+    rest = np.zeros((n, 3), np.float32)
+    rest[:, 0] = np.arange(n)
+    r = Raw(lib, rest, np.ones(n, np.float32), **SELF_CFG)
+    close = rest.copy()
+    close[n - 1, 0] = 0.03
+    r.set_positions(close)
+    r.step(1, 6)
+    out = r.positions()
+    return float(np.linalg.norm(close[0] - close[n - 1])), float(np.linalg.norm(out[0] - out[n - 1])), r.diag(), r
+
+
+def self_vt(lib):
+    n, ntri = 600, 64
+    rest = np.zeros((n, 3), np.float32)
+    rest[0], rest[1], rest[2] = (-1, -1, 0), (1, -1, 0), (0, 1, 0)
+    inv = np.ones(n, np.float32)
+    inv[:3] = 0.0
+    for i in range(3, n):
+        rest[i] = (20.0 + i, 20.0 + (i % 7) * 0.25, (i % 5) * 0.20)
+    rest[n - 1] = (5.0, 5.0, 1.0)
+    tris = [[0, 1, 2]] + [[3 + t * 3, 4 + t * 3, 5 + t * 3] for t in range(1, ntri)]
+    r = Raw(lib, rest, inv, triangles=tris, **SELF_CFG)
+    close = rest.copy()
+    close[n - 1] = (0.0, -0.2, 0.03)
+    r.set_positions(close)
+    r.step(1, 6)
+    out = r.positions()
+    return float(close[n - 1, 2]), float(out[n - 1, 2]), r.diag(), r
+
+
+def self_ee(lib):
+    n, ne = 600, 64
+    rest = np.zeros((n, 3), np.float32)
+    rest[:, 0] = 200.0 + np.arange(n) * 3.0
+    rest[:, 1] = 100.0 + (np.arange(n) % 11)
+    rest[0], rest[1], rest[2], rest[3] = (-1, 0, 0), (1, 0, 0), (5, -1, 0), (5, 1, 0)
+    inv = np.ones(n, np.float32)
+    inv[:2] = 0.0
+    edges = [[0, 1], [2, 3]]
+    for e in range(2, ne):
+        a = 4 + (e - 2) * 2
+        rest[a] = (50.0 + e * 4.0, 30.0, 0.0)
+        rest[a + 1] = (51.0 + e * 4.0, 30.0, 0.0)
+        edges.append([a, a + 1])
+    edges = np.asarray(edges, np.int32)
+    lengths = np.linalg.norm(rest[edges[:, 0]] - rest[edges[:, 1]], axis=1).astype(np.float32)
+    r = Raw(lib, rest, inv, edges=edges, edge_rest=lengths, **SELF_CFG)
+    close = rest.copy()
+    close[2] = (0.0, -1.0, 0.03)
+    close[3] = (0.0, 1.0, 0.03)
+    r.set_positions(close)
+    r.step(1, 6)
+    out = r.positions()
+    return float((close[2, 2] + close[3, 2]) / 2), float((out[2, 2] + out[3, 2]) / 2), r.diag(), r
+
+
+def crumple(scenes, n=40, frames=25, seed=1, self_distance=0.01 / 1.8):
+    """A grid dropped onto a small sphere so it folds over itself."""
+    cloth = scenes.grid_cloth(n, 0.8, 0.5, pin_corners=False, seed=seed)
+    opts = scenes.SolverOptions(use_ground=True, use_sphere=True, sphere_center=(0.0, 0.0, 0.2), sphere_radius=0.12,
+                                contact_friction=0.3, cloth_thickness=0.005, self_collision=True, self_collision_mode=2,
+                                self_collision_distance=self_distance, fast_self_collision_passes=4,
+                                max_self_collision_neighbors=32)
+    return cloth, opts, frames
+
+
+def folded_cloth(scenes, n=30, size=0.6, gap=0.004, seed=2):
+    """A grid whose upper half starts folded onto the lower half, `gap` above it."""
+    cloth = scenes.grid_cloth(n, size, 0.3, pin_corners=False, seed=seed, jitter=0.0)
+    p = cloth.positions_world.astype(np.float64).copy()
+    h = size / (n - 1)
+    # rows are along scene z (solver -y); fold rows >= n/2 back over the first half
+    rows = np.arange(n * n) // n
+    upper = rows >= n // 2
+    mirror_row = (n - 1) - rows[upper]
+    p[upper, 1] = -(mirror_row * h - size / 2)
+    p[upper, 2] += gap
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cloth.start_positions = (p + rng.normal(0, 1e-4, p.shape)).astype(np.float32)
+    opts = scenes.SolverOptions(use_ground=True, contact_friction=0.3, cloth_thickness=0.005, self_collision=True,
+                                self_collision_mode=2, self_collision_distance=0.01 / 1.8, fast_self_collision_passes=1,
+                                max_self_collision_neighbors=32, gravity=(0.0, 0.0, 0.0))
+    return cloth, opts
