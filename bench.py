@@ -52,7 +52,23 @@ def make_workload(name: str, seed: int):
         return scenes.config1(seed=seed)
     if name == "batched_64_uneven":
         return scenes.config3(seed=seed)
+    if name == "selfcol_100x100":
+        return scenes.config2(seed=seed)
     raise SystemExit(f"unknown workload {name}")
+
+
+def single_cloth_line(name: str, seed: int, frames: int, warmup: int, lib) -> dict:
+    """Throughput and per-frame latency of one cloth stepped alone."""
+    sc = make_workload(name, seed)
+    (s,) = create(sc)
+    run_plain([s], sc, warmup)
+    t = timed(lambda: run_plain([s], sc, frames), lib)
+    lat = latencies(lambda: run_plain([s], sc, 1), lib, frames)
+    d = s.diagnostics()
+    return {"workload": sc.name, "vertices": sc.vertex_count, "substeps": sc.substeps, "iterations": sc.iterations,
+            "vertex_substeps_per_s": frames * sc.vertex_count * sc.substeps / t,
+            "frame_ms_p50": float(np.percentile(lat, 50)), "frame_ms_p99": float(np.percentile(lat, 99)),
+            "self_collision": bool(sc.options[0].self_collision), "finite": int(d.finite_flag)}
 
 
 def create(scene, lib=None):
@@ -171,6 +187,7 @@ def main():
     lat_b = latencies(step_frame, lib, args.latency_frames)
     lat_p = latencies(lambda: run_plain(plain, scene, 1), lib, args.latency_frames)
 
+    secondary = single_cloth_line("selfcol_100x100", args.seed, 10, 2, lib) if args.workload != "selfcol_100x100" else None
     stats = kernel_stats(lib, step_frame)
     dominant = max(stats.items(), key=lambda kv: kv[1]["total_ms"]) if stats else ("none", {"launches": 1, "total_ms": 0.0})
     avg_ms = dominant[1]["total_ms"] / max(dominant[1]["launches"], 1)
@@ -205,6 +222,7 @@ def main():
             "frame_ms_p50_plain": float(np.percentile(lat_p, 50)),
             "frame_ms_p99_plain": float(np.percentile(lat_p, 99)),
             "kernels": stats,
+            "selfcol_10k": secondary,
         },
         "roofline": {
             "bound": "hbm",

@@ -369,6 +369,7 @@ template <class R> struct Solver {
     std::vector<int> edge_colors, bend_colors;   // host colour offsets (empty when none)
     bool bend_coloring_valid = false;
     std::vector<int> tris;              // 3T
+    long long wing_count = 0;           // dihedral "bending wings" (diagnostics only, A41:7193-7302)
     std::vector<int> tri_offsets, tri_incident;  // vertex -> incident triangles (A41:7051-7109)
     std::vector<float> area_share, rest_scale;
     float pressure_accel_scale = 1792.0f, pressure_impulse_rest_scale = 0.85f;
@@ -388,6 +389,16 @@ template <class R> struct Solver {
     std::vector<int> self_cells;              // 3V from the last vertex-hash build
     std::vector<unsigned> nb_counts, nb_index;  // V, V*32
     std::vector<Vec> snap_pos, snap_prev;     // positions the last candidate table was built from
+    // hard-stretch PCG (A41:3032-3464, 7304-7394, 7626-7759)
+    struct Sym { R m11, m12, m13, m22, m23, m33; };
+    bool edge_coloring_valid = false;
+    std::vector<int> inc_off, inc_edge;       // vertex -> incident valid edges, ascending
+    std::vector<Sym> pcg_diag, pcg_pinv, pcg_off;
+    std::vector<Vec> pcg_rhs, pcg_sol, pcg_res, pcg_z, pcg_dir, pcg_adir;
+    double pcg_rz = 0, pcg_dad = 0, pcg_rznext = 0, pcg_initial = 0;
+    int pcg_status = 0;
+    long long pcg_guard = 0, pcg_runs = 0;
+    float pcg_max_delta = 0.0f;
     std::vector<int64_t> acc_q;               // 3V fixed point
     std::vector<int> acc_w;
     float max_smoothed = 0.0f;
@@ -420,6 +431,22 @@ template <class R> struct Solver {
     bool contact_response(Vec n, R proxy, Vec* vel_io);
     // helpers
     Vec normal_at(int v, const std::vector<Vec>& P, int* has) const;
+    void populate_sdf_diag() {  // A41:6104-6121
+        diag.static_sdf_rebuild_count = sdf_rebuilds;
+        diag.static_sdf_voxel_count = sdf_ready ? 1LL * sdf_dim[0] * sdf_dim[1] * sdf_dim[2] : 0;
+        diag.static_sdf_grid_x = sdf_dim[0];
+        diag.static_sdf_grid_y = sdf_dim[1];
+        diag.static_sdf_grid_z = sdf_dim[2];
+        diag.static_sdf_build_ms = sdf_build_ms;
+        diag.static_sdf_unsigned_fallback_count = sdf_unsigned;
+        diag.static_triangle_count = cfg.static_triangle_count;
+    }
+    void pcg_run(R dt);
+    void polish(R dt);
+    void final_caps(R dt);
+    void cap_ranges(int kind, int passes, bool reverse_too, R dt);
+    void edge_jacobi(int kind, int lo, int hi, R dt);
+    void gather_average(int kind, R dt);
     R self_radius() const {
         R req = std::max(R(cfg.self_collision_distance), R(0));
         if (req > R(0)) return req;
@@ -445,7 +472,6 @@ template <class R> bool Solver<R>::create(const SsblXpbdConfig* config, const Ss
     cfg.stretch_optimization_strength = std::min(std::max(cfg.stretch_optimization_strength, 0.0f), 1.0f);
     cfg.stretch_optimization_enabled =
         (cfg.edge_count > 0 && cfg.stretch_optimization_enabled && cfg.stretch_optimization_strength > 0.0f) ? 1 : 0;
-    if (cfg.stretch_optimization_enabled) return fail("oracle: stretch optimization (PCG) is not restated yet");
     if (cfg.self_collision && cfg.self_collision_mode == kSelfModeFast)
         return fail("oracle: fast self-collision mode is not restated yet");
     colliders.use_ground = cfg.use_ground;
@@ -506,8 +532,55 @@ template <class R> bool Solver<R>::create(const SsblXpbdConfig* config, const Ss
             }
         }
     }
+    // incident valid edges per vertex, ascending (the PCG CSR row order, A41:7304-7394)
+    inc_off.assign(n + 1, 0);
+    for (const Edge& e : springs) if (e.i < n && e.j < n && e.i != e.j) { inc_off[e.i + 1]++; inc_off[e.j + 1]++; }
+    for (int i = 0; i < n; ++i) inc_off[i + 1] += inc_off[i];
+    inc_edge.assign(inc_off[n], 0);
+    {
+        std::vector<int> cur(inc_off.begin(), inc_off.end() - 1);
+        for (int e = 0; e < cfg.edge_count; ++e) {
+            const Edge& E = springs[e];
+            if (E.i < n && E.j < n && E.i != E.j) { inc_edge[cur[E.i]++] = e; inc_edge[cur[E.j]++] = e; }
+        }
+    }
+    edge_coloring_valid = !edge_colors.empty();
+    if (edge_coloring_valid) {
+        std::vector<int> stamp(n, -1);
+        for (int c = 0; c < cfg.edge_color_count && edge_coloring_valid; ++c) {
+            const int lo = std::max(edge_colors[c], 0), hi = std::min(edge_colors[c + 1], cfg.edge_count);
+            for (int e = lo; e < hi; ++e) {
+                const int i = springs[e].i, j = springs[e].j;
+                if (i >= n || j >= n) continue;
+                if (stamp[i] == c || stamp[j] == c) { edge_coloring_valid = false; break; }
+                stamp[i] = c; stamp[j] = c;
+            }
+        }
+    }
+    if (cfg.stretch_optimization_enabled) {
+        if (inc_off[n] == 0) return fail("PCG stretch optimization found no valid CSR edges");
+        pcg_diag.assign(n, Sym{}); pcg_pinv.assign(n, Sym{}); pcg_off.assign(inc_off[n], Sym{});
+        for (auto* v : {&pcg_rhs, &pcg_sol, &pcg_res, &pcg_z, &pcg_dir, &pcg_adir}) v->assign(n, mk<R>(0, 0, 0));
+    }
     tris.assign(3 * static_cast<size_t>(cfg.triangle_count), 0);
     for (int t = 0; t < 3 * cfg.triangle_count; ++t) tris[t] = mesh->triangles ? std::max(mesh->triangles[t], 0) : 0;
+    // A41:7193-7302: an edge seen again with a different opposite vertex is a
+    // bending wing; the reference builds them for diagnostics only.
+    wing_count = 0;
+    if (cfg.triangle_count > 1) {
+        std::unordered_map<uint64_t, int> owner;
+        for (int t = 0; t < cfg.triangle_count; ++t) {
+            const int ids[3] = {tris[3 * t], tris[3 * t + 1], tris[3 * t + 2]};
+            if (ids[0] >= n || ids[1] >= n || ids[2] >= n) continue;
+            for (int k = 0; k < 3; ++k) {
+                const int x = ids[k], y = ids[(k + 1) % 3], opp = ids[(k + 2) % 3];
+                const uint64_t key = (uint64_t(unsigned(std::min(x, y))) << 32) | uint64_t(unsigned(std::max(x, y)));
+                auto it = owner.find(key);
+                if (it == owner.end()) { owner.emplace(key, opp); continue; }
+                if (it->second != opp) ++wing_count;
+            }
+        }
+    }
     // incident-triangle CSR, A41:7051-7109
     tri_offsets.assign(n + 1, 0);
     for (int k = 0; k < 3 * cfg.triangle_count; ++k) if (tris[k] < n) tri_offsets[tris[k] + 1]++;
@@ -679,7 +752,7 @@ template <class R> bool Solver<R>::build_sdf(const float* triangles, int count) 
     sdf_signed = 0;
     static_tris.clear();
     cfg.static_triangle_count = count;
-    if (count <= 0) return true;
+    if (count <= 0) { populate_sdf_diag(); return true; }
     auto t0 = std::chrono::steady_clock::now();
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int i = 0; i < 3 * count; ++i) {
@@ -730,6 +803,7 @@ template <class R> bool Solver<R>::build_sdf(const float* triangles, int count) 
     sdf_ready = 1;
     sdf_rebuilds++;
     sdf_build_ms = static_cast<float>(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    populate_sdf_diag();
     return true;
 }
 
@@ -1499,6 +1573,353 @@ template <class R> void Solver<R>::self_collision() {
     }
 }
 
+// ------------------------------------------------------ hard-stretch PCG
+// Inner products are summed exactly: every float term is rounded once to a
+// 2^-96 fixed-point integer and the 128-bit sum is order independent; the
+// sum is then rounded to double.
+template <class R> struct ExactSum {
+    __int128 acc = 0;
+    void add(R t) { acc += static_cast<__int128>(std::rint(static_cast<double>(t) * 79228162514264337593543950336.0)); }
+    double value() const { return static_cast<double>(acc) * (1.0 / 79228162514264337593543950336.0); }
+};
+
+// A41:1158-1191 symmetric 3x3 inverse, diagonal fallback when singular.
+template <class S, class R> S sym_inverse(const S& m, long long* guard) {
+    const R a = m.m11, b = m.m12, c = m.m13, d = m.m22, e = m.m23, f = m.m33;
+    const R df_ee = std::fma(d, f, -e * e), bf_ce = std::fma(b, f, -c * e), be_cd = std::fma(b, e, -c * d);
+    const R det = std::fma(c, be_cd, std::fma(-b, bf_ce, a * df_ee));
+    if (std::isfinite(det) && std::fabs(det) > R(1.0e-12)) {
+        const R inv = R(1) / det;
+        return S{df_ee * inv, std::fma(c, e, -b * f) * inv, be_cd * inv, std::fma(a, f, -c * c) * inv, std::fma(b, c, -a * e) * inv,
+                 std::fma(a, d, -b * b) * inv};
+    }
+    ++*guard;
+    return S{R(1) / std::max(std::fabs(a), R(1.0e-4)), R(0), R(0), R(1) / std::max(std::fabs(d), R(1.0e-4)), R(0),
+             R(1) / std::max(std::fabs(f), R(1.0e-4))};
+}
+
+// A41:1193-1199
+template <class S, class R> V3<R> sym_madd(const S& m, V3<R> v, V3<R> acc) {
+    return mk(std::fma(m.m13, v.z, std::fma(m.m12, v.y, std::fma(m.m11, v.x, acc.x))),
+              std::fma(m.m23, v.z, std::fma(m.m22, v.y, std::fma(m.m12, v.x, acc.y))),
+              std::fma(m.m33, v.z, std::fma(m.m23, v.y, std::fma(m.m13, v.x, acc.z))));
+}
+
+template <class R> void Solver<R>::pcg_run(R dt) {
+    (void)dt;
+    const int n = V();
+    const R strength_cfg = R(cfg.stretch_optimization_strength);
+    const R thick = R(cfg.cloth_thickness);
+    const bool hard = cfg.stretch_optimization_strength >= 0.95f && std::isfinite(cfg.cloth_thickness) && cfg.cloth_thickness > kEpsF;
+    const Sym I{R(1), R(0), R(0), R(1), R(0), R(1)};
+    // A41:3032-3069 reset
+    for (int i = 0; i < n; ++i) {
+        pcg_diag[i] = I; pcg_pinv[i] = I;
+        pcg_rhs[i] = pcg_sol[i] = pcg_res[i] = pcg_z[i] = pcg_dir[i] = pcg_adir[i] = mk<R>(0, 0, 0);
+    }
+    std::fill(pcg_off.begin(), pcg_off.end(), Sym{});
+    // A41:3085-3186 system build, rows summed in ascending edge order
+    const R strength = clamp01(strength_cfg * R(2.5));
+    std::vector<int> cursor(inc_off.begin(), inc_off.end() - 1);
+    for (int e = 0; e < cfg.edge_count; ++e) {
+        const Edge& E = springs[e];
+        const int i = E.i, j = E.j;
+        if (i >= n || j >= n || i == j) continue;
+        const int eij = cursor[i]++, eji = cursor[j]++;
+        const R wi = weight(i), wj = weight(j);
+        const bool ai = wi > R(0), aj = wj > R(0);
+        const R w = wi + wj;
+        if (!std::isfinite(w) || w <= R(0)) continue;
+        const R rest_l = R(E.rest);
+        if (!std::isfinite(rest_l) || rest_l <= R(kEpsF)) continue;
+        const V3<R> pi = pos[i], pj = pos[j];
+        if (!finite(pi) || !finite(pj)) continue;
+        const V3<R> delta = pj - pi;
+        const R lsq = dot(delta, delta);
+        if (!std::isfinite(lsq) || lsq <= R(kEpsF)) continue;
+        const R len = std::sqrt(lsq);
+        const R err = len - rest_l;
+        if (!std::isfinite(err) || std::fabs(err) <= R(1.0e-6)) continue;
+        if (strength <= R(0)) continue;
+        R limit = std::max(R(1.0e-5), std::min(std::max(rest_l, thick) * R(0.5), R(0.25)));  // A41:1070-1089
+        if (hard) limit = std::min(limit, std::max(R(1.0e-5), std::min(std::max(rest_l * R(0.18), thick * R(0.06)), R(0.0035))));
+        R proj = err * strength;
+        proj = std::max(-limit, std::min(proj, limit));
+        if (!std::isfinite(proj) || std::fabs(proj) <= R(1.0e-7)) continue;
+        const V3<R> nrm = mul(delta, R(1) / len);
+        const V3<R> desired = mul(nrm, proj / w);
+        if (ai) pcg_rhs[i] = pcg_rhs[i] + mul(desired, wi);
+        if (aj) pcg_rhs[j] = pcg_rhs[j] + mul(desired, -wj);
+        const R k = std::max(strength, R(1.0e-3));
+        const Sym blk{nrm.x * nrm.x * k, nrm.x * nrm.y * k, nrm.x * nrm.z * k, nrm.y * nrm.y * k, nrm.y * nrm.z * k, nrm.z * nrm.z * k};
+        auto addb = [&](Sym& d) { d.m11 += blk.m11; d.m12 += blk.m12; d.m13 += blk.m13; d.m22 += blk.m22; d.m23 += blk.m23; d.m33 += blk.m33; };
+        if (ai) addb(pcg_diag[i]);
+        if (aj) addb(pcg_diag[j]);
+        if (ai && aj) {
+            const Sym off{-blk.m11, -blk.m12, -blk.m13, -blk.m22, -blk.m23, -blk.m33};
+            pcg_off[eij] = off;
+            pcg_off[eji] = off;
+        }
+    }
+    // A41:3188-3226 preconditioner and initial residual
+    ExactSum<R> rz;
+    for (int i = 0; i < n; ++i) {
+        if (inv_mass[i] <= 0.0f || pinned[i]) {
+            pcg_rhs[i] = pcg_res[i] = pcg_z[i] = pcg_dir[i] = pcg_sol[i] = mk<R>(0, 0, 0);
+            pcg_pinv[i] = I;
+            continue;
+        }
+        pcg_pinv[i] = sym_inverse<Sym, R>(pcg_diag[i], &pcg_guard);
+        const V3<R> r = pcg_rhs[i];
+        const V3<R> z = sym_madd(pcg_pinv[i], r, mk<R>(0, 0, 0));
+        pcg_res[i] = r; pcg_z[i] = z; pcg_dir[i] = z; pcg_sol[i] = mk<R>(0, 0, 0);
+        rz.add(dot(r, z));
+    }
+    pcg_rz = rz.value();
+    pcg_initial = pcg_rz;
+    pcg_dad = 0.0;
+    pcg_rznext = 0.0;
+    pcg_status = 0;
+    const R wb_limit = hard ? std::max(R(2.5e-4), std::min(thick * R(0.035), R(9.0e-4))) : R(0.25);  // A41:1091-1105
+    for (int it = 0; it < 8; ++it) {
+        // A41:3228-3267 A*d
+        if (pcg_status == 0) {
+            ExactSum<R> dad;
+            for (int i = 0; i < n; ++i) {
+                if (inv_mass[i] <= 0.0f || pinned[i]) { pcg_adir[i] = mk<R>(0, 0, 0); continue; }
+                V3<R> q = sym_madd(pcg_diag[i], pcg_dir[i], mk<R>(0, 0, 0));
+                for (int c = inc_off[i]; c < inc_off[i + 1]; ++c) {
+                    const Edge& E = springs[inc_edge[c]];
+                    const int col = E.i == i ? E.j : E.i;
+                    q = sym_madd(pcg_off[c], pcg_dir[col], q);
+                }
+                pcg_adir[i] = q;
+                dad.add(dot(pcg_dir[i], q));
+            }
+            pcg_dad += dad.value();
+        }
+        // A41:3320-3405 alpha, update, write-back
+        int status = 0;
+        R alpha = 0;
+        if (!std::isfinite(pcg_rz) || pcg_rz < 0.0) status = 1;
+        else if (pcg_rz <= 1.0e-14) status = 2;
+        else if (!std::isfinite(pcg_dad) || pcg_dad <= 1.0e-14) status = 3;
+        else {
+            alpha = static_cast<R>(pcg_rz / pcg_dad);
+            if (!std::isfinite(alpha)) status = 4;
+        }
+        pcg_status = status;
+        if (status != 0) {
+            if (status != 2) ++pcg_guard;
+            continue;
+        }
+        ExactSum<R> rzn;
+        for (int i = 0; i < n; ++i) {
+            if (inv_mass[i] <= 0.0f || pinned[i]) { pcg_sol[i] = pcg_res[i] = pcg_z[i] = mk<R>(0, 0, 0); continue; }
+            const V3<R> d = pcg_dir[i], ad = pcg_adir[i];
+            V3<R> sol = mk(std::fma(d.x, alpha, pcg_sol[i].x), std::fma(d.y, alpha, pcg_sol[i].y), std::fma(d.z, alpha, pcg_sol[i].z));
+            V3<R> res = mk(std::fma(ad.x, -alpha, pcg_res[i].x), std::fma(ad.y, -alpha, pcg_res[i].y), std::fma(ad.z, -alpha, pcg_res[i].z));
+            if (!finite(sol)) sol = mk<R>(0, 0, 0);
+            if (!finite(res)) res = mk<R>(0, 0, 0);
+            const V3<R> z = sym_madd(pcg_pinv[i], res, mk<R>(0, 0, 0));
+            pcg_sol[i] = sol; pcg_res[i] = res; pcg_z[i] = z;
+            rzn.add(dot(res, z));
+            V3<R> delta = mul(d, alpha);
+            const R l = length(delta);
+            if (l > wb_limit) { delta = mul(delta, wb_limit / std::max(l, R(kEpsF))); count(kCountHardFallbacks); }
+            pos[i] = pos[i] + delta;
+            prev[i] = prev[i] + mul(delta, R(0.08));
+            pcg_max_delta = std::max(pcg_max_delta, static_cast<float>(length(delta)));
+        }
+        pcg_rznext += rzn.value();
+        if (it + 1 < 8) {
+            // A41:3421-3450 search direction, A41:3452-3464 advance
+            if (std::isfinite(pcg_rz) && std::isfinite(pcg_rznext) && pcg_rz > 1.0e-14 && pcg_rznext >= 0.0) {
+                const R beta = static_cast<R>(pcg_rznext / pcg_rz);
+                if (std::isfinite(beta)) {
+                    for (int i = 0; i < n; ++i) {
+                        if (inv_mass[i] <= 0.0f || pinned[i]) { pcg_dir[i] = mk<R>(0, 0, 0); continue; }
+                        const V3<R> d = pcg_dir[i], z = pcg_z[i];
+                        pcg_dir[i] = mk(std::fma(d.x, beta, z.x), std::fma(d.y, beta, z.y), std::fma(d.z, beta, z.z));
+                    }
+                }
+            }
+            if (std::isfinite(pcg_rznext) && pcg_rznext >= 0.0) {
+                pcg_rz = pcg_rznext;
+                pcg_dad = 0.0;
+                pcg_rznext = 0.0;
+            }
+        }
+    }
+    ++pcg_runs;
+}
+
+// Edge passes of the hard-stretch polish and caps.  Kinds: 0 = normal cap
+// (A41:2249-2362), 1 = tiny-edge cap (A41:2496-2601), 2 = extreme cap
+// (A41:2603-2708), 3 = polish contribution (A41:1995-2080), 4 = tiny-edge
+// contribution (A41:2399-2466).  Returns false when the edge does nothing;
+// otherwise the position and previous-position deltas of both ends (a side
+// with zero weight gets apply_* = false).
+template <class R> struct EdgeOut { V3<R> di, qi, dj, qj; bool ai, aj; };
+
+template <class R> bool edge_kind(int kind, V3<R> pi, V3<R> pj, V3<R> qi, V3<R> qj, R wi, R wj, R rest, R thick, R strength, R dt,
+                                  EdgeOut<R>* out) {
+    const R eps = R(kEpsF);
+    const bool tiny_kind = kind == 1 || kind == 4;
+    if (!std::isfinite(rest) || rest <= eps) return false;
+    if (kind != 3 && (tiny_kind ? rest > R(7.5e-4) : rest <= R(7.5e-4))) return false;
+    const R w = wi + wj;
+    if (!std::isfinite(w) || w <= R(0)) return false;
+    if (!finite(pi) || !finite(pj)) return false;
+    if (kind != 4 && (!finite(qi) || !finite(qj))) return false;
+    const V3<R> edge = pj - pi;
+    const R lsq = dot(edge, edge);
+    if (!std::isfinite(lsq) || lsq <= R(1.0e-14)) return false;
+    const R len = std::sqrt(lsq);
+    const bool high = strength >= R(0.95f);
+    out->di = out->qi = out->dj = out->qj = mk<R>(0, 0, 0);
+    out->ai = out->aj = false;
+    if (kind >= 3) {
+        const R target = kind == 3 ? rest * (high ? R(1.08f) : R(1.18f)) : rest * R(1.06f);
+        const R excess = len - target;
+        if (!std::isfinite(excess) || excess <= R(0)) return false;
+        const V3<R> corr = mul(edge, (kind == 3 ? -R(0.90f) : -R(1.0f)) * excess / std::max(w * len, eps));
+        if (!finite(corr)) return false;
+        if (length(corr) <= R(1.0e-20)) return false;
+        const R md = kind == 3 ? std::max(R(5.0e-4), std::min(std::max(rest * R(2), thick * R(0.25)), R(0.008)))
+                               : std::min(std::max(rest * R(3), R(2.5e-4)), R(8.0e-4));
+        if (wi > R(0)) {
+            const V3<R> d = limit_delta(mul(corr, -wi), md);
+            if (finite(d) && length(d) > R(0)) { out->di = d; out->ai = true; }
+        }
+        if (wj > R(0)) {
+            const V3<R> d = limit_delta(mul(corr, wj), md);
+            if (finite(d) && length(d) > R(0)) { out->dj = d; out->aj = true; }
+        }
+        return out->ai || out->aj;
+    }
+    R start, target, md, sync;
+    if (kind == 0) {
+        start = rest * (high ? R(1.12f) : R(1.32f)); target = rest * (high ? R(1.04f) : R(1.22f));
+        md = std::min(std::max(rest * R(0.60f), thick * R(0.10f)), R(0.010f)); sync = R(0.08f);
+    } else if (kind == 1) {
+        start = rest * R(1.35f); target = rest * R(1.12f);
+        md = std::min(std::max(rest * R(6.0f), R(5.0e-4)), R(1.5e-3)); sync = R(1.0f);
+    } else {
+        start = rest * R(1.24f); target = rest * R(1.10f);
+        md = std::min(std::max(rest * R(1.25f), thick * R(0.10f)), R(0.006f)); sync = R(0.30f);
+    }
+    if (!std::isfinite(len) || len <= start) return false;
+    const R excess = len - target;
+    if (!std::isfinite(excess) || excess <= R(0)) return false;
+    const V3<R> corr = mul(edge, -excess / std::max(w * len, eps));
+    if (!finite(corr) || length(corr) <= R(1.0e-20)) return false;
+    V3<R> pdi = mk<R>(0, 0, 0), pdj = mk<R>(0, 0, 0);
+    const V3<R> nrm = mul(edge, R(1) / std::max(len, eps));
+    const R sdt = kind == 0 ? std::max(dt, eps) : dt;
+    if (std::isfinite(sdt) && sdt > eps) {
+        const R inv = R(1) / sdt;
+        const V3<R> vi = mul(pi - qi, inv), vj = mul(pj - qj, inv);
+        const R sep = dot(vj - vi, nrm);
+        if (std::isfinite(sep) && sep > R(0)) {
+            const R trim = sep * sdt * R(1.0f) / w;
+            if (wi > R(0)) pdi = pdi + mul(nrm, -wi * trim);
+            if (wj > R(0)) pdj = pdj + mul(nrm, wj * trim);
+        }
+    }
+    if (wi > R(0)) {
+        const V3<R> d = limit_delta(mul(corr, -wi), md);
+        if (finite(d) && length(d) > R(0)) { out->di = d; out->qi = pdi + mul(d, sync); out->ai = true; }
+    }
+    if (wj > R(0)) {
+        const V3<R> d = limit_delta(mul(corr, wj), md);
+        if (finite(d) && length(d) > R(0)) { out->dj = d; out->qj = pdj + mul(d, sync); out->aj = true; }
+    }
+    return out->ai || out->aj;
+}
+
+// One cap kind over edges [lo, hi) as a Jacobi sweep: every edge reads the
+// positions from before the sweep (for a valid colour range this equals the
+// reference's race-free per-colour launch).
+template <class R> void Solver<R>::edge_jacobi(int kind, int lo, int hi, R dt) {
+    const int n = V();
+    const std::vector<V3<R>> P = pos, Q = prev;
+    const R thick = R(cfg.cloth_thickness), st = R(cfg.stretch_optimization_strength);
+    for (int e = lo; e < hi; ++e) {
+        const Edge& E = springs[e];
+        if (E.i >= n || E.j >= n || E.i == E.j) continue;
+        EdgeOut<R> o;
+        if (!edge_kind<R>(kind, P[E.i], P[E.j], Q[E.i], Q[E.j], weight(E.i), weight(E.j), R(E.rest), thick, st, dt, &o)) continue;
+        if (o.ai) { pos[E.i] = pos[E.i] + o.di; prev[E.i] = prev[E.i] + o.qi; }
+        if (o.aj) { pos[E.j] = pos[E.j] + o.dj; prev[E.j] = prev[E.j] + o.qj; }
+    }
+}
+
+// Colour ranges forward (and then backward) `passes` times (A41:7761-7803);
+// without a colouring, one range over all edges.
+template <class R> void Solver<R>::cap_ranges(int kind, int passes, bool reverse_too, R dt) {
+    if (cfg.stretch_optimization_strength < 0.95f || cfg.edge_count <= 0) return;
+    for (int p = 0; p < passes; ++p) {
+        if (edge_colors.empty()) { edge_jacobi(kind, 0, cfg.edge_count, dt); continue; }
+        for (int c = 0; c < cfg.edge_color_count; ++c)
+            edge_jacobi(kind, std::max(edge_colors[c], 0), std::min(edge_colors[c + 1], cfg.edge_count), dt);
+        if (reverse_too)
+            for (int c = cfg.edge_color_count - 1; c >= 0; --c)
+                edge_jacobi(kind, std::max(edge_colors[c], 0), std::min(edge_colors[c + 1], cfg.edge_count), dt);
+    }
+}
+
+// Polish / tiny-edge averaging: sum the edge contributions per vertex in
+// ascending edge order, then apply the clamped average (A41:2082-2112,
+// 2468-2494).
+template <class R> void Solver<R>::gather_average(int kind, R dt) {
+    const int n = V();
+    const std::vector<V3<R>> P = pos, Q = prev;
+    const R thick = R(cfg.cloth_thickness), st = R(cfg.stretch_optimization_strength);
+    std::vector<V3<R>> sum(n, mk<R>(0, 0, 0));
+    std::vector<int> cnt(n, 0);
+    for (int e = 0; e < cfg.edge_count; ++e) {
+        const Edge& E = springs[e];
+        if (E.i >= n || E.j >= n || E.i == E.j) continue;
+        EdgeOut<R> o;
+        if (!edge_kind<R>(kind, P[E.i], P[E.j], Q[E.i], Q[E.j], weight(E.i), weight(E.j), R(E.rest), thick, st, dt, &o)) continue;
+        if (o.ai) { sum[E.i] = sum[E.i] + o.di; cnt[E.i]++; }
+        if (o.aj) { sum[E.j] = sum[E.j] + o.dj; cnt[E.j]++; }
+    }
+    const R md = kind == 3 ? std::max(R(5.0e-4), std::min(std::max(thick * R(0.20f), R(1.0e-3)), R(0.005f)))
+                           : std::min(std::max(thick * R(0.015f), R(2.5e-4)), R(9.0e-4));
+    const R sync = kind == 3 ? R(0.16f) : R(0.24f);
+    for (int i = 0; i < n; ++i) {
+        if (inv_mass[i] <= 0.0f || pinned[i] || cnt[i] <= 0) continue;
+        V3<R> d = mul(sum[i], R(1) / R(cnt[i]));
+        if (!finite(d)) { count(kCountHardFallbacks); continue; }
+        d = limit_delta(d, md);
+        if (!finite(d) || length(d) <= R(0)) continue;
+        pos[i] = pos[i] + d;
+        prev[i] = prev[i] + mul(d, sync);
+    }
+}
+
+// A41:7823-7857
+template <class R> void Solver<R>::polish(R dt) {
+    if (cfg.edge_count <= 0 || cfg.stretch_optimization_strength < 0.95f) return;
+    for (int p = 0; p < 2; ++p) {
+        gather_average(3, dt);
+        cap_ranges(0, 2, true, dt);
+        for (int t = 0; t < 2; ++t) {
+            gather_average(4, dt);
+            cap_ranges(1, 3, false, dt);
+        }
+    }
+}
+
+// A41:7997-8027
+template <class R> void Solver<R>::final_caps(R dt) {
+    cap_ranges(0, 2, true, dt);
+    cap_ranges(2, 3, false, dt);
+    cap_ranges(1, 3, false, dt);
+}
+
 // A41:9087-9398
 template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch) {
     auto t0 = std::chrono::steady_clock::now();
@@ -1509,6 +1930,11 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
     diag.unsupported_force_field_count = unsupported_fields;
     std::memset(counts, 0, sizeof(counts));
     max_smoothed = 0.0f;
+    pcg_guard = 0;
+    pcg_runs = 0;
+    pcg_max_delta = 0.0f;
+    pcg_rz = pcg_dad = pcg_rznext = pcg_initial = 0.0;
+    pcg_status = 0;
     substeps = std::max(substeps, 1);
     iterations = std::max(iterations, 1);
     const R sub_dt = R(cfg.dt / static_cast<float>(substeps));
@@ -1520,9 +1946,16 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
         if (!pins.empty()) pin_project(exponent, sub_dt);
         for (int it = 0; it < iterations; ++it) {
             const bool last_it = it == iterations - 1;
-            if (cfg.edge_count > 0) {
+            const bool pcg_on = cfg.stretch_optimization_enabled != 0;
+            if (cfg.edge_count > 0 && !pcg_on) {
                 if (stretch_mode == SSBL_EXT_STRETCH_COLORED) stretch_colored(sub_dt);
                 else stretch_jacobi(sub_dt);
+            }
+            const bool run_pcg = pcg_on && last_it;   // substep cadence 1 (A41:284-292)
+            if (run_pcg) {
+                pcg_run(sub_dt);
+                polish(sub_dt);
+                cap_ranges(2, 3, false, sub_dt);   // extreme caps, A41:7974-7995
             }
             const bool final_c = s == substeps - 1 && last_it;
             if (final_c && cfg.lra_count > 0) lra_jacobi(sub_dt);
@@ -1536,6 +1969,7 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
             const bool run_self = scheduled && V() > 0 && r > R(0);
             if ((scheduled && !run_self) || (self_it && !scheduled)) diag.self_collision_skipped_launches += 1;
             if (run_self) self_collision();
+            if (run_pcg && final_c) final_caps(sub_dt);
             // the reference's final pin polish also runs on every last iteration
             // because dynamic collision is scheduled there (A41:9230, 9360-9369)
             const bool polish = !pins.empty() && (analytic_on || sdf_ready || last_it || run_self);
@@ -1556,17 +1990,32 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
         diag.static_sdf_contact_count = static_cast<long long>(counts[kCountStaticSdf]);
         diag.candidate_count = static_cast<long long>(counts[kCountSelfCandidates] + counts[kCountStaticSdf]);
         diag.abi41_lra_tack_count = static_cast<long long>(counts[kCountLraTacks]);
-        diag.abi41_bending_wing_count = static_cast<long long>(counts[kCountBendingWings]);
+        diag.abi41_bending_wing_count = counts[kCountBendingWings] > 0 ? static_cast<long long>(counts[kCountBendingWings]) : wing_count;
         diag.abi41_tack_jitter_guarded = static_cast<long long>(counts[kCountTackGuards]);
         diag.abi41_bending_guarded = static_cast<long long>(counts[kCountBendingGuards]);
     }
-    diag.static_sdf_rebuild_count = sdf_rebuilds;
-    diag.static_sdf_voxel_count = sdf_ready ? 1LL * sdf_dim[0] * sdf_dim[1] * sdf_dim[2] : 0;
-    diag.static_sdf_grid_x = sdf_dim[0];
-    diag.static_sdf_grid_y = sdf_dim[1];
-    diag.static_sdf_grid_z = sdf_dim[2];
-    diag.static_sdf_build_ms = sdf_build_ms;
-    diag.static_sdf_unsigned_fallback_count = sdf_unsigned;
+    if (cfg.stretch_optimization_enabled) {  // A41:7586-7624, 8159-8176
+        diag.abi41_pcg_csr_nnz = inc_off[cfg.vertex_count];
+        diag.abi41_pcg_texture_ready = 1;
+        diag.abi41_pcg_iterations = 8 * pcg_runs;
+        if (fetch && pcg_runs > 0) {
+            double init = pcg_initial;
+            if ((!std::isfinite(init) || init <= 0.0) && std::isfinite(pcg_rz) && pcg_rz >= 0.0) init = pcg_rz;
+            if (std::isfinite(init) && init >= 0.0) {
+                diag.abi41_pcg_initial_residual = static_cast<float>(std::sqrt(std::max(init, 0.0)));
+                diag.abi41_pcg_final_residual = diag.abi41_pcg_initial_residual;
+            }
+            const double fin = pcg_status == 2 ? pcg_rz : pcg_rznext;
+            if ((pcg_status == 0 || pcg_status == 2) && std::isfinite(fin) && fin >= 0.0)
+                diag.abi41_pcg_final_residual = static_cast<float>(std::sqrt(std::max(fin, 0.0)));
+            else if (pcg_status != 0 && pcg_status != 2)
+                diag.abi41_pcg_guarded += 1;
+            diag.abi41_pcg_guarded += pcg_guard;
+            diag.abi41_pcg_max_delta = pcg_max_delta;
+        }
+    }
+    diag.abi41_bending_texture_ready = wing_count > 0 ? 1 : 0;
+    populate_sdf_diag();
     diag.step_ms = static_cast<float>(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     diag.constraints_ms = diag.step_ms;
     diag.resolved_contacts = diag.abi41_soft_contact_count + diag.abi41_exact_impulse_contact_count +
@@ -1745,7 +2194,9 @@ SSBL_API int ssbl_get_diagnostics(void* handle, SsblXpbdDiagnostics* out) {
     return with(handle, [&](auto& s) { *out = s.diag; return 1; });
 }
 
-SSBL_API unsigned int ssbl_capabilities(void) { return SSBL_CAP_PIN_WEIGHTS | SSBL_CAP_HIP_BATCHED_STEP; }
+SSBL_API unsigned int ssbl_capabilities(void) {
+    return SSBL_CAP_STRETCH_OPTIMIZATION | SSBL_CAP_PIN_WEIGHTS | SSBL_CAP_HIP_BATCHED_STEP;
+}
 
 SSBL_API void* ssbl_create_global_dynamic_scene(void) { g_error.clear(); return new int(0); }
 SSBL_API int ssbl_destroy_global_dynamic_scene(void* scene) { g_error.clear(); delete static_cast<int*>(scene); return 1; }

@@ -46,13 +46,14 @@ enum PassOps : unsigned {
     kOpLra = 1u << 6,         // LRA tether gather (src -> dst), instead of stretch
     kOpCount = 1u << 7,       // count diagnostics
     kOpLastIter = 1u << 8,    // last iteration of the substep (pin polish always runs)
+    kOpBend = 1u << 9,        // uncoloured bend gather (src -> dst), instead of stretch
 };
 
 // Everything a kernel needs to know about one cloth.  Lives in device memory
 // (a table of these for a batch); host keeps a mirror.
 struct ClothView {
     int nv, ne, nt, nbend, nlra, npin, nfield;
-    int pad0;
+    int direct_stretch;        // 0 when the cloth uses the hard-stretch PCG path
     float4* pos0;
     float4* pos1;
     float4* prev;
@@ -67,6 +68,8 @@ struct ClothView {
     const int4* tris;          // {a, b, c, 0}
     const int2* bends;         // bend pairs
     const float* bend_rest;
+    const int* bend_off;       // V+1, uncoloured bends only
+    const int4* bend_adj;      // {other | first bit, rest bits, bend index, 0}
     const int* pin_off;        // V+1 -> pins of that vertex (ascending pin index)
     const int* pin_ids;
     const float4* pins;        // {target xyz, relaxation}; relaxation 2 = hard, <=0 ignored

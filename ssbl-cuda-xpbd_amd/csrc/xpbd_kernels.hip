@@ -482,7 +482,7 @@ __global__ __launch_bounds__(kTile) void k_vertex_pass(const ClothView* __restri
     // (inverse mass <= 0, A41:8861-8865)
     const bool pinned = w == 0.0f;
     F3 p = f3(p4), q = f3(c.prev[v]), vel = f3(c.vel[v]);
-    if (ops & kOpStretch) {
+    if ((ops & kOpStretch) && c.direct_stretch) {
         // A41:1927-1980 gathered: each incident edge is evaluated exactly as
         // the edge thread would (the difference vector is negated for the
         // second endpoint, which flips the correction's sign bit-exactly).
@@ -564,6 +564,46 @@ __global__ __launch_bounds__(kTile) void k_vertex_pass(const ClothView* __restri
         }
         if (count && tacks) atomicAdd(&c.counts[kCountLraTacks], static_cast<unsigned long long>(tacks));
         if (count && guards) atomicAdd(&c.counts[kCountTackGuards], static_cast<unsigned long long>(guards));
+    }
+    if (ops & kOpBend) {
+        // A41:2805-2884 (uncoloured bends) gathered like the stretch pass
+        const int b = c.bend_off[v], e = c.bend_off[v + 1];
+        const F3 p0 = p;
+        unsigned wings = 0, guards = 0;
+        for (int k = b; k < e; ++k) {
+            const int4 a = c.bend_adj[k];
+            const int other = a.x & 0x7fffffff;
+            const bool first = a.x < 0;
+            const float restl = __int_as_float(a.y);
+            const float4 o4 = src[other];
+            const float wsum = w + o4.w;
+            if (!isfinite(wsum) || wsum <= 0.0f) continue;
+            if (!isfinite(restl) || restl <= kEps) { guards += first; continue; }
+            const F3 po = f3(o4);
+            if (!finite3(p0) || !finite3(po)) { guards += first; continue; }
+            const F3 delta = po - p0;
+            const float lsq = __builtin_fmaf(delta.x, delta.x, __builtin_fmaf(delta.y, delta.y, delta.z * delta.z));
+            if (!isfinite(lsq) || lsq <= 1.0e-10f) { guards += first; continue; }
+            const float inv_len = rsqrt_exact(lsq);
+            const float len = lsq * inv_len;
+            const float cval = len - restl;
+            if (!isfinite(cval) || fabsf(cval) <= 1.0e-6f) continue;
+            const float dl = -cval / fmaxf(wsum + c.bend_alpha, kEps);
+            F3 corr = mul(delta, 0.10f * dl * inv_len);
+            if (!finite3(corr)) { guards += first; continue; }
+            const float cl = len3(corr);
+            const float mc = fmaxf(2.5e-4f, fminf(fmaxf(restl, c.thickness) * 0.08f, 0.020f));
+            if (!isfinite(cl) || cl <= 0.0f) continue;
+            if (cl > mc) { corr = mul(corr, mc / fmaxf(cl, kEps)); guards += first; }
+            if (w > 0.0f) {
+                const F3 d = mul(corr, -w);
+                p = p + d;
+                q = q + mul(d, 0.06f);
+            }
+            wings += first;
+        }
+        if (count && wings) atomicAdd(&c.counts[kCountBendingWings], static_cast<unsigned long long>(wings));
+        if (count && guards) atomicAdd(&c.counts[kCountBendingGuards], static_cast<unsigned long long>(guards));
     }
     if (ops & kOpPin) apply_pins(c, v, pinned, p, q, vel, dt);
     if ((ops & kOpAnalytic) && !pinned) analytic_contacts(c, p, q, vel, dt, count);
@@ -699,7 +739,9 @@ void launch_integrate(const LaunchCtx& c, int parity, unsigned ops) {
 }
 
 void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops) {
-    const char* name = (ops & kOpStretch) ? "vertex_pass_stretch" : ((ops & kOpLra) ? "vertex_pass_lra" : "vertex_pass_local");
+    const char* name = (ops & kOpStretch) ? "vertex_pass_stretch"
+                       : (ops & kOpLra) ? "vertex_pass_lra"
+                       : (ops & kOpBend) ? "vertex_pass_bend" : "vertex_pass_local";
     TimedLaunch t(name, c.stream);
     hipLaunchKernelGGL(k_vertex_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, src_parity,
                        dst_parity, ops);

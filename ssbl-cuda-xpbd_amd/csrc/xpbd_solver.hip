@@ -14,6 +14,7 @@
 #include "ssbl_xpbd_hip.h"
 #include "ssbl_xpbd_hip_ext.h"
 #include "xpbd_device.hpp"
+#include "xpbd_pcg.hpp"
 #include "xpbd_self.hpp"
 #include "xpbd_timing.hpp"
 
@@ -180,6 +181,7 @@ struct Solver {
     std::vector<float> h_weff;           // effective inverse mass
     std::vector<int> edge_colors, bend_colors;
     bool bend_coloring_valid = false;
+    bool bend_gather = false;
     int lra_invalid = 0;
     std::vector<HostPin> pins;
     int pins_iterations = -1;
@@ -194,9 +196,9 @@ struct Solver {
     float sdf_build_ms = 0.0f;
     // device state
     Dev<float4> pos0, pos1, prev, vel, rest;
-    Dev<int> adj_off, lra_off, tri_off, tri_inc, pin_off, pin_ids;
+    Dev<int> adj_off, lra_off, tri_off, tri_inc, pin_off, pin_ids, bend_off;
     Dev<int2> adj, bends;
-    Dev<int4> lra_adj, tris, bend_segs;
+    Dev<int4> lra_adj, tris, bend_segs, bend_adj;
     Dev<float> bend_rest, area_share, rest_scale, sdf, static_tris;
     Dev<float4> d_pins;
     Dev<SsblXpbdForceField> d_fields;
@@ -209,6 +211,8 @@ struct Solver {
     int ntiles = 0;
     Staging staging;
     SelfCollision self;
+    PcgPath pcg;
+    long long wing_count = 0;
     ClothView view{};
     SsblXpbdDiagnostics diag{};
     bool counts_pending = false;
@@ -225,6 +229,16 @@ struct Solver {
         return std::max(std::max(cfg.cloth_thickness, cfg.collision_margin), 0.0f);
     }
     bool analytic_on() const { return colliders.use_ground || colliders.use_wall || colliders.use_sphere; }
+    void populate_sdf_diag() {  // A41:6104-6121
+        diag.static_sdf_rebuild_count = sdf_rebuilds;
+        diag.static_sdf_voxel_count = sdf_ready ? 1LL * sdf_dim[0] * sdf_dim[1] * sdf_dim[2] : 0;
+        diag.static_sdf_grid_x = sdf_dim[0];
+        diag.static_sdf_grid_y = sdf_dim[1];
+        diag.static_sdf_grid_z = sdf_dim[2];
+        diag.static_sdf_build_ms = sdf_build_ms;
+        diag.static_sdf_unsigned_fallback_count = sdf_unsigned;
+        diag.static_triangle_count = cfg.static_triangle_count;
+    }
 };
 
 SelfCollision* solver_self(Solver* s) { return &s->self; }
@@ -341,7 +355,6 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     cfg.stretch_optimization_strength = std::min(std::max(cfg.stretch_optimization_strength, 0.0f), 1.0f);
     cfg.stretch_optimization_enabled =
         (cfg.edge_count > 0 && cfg.stretch_optimization_enabled && cfg.stretch_optimization_strength > 0.0f) ? 1 : 0;
-    if (cfg.stretch_optimization_enabled) return fail("stretch optimization (PCG) is not available in this build yet");
     if (cfg.self_collision && cfg.self_collision_mode == 1) return fail("fast self-collision mode is not available in this build yet");
     pressure_accel_scale = std::max(1.0f, std::min(env_float("SSBL_ABI41_PRESSURE_UI_ACCEL_SCALE", 1792.0f), 8192.0f));
     pressure_impulse_rest_scale = std::max(0.05f, std::min(env_float("SSBL_ABI41_PRESSURE_IMPULSE_REST_SCALE", 0.85f), 8.0f));
@@ -430,11 +443,33 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
             }
         }
     }
-    if (cfg.bend_count > 0 && !bend_coloring_valid) return fail("bend constraints need a valid colouring in this build");
+    // without a valid colouring the bends run as one Jacobi sweep, gathered
+    // per vertex in ascending bend order (A41:2805-2884 made deterministic)
+    std::vector<int> h_bend_off(n + 1, 0);
+    std::vector<int4> h_bend_adj;
+    if (cfg.bend_count > 0 && !bend_coloring_valid) {
+        for (int b = 0; b < cfg.bend_count; ++b) {
+            const int i = h_bends[b].x, j = h_bends[b].y;
+            if (i < 0 || j < 0 || i >= n || j >= n || i == j) continue;
+            h_bend_off[i + 1]++; h_bend_off[j + 1]++;
+        }
+        for (int i = 0; i < n; ++i) h_bend_off[i + 1] += h_bend_off[i];
+        h_bend_adj.resize(h_bend_off[n]);
+        std::vector<int> cur(h_bend_off.begin(), h_bend_off.end() - 1);
+        for (int b = 0; b < cfg.bend_count; ++b) {
+            const int i = h_bends[b].x, j = h_bends[b].y;
+            if (i < 0 || j < 0 || i >= n || j >= n || i == j) continue;
+            int rb;
+            std::memcpy(&rb, &h_bend_rest[b], 4);
+            h_bend_adj[cur[i]++] = make_int4(static_cast<int>(static_cast<unsigned>(j) | 0x80000000u), rb, b, 0);
+            h_bend_adj[cur[j]++] = make_int4(i, rb, b, 0);
+        }
+    }
+    bend_gather = !h_bend_adj.empty();
     h_bend_segs.clear();
     bend_seg_first.clear();
     bend_seg_count.clear();
-    for (int c = 0; c < cfg.bend_color_count; ++c) {
+    for (int c = 0; c < (bend_coloring_valid ? cfg.bend_color_count : 0); ++c) {
         const int lo = std::max(bend_colors[c], 0), hi = std::min(bend_colors[c + 1], cfg.bend_count);
         bend_seg_first.push_back(static_cast<int>(h_bend_segs.size()));
         for (int b = lo; b < hi; b += kTile) h_bend_segs.push_back(make_int4(0, b, std::min(hi, b + kTile), 0));
@@ -494,6 +529,7 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
               lra_adj.upload(h_lra, "LRA incidence allocation") && tri_off.upload(h_tri_off, "triangle incidence allocation") &&
               tri_inc.upload(h_tri_inc, "triangle incidence allocation") && tris.upload(h_tris, "triangle allocation") &&
               bends.upload(h_bends, "bend allocation") && bend_rest.upload(h_bend_rest, "bend rest allocation") &&
+              bend_off.upload(h_bend_off, "bend incidence allocation") && bend_adj.upload(h_bend_adj, "bend incidence allocation") &&
               bend_segs.upload(h_bend_segs, "bend segment allocation") && area_share.upload(h_share, "pressure share allocation") &&
               rest_scale.upload(h_scale, "pressure scale allocation") && pin_off.upload(h_pin_off, "pin allocation") &&
               counts.alloc(kCountSlots, "diagnostic allocation") && max_smoothed.alloc(1, "diagnostic allocation") &&
@@ -502,6 +538,25 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     if (!hip_ok(hipMemset(counts.p, 0, sizeof(unsigned long long) * kCountSlots), "diagnostic reset")) return false;
     if (!hip_ok(hipMemset(max_smoothed.p, 0, sizeof(float)), "diagnostic reset")) return false;
     if (cfg.self_collision && !self.prepare(cfg, edges, tri, n, counts.p)) return false;
+    if (cfg.stretch_optimization_enabled && !pcg.prepare(cfg, edges, edge_rest, edge_colors, n, h_adj_off[n], ntiles, counts.p))
+        return false;
+    // A41:7193-7302: bending "wings" (an edge met again with another opposite
+    // vertex); the reference builds them for its diagnostics only.
+    wing_count = 0;
+    if (cfg.triangle_count > 1) {
+        std::unordered_map<uint64_t, int> owner;
+        for (int t = 0; t < cfg.triangle_count; ++t) {
+            const int ids[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
+            if (ids[0] >= n || ids[1] >= n || ids[2] >= n) continue;
+            for (int k = 0; k < 3; ++k) {
+                const int x = ids[k], y = ids[(k + 1) % 3], opp = ids[(k + 2) % 3];
+                const uint64_t key = (uint64_t(unsigned(std::min(x, y))) << 32) | uint64_t(unsigned(std::max(x, y)));
+                auto it = owner.find(key);
+                if (it == owner.end()) { owner.emplace(key, opp); continue; }
+                if (it->second != opp) ++wing_count;
+            }
+        }
+    }
     diag.finite_flag = 1;
     const int stc = std::max(cfg.static_triangle_count, 0);
     if (stc > 0 && !mesh->static_triangles) return fail("static triangle data is required when triangle count is nonzero");
@@ -549,7 +604,7 @@ bool Solver::build_sdf(const float* triangles, int count) {
     sdf_signed = 0;
     nstatic = 0;
     cfg.static_triangle_count = count;
-    if (count <= 0) return true;
+    if (count <= 0) { populate_sdf_diag(); return true; }
     const auto t0 = std::chrono::steady_clock::now();
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int i = 0; i < 3 * count; ++i) {
@@ -587,6 +642,7 @@ bool Solver::build_sdf(const float* triangles, int count) {
     sdf_ready = 1;
     sdf_rebuilds++;
     sdf_build_ms = static_cast<float>(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    populate_sdf_diag();
     return true;
 }
 
@@ -653,6 +709,7 @@ void Solver::fill_view(int substeps) {
     v.pos0 = pos0.p; v.pos1 = pos1.p; v.prev = prev.p; v.vel = vel.p; v.rest = rest.p;
     v.adj_off = adj_off.p; v.adj = adj.p; v.lra_off = lra_off.p; v.lra_adj = lra_adj.p;
     v.tri_off = tri_off.p; v.tri_inc = tri_inc.p; v.tris = tris.p; v.bends = bends.p; v.bend_rest = bend_rest.p;
+    v.bend_off = bend_off.p; v.bend_adj = bend_adj.p;
     v.pin_off = pin_off.p; v.pin_ids = pin_ids.p; v.pins = d_pins.p;
     v.area_share = area_share.p; v.rest_scale = rest_scale.p; v.fields = d_fields.p;
     v.sdf = sdf.p; v.static_tris = static_tris.p; v.nstatic = nstatic;
@@ -679,77 +736,102 @@ void Solver::fill_view(int substeps) {
     for (int k = 0; k < 3; ++k) { v.sdf_dim[k] = sdf_dim[k]; v.sdf_origin[k] = sdf_origin[k]; }
     v.sdf_voxel = sdf_voxel;
     v.need_normal = (v.use_pressure || !fields.empty()) ? 1 : 0;
+    v.direct_stretch = cfg.stretch_optimization_enabled ? 0 : 1;
     v.contacts_on = (analytic_on() || sdf_ready) ? 1 : 0;
 }
 
 // ------------------------------------------------------------ schedule
-// One step (A41:9165-9378) of every member of `members` (one launch per phase
-// covers all of them).  `ctx` addresses their views and tiles.
+// One step (A41:9165-9378) of every member of `members`; one launch per
+// phase covers all of them.  `ctx` addresses their views and tiles;
+// `dviews[m]` is member m's view inside that table.
 struct StepPlan {
-    bool any_pins = false, any_edges = false, any_lra = false, any_bend = false, any_self = false;
-    int max_bend_colors = 0;
+    bool any_pins = false, any_edges = false, any_lra = false, any_bend = false, any_self = false, any_pcg = false;
+    bool any_bend_gather = false;
 };
 
-static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const std::vector<int4>* bend_segs_host,
-                         const int4* bend_segs_dev, const std::vector<int>& seg_first, const std::vector<int>& seg_count,
-                         SelfBatch* self_batch, int substeps, int iterations, bool count) {
+static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const int4* bend_segs_dev,
+                         const std::vector<int>& seg_first, const std::vector<int>& seg_count, SelfBatch* self_batch, int substeps,
+                         int iterations, bool count) {
     StepPlan plan;
     for (Solver* s : members) {
         plan.any_pins |= !s->pins.empty();
-        plan.any_edges |= s->cfg.edge_count > 0;
+        plan.any_edges |= s->cfg.edge_count > 0 && !s->cfg.stretch_optimization_enabled;
         plan.any_lra |= s->cfg.lra_count > 0;
         plan.any_bend |= s->cfg.bend_count > 0;
         plan.any_self |= s->cfg.self_collision && s->nv > 0 && s->self_radius() > 0.0f;
+        plan.any_pcg |= s->pcg.enabled();
+        plan.any_bend_gather |= s->bend_gather;
     }
-    (void)bend_segs_host;
     const unsigned cnt = count ? kOpCount : 0u;
     const unsigned pin_ops = plan.any_pins ? (kOpPin | kOpPinPolish) : 0u;
+    const unsigned stretch = plan.any_edges ? kOpStretch : 0u;
+    hipStream_t st = ctx.stream;
     int parity = 0;
-    for (int s = 0; s < substeps; ++s) {
+    // the hard-stretch passes run per member; gathers flip a member's buffers,
+    // so each member is realigned to the common parity afterwards
+    auto per_pcg = [&](auto&& fn) {
+        for (size_t m = 0; m < members.size(); ++m) {
+            Solver* s = members[m];
+            if (!s->pcg.enabled()) continue;
+            const int got = fn(*s, dviews + m);
+            if (got != parity) s->pcg.copy(st, dviews + m, s->d_tiles.p, got);
+        }
+    };
+    for (int sub = 0; sub < substeps; ++sub) {
         launch_integrate(ctx, parity, plan.any_pins ? kOpPin : 0u);
         parity ^= 1;
         for (int it = 0; it < iterations; ++it) {
             const bool last = it == iterations - 1;
-            const bool final_c = last && s == substeps - 1;
+            const bool final_c = last && sub == substeps - 1;
             const unsigned last_op = last ? kOpLastIter : 0u;
-            const unsigned stretch = plan.any_edges ? kOpStretch : 0u;
             bool self_now = false;
             if (plan.any_self && last) {
                 for (Solver* m : members) {
                     const int interval = std::max(m->cfg.self_collision_interval, 1);
-                    if (m->cfg.self_collision && (((s + 1) % interval) == 0 || s == substeps - 1)) self_now = true;
+                    if (m->cfg.self_collision && (((sub + 1) % interval) == 0 || sub == substeps - 1)) self_now = true;
                 }
             }
-            const bool split_final = final_c && (plan.any_lra || plan.any_bend);
-            if (!split_final && !self_now) {
+            const bool run_pcg = plan.any_pcg && last;   // substep cadence 1 (A41:284-292)
+            const bool split = run_pcg || self_now || (final_c && (plan.any_lra || plan.any_bend));
+            if (!split) {
                 launch_vertex_pass(ctx, parity, parity ^ 1,
                                    stretch | pin_ops | kOpAnalytic | kOpSdf | last_op | (last ? kOpVelocity : 0u) | cnt);
                 parity ^= 1;
                 continue;
             }
-            if (split_final) {
-                // stretch, then LRA and the coloured bend sweep, then the
-                // vertex-local contact work (A41:9178-9229 order)
-                launch_vertex_pass(ctx, parity, parity ^ 1, stretch | cnt);
+            // A41:9178-9369 order: stretch (direct or PCG + polish + caps),
+            // final LRA and bends, pins, analytic and SDF contacts, self
+            // collision, final caps, second pin projection
+            launch_vertex_pass(ctx, parity, parity ^ 1, stretch | cnt);
+            parity ^= 1;
+            if (run_pcg) {
+                per_pcg([&](Solver& s, const ClothView* cv) {
+                    s.pcg.solve(st, cv, s.d_tiles.p, parity);
+                    return s.pcg.polish(st, cv, s.d_tiles.p, parity, s.view.dt_sub);
+                });
+            }
+            if (final_c && plan.any_lra) {
+                launch_vertex_pass(ctx, parity, parity ^ 1, kOpLra | cnt);
                 parity ^= 1;
-                if (plan.any_lra) {
-                    launch_vertex_pass(ctx, parity, parity ^ 1, kOpLra | cnt);
+            }
+            if (final_c && plan.any_bend) {
+                for (size_t c = 0; c < seg_first.size(); ++c)
+                    launch_bend_color(ctx, bend_segs_dev + seg_first[c], seg_count[c], parity, cnt);
+                if (plan.any_bend_gather) {
+                    launch_vertex_pass(ctx, parity, parity ^ 1, kOpBend | cnt);
                     parity ^= 1;
                 }
-                if (plan.any_bend) {
-                    for (size_t c = 0; c < seg_first.size(); ++c)
-                        launch_bend_color(ctx, bend_segs_dev + seg_first[c], seg_count[c], parity, cnt);
-                }
-                launch_vertex_pass(ctx, parity, parity,
-                                   pin_ops | kOpAnalytic | kOpSdf | last_op | (self_now ? 0u : kOpVelocity) | cnt);
-            } else {
-                launch_vertex_pass(ctx, parity, parity ^ 1, stretch | (pin_ops & ~kOpPinPolish) | kOpAnalytic | kOpSdf | cnt);
-                parity ^= 1;
             }
+            const bool tail = self_now || (run_pcg && final_c);
+            launch_vertex_pass(ctx, parity, parity,
+                               (tail ? (pin_ops & ~kOpPinPolish) : (pin_ops | last_op | kOpVelocity)) | kOpAnalytic | kOpSdf | cnt);
             if (self_now) {
-                if (!self_batch || !self_batch->run(ctx, parity, s, substeps, count)) return false;
-                launch_vertex_pass(ctx, parity, parity, (pin_ops & kOpPinPolish) | last_op | kOpVelocity | cnt);
+                if (!self_batch || !self_batch->run(ctx, parity, sub, substeps, count)) return false;
             }
+            if (run_pcg && final_c) {
+                per_pcg([&](Solver& s, const ClothView* cv) { return s.pcg.final_caps(st, cv, s.d_tiles.p, parity, s.view.dt_sub); });
+            }
+            if (tail) launch_vertex_pass(ctx, parity, parity, (pin_ops & kOpPinPolish) | last_op | kOpVelocity | cnt);
         }
     }
     if (parity != 0) launch_vertex_pass(ctx, 1, 0, 0u);
@@ -759,16 +841,10 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
 static void reset_diag(Solver& s) {
     s.diag = SsblXpbdDiagnostics{};
     s.diag.finite_flag = 1;
-    s.diag.static_triangle_count = s.cfg.static_triangle_count;
     s.diag.force_field_count = static_cast<long long>(s.fields.size());
     s.diag.unsupported_force_field_count = s.unsupported_fields;
-    s.diag.static_sdf_rebuild_count = s.sdf_rebuilds;
-    s.diag.static_sdf_voxel_count = s.sdf_ready ? 1LL * s.sdf_dim[0] * s.sdf_dim[1] * s.sdf_dim[2] : 0;
-    s.diag.static_sdf_grid_x = s.sdf_dim[0];
-    s.diag.static_sdf_grid_y = s.sdf_dim[1];
-    s.diag.static_sdf_grid_z = s.sdf_dim[2];
-    s.diag.static_sdf_build_ms = s.sdf_build_ms;
-    s.diag.static_sdf_unsigned_fallback_count = s.sdf_unsigned;
+    s.diag.abi41_bending_texture_ready = s.wing_count > 0 ? 1 : 0;
+    s.populate_sdf_diag();
 }
 
 static bool fetch_counts(Solver& s) {  // A41:8059-8179
@@ -789,15 +865,16 @@ static bool fetch_counts(Solver& s) {  // A41:8059-8179
     d.static_sdf_contact_count = static_cast<long long>(c[kCountStaticSdf]);
     d.candidate_count = static_cast<long long>(c[kCountSelfCandidates] + c[kCountStaticSdf]);
     d.abi41_lra_tack_count = static_cast<long long>(c[kCountLraTacks]);
-    d.abi41_bending_wing_count = static_cast<long long>(c[kCountBendingWings]);
+    d.abi41_bending_wing_count = c[kCountBendingWings] > 0 ? static_cast<long long>(c[kCountBendingWings]) : s.wing_count;
     d.abi41_tack_jitter_guarded = static_cast<long long>(c[kCountTackGuards]) + (s.cfg.lra_count > 0 ? s.lra_invalid : 0);
     d.abi41_bending_guarded = static_cast<long long>(c[kCountBendingGuards]);
-    return true;
+    return s.pcg.fetch(&d);
 }
 
 static bool prepare_member(Solver& s, int substeps, int iterations, bool count) {
     if (!s.pins.empty() && !s.refresh_pins(iterations)) return false;
     s.fill_view(substeps);
+    if (s.pcg.enabled() && !s.pcg.reset_diagnostics(lib_stream())) return false;
     if (count) {
         launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.counts.p), 0u, 2 * static_cast<size_t>(kCountSlots));
         launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.max_smoothed.p), 0u, 1);
@@ -817,8 +894,8 @@ static bool step_one(Solver& s, int substeps, int iterations, int fetch, int for
         LaunchCtx ctx{lib_stream(), s.d_view.p, s.d_tiles.p, s.ntiles};
         std::vector<Solver*> one{&s};
         SelfBatch* sb = s.cfg.self_collision ? s.self.single_batch(s.view, s.d_view.p) : nullptr;
-        if (!run_schedule(one, ctx, &s.h_bend_segs, s.bend_segs.p, s.bend_seg_first, s.bend_seg_count, sb, substeps,
-                          iterations, count))
+        if (!run_schedule(one, ctx, s.d_view.p, s.bend_segs.p, s.bend_seg_first, s.bend_seg_count, sb, substeps, iterations,
+                          count))
             return false;
     }
     if (force_sync || fetch) {
@@ -891,7 +968,7 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
     LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles};
     SelfBatch* sb = b.self.get(b.members, b.h_views, b.d_views.p);
-    if (!run_schedule(b.members, ctx, &b.h_bend_segs, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false))
+    if (!run_schedule(b.members, ctx, b.d_views.p, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false))
         return false;
     return force_sync ? sync_stream() : true;
 }
@@ -1044,7 +1121,9 @@ SSBL_API int ssbl_get_diagnostics(void* handle, SsblXpbdDiagnostics* out) {
     return 1;
 }
 
-SSBL_API unsigned int ssbl_capabilities(void) { return SSBL_CAP_PIN_WEIGHTS | SSBL_CAP_HIP_BATCHED_STEP; }
+SSBL_API unsigned int ssbl_capabilities(void) {
+    return SSBL_CAP_STRETCH_OPTIMIZATION | SSBL_CAP_PIN_WEIGHTS | SSBL_CAP_HIP_BATCHED_STEP;
+}
 
 SSBL_API void* ssbl_create_global_dynamic_scene(void) { g_error.clear(); return new int(0); }
 SSBL_API int ssbl_destroy_global_dynamic_scene(void* scene) { g_error.clear(); delete static_cast<int*>(scene); return 1; }

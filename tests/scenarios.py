@@ -186,3 +186,53 @@ def folded_cloth(scenes, n=30, size=0.6, gap=0.004, seed=2):
                                 self_collision_mode=2, self_collision_distance=0.01 / 1.8, fast_self_collision_passes=1,
                                 max_self_collision_neighbors=32, gravity=(0.0, 0.0, 0.0))
     return cloth, opts
+
+
+# --- hard-stretch PCG
+def pcg_chain(lib, strength=0.4, start=None, inv_mass=(0.0, 1.0, 1.0), rests=(1.0, 1.0)):
+    """A pinned-start chain, stretched, one substep of one iteration (no gravity)."""
+    n = len(inv_mass)
+    rest_pos = np.zeros((n, 3), np.float32)
+    rest_pos[:, 0] = np.concatenate([[0.0], np.cumsum(rests)])
+    edges = [[k, k + 1] for k in range(n - 1)]
+    r = Raw(lib, rest_pos, list(inv_mass), edges=edges, edge_rest=list(rests), stretch_optimization_enabled=1,
+            stretch_optimization_strength=strength, stretch_compliance=1.0e6)
+    if start is not None:
+        r.set_positions(start)
+    r.step(1, 1)
+    return r.state(), r.diag(), r
+
+
+def pcg_exact_solution(start, inv_mass, rests, strength):
+    """Dense FP64 solve of the block system I + sum k n n^T (A41:3085-3226)."""
+    p = np.asarray(start, np.float64)
+    n = len(p)
+    s = min(max(strength * 2.5, 0.0), 1.0)
+    k = max(s, 1e-3)
+    A = np.eye(3 * n)
+    b = np.zeros(3 * n)
+    w = np.array(inv_mass, np.float64)
+    for e in range(n - 1):
+        i, j = e, e + 1
+        d = p[j] - p[i]
+        L = np.linalg.norm(d)
+        nrm = d / L
+        err = L - rests[e]
+        limit = max(1e-5, min(max(rests[e], 0.0) * 0.5, 0.25))
+        proj = max(-limit, min(err * s, limit))
+        ws = w[i] + w[j]
+        desired = nrm * proj / ws
+        blk = k * np.outer(nrm, nrm)
+        if w[i] > 0:
+            b[3 * i:3 * i + 3] += desired * w[i]
+            A[3 * i:3 * i + 3, 3 * i:3 * i + 3] += blk
+        if w[j] > 0:
+            b[3 * j:3 * j + 3] -= desired * w[j]
+            A[3 * j:3 * j + 3, 3 * j:3 * j + 3] += blk
+        if w[i] > 0 and w[j] > 0:
+            A[3 * i:3 * i + 3, 3 * j:3 * j + 3] -= blk
+            A[3 * j:3 * j + 3, 3 * i:3 * i + 3] -= blk
+    free = np.repeat(w > 0, 3)
+    x = np.zeros(3 * n)
+    x[free] = np.linalg.solve(A[np.ix_(free, free)], b[free])
+    return p + x.reshape(n, 3)
