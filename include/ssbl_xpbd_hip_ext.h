@@ -48,11 +48,17 @@ SSBL_API void* ssbl_ext_create_batch(void* const* handles, int count);
 SSBL_API int ssbl_ext_destroy_batch(void* batch);
 SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int force_sync);
 
+/* Runs `launches` (rounded up to even) back-to-back fused constraint passes of
+ * the batch (stretch gather, pins, analytic and SDF contacts) between two
+ * events on the library stream and writes their mean duration in ms.  Advances
+ * the members' state by that many Jacobi iterations: a measurement call. */
+SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms);
+
 /* Waits for all work the library queued. */
 SSBL_API int ssbl_ext_synchronize(void);
 
-/* Device-side kernel timing.  While enabled, every launch is bracketed by a
- * pair of device events on the stream it runs on.  ssbl_ext_kernel_stats
+/* Device-side kernel timing.  While enabled, every launch carries a start and
+ * a stop event stamped by its own dispatch, on the stream it runs on.  ssbl_ext_kernel_stats
  * writes a JSON object {"kernel": {"launches": n, "total_ms": t}, ...} and
  * resets the counters. */
 SSBL_API int ssbl_ext_kernel_timing(int enable);

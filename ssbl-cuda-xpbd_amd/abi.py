@@ -126,6 +126,7 @@ EXT_SIGNATURES = {
     "ssbl_ext_create_batch": (_vp, [ctypes.POINTER(_vp), _i]),
     "ssbl_ext_destroy_batch": (_i, [_vp]),
     "ssbl_ext_step_batch": (_i, [_vp, _i, _i, _i]),
+    "ssbl_ext_time_batch_pass": (_i, [_vp, _i, _fp]),
     "ssbl_ext_synchronize": (_i, []),
     "ssbl_ext_kernel_timing": (_i, [_i]),
     "ssbl_ext_kernel_stats": (_i, [ctypes.c_char_p, _i]),

@@ -10,6 +10,7 @@
 //   rest    float4 x V  rest positions; .w = inverse mass as uploaded
 //   adj     int2 x 2E   vertex -> incident stretch edges, ascending edge index:
 //                       {other vertex | first-endpoint bit 31, rest length bits}
+//   ell     int2 x W*Vp the first W <= 8 of those per vertex, slot-major
 // Per-vertex passes (integrate, the fused constraint/contact pass, velocity)
 // read one float4 per array per vertex -- 16-byte coalesced loads -- and gather
 // neighbour positions through L2.
@@ -27,6 +28,7 @@ constexpr int kCountSlots = 38;         // diagnostic counters per cloth
 constexpr int kNeighborSlots = 32;      // vertex-vertex candidates per vertex
 constexpr int kBucketSlots = 32;        // entries a query sees per hash bucket
 constexpr int kMaxForceFields = 64;
+constexpr int kEllMax = 8;              // incidence slots held slot-major per vertex
 constexpr double kFixedScale = 4294967296.0;  // 2^32: self-collision accumulators
 
 enum CountSlot {
@@ -106,6 +108,16 @@ struct ClothView {
     float sdf_voxel;
     int need_normal;
     int contacts_on;              // analytic colliders or a static SDF present
+    // The first ell_width incidences of every vertex again, slot-major
+    // (slot j of vertex v at ell[j * ell_pitch + v], {-1, 0} when unused), so
+    // a wave loads each slot with one coalesced access and without first
+    // reading adj_off.  ell_overflow: some vertex has more incidences, the
+    // rest are read from adj.
+    const int2* ell;
+    int ell_width;
+    int ell_pitch;
+    int ell_overflow;
+    int pad3;
 };
 
 struct Tile { int cloth, begin, end, pad; };
@@ -118,21 +130,11 @@ struct LaunchCtx {
     int ntiles;
 };
 
-void launch_integrate(const LaunchCtx& c, int parity, unsigned ops);
+void launch_integrate(const LaunchCtx& c, int parity, unsigned ops, bool full);
 void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops);
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops);
 void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float ox, float oy, float oz, float voxel,
                       const float* tris, int ntris, int signed_sdf);
 void launch_fill_u32(hipStream_t s, unsigned* p, unsigned value, size_t n);
-
-// Self collision (xpbd_self.hip).
-struct SelfHashDev {
-    int buckets;
-    int* counts;      // buckets
-    int* offsets;     // buckets + 1
-    int* cursor;      // buckets
-    int* entries;     // capacity
-    int capacity;
-};
 
 }  // namespace ssbl

@@ -52,35 +52,61 @@ __device__ inline void contact_response(const ClothView& c, F3 n, float proxy, F
     if (count && corrected) atomicAdd(&c.counts[kCountFriction], 1ull);
 }
 
+// Registers of one vertex during a pass.  The previous position and the
+// velocity are read from HBM only when a step of the pass needs them and
+// written back only when a step changed them: a constraint iteration with no
+// pin or contact on a vertex moves neither array.
+struct VertexState {
+    const ClothView* c;
+    int v;
+    F3 p;
+    F3 q_, vel_;
+    bool q_have, q_dirty, v_have, v_dirty;
+    __device__ F3 q() {
+        if (!q_have) { q_ = f3(c->prev[v]); q_have = true; }
+        return q_;
+    }
+    __device__ void set_q(F3 x) { q_ = x; q_have = true; q_dirty = true; }
+    __device__ F3 vel() {
+        if (!v_have) { vel_ = f3(c->vel[v]); v_have = true; }
+        return vel_;
+    }
+    __device__ void set_vel(F3 x) { vel_ = x; v_have = true; v_dirty = true; }
+};
+
 // A41:3466-3509: every pin of vertex v in ascending pin order.
-__device__ inline void apply_pins(const ClothView& c, int v, bool pinned, F3& p, F3& q, F3& vel, float dt) {
-    const int b = c.pin_off[v], e = c.pin_off[v + 1];
+__device__ inline void apply_pins(const ClothView& c, VertexState& s, bool pinned, float dt) {
+    const int b = c.pin_off[s.v], e = c.pin_off[s.v + 1];
     for (int k = b; k < e; ++k) {
         const float4 pin = c.pins[c.pin_ids[k]];
         const float relax = pin.w;
         if (relax < 0.0f) continue;                       // weight < 0.05 or not finite
         const F3 target = f3(pin.x, pin.y, pin.z);
         if (relax >= 2.0f || pinned) {                    // hard pin
-            p = target; q = target; vel = f3(0.0f, 0.0f, 0.0f);
+            s.p = target;
+            s.set_q(target);
+            s.set_vel(f3(0.0f, 0.0f, 0.0f));
             continue;
         }
         if (relax <= 0.0f) continue;
-        const F3 corrected = p + mul(target - p, relax);
-        const F3 damped = mul(vel, fmaxf(1.0f - relax, 0.0f));
-        p = corrected;
-        q = corrected - mul(damped, fmaxf(dt, kEps));
-        vel = damped;
+        const F3 corrected = s.p + mul(target - s.p, relax);
+        const F3 damped = mul(s.vel(), fmaxf(1.0f - relax, 0.0f));
+        s.p = corrected;
+        s.set_q(corrected - mul(damped, fmaxf(dt, kEps)));
+        s.set_vel(damped);
     }
 }
 
 // A41:3511-3590
-__device__ inline void analytic_contacts(const ClothView& c, F3& p, F3& q, F3& vel, float dt, bool count) {
+__device__ inline void analytic_contacts(const ClothView& c, VertexState& s, float dt, bool count) {
     const float margin = fmaxf(c.collision_margin, 0.0f);
     const float sdt = fmaxf(dt, kEps);
     const SsblXpbdRuntimeColliders& col = c.colliders;
-    F3 v = vel;
+    F3 p = s.p;
+    F3 v = f3(0.0f, 0.0f, 0.0f);
     bool hit = false;
     if (col.use_ground && p.z < col.ground_height + margin) {
+        v = s.vel();
         const float tz = col.ground_height + margin;
         const float pen = tz - p.z;
         const F3 n = f3(0.0f, 0.0f, 1.0f);
@@ -95,6 +121,7 @@ __device__ inline void analytic_contacts(const ClothView& c, F3& p, F3& q, F3& v
         const float dist = sqrtf(fmaxf(dot(d, d), kEps));
         const float target = col.sphere_radius + margin;
         if (dist < target) {
+            if (!hit) v = s.vel();
             const F3 n = dist > kEps ? mul(d, 1.0f / dist) : f3(0.0f, 0.0f, 1.0f);
             const float pen = target - dist;
             const float inward = fmaxf(-dot(v, n), 0.0f);
@@ -110,6 +137,7 @@ __device__ inline void analytic_contacts(const ClothView& c, F3& p, F3& q, F3& v
         n = mul(n, 1.0f / nl);
         const float gap = dot(p - o, n);
         if (gap < margin) {
+            if (!hit) v = s.vel();
             const float pen = margin - gap;
             const float inward = fmaxf(-dot(v, n), 0.0f);
             p = p + mul(n, margin - gap);
@@ -117,7 +145,11 @@ __device__ inline void analytic_contacts(const ClothView& c, F3& p, F3& q, F3& v
             hit = true;
         }
     }
-    if (hit) { vel = v; q = p - mul(v, sdt); }
+    if (hit) {
+        s.p = p;
+        s.set_vel(v);
+        s.set_q(p - mul(v, sdt));
+    }
 }
 
 // A41:1645-1691 trilinear sample; false outside the grid.
@@ -185,31 +217,31 @@ __device__ inline F3 sdf_normal(const ClothView& c, F3 p) {
 }
 
 // A41:1831-1867 (the caller skips pinned vertices).
-__device__ inline void sdf_contact(const ClothView& c, F3& p, F3& q, F3& vel, float dt, bool count) {
+__device__ inline void sdf_contact(const ClothView& c, VertexState& s, float dt, bool count) {
     const float margin = fmaxf(c.collision_margin, 0.0f);
     if (margin <= 0.0f) return;
     float d = 0.0f;
-    if (!sdf_sample(c, p, &d) || d >= margin) return;
-    const F3 n = sdf_normal(c, p);
+    if (!sdf_sample(c, s.p, &d) || d >= margin) return;
+    const F3 n = sdf_normal(c, s.p);
     if (!finite3(n)) return;
     const float pen = margin - d;
     if (!isfinite(pen) || pen <= 0.0f) return;
-    const F3 corrected = p + mul(n, pen);
-    F3 v = vel;
+    const F3 corrected = s.p + mul(n, pen);
+    F3 v = s.vel();
     const float inward = fmaxf(-dot(v, n), 0.0f);
     contact_response(c, n, inward + pen / fmaxf(dt, kEps), v, count);
-    p = corrected;
-    vel = v;
-    q = corrected - mul(v, fmaxf(dt, kEps));
+    s.p = corrected;
+    s.set_vel(v);
+    s.set_q(corrected - mul(v, fmaxf(dt, kEps)));
     if (count) atomicAdd(&c.counts[kCountStaticSdf], 1ull);
 }
 
 // A41:5846-5884
-__device__ inline void velocity_update(const ClothView& c, F3 p, F3 q, F3& vel, float dt) {
+__device__ inline void velocity_update(const ClothView& c, VertexState& s, float dt) {
     const float sdt = fmaxf(dt, kEps);
-    const F3 meas = mul(p - q, 1.0f / sdt);
-    const F3 base = vel;
-    if (!finite3(meas) || !finite3(base)) { vel = f3(0.0f, 0.0f, 0.0f); return; }
+    const F3 meas = mul(s.p - s.q(), 1.0f / sdt);
+    const F3 base = s.vel();
+    if (!finite3(meas) || !finite3(base)) { s.set_vel(f3(0.0f, 0.0f, 0.0f)); return; }
     const F3 pv = meas - base;
     const float pdb = dot(pv, base);
     const float fs = (isfinite(pdb) && pdb > 0.0f) ? 0.9f : 1.0f;
@@ -219,7 +251,7 @@ __device__ inline void velocity_update(const ClothView& c, F3 p, F3 q, F3& vel, 
     const float speed = len3(fb);
     if (fs < 1.0f && isfinite(speed) && speed > max_fs && speed > kEps) fb = mul(fb, max_fs / speed);
     if (!finite3(fb)) fb = f3(0.0f, 0.0f, 0.0f);
-    vel = base + fb;
+    s.set_vel(base + fb);
 }
 
 // ---------------------------------------------------------- force fields
@@ -407,6 +439,10 @@ __device__ inline F3 vertex_normal(const ClothView& c, const float4* src, int v,
     return mul(acc, rsqrt_exact(asq));
 }
 
+// kFull: normals, overpressure and force fields are compiled in (the plain
+// variant keeps the register footprint, and so the occupancy, of the common
+// gravity-only case).
+template <bool kFull>
 __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
                                                      int ntiles, int parity, unsigned ops) {
     const Tile tile = tiles[xcd_tile(ntiles)];
@@ -418,55 +454,243 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
     const float dt = c.dt_sub;
     const float4 p4 = src[v];
     const bool pinned = p4.w == 0.0f;
-    F3 p = f3(p4), q = p, vel;
+    // the previous position becomes the pass's source position
+    VertexState s{&c, v, f3(p4), f3(p4), f3(0.0f, 0.0f, 0.0f), true, true, true, true};
     if (pinned) {
-        vel = f3(0.0f, 0.0f, 0.0f);
-        p = f3(c.rest[v]);
+        s.p = f3(c.rest[v]);
     } else {
-        vel = f3(c.vel[v]);
-        bool has = false;
-        const F3 n = c.need_normal ? vertex_normal(c, src, v, &has) : f3(0.0f, 0.0f, 0.0f);
+        const F3 vel = f3(c.vel[v]);
         F3 acc = f3(c.gravity[0], c.gravity[1], c.gravity[2]);
-        const float im = p4.w;
-        const bool pressure = c.use_pressure && has;
-        if (pressure) {
-            const float pf = fmaxf(c.pressure_strength, 0.0f);
-            const float share = c.area_share[v];
-            if (isfinite(share) && share > 0.0f) acc = acc + mul(n, pf * c.pressure_accel_scale * share * im);
-        }
-        if (c.nfield > 0) {
-            F3 fa = f3(0.0f, 0.0f, 0.0f);
-            const int cnt = min(c.nfield, kMaxForceFields);
-            for (int k = 0; k < cnt; ++k) fa = fa + eval_field(c.fields[k], p, vel, n, has);
-            acc = acc + limit_accel(fa);
-        }
-        F3 nv = vel + mul(acc, dt);
-        nv = mul(nv, c.damping);
-        if (pressure) {
-            const float local_rest = c.rest_scale[v];
-            if (isfinite(local_rest) && local_rest > kEps && isfinite(dt) && dt > kEps) {
-                const float pa = fmaxf(c.pressure_strength, 0.0f) * fmaxf(c.pressure_accel_scale, 0.0f) *
-                                 fmaxf(c.area_share[v], 0.0f) * fmaxf(im, 0.0f);
-                const float max_pd = fmaxf(local_rest * fmaxf(c.pressure_impulse_rest_scale, 0.0f), 1.0e-7f);
-                const float pd = pa * dt * dt;
-                if (isfinite(pd) && pd > max_pd) nv = nv - mul(n, (pd - max_pd) / dt);
-                const float vr = fmaxf(local_rest, fmaxf(c.pressure_velocity_length_scale, 0.0f));
-                const float max_out = fmaxf(vr * fmaxf(c.pressure_velocity_rest_scale, 0.0f) / dt, 1.0e-6f);
-                const float out = dot(nv, n);
-                if (isfinite(out) && out > max_out) nv = nv - mul(n, out - max_out);
+        F3 nv;
+        if constexpr (kFull) {
+            const F3 p = s.p;
+            bool has = false;
+            const F3 n = c.need_normal ? vertex_normal(c, src, v, &has) : f3(0.0f, 0.0f, 0.0f);
+            const float im = p4.w;
+            const bool pressure = c.use_pressure && has;
+            if (pressure) {
+                const float pf = fmaxf(c.pressure_strength, 0.0f);
+                const float share = c.area_share[v];
+                if (isfinite(share) && share > 0.0f) acc = acc + mul(n, pf * c.pressure_accel_scale * share * im);
             }
+            if (c.nfield > 0) {
+                F3 fa = f3(0.0f, 0.0f, 0.0f);
+                const int cnt = min(c.nfield, kMaxForceFields);
+                for (int k = 0; k < cnt; ++k) fa = fa + eval_field(c.fields[k], p, vel, n, has);
+                acc = acc + limit_accel(fa);
+            }
+            nv = vel + mul(acc, dt);
+            nv = mul(nv, c.damping);
+            if (pressure) {
+                const float local_rest = c.rest_scale[v];
+                if (isfinite(local_rest) && local_rest > kEps && isfinite(dt) && dt > kEps) {
+                    const float pa = fmaxf(c.pressure_strength, 0.0f) * fmaxf(c.pressure_accel_scale, 0.0f) *
+                                     fmaxf(c.area_share[v], 0.0f) * fmaxf(im, 0.0f);
+                    const float max_pd = fmaxf(local_rest * fmaxf(c.pressure_impulse_rest_scale, 0.0f), 1.0e-7f);
+                    const float pd = pa * dt * dt;
+                    if (isfinite(pd) && pd > max_pd) nv = nv - mul(n, (pd - max_pd) / dt);
+                    const float vr = fmaxf(local_rest, fmaxf(c.pressure_velocity_length_scale, 0.0f));
+                    const float max_out = fmaxf(vr * fmaxf(c.pressure_velocity_rest_scale, 0.0f) / dt, 1.0e-6f);
+                    const float out = dot(nv, n);
+                    if (isfinite(out) && out > max_out) nv = nv - mul(n, out - max_out);
+                }
+            }
+        } else {
+            nv = vel + mul(acc, dt);
+            nv = mul(nv, c.damping);
         }
-        vel = nv;
-        p = p + mul(nv, dt);
+        s.vel_ = nv;
+        s.p = s.p + mul(nv, dt);
     }
-    if (ops & kOpPin) apply_pins(c, v, pinned, p, q, vel, dt);
-    dst[v] = with_w(p, p4.w);
-    c.prev[v] = with_w(q, 0.0f);
-    c.vel[v] = with_w(vel, 0.0f);
+    if (ops & kOpPin) apply_pins(c, s, pinned, dt);
+    dst[v] = with_w(s.p, p4.w);
+    c.prev[v] = with_w(s.q_, 0.0f);
+    c.vel[v] = with_w(s.vel_, 0.0f);
 }
 
 // ---------------------------------------------------------- vertex pass
-__global__ __launch_bounds__(kTile) void k_vertex_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
+// Neighbour positions loaded together before any of their edges is
+// evaluated: one round of memory latency per chunk instead of one per edge.
+#ifndef SSBL_GATHER_CHUNK
+#define SSBL_GATHER_CHUNK 4
+#endif
+constexpr int kGatherChunk = SSBL_GATHER_CHUNK;
+static_assert(kEllMax % kGatherChunk == 0, "the gather chunk must divide the slot count");
+
+// Element i of a wave-uniform array, addressed as base + 32-bit byte offset
+// (a scalar base register and one vector register per load in flight).
+template <typename T>
+__device__ __forceinline__ T load_at(const T* __restrict__ base, unsigned i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<unsigned>(sizeof(T)));
+}
+
+// A41:1927-1980, one edge as seen from vertex s.v: the edge thread's
+// correction, with the difference vector negated for the second endpoint
+// (which flips the correction's sign bit-exactly).
+__device__ __forceinline__ void stretch_edge(int2 a, float4 o4, F3 p0, float w, float alpha, F3& p, F3& q, bool& q_moved,
+                                             unsigned& clamps) {
+    const float restl = __int_as_float(a.y);
+    const float wsum = w + o4.w;
+    if (wsum <= 0.0f) return;
+    const F3 delta = f3(o4) - p0;
+    const float len = sqrtf(fmaxf(dot(delta, delta), kEps));
+    const float cval = len - restl;
+    const float dl = -cval / (wsum + alpha);
+    F3 corr = mul(delta, 0.18f * dl / fmaxf(len, kEps));
+    const float cl = sqrtf(fmaxf(dot(corr, corr), 0.0f));
+    const float mc = fmaxf(0.0025f, fminf(fmaxf(restl, 0.0f) * 0.10f, 0.025f));
+    if (cl > mc) {
+        corr = mul(corr, mc / fmaxf(cl, kEps));
+        if (a.x < 0) ++clamps;
+    }
+    if (w > 0.0f) {
+        const F3 d = mul(corr, -w);
+        p = p + d;
+        if (cval > 0.0f) { q = q + mul(d, 0.08f); q_moved = true; }
+    }
+}
+
+// Direct stretch gathered per vertex, incident edges in ascending edge order:
+// the slot-major first ell_width incidences, then any overflow from adj.
+__device__ inline void stretch_gather(const ClothView& c, const float4* __restrict__ src, VertexState& s, float w,
+                                      bool count) {
+    const int v = s.v;
+    const int width = c.ell_width;
+    const float alpha = c.stretch_alpha;
+    int2 a[kEllMax];
+#pragma unroll
+    for (int j = 0; j < kEllMax; ++j)
+        a[j] = j < width ? load_at(c.ell, static_cast<unsigned>(j * c.ell_pitch + v)) : make_int2(-1, 0);
+    const F3 p0 = s.p;
+    F3 p = p0;
+    F3 q = w > 0.0f ? s.q() : p0;
+    bool q_moved = false;
+    unsigned clamps = 0;
+#pragma unroll
+    for (int j0 = 0; j0 < kEllMax; j0 += kGatherChunk) {
+        if (j0 >= width) break;
+        float4 o[kGatherChunk];
+#pragma unroll
+        for (int j = 0; j < kGatherChunk; ++j) {
+            const int x = a[j0 + j].x;
+            o[j] = load_at(src, static_cast<unsigned>(x == -1 ? v : (x & 0x7fffffff)));
+        }
+#pragma unroll
+        for (int j = 0; j < kGatherChunk; ++j)
+            if (a[j0 + j].x != -1) stretch_edge(a[j0 + j], o[j], p0, w, alpha, p, q, q_moved, clamps);
+    }
+    if (c.ell_overflow) {
+        const int e = c.adj_off[v + 1];
+        for (int k = c.adj_off[v] + width; k < e; ++k) {
+            const int2 ak = c.adj[k];
+            stretch_edge(ak, src[ak.x & 0x7fffffff], p0, w, alpha, p, q, q_moved, clamps);
+        }
+    }
+    s.p = p;
+    if (q_moved) s.set_q(q);
+    if (count && clamps) atomicAdd(&c.counts[kCountHardFallbacks], static_cast<unsigned long long>(clamps));
+}
+
+// A41:2710-2803 gathered the same way (tethers are one-sided: only a
+// stretched tether acts).
+__device__ inline void lra_gather(const ClothView& c, const float4* __restrict__ src, VertexState& s, float w, float dt,
+                                  bool count) {
+    const int b = c.lra_off[s.v], e = c.lra_off[s.v + 1];
+    const F3 p0 = s.p;
+    unsigned tacks = 0, guards = 0;
+    for (int k = b; k < e; ++k) {
+        const int4 a = c.lra_adj[k];
+        const int other = a.x & 0x7fffffff;
+        const bool first = a.x < 0;
+        const float restl = __int_as_float(a.y);
+        const float4 o4 = src[other];
+        const float wsum = w + o4.w;
+        if (!isfinite(wsum) || wsum <= 0.0f) { guards += first; continue; }
+        const F3 po = f3(o4);
+        if (!finite3(p0) || !finite3(po)) { guards += first; continue; }
+        F3 dx = po - p0;
+        const float lsq = __builtin_fmaf(dx.x, dx.x, __builtin_fmaf(dx.y, dx.y, dx.z * dx.z));
+        if (!isfinite(lsq) || lsq <= 1.0e-10f || !isfinite(restl) || restl <= kEps) { guards += first; continue; }
+        const float inv_len = rsqrt_exact(lsq);
+        const float len = lsq * inv_len;
+        const float stretch = len - restl;
+        if (!isfinite(stretch) || stretch <= 0.0f) continue;
+        dx = mul(dx, inv_len);
+        const float bc = fmaxf(c.lra_compliance, 1.0e-12f);
+        const float stiff = (1.0f / bc) * 500000.0f;
+        const float ealpha = (bc / 500000.0f) / fmaxf(dt * dt, kEps);
+        const float sw = stiff / fmaxf(stiff + 500000.0f, 1.0f);
+        const float dl = -stretch / fmaxf(wsum + ealpha, kEps);
+        F3 corr = mul(dx, 0.35f * sw * dl);
+        if (!finite3(corr)) { guards += first; continue; }
+        const float cl = len3(corr);
+        const float mc = fmaxf(5.0e-4f, fminf(fmaxf(restl, c.thickness) * 0.20f, 0.035f));
+        if (!isfinite(cl) || cl <= 0.0f) continue;
+        if (cl > mc) { corr = mul(corr, mc / fmaxf(cl, kEps)); guards += first; }
+        const bool th = c.thickness > 0.0f;
+        const float cap = th ? 0.0025f : 0.012f;
+        const float rl = fmaxf(restl, 0.0f) * (th ? 0.02f : 0.08f);
+        const float tl = th ? c.thickness * 0.12f : cap;
+        const float al = fmaxf(5.0e-4f, fminf(fminf(rl, tl), cap));
+        if (w > 0.0f) {
+            const F3 d = limit_delta(mul(corr, -w / fmaxf(wsum, kEps)), al);
+            s.p = s.p + d;
+            s.set_q(s.q() + mul(d, 0.32f));
+        }
+        tacks += first;
+    }
+    if (count && tacks) atomicAdd(&c.counts[kCountLraTacks], static_cast<unsigned long long>(tacks));
+    if (count && guards) atomicAdd(&c.counts[kCountTackGuards], static_cast<unsigned long long>(guards));
+}
+
+// A41:2805-2884 (uncoloured bends) gathered like the stretch pass.
+__device__ inline void bend_gather(const ClothView& c, const float4* __restrict__ src, VertexState& s, float w,
+                                   bool count) {
+    const int b = c.bend_off[s.v], e = c.bend_off[s.v + 1];
+    const F3 p0 = s.p;
+    unsigned wings = 0, guards = 0;
+    for (int k = b; k < e; ++k) {
+        const int4 a = c.bend_adj[k];
+        const int other = a.x & 0x7fffffff;
+        const bool first = a.x < 0;
+        const float restl = __int_as_float(a.y);
+        const float4 o4 = src[other];
+        const float wsum = w + o4.w;
+        if (!isfinite(wsum) || wsum <= 0.0f) continue;
+        if (!isfinite(restl) || restl <= kEps) { guards += first; continue; }
+        const F3 po = f3(o4);
+        if (!finite3(p0) || !finite3(po)) { guards += first; continue; }
+        const F3 delta = po - p0;
+        const float lsq = __builtin_fmaf(delta.x, delta.x, __builtin_fmaf(delta.y, delta.y, delta.z * delta.z));
+        if (!isfinite(lsq) || lsq <= 1.0e-10f) { guards += first; continue; }
+        const float inv_len = rsqrt_exact(lsq);
+        const float len = lsq * inv_len;
+        const float cval = len - restl;
+        if (!isfinite(cval) || fabsf(cval) <= 1.0e-6f) continue;
+        const float dl = -cval / fmaxf(wsum + c.bend_alpha, kEps);
+        F3 corr = mul(delta, 0.10f * dl * inv_len);
+        if (!finite3(corr)) { guards += first; continue; }
+        const float cl = len3(corr);
+        const float mc = fmaxf(2.5e-4f, fminf(fmaxf(restl, c.thickness) * 0.08f, 0.020f));
+        if (!isfinite(cl) || cl <= 0.0f) continue;
+        if (cl > mc) { corr = mul(corr, mc / fmaxf(cl, kEps)); guards += first; }
+        if (w > 0.0f) {
+            const F3 d = mul(corr, -w);
+            s.p = s.p + d;
+            s.set_q(s.q() + mul(d, 0.06f));
+        }
+        wings += first;
+    }
+    if (count && wings) atomicAdd(&c.counts[kCountBendingWings], static_cast<unsigned long long>(wings));
+    if (count && guards) atomicAdd(&c.counts[kCountBendingGuards], static_cast<unsigned long long>(guards));
+}
+
+enum GatherKind { kGatherNone = 0, kGatherStretch = 1, kGatherLra = 2, kGatherBend = 3 };
+
+// One solver iteration for every vertex of the tile: the constraint gather
+// selected at compile time, then the vertex-local steps selected by `ops`.
+template <int kGather>
+__global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
                                                        int ntiles, int src_parity, int dst_parity, unsigned ops) {
     const Tile tile = tiles[xcd_tile(ntiles)];
     const int v = tile.begin + static_cast<int>(threadIdx.x);
@@ -481,143 +705,27 @@ __global__ __launch_bounds__(kTile) void k_vertex_pass(const ClothView* __restri
     // .w holds 0 exactly for the vertices the reference flags as pinned
     // (inverse mass <= 0, A41:8861-8865)
     const bool pinned = w == 0.0f;
-    F3 p = f3(p4), q = f3(c.prev[v]), vel = f3(c.vel[v]);
-    if ((ops & kOpStretch) && c.direct_stretch) {
-        // A41:1927-1980 gathered: each incident edge is evaluated exactly as
-        // the edge thread would (the difference vector is negated for the
-        // second endpoint, which flips the correction's sign bit-exactly).
-        const float alpha = c.stretch_alpha;
-        const int b = c.adj_off[v], e = c.adj_off[v + 1];
-        const F3 p0 = p;
-        unsigned clamps = 0;
-        for (int k = b; k < e; ++k) {
-            const int2 a = c.adj[k];
-            const int other = a.x & 0x7fffffff;
-            const float restl = __int_as_float(a.y);
-            const float4 o4 = src[other];
-            const float wsum = w + o4.w;
-            if (wsum <= 0.0f) continue;
-            const F3 delta = f3(o4) - p0;
-            const float len = sqrtf(fmaxf(dot(delta, delta), kEps));
-            const float cval = len - restl;
-            const float dl = -cval / (wsum + alpha);
-            F3 corr = mul(delta, 0.18f * dl / fmaxf(len, kEps));
-            const float cl = sqrtf(fmaxf(dot(corr, corr), 0.0f));
-            const float mc = fmaxf(0.0025f, fminf(fmaxf(restl, 0.0f) * 0.10f, 0.025f));
-            if (cl > mc) {
-                corr = mul(corr, mc / fmaxf(cl, kEps));
-                if (a.x < 0) ++clamps;
-            }
-            if (w > 0.0f) {
-                const F3 d = mul(corr, -w);
-                p = p + d;
-                if (cval > 0.0f) q = q + mul(d, 0.08f);
-            }
-        }
-        if (count && clamps) atomicAdd(&c.counts[kCountHardFallbacks], static_cast<unsigned long long>(clamps));
-    } else if (ops & kOpLra) {
-        // A41:2710-2803 gathered the same way (tethers are one-sided: only a
-        // stretched tether acts).
-        const int b = c.lra_off[v], e = c.lra_off[v + 1];
-        const F3 p0 = p;
-        unsigned tacks = 0, guards = 0;
-        for (int k = b; k < e; ++k) {
-            const int4 a = c.lra_adj[k];
-            const int other = a.x & 0x7fffffff;
-            const bool first = a.x < 0;
-            const float restl = __int_as_float(a.y);
-            const float4 o4 = src[other];
-            const float wsum = w + o4.w;
-            if (!isfinite(wsum) || wsum <= 0.0f) { guards += first; continue; }
-            const F3 po = f3(o4);
-            if (!finite3(p0) || !finite3(po)) { guards += first; continue; }
-            F3 dx = po - p0;
-            const float lsq = __builtin_fmaf(dx.x, dx.x, __builtin_fmaf(dx.y, dx.y, dx.z * dx.z));
-            if (!isfinite(lsq) || lsq <= 1.0e-10f || !isfinite(restl) || restl <= kEps) { guards += first; continue; }
-            const float inv_len = rsqrt_exact(lsq);
-            const float len = lsq * inv_len;
-            const float stretch = len - restl;
-            if (!isfinite(stretch) || stretch <= 0.0f) continue;
-            dx = mul(dx, inv_len);
-            const float bc = fmaxf(c.lra_compliance, 1.0e-12f);
-            const float stiff = (1.0f / bc) * 500000.0f;
-            const float ealpha = (bc / 500000.0f) / fmaxf(dt * dt, kEps);
-            const float sw = stiff / fmaxf(stiff + 500000.0f, 1.0f);
-            const float dl = -stretch / fmaxf(wsum + ealpha, kEps);
-            F3 corr = mul(dx, 0.35f * sw * dl);
-            if (!finite3(corr)) { guards += first; continue; }
-            const float cl = len3(corr);
-            const float mc = fmaxf(5.0e-4f, fminf(fmaxf(restl, c.thickness) * 0.20f, 0.035f));
-            if (!isfinite(cl) || cl <= 0.0f) continue;
-            if (cl > mc) { corr = mul(corr, mc / fmaxf(cl, kEps)); guards += first; }
-            const bool th = c.thickness > 0.0f;
-            const float cap = th ? 0.0025f : 0.012f;
-            const float rl = fmaxf(restl, 0.0f) * (th ? 0.02f : 0.08f);
-            const float tl = th ? c.thickness * 0.12f : cap;
-            const float al = fmaxf(5.0e-4f, fminf(fminf(rl, tl), cap));
-            if (w > 0.0f) {
-                const F3 d = limit_delta(mul(corr, -w / fmaxf(wsum, kEps)), al);
-                p = p + d;
-                q = q + mul(d, 0.32f);
-            }
-            tacks += first;
-        }
-        if (count && tacks) atomicAdd(&c.counts[kCountLraTacks], static_cast<unsigned long long>(tacks));
-        if (count && guards) atomicAdd(&c.counts[kCountTackGuards], static_cast<unsigned long long>(guards));
+    VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
+    if constexpr (kGather == kGatherStretch) {
+        if (c.direct_stretch) stretch_gather(c, src, s, w, count);
+    } else if constexpr (kGather == kGatherLra) {
+        lra_gather(c, src, s, w, dt, count);
+    } else if constexpr (kGather == kGatherBend) {
+        bend_gather(c, src, s, w, count);
     }
-    if (ops & kOpBend) {
-        // A41:2805-2884 (uncoloured bends) gathered like the stretch pass
-        const int b = c.bend_off[v], e = c.bend_off[v + 1];
-        const F3 p0 = p;
-        unsigned wings = 0, guards = 0;
-        for (int k = b; k < e; ++k) {
-            const int4 a = c.bend_adj[k];
-            const int other = a.x & 0x7fffffff;
-            const bool first = a.x < 0;
-            const float restl = __int_as_float(a.y);
-            const float4 o4 = src[other];
-            const float wsum = w + o4.w;
-            if (!isfinite(wsum) || wsum <= 0.0f) continue;
-            if (!isfinite(restl) || restl <= kEps) { guards += first; continue; }
-            const F3 po = f3(o4);
-            if (!finite3(p0) || !finite3(po)) { guards += first; continue; }
-            const F3 delta = po - p0;
-            const float lsq = __builtin_fmaf(delta.x, delta.x, __builtin_fmaf(delta.y, delta.y, delta.z * delta.z));
-            if (!isfinite(lsq) || lsq <= 1.0e-10f) { guards += first; continue; }
-            const float inv_len = rsqrt_exact(lsq);
-            const float len = lsq * inv_len;
-            const float cval = len - restl;
-            if (!isfinite(cval) || fabsf(cval) <= 1.0e-6f) continue;
-            const float dl = -cval / fmaxf(wsum + c.bend_alpha, kEps);
-            F3 corr = mul(delta, 0.10f * dl * inv_len);
-            if (!finite3(corr)) { guards += first; continue; }
-            const float cl = len3(corr);
-            const float mc = fmaxf(2.5e-4f, fminf(fmaxf(restl, c.thickness) * 0.08f, 0.020f));
-            if (!isfinite(cl) || cl <= 0.0f) continue;
-            if (cl > mc) { corr = mul(corr, mc / fmaxf(cl, kEps)); guards += first; }
-            if (w > 0.0f) {
-                const F3 d = mul(corr, -w);
-                p = p + d;
-                q = q + mul(d, 0.06f);
-            }
-            wings += first;
-        }
-        if (count && wings) atomicAdd(&c.counts[kCountBendingWings], static_cast<unsigned long long>(wings));
-        if (count && guards) atomicAdd(&c.counts[kCountBendingGuards], static_cast<unsigned long long>(guards));
-    }
-    if (ops & kOpPin) apply_pins(c, v, pinned, p, q, vel, dt);
-    if ((ops & kOpAnalytic) && !pinned) analytic_contacts(c, p, q, vel, dt, count);
-    if ((ops & kOpSdf) && c.sdf_ready && !pinned) sdf_contact(c, p, q, vel, dt, count);
+    if (ops & kOpPin) apply_pins(c, s, pinned, dt);
+    if ((ops & kOpAnalytic) && !pinned) analytic_contacts(c, s, dt, count);
+    if ((ops & kOpSdf) && c.sdf_ready && !pinned) sdf_contact(c, s, dt, count);
     // A41:9360-9369: the second pin projection runs after any contact work,
     // and on every last iteration (dynamic collision is scheduled there).
-    if ((ops & kOpPinPolish) && ((ops & kOpLastIter) || c.contacts_on)) apply_pins(c, v, pinned, p, q, vel, dt);
+    if ((ops & kOpPinPolish) && ((ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt);
     if (ops & kOpVelocity) {
-        if (pinned) vel = f3(0.0f, 0.0f, 0.0f);
-        else velocity_update(c, p, q, vel, dt);
+        if (pinned) s.set_vel(f3(0.0f, 0.0f, 0.0f));
+        else velocity_update(c, s, dt);
     }
-    dst[v] = with_w(p, w);
-    c.prev[v] = with_w(q, 0.0f);
-    c.vel[v] = with_w(vel, 0.0f);
+    dst[v] = with_w(s.p, w);
+    if (s.q_dirty) c.prev[v] = with_w(s.q_, 0.0f);
+    if (s.v_dirty) c.vel[v] = with_w(s.vel_, 0.0f);
 }
 
 // -------------------------------------------------------------- bending
@@ -733,9 +841,12 @@ __global__ void k_fill_u32(unsigned* p, unsigned value, size_t n) {
 }
 
 // ------------------------------------------------------------ launches
-void launch_integrate(const LaunchCtx& c, int parity, unsigned ops) {
+void launch_integrate(const LaunchCtx& c, int parity, unsigned ops, bool full) {
     TimedLaunch t("integrate", c.stream);
-    hipLaunchKernelGGL(k_integrate, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
+    if (full)
+        launch_k(k_integrate<true>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
+    else
+        launch_k(k_integrate<false>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
 }
 
 void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops) {
@@ -743,14 +854,21 @@ void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsi
                        : (ops & kOpLra) ? "vertex_pass_lra"
                        : (ops & kOpBend) ? "vertex_pass_bend" : "vertex_pass_local";
     TimedLaunch t(name, c.stream);
-    hipLaunchKernelGGL(k_vertex_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, src_parity,
-                       dst_parity, ops);
+    const dim3 grid(c.ntiles), block(kTile);
+    if (ops & kOpStretch)
+        launch_k(k_vertex_pass<kGatherStretch>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+    else if (ops & kOpLra)
+        launch_k(k_vertex_pass<kGatherLra>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+    else if (ops & kOpBend)
+        launch_k(k_vertex_pass<kGatherBend>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+    else
+        launch_k(k_vertex_pass<kGatherNone>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
 }
 
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops) {
     if (nseg_blocks <= 0) return;
     TimedLaunch t("bend_color", c.stream);
-    hipLaunchKernelGGL(k_bend_color, dim3(nseg_blocks), dim3(kTile), 0, c.stream, c.views, segments, parity, ops);
+    launch_k(k_bend_color, dim3(nseg_blocks), dim3(kTile), 0, c.stream, c.views, segments, parity, ops);
 }
 
 void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float ox, float oy, float oz, float voxel,
@@ -758,13 +876,13 @@ void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float o
     const long long total = static_cast<long long>(dx) * dy * dz;
     const unsigned blocks = static_cast<unsigned>((total + kTile - 1) / kTile);
     TimedLaunch t("sdf_build", s);
-    hipLaunchKernelGGL(k_sdf_build, dim3(blocks), dim3(kTile), 0, s, sdf, dx, dy, dz, ox, oy, oz, voxel, tris, ntris, signed_sdf);
+    launch_k(k_sdf_build, dim3(blocks), dim3(kTile), 0, s, sdf, dx, dy, dz, ox, oy, oz, voxel, tris, ntris, signed_sdf);
 }
 
 void launch_fill_u32(hipStream_t s, unsigned* p, unsigned value, size_t n) {
     if (n == 0) return;
     TimedLaunch t("fill", s);
-    hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, p, value, n);
+    launch_k(k_fill_u32, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, p, value, n);
 }
 
 }  // namespace ssbl

@@ -536,22 +536,22 @@ bool PcgPath::reset_diagnostics(hipStream_t st) {
 void PcgPath::solve(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity) {
     TimedLaunch t("pcg", st);
     const dim3 grid(ntiles_), block(kTile);
-    hipLaunchKernelGGL(k_pcg_build, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
-    hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, pv_, ntiles_, 0, 0);
+    launch_k(k_pcg_build, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
+    launch_k(k_pcg_reduce, dim3(1), dim3(1024), 0, st, pv_, ntiles_, 0, 0);
     for (int it = 0; it < kPcgIterations; ++it) {
         const int more = it + 1 < kPcgIterations ? 1 : 0;
-        hipLaunchKernelGGL(k_pcg_ad, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
-        hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, pv_, ntiles_, 1, more);
-        hipLaunchKernelGGL(k_pcg_update, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
-        hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, pv_, ntiles_, 2, more);
-        if (more) hipLaunchKernelGGL(k_pcg_dir, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
+        launch_k(k_pcg_ad, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
+        launch_k(k_pcg_reduce, dim3(1), dim3(1024), 0, st, pv_, ntiles_, 1, more);
+        launch_k(k_pcg_update, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
+        launch_k(k_pcg_reduce, dim3(1), dim3(1024), 0, st, pv_, ntiles_, 2, more);
+        if (more) launch_k(k_pcg_dir, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
     }
 }
 
 // Returns the new parity (gathers flip it).
 int PcgPath::gather(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity, int kind, float dt) {
     TimedLaunch t("edge_gather", st);
-    hipLaunchKernelGGL(k_edge_gather, dim3(ntiles_), dim3(kTile), 0, st, cv, tiles, ntiles_, parity, kind, dt, pv_);
+    launch_k(k_edge_gather, dim3(ntiles_), dim3(kTile), 0, st, cv, tiles, ntiles_, parity, kind, dt, pv_);
     return parity ^ 1;
 }
 
@@ -562,11 +562,11 @@ int PcgPath::caps(hipStream_t st, const ClothView* cv, const Tile* tiles, int pa
         const int ncol = static_cast<int>(seg_first_.size());
         for (int c = 0; c < ncol; ++c)
             if (seg_count_[c] > 0)
-                hipLaunchKernelGGL(k_edge_color, dim3(seg_count_[c]), dim3(kTile), 0, st, cv, segs_ + seg_first_[c], parity, kind, dt, pv_);
+                launch_k(k_edge_color, dim3(seg_count_[c]), dim3(kTile), 0, st, cv, segs_ + seg_first_[c], parity, kind, dt, pv_);
         if (reverse_too)
             for (int c = ncol - 1; c >= 0; --c)
                 if (seg_count_[c] > 0)
-                    hipLaunchKernelGGL(k_edge_color, dim3(seg_count_[c]), dim3(kTile), 0, st, cv, segs_ + seg_first_[c], parity, kind, dt, pv_);
+                    launch_k(k_edge_color, dim3(seg_count_[c]), dim3(kTile), 0, st, cv, segs_ + seg_first_[c], parity, kind, dt, pv_);
     }
     return parity;
 }

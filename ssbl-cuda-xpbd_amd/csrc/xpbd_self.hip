@@ -10,9 +10,9 @@
 //
 // The reference's buckets are fixed 32-slot arrays filled by atomics, so their
 // order (and which entries overflow) depends on scheduling.  Here a bucket is
-// a counting-sorted segment whose first 32 entries are put in ascending key
-// order -- the order a query visits -- so a build is deterministic and a query
-// sees exactly the entries the CPU oracle sees.  Responses accumulate into
+// a counting-sorted segment whose first 32 entries are placed by rank in
+// ascending key order -- the order a query visits -- so a build is
+// deterministic and a query sees exactly the entries the CPU oracle sees.  Responses accumulate into
 // per-vertex 64-bit fixed-point sums (2^-32 m), which makes the sum independent
 // of the order the atomics land in.
 #include "xpbd_self.hpp"
@@ -123,9 +123,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int* __restrict
 void exclusive_scan(hipStream_t s, const int* in, int n, int* out, int* tmp) {
     const int nb = (n + kScanChunk - 1) / kScanChunk;
     TimedLaunch t("self_hash_scan", s);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp);
-    hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, nb);
-    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp, out);
+    launch_k(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp);
+    launch_k(k_scan_block_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, nb);
+    launch_k(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp, out);
 }
 
 // ------------------------------------------------------------ hash builds
@@ -136,23 +136,45 @@ __global__ void k_fill_i32(int* p, int value, int n) {
     if (i < n) p[i] = value;
 }
 
-// A41:4550-4575: vertex cell and bucket.
-__global__ __launch_bounds__(256) void k_vhash_count(const ClothView* __restrict__ cv, SelfView sv, int parity) {
-    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= sv.nv) return;
-    const float4 p = cur_pos(*cv, parity)[i];
-    const int cx = cell_coord(p.x, sv.cell), cy = cell_coord(p.y, sv.cell), cz = cell_coord(p.z, sv.cell);
-    const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.vh.buckets));
-    sv.vcell[i] = make_int4(cx, cy, cz, b);
-    atomicAdd(&sv.vh.count[b], 1);
+// Rank of `key` among the keys raw[base .. base + n) of its bucket, counted up
+// to kBucketSlots (a rank of kBucketSlots means "not among the first 32").
+__device__ __forceinline__ int bucket_rank(const int* __restrict__ raw, int base, int n, int key) {
+    int rank = 0;
+    for (int k = 0; k < n && rank < kBucketSlots; ++k) rank += raw[base + k] < key ? 1 : 0;
+    return rank;
 }
 
-__global__ __launch_bounds__(256) void k_vhash_scatter(SelfView sv) {
+// A41:4550-4575: vertex cell and bucket.  phase 0 counts, 1 scatters, 2 ranks
+// (items beyond the first 32 of a bucket only count as overflow: A41:4570-4574
+// drops them).
+__global__ __launch_bounds__(256) void k_vhash(const ClothView* __restrict__ cv, SelfView sv, int parity, int phase, int count_diag) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= sv.nv) return;
-    const int b = sv.vcell[i].w;
-    const int slot = atomicAdd(&sv.vh.cursor[b], 1);
-    sv.vh.entries[sv.vh.off[b] + slot] = i;
+    const bool active = i < sv.nv;
+    if (phase == 0) {
+        if (!active) return;
+        const float4 p = cur_pos(*cv, parity)[i];
+        const int cx = cell_coord(p.x, sv.cell), cy = cell_coord(p.y, sv.cell), cz = cell_coord(p.z, sv.cell);
+        const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.vh.buckets));
+        sv.vcell[i] = make_int4(cx, cy, cz, b);
+        sv.vh.slot[i] = atomicAdd(&sv.vh.count[b], 1);
+        return;
+    }
+    if (phase == 1) {
+        if (active) {
+            const int b = sv.vcell[i].w;
+            sv.vh.raw[sv.vh.off[b] + sv.vh.slot[i]] = i;
+        }
+        return;
+    }
+    bool over = false;
+    if (active) {
+        const int b = sv.vcell[i].w;
+        const int base = sv.vh.off[b];
+        const int rank = bucket_rank(sv.vh.raw, base, sv.vh.count[b], i);
+        if (rank < kBucketSlots) sv.vh.entries[base + rank] = i;
+        else over = true;
+    }
+    if (count_diag) wave_count(sv.counts, kCountSelfOverflow, over);
 }
 
 // Primitive box in cells (A41:4632-4694, 5532-5593) with the >64-cell guard
@@ -204,54 +226,50 @@ __device__ inline bool edge_box(const ClothView& c, const SelfView& sv, int pari
     return true;
 }
 
-// kind 0 = triangles, 1 = edges; phase 0 counts, phase 1 scatters keys.
+// One wave per primitive, one lane per cell of its box (at most 64 cells:
+// larger boxes are reduced to one cell by the guard).  kind 0 = triangles,
+// 1 = edges; phase 0 counts, 1 scatters.  Key = primitive * 64 + the cell's
+// index in the box loop (z outer, then y, then x).
 __global__ __launch_bounds__(256) void k_prim_hash(const ClothView* __restrict__ cv, SelfView sv, int parity, int kind,
                                                    int phase, int count_diag) {
-    const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int k = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int idx = static_cast<int>(threadIdx.x & 63u);
     const int n = kind == 0 ? sv.nt : sv.ne;
     if (k >= n) return;
     int b0[3], b1[3];
     bool guarded = false;
     const bool ok = kind == 0 ? tri_box(*cv, sv, parity, k, b0, b1, &guarded) : edge_box(*cv, sv, parity, k, b0, b1, &guarded);
     if (!ok) return;
+    if (phase == 0 && guarded && count_diag && idx == 0) atomicAdd(&cv->counts[kCountSelfOverflow], 1ull);
+    const int sx = b1[0] - b0[0] + 1, sy = b1[1] - b0[1] + 1, sz = b1[2] - b0[2] + 1;
+    if (sx <= 0 || sy <= 0 || sz <= 0 || idx >= sx * sy * sz) return;
+    const int x = b0[0] + idx % sx, y = b0[1] + (idx / sx) % sy, z = b0[2] + idx / (sx * sy);
     const HashView& h = kind == 0 ? sv.th : sv.eh;
-    if (phase == 0 && guarded && count_diag) atomicAdd(&cv->counts[kCountSelfOverflow], 1ull);
-    int idx = 0;
-    for (int z = b0[2]; z <= b1[2]; ++z)
-        for (int y = b0[1]; y <= b1[1]; ++y)
-            for (int x = b0[0]; x <= b1[0]; ++x, ++idx) {
-                const int b = static_cast<int>(hash_cell(x, y, z, h.buckets));
-                if (phase == 0) {
-                    atomicAdd(&h.count[b], 1);
-                } else {
-                    const int slot = atomicAdd(&h.cursor[b], 1);
-                    h.entries[h.off[b] + slot] = k * kMaxPrimCells + idx;
-                }
-            }
+    const int b = static_cast<int>(hash_cell(x, y, z, h.buckets));
+    const int key = k * kMaxPrimCells + idx;
+    if (phase == 0) {
+        h.slot[key] = atomicAdd(&h.count[b], 1);
+    } else {
+        const int j = h.off[b] + h.slot[key];
+        h.raw[j] = key;
+        h.rawb[j] = b;
+    }
 }
 
-// Puts the smallest min(count, 32) keys of every bucket first, ascending.
-__global__ __launch_bounds__(256) void k_bucket_order(HashView h, unsigned long long* counts, int count_diag) {
-    const int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (b >= h.buckets) return;
-    const int s = h.off[b], n = h.count[b];
-    if (n <= 1) return;
-    int* e = h.entries + s;
-    if (n > kBucketSlots && count_diag) atomicAdd(&counts[kCountSelfOverflow], static_cast<unsigned long long>(n - kBucketSlots));
-    if (n <= 2 * kBucketSlots) {
-        for (int i = 1; i < n; ++i) {
-            const int key = e[i];
-            int j = i - 1;
-            while (j >= 0 && e[j] > key) { e[j + 1] = e[j]; --j; }
-            e[j + 1] = key;
-        }
-        return;
+// Every scattered item of a primitive hash finds its rank in its bucket and
+// lands there if it is among the first 32 (the others count as overflow).
+__global__ __launch_bounds__(256) void k_hash_rank(HashView h, unsigned long long* counts, int count_diag) {
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int total = h.off[h.buckets];
+    bool over = false;
+    if (j < total) {
+        const int b = h.rawb[j], key = h.raw[j];
+        const int base = h.off[b];
+        const int rank = bucket_rank(h.raw, base, h.count[b], key);
+        if (rank < kBucketSlots) h.entries[base + rank] = key;
+        else over = true;
     }
-    for (int i = 0; i < kBucketSlots; ++i) {   // partial selection sort
-        int m = i;
-        for (int j = i + 1; j < n; ++j) if (e[j] < e[m]) m = j;
-        const int t = e[i]; e[i] = e[m]; e[m] = t;
-    }
+    if (count_diag) wave_count(counts, kCountSelfOverflow, over);
 }
 
 // ------------------------------------------------------------- responses
@@ -628,11 +646,14 @@ bool SelfCollision::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& e
     auto make_hash = [&](HashView& h, int n, int capacity) -> bool {
         h.buckets = next_pow2(std::max(1024, n * 4));
         h.capacity = capacity;
+        const size_t cap = static_cast<size_t>(std::max(capacity, 1));
         h.count = static_cast<int*>(alloc_bytes(sizeof(int) * h.buckets));
         h.off = static_cast<int*>(alloc_bytes(sizeof(int) * (h.buckets + 1)));
-        h.cursor = static_cast<int*>(alloc_bytes(sizeof(int) * h.buckets));
-        h.entries = static_cast<int*>(alloc_bytes(sizeof(int) * static_cast<size_t>(std::max(capacity, 1))));
-        return h.count && h.off && h.cursor && h.entries;
+        h.slot = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
+        h.raw = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
+        h.rawb = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
+        h.entries = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
+        return h.count && h.off && h.slot && h.raw && h.rawb && h.entries;
     };
     if (s.use_vhash && !make_hash(s.vh, nv, nv)) return false;
     if (s.use_thash && !make_hash(s.th, s.nt, s.nt * kMaxPrimCells)) return false;
@@ -676,11 +697,12 @@ bool SelfCollision::run(const LaunchCtx& ctx, int parity, int substep, int subst
 
 static void build_prim_hash(hipStream_t st, const ClothView* cv, const SelfView& s, const HashView& h, int kind, int n,
                             int parity, bool count) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(h.buckets, 256)), dim3(256), 0, st, h.count, 0, h.buckets);
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(h.buckets, 256)), dim3(256), 0, st, h.cursor, 0, h.buckets);
-    hipLaunchKernelGGL(k_prim_hash, dim3(grid_for(n, 256)), dim3(256), 0, st, cv, s, parity, kind, 0, count ? 1 : 0);
+    const dim3 grid(grid_for(static_cast<long long>(n) * kMaxPrimCells, 256)), block(256);
+    launch_k(k_fill_i32, dim3(grid_for(h.buckets, 256)), block, 0, st, h.count, 0, h.buckets);
+    launch_k(k_prim_hash, grid, block, 0, st, cv, s, parity, kind, 0, count ? 1 : 0);
     exclusive_scan(st, h.count, h.buckets, h.off, s.scan_tmp);
-    hipLaunchKernelGGL(k_prim_hash, dim3(grid_for(n, 256)), dim3(256), 0, st, cv, s, parity, kind, 1, 0);
+    launch_k(k_prim_hash, grid, block, 0, st, cv, s, parity, kind, 1, 0);
+    launch_k(k_hash_rank, dim3(grid_for(h.capacity, 256)), block, 0, st, h, s.counts, count ? 1 : 0);
 }
 
 bool SelfCollision::run_for(hipStream_t st, const ClothView* cv, int parity, bool count) {
@@ -689,31 +711,25 @@ bool SelfCollision::run_for(hipStream_t st, const ClothView* cv, int parity, boo
     {
         TimedLaunch t("self_hash", st);
         if (s.use_vhash) {
-            hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(s.vh.buckets, 256)), dim3(256), 0, st, s.vh.count, 0, s.vh.buckets);
-            hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(s.vh.buckets, 256)), dim3(256), 0, st, s.vh.cursor, 0, s.vh.buckets);
-            hipLaunchKernelGGL(k_vhash_count, dim3(grid_for(s.nv, 256)), dim3(256), 0, st, cv, s, parity);
+            const dim3 grid(grid_for(s.nv, 256)), block(256);
+            launch_k(k_fill_i32, dim3(grid_for(s.vh.buckets, 256)), block, 0, st, s.vh.count, 0, s.vh.buckets);
+            launch_k(k_vhash, grid, block, 0, st, cv, s, parity, 0, 0);
             exclusive_scan(st, s.vh.count, s.vh.buckets, s.vh.off, s.scan_tmp);
-            hipLaunchKernelGGL(k_vhash_scatter, dim3(grid_for(s.nv, 256)), dim3(256), 0, st, s);
+            launch_k(k_vhash, grid, block, 0, st, cv, s, parity, 1, 0);
+            launch_k(k_vhash, grid, block, 0, st, cv, s, parity, 2, count ? 1 : 0);
         }
         if (s.use_thash) build_prim_hash(st, cv, s, s.th, 0, s.nt, parity, count);
         if (s.use_ehash) build_prim_hash(st, cv, s, s.eh, 1, s.ne, parity, count);
     }
-    {
-        TimedLaunch t("self_bucket_order", st);
-        const int cd = count ? 1 : 0;
-        if (s.use_vhash) hipLaunchKernelGGL(k_bucket_order, dim3(grid_for(s.vh.buckets, 256)), dim3(256), 0, st, s.vh, s.counts, cd);
-        if (s.use_thash) hipLaunchKernelGGL(k_bucket_order, dim3(grid_for(s.th.buckets, 256)), dim3(256), 0, st, s.th, s.counts, cd);
-        if (s.use_ehash) hipLaunchKernelGGL(k_bucket_order, dim3(grid_for(s.eh.buckets, 256)), dim3(256), 0, st, s.eh, s.counts, cd);
-    }
     for (int pass = 0; pass < passes_; ++pass) {
         {
             TimedLaunch t("self_contacts", st);
-            hipLaunchKernelGGL(k_self_contacts, dim3(grid_for(static_cast<long long>(s.nv) + s.ne, 256)), dim3(256), 0, st, cv, s,
-                               parity, count ? 1 : 0);
+            launch_k(k_self_contacts, dim3(grid_for(static_cast<long long>(s.nv) + s.ne, 256)), dim3(256), 0, st, cv, s,
+                     parity, count ? 1 : 0);
         }
         {
             TimedLaunch t("self_apply", st);
-            hipLaunchKernelGGL(k_self_apply, dim3(grid_for(s.nv, 256)), dim3(256), 0, st, cv, s, parity);
+            launch_k(k_self_apply, dim3(grid_for(s.nv, 256)), dim3(256), 0, st, cv, s, parity);
         }
     }
     return hip_ok(hipGetLastError(), "launch self collision");

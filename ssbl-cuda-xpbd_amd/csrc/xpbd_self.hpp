@@ -17,17 +17,22 @@ bool hip_ok(hipError_t err, const char* what);
 
 struct Solver;
 
-// One spatial hash as a counting-sorted bucket table: bucket b holds
-// entries[off[b] .. off[b+1]), the first min(count, 32) of them in ascending
-// key order (key = vertex, or primitive * 64 + cell index in the primitive's
-// box loop), which is the order queries visit.
+// One spatial hash as a counting-sorted bucket table.  Items are counted per
+// bucket (each item keeps the arrival slot its atomic returned), scattered to
+// raw[off[b] + slot], and then every item computes its rank among the keys of
+// its bucket: entries[off[b] + rank] holds the min(count, 32) smallest keys of
+// bucket b in ascending order (key = vertex, or primitive * 64 + cell index in
+// the primitive's box loop), which is the order queries visit.  The ranks do
+// not depend on the order the atomics landed in.
 struct HashView {
     int buckets;
     int capacity;
     int* count;      // buckets
     int* off;        // buckets + 1
-    int* cursor;     // buckets
-    int* entries;    // capacity
+    int* slot;       // capacity: arrival slot of item (vertex, or prim * 64 + cell)
+    int* raw;        // capacity: keys in arrival order
+    int* rawb;       // capacity: bucket of raw[j]
+    int* entries;    // capacity: first min(count, 32) keys of a bucket, ascending
 };
 
 struct SelfView {

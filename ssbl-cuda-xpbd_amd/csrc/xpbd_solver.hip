@@ -89,15 +89,17 @@ void resolve_pending() {
 }  // namespace
 
 bool timing_enabled() { return g_timing.load(std::memory_order_relaxed); }
-void timing_record(const char*, hipStream_t stream, hipEvent_t* begin_out) {
-    std::lock_guard<std::mutex> lock(g_timing_mutex);
-    *begin_out = take_event();
-    hipEventRecord(*begin_out, stream);
+const char*& timing_scope_slot() {
+    thread_local const char* name = nullptr;
+    return name;
 }
-void timing_finish(const char* name, hipStream_t stream, hipEvent_t begin) {
+void timing_take(hipEvent_t* begin, hipEvent_t* end) {
     std::lock_guard<std::mutex> lock(g_timing_mutex);
-    hipEvent_t end = take_event();
-    hipEventRecord(end, stream);
+    *begin = take_event();
+    *end = take_event();
+}
+void timing_push(const char* name, hipEvent_t begin, hipEvent_t end) {
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
     g_pending.push_back({name, begin, end});
     if (g_pending.size() > 4096) resolve_pending();
 }
@@ -197,7 +199,8 @@ struct Solver {
     // device state
     Dev<float4> pos0, pos1, prev, vel, rest;
     Dev<int> adj_off, lra_off, tri_off, tri_inc, pin_off, pin_ids, bend_off;
-    Dev<int2> adj, bends;
+    Dev<int2> adj, bends, ell;
+    int ell_width = 0, ell_pitch = 0, ell_overflow = 0;
     Dev<int4> lra_adj, tris, bend_segs, bend_adj;
     Dev<float> bend_rest, area_share, rest_scale, sdf, static_tris;
     Dev<float4> d_pins;
@@ -408,6 +411,16 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
             h_adj[cur[j]++] = make_int2(i, rb);
         }
     }
+    // slot-major copy of the first incidences (ClothView::ell)
+    int max_deg = 0;
+    for (int i = 0; i < n; ++i) max_deg = std::max(max_deg, h_adj_off[i + 1] - h_adj_off[i]);
+    ell_width = std::min(max_deg, kEllMax);
+    ell_pitch = (n + kTile - 1) / kTile * kTile;
+    ell_overflow = max_deg > kEllMax ? 1 : 0;
+    std::vector<int2> h_ell(static_cast<size_t>(std::max(ell_width, 1)) * std::max(ell_pitch, 1), make_int2(-1, 0));
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < std::min(h_adj_off[i + 1] - h_adj_off[i], ell_width); ++j)
+            h_ell[static_cast<size_t>(j) * ell_pitch + i] = h_adj[h_adj_off[i] + j];
     edge_colors.clear();
     if (cfg.edge_color_count > 0) {
         if (!mesh->edge_color_offsets) return fail("ABI41 edge color offsets are required");
@@ -525,7 +538,7 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     bool ok = pos0.upload(h_pos, "position allocation") && pos1.upload(h_pos, "position allocation") &&
               prev.upload(h_prev, "previous position allocation") && vel.upload(h_zero, "velocity allocation") &&
               rest.upload(h_rest, "rest position allocation") && adj_off.upload(h_adj_off, "edge incidence allocation") &&
-              adj.upload(h_adj, "edge incidence allocation") && lra_off.upload(h_lra_off, "LRA incidence allocation") &&
+              adj.upload(h_adj, "edge incidence allocation") && ell.upload(h_ell, "edge incidence allocation") && lra_off.upload(h_lra_off, "LRA incidence allocation") &&
               lra_adj.upload(h_lra, "LRA incidence allocation") && tri_off.upload(h_tri_off, "triangle incidence allocation") &&
               tri_inc.upload(h_tri_inc, "triangle incidence allocation") && tris.upload(h_tris, "triangle allocation") &&
               bends.upload(h_bends, "bend allocation") && bend_rest.upload(h_bend_rest, "bend rest allocation") &&
@@ -707,7 +720,8 @@ void Solver::fill_view(int substeps) {
     v.nv = nv; v.ne = cfg.edge_count; v.nt = cfg.triangle_count; v.nbend = cfg.bend_count; v.nlra = cfg.lra_count;
     v.npin = static_cast<int>(pins.size()); v.nfield = static_cast<int>(fields.size());
     v.pos0 = pos0.p; v.pos1 = pos1.p; v.prev = prev.p; v.vel = vel.p; v.rest = rest.p;
-    v.adj_off = adj_off.p; v.adj = adj.p; v.lra_off = lra_off.p; v.lra_adj = lra_adj.p;
+    v.adj_off = adj_off.p; v.adj = adj.p;
+    v.ell = ell.p; v.ell_width = ell_width; v.ell_pitch = ell_pitch; v.ell_overflow = ell_overflow; v.lra_off = lra_off.p; v.lra_adj = lra_adj.p;
     v.tri_off = tri_off.p; v.tri_inc = tri_inc.p; v.tris = tris.p; v.bends = bends.p; v.bend_rest = bend_rest.p;
     v.bend_off = bend_off.p; v.bend_adj = bend_adj.p;
     v.pin_off = pin_off.p; v.pin_ids = pin_ids.p; v.pins = d_pins.p;
@@ -746,7 +760,7 @@ void Solver::fill_view(int substeps) {
 // `dviews[m]` is member m's view inside that table.
 struct StepPlan {
     bool any_pins = false, any_edges = false, any_lra = false, any_bend = false, any_self = false, any_pcg = false;
-    bool any_bend_gather = false;
+    bool any_bend_gather = false, any_forces = false;
 };
 
 static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const int4* bend_segs_dev,
@@ -761,6 +775,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
         plan.any_self |= s->cfg.self_collision && s->nv > 0 && s->self_radius() > 0.0f;
         plan.any_pcg |= s->pcg.enabled();
         plan.any_bend_gather |= s->bend_gather;
+        plan.any_forces |= s->view.need_normal != 0 || s->view.nfield > 0;
     }
     const unsigned cnt = count ? kOpCount : 0u;
     const unsigned pin_ops = plan.any_pins ? (kOpPin | kOpPinPolish) : 0u;
@@ -778,7 +793,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
         }
     };
     for (int sub = 0; sub < substeps; ++sub) {
-        launch_integrate(ctx, parity, plan.any_pins ? kOpPin : 0u);
+        launch_integrate(ctx, parity, plan.any_pins ? kOpPin : 0u, plan.any_forces);
         parity ^= 1;
         for (int it = 0; it < iterations; ++it) {
             const bool last = it == iterations - 1;
@@ -971,6 +986,33 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     if (!run_schedule(b.members, ctx, b.d_views.p, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false))
         return false;
     return force_sync ? sync_stream() : true;
+}
+
+// Runs `launches` (even) back-to-back fused constraint passes of a batch --
+// the pass of every iteration but a substep's last: stretch gather, pins,
+// analytic and SDF contacts -- between two events on the library stream, and
+// returns the mean duration.  Consecutive launches on one stream: the mean is
+// the kernel time plus the dispatch gap, with no per-launch instrumentation.
+static bool time_batch_pass(Batch& b, int launches, float* avg_ms) {
+    launches = std::max(2, launches + (launches & 1));
+    bool pins = false, edges = false;
+    for (Solver* s : b.members) {
+        pins |= !s->pins.empty();
+        edges |= s->cfg.edge_count > 0 && !s->cfg.stretch_optimization_enabled;
+    }
+    const unsigned ops = (edges ? kOpStretch : 0u) | (pins ? (kOpPin | kOpPinPolish) : 0u) | kOpAnalytic | kOpSdf;
+    LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!hip_ok(hipEventCreate(&e0), "timing event") || !hip_ok(hipEventCreate(&e1), "timing event")) return false;
+    bool ok = hip_ok(hipEventRecord(e0, ctx.stream), "timing event");
+    for (int k = 0; ok && k < launches; ++k) launch_vertex_pass(ctx, k & 1, (k & 1) ^ 1, ops);
+    ok = ok && hip_ok(hipEventRecord(e1, ctx.stream), "timing event") && hip_ok(hipEventSynchronize(e1), "timing event");
+    float ms = 0.0f;
+    ok = ok && hip_ok(hipEventElapsedTime(&ms, e0, e1), "timing event");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (ok) *avg_ms = ms / static_cast<float>(launches);
+    return ok;
 }
 
 }  // namespace ssbl
@@ -1218,6 +1260,11 @@ SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int 
     g_error.clear();
     if (!batch) return fail("invalid batch") ? 1 : 0;
     return step_batch(*static_cast<Batch*>(batch), substeps, iterations, force_sync) ? 1 : 0;
+}
+SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms) {
+    g_error.clear();
+    if (!batch || !avg_ms) return fail("invalid batch timing request") ? 1 : 0;
+    return time_batch_pass(*static_cast<Batch*>(batch), launches, avg_ms) ? 1 : 0;
 }
 SSBL_API int ssbl_ext_synchronize(void) { g_error.clear(); return sync_stream() ? 1 : 0; }
 SSBL_API int ssbl_ext_kernel_timing(int enable) {

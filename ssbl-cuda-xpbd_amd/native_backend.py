@@ -275,6 +275,13 @@ class SolverBatch:
         if not self._lib.ssbl_ext_step_batch(self._handle, int(substeps), int(iterations), int(bool(synchronize))):
             raise NativeSolverError(last_error(self._lib))
 
+    def time_pass(self, launches: int = 64) -> float:
+        """Mean ms of `launches` back-to-back fused constraint passes (advances the state)."""
+        out = np.zeros(1, np.float32)
+        if not self._lib.ssbl_ext_time_batch_pass(self._handle, int(launches), _fptr(out)):
+            raise NativeSolverError(last_error(self._lib))
+        return float(out[0])
+
     def close(self) -> None:
         if getattr(self, "_handle", None):
             self._lib.ssbl_ext_destroy_batch(self._handle)
