@@ -88,6 +88,27 @@ def single_cloth_line(name: str, seed: int, frames: int, warmup: int, lib) -> di
             "self_collision": bool(sc.options[0].self_collision), "finite": int(d.finite_flag)}
 
 
+def serving_line(seed: int, window_s: float, max_s: float, lib) -> dict:
+    """Served cloth sessions: Poisson(200/s) arrivals over `window_s` seconds,
+    sizes log-uniform in [1k, 250k] vertices, 60-600 frames each, wind and a
+    Lissajous sphere per session, admitted against a 200 GB HBM budget and
+    stepped by continuous batching (10 substeps x 8 iterations)."""
+    from ssbl_xpbd_amd import serving
+    t0 = time.perf_counter()
+    reqs = serving.served_requests(seed=seed, rate=200.0, window_s=window_s)
+    build_s = time.perf_counter() - t0
+    srv = serving.ClothServer(hbm_budget_bytes=200 * 10**9, substeps=10, iterations=8, max_sessions=512,
+                              max_batch_vertices=16 * 10**6, lib=lib)
+    rep = srv.run(reqs, clock="wall", max_seconds=max_s)
+    out = rep.summary()
+    out.update({"workload": "served_sessions", "arrival_rate_per_s": 200.0, "arrival_window_s": window_s,
+                "sessions_requested": len(reqs), "request_build_s": build_s, "substeps": 10, "iterations": 8,
+                "hbm_budget_bytes": 200 * 10**9})
+    print(f"# serving: {len(rep.results)}/{len(reqs)} sessions in {rep.wall_s:.1f} s, "
+          f"p50 {rep.frame_ms_p50:.2f} ms p99 {rep.frame_ms_p99:.2f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def create(scene, lib=None):
     return [nb.NativeXpbdSolver(c, o, scene.static_triangles, lib=lib) for c, o in zip(scene.cloths, scene.options)]
 
@@ -164,6 +185,9 @@ def main():
     ap.add_argument("--latency-frames", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-window", type=float, default=0.5,
+                    help="seconds of Poisson(200/s) session arrivals in the served-sessions run (0 = skip it)")
+    ap.add_argument("--serving-max-s", type=float, default=25.0, help="wall-clock cap of the served-sessions run")
     ap.add_argument("--batched-only", action="store_true",
                     help="only step the batched form (warmup + steps frames) and exit: for counter passes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -214,6 +238,7 @@ def main():
     lat_p = latencies(lambda: run_plain(plain, scene, 1), lib, args.latency_frames)
 
     secondary = single_cloth_line("selfcol_100x100", args.seed, 10, 2, lib) if args.workload != "selfcol_100x100" else None
+    serving = serving_line(args.seed, args.serving_window, args.serving_max_s, lib) if args.serving_window > 0 else None
     stats = kernel_stats(lib, step_frame)
     # the dominant kernel's mean duration: back-to-back launches between two
     # events on the library stream (per-launch instrumentation slows each
@@ -253,6 +278,7 @@ def main():
             "frame_ms_p99_plain": float(np.percentile(lat_p, 99)),
             "kernels": stats,
             "selfcol_10k": secondary,
+            "serving": serving,
         },
         "roofline": {
             "bound": "hbm",

@@ -54,6 +54,11 @@ SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int 
  * the members' state by that many Jacobi iterations: a measurement call. */
 SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms);
 
+/* Device memory (bytes) the handle holds: state, constraint tables, SDF,
+ * self-collision hashes and PCG arrays.  -1 for an invalid handle.  A
+ * serving loop sums these against its HBM budget. */
+SSBL_API long long ssbl_ext_device_bytes(void* handle);
+
 /* Waits for all work the library queued. */
 SSBL_API int ssbl_ext_synchronize(void);
 

@@ -2302,6 +2302,7 @@ SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int 
         if (!ssbl_step_solver_ex(h, substeps, iterations, 0, force_sync)) return 0;
     return 1;
 }
+SSBL_API long long ssbl_ext_device_bytes(void*) { fail("oracle: no device memory"); return -1; }
 SSBL_API int ssbl_ext_time_batch_pass(void*, int, float*) { return fail("oracle: no device timing") ? 1 : 0; }
 SSBL_API int ssbl_ext_synchronize(void) { return 1; }
 SSBL_API int ssbl_ext_kernel_timing(int) { return 1; }

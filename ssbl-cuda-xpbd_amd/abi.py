@@ -127,6 +127,7 @@ EXT_SIGNATURES = {
     "ssbl_ext_destroy_batch": (_i, [_vp]),
     "ssbl_ext_step_batch": (_i, [_vp, _i, _i, _i]),
     "ssbl_ext_time_batch_pass": (_i, [_vp, _i, _fp]),
+    "ssbl_ext_device_bytes": (ctypes.c_longlong, [_vp]),
     "ssbl_ext_synchronize": (_i, []),
     "ssbl_ext_kernel_timing": (_i, [_i]),
     "ssbl_ext_kernel_stats": (_i, [ctypes.c_char_p, _i]),

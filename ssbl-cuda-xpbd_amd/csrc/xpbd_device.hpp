@@ -137,4 +137,10 @@ void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float o
                       const float* tris, int ntris, int signed_sdf);
 void launch_fill_u32(hipStream_t s, unsigned* p, unsigned value, size_t n);
 
+// Device allocations of the library go through these, so the bytes a solver
+// holds can be reported (ssbl_ext_device_bytes).
+void* device_alloc(size_t bytes, const char* what);
+void device_free(void* p, size_t bytes);
+long long device_bytes_in_use();
+
 }  // namespace ssbl

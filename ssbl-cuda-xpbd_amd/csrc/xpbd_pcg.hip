@@ -443,13 +443,13 @@ __global__ __launch_bounds__(kTile) void k_edge_color(const ClothView* __restric
 
 // ------------------------------------------------------------------ host
 PcgPath::~PcgPath() {
-    for (void* p : allocations_) (void)hipFree(p);
+    for (auto& a : allocations_) device_free(a.first, a.second);
 }
 
 void* PcgPath::alloc_bytes(size_t bytes) {
-    void* p = nullptr;
-    if (!hip_ok(hipMalloc(&p, std::max<size_t>(bytes, 256)), "PCG allocation")) return nullptr;
-    allocations_.push_back(p);
+    const size_t n = std::max<size_t>(bytes, 256);
+    void* p = device_alloc(n, "PCG allocation");
+    if (p) allocations_.emplace_back(p, n);
     return p;
 }
 

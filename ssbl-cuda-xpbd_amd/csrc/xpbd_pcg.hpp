@@ -2,6 +2,7 @@
 #pragma once
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "xpbd_device.hpp"
@@ -59,7 +60,7 @@ private:
     PcgView pv_{};
     int4* segs_ = nullptr;
     std::vector<int> seg_first_, seg_count_;
-    std::vector<void*> allocations_;
+    std::vector<std::pair<void*, size_t>> allocations_;
 };
 
 }  // namespace ssbl

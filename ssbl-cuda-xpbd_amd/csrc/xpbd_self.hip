@@ -609,13 +609,13 @@ __global__ __launch_bounds__(256) void k_self_apply(const ClothView* __restrict_
 
 // ------------------------------------------------------------------ host
 SelfCollision::~SelfCollision() {
-    for (void* p : allocations_) (void)hipFree(p);
+    for (auto& a : allocations_) device_free(a.first, a.second);
 }
 
 void* SelfCollision::alloc_bytes(size_t bytes) {
-    void* p = nullptr;
-    if (!hip_ok(hipMalloc(&p, std::max<size_t>(bytes, 256)), "self collision allocation")) return nullptr;
-    allocations_.push_back(p);
+    const size_t n = std::max<size_t>(bytes, 256);
+    void* p = device_alloc(n, "self collision allocation");
+    if (p) allocations_.emplace_back(p, n);
     return p;
 }
 

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "xpbd_device.hpp"
@@ -82,7 +83,7 @@ private:
     int passes_ = 1;
     int interval_ = 1;
     SelfView sv_{};
-    std::vector<void*> allocations_;
+    std::vector<std::pair<void*, size_t>> allocations_;
     const ClothView* dview_ = nullptr;
     void* alloc_bytes(size_t bytes);
 };

@@ -105,6 +105,23 @@ void timing_push(const char* name, hipEvent_t begin, hipEvent_t end) {
 }
 
 // ------------------------------------------------------- device buffers
+namespace {
+std::atomic<long long> g_device_bytes{0};
+}  // namespace
+
+void* device_alloc(size_t bytes, const char* what) {
+    void* p = nullptr;
+    if (!hip_ok(hipMalloc(&p, bytes), what)) return nullptr;
+    g_device_bytes.fetch_add(static_cast<long long>(bytes), std::memory_order_relaxed);
+    return p;
+}
+void device_free(void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    g_device_bytes.fetch_sub(static_cast<long long>(bytes), std::memory_order_relaxed);
+}
+long long device_bytes_in_use() { return g_device_bytes.load(std::memory_order_relaxed); }
+
 template <class T> struct Dev {
     T* p = nullptr;
     size_t n = 0;
@@ -112,11 +129,12 @@ template <class T> struct Dev {
     Dev(const Dev&) = delete;
     Dev& operator=(const Dev&) = delete;
     ~Dev() { release(); }
-    void release() { if (p) hipFree(p); p = nullptr; n = 0; }
+    void release() { device_free(p, sizeof(T) * n); p = nullptr; n = 0; }
     bool alloc(size_t count, const char* what) {
         release();
         if (count == 0) return true;
-        if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count), what)) { p = nullptr; return false; }
+        p = static_cast<T*>(device_alloc(sizeof(T) * count, what));
+        if (!p) return false;
         n = count;
         return true;
     }
@@ -174,6 +192,7 @@ int next_pow2(int v) {
 struct HostPin { int index; float target[3]; float weight; };
 
 struct Solver {
+    long long dev_bytes = 0;   // device memory held (ssbl_ext_device_bytes)
     SsblXpbdConfig cfg{};
     SsblXpbdRuntimeColliders colliders{};
     int stretch_mode = SSBL_EXT_STRETCH_JACOBI;
@@ -1015,6 +1034,14 @@ static bool time_batch_pass(Batch& b, int launches, float* avg_ms) {
     return ok;
 }
 
+// Adds the device memory allocated (or freed) during its scope to a solver.
+struct ByteScope {
+    Solver* s;
+    long long before;
+    explicit ByteScope(Solver* solver) : s(solver), before(device_bytes_in_use()) {}
+    ~ByteScope() { if (s) s->dev_bytes += device_bytes_in_use() - before; }
+};
+
 }  // namespace ssbl
 
 using namespace ssbl;
@@ -1030,7 +1057,12 @@ SSBL_API void* ssbl_create_solver(const SsblXpbdConfig* config, const SsblXpbdMe
         return nullptr;
     }
     Solver* s = new Solver();
-    if (!s->create(config, mesh)) {
+    bool created;
+    {
+        ByteScope bytes(s);
+        created = s->create(config, mesh);
+    }
+    if (!created) {
         const std::string err = g_error;
         delete s;
         g_error = err;
@@ -1067,6 +1099,7 @@ SSBL_API int ssbl_update_pin_targets(void* handle, const int* indices, const flo
     g_error.clear();
     Solver* s = static_cast<Solver*>(handle);
     if (!s) return fail("invalid solver handle") ? 1 : 0;
+    ByteScope bytes(s);
     return s->upload_pins(indices, positions, weights, count) ? 1 : 0;
 }
 
@@ -1097,6 +1130,7 @@ SSBL_API int ssbl_update_static_triangles(void* handle, const float* triangles, 
     if (!s) return fail("invalid static triangle update") ? 1 : 0;
     triangle_count = std::max(triangle_count, 0);
     if (triangle_count > 0 && !triangles) return fail("static triangle data is required when triangle count is nonzero") ? 1 : 0;
+    ByteScope bytes(s);
     return s->build_sdf(triangles, triangle_count) ? 1 : 0;
 }
 
@@ -1112,6 +1146,7 @@ SSBL_API int ssbl_update_frame_inputs(void* handle, const SsblXpbdFrameInputs* i
     g_error.clear();
     Solver* s = static_cast<Solver*>(handle);
     if (!s || !in) return fail("invalid frame input update") ? 1 : 0;
+    ByteScope bytes(s);
     if (in->update_pin_targets && !s->upload_pins(in->pin_indices, in->pin_positions, in->pin_weights, in->pin_count)) return 0;
     if (in->update_runtime_colliders) s->colliders = in->runtime_colliders;
     if ((in->update_dynamic_triangles && in->dynamic_triangle_count > 0) ||
@@ -1260,6 +1295,11 @@ SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int 
     g_error.clear();
     if (!batch) return fail("invalid batch") ? 1 : 0;
     return step_batch(*static_cast<Batch*>(batch), substeps, iterations, force_sync) ? 1 : 0;
+}
+SSBL_API long long ssbl_ext_device_bytes(void* handle) {
+    g_error.clear();
+    if (!handle) { fail("invalid solver handle"); return -1; }
+    return static_cast<Solver*>(handle)->dev_bytes;
 }
 SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms) {
     g_error.clear();

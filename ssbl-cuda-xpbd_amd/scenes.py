@@ -22,6 +22,10 @@ _EPS = 1.0e-7
 
 
 # ------------------------------------------------------------- builders
+def _pair_keys(lo: np.ndarray, hi: np.ndarray, n: int) -> np.ndarray:
+    return lo.astype(np.int64) * n + hi.astype(np.int64)
+
+
 def edge_constraints(triangles: np.ndarray, rest: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     """Unique undirected edges, sorted, with rest lengths (xpbd_core.py:715-727)."""
     tri = np.asarray(triangles, dtype=np.int64).reshape(-1, 3)
@@ -29,7 +33,9 @@ def edge_constraints(triangles: np.ndarray, rest: np.ndarray) -> tuple[np.ndarra
         return np.empty((0, 2), np.int32), np.empty(0, np.float32)
     e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
     e = np.sort(e, axis=1)
-    e = np.unique(e, axis=0)
+    n = int(e.max()) + 1
+    keys = np.unique(_pair_keys(e[:, 0], e[:, 1], n))      # lexicographic (lo, hi) order
+    e = np.stack([keys // n, keys % n], axis=1)
     d = rest[e[:, 1]].astype(np.float64) - rest[e[:, 0]].astype(np.float64)
     lengths = np.maximum(np.linalg.norm(d, axis=1), _EPS).astype(np.float32)
     return e.astype(np.int32), lengths
@@ -38,23 +44,31 @@ def edge_constraints(triangles: np.ndarray, rest: np.ndarray) -> tuple[np.ndarra
 def bend_constraints(triangles: np.ndarray, rest: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     """Pairs of vertices opposite each interior edge (xpbd_core.py:730-747).
 
-    The first triangle seen on an edge records its opposite vertex; a later
-    triangle with a different opposite vertex forms the pair.
+    Triangle edges are visited in order (triangle by triangle, edges ab, bc,
+    ca).  The first triangle seen on an edge records its opposite vertex; a
+    later triangle with a different opposite vertex forms the pair.  Computed
+    by a stable sort on the edge key, so "first seen" is the first of each
+    key's run.
     """
     tri = np.asarray(triangles, dtype=np.int64).reshape(-1, 3)
-    first: dict[tuple[int, int], int] = {}
-    pairs: set[tuple[int, int]] = set()
-    for a, b, c in tri.tolist():
-        for x, y, opp in ((a, b, c), (b, c, a), (c, a, b)):
-            key = (x, y) if x < y else (y, x)
-            prev = first.get(key)
-            if prev is None:
-                first[key] = opp
-            elif prev != opp:
-                pairs.add((prev, opp) if prev < opp else (opp, prev))
-    if not pairs:
+    if len(tri) == 0:
         return np.empty((0, 2), np.int32), np.empty(0, np.float32)
-    arr = np.asarray(sorted(pairs), dtype=np.int64)
+    x = tri[:, [0, 1, 2]].ravel()
+    y = tri[:, [1, 2, 0]].ravel()
+    opp = tri[:, [2, 0, 1]].ravel()
+    n = int(tri.max()) + 1
+    keys = _pair_keys(np.minimum(x, y), np.maximum(x, y), n)
+    order = np.argsort(keys, kind="stable")
+    k_sorted, o_sorted = keys[order], opp[order]
+    run_start = np.ones(len(k_sorted), bool)
+    run_start[1:] = k_sorted[1:] != k_sorted[:-1]
+    first_opp = o_sorted[np.maximum.accumulate(np.where(run_start, np.arange(len(k_sorted)), 0))]
+    later = ~run_start & (o_sorted != first_opp)
+    a, b = first_opp[later], o_sorted[later]
+    if len(a) == 0:
+        return np.empty((0, 2), np.int32), np.empty(0, np.float32)
+    pk = np.unique(_pair_keys(np.minimum(a, b), np.maximum(a, b), n))
+    arr = np.stack([pk // n, pk % n], axis=1)
     d = rest[arr[:, 1]].astype(np.float64) - rest[arr[:, 0]].astype(np.float64)
     return arr.astype(np.int32), np.maximum(np.linalg.norm(d, axis=1), _EPS).astype(np.float32)
 

@@ -1,0 +1,372 @@
+"""Serving loop for cloth sessions on one GPU.
+
+A session is one cloth advanced frame by frame. It has a wind force field,
+an optional moving sphere collider, and a number of frames to run. The server:
+
+* admits waiting sessions in arrival order, while the HBM budget, the session
+  slots and the per-tick vertex capacity allow;
+* keeps every admitted session's state resident on the device (one ABI41
+  handle each) until its last frame;
+* steps all active sessions together each tick (continuous batching).
+  A tick is one frame of every active session, run as one batched step
+  (ssbl_ext_step_batch), so one launch per solver phase covers them all.
+  Sessions that arrive while a tick runs join at the next tick. Finished
+  sessions leave, and their memory frees slots for the queue;
+* records each session's per-frame latency (from the start of the tick's
+  input updates to the end of its step) and the aggregate vertex-substeps/s.
+
+The per-frame order follows the reference's session manager
+(session_manager.py:2520-2591): frame inputs of every slot, then the steps,
+then downloads (here only at a session's end).
+
+Batched stepping gives each handle the same result as stepping it alone
+(ssbl_ext_step_batch), so a session's result does not depend on the mix it
+was served in.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field, replace
+from functools import lru_cache
+
+import numpy as np
+
+from . import native_backend as nb
+from .scenes import ClothBuildData, SolverOptions, build_cloth, grid_mesh, y_up_to_z_up
+
+
+# ------------------------------------------------------------- requests
+@dataclass
+class SpherePath:
+    """Lissajous path of the sphere collider: center + amplitude * sin(2 pi f t + phase)."""
+    center: tuple
+    amplitude: tuple
+    frequency: tuple
+    phase: tuple
+    radius: float
+
+    def at(self, t: float) -> tuple:
+        return tuple(float(c + a * math.sin(2.0 * math.pi * f * t + p))
+                     for c, a, f, p in zip(self.center, self.amplitude, self.frequency, self.phase))
+
+
+@dataclass
+class SessionRequest:
+    sid: int
+    arrival: float                      # seconds after start (wall clock) or tick index (tick clock)
+    cloth: ClothBuildData
+    options: SolverOptions
+    frames: int
+    wind: dict | None = None            # SsblXpbdForceField fields of a wind field (type 1)
+    sphere: SpherePath | None = None
+
+    @property
+    def vertex_count(self) -> int:
+        return len(self.cloth.positions_world)
+
+
+def wind_field(direction, strength: float) -> dict:
+    """A wind force field (A41:1433-1447): force along `direction` scaled by the facing of the surface."""
+    d = np.asarray(direction, np.float64)
+    d = d / max(float(np.linalg.norm(d)), 1.0e-12)
+    return {"type": 1, "strength": float(strength), "direction": tuple(float(x) for x in d), "axis": (0.0, 0.0, 1.0),
+            "apply_to_location": 1, "noise": 0.0, "flow": 0.0}
+
+
+def _next_pow2(v: int) -> int:
+    r = 1
+    while r < max(v, 1):
+        r <<= 1
+    return r
+
+
+def estimate_device_bytes(cloth: ClothBuildData, options: SolverOptions) -> int:
+    """Upper bound of the device memory one handle holds, for admission.
+
+    Sums the library's allocations per element (float4 state x 5, edge
+    incidences at both endpoints plus the 8-slot table, triangles and their
+    incidences, bends, pins, per-vertex pressure terms, the static SDF at its
+    maximum resolution, the three self-collision hashes with their tables,
+    and the PCG vectors), plus a per-allocation allowance. The admission
+    test checks it against ssbl_ext_device_bytes.
+    """
+    v = len(cloth.positions_world)
+    e = len(cloth.edges)
+    t = len(cloth.triangles)
+    b = len(cloth.bends)
+    lra = len(cloth.lra_edges)
+    p = len(cloth.pin_indices)
+    core = (5 * 16 + 6 * 4 + 8 * 8 + 2 * 4) * v + 16 * e + 8 * e + (16 + 12) * t + (8 + 4 + 32 + 16) * b + 32 * lra + 24 * p
+    res = max(int(options.static_sdf_max_resolution), 2)
+    sdf = 4 * res ** 3
+    self_bytes = 0
+    if options.self_collision:
+        def hash_bytes(n, cap):
+            return 8 * _next_pow2(max(1024, 4 * n)) + 16 * cap
+        self_bytes = (hash_bytes(v, v) + hash_bytes(t, 64 * t) + hash_bytes(e, 64 * e)
+                      + (16 + 4 + 128 + 24 + 4) * v + 16 * t + 8 * e + 4096)
+    pcg = (144 * v + 60 * e) if options.stretch_optimization_enabled else 0
+    return int(1.1 * (core + sdf + self_bytes + pcg)) + 64 * 4096 + (1 << 20)
+
+
+# ------------------------------------------------------------- sessions
+@dataclass
+class SessionResult:
+    sid: int
+    vertex_count: int
+    frames: int
+    positions: np.ndarray
+    frame_ms: list
+    arrival: float
+    admitted: float
+    finished: float
+    device_bytes: int
+
+
+@dataclass
+class _Active:
+    req: SessionRequest
+    solver: nb.NativeXpbdSolver
+    device_bytes: int
+    admitted: float
+    frame: int = 0
+    frame_ms: list = field(default_factory=list)
+
+
+@dataclass
+class ServeReport:
+    results: dict
+    ticks: int
+    wall_s: float
+    vertex_substeps: int
+    frame_ms_p50: float
+    frame_ms_p99: float
+    peak_device_bytes: int
+    peak_active_vertices: int
+    peak_sessions: int
+    mean_queue_wait: float
+    unfinished: int
+
+    @property
+    def vertex_substeps_per_s(self) -> float:
+        return self.vertex_substeps / self.wall_s if self.wall_s > 0 else 0.0
+
+    def summary(self) -> dict:
+        return {"sessions_completed": len(self.results), "sessions_unfinished": self.unfinished, "ticks": self.ticks,
+                "wall_s": self.wall_s, "vertex_substeps_per_s": self.vertex_substeps_per_s,
+                "frame_ms_p50": self.frame_ms_p50, "frame_ms_p99": self.frame_ms_p99,
+                "peak_device_bytes": self.peak_device_bytes, "peak_active_vertices": self.peak_active_vertices,
+                "peak_sessions": self.peak_sessions, "mean_queue_wait": self.mean_queue_wait}
+
+
+class ClothServer:
+    """Admission + continuous batching over ABI41 handles of one library."""
+
+    def __init__(self, *, hbm_budget_bytes: int, substeps: int, iterations: int, max_sessions: int = 1024,
+                 max_batch_vertices: int = 1 << 62, static_triangles: np.ndarray | None = None, lib=None):
+        self.lib = lib if lib is not None else nb.load_library()
+        self.budget = int(hbm_budget_bytes)
+        self.substeps = int(substeps)
+        self.iterations = int(iterations)
+        self.max_sessions = int(max_sessions)
+        self.max_batch_vertices = int(max_batch_vertices)
+        self.static_triangles = static_triangles
+        self.pending: list[SessionRequest] = []
+        self.active: list[_Active] = []
+        self.results: dict[int, SessionResult] = {}
+        self.used_bytes = 0
+        self.active_vertices = 0
+        self.ticks = 0
+        self.vertex_substeps = 0
+        self._batch: nb.SolverBatch | None = None
+        self._batch_stale = True
+        self.peak_bytes = 0
+        self.peak_vertices = 0
+        self.peak_sessions = 0
+
+    # -- admission
+    def submit(self, req: SessionRequest) -> None:
+        if estimate_device_bytes(req.cloth, req.options) > self.budget or req.vertex_count > self.max_batch_vertices:
+            raise ValueError(f"session {req.sid} can never fit the server's budget")
+        self.pending.append(req)
+        self.pending.sort(key=lambda r: (r.arrival, r.sid))
+
+    def _fits(self, req: SessionRequest) -> bool:
+        return (len(self.active) < self.max_sessions
+                and self.active_vertices + req.vertex_count <= self.max_batch_vertices
+                and self.used_bytes + estimate_device_bytes(req.cloth, req.options) <= self.budget)
+
+    def admit(self, now: float) -> int:
+        """Admits arrived sessions in arrival order while they fit; returns how many."""
+        n = 0
+        while self.pending and self.pending[0].arrival <= now and self._fits(self.pending[0]):
+            req = self.pending.pop(0)
+            solver = nb.NativeXpbdSolver(req.cloth, req.options, self.static_triangles, lib=self.lib)
+            if req.wind is not None:
+                solver.update_force_fields([req.wind])
+            held = int(self.lib.ssbl_ext_device_bytes(solver.handle))
+            if held < 0:                                # library without device accounting
+                held = estimate_device_bytes(req.cloth, req.options)
+            self.active.append(_Active(req, solver, held, now))
+            self.used_bytes += held
+            self.active_vertices += req.vertex_count
+            self._batch_stale = True
+            n += 1
+        self.peak_bytes = max(self.peak_bytes, self.used_bytes)
+        self.peak_vertices = max(self.peak_vertices, self.active_vertices)
+        self.peak_sessions = max(self.peak_sessions, len(self.active))
+        return n
+
+    # -- one frame of every active session
+    def _frame_inputs(self, s: _Active) -> None:
+        if s.req.sphere is None:
+            return
+        t = s.frame * float(s.req.options.dt)
+        s.solver.update_runtime_colliders(replace(s.req.options, use_sphere=True, sphere_center=s.req.sphere.at(t),
+                                                  sphere_radius=s.req.sphere.radius))
+
+    def tick(self, now: float) -> bool:
+        self.admit(now)
+        if not self.active:
+            return False
+        t0 = time.perf_counter()
+        for s in self.active:
+            self._frame_inputs(s)
+        if self._batch_stale or self._batch is None:
+            if self._batch is not None:
+                self._batch.close()
+            self._batch = nb.SolverBatch([s.solver for s in self.active])
+            self._batch_stale = False
+        self._batch.step(self.substeps, self.iterations, synchronize=True)
+        ms = (time.perf_counter() - t0) * 1.0e3
+        self.ticks += 1
+        self.vertex_substeps += self.active_vertices * self.substeps
+        done = []
+        for s in self.active:
+            s.frame += 1
+            s.frame_ms.append(ms)
+            if s.frame >= s.req.frames:
+                done.append(s)
+        if done:
+            self._batch.close()
+            self._batch = None
+            self._batch_stale = True
+            for s in done:
+                self._retire(s, now)
+        return True
+
+    def _retire(self, s: _Active, now: float) -> None:
+        pos = s.solver.download_positions()
+        s.solver.close()
+        self.active.remove(s)
+        self.used_bytes -= s.device_bytes
+        self.active_vertices -= s.req.vertex_count
+        self.results[s.req.sid] = SessionResult(s.req.sid, s.req.vertex_count, s.frame, pos, s.frame_ms, s.req.arrival,
+                                                s.admitted, now, s.device_bytes)
+
+    def close(self) -> None:
+        if self._batch is not None:
+            self._batch.close()
+            self._batch = None
+        for s in list(self.active):
+            s.solver.close()
+        self.active.clear()
+
+    # -- driver
+    def run(self, requests, *, clock: str = "wall", max_seconds: float | None = None, max_ticks: int | None = None) -> ServeReport:
+        """Serves `requests` until all finish or a limit is reached.
+
+        clock "wall": arrivals are seconds after the start; "tick": arrivals
+        are tick indices (deterministic, for tests)."""
+        for r in requests:
+            self.submit(r)
+        start = time.perf_counter()
+        tick_index = 0
+        while self.pending or self.active:
+            now = (time.perf_counter() - start) if clock == "wall" else float(tick_index)
+            if max_seconds is not None and clock == "wall" and now > max_seconds:
+                break
+            if max_ticks is not None and tick_index >= max_ticks:
+                break
+            ran = self.tick(now)
+            tick_index += 1
+            if not ran and self.pending and clock == "wall":
+                time.sleep(min(max(self.pending[0].arrival - now, 0.0), 0.01))
+        wall = time.perf_counter() - start
+        unfinished = len(self.active) + len(self.pending)
+        self.close()
+        lat = [m for r in self.results.values() for m in r.frame_ms]
+        waits = [r.admitted - r.arrival for r in self.results.values()]
+        return ServeReport(dict(self.results), self.ticks, wall, self.vertex_substeps,
+                           float(np.percentile(lat, 50)) if lat else 0.0, float(np.percentile(lat, 99)) if lat else 0.0,
+                           self.peak_bytes, self.peak_vertices, self.peak_sessions,
+                           float(np.mean(waits)) if waits else 0.0, unfinished)
+
+
+def run_alone(req: SessionRequest, substeps: int, iterations: int, static_triangles=None, lib=None) -> np.ndarray:
+    """The same session stepped by itself with ssbl_step_solver_ex (no batch)."""
+    solver = nb.NativeXpbdSolver(req.cloth, req.options, static_triangles, lib=lib)
+    try:
+        if req.wind is not None:
+            solver.update_force_fields([req.wind])
+        for f in range(req.frames):
+            if req.sphere is not None:
+                t = f * float(req.options.dt)
+                solver.update_runtime_colliders(replace(req.options, use_sphere=True, sphere_center=req.sphere.at(t),
+                                                        sphere_radius=req.sphere.radius))
+            solver.step(substeps, iterations, diagnostics=False, synchronize=False)
+        return solver.download_positions()
+    finally:
+        solver.close()
+
+
+# ------------------------------------------------------ served workload
+@lru_cache(maxsize=64)
+def _grid_template(side: int, spacing: float) -> ClothBuildData:
+    pos, tris = grid_mesh(side, (side - 1) * spacing)
+    pos[:, 1] += 1.5
+    pin = np.zeros(side * side, bool)
+    pin[side * (side - 1)] = True
+    pin[side * side - 1] = True
+    return build_cloth(y_up_to_z_up(pos), tris, pin)
+
+
+def session_cloth(side: int, seed: int, spacing: float = 0.01, jitter: float = 1.0e-4) -> ClothBuildData:
+    """A side x side grid (spacing 1 cm, 1.5 m up, two far corners pinned), seed-jittered start."""
+    tpl = _grid_template(int(side), float(spacing))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    start = (tpl.positions_world.astype(np.float64) + rng.normal(0.0, jitter, size=tpl.positions_world.shape)).astype(np.float32)
+    return replace(tpl, start_positions=start)
+
+
+def served_requests(*, seed: int = 0, rate: float = 200.0, window_s: float = 1.0, vmin: int = 1000, vmax: int = 250000,
+                    fmin: int = 60, fmax: int = 600, size_classes: int = 16, max_sessions: int | None = None,
+                    arrival_unit: str = "s") -> list[SessionRequest]:
+    """Poisson arrivals of grid-cloth sessions: sizes log-uniform in [vmin, vmax]
+    vertices (snapped to `size_classes` geometric grid sides so meshes can be
+    shared), frames uniform in [fmin, fmax], a wind field with a seeded
+    direction and strength U[0, 20] N, and a sphere on a seeded Lissajous path."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    smin, smax = math.sqrt(vmin), math.sqrt(vmax)
+    classes = np.unique(np.rint(np.geomspace(smin, smax, size_classes)).astype(int))
+    reqs, t, sid = [], 0.0, 0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > window_s or (max_sessions is not None and sid >= max_sessions):
+            break
+        v = math.exp(rng.uniform(math.log(vmin), math.log(vmax)))
+        side = int(classes[np.argmin(np.abs(classes - math.sqrt(v)))])
+        frames = int(rng.integers(fmin, fmax + 1))
+        d = rng.normal(size=3)
+        d[2] = abs(d[2]) * 0.25
+        wind = wind_field(d, float(rng.uniform(0.0, 20.0)))
+        half = 0.5 * (side - 1) * 0.01
+        sphere = SpherePath(center=(0.0, 0.0, 0.9), amplitude=(0.6 * half, 0.6 * half, 0.2),
+                            frequency=tuple(float(x) for x in rng.uniform(0.1, 0.5, size=3)),
+                            phase=tuple(float(x) for x in rng.uniform(0.0, 2.0 * math.pi, size=3)), radius=0.3)
+        opts = SolverOptions(use_ground=True, contact_friction=0.3, collision_margin=0.005, cloth_thickness=0.005)
+        reqs.append(SessionRequest(sid, t if arrival_unit == "s" else float(int(t)), session_cloth(side, seed * 100003 + sid),
+                                   opts, frames, wind, sphere))
+        sid += 1
+    return reqs

@@ -1,0 +1,70 @@
+"""Serving host logic on the CPU: admission against an HBM budget, continuous
+batching, and mix-vs-alone equality, run through the oracle library (which
+exports the same ABI and batch entry points)."""
+import math
+
+import numpy as np
+import pytest
+
+
+def small_requests(serving, scenes, sides, frames, arrivals):
+    reqs = []
+    for sid, (side, nf, t) in enumerate(zip(sides, frames, arrivals)):
+        cloth = serving.session_cloth(side, seed=sid, spacing=0.05)
+        opts = scenes.SolverOptions(use_ground=True, contact_friction=0.3, collision_margin=0.005, cloth_thickness=0.005)
+        wind = serving.wind_field((1.0, 0.3, 0.2), 5.0 + sid)
+        half = 0.5 * (side - 1) * 0.05
+        sphere = serving.SpherePath((0.0, 0.0, 1.2), (0.5 * half, 0.5 * half, 0.1), (0.3, 0.4, 0.2), (0.0, 1.0, 2.0), 0.2)
+        reqs.append(serving.SessionRequest(sid, float(t), cloth, opts, nf, wind, sphere))
+    return reqs
+
+
+@pytest.fixture(scope="module")
+def serving():
+    from ssbl_xpbd_amd import serving as s
+    return s
+
+
+def test_budget_admission_and_mix_equals_alone(serving, scenes, oracle):
+    reqs = small_requests(serving, scenes, sides=[6, 8, 7, 5], frames=[3, 2, 4, 2], arrivals=[0, 0, 0, 1])
+    est = [serving.estimate_device_bytes(r.cloth, r.options) for r in reqs]
+    budget = max(est[0] + est[1], est[2] + est[3]) + 1          # two sessions at a time
+    srv = serving.ClothServer(hbm_budget_bytes=budget, substeps=2, iterations=2, lib=oracle)
+    rep = srv.run(reqs, clock="tick")
+    assert rep.unfinished == 0 and len(rep.results) == 4
+    assert rep.peak_sessions == 2
+    assert rep.peak_device_bytes <= budget
+    # session 2 arrived at tick 0 but had to wait for memory
+    assert rep.results[2].admitted > rep.results[2].arrival
+    for r in reqs:
+        got = rep.results[r.sid]
+        assert got.frames == r.frames and len(got.frame_ms) == r.frames
+        alone = serving.run_alone(r, 2, 2, lib=oracle)
+        assert np.array_equal(got.positions, alone), r.sid
+
+
+def test_vertex_capacity_and_slots(serving, scenes, oracle):
+    reqs = small_requests(serving, scenes, sides=[6, 6, 6], frames=[2, 2, 2], arrivals=[0, 0, 0])
+    srv = serving.ClothServer(hbm_budget_bytes=1 << 40, substeps=1, iterations=1, max_batch_vertices=72, lib=oracle)
+    rep = srv.run(reqs, clock="tick")
+    assert rep.peak_active_vertices <= 72 and rep.peak_sessions == 2 and len(rep.results) == 3
+    srv = serving.ClothServer(hbm_budget_bytes=1 << 40, substeps=1, iterations=1, max_sessions=1, lib=oracle)
+    rep = srv.run(reqs, clock="tick")
+    assert rep.peak_sessions == 1 and rep.ticks == 6
+
+
+def test_oversized_session_is_rejected(serving, scenes, oracle):
+    (r,) = small_requests(serving, scenes, sides=[6], frames=[1], arrivals=[0])
+    srv = serving.ClothServer(hbm_budget_bytes=1000, substeps=1, iterations=1, lib=oracle)
+    with pytest.raises(ValueError):
+        srv.submit(r)
+
+
+def test_served_requests_follow_the_workload(serving):
+    reqs = serving.served_requests(seed=3, rate=200.0, window_s=0.25, vmin=1000, vmax=250000)
+    assert 20 <= len(reqs) <= 90                      # Poisson(50)
+    sides = sorted({math.isqrt(r.vertex_count) for r in reqs})
+    assert sides[0] >= 31 and sides[-1] <= 500
+    assert all(60 <= r.frames <= 600 for r in reqs)
+    assert all(0.0 <= r.wind["strength"] <= 20.0 for r in reqs)
+    assert all(a.arrival <= b.arrival for a, b in zip(reqs, reqs[1:]))
