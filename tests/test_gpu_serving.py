@@ -22,12 +22,13 @@ def test_mix_equals_alone_on_gpu(serving, scenes, hip):
     reqs = small_requests(serving, scenes, sides=[40, 100, 24, 64, 32, 80], frames=[8, 5, 12, 6, 10, 4],
                           arrivals=[0, 0, 1, 2, 2, 5])
     est = sorted(serving.estimate_device_bytes(r.cloth, r.options) for r in reqs)
-    budget = sum(est[-4:])                           # at most four sessions resident at once
+    budget = sum(est[-3:])        # the largest sessions cannot all be resident at once
     srv = serving.ClothServer(hbm_budget_bytes=budget, substeps=10, iterations=8, lib=hip)
     rep = srv.run(reqs, clock="tick")
     print("served:", rep.summary())
     assert rep.unfinished == 0 and len(rep.results) == len(reqs)
-    assert rep.peak_sessions <= 4 and rep.peak_device_bytes <= budget
+    assert rep.peak_device_bytes <= budget
+    assert rep.peak_sessions >= 3                   # sessions overlapped in one batch
     for r in reqs:
         alone = serving.run_alone(r, 10, 8, lib=hip)
         got = rep.results[r.sid].positions
