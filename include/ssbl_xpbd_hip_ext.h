@@ -54,6 +54,18 @@ SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int 
  * the members' state by that many Jacobi iterations: a measurement call. */
 SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms);
 
+/* Global dynamic scene from cloth handles, gathered on the device: particle k
+ * of handle h is that handle's vertex k at its current position with radius
+ * particle_radii[h]; triangles are each handle's mesh triangles at current
+ * positions; source ids and layers per handle.  The same scene as
+ * ssbl_update_global_dynamic_scene built from host copies of those arrays
+ * (session_manager.py:3242-3300 packs them on the host), with no host round
+ * trip: the gather and the hashes run on the library stream, ordered after
+ * the handles' queued steps.  Returns without waiting. */
+SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const* handles, const int* source_ids,
+                                                      const int* collision_layers, const float* particle_radii, int count,
+                                                      float collision_margin, float cloth_thickness);
+
 /* Device memory (bytes) the handle holds: state, constraint tables, SDF,
  * self-collision hashes and PCG arrays.  -1 for an invalid handle.  A
  * serving loop sums these against its HBM budget. */

@@ -48,6 +48,7 @@ namespace {
 
 thread_local std::string g_error;
 int g_precision_bits = 32;
+bool g_dynamic_hash = true;   // test hook: false makes every dynamic query a brute-force scan
 
 bool fail(const char* message) {
     g_error = message ? message : "oracle error";
@@ -353,6 +354,203 @@ template <class R> V3<R> eval_field(const SsblXpbdForceField& f, V3<R> p, V3<R> 
     }
 }
 
+// ------------------------------------------------------- dynamic colliders
+// Triangle soups and particles that other objects (other cloths through the
+// global dynamic scene, or animated colliders) present to a cloth, with their
+// spatial hashes: uploads A41:8319-8420 and 8575-8697, bounds and hashes
+// A41:3694-3932, 4325-4352 and 6220-6449, global scene A41:6597-6782.
+// Bucket contents are kept in ascending primitive order (the reference's
+// atomic fills leave that order to scheduling); a particle bucket shows its
+// first 16 particles (A41:4345-4350).
+constexpr int kParticleHashMinCount = 128;       // A41:25
+constexpr int kParticleBucketSlots = 16;         // A41:26
+constexpr int kMinParticleBuckets = 1024;        // A41:27
+constexpr int kTriangleHashMinCount = 64;        // A41:39
+constexpr int kMinTriangleBuckets = 1024;        // A41:40
+constexpr int kDynamicMaxHashCells = 64;         // A41:79
+constexpr float kNeighborImpulseScale = 0.5f;    // A41:129
+constexpr int kCrossOff = 0, kCrossLower = 2;    // A41:167-169
+enum DynCountSlot {
+    kCountDynParticleCandidates = 4, kCountDynParticleContacts = 5, kCountDynParticleOverflow = 6, kCountTrianglePairs = 7,
+    kCountDynTriCandidates = 34, kCountDynTriBucketOverflow = 35, kCountDynTriLarge = 36, kCountDynTriAabbRejects = 37
+};
+
+int next_pow2_int(int v) {
+    int r = 1;
+    while (r < v && r < (1 << 30)) r <<= 1;
+    return r;
+}
+
+int cell_coord_f(float value, float cell) { return cell_coord<float>(value, cell); }
+
+// A41:6590-6601
+float dynamic_triangle_cell_size(float max_extent, float margin, float thickness, bool fine) {
+    const float coarse = std::max(std::max(max_extent + margin * 2.0f, thickness), 1.0e-3f);
+    if (!fine) return coarse;
+    const float fine_cell = std::max(std::max(thickness * 1.5f, margin * 4.0f), 0.02f);
+    return std::max(std::min(coarse, fine_cell), 1.0e-3f);
+}
+
+struct DynamicSet {
+    // triangles (9 floats each)
+    int nt = 0;
+    std::vector<float> tv;
+    std::vector<int> tsrc, tlayer;           // empty: no source filter
+    float tri_cell = 0.0f, span_margin = 0.0f;
+    bool tri_hash = false;
+    int tri_buckets = 0;
+    std::vector<float> lo, hi, ctr, rad;     // 3T, 3T, 3T, T
+    std::vector<int> cmin, cmax;             // 3T (hashed sets only)
+    std::vector<int> tri_off, tri_idx;       // buckets + 1, entries ascending per bucket
+    std::vector<int> large;                  // ascending
+    int max_occupancy = 0;
+    // particles
+    int np = 0;
+    std::vector<float> pp, pprev, pr;        // 3P, 3P, P
+    std::vector<int> psrc, player;
+    std::vector<float> radii_seen;
+    float part_cell = 0.0f;
+    bool part_hash = false;
+    int part_buckets = 0;
+    std::vector<int> pcell;                  // 3P
+    std::vector<int> pcount, pidx;           // buckets, buckets * 16
+
+    // A41:8338-8420 (soup) and 6692-6722 (scene): cell size from the largest extent
+    void set_triangles(const float* t, int n, const int* src, const int* layer, float margin, float thickness, bool fine) {
+        nt = std::max(n, 0);
+        tv.assign(t, t + 9 * static_cast<size_t>(nt));
+        tsrc.clear();
+        tlayer.clear();
+        if (src) tsrc.assign(src, src + nt);
+        if (layer) tlayer.assign(layer, layer + nt);
+        float max_extent = 0.0f;
+        for (int k = 0; k < nt; ++k) {
+            const float* T = &tv[9 * static_cast<size_t>(k)];
+            const float ext = std::max(std::max(T[0], std::max(T[3], T[6])) - std::min(T[0], std::min(T[3], T[6])),
+                                       std::max(std::max(T[1], std::max(T[4], T[7])) - std::min(T[1], std::min(T[4], T[7])),
+                                                std::max(T[2], std::max(T[5], T[8])) - std::min(T[2], std::min(T[5], T[8]))));
+            if (std::isfinite(ext) && ext >= 0.0f) max_extent = std::max(max_extent, ext);
+        }
+        tri_cell = dynamic_triangle_cell_size(max_extent, std::max(margin, 0.0f), std::max(thickness, 0.0f), fine);
+        span_margin = std::max(margin, 0.0f);
+    }
+    // A41:8575-8697 (per cloth) and 6620-6668 (scene)
+    void set_particles(const float* p, const float* prev, const float* r, const int* src, const int* layer, int n, float margin,
+                       float thickness) {
+        n = std::max(n, 0);
+        std::vector<float> clean(n);
+        float max_radius = 0.0f;
+        for (int i = 0; i < n; ++i) {
+            float v = std::isfinite(r[i]) ? r[i] : 0.0f;
+            clean[i] = std::max(v, 0.0f);
+            max_radius = std::max(max_radius, clean[i]);
+        }
+        if (part_cell <= 0.0f || clean != radii_seen || np != n) {
+            part_cell = std::max(std::max(max_radius + std::max(margin, 0.0f), std::max(thickness, 0.0f)), 1.0e-3f);
+            radii_seen = clean;
+        }
+        const bool had_previous = np == n && !pp.empty();
+        std::vector<float> old = pp;
+        np = n;
+        pp.assign(p, p + 3 * static_cast<size_t>(n));
+        if (prev) pprev.assign(prev, prev + 3 * static_cast<size_t>(n));
+        else pprev = had_previous ? old : pp;
+        pr.assign(r, r + n);
+        psrc.clear();
+        player.clear();
+        if (src) psrc.assign(src, src + n);
+        if (layer) player.assign(layer, layer + n);
+    }
+    void clear_triangles() { nt = 0; tv.clear(); tsrc.clear(); tlayer.clear(); tri_hash = false; large.clear(); max_occupancy = 0; }
+    void clear_particles() { np = 0; pp.clear(); pprev.clear(); pr.clear(); psrc.clear(); player.clear(); part_hash = false; }
+
+    // A41:3694-3760 bounds, 3762-3932 hash (compact), 4325-4352 particle hash
+    void build(unsigned long long* counts) {
+        lo.assign(3 * static_cast<size_t>(nt), 0.0f);
+        hi = lo;
+        ctr = lo;
+        rad.assign(nt, 0.0f);
+        tri_hash = g_dynamic_hash && nt >= kTriangleHashMinCount;
+        cmin.clear();
+        cmax.clear();
+        large.clear();
+        max_occupancy = 0;
+        const float cs = std::max(tri_cell, 1.0e-3f);
+        if (tri_hash) {
+            cmin.assign(3 * static_cast<size_t>(nt), 0);
+            cmax = cmin;
+            tri_buckets = next_pow2_int(std::max(kMinTriangleBuckets, nt * 4));
+        }
+        std::vector<std::vector<int>> lists(tri_hash ? tri_buckets : 0);
+        for (int k = 0; k < nt; ++k) {
+            const float* T = &tv[9 * static_cast<size_t>(k)];
+            const V3<float> a = mk(T[0], T[1], T[2]), b = mk(T[3], T[4], T[5]), c = mk(T[6], T[7], T[8]);
+            const V3<float> mn = mk(std::min(a.x, std::min(b.x, c.x)), std::min(a.y, std::min(b.y, c.y)), std::min(a.z, std::min(b.z, c.z)));
+            const V3<float> mx = mk(std::max(a.x, std::max(b.x, c.x)), std::max(a.y, std::max(b.y, c.y)), std::max(a.z, std::max(b.z, c.z)));
+            const V3<float> cen = mul((a + b) + c, 1.0f / 3.0f);
+            const float r = std::sqrt(std::max(std::max(dot(a - cen, a - cen), dot(b - cen, b - cen)), dot(c - cen, c - cen)));
+            lo[3 * k] = mn.x; lo[3 * k + 1] = mn.y; lo[3 * k + 2] = mn.z;
+            hi[3 * k] = mx.x; hi[3 * k + 1] = mx.y; hi[3 * k + 2] = mx.z;
+            ctr[3 * k] = cen.x; ctr[3 * k + 1] = cen.y; ctr[3 * k + 2] = cen.z;
+            rad[k] = r;
+            if (!tri_hash) continue;
+            const float m = span_margin;
+            int x0 = cell_coord_f(mn.x - m, cs), y0 = cell_coord_f(mn.y - m, cs), z0 = cell_coord_f(mn.z - m, cs);
+            int x1 = cell_coord_f(mx.x + m, cs), y1 = cell_coord_f(mx.y + m, cs), z1 = cell_coord_f(mx.z + m, cs);
+            const long long sx = 1LL * x1 - x0 + 1, sy = 1LL * y1 - y0 + 1, sz = 1LL * z1 - z0 + 1;
+            long long cells = (sx > 0 && sy > 0 && sz > 0) ? sx * sy * sz : 0;
+            if (cells <= 0) {
+                x0 = x1 = cell_coord_f(cen.x, cs);
+                y0 = y1 = cell_coord_f(cen.y, cs);
+                z0 = z1 = cell_coord_f(cen.z, cs);
+                cells = 1;
+            }
+            cmin[3 * k] = x0; cmin[3 * k + 1] = y0; cmin[3 * k + 2] = z0;
+            cmax[3 * k] = x1; cmax[3 * k + 1] = y1; cmax[3 * k + 2] = z1;
+            if (cells > kDynamicMaxHashCells) {
+                counts[kCountDynTriLarge] += 1;
+                large.push_back(k);
+                continue;
+            }
+            for (int z = z0; z <= z1; ++z)
+                for (int y = y0; y <= y1; ++y)
+                    for (int x = x0; x <= x1; ++x) lists[hash_cell(x, y, z, tri_buckets)].push_back(k);
+        }
+        tri_off.assign(tri_hash ? tri_buckets + 1 : 0, 0);
+        tri_idx.clear();
+        for (size_t bk = 0; bk < lists.size(); ++bk) {
+            tri_off[bk] = static_cast<int>(tri_idx.size());
+            tri_idx.insert(tri_idx.end(), lists[bk].begin(), lists[bk].end());
+            max_occupancy = std::max(max_occupancy, static_cast<int>(lists[bk].size()));
+        }
+        if (tri_hash) tri_off[tri_buckets] = static_cast<int>(tri_idx.size());
+        // particles
+        part_hash = g_dynamic_hash && np >= kParticleHashMinCount;
+        if (!part_hash) return;
+        part_buckets = next_pow2_int(std::max(kMinParticleBuckets, np * 4));
+        pcell.assign(3 * static_cast<size_t>(np), 0);
+        pcount.assign(part_buckets, 0);
+        pidx.assign(static_cast<size_t>(part_buckets) * kParticleBucketSlots, -1);
+        for (int i = 0; i < np; ++i) {
+            const int cx = cell_coord_f(pp[3 * i], part_cell), cy = cell_coord_f(pp[3 * i + 1], part_cell),
+                      cz = cell_coord_f(pp[3 * i + 2], part_cell);
+            pcell[3 * i] = cx; pcell[3 * i + 1] = cy; pcell[3 * i + 2] = cz;
+            const unsigned bk = hash_cell(cx, cy, cz, part_buckets);
+            const int slot = pcount[bk]++;
+            if (slot < kParticleBucketSlots) pidx[bk * kParticleBucketSlots + slot] = i;
+            else counts[kCountDynParticleOverflow] += 1;
+        }
+    }
+};
+
+// A global dynamic scene plus the counters its builds write (A41:309-340).
+struct Scene {
+    DynamicSet set;
+    float margin = 0.0f, thickness = 0.0f;
+    unsigned long long counts[kCountSlots] = {};
+    long long hash_overflow = 0;
+};
+
 // ------------------------------------------------------------------ solver
 struct Pin { int index; float target[3]; float weight; };
 
@@ -404,6 +602,12 @@ template <class R> struct Solver {
     float max_smoothed = 0.0f;
     unsigned long long counts[kCountSlots] = {};
     SsblXpbdDiagnostics diag{};
+    // dynamic colliders: the cloth's own, or an attached global scene
+    DynamicSet dyn;
+    bool dyn_tri_allocated = false;
+    Scene* scene = nullptr;
+    int target_src = 0, target_layer = 0, cross_mode = kCrossOff;
+    bool use_scene() const { return scene != nullptr && cross_mode != kCrossOff; }
 
     void count(int slot, unsigned long long n = 1) { counts[slot] += n; }
     bool w_zero(int v) const { return pinned[v] != 0; }
@@ -428,6 +632,7 @@ template <class R> struct Solver {
     void static_sdf_contacts(R dt);
     void velocity_update(R dt);
     void self_collision();
+    void dynamic_contacts(R dt);
     bool contact_response(Vec n, R proxy, Vec* vel_io);
     // helpers
     Vec normal_at(int v, const std::vector<Vec>& P, int* has) const;
@@ -1921,6 +2126,133 @@ template <class R> void Solver<R>::final_caps(R dt) {
 }
 
 // A41:9087-9398
+// A41:4251-4323 (triangles) then A41:4415-4511 (particles), unsigned
+// contacts: the deepest contact of each kind pushes the vertex by half its
+// depth (A41:129).  Only the vertex's own position is read and written.
+template <class R> void Solver<R>::dynamic_contacts(R dt) {
+    const DynamicSet& D = use_scene() ? scene->set : dyn;
+    const int mode = use_scene() ? cross_mode : kCrossOff;
+    auto enabled = [&](int src, int layer) {   // A41:3984-3997
+        if (mode == kCrossOff) return true;
+        if (target_src != 0 && src == target_src) return false;
+        if (mode == kCrossLower) return layer < target_layer;
+        return true;
+    };
+    const R eps = R(kEpsF);
+    const R cm = std::max(R(cfg.collision_margin), R(0));
+    if (D.nt > 0 && cm > R(0)) {
+        for (int i = 0; i < V(); ++i) {
+            if (w_zero(i)) continue;
+            const Vec p = pos[i];
+            Vec best_delta = mk<R>(0, 0, 0);
+            R best_pen = R(0);
+            unsigned long long cand = 0, rej = 0;
+            auto consider = [&](int t) {   // A41:3999-4113
+                if (t < 0 || t >= D.nt) return;
+                if (!D.tsrc.empty() && !enabled(D.tsrc[t], D.tlayer.empty() ? 0 : D.tlayer[t])) return;
+                ++cand;
+                const Vec mn = mk<R>(D.lo[3 * t], D.lo[3 * t + 1], D.lo[3 * t + 2]);
+                const Vec mx = mk<R>(D.hi[3 * t], D.hi[3 * t + 1], D.hi[3 * t + 2]);
+                const Vec q0 = mk(std::min(std::max(p.x, mn.x), mx.x), std::min(std::max(p.y, mn.y), mx.y), std::min(std::max(p.z, mn.z), mx.z));
+                const Vec da = p - q0;
+                if (dot(da, da) > cm * cm) { ++rej; return; }
+                const Vec cen = mk<R>(D.ctr[3 * t], D.ctr[3 * t + 1], D.ctr[3 * t + 2]);
+                const R rr = R(D.rad[t]) + cm;
+                const Vec dc = p - cen;
+                if (dot(dc, dc) > rr * rr) { ++rej; return; }
+                const float* T = &D.tv[9 * static_cast<size_t>(t)];
+                const Vec a = mk<R>(T[0], T[1], T[2]), b = mk<R>(T[3], T[4], T[5]), c = mk<R>(T[6], T[7], T[8]);
+                const Vec q = closest_point_on_triangle(p, a, b, c);
+                const Vec d = p - q;
+                const R dsq = dot(d, d);
+                const R dist = std::sqrt(std::max(dsq, eps));
+                const Vec tn = cross(b - a, c - a);
+                const R tl = std::sqrt(std::max(dot(tn, tn), eps));
+                const Vec tnorm = mul(tn, R(1) / tl);
+                const Vec n = dsq > eps * R(4) ? mul(d, R(1) / std::max(dist, eps)) : tnorm;
+                const R pen = cm - dist;
+                if (dist >= cm) return;
+                if (pen > best_pen) { best_pen = pen; best_delta = mul(n, pen); }
+            };
+            if (D.tri_hash) {   // A41:4115-4249, own cell only (unsigned query radius 0)
+                const int cx = cell_coord<R>(p.x, R(D.tri_cell)), cy = cell_coord<R>(p.y, R(D.tri_cell)), cz = cell_coord<R>(p.z, R(D.tri_cell));
+                const unsigned bk = hash_cell(cx, cy, cz, D.tri_buckets);
+                for (int k = D.tri_off[bk]; k < D.tri_off[bk + 1]; ++k) {
+                    const int t = D.tri_idx[k];
+                    if (cx < D.cmin[3 * t] || cx > D.cmax[3 * t] || cy < D.cmin[3 * t + 1] || cy > D.cmax[3 * t + 1] ||
+                        cz < D.cmin[3 * t + 2] || cz > D.cmax[3 * t + 2])
+                        continue;   // A41:3934-3982: the query cell is not this triangle's canonical cell
+                    consider(t);
+                }
+                for (int t : D.large) consider(t);
+            } else {
+                for (int t = 0; t < D.nt; ++t) consider(t);
+            }
+            count(kCountDynTriCandidates, cand);
+            count(kCountDynTriAabbRejects, rej);
+            if (best_pen > R(0)) {
+                count(kCountTrianglePairs);
+                const Vec applied = mul(best_delta, R(kNeighborImpulseScale));
+                pos[i] = p + applied;
+                const R vn = dot(vel[i], applied);
+                if (vn < R(0)) {
+                    const R lsq = std::max(dot(applied, applied), eps);
+                    vel[i] = vel[i] - mul(applied, R(kNeighborImpulseScale) * vn / lsq);
+                }
+                prev[i] = pos[i] - mul(vel[i], std::max(dt, eps));
+                count(kCountExactImpulse);
+            }
+        }
+    }
+    if (D.np > 0) {
+        const R margin = std::max(R(cfg.collision_margin), R(0));
+        for (int i = 0; i < V(); ++i) {
+            if (w_zero(i)) continue;
+            const Vec p = pos[i];
+            Vec best_delta = mk<R>(0, 0, 0);
+            R best_pen = R(0);
+            auto consider = [&](int j) {   // A41:4354-4413
+                if (j < 0 || j >= D.np) return;
+                if (!D.psrc.empty() && !enabled(D.psrc[j], D.player.empty() ? 0 : D.player[j])) return;
+                const Vec cen = mk<R>(D.pp[3 * j], D.pp[3 * j + 1], D.pp[3 * j + 2]);
+                const R radius = R(std::fmax(D.pr[j], 0.0f)) + margin;
+                if (radius <= R(0)) return;
+                const Vec d = p - cen;
+                const R dsq = dot(d, d);
+                const R dist = std::sqrt(std::max(dsq, eps));
+                if (dist >= radius) return;
+                count(kCountDynParticleCandidates);
+                const Vec n = dist > eps ? mul(d, R(1) / dist) : mk<R>(0, 0, 1);
+                const R pen = radius - dist;
+                if (pen > best_pen) { best_pen = pen; best_delta = mul(n, pen); }
+            };
+            if (D.part_hash) {
+                const int cx = cell_coord<R>(p.x, R(D.part_cell)), cy = cell_coord<R>(p.y, R(D.part_cell)),
+                          cz = cell_coord<R>(p.z, R(D.part_cell));
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
+                            const unsigned bk = hash_cell(qx, qy, qz, D.part_buckets);
+                            const int lim = std::min(D.pcount[bk], kParticleBucketSlots);
+                            for (int k = 0; k < lim; ++k) {
+                                const int j = D.pidx[bk * kParticleBucketSlots + k];
+                                if (D.pcell[3 * j] != qx || D.pcell[3 * j + 1] != qy || D.pcell[3 * j + 2] != qz) continue;
+                                consider(j);
+                            }
+                        }
+            } else {
+                for (int j = 0; j < D.np; ++j) consider(j);
+            }
+            if (best_pen > R(0)) {
+                pos[i] = p + mul(best_delta, R(kNeighborImpulseScale));
+                count(kCountExactImpulse);
+                count(kCountDynParticleContacts);
+            }
+        }
+    }
+}
+
 template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch) {
     auto t0 = std::chrono::steady_clock::now();
     diag = SsblXpbdDiagnostics{};
@@ -1929,6 +2261,16 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
     diag.force_field_count = static_cast<long long>(fields.size());
     diag.unsupported_force_field_count = unsupported_fields;
     std::memset(counts, 0, sizeof(counts));
+    diag.dynamic_triangle_count = use_scene() ? scene->set.nt : dyn.nt;   // A41:9103-9122
+    diag.dynamic_particle_count = use_scene() ? scene->set.np : dyn.np;
+    if (use_scene()) {
+        diag.global_dynamic_particle_count = scene->set.np;
+        diag.global_dynamic_triangle_count = scene->set.nt;
+        diag.global_dynamic_hash_overflow = scene->hash_overflow;
+    } else {
+        dyn.build(counts);                                                   // A41:9129-9136
+    }
+    const bool dynamic_on = use_scene() ? (scene->set.nt > 0 || scene->set.np > 0) : (dyn.nt > 0 || dyn.np > 0);
     max_smoothed = 0.0f;
     pcg_guard = 0;
     pcg_runs = 0;
@@ -1963,6 +2305,7 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
             if (!pins.empty()) pin_project(exponent, sub_dt);
             if (analytic_on) analytic(sub_dt);
             if (sdf_ready) static_sdf_contacts(sub_dt);
+            if (last_it && dynamic_on) dynamic_contacts(sub_dt);                 // A41:9230-9252
             const int interval = std::max(cfg.self_collision_interval, 1);
             const bool self_it = cfg.self_collision && last_it;
             const bool scheduled = self_it && (((s + 1) % interval) == 0 || s == substeps - 1);
@@ -1988,7 +2331,25 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
         diag.external_friction_corrections = static_cast<long long>(counts[kCountFriction]);
         diag.self_candidate_count = static_cast<long long>(counts[kCountSelfCandidates]);
         diag.static_sdf_contact_count = static_cast<long long>(counts[kCountStaticSdf]);
-        diag.candidate_count = static_cast<long long>(counts[kCountSelfCandidates] + counts[kCountStaticSdf]);
+        diag.candidate_count = static_cast<long long>(counts[kCountDynParticleCandidates] + counts[kCountDynTriCandidates] +
+                                                      counts[kCountSelfCandidates] + counts[kCountStaticSdf]);
+        // A41:8078-8110
+        diag.dynamic_particle_candidate_count = static_cast<long long>(counts[kCountDynParticleCandidates]);
+        diag.dynamic_particle_contacts = static_cast<long long>(counts[kCountDynParticleContacts]);
+        diag.dynamic_particle_overflow = 0;
+        diag.dynamic_triangle_candidate_count = static_cast<long long>(counts[kCountDynTriCandidates]);
+        diag.dynamic_triangle_bucket_overflow = static_cast<long long>(counts[kCountDynTriBucketOverflow]);
+        diag.dynamic_triangle_large_primitive_count = static_cast<long long>(counts[kCountDynTriLarge]);
+        diag.dynamic_triangle_aabb_reject_count = static_cast<long long>(counts[kCountDynTriAabbRejects]);
+        diag.dynamic_triangle_max_bucket_occupancy = 0;
+        if (dyn_tri_allocated) {
+            diag.dynamic_triangle_large_primitive_count =
+                std::max(diag.dynamic_triangle_large_primitive_count, static_cast<long long>(dyn.large.size()));
+            diag.dynamic_triangle_max_bucket_occupancy = dyn.max_occupancy;
+        }
+        diag.fast_triangle_pair_candidates = diag.dynamic_triangle_candidate_count;
+        diag.fast_triangle_pair_contacts = static_cast<long long>(counts[kCountTrianglePairs]);
+        diag.fast_triangle_pair_skipped_rest = diag.dynamic_triangle_aabb_reject_count;
         diag.abi41_lra_tack_count = static_cast<long long>(counts[kCountLraTacks]);
         diag.abi41_bending_wing_count = counts[kCountBendingWings] > 0 ? static_cast<long long>(counts[kCountBendingWings]) : wing_count;
         diag.abi41_tack_jitter_guarded = static_cast<long long>(counts[kCountTackGuards]);
@@ -2066,11 +2427,36 @@ template <class S> void copy_out(const S& s, const decltype(s.pos)& src, float* 
     }
 }
 
+// A41:8338-8420 (soup) and 8575-8697 (particles) for the cloth's own colliders
+template <class S> bool set_dynamic_triangles(S& s, const float* t, int count) {
+    count = std::max(count, 0);
+    if (count == 0) { s.dyn.clear_triangles(); return true; }
+    if (!t) return fail("dynamic triangle data is required when triangle count is nonzero");
+    s.dyn.set_triangles(t, count, nullptr, nullptr, s.cfg.collision_margin, s.cfg.cloth_thickness, false);
+    s.dyn_tri_allocated = true;
+    return true;
+}
+template <class S> bool set_dynamic_particles(S& s, const float* p, const float* r, int count) {
+    count = std::max(count, 0);
+    if (count == 0) { s.dyn.clear_particles(); return true; }
+    if (!p || !r) return fail("dynamic particle position/radius data is required when particle count is nonzero");
+    s.dyn.set_particles(p, nullptr, r, nullptr, nullptr, count, s.cfg.collision_margin, s.cfg.cloth_thickness);
+    return true;
+}
+
 struct Batch { std::vector<void*> handles; };
 
 }  // namespace
 
 extern "C" {
+
+// Test hook: 0 disables the dynamic collider hashes (brute-force queries over
+// every triangle and particle, the reference's small-set path) for sets built
+// afterwards.
+SSBL_API int ssbl_oracle_dynamic_hash(int enable) {
+    g_dynamic_hash = enable != 0;
+    return 1;
+}
 
 SSBL_API int ssbl_oracle_set_precision(int bits) {
     if (bits != 32 && bits != 64) return fail("precision must be 32 or 64") ? 1 : 0;
@@ -2143,22 +2529,22 @@ SSBL_API int ssbl_update_static_triangles(void* handle, const float* t, int coun
 
 SSBL_API int ssbl_update_dynamic_triangles(void* handle, const float* t, int count) {
     g_error.clear();
-    (void)t;
     if (!handle) return fail("invalid dynamic triangle update") ? 1 : 0;
-    if (count > 0) return fail("oracle: dynamic triangle colliders are not restated") ? 1 : 0;
-    return 1;
+    return with(handle, [&](auto& s) { return set_dynamic_triangles(s, t, count) ? 1 : 0; });
 }
 
-SSBL_API int ssbl_update_frame_inputs(void* handle, const SsblXpbdFrameInputs* in) {
+SSBL_API int ssbl_update_frame_inputs(void* handle, const SsblXpbdFrameInputs* in) {  // A41:9035-9085
     g_error.clear();
     if (!handle || !in) return fail("invalid frame input update") ? 1 : 0;
     return with(handle, [&](auto& s) {
         if (in->update_pin_targets && !upload_pins(s, in->pin_indices, in->pin_positions, in->pin_weights, in->pin_count)) return 0;
         if (in->update_runtime_colliders) s.colliders = in->runtime_colliders;
-        if ((in->update_dynamic_triangles && in->dynamic_triangle_count > 0) ||
-            (in->update_dynamic_indexed_triangles && in->dynamic_indexed_triangle_count > 0) ||
-            (in->update_dynamic_particles && in->dynamic_particle_count > 0))
-            return fail("oracle: dynamic colliders are not restated") ? 1 : 0;
+        if (in->update_dynamic_triangles && !set_dynamic_triangles(s, in->dynamic_triangles, in->dynamic_triangle_count)) return 0;
+        if (in->update_dynamic_indexed_triangles && in->dynamic_indexed_triangle_count > 0 && in->dynamic_triangle_vertex_count > 0)
+            return fail("oracle: signed indexed dynamic triangles are not restated") ? 1 : 0;
+        if (in->update_dynamic_particles &&
+            !set_dynamic_particles(s, in->dynamic_particle_positions, in->dynamic_particle_radii, in->dynamic_particle_count))
+            return 0;
         if (in->update_static_triangles) {
             int c = std::max(in->static_triangle_count, 0);
             if (c > 0 && !in->static_triangles) return fail("static triangle data is required when triangle count is nonzero") ? 1 : 0;
@@ -2195,26 +2581,59 @@ SSBL_API int ssbl_get_diagnostics(void* handle, SsblXpbdDiagnostics* out) {
 }
 
 SSBL_API unsigned int ssbl_capabilities(void) {
-    return SSBL_CAP_STRETCH_OPTIMIZATION | SSBL_CAP_PIN_WEIGHTS | SSBL_CAP_HIP_BATCHED_STEP;
+    return SSBL_CAP_STRETCH_OPTIMIZATION | SSBL_CAP_PIN_WEIGHTS | SSBL_CAP_GLOBAL_DYNAMIC_SCENE | SSBL_CAP_HIP_BATCHED_STEP;
 }
 
-SSBL_API void* ssbl_create_global_dynamic_scene(void) { g_error.clear(); return new int(0); }
-SSBL_API int ssbl_destroy_global_dynamic_scene(void* scene) { g_error.clear(); delete static_cast<int*>(scene); return 1; }
-SSBL_API int ssbl_update_global_dynamic_scene(void* scene, const SsblXpbdGlobalDynamicSceneInputs* in) {
+SSBL_API void* ssbl_create_global_dynamic_scene(void) { g_error.clear(); return new Scene(); }
+SSBL_API int ssbl_destroy_global_dynamic_scene(void* scene) {
     g_error.clear();
-    if (!scene || !in) return fail("invalid global dynamic scene update") ? 1 : 0;
-    if (in->particle_count > 0 || in->triangle_count > 0) return fail("oracle: global dynamic scene is not restated") ? 1 : 0;
+    if (!scene) return fail("invalid global dynamic scene handle") ? 1 : 0;
+    delete static_cast<Scene*>(scene);
     return 1;
 }
-SSBL_API int ssbl_attach_global_dynamic_scene(void* handle, void* scene, int, int, int) {
+SSBL_API int ssbl_update_global_dynamic_scene(void* handle, const SsblXpbdGlobalDynamicSceneInputs* in) {  // A41:6597-6782
+    g_error.clear();
+    Scene* sc = static_cast<Scene*>(handle);
+    if (!sc || !in) return fail("invalid global dynamic scene update") ? 1 : 0;
+    const int np = std::max(in->particle_count, 0), nt = std::max(in->triangle_count, 0);
+    if (np > 0 && (!in->particle_positions || !in->particle_radii || !in->particle_source_ids || !in->particle_collision_layers))
+        return fail("global dynamic particle arrays are incomplete") ? 1 : 0;
+    if (nt > 0 && (!in->triangle_vertices || !in->triangle_source_ids || !in->triangle_collision_layers))
+        return fail("global dynamic triangle arrays are incomplete") ? 1 : 0;
+    sc->margin = std::max(in->collision_margin, 0.0f);
+    sc->thickness = std::max(in->cloth_thickness, 0.0f);
+    std::memset(sc->counts, 0, sizeof(sc->counts));
+    if (np == 0) sc->set.clear_particles();
+    else
+        sc->set.set_particles(in->particle_positions, in->particle_prev_positions ? in->particle_prev_positions : in->particle_positions,
+                              in->particle_radii, in->particle_source_ids, in->particle_collision_layers, np, sc->margin, sc->thickness);
+    if (nt == 0) sc->set.clear_triangles();
+    else sc->set.set_triangles(in->triangle_vertices, nt, in->triangle_source_ids, in->triangle_collision_layers, sc->margin, sc->thickness, true);
+    sc->set.build(sc->counts);
+    sc->hash_overflow = static_cast<long long>(sc->counts[kCountDynParticleOverflow] + sc->counts[kCountDynTriBucketOverflow]);
+    return 1;
+}
+SSBL_API int ssbl_attach_global_dynamic_scene(void* handle, void* scene, int target_source_id, int target_layer, int mode) {
     g_error.clear();
     if (!handle || !scene) return fail("invalid global dynamic scene attach") ? 1 : 0;
-    return fail("oracle: global dynamic scene is not restated") ? 1 : 0;
+    return with(handle, [&](auto& s) {
+        s.scene = static_cast<Scene*>(scene);
+        s.target_src = target_source_id;
+        s.target_layer = target_layer;
+        s.cross_mode = mode;
+        return 1;
+    });
 }
 SSBL_API int ssbl_clear_global_dynamic_scene(void* handle) {
     g_error.clear();
     if (!handle) return fail("invalid global dynamic scene clear") ? 1 : 0;
-    return 1;
+    return with(handle, [&](auto& s) {
+        s.scene = nullptr;
+        s.target_src = 0;
+        s.target_layer = 0;
+        s.cross_mode = kCrossOff;
+        return 1;
+    });
 }
 SSBL_API const char* ssbl_last_error(void) { return g_error.c_str(); }
 
@@ -2301,6 +2720,54 @@ SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int 
     for (void* h : static_cast<Batch*>(batch)->handles)
         if (!ssbl_step_solver_ex(h, substeps, iterations, 0, force_sync)) return 0;
     return 1;
+}
+SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const* handles, const int* source_ids, const int* layers,
+                                                      const float* radii, int count, float margin, float thickness) {
+    // the host packing of session_manager.py:3242-3300 from each handle's positions
+    g_error.clear();
+    if (!scene || count < 0 || (count > 0 && (!handles || !source_ids || !layers || !radii))) return fail("invalid global scene gather") ? 1 : 0;
+    std::vector<float> pos, tv, pr;
+    std::vector<int> psrc, play, tsrc, tlay;
+    for (int k = 0; k < count; ++k) {
+        if (!handles[k]) return fail("invalid solver handle in global scene gather") ? 1 : 0;
+        with(handles[k], [&](auto& s) {
+            const int n = s.cfg.vertex_count;
+            for (int i = 0; i < n; ++i) {
+                pos.push_back(static_cast<float>(s.pos[i].x));
+                pos.push_back(static_cast<float>(s.pos[i].y));
+                pos.push_back(static_cast<float>(s.pos[i].z));
+                pr.push_back(radii[k]);
+                psrc.push_back(source_ids[k]);
+                play.push_back(layers[k]);
+            }
+            for (int t = 0; t < s.cfg.triangle_count; ++t) {
+                const int ia = s.tris[3 * t], ib = s.tris[3 * t + 1], ic = s.tris[3 * t + 2];
+                const bool ok = ia >= 0 && ib >= 0 && ic >= 0 && ia < n && ib < n && ic < n;
+                for (int v : {ia, ib, ic}) {
+                    tv.push_back(ok ? static_cast<float>(s.pos[v].x) : 0.0f);
+                    tv.push_back(ok ? static_cast<float>(s.pos[v].y) : 0.0f);
+                    tv.push_back(ok ? static_cast<float>(s.pos[v].z) : 0.0f);
+                }
+                tsrc.push_back(source_ids[k]);
+                tlay.push_back(layers[k]);
+            }
+            return 0;
+        });
+    }
+    SsblXpbdGlobalDynamicSceneInputs in{};
+    in.particle_positions = pos.data();
+    in.particle_prev_positions = nullptr;
+    in.particle_radii = pr.data();
+    in.particle_source_ids = psrc.data();
+    in.particle_collision_layers = play.data();
+    in.particle_count = static_cast<int>(pr.size());
+    in.triangle_vertices = tv.data();
+    in.triangle_source_ids = tsrc.data();
+    in.triangle_collision_layers = tlay.data();
+    in.triangle_count = static_cast<int>(tsrc.size());
+    in.collision_margin = margin;
+    in.cloth_thickness = thickness;
+    return ssbl_update_global_dynamic_scene(scene, &in);
 }
 SSBL_API long long ssbl_ext_device_bytes(void*) { fail("oracle: no device memory"); return -1; }
 SSBL_API int ssbl_ext_time_batch_pass(void*, int, float*) { return fail("oracle: no device timing") ? 1 : 0; }

@@ -13,7 +13,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SOURCES = ["csrc/xpbd_kernels.hip", "csrc/xpbd_self.hip", "csrc/xpbd_pcg.hip", "csrc/xpbd_solver.hip"]
+SOURCES = ["csrc/xpbd_kernels.hip", "csrc/xpbd_self.hip", "csrc/xpbd_pcg.hip", "csrc/xpbd_dynamic.hip", "csrc/xpbd_solver.hip"]
 OUTPUT = PKG / "libssbl_xpbd_hip_abi41.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

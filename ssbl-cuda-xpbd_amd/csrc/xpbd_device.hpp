@@ -49,7 +49,10 @@ enum PassOps : unsigned {
     kOpCount = 1u << 7,       // count diagnostics
     kOpLastIter = 1u << 8,    // last iteration of the substep (pin polish always runs)
     kOpBend = 1u << 9,        // uncoloured bend gather (src -> dst), instead of stretch
+    kOpDynamic = 1u << 10,    // dynamic triangle and particle contacts (last iteration)
 };
+
+struct DynView;
 
 // Everything a kernel needs to know about one cloth.  Lives in device memory
 // (a table of these for a batch); host keeps a mirror.
@@ -118,6 +121,13 @@ struct ClothView {
     int ell_pitch;
     int ell_overflow;
     int pad3;
+    // Dynamic colliders (null when none): the cloth's own, or an attached
+    // global scene filtered by source and layer (A41:3984-3997).
+    const DynView* dyn;
+    int dyn_mode;                 // 0 no filter, 1 every other source, 2 lower layers only
+    int dyn_target_src;
+    int dyn_target_layer;
+    int pad4;
 };
 
 struct Tile { int cloth, begin, end, pad; };
@@ -136,6 +146,10 @@ void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks
 void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float ox, float oy, float oz, float voxel,
                       const float* tris, int ntris, int signed_sdf);
 void launch_fill_u32(hipStream_t s, unsigned* p, unsigned value, size_t n);
+// Exclusive prefix sum of n ints into out[0..n] (out[n] = total), on the
+// stream; tmp holds scan_tmp_ints(n) ints.
+void exclusive_scan(hipStream_t s, const int* in, int n, int* out, int* tmp);
+int scan_tmp_ints(int n);
 
 // Device allocations of the library go through these, so the bytes a solver
 // holds can be reported (ssbl_ext_device_bytes).

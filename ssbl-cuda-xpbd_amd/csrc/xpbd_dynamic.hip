@@ -1,0 +1,529 @@
+// xpbd_dynamic.hip -- dynamic collider sets on gfx950: uploads, device
+// gathers of a scene from cloth handles, bounds and the two hashes.
+//
+// Reference (native/src/ssbl_xpbd_cuda_abi41.cu, A41:<line>): bounds
+// A41:3694-3760, triangle hash A41:3762-3932 with the >64-cell list, compact
+// offsets A41:6354-6390 (a host scan in the reference; a device scan here),
+// particle hash A41:4325-4352, cell sizes A41:6590-6601 and 8619-8627.
+#include "xpbd_dynamic.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "xpbd_math.hpp"
+#include "xpbd_timing.hpp"
+
+namespace ssbl {
+
+bool fail(const std::string& message);
+bool hip_ok(hipError_t err, const char* what);
+bool sync_stream();
+
+namespace {
+
+unsigned grid_for(long long n, int block) { return static_cast<unsigned>(std::max<long long>((n + block - 1) / block, 1)); }
+
+int next_pow2(int v) {
+    int r = 1;
+    while (r < v && r < (1 << 30)) r <<= 1;
+    return r;
+}
+
+__device__ __host__ inline float tri_cell_size(float max_extent, float margin, float thickness, bool fine) {  // A41:6590-6601
+    const float coarse = fmaxf(fmaxf(max_extent + margin * 2.0f, thickness), 1.0e-3f);
+    if (!fine) return coarse;
+    const float fine_cell = fmaxf(fmaxf(thickness * 1.5f, margin * 4.0f), 0.02f);
+    return fmaxf(fminf(coarse, fine_cell), 1.0e-3f);
+}
+
+__global__ void k_set_view(DynView* dst, DynView v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = v;
+}
+
+__global__ void k_fill_i32(int* p, int value, int n) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = value;
+}
+
+// A41:3694-3760: AABB, centre, bounding radius and (hashed sets) the cell
+// span with the margin; a span of more than 64 cells goes to the large list.
+__global__ __launch_bounds__(256) void k_bounds(const DynView* __restrict__ dv, float span_margin, float4* lo, float4* hi,
+                                                float4* ctr, int4* cmin, int4* cmax, int* large_flag,
+                                                unsigned long long* counts, int count_diag) {
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int nt = dv->nt;
+    bool large = false;
+    if (t < nt) {
+        const F3 a = f3(dv->tv[3 * t]), b = f3(dv->tv[3 * t + 1]), c = f3(dv->tv[3 * t + 2]);
+        const F3 mn = f3(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)));
+        const F3 mx = f3(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)));
+        const F3 cen = mul((a + b) + c, 1.0f / 3.0f);
+        const float r = sqrtf(fmaxf(fmaxf(dot(a - cen, a - cen), dot(b - cen, b - cen)), dot(c - cen, c - cen)));
+        lo[t] = with_w(mn, 0.0f);
+        hi[t] = with_w(mx, 0.0f);
+        ctr[t] = with_w(cen, r);
+        if (dv->tri_hash) {
+            const float cs = fmaxf(dv->tri_cell, 1.0e-3f), m = span_margin;
+            int x0 = cell_coord(mn.x - m, cs), y0 = cell_coord(mn.y - m, cs), z0 = cell_coord(mn.z - m, cs);
+            int x1 = cell_coord(mx.x + m, cs), y1 = cell_coord(mx.y + m, cs), z1 = cell_coord(mx.z + m, cs);
+            const long long sx = 1LL * x1 - x0 + 1, sy = 1LL * y1 - y0 + 1, sz = 1LL * z1 - z0 + 1;
+            long long cells = (sx > 0 && sy > 0 && sz > 0) ? sx * sy * sz : 0;
+            if (cells <= 0) {
+                x0 = x1 = cell_coord(cen.x, cs);
+                y0 = y1 = cell_coord(cen.y, cs);
+                z0 = z1 = cell_coord(cen.z, cs);
+                cells = 1;
+            }
+            cmin[t] = make_int4(x0, y0, z0, 0);
+            cmax[t] = make_int4(x1, y1, z1, 0);
+            large = cells > kDynMaxHashCells;
+        }
+        large_flag[t] = large ? 1 : 0;
+    }
+    if (count_diag) wave_count(counts, kCountDynTriLarge, large);
+}
+
+__global__ void k_large_scatter(int nt, const int* __restrict__ flag, const int* __restrict__ off, int* large) {
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t < nt && flag[t]) large[off[t]] = t;
+}
+
+// Cell `idx` of triangle t's span (z outer, then y, then x: the reference's
+// loop order); false past the span.
+__device__ __forceinline__ bool span_cell(int4 a, int4 b, int idx, int* x, int* y, int* z) {
+    const int sx = b.x - a.x + 1, sy = b.y - a.y + 1, sz = b.z - a.z + 1;
+    if (sx <= 0 || sy <= 0 || sz <= 0 || idx >= sx * sy * sz) return false;
+    *x = a.x + idx % sx;
+    *y = a.y + (idx / sx) % sy;
+    *z = a.z + idx / (sx * sy);
+    return true;
+}
+
+// One wave per triangle, one lane per cell.  phase 0 counts (keeping the
+// arrival slot), 1 scatters keys to raw[off[b] + slot].
+__global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv, const int4* __restrict__ cmin,
+                                                  const int4* __restrict__ cmax, const int* __restrict__ large_flag, int* count,
+                                                  const int* __restrict__ off, int* slot, int* raw, int* rawb, int* occupancy,
+                                                  int phase) {
+    const int t = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int idx = static_cast<int>(threadIdx.x & 63u);
+    if (t >= dv->nt || large_flag[t]) return;
+    int x, y, z;
+    if (!span_cell(cmin[t], cmax[t], idx, &x, &y, &z)) return;
+    const int b = static_cast<int>(hash_cell(x, y, z, dv->tri_buckets));
+    const int key = t * kDynMaxHashCells + idx;
+    if (phase == 0) {
+        const int s = atomicAdd(&count[b], 1);
+        slot[key] = s;
+        atomicMax(occupancy, s + 1);
+    } else {
+        const int j = off[b] + slot[key];
+        raw[j] = key;
+        rawb[j] = b;
+    }
+}
+
+// Every scattered key lands at its rank among its bucket's keys.
+__global__ __launch_bounds__(256) void k_tri_rank(const int* __restrict__ off, int buckets, const int* __restrict__ count,
+                                                  const int* __restrict__ raw, const int* __restrict__ rawb, int* keys) {
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= off[buckets]) return;
+    const int b = rawb[j], key = raw[j];
+    const int base = off[b], n = count[b];
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += raw[base + k] < key ? 1 : 0;
+    keys[base + rank] = key;
+}
+
+// A41:4325-4352 with rank placement: phase 0 counts, 1 scatters, 2 ranks
+// (the first 16 particles of a bucket by index are visible to queries).
+__global__ __launch_bounds__(256) void k_part_hash(const DynView* __restrict__ dv, int4* pcell, int* pcount, const int* __restrict__ poff,
+                                                   int* pslot, int* praw, int* pkeys, unsigned long long* counts, int count_diag,
+                                                   int phase) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool active = i < dv->np;
+    if (phase == 0) {
+        if (!active) return;
+        const float4 p = dv->pp[i];
+        const float cs = dv->part_cell;
+        const int cx = cell_coord(p.x, cs), cy = cell_coord(p.y, cs), cz = cell_coord(p.z, cs);
+        const int b = static_cast<int>(hash_cell(cx, cy, cz, dv->part_buckets));
+        pcell[i] = make_int4(cx, cy, cz, b);
+        pslot[i] = atomicAdd(&pcount[b], 1);
+        return;
+    }
+    if (phase == 1) {
+        if (active) praw[poff[pcell[i].w] + pslot[i]] = i;
+        return;
+    }
+    bool over = false;
+    if (active) {
+        const int b = pcell[i].w, base = poff[b], n = pcount[b];
+        int rank = 0;
+        for (int k = 0; k < n && rank < kDynParticleSlots; ++k) rank += praw[base + k] < i ? 1 : 0;
+        if (rank < kDynParticleSlots) pkeys[base + rank] = i;
+        else over = true;
+    }
+    if (count_diag) wave_count(counts, kCountDynParticleOverflow, over);
+}
+
+// Scene gathers: a segment {source, begin, end} is a block's range.
+__global__ __launch_bounds__(256) void k_gather_particles(const GatherSource* __restrict__ src, const int4* __restrict__ segs,
+                                                          float4* pp, float4* pprev, int* psrc, int* player, int keep_prev) {
+    const int4 sg = segs[blockIdx.x];
+    const int v = sg.y + static_cast<int>(threadIdx.x);
+    if (v >= sg.z) return;
+    const GatherSource S = src[sg.x];
+    const int i = S.pbase + v;
+    const float4 p = S.pos[v];
+    const float4 now = make_float4(p.x, p.y, p.z, S.radius);
+    pprev[i] = keep_prev ? pp[i] : now;
+    pp[i] = now;
+    psrc[i] = S.source_id;
+    player[i] = S.layer;
+}
+
+__global__ __launch_bounds__(256) void k_gather_triangles(const GatherSource* __restrict__ src, const int4* __restrict__ segs,
+                                                          float4* tv, int* tsrc, int* tlayer, int* extent_bits) {
+    const int4 sg = segs[blockIdx.x];
+    const int t = sg.y + static_cast<int>(threadIdx.x);
+    if (t >= sg.z) return;
+    const GatherSource S = src[sg.x];
+    const int i = S.tbase + t;
+    const int4 tr = S.tris[t];
+    const bool ok = tr.x >= 0 && tr.y >= 0 && tr.z >= 0 && tr.x < S.nv && tr.y < S.nv && tr.z < S.nv;
+    const F3 a = ok ? f3(S.pos[tr.x]) : f3(0.0f, 0.0f, 0.0f);
+    const F3 b = ok ? f3(S.pos[tr.y]) : f3(0.0f, 0.0f, 0.0f);
+    const F3 c = ok ? f3(S.pos[tr.z]) : f3(0.0f, 0.0f, 0.0f);
+    tv[3 * i] = with_w(a, 0.0f);
+    tv[3 * i + 1] = with_w(b, 0.0f);
+    tv[3 * i + 2] = with_w(c, 0.0f);
+    tsrc[i] = S.source_id;
+    tlayer[i] = S.layer;
+    const float ext = fmaxf(fmaxf(a.x, fmaxf(b.x, c.x)) - fminf(a.x, fminf(b.x, c.x)),
+                            fmaxf(fmaxf(a.y, fmaxf(b.y, c.y)) - fminf(a.y, fminf(b.y, c.y)),
+                                  fmaxf(a.z, fmaxf(b.z, c.z)) - fminf(a.z, fminf(b.z, c.z))));
+    if (isfinite(ext) && ext >= 0.0f) atomicMax(extent_bits, __float_as_int(ext));   // non-negative floats order as ints
+}
+
+// The scene's triangle cell from the gathered extent (A41:6697-6702).
+__global__ void k_scene_cell(DynView* dv, const int* extent_bits, float margin, float thickness) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) dv->tri_cell = tri_cell_size(__int_as_float(*extent_bits), margin, thickness, true);
+}
+
+}  // namespace
+
+float dynamic_triangle_cell_size(float max_extent, float margin, float thickness, bool fine) {
+    return tri_cell_size(max_extent, margin, thickness, fine);
+}
+
+// ------------------------------------------------------------------ host
+DynamicColliders::~DynamicColliders() { release_all(); }
+
+void* DynamicColliders::alloc(int group, size_t bytes) {
+    bytes = std::max<size_t>(bytes, 256);
+    void* p = device_alloc(bytes, "dynamic collider allocation");
+    if (p) blocks_[group].emplace_back(p, bytes);
+    return p;
+}
+
+void DynamicColliders::release(int group) {
+    for (auto& b : blocks_[group]) device_free(b.first, b.second);
+    blocks_[group].clear();
+}
+
+void DynamicColliders::release_all() {
+    for (int g = 0; g < 5; ++g) release(g);
+}
+
+bool DynamicColliders::reserve_triangles(int n) {
+    if (!d_view_) {
+        d_view_ = static_cast<DynView*>(alloc(3, sizeof(DynView)));
+        if (!d_view_) return false;
+    }
+    if (n <= tri_cap_) return true;
+    release(0);
+    tri_cap_ = 0;
+    const size_t m = static_cast<size_t>(n);
+    tv_ = static_cast<float4*>(alloc(0, sizeof(float4) * 3 * m));
+    tsrc_ = static_cast<int*>(alloc(0, sizeof(int) * m));
+    tlayer_ = static_cast<int*>(alloc(0, sizeof(int) * m));
+    lo_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
+    hi_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
+    ctr_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
+    cmin_ = static_cast<int4*>(alloc(0, sizeof(int4) * m));
+    cmax_ = static_cast<int4*>(alloc(0, sizeof(int4) * m));
+    large_flag_ = static_cast<int*>(alloc(0, sizeof(int) * m));
+    large_off_ = static_cast<int*>(alloc(0, sizeof(int) * (m + 1)));
+    large_ = static_cast<int*>(alloc(0, sizeof(int) * m));
+    const int buckets = next_pow2(std::max(kDynMinBuckets, n * 4));
+    tri_count_ = static_cast<int*>(alloc(0, sizeof(int) * buckets));
+    tri_off_ = static_cast<int*>(alloc(0, sizeof(int) * (buckets + 1)));
+    tri_slot_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
+    tri_raw_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
+    tri_rawb_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
+    tri_keys_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
+    occupancy_ = static_cast<int*>(alloc(0, sizeof(int)));
+    extent_bits_ = static_cast<int*>(alloc(0, sizeof(int)));
+    const int need_scan = std::max(n, buckets);
+    if (need_scan > scan_cap_) {
+        release(2);
+        scan_tmp_ = static_cast<int*>(alloc(2, sizeof(int) * scan_tmp_ints(need_scan)));
+        scan_cap_ = scan_tmp_ ? need_scan : 0;
+    }
+    if (!tv_ || !tsrc_ || !tlayer_ || !lo_ || !hi_ || !ctr_ || !cmin_ || !cmax_ || !large_flag_ || !large_off_ || !large_ ||
+        !tri_count_ || !tri_off_ || !tri_slot_ || !tri_raw_ || !tri_rawb_ || !tri_keys_ || !occupancy_ || !extent_bits_ || !scan_tmp_)
+        return false;
+    tri_cap_ = n;
+    tri_buckets_ = buckets;
+    return true;
+}
+
+bool DynamicColliders::reserve_particles(int n) {
+    if (!d_view_) {
+        d_view_ = static_cast<DynView*>(alloc(3, sizeof(DynView)));
+        if (!d_view_) return false;
+    }
+    if (n <= part_cap_) return true;
+    release(1);
+    part_cap_ = 0;
+    const size_t m = static_cast<size_t>(n);
+    pp_ = static_cast<float4*>(alloc(1, sizeof(float4) * m));
+    pprev_ = static_cast<float4*>(alloc(1, sizeof(float4) * m));
+    psrc_ = static_cast<int*>(alloc(1, sizeof(int) * m));
+    player_ = static_cast<int*>(alloc(1, sizeof(int) * m));
+    pcell_ = static_cast<int4*>(alloc(1, sizeof(int4) * m));
+    pslot_ = static_cast<int*>(alloc(1, sizeof(int) * m));
+    praw_ = static_cast<int*>(alloc(1, sizeof(int) * m));
+    pkeys_ = static_cast<int*>(alloc(1, sizeof(int) * m));
+    const int buckets = next_pow2(std::max(kDynMinBuckets, n * 4));
+    pcount_ = static_cast<int*>(alloc(1, sizeof(int) * buckets));
+    poff_ = static_cast<int*>(alloc(1, sizeof(int) * (buckets + 1)));
+    if (buckets > scan_cap_) {
+        release(2);
+        scan_tmp_ = static_cast<int*>(alloc(2, sizeof(int) * scan_tmp_ints(buckets)));
+        scan_cap_ = scan_tmp_ ? buckets : 0;
+    }
+    if (!pp_ || !pprev_ || !psrc_ || !player_ || !pcell_ || !pslot_ || !praw_ || !pkeys_ || !pcount_ || !poff_ || !scan_tmp_) return false;
+    part_cap_ = n;
+    part_buckets_ = buckets;
+    return true;
+}
+
+bool DynamicColliders::upload_view(hipStream_t st) {
+    DynView& v = view_;
+    v = DynView{};
+    v.nt = nt_;
+    v.tri_hash = nt_ >= kDynTriangleHashMin ? 1 : 0;
+    v.tri_buckets = next_pow2(std::max(kDynMinBuckets, nt_ * 4));
+    v.tri_cell = tri_cell_;
+    v.tv = tv_;
+    v.tsrc = tri_filter_ ? tsrc_ : nullptr;
+    v.tlayer = tri_filter_ ? tlayer_ : nullptr;
+    v.lo = lo_; v.hi = hi_; v.ctr = ctr_; v.cmin = cmin_; v.cmax = cmax_;
+    v.tri_off = tri_off_; v.tri_keys = tri_keys_; v.large = large_;
+    v.nlarge = large_off_ ? large_off_ + nt_ : nullptr;
+    v.np = np_;
+    v.part_hash = np_ >= kDynParticleHashMin ? 1 : 0;
+    v.part_buckets = next_pow2(std::max(kDynMinBuckets, np_ * 4));
+    v.part_cell = part_cell_;
+    v.pp = pp_;
+    v.psrc = part_filter_ ? psrc_ : nullptr;
+    v.player = part_filter_ ? player_ : nullptr;
+    v.pcount = pcount_; v.poff = poff_; v.pkeys = pkeys_; v.pcell = pcell_;
+    if (v.tri_hash && v.tri_buckets > tri_buckets_) return fail("dynamic triangle hash capacity");
+    if (v.part_hash && v.part_buckets > part_buckets_) return fail("dynamic particle hash capacity");
+    launch_k(k_set_view, dim3(1), dim3(1), 0, st, d_view_, v);
+    return hip_ok(hipGetLastError(), "dynamic collider view");
+}
+
+// A41:8338-8420 / 6686-6722: soup upload; the cell size from the largest extent.
+bool DynamicColliders::set_triangles(hipStream_t st, const float* soup, int n, const int* src, const int* layer, float margin,
+                                     float thickness, bool fine) {
+    n = std::max(n, 0);
+    if (!sync_stream()) return false;
+    if (n == 0) { nt_ = 0; return d_view_ ? upload_view(st) : true; }
+    if (!reserve_triangles(n)) return false;
+    float max_extent = 0.0f;
+    std::vector<float4> h(3 * static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) {
+        const float* T = soup + 9 * static_cast<size_t>(k);
+        for (int j = 0; j < 3; ++j) h[3 * k + j] = make_float4(T[3 * j], T[3 * j + 1], T[3 * j + 2], 0.0f);
+        const float ext = std::max(std::max(T[0], std::max(T[3], T[6])) - std::min(T[0], std::min(T[3], T[6])),
+                                   std::max(std::max(T[1], std::max(T[4], T[7])) - std::min(T[1], std::min(T[4], T[7])),
+                                            std::max(T[2], std::max(T[5], T[8])) - std::min(T[2], std::min(T[5], T[8]))));
+        if (std::isfinite(ext) && ext >= 0.0f) max_extent = std::max(max_extent, ext);
+    }
+    margin = std::max(margin, 0.0f);
+    tri_cell_ = tri_cell_size(max_extent, margin, std::max(thickness, 0.0f), fine);
+    span_margin_ = margin;
+    tri_filter_ = src != nullptr;
+    bool ok = hip_ok(hipMemcpy(tv_, h.data(), sizeof(float4) * h.size(), hipMemcpyHostToDevice), "dynamic triangle upload");
+    if (ok && src) ok = hip_ok(hipMemcpy(tsrc_, src, sizeof(int) * n, hipMemcpyHostToDevice), "dynamic triangle source upload");
+    if (ok && src) {
+        std::vector<int> zeros;
+        const int* l = layer;
+        if (!l) { zeros.assign(n, 0); l = zeros.data(); }
+        ok = hip_ok(hipMemcpy(tlayer_, l, sizeof(int) * n, hipMemcpyHostToDevice), "dynamic triangle layer upload");
+    }
+    if (!ok) return false;
+    nt_ = n;
+    return upload_view(st);
+}
+
+// A41:8575-8697 / 6620-6668: the particle cell is recomputed when the radii change.
+bool DynamicColliders::set_particles(hipStream_t st, const float* pos, const float* prev, const float* radii, const int* src,
+                                     const int* layer, int n, float margin, float thickness) {
+    n = std::max(n, 0);
+    if (!sync_stream()) return false;
+    if (n == 0) { np_ = 0; return d_view_ ? upload_view(st) : true; }
+    std::vector<float> clean(n);
+    float max_radius = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        const float r = std::isfinite(radii[i]) ? radii[i] : 0.0f;
+        clean[i] = std::max(r, 0.0f);
+        max_radius = std::max(max_radius, clean[i]);
+    }
+    if (part_cell_ <= 0.0f || clean != radii_seen_ || np_ != n) {
+        part_cell_ = std::max(std::max(max_radius + std::max(margin, 0.0f), std::max(thickness, 0.0f)), 1.0e-3f);
+        radii_seen_ = clean;
+    }
+    const bool had_previous = np_ == n && part_cap_ >= n;
+    if (!reserve_particles(n)) return false;
+    std::vector<float4> h(n);
+    for (int i = 0; i < n; ++i) h[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], radii[i]);
+    bool ok = true;
+    if (prev) {
+        std::vector<float4> q(n);
+        for (int i = 0; i < n; ++i) q[i] = make_float4(prev[3 * i], prev[3 * i + 1], prev[3 * i + 2], radii[i]);
+        ok = hip_ok(hipMemcpy(pprev_, q.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "dynamic particle upload");
+    } else if (had_previous) {
+        ok = hip_ok(hipMemcpy(pprev_, pp_, sizeof(float4) * n, hipMemcpyDeviceToDevice), "dynamic particle upload");
+    } else {
+        ok = hip_ok(hipMemcpy(pprev_, h.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "dynamic particle upload");
+    }
+    ok = ok && hip_ok(hipMemcpy(pp_, h.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "dynamic particle upload");
+    part_filter_ = src != nullptr;
+    if (ok && src) ok = hip_ok(hipMemcpy(psrc_, src, sizeof(int) * n, hipMemcpyHostToDevice), "dynamic particle source upload");
+    if (ok && src) {
+        std::vector<int> zeros;
+        const int* l = layer;
+        if (!l) { zeros.assign(n, 0); l = zeros.data(); }
+        ok = hip_ok(hipMemcpy(player_, l, sizeof(int) * n, hipMemcpyHostToDevice), "dynamic particle layer upload");
+    }
+    if (!ok) return false;
+    np_ = n;
+    return upload_view(st);
+}
+
+bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& sources_in, float margin, float thickness) {
+    std::vector<GatherSource> sources = sources_in;
+    int np = 0, nt = 0;
+    float max_radius = 0.0f;
+    for (GatherSource& s : sources) {
+        s.pbase = np;
+        s.tbase = nt;
+        np += std::max(s.nv, 0);
+        nt += std::max(s.nt, 0);
+        const float r = std::isfinite(s.radius) ? s.radius : 0.0f;
+        max_radius = std::max(max_radius, std::max(r, 0.0f));
+    }
+    margin = std::max(margin, 0.0f);
+    thickness = std::max(thickness, 0.0f);
+    const bool keep_prev = np == np_ && np > 0;
+    if (!sync_stream()) return false;
+    if (!reserve_particles(std::max(np, 1)) || !reserve_triangles(std::max(nt, 1))) return false;
+    // segment tables: {source, begin, end} per 256 items
+    std::vector<int4> segs;
+    for (size_t k = 0; k < sources.size(); ++k)
+        for (int b = 0; b < sources[k].nv; b += 256) segs.push_back(make_int4(static_cast<int>(k), b, std::min(sources[k].nv, b + 256), 0));
+    const int npseg = static_cast<int>(segs.size());
+    for (size_t k = 0; k < sources.size(); ++k)
+        for (int b = 0; b < sources[k].nt; b += 256) segs.push_back(make_int4(static_cast<int>(k), b, std::min(sources[k].nt, b + 256), 1));
+    if (static_cast<int>(sources.size()) > sources_cap_ || static_cast<int>(segs.size()) > segments_cap_) {
+        release(4);
+        const size_t ns = std::max<size_t>(sources.size(), static_cast<size_t>(sources_cap_));
+        const size_t ng = std::max<size_t>(segs.size(), static_cast<size_t>(segments_cap_));
+        d_sources_ = static_cast<GatherSource*>(alloc(4, sizeof(GatherSource) * ns));
+        d_segments_ = static_cast<int4*>(alloc(4, sizeof(int4) * ng));
+        sources_cap_ = d_sources_ ? static_cast<int>(ns) : 0;
+        segments_cap_ = d_segments_ ? static_cast<int>(ng) : 0;
+    }
+    if ((!sources.empty() && !d_sources_) || (!segs.empty() && !d_segments_)) return false;
+    if (!sources.empty() &&
+        !hip_ok(hipMemcpy(d_sources_, sources.data(), sizeof(GatherSource) * sources.size(), hipMemcpyHostToDevice), "scene sources"))
+        return false;
+    if (!segs.empty() && !hip_ok(hipMemcpy(d_segments_, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice), "scene sources"))
+        return false;
+    np_ = np;
+    nt_ = nt;
+    part_cell_ = std::max(std::max(max_radius + margin, thickness), 1.0e-3f);
+    radii_seen_.clear();
+    span_margin_ = margin;
+    tri_cell_ = 0.0f;   // computed on the device below
+    tri_filter_ = part_filter_ = true;
+    if (!upload_view(st)) return false;
+    TimedLaunch t("scene_gather", st);
+    if (npseg > 0) launch_k(k_gather_particles, dim3(npseg), dim3(256), 0, st, d_sources_, d_segments_, pp_, pprev_, psrc_, player_, keep_prev ? 1 : 0);
+    launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, extent_bits_, 0, 1);
+    const int ntseg = static_cast<int>(segs.size()) - npseg;
+    if (ntseg > 0) launch_k(k_gather_triangles, dim3(ntseg), dim3(256), 0, st, d_sources_, d_segments_ + npseg, tv_, tsrc_, tlayer_, extent_bits_);
+    launch_k(k_scene_cell, dim3(1), dim3(1), 0, st, d_view_, extent_bits_, margin, thickness);
+    return hip_ok(hipGetLastError(), "scene gather");
+}
+
+bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool count_diag) {
+    if (!d_view_) return true;
+    const int cd = count_diag ? 1 : 0;
+    TimedLaunch t("dynamic_hash", st);
+    if (nt_ > 0) {
+        const dim3 g(grid_for(nt_, 256)), blk(256);
+        launch_k(k_bounds, g, blk, 0, st, static_cast<const DynView*>(d_view_), span_margin_, lo_, hi_, ctr_, cmin_, cmax_, large_flag_,
+                 counts, cd);
+        exclusive_scan(st, large_flag_, nt_, large_off_, scan_tmp_);
+        launch_k(k_large_scatter, g, blk, 0, st, nt_, static_cast<const int*>(large_flag_), static_cast<const int*>(large_off_), large_);
+        if (nt_ >= kDynTriangleHashMin) {
+            const int buckets = next_pow2(std::max(kDynMinBuckets, nt_ * 4));
+            const dim3 gw(grid_for(static_cast<long long>(nt_) * kDynMaxHashCells, 256));
+            launch_k(k_fill_i32, dim3(grid_for(buckets, 256)), blk, 0, st, tri_count_, 0, buckets);
+            launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, occupancy_, 0, 1);
+            launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const int4*>(cmin_),
+                     static_cast<const int4*>(cmax_), static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_),
+                     tri_slot_, tri_raw_, tri_rawb_, occupancy_, 0);
+            exclusive_scan(st, tri_count_, buckets, tri_off_, scan_tmp_);
+            launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const int4*>(cmin_),
+                     static_cast<const int4*>(cmax_), static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_),
+                     tri_slot_, tri_raw_, tri_rawb_, occupancy_, 1);
+            launch_k(k_tri_rank, gw, blk, 0, st, static_cast<const int*>(tri_off_), buckets, static_cast<const int*>(tri_count_),
+                     static_cast<const int*>(tri_raw_), static_cast<const int*>(tri_rawb_), tri_keys_);
+        }
+    }
+    if (np_ >= kDynParticleHashMin) {
+        const int buckets = next_pow2(std::max(kDynMinBuckets, np_ * 4));
+        const dim3 g(grid_for(np_, 256)), blk(256);
+        launch_k(k_fill_i32, dim3(grid_for(buckets, 256)), blk, 0, st, pcount_, 0, buckets);
+        for (int phase = 0; phase < 3; ++phase) {
+            if (phase == 1) exclusive_scan(st, pcount_, buckets, poff_, scan_tmp_);
+            launch_k(k_part_hash, g, blk, 0, st, static_cast<const DynView*>(d_view_), pcell_, pcount_, static_cast<const int*>(poff_),
+                     pslot_, praw_, pkeys_, counts, cd, phase);
+        }
+    }
+    return hip_ok(hipGetLastError(), "dynamic collider hash");
+}
+
+int DynamicColliders::large_count() const {
+    if (!large_off_ || nt_ <= 0) return 0;
+    int n = 0;
+    if (!hip_ok(hipMemcpy(&n, large_off_ + nt_, sizeof(int), hipMemcpyDeviceToHost), "dynamic large count")) return 0;
+    return std::max(0, std::min(n, nt_));
+}
+
+int DynamicColliders::max_occupancy() const {
+    if (!occupancy_ || nt_ < kDynTriangleHashMin) return 0;
+    int n = 0;
+    if (!hip_ok(hipMemcpy(&n, occupancy_, sizeof(int), hipMemcpyDeviceToHost), "dynamic bucket occupancy")) return 0;
+    return std::max(n, 0);
+}
+
+}  // namespace ssbl

@@ -1,0 +1,144 @@
+// xpbd_dynamic.hpp -- dynamic colliders a cloth collides with: its own
+// triangle soup and particles, or a global dynamic scene holding every cloth
+// of a session, with their spatial hashes built on the device.
+//
+// Reference: uploads A41:8319-8420 and 8575-8697, bounds and hashes
+// A41:3694-3932, 4325-4352 and 6220-6449, global scene A41:6451-6782 and
+// 8733-8789, contacts A41:3934-4511 (unsigned).
+//
+// The reference fills its buckets with atomics, so the order of a bucket (and
+// which particles a full 16-slot bucket keeps) depends on scheduling.  Here
+// every bucket is a counting-sorted segment whose entries are placed by rank:
+// triangle buckets hold (triangle * 64 + cell index) keys in ascending order,
+// particle buckets their first 16 particles in ascending order, and the list
+// of triangles too large to hash is in ascending order (a prefix sum of
+// flags).  A query therefore visits the same primitives in the same order as
+// the CPU oracle.
+#pragma once
+
+#include <vector>
+
+#include "xpbd_device.hpp"
+
+namespace ssbl {
+
+constexpr int kDynParticleSlots = 16;       // A41:26
+constexpr int kDynMaxHashCells = 64;        // A41:79
+constexpr int kDynParticleHashMin = 128;    // A41:25
+constexpr int kDynTriangleHashMin = 64;     // A41:39
+constexpr int kDynMinBuckets = 1024;        // A41:27, 40
+
+enum DynCountSlot {
+    kCountDynParticleCandidates = 4, kCountDynParticleContacts = 5, kCountDynParticleOverflow = 6, kCountTrianglePairs = 7,
+    kCountDynTriCandidates = 34, kCountDynTriBucketOverflow = 35, kCountDynTriLarge = 36, kCountDynTriAabbRejects = 37
+};
+
+// What the contact pass reads; lives in device memory (one per collider set).
+struct DynView {
+    int nt;
+    int tri_hash, tri_buckets;
+    float tri_cell;
+    const float4* tv;        // 3 per triangle (.w unused)
+    const int* tsrc;         // null: no source filter
+    const int* tlayer;
+    const float4* lo;        // AABB (.w unused)
+    const float4* hi;
+    const float4* ctr;       // centre, .w = bounding radius
+    const int4* cmin;        // hashed cell span (hashed sets only)
+    const int4* cmax;
+    const int* tri_off;      // buckets + 1
+    const int* tri_keys;     // triangle * 64 + cell index, ascending per bucket
+    const int* large;        // triangles spanning more than 64 cells, ascending
+    const int* nlarge;       // their count (device)
+    int np;
+    int part_hash, part_buckets;
+    float part_cell;
+    const float4* pp;        // position, .w = radius as uploaded
+    const int* psrc;
+    const int* player;
+    const int* pcount;       // buckets
+    const int* poff;         // buckets + 1
+    const int* pkeys;        // first min(count, 16) particles of a bucket, ascending
+    const int4* pcell;
+};
+
+// One contributing cloth of a device-gathered scene.
+struct GatherSource {
+    const float4* pos;       // current positions
+    const int4* tris;        // its triangles
+    int nv, nt;
+    int pbase, tbase;        // first particle / triangle in the scene
+    float radius;
+    int source_id, layer;
+};
+
+class DynamicColliders {
+public:
+    DynamicColliders() = default;
+    DynamicColliders(const DynamicColliders&) = delete;
+    DynamicColliders& operator=(const DynamicColliders&) = delete;
+    ~DynamicColliders();
+
+    // Host arrays (copied).  `fine` selects the scene's finer triangle cells
+    // (A41:6590-6601).  Both sync the library stream before replacing data.
+    bool set_triangles(hipStream_t st, const float* soup, int n, const int* src, const int* layer, float margin, float thickness,
+                       bool fine);
+    bool set_particles(hipStream_t st, const float* pos, const float* prev, const float* radii, const int* src, const int* layer,
+                       int n, float margin, float thickness);
+    // Device gather of every source's current positions (particles) and
+    // triangles, ordered by source; asynchronous on `st`.
+    bool gather(hipStream_t st, const std::vector<GatherSource>& sources, float margin, float thickness);
+    // Bounds and both hashes; build-time counters go to `counts` when count_diag.
+    bool build(hipStream_t st, unsigned long long* counts, bool count_diag);
+
+    const DynView* device_view() const { return d_view_; }
+    int triangles() const { return nt_; }
+    int particles() const { return np_; }
+    bool triangles_allocated() const { return tri_cap_ > 0; }
+    // Synchronous reads for diagnostics (the caller has synchronised).
+    int large_count() const;
+    int max_occupancy() const;
+
+private:
+    bool reserve_triangles(int n);
+    bool reserve_particles(int n);
+    bool upload_view(hipStream_t st);
+    // allocation groups: 0 triangles, 1 particles, 2 scan scratch, 3 the view, 4 gather tables
+    void* alloc(int group, size_t bytes);
+    void release(int group);
+    void release_all();
+
+    int nt_ = 0, np_ = 0, tri_cap_ = 0, part_cap_ = 0, tri_buckets_ = 0, part_buckets_ = 0, scan_cap_ = 0;
+    float tri_cell_ = 0.0f, span_margin_ = 0.0f, part_cell_ = 0.0f;
+    std::vector<float> radii_seen_;
+    std::vector<std::pair<void*, size_t>> blocks_[5];
+    // triangles
+    float4* tv_ = nullptr;
+    int* tsrc_ = nullptr;
+    int* tlayer_ = nullptr;
+    bool tri_filter_ = false;
+    float4 *lo_ = nullptr, *hi_ = nullptr, *ctr_ = nullptr;
+    int4 *cmin_ = nullptr, *cmax_ = nullptr;
+    int *large_flag_ = nullptr, *large_off_ = nullptr, *large_ = nullptr;
+    int *tri_count_ = nullptr, *tri_off_ = nullptr, *tri_slot_ = nullptr, *tri_raw_ = nullptr, *tri_rawb_ = nullptr, *tri_keys_ = nullptr;
+    int* occupancy_ = nullptr;
+    int* extent_bits_ = nullptr;
+    // particles
+    float4 *pp_ = nullptr, *pprev_ = nullptr;
+    int *psrc_ = nullptr, *player_ = nullptr;
+    bool part_filter_ = false;
+    int4* pcell_ = nullptr;
+    int *pslot_ = nullptr, *pcount_ = nullptr, *poff_ = nullptr, *praw_ = nullptr, *pkeys_ = nullptr;
+    int* scan_tmp_ = nullptr;
+    GatherSource* d_sources_ = nullptr;
+    int sources_cap_ = 0;
+    int4* d_segments_ = nullptr;
+    int segments_cap_ = 0;
+    DynView* d_view_ = nullptr;
+    DynView view_{};
+};
+
+// Triangle hash cell of a soup whose largest triangle extent is max_extent.
+float dynamic_triangle_cell_size(float max_extent, float margin, float thickness, bool fine);
+
+}  // namespace ssbl

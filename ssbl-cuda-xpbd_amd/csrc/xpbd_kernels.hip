@@ -16,6 +16,7 @@
 // vertex's registers before one coalesced write.  A batch of cloths is a table
 // of ClothView + a tile table; every launch covers all tiles of all cloths.
 #include "xpbd_device.hpp"
+#include "xpbd_dynamic.hpp"
 #include "xpbd_math.hpp"
 #include "xpbd_timing.hpp"
 
@@ -234,6 +235,138 @@ __device__ inline void sdf_contact(const ClothView& c, VertexState& s, float dt,
     s.set_vel(v);
     s.set_q(corrected - mul(v, fmaxf(dt, kEps)));
     if (count) atomicAdd(&c.counts[kCountStaticSdf], 1ull);
+}
+
+// Dynamic colliders, unsigned contacts: the deepest triangle contact
+// (A41:3999-4323) then the deepest particle contact (A41:4354-4511) push the
+// vertex by half their depth (A41:129).  Only the vertex's own position is
+// read and written, so running both here equals the reference's two
+// vertex-parallel launches.
+__device__ inline bool dyn_enabled(const ClothView& c, int src, int layer) {   // A41:3984-3997
+    if (c.dyn_mode == 0) return true;
+    if (c.dyn_target_src != 0 && src == c.dyn_target_src) return false;
+    if (c.dyn_mode == 2) return layer < c.dyn_target_layer;
+    return true;
+}
+
+__device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, float dt, bool count) {
+    const DynView& D = *c.dyn;
+    const float cm = fmaxf(c.collision_margin, 0.0f);
+    const int nt = D.nt;
+    if (nt > 0 && cm > 0.0f) {
+        const F3 p = s.p;
+        F3 best_delta = f3(0.0f, 0.0f, 0.0f);
+        float best_pen = 0.0f;
+        unsigned cand = 0, rej = 0;
+        auto consider = [&](int t) {
+            if (t < 0 || t >= nt) return;
+            if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) return;
+            ++cand;
+            const float4 mn = D.lo[t], mx = D.hi[t];
+            const F3 q0 = f3(fminf(fmaxf(p.x, mn.x), mx.x), fminf(fmaxf(p.y, mn.y), mx.y), fminf(fmaxf(p.z, mn.z), mx.z));
+            const F3 da = p - q0;
+            if (dot(da, da) > cm * cm) { ++rej; return; }
+            const float4 ce = D.ctr[t];
+            const float rr = ce.w + cm;
+            const F3 dc = p - f3(ce);
+            if (dot(dc, dc) > rr * rr) { ++rej; return; }
+            const F3 a = f3(D.tv[3 * t]), b = f3(D.tv[3 * t + 1]), cc = f3(D.tv[3 * t + 2]);
+            const F3 q = closest_point_on_triangle(p, a, b, cc);
+            const F3 d = p - q;
+            const float dsq = dot(d, d);
+            const float dist = sqrtf(fmaxf(dsq, kEps));
+            const F3 tn = cross(b - a, cc - a);
+            const float tl = sqrtf(fmaxf(dot(tn, tn), kEps));
+            const F3 tnorm = mul(tn, 1.0f / tl);
+            const F3 n = dsq > kEps * 4.0f ? mul(d, 1.0f / fmaxf(dist, kEps)) : tnorm;
+            const float pen = cm - dist;
+            if (dist >= cm) return;
+            if (pen > best_pen) { best_pen = pen; best_delta = mul(n, pen); }
+        };
+        if (D.tri_hash) {   // own cell only: the unsigned query radius is 0 (A41:3623-3633)
+            const int cx = cell_coord(p.x, D.tri_cell), cy = cell_coord(p.y, D.tri_cell), cz = cell_coord(p.z, D.tri_cell);
+            const int b = static_cast<int>(hash_cell(cx, cy, cz, D.tri_buckets));
+            const int e = D.tri_off[b + 1];
+            for (int k = D.tri_off[b]; k < e; ++k) {
+                const int t = D.tri_keys[k] / kDynMaxHashCells;
+                const int4 lo = D.cmin[t], hi = D.cmax[t];
+                if (cx < lo.x || cx > hi.x || cy < lo.y || cy > hi.y || cz < lo.z || cz > hi.z) continue;   // A41:3934-3982
+                consider(t);
+            }
+            const int nl = *D.nlarge;
+            for (int k = 0; k < nl; ++k) consider(D.large[k]);
+        } else {
+            for (int t = 0; t < nt; ++t) consider(t);
+        }
+        if (count && cand) atomicAdd(&c.counts[kCountDynTriCandidates], static_cast<unsigned long long>(cand));
+        if (count && rej) atomicAdd(&c.counts[kCountDynTriAabbRejects], static_cast<unsigned long long>(rej));
+        if (best_pen > 0.0f) {
+            const F3 applied = mul(best_delta, 0.5f);
+            const F3 np = p + applied;
+            F3 v = s.vel();
+            const float vn = dot(v, applied);
+            if (vn < 0.0f) {
+                const float lsq = fmaxf(dot(applied, applied), kEps);
+                v = v - mul(applied, 0.5f * vn / lsq);
+                s.set_vel(v);
+            }
+            s.p = np;
+            s.set_q(np - mul(v, fmaxf(dt, kEps)));
+            if (count) {
+                atomicAdd(&c.counts[kCountTrianglePairs], 1ull);
+                atomicAdd(&c.counts[kCountExactImpulse], 1ull);
+            }
+        }
+    }
+    const int np_count = D.np;
+    if (np_count > 0) {
+        const float margin = fmaxf(c.collision_margin, 0.0f);
+        const F3 p = s.p;
+        F3 best_delta = f3(0.0f, 0.0f, 0.0f);
+        float best_pen = 0.0f;
+        unsigned cand = 0;
+        auto consider = [&](int j) {
+            if (j < 0 || j >= np_count) return;
+            if (D.psrc && !dyn_enabled(c, D.psrc[j], D.player ? D.player[j] : 0)) return;
+            const float4 pj = D.pp[j];
+            const float radius = fmaxf(pj.w, 0.0f) + margin;
+            if (radius <= 0.0f) return;
+            const F3 d = p - f3(pj);
+            const float dsq = dot(d, d);
+            const float dist = sqrtf(fmaxf(dsq, kEps));
+            if (dist >= radius) return;
+            ++cand;
+            const F3 n = dist > kEps ? mul(d, 1.0f / dist) : f3(0.0f, 0.0f, 1.0f);
+            const float pen = radius - dist;
+            if (pen > best_pen) { best_pen = pen; best_delta = mul(n, pen); }
+        };
+        if (D.part_hash) {
+            const int cx = cell_coord(p.x, D.part_cell), cy = cell_coord(p.y, D.part_cell), cz = cell_coord(p.z, D.part_cell);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
+                        const int b = static_cast<int>(hash_cell(qx, qy, qz, D.part_buckets));
+                        const int lim = min(D.pcount[b], kDynParticleSlots), base = D.poff[b];
+                        for (int k = 0; k < lim; ++k) {
+                            const int j = D.pkeys[base + k];
+                            const int4 pc = D.pcell[j];
+                            if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
+                            consider(j);
+                        }
+                    }
+        } else {
+            for (int j = 0; j < np_count; ++j) consider(j);
+        }
+        if (count && cand) atomicAdd(&c.counts[kCountDynParticleCandidates], static_cast<unsigned long long>(cand));
+        if (best_pen > 0.0f) {
+            s.p = p + mul(best_delta, 0.5f);
+            if (count) {
+                atomicAdd(&c.counts[kCountExactImpulse], 1ull);
+                atomicAdd(&c.counts[kCountDynParticleContacts], 1ull);
+            }
+        }
+    }
 }
 
 // A41:5846-5884
@@ -689,7 +822,7 @@ enum GatherKind { kGatherNone = 0, kGatherStretch = 1, kGatherLra = 2, kGatherBe
 
 // One solver iteration for every vertex of the tile: the constraint gather
 // selected at compile time, then the vertex-local steps selected by `ops`.
-template <int kGather>
+template <int kGather, bool kDynamic>
 __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
                                                        int ntiles, int src_parity, int dst_parity, unsigned ops) {
     const Tile tile = tiles[xcd_tile(ntiles)];
@@ -716,6 +849,9 @@ __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __res
     if (ops & kOpPin) apply_pins(c, s, pinned, dt);
     if ((ops & kOpAnalytic) && !pinned) analytic_contacts(c, s, dt, count);
     if ((ops & kOpSdf) && c.sdf_ready && !pinned) sdf_contact(c, s, dt, count);
+    if constexpr (kDynamic) {
+        if ((ops & kOpDynamic) && c.dyn && !pinned) dynamic_contacts(c, s, dt, count);   // A41:9230-9252
+    }
     // A41:9360-9369: the second pin projection runs after any contact work,
     // and on every last iteration (dynamic collision is scheduled there).
     if ((ops & kOpPinPolish) && ((ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt);
@@ -855,14 +991,19 @@ void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsi
                        : (ops & kOpBend) ? "vertex_pass_bend" : "vertex_pass_local";
     TimedLaunch t(name, c.stream);
     const dim3 grid(c.ntiles), block(kTile);
-    if (ops & kOpStretch)
-        launch_k(k_vertex_pass<kGatherStretch>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+    const bool dyn = (ops & kOpDynamic) != 0u;
+    if ((ops & kOpStretch) && dyn)
+        launch_k(k_vertex_pass<kGatherStretch, true>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+    else if (ops & kOpStretch)
+        launch_k(k_vertex_pass<kGatherStretch, false>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
     else if (ops & kOpLra)
-        launch_k(k_vertex_pass<kGatherLra>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+        launch_k(k_vertex_pass<kGatherLra, false>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
     else if (ops & kOpBend)
-        launch_k(k_vertex_pass<kGatherBend>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+        launch_k(k_vertex_pass<kGatherBend, false>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+    else if (dyn)
+        launch_k(k_vertex_pass<kGatherNone, true>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
     else
-        launch_k(k_vertex_pass<kGatherNone>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+        launch_k(k_vertex_pass<kGatherNone, false>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
 }
 
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops) {

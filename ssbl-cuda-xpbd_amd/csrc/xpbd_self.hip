@@ -120,13 +120,19 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int* __restrict
     if (first <= n - 1 && first + kScanItems > n - 1) out[n] = run;   // grand total after the last item
 }
 
+}  // namespace
+
+int scan_tmp_ints(int n) { return (std::max(n, 1) + kScanChunk - 1) / kScanChunk + 2; }
+
 void exclusive_scan(hipStream_t s, const int* in, int n, int* out, int* tmp) {
     const int nb = (n + kScanChunk - 1) / kScanChunk;
-    TimedLaunch t("self_hash_scan", s);
+    TimedLaunch t("hash_scan", s);
     launch_k(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp);
     launch_k(k_scan_block_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, nb);
     launch_k(k_scan_final, dim3(nb), dim3(kScanBlock), 0, s, in, n, tmp, out);
 }
+
+namespace {
 
 // ------------------------------------------------------------ hash builds
 __device__ __forceinline__ const float4* cur_pos(const ClothView& c, int parity) { return parity ? c.pos1 : c.pos0; }
@@ -659,7 +665,7 @@ bool SelfCollision::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& e
     if (s.use_thash && !make_hash(s.th, s.nt, s.nt * kMaxPrimCells)) return false;
     if (s.use_ehash && !make_hash(s.eh, s.ne, s.ne * kMaxPrimCells)) return false;
     int max_buckets = std::max(s.vh.buckets, std::max(s.th.buckets, s.eh.buckets));
-    s.scan_tmp = static_cast<int*>(alloc_bytes(sizeof(int) * (max_buckets / kScanChunk + 2)));
+    s.scan_tmp = static_cast<int*>(alloc_bytes(sizeof(int) * scan_tmp_ints(max_buckets)));
     s.vcell = static_cast<int4*>(alloc_bytes(sizeof(int4) * nv));
     s.nb_count = static_cast<unsigned*>(alloc_bytes(sizeof(unsigned) * nv));
     s.nb_idx = static_cast<unsigned*>(alloc_bytes(sizeof(unsigned) * static_cast<size_t>(nv) * kNeighborSlots));

@@ -14,6 +14,7 @@
 #include "ssbl_xpbd_hip.h"
 #include "ssbl_xpbd_hip_ext.h"
 #include "xpbd_device.hpp"
+#include "xpbd_dynamic.hpp"
 #include "xpbd_pcg.hpp"
 #include "xpbd_self.hpp"
 #include "xpbd_timing.hpp"
@@ -191,6 +192,16 @@ int next_pow2(int v) {
 // ----------------------------------------------------------------- solver
 struct HostPin { int index; float target[3]; float weight; };
 
+// A global dynamic scene (A41:309-340): the colliders every attached cloth
+// sees, built by an update from host arrays or gathered on the device.
+struct GlobalScene {
+    DynamicColliders set;
+    float margin = 0.0f, thickness = 0.0f;
+    Dev<unsigned long long> counts;   // build-time counters
+    long long hash_overflow = 0;
+    bool overflow_pending = false;    // counters of an asynchronous build not read yet
+};
+
 struct Solver {
     long long dev_bytes = 0;   // device memory held (ssbl_ext_device_bytes)
     SsblXpbdConfig cfg{};
@@ -234,6 +245,12 @@ struct Solver {
     Staging staging;
     SelfCollision self;
     PcgPath pcg;
+    DynamicColliders dyn;              // the cloth's own dynamic colliders
+    GlobalScene* scene = nullptr;      // attached global scene
+    int target_src = 0, target_layer = 0, cross_mode = 0;
+    bool use_scene() const { return scene != nullptr && cross_mode != 0; }
+    const DynamicColliders& dyn_set() const { return use_scene() ? scene->set : dyn; }
+    bool dyn_on() const { return dyn_set().triangles() > 0 || dyn_set().particles() > 0; }
     long long wing_count = 0;
     ClothView view{};
     SsblXpbdDiagnostics diag{};
@@ -771,6 +788,10 @@ void Solver::fill_view(int substeps) {
     v.need_normal = (v.use_pressure || !fields.empty()) ? 1 : 0;
     v.direct_stretch = cfg.stretch_optimization_enabled ? 0 : 1;
     v.contacts_on = (analytic_on() || sdf_ready) ? 1 : 0;
+    v.dyn = dyn_on() ? dyn_set().device_view() : nullptr;
+    v.dyn_mode = use_scene() ? cross_mode : 0;
+    v.dyn_target_src = target_src;
+    v.dyn_target_layer = target_layer;
 }
 
 // ------------------------------------------------------------ schedule
@@ -779,7 +800,7 @@ void Solver::fill_view(int substeps) {
 // `dviews[m]` is member m's view inside that table.
 struct StepPlan {
     bool any_pins = false, any_edges = false, any_lra = false, any_bend = false, any_self = false, any_pcg = false;
-    bool any_bend_gather = false, any_forces = false;
+    bool any_bend_gather = false, any_forces = false, any_dyn = false;
 };
 
 static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const int4* bend_segs_dev,
@@ -795,6 +816,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
         plan.any_pcg |= s->pcg.enabled();
         plan.any_bend_gather |= s->bend_gather;
         plan.any_forces |= s->view.need_normal != 0 || s->view.nfield > 0;
+        plan.any_dyn |= s->view.dyn != nullptr;
     }
     const unsigned cnt = count ? kOpCount : 0u;
     const unsigned pin_ops = plan.any_pins ? (kOpPin | kOpPinPolish) : 0u;
@@ -818,6 +840,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             const bool last = it == iterations - 1;
             const bool final_c = last && sub == substeps - 1;
             const unsigned last_op = last ? kOpLastIter : 0u;
+            const unsigned dyn_op = (last && plan.any_dyn) ? kOpDynamic : 0u;   // A41:9230-9252
             bool self_now = false;
             if (plan.any_self && last) {
                 for (Solver* m : members) {
@@ -829,7 +852,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             const bool split = run_pcg || self_now || (final_c && (plan.any_lra || plan.any_bend));
             if (!split) {
                 launch_vertex_pass(ctx, parity, parity ^ 1,
-                                   stretch | pin_ops | kOpAnalytic | kOpSdf | last_op | (last ? kOpVelocity : 0u) | cnt);
+                                   stretch | pin_ops | kOpAnalytic | kOpSdf | dyn_op | last_op | (last ? kOpVelocity : 0u) | cnt);
                 parity ^= 1;
                 continue;
             }
@@ -858,7 +881,8 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             }
             const bool tail = self_now || (run_pcg && final_c);
             launch_vertex_pass(ctx, parity, parity,
-                               (tail ? (pin_ops & ~kOpPinPolish) : (pin_ops | last_op | kOpVelocity)) | kOpAnalytic | kOpSdf | cnt);
+                               (tail ? (pin_ops & ~kOpPinPolish) : (pin_ops | last_op | kOpVelocity)) | kOpAnalytic | kOpSdf | dyn_op |
+                                   cnt);
             if (self_now) {
                 if (!self_batch || !self_batch->run(ctx, parity, sub, substeps, count)) return false;
             }
@@ -879,6 +903,22 @@ static void reset_diag(Solver& s) {
     s.diag.unsupported_force_field_count = s.unsupported_fields;
     s.diag.abi41_bending_texture_ready = s.wing_count > 0 ? 1 : 0;
     s.populate_sdf_diag();
+    s.diag.dynamic_triangle_count = s.dyn_set().triangles();   // A41:9103-9122
+    s.diag.dynamic_particle_count = s.dyn_set().particles();
+    if (s.use_scene()) {
+        s.diag.global_dynamic_particle_count = s.scene->set.particles();
+        s.diag.global_dynamic_triangle_count = s.scene->set.triangles();
+        s.diag.global_dynamic_hash_overflow = s.scene->hash_overflow;
+    }
+}
+
+// Counters of a scene build that ran asynchronously (the stream is synchronised).
+static void settle_scene_overflow(GlobalScene& sc) {
+    if (!sc.overflow_pending) return;
+    unsigned long long c[kCountSlots] = {};
+    if (hip_ok(hipMemcpy(c, sc.counts.p, sizeof(c), hipMemcpyDeviceToHost), "global scene counters"))
+        sc.hash_overflow = static_cast<long long>(c[kCountDynParticleOverflow] + c[kCountDynTriBucketOverflow]);
+    sc.overflow_pending = false;
 }
 
 static bool fetch_counts(Solver& s) {  // A41:8059-8179
@@ -897,7 +937,29 @@ static bool fetch_counts(Solver& s) {  // A41:8059-8179
     d.external_friction_corrections = static_cast<long long>(c[kCountFriction]);
     d.self_candidate_count = static_cast<long long>(c[kCountSelfCandidates]);
     d.static_sdf_contact_count = static_cast<long long>(c[kCountStaticSdf]);
-    d.candidate_count = static_cast<long long>(c[kCountSelfCandidates] + c[kCountStaticSdf]);
+    d.candidate_count = static_cast<long long>(c[kCountDynParticleCandidates] + c[kCountDynTriCandidates] +
+                                               c[kCountSelfCandidates] + c[kCountStaticSdf]);
+    // A41:8078-8110
+    d.dynamic_particle_candidate_count = static_cast<long long>(c[kCountDynParticleCandidates]);
+    d.dynamic_particle_contacts = static_cast<long long>(c[kCountDynParticleContacts]);
+    d.dynamic_particle_overflow = 0;
+    d.dynamic_triangle_candidate_count = static_cast<long long>(c[kCountDynTriCandidates]);
+    d.dynamic_triangle_bucket_overflow = static_cast<long long>(c[kCountDynTriBucketOverflow]);
+    d.dynamic_triangle_large_primitive_count = static_cast<long long>(c[kCountDynTriLarge]);
+    d.dynamic_triangle_aabb_reject_count = static_cast<long long>(c[kCountDynTriAabbRejects]);
+    d.dynamic_triangle_max_bucket_occupancy = 0;
+    if (s.dyn.triangles_allocated()) {
+        d.dynamic_triangle_large_primitive_count =
+            std::max(d.dynamic_triangle_large_primitive_count, static_cast<long long>(s.dyn.large_count()));
+        d.dynamic_triangle_max_bucket_occupancy = s.dyn.max_occupancy();
+    }
+    d.fast_triangle_pair_candidates = d.dynamic_triangle_candidate_count;
+    d.fast_triangle_pair_contacts = static_cast<long long>(c[kCountTrianglePairs]);
+    d.fast_triangle_pair_skipped_rest = d.dynamic_triangle_aabb_reject_count;
+    if (s.use_scene()) {
+        settle_scene_overflow(*s.scene);
+        d.global_dynamic_hash_overflow = s.scene->hash_overflow;
+    }
     d.abi41_lra_tack_count = static_cast<long long>(c[kCountLraTacks]);
     d.abi41_bending_wing_count = c[kCountBendingWings] > 0 ? static_cast<long long>(c[kCountBendingWings]) : s.wing_count;
     d.abi41_tack_jitter_guarded = static_cast<long long>(c[kCountTackGuards]) + (s.cfg.lra_count > 0 ? s.lra_invalid : 0);
@@ -913,6 +975,8 @@ static bool prepare_member(Solver& s, int substeps, int iterations, bool count) 
         launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.counts.p), 0u, 2 * static_cast<size_t>(kCountSlots));
         launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.max_smoothed.p), 0u, 1);
     }
+    // the cloth's own dynamic colliders are re-hashed every step (A41:9129-9136)
+    if (!s.use_scene() && s.dyn_on() && !s.dyn.build(lib_stream(), s.counts.p, count)) return false;
     return true;
 }
 
@@ -1134,12 +1198,15 @@ SSBL_API int ssbl_update_static_triangles(void* handle, const float* triangles, 
     return s->build_sdf(triangles, triangle_count) ? 1 : 0;
 }
 
-SSBL_API int ssbl_update_dynamic_triangles(void* handle, const float* triangles, int triangle_count) {
+SSBL_API int ssbl_update_dynamic_triangles(void* handle, const float* triangles, int triangle_count) {  // A41:8338-8420
     g_error.clear();
-    (void)triangles;
-    if (!handle) return fail("invalid dynamic triangle update") ? 1 : 0;
-    if (triangle_count > 0) return fail("dynamic triangle colliders are not available in this build yet") ? 1 : 0;
-    return 1;
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail("invalid dynamic triangle update") ? 1 : 0;
+    triangle_count = std::max(triangle_count, 0);
+    if (triangle_count > 0 && !triangles) return fail("dynamic triangle data is required when triangle count is nonzero") ? 1 : 0;
+    ByteScope bytes(s);
+    return s->dyn.set_triangles(lib_stream(), triangles, triangle_count, nullptr, nullptr, s->cfg.collision_margin,
+                                s->cfg.cloth_thickness, false) ? 1 : 0;
 }
 
 SSBL_API int ssbl_update_frame_inputs(void* handle, const SsblXpbdFrameInputs* in) {  // A41:9035-9085
@@ -1149,10 +1216,23 @@ SSBL_API int ssbl_update_frame_inputs(void* handle, const SsblXpbdFrameInputs* i
     ByteScope bytes(s);
     if (in->update_pin_targets && !s->upload_pins(in->pin_indices, in->pin_positions, in->pin_weights, in->pin_count)) return 0;
     if (in->update_runtime_colliders) s->colliders = in->runtime_colliders;
-    if ((in->update_dynamic_triangles && in->dynamic_triangle_count > 0) ||
-        (in->update_dynamic_indexed_triangles && in->dynamic_indexed_triangle_count > 0) ||
-        (in->update_dynamic_particles && in->dynamic_particle_count > 0))
-        return fail("dynamic colliders are not available in this build yet") ? 1 : 0;
+    if (in->update_dynamic_triangles) {
+        const int c = std::max(in->dynamic_triangle_count, 0);
+        if (c > 0 && !in->dynamic_triangles) return fail("dynamic triangle data is required when triangle count is nonzero") ? 1 : 0;
+        if (!s->dyn.set_triangles(lib_stream(), in->dynamic_triangles, c, nullptr, nullptr, s->cfg.collision_margin,
+                                  s->cfg.cloth_thickness, false))
+            return 0;
+    }
+    if (in->update_dynamic_indexed_triangles && in->dynamic_indexed_triangle_count > 0 && in->dynamic_triangle_vertex_count > 0)
+        return fail("signed indexed dynamic triangle colliders are not available in this build yet") ? 1 : 0;
+    if (in->update_dynamic_particles) {   // A41:8575-8697
+        const int c = std::max(in->dynamic_particle_count, 0);
+        if (c > 0 && (!in->dynamic_particle_positions || !in->dynamic_particle_radii))
+            return fail("dynamic particle position/radius data is required when particle count is nonzero") ? 1 : 0;
+        if (!s->dyn.set_particles(lib_stream(), in->dynamic_particle_positions, nullptr, in->dynamic_particle_radii, nullptr, nullptr, c,
+                                  s->cfg.collision_margin, s->cfg.cloth_thickness))
+            return 0;
+    }
     if (in->update_static_triangles) {
         const int c = std::max(in->static_triangle_count, 0);
         if (c > 0 && !in->static_triangles) return fail("static triangle data is required when triangle count is nonzero") ? 1 : 0;
@@ -1199,27 +1279,76 @@ SSBL_API int ssbl_get_diagnostics(void* handle, SsblXpbdDiagnostics* out) {
 }
 
 SSBL_API unsigned int ssbl_capabilities(void) {
-    return SSBL_CAP_STRETCH_OPTIMIZATION | SSBL_CAP_PIN_WEIGHTS | SSBL_CAP_HIP_BATCHED_STEP;
+    return SSBL_CAP_STRETCH_OPTIMIZATION | SSBL_CAP_PIN_WEIGHTS | SSBL_CAP_GLOBAL_DYNAMIC_SCENE | SSBL_CAP_HIP_BATCHED_STEP;
 }
 
-SSBL_API void* ssbl_create_global_dynamic_scene(void) { g_error.clear(); return new int(0); }
-SSBL_API int ssbl_destroy_global_dynamic_scene(void* scene) { g_error.clear(); delete static_cast<int*>(scene); return 1; }
-SSBL_API int ssbl_update_global_dynamic_scene(void* scene, const SsblXpbdGlobalDynamicSceneInputs* in) {
+SSBL_API void* ssbl_create_global_dynamic_scene(void) {  // A41:8733-8736
     g_error.clear();
-    if (!scene || !in) return fail("invalid global dynamic scene update") ? 1 : 0;
-    if (in->particle_count > 0 || in->triangle_count > 0) return fail("the global dynamic scene is not available in this build yet") ? 1 : 0;
+    GlobalScene* sc = new GlobalScene();
+    if (!sc->counts.alloc(kCountSlots, "global dynamic scene counts allocation")) {
+        delete sc;
+        return nullptr;
+    }
+    return sc;
+}
+
+SSBL_API int ssbl_destroy_global_dynamic_scene(void* scene) {
+    g_error.clear();
+    if (!scene) return fail("invalid global dynamic scene handle") ? 1 : 0;
+    sync_stream();
+    delete static_cast<GlobalScene*>(scene);
     return 1;
 }
-SSBL_API int ssbl_attach_global_dynamic_scene(void* handle, void* scene, int, int, int) {
+
+SSBL_API int ssbl_update_global_dynamic_scene(void* scene, const SsblXpbdGlobalDynamicSceneInputs* in) {  // A41:6597-6782
     g_error.clear();
-    if (!handle || !scene) return fail("invalid global dynamic scene attach") ? 1 : 0;
-    return fail("the global dynamic scene is not available in this build yet") ? 1 : 0;
-}
-SSBL_API int ssbl_clear_global_dynamic_scene(void* handle) {
-    g_error.clear();
-    if (!handle) return fail("invalid global dynamic scene clear") ? 1 : 0;
+    GlobalScene* sc = static_cast<GlobalScene*>(scene);
+    if (!sc || !in) return fail("invalid global dynamic scene update") ? 1 : 0;
+    const int np = std::max(in->particle_count, 0), nt = std::max(in->triangle_count, 0);
+    if (np > 0 && (!in->particle_positions || !in->particle_radii || !in->particle_source_ids || !in->particle_collision_layers))
+        return fail("global dynamic particle arrays are incomplete") ? 1 : 0;
+    if (nt > 0 && (!in->triangle_vertices || !in->triangle_source_ids || !in->triangle_collision_layers))
+        return fail("global dynamic triangle arrays are incomplete") ? 1 : 0;
+    sc->margin = std::max(in->collision_margin, 0.0f);
+    sc->thickness = std::max(in->cloth_thickness, 0.0f);
+    hipStream_t st = lib_stream();
+    launch_fill_u32(st, reinterpret_cast<unsigned*>(sc->counts.p), 0u, 2 * static_cast<size_t>(kCountSlots));
+    const float* prev = in->particle_prev_positions ? in->particle_prev_positions : in->particle_positions;
+    if (!sc->set.set_particles(st, in->particle_positions, prev, in->particle_radii, in->particle_source_ids, in->particle_collision_layers,
+                               np, sc->margin, sc->thickness) ||
+        !sc->set.set_triangles(st, in->triangle_vertices, nt, in->triangle_source_ids, in->triangle_collision_layers, sc->margin,
+                               sc->thickness, true) ||
+        !sc->set.build(st, sc->counts.p, true))
+        return 0;
+    sc->overflow_pending = true;
+    if (!sync_stream()) return 0;
+    settle_scene_overflow(*sc);
     return 1;
 }
+
+SSBL_API int ssbl_attach_global_dynamic_scene(void* handle, void* scene, int target_source_id, int target_collision_layer,
+                                              int collision_mode) {  // A41:8758-8776
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s || !scene) return fail("invalid global dynamic scene attach") ? 1 : 0;
+    s->scene = static_cast<GlobalScene*>(scene);
+    s->target_src = target_source_id;
+    s->target_layer = target_collision_layer;
+    s->cross_mode = collision_mode;
+    return 1;
+}
+
+SSBL_API int ssbl_clear_global_dynamic_scene(void* handle) {  // A41:8778-8789
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail("invalid global dynamic scene clear") ? 1 : 0;
+    s->scene = nullptr;
+    s->target_src = 0;
+    s->target_layer = 0;
+    s->cross_mode = 0;
+    return 1;
+}
+
 SSBL_API const char* ssbl_last_error(void) { return g_error.c_str(); }
 
 // ============================================================== extensions
@@ -1296,6 +1425,37 @@ SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int 
     if (!batch) return fail("invalid batch") ? 1 : 0;
     return step_batch(*static_cast<Batch*>(batch), substeps, iterations, force_sync) ? 1 : 0;
 }
+SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const* handles, const int* source_ids,
+                                                      const int* collision_layers, const float* particle_radii, int count,
+                                                      float collision_margin, float cloth_thickness) {
+    g_error.clear();
+    GlobalScene* sc = static_cast<GlobalScene*>(scene);
+    if (!sc || count < 0 || (count > 0 && (!handles || !source_ids || !collision_layers || !particle_radii)))
+        return fail("invalid global scene gather") ? 1 : 0;
+    std::vector<GatherSource> src;
+    for (int k = 0; k < count; ++k) {
+        Solver* s = static_cast<Solver*>(handles[k]);
+        if (!s) return fail("invalid solver handle in global scene gather") ? 1 : 0;
+        GatherSource g{};
+        g.pos = s->pos0.p;
+        g.tris = s->tris.p;
+        g.nv = s->nv;
+        g.nt = s->cfg.triangle_count;
+        g.radius = particle_radii[k];
+        g.source_id = source_ids[k];
+        g.layer = collision_layers[k];
+        src.push_back(g);
+    }
+    sc->margin = std::max(collision_margin, 0.0f);
+    sc->thickness = std::max(cloth_thickness, 0.0f);
+    hipStream_t st = lib_stream();
+    if (!sc->set.gather(st, src, sc->margin, sc->thickness)) return 0;
+    launch_fill_u32(st, reinterpret_cast<unsigned*>(sc->counts.p), 0u, 2 * static_cast<size_t>(kCountSlots));
+    if (!sc->set.build(st, sc->counts.p, true)) return 0;
+    sc->overflow_pending = true;
+    return 1;
+}
+
 SSBL_API long long ssbl_ext_device_bytes(void* handle) {
     g_error.clear();
     if (!handle) { fail("invalid solver handle"); return -1; }

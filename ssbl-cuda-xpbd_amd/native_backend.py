@@ -135,6 +135,7 @@ class NativeXpbdSolver:
             raise NativeSolverError("loaded ABI41 library does not support vertex-group pin weights")
         static = _f32(np.zeros((0, 9)) if static_triangles is None else static_triangles, (-1, 9))
         self.vertex_count = len(cloth.positions_world)
+        self.triangles = np.array(cloth.triangles, dtype=np.int32).reshape(-1, 3)
         self._keep = dict(
             positions=_f32(cloth.positions_world, (-1, 3)), inv_mass=_f32(cloth.inv_mass), edges=_i32(cloth.edges),
             edge_rest=_f32(cloth.edge_rest_lengths), edge_off=_i32(cloth.edge_color_offsets), bends=_i32(cloth.bends),
@@ -214,6 +215,29 @@ class NativeXpbdSolver:
         fi.force_field_count = len(fields)
         self._check(self._lib.ssbl_update_frame_inputs(self._handle, ctypes.byref(fi)))
 
+    def update_dynamic_triangles(self, triangles) -> None:
+        """Triangle soup (T x 3 x 3) this cloth collides with (native_backend.py ssbl_update_dynamic_triangles)."""
+        t = _f32(triangles, (-1, 9))
+        self._check(self._lib.ssbl_update_dynamic_triangles(self._handle, _fptr(t), len(t)))
+
+    def update_dynamic_particles(self, positions, radii) -> None:
+        p = _f32(positions, (-1, 3))
+        r = _f32(radii)
+        fi = abi.FrameInputs()
+        fi.update_dynamic_particles = 1
+        fi.dynamic_particle_positions = _fptr(p)
+        fi.dynamic_particle_radii = _fptr(r)
+        fi.dynamic_particle_count = len(p)
+        self._check(self._lib.ssbl_update_frame_inputs(self._handle, ctypes.byref(fi)))
+
+    def attach_global_dynamic_scene(self, scene, target_source_id: int = 0, target_collision_layer: int = 0,
+                                    cross_mode: int = 1) -> None:
+        self._check(self._lib.ssbl_attach_global_dynamic_scene(self._handle, scene.handle, int(target_source_id),
+                                                               int(target_collision_layer), int(cross_mode)))
+
+    def clear_global_dynamic_scene(self) -> None:
+        self._check(self._lib.ssbl_clear_global_dynamic_scene(self._handle))
+
     def step(self, substeps: int, iterations: int, diagnostics: bool = True, synchronize: bool = True) -> None:
         self._check(self._lib.ssbl_step_solver_ex(self._handle, int(substeps), int(iterations), int(bool(diagnostics)),
                                                    int(bool(synchronize or diagnostics))))
@@ -256,6 +280,66 @@ class NativeXpbdSolver:
         vals = np.empty(int(np.prod(dims)), np.float32)
         self._check(self._lib.ssbl_ext_static_sdf(self._handle, _iptr(dims), _fptr(ov), _fptr(vals), vals.size))
         return vals.reshape(dims[2], dims[1], dims[0]), ov
+
+
+class GlobalDynamicScene:
+    """Colliders shared by the cloths of a session (ssbl_*_global_dynamic_scene)."""
+
+    CROSS_OFF, CROSS_ALL, CROSS_LOWER_LAYERS = 0, 1, 2
+
+    def __init__(self, lib=None):
+        self._lib = lib if lib is not None else load_library()
+        self._handle = self._lib.ssbl_create_global_dynamic_scene()
+        if not self._handle:
+            raise NativeSolverError(last_error(self._lib))
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def update(self, *, particle_positions, particle_radii, particle_source_ids, particle_collision_layers,
+               triangle_vertices, triangle_source_ids, triangle_collision_layers, collision_margin: float,
+               cloth_thickness: float, particle_prev_positions=None) -> None:
+        keep = dict(pp=_f32(particle_positions, (-1, 3)), pr=_f32(particle_radii), ps=_i32(particle_source_ids),
+                    pl=_i32(particle_collision_layers), tv=_f32(triangle_vertices, (-1, 9)), ts=_i32(triangle_source_ids),
+                    tl=_i32(triangle_collision_layers),
+                    pq=None if particle_prev_positions is None else _f32(particle_prev_positions, (-1, 3)))
+        inp = abi.GlobalDynamicSceneInputs()
+        inp.particle_positions = _fptr(keep["pp"])
+        inp.particle_prev_positions = _fptr(keep["pq"])
+        inp.particle_radii = _fptr(keep["pr"])
+        inp.particle_source_ids = _iptr(keep["ps"])
+        inp.particle_collision_layers = _iptr(keep["pl"])
+        inp.particle_count = len(keep["pp"])
+        inp.triangle_vertices = _fptr(keep["tv"])
+        inp.triangle_source_ids = _iptr(keep["ts"])
+        inp.triangle_collision_layers = _iptr(keep["tl"])
+        inp.triangle_count = len(keep["tv"])
+        inp.collision_margin = float(collision_margin)
+        inp.cloth_thickness = float(cloth_thickness)
+        if not self._lib.ssbl_update_global_dynamic_scene(self._handle, ctypes.byref(inp)):
+            raise NativeSolverError(last_error(self._lib))
+
+    def update_from_solvers(self, solvers, source_ids, collision_layers, particle_radii, collision_margin: float,
+                            cloth_thickness: float) -> None:
+        """Every solver's current vertices (particles) and mesh triangles, gathered by the library."""
+        handles = (ctypes.c_void_p * len(solvers))(*[s.handle for s in solvers])
+        ids, layers, radii = _i32(source_ids), _i32(collision_layers), _f32(particle_radii)
+        if not self._lib.ssbl_ext_update_global_scene_from_handles(self._handle, handles, _iptr(ids), _iptr(layers),
+                                                                   _fptr(radii), len(solvers), float(collision_margin),
+                                                                   float(cloth_thickness)):
+            raise NativeSolverError(last_error(self._lib))
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            self._lib.ssbl_destroy_global_dynamic_scene(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class SolverBatch:

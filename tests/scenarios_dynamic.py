@@ -1,0 +1,132 @@
+"""Dynamic collider scenes (triangle soups, particles, a global scene of
+several cloths) shared by the oracle tests and the GPU parity tests."""
+import ctypes
+
+import numpy as np
+
+from raw_abi import Raw
+from ssbl_xpbd_amd import abi
+
+
+def plane_soup(z, half=1.0, n=1):
+    """An n x n grid of quads (2 n^2 triangles) covering [-half, half]^2 at height z."""
+    s = np.linspace(-half, half, n + 1)
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            a = (s[i], s[j], z)
+            b = (s[i + 1], s[j], z)
+            c = (s[i + 1], s[j + 1], z)
+            d = (s[i], s[j + 1], z)
+            tris.append([a, b, c])
+            tris.append([a, c, d])
+    return np.asarray(tris, np.float32).reshape(-1, 9)
+
+
+def square(lib, margin=0.01):
+    return Raw(lib, [[-0.2, -0.2, 0.0], [0.2, -0.2, 0.0], [0.2, 0.2, 0.0], [-0.2, 0.2, 0.0]], [1.0] * 4,
+               collision_margin=margin, cloth_thickness=0.005)
+
+
+def plane_contact(lib, quads_per_side, depth=0.004, margin=0.01):
+    """Four free vertices at z = 0 over a dynamic plane at z = -depth: each is
+    pushed up by half the penetration, margin - depth (A41:4310-4322)."""
+    r = square(lib, margin)
+    soup = plane_soup(-depth, 1.0, quads_per_side)
+    r.check(lib.ssbl_update_dynamic_triangles(r.h, soup.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(soup)))
+    r.step(1, 1)
+    return r.state(), r.diag()
+
+
+def particle_contact(lib, far_particles=0, distance=0.01, radius=0.008, margin=0.005):
+    """A free vertex over one particle (plus far-away ones): pushed up by half of
+    radius + margin - distance (A41:4504-4507)."""
+    r = Raw(lib, [[0.0, 0.0, 0.0]], [1.0], collision_margin=margin, cloth_thickness=0.005)
+    pos = [[0.0, 0.0, -distance]] + [[10.0 + 0.5 * k, 3.0, 0.0] for k in range(far_particles)]
+    p = np.asarray(pos, np.float32)
+    rad = np.full(len(p), radius, np.float32)
+    fi = abi.FrameInputs()
+    fi.update_dynamic_particles = 1
+    fi.dynamic_particle_positions = p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    fi.dynamic_particle_radii = rad.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    fi.dynamic_particle_count = len(p)
+    r.check(lib.ssbl_update_frame_inputs(r.h, ctypes.byref(fi)))
+    r.step(1, 1)
+    return r.state(), r.diag()
+
+
+def wavy_soup(n=24, half=0.8, z0=0.6, amp=0.08, seed=0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = np.linspace(-half, half, n + 1)
+    x, y = np.meshgrid(s, s, indexing="ij")
+    z = z0 + amp * np.sin(3.0 * x) * np.cos(2.0 * y) + rng.normal(0.0, 0.002, size=x.shape)
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            a = (x[i, j], y[i, j], z[i, j])
+            b = (x[i + 1, j], y[i + 1, j], z[i + 1, j])
+            c = (x[i + 1, j + 1], y[i + 1, j + 1], z[i + 1, j + 1])
+            d = (x[i, j + 1], y[i, j + 1], z[i, j + 1])
+            tris.append([a, b, c])
+            tris.append([a, c, d])
+    return np.asarray(tris, np.float32).reshape(-1, 9)
+
+
+def particle_cloud(count=400, seed=1):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = np.stack([rng.uniform(-0.5, 0.5, count), rng.uniform(-0.5, 0.5, count), rng.uniform(0.45, 0.75, count)], 1)
+    return p.astype(np.float32), rng.uniform(0.01, 0.03, count).astype(np.float32)
+
+
+def cloth_over_colliders(nb, lib, scenes, *, triangles=True, particles=True, soup_n=24, n_particles=400):
+    """A 20 x 20 cloth falling onto a wavy triangle soup and a particle cloud."""
+    cloth = scenes.grid_cloth(20, 1.0, 1.0, pin_corners=False, seed=4)
+    opts = scenes.SolverOptions(collision_margin=0.01, cloth_thickness=0.005)
+    s = nb.NativeXpbdSolver(cloth, opts, None, lib=lib)
+    if triangles:
+        s.update_dynamic_triangles(wavy_soup(soup_n))
+    if particles:
+        p, r = particle_cloud(n_particles)
+        s.update_dynamic_particles(p, r)
+    return s
+
+
+def two_cloth_scene(nb, lib, scenes, mode):
+    """Cloth A (source 1, layer 0) below cloth B (source 2, layer 1); both
+    attached to one global scene refreshed from their positions every frame."""
+    pos, tris = scenes.grid_mesh(16, 0.8)
+    pos[:, 1] += 0.5
+    pin = np.zeros(16 * 16, bool)
+    pin[[0, 15, 240, 255]] = True                      # a trampoline: four corners held
+    cloth_a = scenes.build_cloth(scenes.y_up_to_z_up(pos), tris, pin, jitter=1.0e-4, seed=1)
+    a = nb.NativeXpbdSolver(cloth_a, scenes.SolverOptions(collision_margin=0.01, cloth_thickness=0.005), None, lib=lib)
+    b = nb.NativeXpbdSolver(scenes.grid_cloth(10, 0.3, 0.58, pin_corners=False, seed=2),
+                            scenes.SolverOptions(collision_margin=0.01, cloth_thickness=0.005), None, lib=lib)
+    scene = nb.GlobalDynamicScene(lib=lib)
+    a.attach_global_dynamic_scene(scene, 1, 0, mode)
+    b.attach_global_dynamic_scene(scene, 2, 1, mode)
+    return a, b, scene
+
+
+def refresh_scene(scene, solvers, device_gather):
+    ids, layers, radii = [1, 2], [0, 1], [0.01, 0.01]
+    if device_gather:
+        scene.update_from_solvers(solvers, ids, layers, radii, 0.01, 0.01)
+        return
+    pos = [s.download_positions() for s in solvers]
+    tris = _triangles_of(solvers, pos)
+    scene.update(particle_positions=np.concatenate(pos), particle_radii=np.concatenate([np.full(len(p), r, np.float32) for p, r in zip(pos, radii)]),
+                 particle_source_ids=np.concatenate([np.full(len(p), i, np.int32) for p, i in zip(pos, ids)]),
+                 particle_collision_layers=np.concatenate([np.full(len(p), l, np.int32) for p, l in zip(pos, layers)]),
+                 triangle_vertices=np.concatenate([t for t, _ in tris]),
+                 triangle_source_ids=np.concatenate([np.full(len(t), i, np.int32) for (t, _), i in zip(tris, ids)]),
+                 triangle_collision_layers=np.concatenate([np.full(len(t), l, np.int32) for (t, _), l in zip(tris, layers)]),
+                 collision_margin=0.01, cloth_thickness=0.01)
+
+
+def _triangles_of(solvers, pos):
+    out = []
+    for s, p in zip(solvers, pos):
+        t = s.triangles
+        out.append((p[t].reshape(-1, 9), t))
+    return out

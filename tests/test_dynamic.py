@@ -1,0 +1,93 @@
+"""Dynamic colliders in the oracle: closed forms for one triangle contact and
+one particle contact (both on the brute-force and the hashed query path),
+hash queries equal to brute-force scans, and the global scene's source and
+layer filter (A41:3984-3997)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import scenarios_dynamic as sd
+
+
+@pytest.mark.parametrize("quads", [1, 8])          # 2 triangles: scan; 128: hashed
+def test_plane_contact_closed_form(oracle, quads):
+    (p, q, v), d = sd.plane_contact(oracle, quads)
+    assert np.allclose(p[:, 2], 0.5 * (0.01 - 0.004), atol=1.0e-8)
+    assert np.allclose(p[:, :2], [[-0.2, -0.2], [0.2, -0.2], [0.2, 0.2], [-0.2, 0.2]], atol=0)
+    assert np.all(v == 0.0)
+    assert d.fast_triangle_pair_contacts == 4 and d.abi41_exact_impulse_contact_count == 4
+    assert d.dynamic_triangle_count == 2 * quads * quads
+    assert d.dynamic_triangle_candidate_count >= 4
+
+
+@pytest.mark.parametrize("far", [0, 200])         # 1 particle: scan; 201: hashed
+def test_particle_contact_closed_form(oracle, far):
+    (p, q, v), d = sd.particle_contact(oracle, far_particles=far)
+    expected = np.float32(0.5) * (np.float32(0.008) + np.float32(0.005) - np.float32(0.01))
+    assert abs(float(p[0, 2]) - float(expected)) < 1.0e-8
+    assert d.dynamic_particle_contacts == 1 and d.dynamic_particle_candidate_count == 1
+
+
+COUNTERS = ("fast_triangle_pair_contacts", "dynamic_particle_contacts", "dynamic_triangle_candidate_count",
+            "dynamic_particle_candidate_count", "dynamic_triangle_aabb_reject_count", "abi41_exact_impulse_contact_count")
+
+
+def run_cloth(nb, lib, scenes, frames=30, **kw):
+    """Positions per frame and the dynamic counters summed over the frames."""
+    s = sd.cloth_over_colliders(nb, lib, scenes, **kw)
+    out, tot = [], dict.fromkeys(COUNTERS, 0)
+    for _ in range(frames):
+        s.step(6, 4, diagnostics=True, synchronize=True)
+        out.append(s.download_positions())
+        d = s.diagnostics()
+        for k in COUNTERS:
+            tot[k] += getattr(d, k)
+    s.close()
+    return out, tot
+
+
+def test_hash_queries_equal_brute_force(nb, oracle, scenes):
+    hashed, dh = run_cloth(nb, oracle, scenes)
+    oracle.ssbl_oracle_dynamic_hash.argtypes = [ctypes.c_int]
+    oracle.ssbl_oracle_dynamic_hash(0)
+    try:
+        scanned, ds = run_cloth(nb, oracle, scenes)
+    finally:
+        oracle.ssbl_oracle_dynamic_hash(1)
+    assert dh["fast_triangle_pair_contacts"] > 0 and dh["dynamic_particle_contacts"] > 0
+    for a, b in zip(hashed, scanned):
+        assert np.array_equal(a, b)
+    assert dh["fast_triangle_pair_contacts"] == ds["fast_triangle_pair_contacts"]
+    assert dh["dynamic_particle_contacts"] == ds["dynamic_particle_contacts"]
+    # the hash visits fewer triangles than the scan
+    assert dh["dynamic_triangle_candidate_count"] < ds["dynamic_triangle_candidate_count"]
+
+
+def test_scene_layers_and_sources(nb, oracle, scenes):
+    a, b, scene = sd.two_cloth_scene(nb, oracle, scenes, nb.GlobalDynamicScene.CROSS_LOWER_LAYERS)
+    seen_a = seen_b = contacts_b = 0
+    for _ in range(30):
+        sd.refresh_scene(scene, [a, b], device_gather=False)
+        a.step(6, 4, diagnostics=True)
+        b.step(6, 4, diagnostics=True)
+        da, db = a.diagnostics(), b.diagnostics()
+        seen_a += da.dynamic_triangle_candidate_count + da.dynamic_particle_candidate_count
+        seen_b += db.dynamic_triangle_candidate_count + db.dynamic_particle_candidate_count
+        contacts_b += db.dynamic_particle_contacts + db.fast_triangle_pair_contacts
+    # lower layers: B (layer 1) sees A (layer 0); A sees nothing, and nobody sees itself
+    assert seen_a == 0 and seen_b > 0 and contacts_b > 0
+    assert db.global_dynamic_particle_count == 16 * 16 + 10 * 10
+    assert db.global_dynamic_triangle_count == 2 * 15 * 15 + 2 * 9 * 9
+
+
+def test_scene_gather_equals_host_packing(nb, oracle, scenes):
+    res = []
+    for gather in (False, True):
+        a, b, scene = sd.two_cloth_scene(nb, oracle, scenes, nb.GlobalDynamicScene.CROSS_ALL)
+        for _ in range(6):
+            sd.refresh_scene(scene, [a, b], device_gather=gather)
+            a.step(4, 3, diagnostics=False)
+            b.step(4, 3, diagnostics=False)
+        res.append((a.download_positions(), b.download_positions()))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
