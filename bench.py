@@ -29,7 +29,7 @@ sys.path.insert(0, str(ROOT))
 import __graft_entry__  # noqa: E402
 
 __graft_entry__.load_package()
-from ssbl_xpbd_amd import native_backend as nb, scenes  # noqa: E402
+from ssbl_xpbd_amd import cross_cloth, native_backend as nb, scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -113,14 +113,35 @@ def create(scene, lib=None):
     return [nb.NativeXpbdSolver(c, o, scene.static_triangles, lib=lib) for c, o in zip(scene.cloths, scene.options)]
 
 
-def run_plain(solvers, scene, frames):
+def cross_scene(solvers, scene, lib=None):
+    if not scene.layers or len(solvers) < 2:
+        return None
+    return cross_cloth.CrossClothScene(solvers, scene.layers[:len(solvers)], contact_distance=scene.contact_distance, lib=lib)
+
+
+def cross_contacts(solvers, scene):
+    """Dynamic (cross-cloth) contacts of one frame, summed over the cloths."""
+    total = 0
+    for s in solvers:
+        s.step(scene.substeps, scene.iterations, diagnostics=True, synchronize=True)
+        d = s.diagnostics()
+        total += int(d.fast_triangle_pair_contacts + d.dynamic_particle_contacts)
+    return total
+
+
+def run_plain(solvers, scene, frames, cross=None):
+    """The reference's per-slot loop: scene packed on the host, then every slot stepped."""
     for _ in range(frames):
+        if cross is not None:
+            cross.refresh(device=False)
         for s in solvers:
             s.step(scene.substeps, scene.iterations, diagnostics=False, synchronize=False)
 
 
-def run_batched(batch, scene, frames):
+def run_batched(batch, scene, frames, cross=None):
     for _ in range(frames):
+        if cross is not None:
+            cross.refresh(device=True)
         batch.step(scene.substeps, scene.iterations)
 
 
@@ -157,22 +178,29 @@ def kernel_stats(lib, fn):
 def cpu_baseline(scene, budget_s: float):
     """The CPU oracle (FP32, single thread) on a bounded sample of the workload."""
     lib = nb.load_library(str(ROOT / "oracle" / "liboracle_abi41.so"))
-    order = np.argsort([len(c.positions_world) for c in scene.cloths])
-    picked, work = [], 0
-    for k in order[::max(1, len(order) // 8)]:
-        picked.append(int(k))
+    if scene.layers and len(scene.cloths) >= 8:
+        picked = list(range(8))            # the first two stacks of four (cross-cloth collision within them)
+    else:
+        order = np.argsort([len(c.positions_world) for c in scene.cloths])
+        picked = [int(k) for k in order[::max(1, len(order) // 8)]]
     solvers = [nb.NativeXpbdSolver(scene.cloths[k], scene.options[k], scene.static_triangles, lib=lib) for k in picked]
+    cross = None
+    if scene.layers and len(solvers) > 1:
+        cross = cross_cloth.CrossClothScene(solvers, [scene.layers[k] for k in picked], contact_distance=scene.contact_distance, lib=lib)
     verts = sum(s.vertex_count for s in solvers)
     frames = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < budget_s and frames < scene.frames:
+        if cross is not None:
+            cross.refresh(device=True)
         for s in solvers:
             s.step(scene.substeps, scene.iterations, diagnostics=False, synchronize=False)
         frames += 1
     dt = time.perf_counter() - t0
     work = verts * scene.substeps * frames
     return {"value": work / dt, "unit": "vertex-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP32 single thread, {len(solvers)} of {len(scene.cloths)} cloths ({verts} vertices) x {frames} frames"}
+            "sample": f"oracle FP32 single thread, {len(solvers)} of {len(scene.cloths)} cloths ({verts} vertices) x {frames} frames"
+                      + (", cross-cloth scene packed every frame" if cross is not None else "")}
 
 
 def main():
@@ -214,28 +242,30 @@ def main():
     if args.batched_only:
         members = create(scene)
         batch = nb.SolverBatch(members)
-        run_batched(batch, scene, args.warmup + args.steps)
+        run_batched(batch, scene, args.warmup + args.steps, cross_scene(members, scene))
         nb.synchronize(lib)
         print(json.dumps({"batched_only_frames": args.warmup + args.steps, "workload": scene.name}), flush=True)
         return
     plain = create(scene)
     members = create(scene)
+    cross_p, cross_b = cross_scene(plain, scene), cross_scene(members, scene)
     batch = nb.SolverBatch(members) if len(members) > 1 else None
     work_per_frame = scene.vertex_count * scene.substeps
 
-    run_plain(plain, scene, args.warmup)
-    t_plain = timed(lambda: run_plain(plain, scene, args.steps), lib)
+    run_plain(plain, scene, args.warmup, cross_p)
+    t_plain = timed(lambda: run_plain(plain, scene, args.steps, cross_p), lib)
     print(f"# plain: {args.steps} frames in {t_plain:.3f} s", file=sys.stderr, flush=True)
     if batch is not None:
-        run_batched(batch, scene, args.warmup)
-        t_batch = timed(lambda: run_batched(batch, scene, args.steps), lib)
-        step_frame = lambda: run_batched(batch, scene, 1)  # noqa: E731
+        run_batched(batch, scene, args.warmup, cross_b)
+        t_batch = timed(lambda: run_batched(batch, scene, args.steps, cross_b), lib)
+        step_frame = lambda: run_batched(batch, scene, 1, cross_b)  # noqa: E731
     else:
         t_batch = t_plain
-        step_frame = lambda: run_plain(plain, scene, 1)  # noqa: E731
+        step_frame = lambda: run_plain(plain, scene, 1, cross_p)  # noqa: E731
     print(f"# batched: {args.steps} frames in {t_batch:.3f} s", file=sys.stderr, flush=True)
     lat_b = latencies(step_frame, lib, args.latency_frames)
-    lat_p = latencies(lambda: run_plain(plain, scene, 1), lib, args.latency_frames)
+    lat_p = latencies(lambda: run_plain(plain, scene, 1, cross_p), lib, args.latency_frames)
+    contacts = cross_contacts(members, scene) if cross_b is not None else None
 
     secondary = single_cloth_line("selfcol_100x100", args.seed, 10, 2, lib) if args.workload != "selfcol_100x100" else None
     serving = serving_line(args.seed, args.serving_window, args.serving_max_s, lib) if args.serving_window > 0 else None
@@ -269,7 +299,9 @@ def main():
             "vertices": scene.vertex_count,
             "substeps": scene.substeps,
             "iterations": scene.iterations,
-            "cross_cloth_collision": False,
+            "cross_cloth_collision": cross_b is not None,
+            "cross_cloth_layers": 4 if cross_b is not None else 0,
+            "cross_cloth_contacts_per_frame": contacts,
             "plain_vertex_substeps_per_s": plain_value,
             "batched_over_plain": value / plain_value,
             "frame_ms_p50_batched": float(np.percentile(lat_b, 50)),

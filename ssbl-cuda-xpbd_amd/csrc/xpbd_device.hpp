@@ -49,7 +49,6 @@ enum PassOps : unsigned {
     kOpCount = 1u << 7,       // count diagnostics
     kOpLastIter = 1u << 8,    // last iteration of the substep (pin polish always runs)
     kOpBend = 1u << 9,        // uncoloured bend gather (src -> dst), instead of stretch
-    kOpDynamic = 1u << 10,    // dynamic triangle and particle contacts (last iteration)
 };
 
 struct DynView;
@@ -142,6 +141,7 @@ struct LaunchCtx {
 
 void launch_integrate(const LaunchCtx& c, int parity, unsigned ops, bool full);
 void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops);
+void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops);
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops);
 void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float ox, float oy, float oz, float voxel,
                       const float* tris, int ntris, int signed_sdf);

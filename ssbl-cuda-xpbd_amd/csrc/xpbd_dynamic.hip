@@ -100,47 +100,98 @@ __device__ __forceinline__ bool span_cell(int4 a, int4 b, int idx, int* x, int* 
     return true;
 }
 
-// One wave per triangle, one lane per cell.  phase 0 counts (keeping the
-// arrival slot), 1 scatters keys to raw[off[b] + slot].
+constexpr int kTriLanes = 16;   // lanes per triangle in the hash builds (cells beyond 16 loop)
+
+// kTriLanes lanes per triangle, lane l taking cells l, l + 16, ...  phase 0
+// counts (keeping the arrival slot), 1 scatters keys to raw[off[b] + slot].
 __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv, const int4* __restrict__ cmin,
                                                   const int4* __restrict__ cmax, const int* __restrict__ large_flag, int* count,
-                                                  const int* __restrict__ off, int* slot, int* raw, int* rawb, int* occupancy,
-                                                  int phase) {
-    const int t = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int idx = static_cast<int>(threadIdx.x & 63u);
+                                                  const int* __restrict__ off, int* slot, int* raw, int* rawb, int phase) {
+    const int t = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kTriLanes);
+    const int lane = static_cast<int>(threadIdx.x % kTriLanes);
     if (t >= dv->nt || large_flag[t]) return;
-    int x, y, z;
-    if (!span_cell(cmin[t], cmax[t], idx, &x, &y, &z)) return;
-    const int b = static_cast<int>(hash_cell(x, y, z, dv->tri_buckets));
-    const int key = t * kDynMaxHashCells + idx;
-    if (phase == 0) {
-        const int s = atomicAdd(&count[b], 1);
-        slot[key] = s;
-        atomicMax(occupancy, s + 1);
-    } else {
-        const int j = off[b] + slot[key];
-        raw[j] = key;
-        rawb[j] = b;
+    const int4 lo = cmin[t], hi = cmax[t];
+    for (int idx = lane; idx < kDynMaxHashCells; idx += kTriLanes) {
+        int x, y, z;
+        if (!span_cell(lo, hi, idx, &x, &y, &z)) return;
+        const int b = static_cast<int>(hash_cell(x, y, z, dv->tri_buckets));
+        const int key = t * kDynMaxHashCells + idx;
+        if (phase == 0) {
+            slot[key] = atomicAdd(&count[b], 1);
+        } else {
+            const int j = off[b] + slot[key];
+            raw[j] = key;
+            rawb[j] = b;
+        }
     }
 }
 
-// Every scattered key lands at its rank among its bucket's keys.
+// Max over a 256-thread block (wave shuffles, then the four wave maxima).
+__device__ __forceinline__ int block_max(int m) {
+    __shared__ int wave_max[4];
+    for (int d = 32; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63u) == 0u) wave_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    return max(max(wave_max[0], wave_max[1]), max(wave_max[2], wave_max[3]));
+}
+
+// Largest bucket (diagnostics): one atomic per block.
+__global__ __launch_bounds__(256) void k_bucket_max(const int* __restrict__ count, int buckets, int* out) {
+    int m = 0;
+    for (int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); b < buckets; b += static_cast<int>(gridDim.x * blockDim.x))
+        m = max(m, count[b]);
+    m = block_max(m);
+    if (threadIdx.x == 0 && m > 0) atomicMax(out, m);
+}
+
+// Every scattered key lands at its rank among its bucket's keys, with its
+// triangle's {source, layer} beside it (grid-stride over the entries).
 __global__ __launch_bounds__(256) void k_tri_rank(const int* __restrict__ off, int buckets, const int* __restrict__ count,
-                                                  const int* __restrict__ raw, const int* __restrict__ rawb, int* keys) {
-    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j >= off[buckets]) return;
-    const int b = rawb[j], key = raw[j];
-    const int base = off[b], n = count[b];
-    int rank = 0;
-    for (int k = 0; k < n; ++k) rank += raw[base + k] < key ? 1 : 0;
-    keys[base + rank] = key;
+                                                  const int* __restrict__ raw, const int* __restrict__ rawb, int* keys,
+                                                  const int* __restrict__ tsrc, const int* __restrict__ tlayer, int2* ksl) {
+    const int total = off[buckets];
+    for (int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); j < total; j += static_cast<int>(gridDim.x * blockDim.x)) {
+        const int b = rawb[j], key = raw[j];
+        const int base = off[b], n = count[b];
+        int rank = 0;
+        for (int k = 0; k < n; ++k) rank += raw[base + k] < key ? 1 : 0;
+        keys[base + rank] = key;
+        if (tsrc) {
+            const int t = key / kDynMaxHashCells;
+            ksl[base + rank] = make_int2(tsrc[t], tlayer[t]);
+        }
+    }
+}
+
+// One 16-byte bin per bucket for the queries: {offset, count, smallest layer,
+// the source every one of the first `cap` entries shares (-1 when they
+// differ, -2 when the bucket is empty)}.  A query whose filter rejects every
+// entry of a bucket skips it (A41:3984-3997 filters each entry the same way,
+// so results are equal).  Sets without sources get layer INT_MIN, source -1.
+__global__ __launch_bounds__(256) void k_bucket_bins(const int* __restrict__ off, const int* __restrict__ count, int buckets,
+                                                     int cap, const int2* __restrict__ ksl, int4* bins) {
+    for (int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); b < buckets; b += static_cast<int>(gridDim.x * blockDim.x)) {
+        const int n = count[b], base = off[b];
+        int min_layer = ksl ? 0x7fffffff : static_cast<int>(0x80000000u), src = -2;
+        if (ksl) {
+            for (int k = 0; k < min(n, cap); ++k) {
+                const int2 e = ksl[base + k];
+                min_layer = min(min_layer, e.y);
+                src = (src == -2 || src == e.x) ? e.x : -1;
+            }
+        } else if (n > 0) {
+            src = -1;
+        }
+        bins[b] = make_int4(base, n, min_layer, src);
+    }
 }
 
 // A41:4325-4352 with rank placement: phase 0 counts, 1 scatters, 2 ranks
 // (the first 16 particles of a bucket by index are visible to queries).
 __global__ __launch_bounds__(256) void k_part_hash(const DynView* __restrict__ dv, int4* pcell, int* pcount, const int* __restrict__ poff,
-                                                   int* pslot, int* praw, int* pkeys, unsigned long long* counts, int count_diag,
-                                                   int phase) {
+                                                   int* pslot, int* praw, int* pkeys, int2* pksl, int4* pkcell,
+                                                   const int* __restrict__ psrc, const int* __restrict__ player,
+                                                   unsigned long long* counts, int count_diag, int phase) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const bool active = i < dv->np;
     if (phase == 0) {
@@ -162,8 +213,13 @@ __global__ __launch_bounds__(256) void k_part_hash(const DynView* __restrict__ d
         const int b = pcell[i].w, base = poff[b], n = pcount[b];
         int rank = 0;
         for (int k = 0; k < n && rank < kDynParticleSlots; ++k) rank += praw[base + k] < i ? 1 : 0;
-        if (rank < kDynParticleSlots) pkeys[base + rank] = i;
-        else over = true;
+        if (rank < kDynParticleSlots) {
+            pkeys[base + rank] = i;
+            pkcell[base + rank] = pcell[i];
+            if (psrc) pksl[base + rank] = make_int2(psrc[i], player[i]);
+        } else {
+            over = true;
+        }
     }
     if (count_diag) wave_count(counts, kCountDynParticleOverflow, over);
 }
@@ -188,23 +244,27 @@ __global__ __launch_bounds__(256) void k_gather_triangles(const GatherSource* __
                                                           float4* tv, int* tsrc, int* tlayer, int* extent_bits) {
     const int4 sg = segs[blockIdx.x];
     const int t = sg.y + static_cast<int>(threadIdx.x);
-    if (t >= sg.z) return;
+    const bool active = t < sg.z;
     const GatherSource S = src[sg.x];
-    const int i = S.tbase + t;
-    const int4 tr = S.tris[t];
+    const int i = S.tbase + (active ? t : 0);
+    const int4 tr = active ? S.tris[t] : make_int4(-1, -1, -1, 0);
     const bool ok = tr.x >= 0 && tr.y >= 0 && tr.z >= 0 && tr.x < S.nv && tr.y < S.nv && tr.z < S.nv;
     const F3 a = ok ? f3(S.pos[tr.x]) : f3(0.0f, 0.0f, 0.0f);
     const F3 b = ok ? f3(S.pos[tr.y]) : f3(0.0f, 0.0f, 0.0f);
     const F3 c = ok ? f3(S.pos[tr.z]) : f3(0.0f, 0.0f, 0.0f);
-    tv[3 * i] = with_w(a, 0.0f);
-    tv[3 * i + 1] = with_w(b, 0.0f);
-    tv[3 * i + 2] = with_w(c, 0.0f);
-    tsrc[i] = S.source_id;
-    tlayer[i] = S.layer;
+    if (active) {
+        tv[3 * i] = with_w(a, 0.0f);
+        tv[3 * i + 1] = with_w(b, 0.0f);
+        tv[3 * i + 2] = with_w(c, 0.0f);
+        tsrc[i] = S.source_id;
+        tlayer[i] = S.layer;
+    }
     const float ext = fmaxf(fmaxf(a.x, fmaxf(b.x, c.x)) - fminf(a.x, fminf(b.x, c.x)),
                             fmaxf(fmaxf(a.y, fmaxf(b.y, c.y)) - fminf(a.y, fminf(b.y, c.y)),
                                   fmaxf(a.z, fmaxf(b.z, c.z)) - fminf(a.z, fminf(b.z, c.z))));
-    if (isfinite(ext) && ext >= 0.0f) atomicMax(extent_bits, __float_as_int(ext));   // non-negative floats order as ints
+    // non-negative finite floats order as ints; one atomic per block
+    const int e = block_max((active && isfinite(ext) && ext >= 0.0f) ? __float_as_int(ext) : 0);
+    if (threadIdx.x == 0) atomicMax(extent_bits, e);
 }
 
 // The scene's triangle cell from the gathered extent (A41:6697-6702).
@@ -264,6 +324,8 @@ bool DynamicColliders::reserve_triangles(int n) {
     tri_raw_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
     tri_rawb_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
     tri_keys_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
+    tri_ksl_ = static_cast<int2*>(alloc(0, sizeof(int2) * m * kDynMaxHashCells));
+    tri_bin_ = static_cast<int4*>(alloc(0, sizeof(int4) * buckets));
     occupancy_ = static_cast<int*>(alloc(0, sizeof(int)));
     extent_bits_ = static_cast<int*>(alloc(0, sizeof(int)));
     const int need_scan = std::max(n, buckets);
@@ -273,7 +335,7 @@ bool DynamicColliders::reserve_triangles(int n) {
         scan_cap_ = scan_tmp_ ? need_scan : 0;
     }
     if (!tv_ || !tsrc_ || !tlayer_ || !lo_ || !hi_ || !ctr_ || !cmin_ || !cmax_ || !large_flag_ || !large_off_ || !large_ ||
-        !tri_count_ || !tri_off_ || !tri_slot_ || !tri_raw_ || !tri_rawb_ || !tri_keys_ || !occupancy_ || !extent_bits_ || !scan_tmp_)
+        !tri_count_ || !tri_off_ || !tri_slot_ || !tri_raw_ || !tri_rawb_ || !tri_keys_ || !tri_ksl_ || !tri_bin_ || !occupancy_ || !extent_bits_ || !scan_tmp_)
         return false;
     tri_cap_ = n;
     tri_buckets_ = buckets;
@@ -297,7 +359,10 @@ bool DynamicColliders::reserve_particles(int n) {
     pslot_ = static_cast<int*>(alloc(1, sizeof(int) * m));
     praw_ = static_cast<int*>(alloc(1, sizeof(int) * m));
     pkeys_ = static_cast<int*>(alloc(1, sizeof(int) * m));
+    pksl_ = static_cast<int2*>(alloc(1, sizeof(int2) * m));
+    pkcell_ = static_cast<int4*>(alloc(1, sizeof(int4) * m));
     const int buckets = next_pow2(std::max(kDynMinBuckets, n * 4));
+    pbin_ = static_cast<int4*>(alloc(1, sizeof(int4) * buckets));
     pcount_ = static_cast<int*>(alloc(1, sizeof(int) * buckets));
     poff_ = static_cast<int*>(alloc(1, sizeof(int) * (buckets + 1)));
     if (buckets > scan_cap_) {
@@ -305,7 +370,7 @@ bool DynamicColliders::reserve_particles(int n) {
         scan_tmp_ = static_cast<int*>(alloc(2, sizeof(int) * scan_tmp_ints(buckets)));
         scan_cap_ = scan_tmp_ ? buckets : 0;
     }
-    if (!pp_ || !pprev_ || !psrc_ || !player_ || !pcell_ || !pslot_ || !praw_ || !pkeys_ || !pcount_ || !poff_ || !scan_tmp_) return false;
+    if (!pp_ || !pprev_ || !psrc_ || !player_ || !pcell_ || !pslot_ || !praw_ || !pkeys_ || !pksl_ || !pbin_ || !pkcell_ || !pcount_ || !poff_ || !scan_tmp_) return false;
     part_cap_ = n;
     part_buckets_ = buckets;
     return true;
@@ -322,7 +387,8 @@ bool DynamicColliders::upload_view(hipStream_t st) {
     v.tsrc = tri_filter_ ? tsrc_ : nullptr;
     v.tlayer = tri_filter_ ? tlayer_ : nullptr;
     v.lo = lo_; v.hi = hi_; v.ctr = ctr_; v.cmin = cmin_; v.cmax = cmax_;
-    v.tri_off = tri_off_; v.tri_keys = tri_keys_; v.large = large_;
+    v.tri_off = tri_off_; v.tri_keys = tri_keys_; v.tri_ksl = tri_filter_ ? tri_ksl_ : nullptr; v.large = large_;
+    v.tri_bin = tri_bin_;
     v.nlarge = large_off_ ? large_off_ + nt_ : nullptr;
     v.np = np_;
     v.part_hash = np_ >= kDynParticleHashMin ? 1 : 0;
@@ -332,6 +398,9 @@ bool DynamicColliders::upload_view(hipStream_t st) {
     v.psrc = part_filter_ ? psrc_ : nullptr;
     v.player = part_filter_ ? player_ : nullptr;
     v.pcount = pcount_; v.poff = poff_; v.pkeys = pkeys_; v.pcell = pcell_;
+    v.pksl = part_filter_ ? pksl_ : nullptr;
+    v.pbin = pbin_;
+    v.pkcell = pkcell_;
     if (v.tri_hash && v.tri_buckets > tri_buckets_) return fail("dynamic triangle hash capacity");
     if (v.part_hash && v.part_buckets > part_buckets_) return fail("dynamic particle hash capacity");
     launch_k(k_set_view, dim3(1), dim3(1), 0, st, d_view_, v);
@@ -473,7 +542,7 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
     return hip_ok(hipGetLastError(), "scene gather");
 }
 
-bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool count_diag) {
+bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool count_diag, bool occupancy) {
     if (!d_view_) return true;
     const int cd = count_diag ? 1 : 0;
     TimedLaunch t("dynamic_hash", st);
@@ -485,18 +554,27 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
         launch_k(k_large_scatter, g, blk, 0, st, nt_, static_cast<const int*>(large_flag_), static_cast<const int*>(large_off_), large_);
         if (nt_ >= kDynTriangleHashMin) {
             const int buckets = next_pow2(std::max(kDynMinBuckets, nt_ * 4));
-            const dim3 gw(grid_for(static_cast<long long>(nt_) * kDynMaxHashCells, 256));
+            const dim3 gw(grid_for(static_cast<long long>(nt_) * kTriLanes, 256));
             launch_k(k_fill_i32, dim3(grid_for(buckets, 256)), blk, 0, st, tri_count_, 0, buckets);
-            launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, occupancy_, 0, 1);
             launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const int4*>(cmin_),
                      static_cast<const int4*>(cmax_), static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_),
-                     tri_slot_, tri_raw_, tri_rawb_, occupancy_, 0);
+                     tri_slot_, tri_raw_, tri_rawb_, 0);
             exclusive_scan(st, tri_count_, buckets, tri_off_, scan_tmp_);
             launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const int4*>(cmin_),
                      static_cast<const int4*>(cmax_), static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_),
-                     tri_slot_, tri_raw_, tri_rawb_, occupancy_, 1);
-            launch_k(k_tri_rank, gw, blk, 0, st, static_cast<const int*>(tri_off_), buckets, static_cast<const int*>(tri_count_),
-                     static_cast<const int*>(tri_raw_), static_cast<const int*>(tri_rawb_), tri_keys_);
+                     tri_slot_, tri_raw_, tri_rawb_, 1);
+            const dim3 gr(std::min<unsigned>(grid_for(static_cast<long long>(nt_) * 16, 256), 16384u));
+            launch_k(k_tri_rank, gr, blk, 0, st, static_cast<const int*>(tri_off_), buckets, static_cast<const int*>(tri_count_),
+                     static_cast<const int*>(tri_raw_), static_cast<const int*>(tri_rawb_), tri_keys_,
+                     static_cast<const int*>(tri_filter_ ? tsrc_ : nullptr), static_cast<const int*>(tlayer_), tri_ksl_);
+            launch_k(k_bucket_bins, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
+                     static_cast<const int*>(tri_off_), static_cast<const int*>(tri_count_), buckets, 0x7fffffff,
+                     static_cast<const int2*>(tri_filter_ ? tri_ksl_ : nullptr), tri_bin_);
+            if (count_diag && occupancy) {
+                launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, occupancy_, 0, 1);
+                launch_k(k_bucket_max, dim3(std::min<unsigned>(grid_for(buckets, 256), 4096u)), blk, 0, st,
+                         static_cast<const int*>(tri_count_), buckets, occupancy_);
+            }
         }
     }
     if (np_ >= kDynParticleHashMin) {
@@ -506,8 +584,12 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
         for (int phase = 0; phase < 3; ++phase) {
             if (phase == 1) exclusive_scan(st, pcount_, buckets, poff_, scan_tmp_);
             launch_k(k_part_hash, g, blk, 0, st, static_cast<const DynView*>(d_view_), pcell_, pcount_, static_cast<const int*>(poff_),
-                     pslot_, praw_, pkeys_, counts, cd, phase);
+                     pslot_, praw_, pkeys_, pksl_, pkcell_, static_cast<const int*>(part_filter_ ? psrc_ : nullptr),
+                     static_cast<const int*>(player_), counts, cd, phase);
         }
+        launch_k(k_bucket_bins, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
+                 static_cast<const int*>(poff_), static_cast<const int*>(pcount_), buckets, kDynParticleSlots,
+                 static_cast<const int2*>(part_filter_ ? pksl_ : nullptr), pbin_);
     }
     return hip_ok(hipGetLastError(), "dynamic collider hash");
 }

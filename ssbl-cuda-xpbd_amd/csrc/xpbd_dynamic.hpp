@@ -48,6 +48,8 @@ struct DynView {
     const int4* cmax;
     const int* tri_off;      // buckets + 1
     const int* tri_keys;     // triangle * 64 + cell index, ascending per bucket
+    const int2* tri_ksl;     // {source id, layer} of each key's triangle (filtered sets)
+    const int4* tri_bin;     // per bucket: {offset, count, min layer, the one source of all entries | -1 mixed | -2 empty}
     const int* large;        // triangles spanning more than 64 cells, ascending
     const int* nlarge;       // their count (device)
     int np;
@@ -59,6 +61,9 @@ struct DynView {
     const int* pcount;       // buckets
     const int* poff;         // buckets + 1
     const int* pkeys;        // first min(count, 16) particles of a bucket, ascending
+    const int2* pksl;        // {source id, layer} beside each key (filtered sets)
+    const int4* pbin;        // per bucket, over its visible entries: {offset, count, min layer, one source | -1 | -2}
+    const int4* pkcell;      // the key's particle cell beside each key
     const int4* pcell;
 };
 
@@ -88,8 +93,9 @@ public:
     // Device gather of every source's current positions (particles) and
     // triangles, ordered by source; asynchronous on `st`.
     bool gather(hipStream_t st, const std::vector<GatherSource>& sources, float margin, float thickness);
-    // Bounds and both hashes; build-time counters go to `counts` when count_diag.
-    bool build(hipStream_t st, unsigned long long* counts, bool count_diag);
+    // Bounds and both hashes; build-time counters go to `counts` when
+    // count_diag, and the largest triangle bucket is measured when occupancy.
+    bool build(hipStream_t st, unsigned long long* counts, bool count_diag, bool occupancy);
 
     const DynView* device_view() const { return d_view_; }
     int triangles() const { return nt_; }
@@ -121,6 +127,8 @@ private:
     int4 *cmin_ = nullptr, *cmax_ = nullptr;
     int *large_flag_ = nullptr, *large_off_ = nullptr, *large_ = nullptr;
     int *tri_count_ = nullptr, *tri_off_ = nullptr, *tri_slot_ = nullptr, *tri_raw_ = nullptr, *tri_rawb_ = nullptr, *tri_keys_ = nullptr;
+    int2* tri_ksl_ = nullptr;
+    int4* tri_bin_ = nullptr;
     int* occupancy_ = nullptr;
     int* extent_bits_ = nullptr;
     // particles
@@ -129,6 +137,9 @@ private:
     bool part_filter_ = false;
     int4* pcell_ = nullptr;
     int *pslot_ = nullptr, *pcount_ = nullptr, *poff_ = nullptr, *praw_ = nullptr, *pkeys_ = nullptr;
+    int2* pksl_ = nullptr;
+    int4* pbin_ = nullptr;
+    int4* pkcell_ = nullptr;
     int* scan_tmp_ = nullptr;
     GatherSource* d_sources_ = nullptr;
     int sources_cap_ = 0;

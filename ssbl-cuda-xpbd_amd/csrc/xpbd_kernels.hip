@@ -249,6 +249,15 @@ __device__ inline bool dyn_enabled(const ClothView& c, int src, int layer) {   /
     return true;
 }
 
+// True when the filter rejects every entry a bucket bin covers
+// ({offset, count, min layer, the one source of all entries | -1 | -2 empty}).
+__device__ __forceinline__ bool dyn_bucket_rejected(const ClothView& c, int4 bin) {
+    if (bin.w == -2) return true;
+    if (c.dyn_mode == 0) return false;
+    if (c.dyn_target_src != 0 && bin.w == c.dyn_target_src) return true;
+    return c.dyn_mode == 2 && bin.z >= c.dyn_target_layer;
+}
+
 __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, float dt, bool count) {
     const DynView& D = *c.dyn;
     const float cm = fmaxf(c.collision_margin, 0.0f);
@@ -258,9 +267,10 @@ __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, floa
         F3 best_delta = f3(0.0f, 0.0f, 0.0f);
         float best_pen = 0.0f;
         unsigned cand = 0, rej = 0;
-        auto consider = [&](int t) {
+        // the source filter has already passed when `filtered` (hash entries carry {source, layer})
+        auto consider = [&](int t, bool filtered) {
             if (t < 0 || t >= nt) return;
-            if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) return;
+            if (!filtered && D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) return;
             ++cand;
             const float4 mn = D.lo[t], mx = D.hi[t];
             const F3 q0 = f3(fminf(fmaxf(p.x, mn.x), mx.x), fminf(fmaxf(p.y, mn.y), mx.y), fminf(fmaxf(p.z, mn.z), mx.z));
@@ -285,18 +295,24 @@ __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, floa
         };
         if (D.tri_hash) {   // own cell only: the unsigned query radius is 0 (A41:3623-3633)
             const int cx = cell_coord(p.x, D.tri_cell), cy = cell_coord(p.y, D.tri_cell), cz = cell_coord(p.z, D.tri_cell);
-            const int b = static_cast<int>(hash_cell(cx, cy, cz, D.tri_buckets));
-            const int e = D.tri_off[b + 1];
-            for (int k = D.tri_off[b]; k < e; ++k) {
+            const int4 bin = D.tri_bin[hash_cell(cx, cy, cz, D.tri_buckets)];
+            const int e = dyn_bucket_rejected(c, bin) ? 0 : bin.x + bin.y;
+            for (int k = bin.x; k < e; ++k) {
+                // the source filter and the canonical-cell test (A41:3934-3982) are both
+                // side-effect free, so the cheaper one (beside the key) runs first
+                if (D.tri_ksl) {
+                    const int2 sl = D.tri_ksl[k];
+                    if (!dyn_enabled(c, sl.x, sl.y)) continue;
+                }
                 const int t = D.tri_keys[k] / kDynMaxHashCells;
                 const int4 lo = D.cmin[t], hi = D.cmax[t];
-                if (cx < lo.x || cx > hi.x || cy < lo.y || cy > hi.y || cz < lo.z || cz > hi.z) continue;   // A41:3934-3982
-                consider(t);
+                if (cx < lo.x || cx > hi.x || cy < lo.y || cy > hi.y || cz < lo.z || cz > hi.z) continue;
+                consider(t, true);
             }
             const int nl = *D.nlarge;
-            for (int k = 0; k < nl; ++k) consider(D.large[k]);
+            for (int k = 0; k < nl; ++k) consider(D.large[k], false);
         } else {
-            for (int t = 0; t < nt; ++t) consider(t);
+            for (int t = 0; t < nt; ++t) consider(t, false);
         }
         if (count && cand) atomicAdd(&c.counts[kCountDynTriCandidates], static_cast<unsigned long long>(cand));
         if (count && rej) atomicAdd(&c.counts[kCountDynTriAabbRejects], static_cast<unsigned long long>(rej));
@@ -325,9 +341,9 @@ __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, floa
         F3 best_delta = f3(0.0f, 0.0f, 0.0f);
         float best_pen = 0.0f;
         unsigned cand = 0;
-        auto consider = [&](int j) {
+        auto consider = [&](int j, bool filtered) {
             if (j < 0 || j >= np_count) return;
-            if (D.psrc && !dyn_enabled(c, D.psrc[j], D.player ? D.player[j] : 0)) return;
+            if (!filtered && D.psrc && !dyn_enabled(c, D.psrc[j], D.player ? D.player[j] : 0)) return;
             const float4 pj = D.pp[j];
             const float radius = fmaxf(pj.w, 0.0f) + margin;
             if (radius <= 0.0f) return;
@@ -342,21 +358,30 @@ __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, floa
         };
         if (D.part_hash) {
             const int cx = cell_coord(p.x, D.part_cell), cy = cell_coord(p.y, D.part_cell), cz = cell_coord(p.z, D.part_cell);
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
-                        const int b = static_cast<int>(hash_cell(qx, qy, qz, D.part_buckets));
-                        const int lim = min(D.pcount[b], kDynParticleSlots), base = D.poff[b];
-                        for (int k = 0; k < lim; ++k) {
-                            const int j = D.pkeys[base + k];
-                            const int4 pc = D.pcell[j];
-                            if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
-                            consider(j);
+            // one bin per bucket; the bins of a z plane are loaded together, then
+            // visited in the reference's order (z, then y, then x innermost)
+            for (int dz = -1; dz <= 1; ++dz) {
+                int4 bins[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) bins[k] = D.pbin[hash_cell(cx + k % 3 - 1, cy + k / 3 - 1, cz + dz, D.part_buckets)];
+#pragma unroll
+                for (int k9 = 0; k9 < 9; ++k9) {
+                    if (dyn_bucket_rejected(c, bins[k9])) continue;
+                    const int qx = cx + k9 % 3 - 1, qy = cy + k9 / 3 - 1, qz = cz + dz;
+                    const int base = bins[k9].x, lim = min(bins[k9].y, kDynParticleSlots);
+                    for (int k = 0; k < lim; ++k) {
+                        const int4 pc = D.pkcell[base + k];
+                        if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
+                        if (D.pksl) {
+                            const int2 sl = D.pksl[base + k];
+                            if (!dyn_enabled(c, sl.x, sl.y)) continue;
                         }
+                        consider(D.pkeys[base + k], true);
                     }
+                }
+            }
         } else {
-            for (int j = 0; j < np_count; ++j) consider(j);
+            for (int j = 0; j < np_count; ++j) consider(j, false);
         }
         if (count && cand) atomicAdd(&c.counts[kCountDynParticleCandidates], static_cast<unsigned long long>(cand));
         if (best_pen > 0.0f) {
@@ -641,6 +666,25 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
     c.vel[v] = with_w(s.vel_, 0.0f);
 }
 
+// Dynamic collider contacts of the last iteration (A41:9230-9252) as a pass
+// of their own: the hash probes want registers the constraint passes do not.
+__global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
+                                                        int ntiles, int parity, unsigned ops) {
+    const Tile tile = tiles[xcd_tile(ntiles)];
+    const int v = tile.begin + static_cast<int>(threadIdx.x);
+    if (v >= tile.end) return;
+    const ClothView& c = views[tile.cloth];
+    if (!c.dyn) return;
+    float4* pos = parity ? c.pos1 : c.pos0;
+    const float4 p4 = pos[v];
+    if (p4.w == 0.0f) return;
+    VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
+    dynamic_contacts(c, s, c.dt_sub, (ops & kOpCount) != 0u);
+    if (s.p.x != p4.x || s.p.y != p4.y || s.p.z != p4.z) pos[v] = with_w(s.p, p4.w);
+    if (s.q_dirty) c.prev[v] = with_w(s.q_, 0.0f);
+    if (s.v_dirty) c.vel[v] = with_w(s.vel_, 0.0f);
+}
+
 // ---------------------------------------------------------- vertex pass
 // Neighbour positions loaded together before any of their edges is
 // evaluated: one round of memory latency per chunk instead of one per edge.
@@ -822,7 +866,7 @@ enum GatherKind { kGatherNone = 0, kGatherStretch = 1, kGatherLra = 2, kGatherBe
 
 // One solver iteration for every vertex of the tile: the constraint gather
 // selected at compile time, then the vertex-local steps selected by `ops`.
-template <int kGather, bool kDynamic>
+template <int kGather>
 __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
                                                        int ntiles, int src_parity, int dst_parity, unsigned ops) {
     const Tile tile = tiles[xcd_tile(ntiles)];
@@ -849,9 +893,6 @@ __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __res
     if (ops & kOpPin) apply_pins(c, s, pinned, dt);
     if ((ops & kOpAnalytic) && !pinned) analytic_contacts(c, s, dt, count);
     if ((ops & kOpSdf) && c.sdf_ready && !pinned) sdf_contact(c, s, dt, count);
-    if constexpr (kDynamic) {
-        if ((ops & kOpDynamic) && c.dyn && !pinned) dynamic_contacts(c, s, dt, count);   // A41:9230-9252
-    }
     // A41:9360-9369: the second pin projection runs after any contact work,
     // and on every last iteration (dynamic collision is scheduled there).
     if ((ops & kOpPinPolish) && ((ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt);
@@ -991,19 +1032,19 @@ void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsi
                        : (ops & kOpBend) ? "vertex_pass_bend" : "vertex_pass_local";
     TimedLaunch t(name, c.stream);
     const dim3 grid(c.ntiles), block(kTile);
-    const bool dyn = (ops & kOpDynamic) != 0u;
-    if ((ops & kOpStretch) && dyn)
-        launch_k(k_vertex_pass<kGatherStretch, true>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
-    else if (ops & kOpStretch)
-        launch_k(k_vertex_pass<kGatherStretch, false>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+    if (ops & kOpStretch)
+        launch_k(k_vertex_pass<kGatherStretch>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
     else if (ops & kOpLra)
-        launch_k(k_vertex_pass<kGatherLra, false>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+        launch_k(k_vertex_pass<kGatherLra>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
     else if (ops & kOpBend)
-        launch_k(k_vertex_pass<kGatherBend, false>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
-    else if (dyn)
-        launch_k(k_vertex_pass<kGatherNone, true>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+        launch_k(k_vertex_pass<kGatherBend>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
     else
-        launch_k(k_vertex_pass<kGatherNone, false>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+        launch_k(k_vertex_pass<kGatherNone>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
+}
+
+void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops) {
+    TimedLaunch t("dynamic_contacts", c.stream);
+    launch_k(k_dynamic_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
 }
 
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops) {

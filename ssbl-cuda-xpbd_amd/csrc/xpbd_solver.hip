@@ -840,7 +840,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             const bool last = it == iterations - 1;
             const bool final_c = last && sub == substeps - 1;
             const unsigned last_op = last ? kOpLastIter : 0u;
-            const unsigned dyn_op = (last && plan.any_dyn) ? kOpDynamic : 0u;   // A41:9230-9252
+            const bool dyn_now = last && plan.any_dyn;   // A41:9230-9252
             bool self_now = false;
             if (plan.any_self && last) {
                 for (Solver* m : members) {
@@ -849,10 +849,10 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
                 }
             }
             const bool run_pcg = plan.any_pcg && last;   // substep cadence 1 (A41:284-292)
-            const bool split = run_pcg || self_now || (final_c && (plan.any_lra || plan.any_bend));
+            const bool split = run_pcg || self_now || dyn_now || (final_c && (plan.any_lra || plan.any_bend));
             if (!split) {
                 launch_vertex_pass(ctx, parity, parity ^ 1,
-                                   stretch | pin_ops | kOpAnalytic | kOpSdf | dyn_op | last_op | (last ? kOpVelocity : 0u) | cnt);
+                                   stretch | pin_ops | kOpAnalytic | kOpSdf | last_op | (last ? kOpVelocity : 0u) | cnt);
                 parity ^= 1;
                 continue;
             }
@@ -879,10 +879,10 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
                     parity ^= 1;
                 }
             }
-            const bool tail = self_now || (run_pcg && final_c);
+            const bool tail = self_now || dyn_now || (run_pcg && final_c);
             launch_vertex_pass(ctx, parity, parity,
-                               (tail ? (pin_ops & ~kOpPinPolish) : (pin_ops | last_op | kOpVelocity)) | kOpAnalytic | kOpSdf | dyn_op |
-                                   cnt);
+                               (tail ? (pin_ops & ~kOpPinPolish) : (pin_ops | last_op | kOpVelocity)) | kOpAnalytic | kOpSdf | cnt);
+            if (dyn_now) launch_dynamic_pass(ctx, parity, cnt);
             if (self_now) {
                 if (!self_batch || !self_batch->run(ctx, parity, sub, substeps, count)) return false;
             }
@@ -976,7 +976,7 @@ static bool prepare_member(Solver& s, int substeps, int iterations, bool count) 
         launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.max_smoothed.p), 0u, 1);
     }
     // the cloth's own dynamic colliders are re-hashed every step (A41:9129-9136)
-    if (!s.use_scene() && s.dyn_on() && !s.dyn.build(lib_stream(), s.counts.p, count)) return false;
+    if (!s.use_scene() && s.dyn_on() && !s.dyn.build(lib_stream(), s.counts.p, count, count)) return false;
     return true;
 }
 
@@ -1318,7 +1318,7 @@ SSBL_API int ssbl_update_global_dynamic_scene(void* scene, const SsblXpbdGlobalD
                                np, sc->margin, sc->thickness) ||
         !sc->set.set_triangles(st, in->triangle_vertices, nt, in->triangle_source_ids, in->triangle_collision_layers, sc->margin,
                                sc->thickness, true) ||
-        !sc->set.build(st, sc->counts.p, true))
+        !sc->set.build(st, sc->counts.p, true, false))
         return 0;
     sc->overflow_pending = true;
     if (!sync_stream()) return 0;
@@ -1451,7 +1451,7 @@ SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const*
     hipStream_t st = lib_stream();
     if (!sc->set.gather(st, src, sc->margin, sc->thickness)) return 0;
     launch_fill_u32(st, reinterpret_cast<unsigned*>(sc->counts.p), 0u, 2 * static_cast<size_t>(kCountSlots));
-    if (!sc->set.build(st, sc->counts.p, true)) return 0;
+    if (!sc->set.build(st, sc->counts.p, true, false)) return 0;
     sc->overflow_pending = true;
     return 1;
 }

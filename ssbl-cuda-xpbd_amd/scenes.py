@@ -272,6 +272,8 @@ class Scene:
     substeps: int
     iterations: int
     frames: int
+    layers: list | None = None          # collision layer per cloth when cross-cloth collision is on
+    contact_distance: float = 0.0       # particle radius and scene margin of the cross-cloth scene
 
     @property
     def vertex_count(self) -> int:
@@ -313,20 +315,28 @@ def config2(seed: int = 0, n: int = 100, frames: int = 240) -> Scene:
     return Scene("selfcol_100x100", [cloth], [opts], box, 20, 5, frames)
 
 
-def config3(seed: int = 0, count: int = 64, frames: int = 120, self_collision: bool = False) -> Scene:
-    """64 uneven grid cloths (sides log-uniform in [16, 200]) over a shared ground and box."""
+def config3(seed: int = 0, count: int = 64, frames: int = 120, self_collision: bool = False,
+            cross_cloth: bool = True) -> Scene:
+    """64 uneven grid cloths (sides log-uniform in [16, 200]) over a shared ground and box.
+
+    The cloths stand in stacks of four (collision layers 0-3, 10 cm apart in
+    height) on a 4 m grid, so with cross-cloth collision on (lower layers
+    only) each cloth falls onto the ones below it.
+    """
     rng = np.random.Generator(np.random.PCG64(seed))
     sides = np.rint(np.exp(rng.uniform(math.log(16), math.log(200), size=count))).astype(int)
-    cloths, opts = [], []
+    cloths, opts, layers = [], [], []
     spacing = 0.02
-    cols = int(math.ceil(math.sqrt(count)))
+    stacks = (count + 3) // 4
+    cols = int(math.ceil(math.sqrt(stacks)))
     pitch = 200 * spacing + 0.5
     for k, side in enumerate(sides.tolist()):
         size = (side - 1) * spacing
-        ox = (k % cols - cols / 2) * pitch
-        oz = (k // cols - cols / 2) * pitch
+        stack, layer = k // 4, k % 4
+        ox = (stack % cols - cols / 2) * pitch
+        oz = (stack // cols - cols / 2) * pitch
         pos, tris = grid_mesh(side, size)
-        pos[:, 1] += 1.0 + 0.1 * (k % 4)
+        pos[:, 1] += 1.0 + 0.1 * layer
         pos[:, 0] += ox
         pos[:, 2] += oz
         pin = np.zeros(side * side, bool)
@@ -346,5 +356,7 @@ def config3(seed: int = 0, count: int = 64, frames: int = 120, self_collision: b
         cloths.append(build_cloth(y_up_to_z_up(pos), tris, pin, jitter=1.0e-4, seed=seed * 1000 + k))
         opts.append(SolverOptions(use_ground=True, contact_friction=0.3, collision_margin=0.005, cloth_thickness=0.005,
                                   self_collision=self_collision, static_sdf_max_resolution=128))
+        layers.append(layer)
     box = box_triangles((0.0, 0.0, 0.3), 0.3)
-    return Scene("batched_64_uneven", cloths, opts, box, 10, 8, frames)
+    return Scene("batched_64_uneven", cloths, opts, box, 10, 8, frames, layers if cross_cloth else None,
+                 0.005 if cross_cloth else 0.0)
