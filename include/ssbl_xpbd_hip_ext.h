@@ -41,6 +41,13 @@ SSBL_API int ssbl_ext_self_neighbors(void* handle, unsigned int* counts, unsigne
  * (values may be NULL to query the shape).  Returns 0 when no SDF is built. */
 SSBL_API int ssbl_ext_static_sdf(void* handle, int* dims, float* origin_and_voxel, float* values, int max_values);
 
+/* Static SDF given as a grid instead of built from static triangles (the
+ * reference builds it from triangles only, ssbl_xpbd_cuda_abi41.cu:6123-6218):
+ * dims int[3] (each >= 2), origin_and_voxel float[4], values float[dx*dy*dz]
+ * with x fastest, negative inside.  Sampling and contacts are those of a built
+ * grid; static triangles uploaded before are kept for the normal fallback. */
+SSBL_API int ssbl_ext_set_static_sdf(void* handle, const int* dims, const float* origin_and_voxel, const float* values, int count);
+
 /* Batched stepping: every handle advances by the same substeps/iterations; the
  * result of each handle equals what ssbl_step_solver_ex on that handle alone
  * produces.  The batch refers to the handles; it does not own them. */

@@ -617,6 +617,7 @@ template <class R> struct Solver {
     // ------------------------------------------------------------- create
     bool create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh);
     bool build_sdf(const float* triangles, int count);
+    bool set_sdf_grid(const int* dims, const float* ov, const float* values, int count);
     void pressure_setup(const std::vector<float>& rest_f);
 
     // -------------------------------------------------------------- step
@@ -951,6 +952,26 @@ template <class R> bool ray_hits(V3<R> p, V3<R> a, V3<R> b, V3<R> c) {
 }
 
 // A41:6123-6218 grid set-up, A41:1787-1829 brute-force voxel distances.
+// A static SDF given as a grid (a counterpart of build_sdf's output for
+// callers that hold one; the static triangles stay for the normal fallback).
+template <class R> bool Solver<R>::set_sdf_grid(const int* dims, const float* ov, const float* values, int count) {
+    if (!dims || !ov || !values) return fail("static SDF grid, origin and values are required");
+    const long long voxels = 1LL * dims[0] * dims[1] * dims[2];
+    if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2 || voxels > std::numeric_limits<int>::max() || count != voxels)
+        return fail("static SDF grid needs at least 2 voxels per axis and dx*dy*dz values");
+    if (!std::isfinite(ov[0]) || !std::isfinite(ov[1]) || !std::isfinite(ov[2]) || !std::isfinite(ov[3]) || ov[3] <= 0.0f)
+        return fail("static SDF origin and voxel size must be finite, the voxel size positive");
+    sdf.assign(values, values + voxels);
+    for (int k = 0; k < 3; ++k) sdf_dim[k] = dims[k];
+    sdf_origin = mk<R>(ov[0], ov[1], ov[2]);
+    sdf_voxel = R(ov[3]);
+    sdf_signed = 1;
+    sdf_ready = 1;
+    sdf_rebuilds++;
+    populate_sdf_diag();
+    return true;
+}
+
 template <class R> bool Solver<R>::build_sdf(const float* triangles, int count) {
     sdf_ready = 0;
     sdf_dim[0] = sdf_dim[1] = sdf_dim[2] = 0;
@@ -2768,6 +2789,11 @@ SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const*
     in.collision_margin = margin;
     in.cloth_thickness = thickness;
     return ssbl_update_global_dynamic_scene(scene, &in);
+}
+SSBL_API int ssbl_ext_set_static_sdf(void* handle, const int* dims, const float* ov, const float* values, int count) {
+    g_error.clear();
+    if (!handle) return fail("invalid solver handle") ? 1 : 0;
+    return with(handle, [&](auto& s) { return s.set_sdf_grid(dims, ov, values, count) ? 1 : 0; });
 }
 SSBL_API long long ssbl_ext_device_bytes(void*) { fail("oracle: no device memory"); return -1; }
 SSBL_API int ssbl_ext_time_batch_pass(void*, int, float*) { return fail("oracle: no device timing") ? 1 : 0; }

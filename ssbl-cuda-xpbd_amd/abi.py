@@ -123,6 +123,7 @@ EXT_SIGNATURES = {
     "ssbl_ext_self_hash_cells": (_i, [_vp, _ip, _i]),
     "ssbl_ext_self_neighbors": (_i, [_vp, _up, _up, _i]),
     "ssbl_ext_static_sdf": (_i, [_vp, _ip, _fp, _fp, _i]),
+    "ssbl_ext_set_static_sdf": (_i, [_vp, _ip, _fp, _fp, _i]),
     "ssbl_ext_create_batch": (_vp, [ctypes.POINTER(_vp), _i]),
     "ssbl_ext_destroy_batch": (_i, [_vp]),
     "ssbl_ext_step_batch": (_i, [_vp, _i, _i, _i]),

@@ -258,6 +258,7 @@ struct Solver {
 
     bool create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh);
     bool build_sdf(const float* triangles, int count);
+    bool set_sdf_grid(const int* dims, const float* origin_and_voxel, const float* values, int count);
     bool upload_pins(const int* idx, const float* p, const float* w, int count);
     bool refresh_pins(int iterations);
     bool upload_fields(const SsblXpbdForceField* f, int count, int unsupported);
@@ -688,6 +689,31 @@ bool Solver::build_sdf(const float* triangles, int count) {
     sdf_voxel = voxel;
     sdf_signed = sgn;
     nstatic = count;
+    sdf_ready = 1;
+    sdf_rebuilds++;
+    sdf_build_ms = static_cast<float>(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    populate_sdf_diag();
+    return true;
+}
+
+// A static SDF given as a grid (the reference builds its grid from static
+// triangles only, A41:6123-6218).  Sampling, normals and contacts are those
+// of a built grid; static triangles uploaded before stay for the normal
+// fallback of A41:1749-1769.
+bool Solver::set_sdf_grid(const int* dims, const float* ov, const float* values, int count) {
+    if (!dims || !ov || !values) return fail("static SDF grid, origin and values are required");
+    const long long voxels = 1LL * dims[0] * dims[1] * dims[2];
+    if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2 || voxels > std::numeric_limits<int>::max() || count != voxels)
+        return fail("static SDF grid needs at least 2 voxels per axis and dx*dy*dz values");
+    if (!std::isfinite(ov[0]) || !std::isfinite(ov[1]) || !std::isfinite(ov[2]) || !std::isfinite(ov[3]) || ov[3] <= 0.0f)
+        return fail("static SDF origin and voxel size must be finite, the voxel size positive");
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!sync_stream()) return false;
+    std::vector<float> h(values, values + voxels);
+    if (!sdf.upload(h, "static SDF upload")) return false;
+    for (int k = 0; k < 3; ++k) { sdf_dim[k] = dims[k]; sdf_origin[k] = ov[k]; }
+    sdf_voxel = ov[3];
+    sdf_signed = 1;
     sdf_ready = 1;
     sdf_rebuilds++;
     sdf_build_ms = static_cast<float>(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
@@ -1454,6 +1480,14 @@ SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const*
     if (!sc->set.build(st, sc->counts.p, true, false)) return 0;
     sc->overflow_pending = true;
     return 1;
+}
+
+SSBL_API int ssbl_ext_set_static_sdf(void* handle, const int* dims, const float* origin_and_voxel, const float* values, int count) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail("invalid solver handle") ? 1 : 0;
+    ByteScope bytes(s);
+    return s->set_sdf_grid(dims, origin_and_voxel, values, count) ? 1 : 0;
 }
 
 SSBL_API long long ssbl_ext_device_bytes(void* handle) {

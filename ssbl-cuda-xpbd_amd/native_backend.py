@@ -215,6 +215,14 @@ class NativeXpbdSolver:
         fi.force_field_count = len(fields)
         self._check(self._lib.ssbl_update_frame_inputs(self._handle, ctypes.byref(fi)))
 
+    def set_static_sdf(self, values, origin, voxel: float) -> None:
+        """A static SDF grid (values indexed [z, y, x], negative inside) in place of one built from triangles."""
+        v = np.ascontiguousarray(values, np.float32)
+        dims = np.asarray([v.shape[2], v.shape[1], v.shape[0]], np.int32)
+        ov = np.asarray([origin[0], origin[1], origin[2], voxel], np.float32)
+        flat = v.reshape(-1)
+        self._check(self._lib.ssbl_ext_set_static_sdf(self._handle, _iptr(dims), _fptr(ov), _fptr(flat), flat.size))
+
     def update_dynamic_triangles(self, triangles) -> None:
         """Triangle soup (T x 3 x 3) this cloth collides with (native_backend.py ssbl_update_dynamic_triangles)."""
         t = _f32(triangles, (-1, 9))

@@ -130,3 +130,29 @@ def _triangles_of(solvers, pos):
         t = s.triangles
         out.append((p[t].reshape(-1, 9), t))
     return out
+
+
+def box_sdf_grid(n=128, extent=2.0, center=(0.0, 0.0, 0.3), half=0.3):
+    """An n^3 FP32 signed distance grid of an axis-aligned box over a cube of
+    side `extent` centred on the box (values [z, y, x], negative inside)."""
+    s = np.linspace(-extent / 2, extent / 2, n)
+    voxel = float(s[1] - s[0])
+    origin = (center[0] - extent / 2, center[1] - extent / 2, center[2] - extent / 2)
+    z, y, x = np.meshgrid(s + center[2], s + center[1], s + center[0], indexing="ij")
+    q = np.stack([np.abs(x - center[0]), np.abs(y - center[1]), np.abs(z - center[2])], -1) - half
+    outside = np.linalg.norm(np.maximum(q, 0.0), axis=-1)
+    inside = np.minimum(np.max(q, axis=-1), 0.0)
+    return (outside + inside).astype(np.float32), origin, voxel
+
+
+def cloth_on_sdf_grid(nb, lib, scenes, grid, frames=20):
+    values, origin, voxel = grid
+    cloth = scenes.grid_cloth(20, 0.8, 0.9, pin_corners=False, seed=6)
+    opts = scenes.SolverOptions(collision_margin=0.01, cloth_thickness=0.005, use_ground=True)
+    s = nb.NativeXpbdSolver(cloth, opts, None, lib=lib)
+    s.set_static_sdf(values, origin, voxel)
+    out = []
+    for _ in range(frames):
+        s.step(8, 4, diagnostics=True)
+        out.append(s.download_positions())
+    return out, s.diagnostics()

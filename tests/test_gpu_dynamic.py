@@ -71,3 +71,14 @@ def test_two_cloth_scene_gpu(nb, hip, oracle, scenes):
             assert np.array_equal(x[k], y[k]), ("host-packed scene vs oracle", f, k)
             assert np.array_equal(y[k], z[k]), ("device gather vs host packing", f, k)
             assert np.array_equal(z[k], w[k]), ("batched vs single", f, k)
+
+
+def test_given_128_box_grid_gpu(nb, hip, oracle, scenes):
+    """The 128^3 FP32 box SDF over a 2 m cube handed over as a grid: HIP equals the oracle."""
+    grid = sd.box_sdf_grid(n=128)
+    g, dg = sd.cloth_on_sdf_grid(nb, hip, scenes, grid)
+    r, dr = sd.cloth_on_sdf_grid(nb, oracle, scenes, grid)
+    worst = max(float(np.max(np.abs(a - b))) for a, b in zip(g, r))
+    print(f"cloth on a given 128^3 box SDF: max |hip - oracle| = {worst:.3e}")
+    assert worst == 0.0
+    assert dg.static_sdf_contact_count == dr.static_sdf_contact_count > 0
