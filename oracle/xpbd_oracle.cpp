@@ -72,8 +72,24 @@ enum CountSlot {
     kCountSoft = 0, kCountExactImpulse = 1, kCountEdgeEdge = 2, kCountHardFallbacks = 3,
     kCountSelfCandidates = 9, kCountSelfOverflow = 10, kCountStaticSdf = 11, kCountLraTacks = 12,
     kCountBendingWings = 13, kCountTackGuards = 14, kCountBendingGuards = 15,
+    // fast self-collision mode, A41:56-72
+    kCountIslandCandidates = 16, kCountIslandClusters = 17, kCountIslandVertexRefs = 18, kCountIslandApplied = 19,
+    kCountIslandGuarded = 20, kCountIslandMaxDeltaMicro = 21, kCountFilterSeen = 22, kCountFilterAcceptedVv = 23,
+    kCountFilterAcceptedVt = 24, kCountFilterAcceptedEe = 25, kCountFilterRejectedRest = 26, kCountFilterRejectedDuplicate = 27,
+    kCountFilterRejectedOwnership = 28, kCountFilterCacheHits = 29, kCountFilterCacheMisses = 30, kCountClusterCount = 31,
+    kCountClusterOwnedContacts = 32,
     kCountFriction = 33, kCountSlots = 38
 };
+// fast self-collision filters, A41:142-149
+constexpr int kFilterLocalSlots = 64;
+constexpr int kClusterMaxLocal = 16;
+constexpr int kClusterMaxContactsPerOwner = 1;
+constexpr int kIslandMinContacts = 3;
+constexpr int kIslandMaxContacts = 48;
+constexpr float kIslandCorrectionScale = 0.50f;
+constexpr float kIslandMaxDeltaScale = 0.30f;
+constexpr float kIslandMinAvgDepthScale = 0.04f;
+enum PairType { kPairVv = 0, kPairVt = 1, kPairEe = 2 };
 
 template <class R> struct V3 { R x, y, z; };
 template <class R> inline V3<R> mk(R x, R y, R z) { return V3<R>{x, y, z}; }
@@ -633,6 +649,8 @@ template <class R> struct Solver {
     void static_sdf_contacts(R dt);
     void velocity_update(R dt);
     void self_collision();
+    template <class H> void island_aggregate(const H& TH, R cs);
+    void self_accumulate(int v, V3<R> d);
     void dynamic_contacts(R dt);
     bool contact_response(Vec n, R proxy, Vec* vel_io);
     // helpers
@@ -678,8 +696,6 @@ template <class R> bool Solver<R>::create(const SsblXpbdConfig* config, const Ss
     cfg.stretch_optimization_strength = std::min(std::max(cfg.stretch_optimization_strength, 0.0f), 1.0f);
     cfg.stretch_optimization_enabled =
         (cfg.edge_count > 0 && cfg.stretch_optimization_enabled && cfg.stretch_optimization_strength > 0.0f) ? 1 : 0;
-    if (cfg.self_collision && cfg.self_collision_mode == kSelfModeFast)
-        return fail("oracle: fast self-collision mode is not restated yet");
     colliders.use_ground = cfg.use_ground;
     colliders.ground_height = cfg.ground_height;
     colliders.use_wall = cfg.use_wall;
@@ -1483,8 +1499,145 @@ struct HashTable {
     void reset(int b) { buckets = b; lists.assign(b, {}); }
 };
 
+// Fast-mode filters, A41:708-849: per-thread seen-id list and contact clusters.
+struct SelfFilterState {
+    int seen[kFilterLocalSlots];
+    int nseen = 0;
+    unsigned keys[kClusterMaxLocal];
+    unsigned char owned[kClusterMaxLocal];
+    int nclusters = 0;
+};
+inline int filter_normal_bucket(float v) { return v > 0.45f ? 2 : (v < -0.45f ? 0 : 1); }
+// A41:742-760
+template <class R> inline unsigned filter_cluster_key(int type, V3<R> point, V3<R> normal, R cell) {
+    const R cs = std::max(cell, R(1.0e-3));
+    unsigned key = hash_cell(cell_coord(point.x, cs), cell_coord(point.y, cs), cell_coord(point.z, cs), 1 << 20);
+    const unsigned nk = static_cast<unsigned>(filter_normal_bucket(static_cast<float>(normal.x))) |
+                        (static_cast<unsigned>(filter_normal_bucket(static_cast<float>(normal.y))) << 2) |
+                        (static_cast<unsigned>(filter_normal_bucket(static_cast<float>(normal.z))) << 4);
+    key ^= nk * 2654435761u;
+    key ^= static_cast<unsigned>(type + 1) * 2246822519u;
+    return key;
+}
+
+// A41:1574-1587; the sum is kept in 2^-32 fixed point so that its value does
+// not depend on the order of the contributions.
+template <class R> void Solver<R>::self_accumulate(int v, V3<R> d) {
+    if (v < 0 || v >= V() || inv_mass[v] <= 0.0f || pinned[v]) return;
+    const double s = 4294967296.0;
+    acc_q[3 * v] += static_cast<int64_t>(std::llrint(double(d.x) * s));
+    acc_q[3 * v + 1] += static_cast<int64_t>(std::llrint(double(d.y) * s));
+    acc_q[3 * v + 2] += static_cast<int64_t>(std::llrint(double(d.z) * s));
+    acc_w[v] += 1;
+}
+
+// A41:5226-5439: a vertex inside several nearby triangles (three or more
+// penetrating contacts within the 27 triangle-hash cells around it) is pushed
+// out along their penetration-weighted mean normal.
+template <class R> template <class H> void Solver<R>::island_aggregate(const H& TH, R cs) {
+    const int n = V();
+    const R r = self_radius();
+    const R onset = std::max(r * R(1.8), r + R(kEpsF));
+    const R rest_skip = std::max(onset * R(1.25), r * R(2.5));
+    if (r <= R(0)) return;
+    const int max_nb = std::max(cfg.max_self_collision_neighbors, 1);
+    const int max_contacts = std::max(std::min(max_nb * 2, kIslandMaxContacts), kIslandMinContacts);
+    const int scan_limit = max_contacts * 10;
+    for (int v = 0; v < n; ++v) {
+        if (inv_mass[v] <= 0.0f || pinned[v]) continue;
+        const V3<R> p = pos[v], pp = prev[v];
+        if (!finite(p) || !finite(pp)) { count(kCountIslandGuarded); continue; }
+        const int cx = cell_coord(p.x, cs), cy = cell_coord(p.y, cs), cz = cell_coord(p.z, cs);
+        int accepted[kIslandMaxContacts];
+        int scanned = 0, contacts = 0;
+        R pen_sum = 0, sample_sum = 0, max_pen = 0;
+        V3<R> axis = mk<R>(0, 0, 0), best = mk<R>(0, 0, 1);
+        for (int dz = -1; dz <= 1 && contacts < max_contacts && scanned < scan_limit; ++dz)
+            for (int dy = -1; dy <= 1 && contacts < max_contacts && scanned < scan_limit; ++dy)
+                for (int dx = -1; dx <= 1 && contacts < max_contacts && scanned < scan_limit; ++dx) {
+                    const auto& L = TH.lists[hash_cell(cx + dx, cy + dy, cz + dz, TH.buckets)];
+                    const int lim = std::min(static_cast<int>(L.size()), kBucketSlots);
+                    for (int k = 0; k < lim && contacts < max_contacts && scanned < scan_limit; ++k) {
+                        ++scanned;
+                        const int t = L[k];
+                        bool dup = false;
+                        for (int i = 0; i < contacts; ++i) if (accepted[i] == t) { dup = true; break; }
+                        if (dup) continue;
+                        const int ia = tris[3 * t], ib = tris[3 * t + 1], ic = tris[3 * t + 2];
+                        if (ia >= n || ib >= n || ic >= n || ia == ib || ib == ic || ia == ic) continue;
+                        if (v == ia || v == ib || v == ic) continue;
+                        {
+                            const V3<R> rd = rest[v] - closest_point_on_triangle(rest[v], rest[ia], rest[ib], rest[ic]);
+                            if (dot(rd, rd) < rest_skip * rest_skip) continue;
+                        }
+                        const V3<R> a = pos[ia], b = pos[ib], c = pos[ic];
+                        const V3<R> ap = prev[ia], bp = prev[ib], cp = prev[ic];
+                        if (!finite(a) || !finite(b) || !finite(c) || !finite(ap) || !finite(bp) || !finite(cp)) {
+                            count(kCountIslandGuarded);
+                            continue;
+                        }
+                        const V3<R> q = closest_point_on_triangle(p, a, b, c);
+                        V3<R> d = p - q;
+                        const R dsq = dot(d, d);
+                        R dist = std::sqrt(std::max(dsq, R(kEpsF)));
+                        if (dist >= r) continue;
+                        V3<R> nn = mk<R>(0, 0, 1);
+                        if (dsq > R(kEpsF)) nn = mul(d, R(1) / dist);
+                        else {
+                            nn = cross(b - a, c - a);
+                            const R nsq = dot(nn, nn);
+                            if (nsq > R(kEpsF)) nn = mul(nn, rsqrt(nsq));
+                            else {
+                                const V3<R> rd = rest[v] - closest_point_on_triangle(rest[v], rest[ia], rest[ib], rest[ic]);
+                                const R rsq = dot(rd, rd);
+                                if (rsq > R(kEpsF)) nn = mul(rd, rsqrt(rsq));
+                            }
+                            d = mul(nn, dist);
+                            dist = R(0);
+                        }
+                        if (!finite(nn)) { count(kCountIslandGuarded); continue; }
+                        const V3<R> pd = pp - closest_point_on_triangle(pp, ap, bp, cp);
+                        if (dot(pd, nn) > r * R(0.25) && dot(d, nn) > -r * R(0.10)) continue;
+                        R wa = R(1) / R(3), wb = wa, wc = wa;
+                        triangle_barycentric(q, a, b, c, &wa, &wb, &wc);
+                        const R pen = r - dist;
+                        if (pen <= R(0)) continue;
+                        accepted[contacts++] = t;
+                        pen_sum += pen;
+                        sample_sum += R(inv_mass[ia]) * wa + R(inv_mass[ib]) * wb + R(inv_mass[ic]) * wc;
+                        axis = axis + mul(nn, pen);
+                        if (pen > max_pen) { max_pen = pen; best = nn; }
+                    }
+                }
+        if (contacts <= 0) continue;
+        counts[kCountIslandCandidates] += static_cast<unsigned long long>(contacts);
+        if (contacts < kIslandMinContacts) { count(kCountIslandGuarded); continue; }
+        const R avg = pen_sum / R(contacts);
+        if (avg < r * R(kIslandMinAvgDepthScale)) { count(kCountIslandGuarded); continue; }
+        const R axis_len = length(axis);
+        const V3<R> agg = axis_len > R(kEpsF) ? mul(axis, R(1) / axis_len) : best;
+        const R source_w = R(inv_mass[v]);
+        const R sample_w = sample_sum / R(contacts);
+        const R share = source_w / std::max(source_w + sample_w, R(kEpsF));
+        const R max_delta = std::max(r * R(kIslandMaxDeltaScale), R(1.0e-5));
+        const R dl = std::min(avg * R(kIslandCorrectionScale) * share, max_delta);
+        if (!finite(agg) || !std::isfinite(dl) || dl <= R(kEpsF)) { count(kCountIslandGuarded); continue; }
+        self_accumulate(v, mul(agg, dl));
+        count(kCountIslandClusters);
+        counts[kCountIslandVertexRefs] += static_cast<unsigned long long>(contacts);
+        count(kCountIslandApplied);
+        // A41:852-863
+        const float fd = static_cast<float>(dl);
+        if (std::isfinite(fd) && fd > 0.0f) {
+            const unsigned long long micro = static_cast<unsigned long long>(std::min(fd, 1024.0f) * 1000000.0f);
+            counts[kCountIslandMaxDeltaMicro] = std::max(counts[kCountIslandMaxDeltaMicro], micro);
+        }
+    }
+}
+
 template <class R> void Solver<R>::self_collision() {
     const int n = V();
+    const bool fast = cfg.self_collision_mode == kSelfModeFast;
     const R r = self_radius();
     const R onset = std::max(r * R(1.8), r + R(kEpsF));
     const R rest_skip = std::max(onset * R(1.25), r * R(2.5));
@@ -1550,6 +1703,7 @@ template <class R> void Solver<R>::self_collision() {
     }
     const int passes = std::max(1, std::min(cfg.fast_self_collision_passes, 8));
     int max_nb = std::max(1, std::min(cfg.max_self_collision_neighbors, kBucketSlots));
+    if (fast && max_nb > 16) max_nb = 16;  // A41:5133-5135, 5749-5751
     const R radius_half = r * R(0.5);
     const R thickness = radius_half + radius_half;
     auto valid_pair = [&](int s, int o) {  // A41:4696-4717
@@ -1559,20 +1713,41 @@ template <class R> void Solver<R>::self_collision() {
         V3<R> cd = pos[s] - pos[o], pd = prev[s] - prev[o];
         return dot(cd, cd) < onset * onset || dot(pd, pd) < onset * onset;
     };
-    auto accumulate = [&](int v, V3<R> d) {  // A41:1574-1587, fixed point
-        if (v < 0 || v >= n || inv_mass[v] <= 0.0f || pinned[v]) return;
-        const double s = 4294967296.0;
-        acc_q[3 * v] += static_cast<int64_t>(std::llrint(double(d.x) * s));
-        acc_q[3 * v + 1] += static_cast<int64_t>(std::llrint(double(d.y) * s));
-        acc_q[3 * v + 2] += static_cast<int64_t>(std::llrint(double(d.z) * s));
-        acc_w[v] += 1;
+    auto accumulate = [&](int v, V3<R> d) { self_accumulate(v, d); };
+    // A41:762-791; a null state counts the visit only (the brute-force loops)
+    auto try_note_seen = [&](SelfFilterState* st, int key) -> bool {
+        if (!fast) return true;
+        count(kCountFilterSeen);
+        if (!st) return true;
+        for (int i = 0; i < std::min(st->nseen, kFilterLocalSlots); ++i)
+            if (st->seen[i] == key) { count(kCountFilterRejectedDuplicate); return false; }
+        if (st->nseen >= kFilterLocalSlots) { count(kCountSelfOverflow); count(kCountFilterRejectedOwnership); return false; }
+        st->seen[st->nseen++] = key;
+        return true;
     };
-    for (int pass = 0; pass < passes; ++pass) {
-        std::fill(acc_q.begin(), acc_q.end(), 0);
-        std::fill(acc_w.begin(), acc_w.end(), 0);
-        snap_pos = pos;
-        snap_prev = prev;
-        // neighbour table, A41:4738-4788 (hash) / 4790-4814 (brute force)
+    // A41:793-829
+    auto try_local_accept = [&](SelfFilterState* st, int type, unsigned key) -> bool {
+        if (!fast) return true;
+        const int accept_slot = type == kPairVv ? kCountFilterAcceptedVv : (type == kPairVt ? kCountFilterAcceptedVt : kCountFilterAcceptedEe);
+        for (int i = 0; i < std::min(st->nclusters, kClusterMaxLocal); ++i) {
+            if (st->keys[i] != key) continue;
+            if (st->owned[i] >= kClusterMaxContactsPerOwner) { count(kCountFilterRejectedOwnership); return false; }
+            st->owned[i] += 1;
+            count(kCountClusterOwnedContacts);
+            count(accept_slot);
+            return true;
+        }
+        if (st->nclusters >= kClusterMaxLocal) { count(kCountFilterRejectedOwnership); return false; }
+        st->keys[st->nclusters] = key;
+        st->owned[st->nclusters] = 1;
+        st->nclusters += 1;
+        count(kCountClusterCount);
+        count(kCountClusterOwnedContacts);
+        count(accept_slot);
+        return true;
+    };
+    // neighbour table, A41:4738-4788 (hash) / 4790-4814 (brute force)
+    auto build_neighbors = [&]() {
         for (int s = 0; s < n; ++s) {
             unsigned acc = 0;
             unsigned* out = &nb_index[static_cast<size_t>(s) * kNeighborSlots];
@@ -1596,14 +1771,29 @@ template <class R> void Solver<R>::self_collision() {
             if (acc >= kNeighborSlots) count(kCountSelfOverflow);
             nb_counts[s] = acc;
         }
-        // vertex-vertex, A41:4816-4951 (strict mode: no filters)
+    };
+    // the fast mode builds the table once per self-collision substep and
+    // reuses it on every pass (A41:9291-9301)
+    if (fast) {
+        build_neighbors();
+        if (passes > 1) counts[kCountFilterCacheHits] += static_cast<unsigned long long>(n) * (passes - 1);
+        counts[kCountFilterCacheMisses] += static_cast<unsigned long long>(n);
+    }
+    for (int pass = 0; pass < passes; ++pass) {
+        std::fill(acc_q.begin(), acc_q.end(), 0);
+        std::fill(acc_w.begin(), acc_w.end(), 0);
+        snap_pos = pos;
+        snap_prev = prev;
+        if (!fast) build_neighbors();
+        // vertex-vertex, A41:4816-4951
         for (int s = 0; s < n; ++s) {
             if (inv_mass[s] <= 0.0f || pinned[s] || nb_counts[s] == 0) continue;
             V3<R> pc = pos[s], pp = prev[s];
+            SelfFilterState fs;
             for (unsigned k = 0; k < nb_counts[s]; ++k) {
                 int o = static_cast<int>(nb_index[static_cast<size_t>(s) * kNeighborSlots + k]);
                 if (o < 0 || o >= n || o == s) continue;
-                count(kCountSelfCandidates);
+                count(fast ? kCountFilterSeen : kCountSelfCandidates);
                 V3<R> qc = pos[o], qp = prev[o];
                 if (thickness <= R(0)) continue;
                 V3<R> resp = mk<R>(0, 0, 0);
@@ -1651,20 +1841,31 @@ template <class R> void Solver<R>::self_collision() {
                     R ml = std::max(thickness * R(0.35), R(1.0e-4));
                     R rl = length(resp);
                     if (rl > ml) resp = mul(resp, ml / rl);
+                    if (fast) {
+                        const V3<R> nn = mul(resp, R(1) / std::max(length(resp), R(kEpsF)));
+                        const V3<R> mid = mul(pc + qc, R(0.5));
+                        if (!try_local_accept(&fs, kPairVv, filter_cluster_key(kPairVv, mid, nn, std::max(thickness * R(2.5), R(1.0e-3)))))
+                            continue;
+                        count(kCountSelfCandidates);
+                    }
                     accumulate(s, resp);
                     count(kCountExactImpulse);
                 }
             }
         }
         // vertex-triangle, A41:4990-5107 pair, 5109-5182 hash, 5184-5224 brute
-        auto vt_pair = [&](int v, int t) -> bool {
+        auto vt_pair = [&](int v, int t, SelfFilterState* fs) -> bool {
             if (t < 0 || t >= cfg.triangle_count || r <= R(0) || onset <= r || !tri_valid(t)) return false;
             int ia = tris[3 * t], ib = tris[3 * t + 1], ic = tris[3 * t + 2];
-            if (v == ia || v == ib || v == ic) return false;
-            {
+            bool rest_near = v == ia || v == ib || v == ic;  // A41:4615-4630
+            if (!rest_near) {
                 V3<R> rq = closest_point_on_triangle(rest[v], rest[ia], rest[ib], rest[ic]);
                 V3<R> rdd = rest[v] - rq;
-                if (dot(rdd, rdd) < rest_skip * rest_skip) return false;
+                rest_near = dot(rdd, rdd) < rest_skip * rest_skip;
+            }
+            if (rest_near) {
+                if (fast) count(kCountFilterRejectedRest);
+                return false;
             }
             V3<R> p = pos[v], a = pos[ia], b = pos[ib], c = pos[ic];
             V3<R> q = closest_point_on_triangle(p, a, b, c);
@@ -1685,6 +1886,8 @@ template <class R> void Solver<R>::self_collision() {
                 }
                 dist = R(0);
             }
+            if (fast && !try_local_accept(fs, kPairVt, filter_cluster_key(kPairVt, q, mk<R>(0, 0, 0), std::max(onset * R(12), R(1.0e-3)))))
+                return true;
             count(kCountSelfCandidates);
             R wa = R(1) / R(3), wb = wa, wc = wa;
             triangle_barycentric(q, a, b, c, &wa, &wb, &wc);
@@ -1705,28 +1908,39 @@ template <class R> void Solver<R>::self_collision() {
         if (cfg.triangle_count > 0) {
             for (int v = 0; v < n; ++v) {
                 int accepted = 0;
+                SelfFilterState fs;
                 if (thash) {
                     int cx = cell_coord(pos[v].x, cs), cy = cell_coord(pos[v].y, cs), cz = cell_coord(pos[v].z, cs);
                     const auto& L = TH.lists[hash_cell(cx, cy, cz, TH.buckets)];
                     int lim = std::min(static_cast<int>(L.size()), kBucketSlots);
-                    for (int k = 0; k < lim && accepted < max_nb; ++k) if (vt_pair(v, L[k])) ++accepted;
+                    for (int k = 0; k < lim && accepted < max_nb; ++k) {
+                        if (!try_note_seen(&fs, L[k])) continue;
+                        if (vt_pair(v, L[k], &fs)) ++accepted;
+                    }
                 } else {
-                    for (int t = 0; t < cfg.triangle_count && accepted < max_nb; ++t) if (vt_pair(v, t)) ++accepted;
+                    for (int t = 0; t < cfg.triangle_count && accepted < max_nb; ++t) {
+                        if (!try_note_seen(nullptr, t)) continue;
+                        if (vt_pair(v, t, &fs)) ++accepted;
+                    }
                 }
             }
         }
         // edge-edge, A41:5595-5722 pair, 5724-5802 hash, 5804-5844 brute
-        auto ee_pair = [&](int ea, int eb) -> bool {
+        auto ee_pair = [&](int ea, int eb, SelfFilterState* fs) -> bool {
             if (eb <= ea || ea < 0 || eb >= cfg.edge_count || r <= R(0) || onset <= r) return false;
             if (!edge_valid(ea) || !edge_valid(eb)) return false;
             int a0 = springs[ea].i, a1 = springs[ea].j, b0 = springs[eb].i, b1 = springs[eb].j;
-            if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
-            {
+            bool rest_near = a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1;  // A41:5507-5530
+            if (!rest_near) {
                 R s = 0, t = 0;
                 closest_segment_parameters(rest[a0], rest[a1], rest[b0], rest[b1], &s, &t);
                 V3<R> pa = rest[a0] + mul(rest[a1] - rest[a0], s), pb = rest[b0] + mul(rest[b1] - rest[b0], t);
                 V3<R> dd = pa - pb;
-                if (dot(dd, dd) < rest_skip * rest_skip) return false;
+                rest_near = dot(dd, dd) < rest_skip * rest_skip;
+            }
+            if (rest_near) {
+                if (fast) count(kCountFilterRejectedRest);
+                return false;
             }
             R s = 0, t = 0;
             closest_segment_parameters(pos[a0], pos[a1], pos[b0], pos[b1], &s, &t);
@@ -1750,6 +1964,9 @@ template <class R> void Solver<R>::self_collision() {
                 }
                 dist = R(0);
             }
+            if (fast && !try_local_accept(fs, kPairEe, filter_cluster_key(kPairEe, mul(pa + pb, R(0.5)), mk<R>(0, 0, 0),
+                                                                          std::max(onset * R(12), R(1.0e-3)))))
+                return true;
             count(kCountSelfCandidates);
             R pen = r - dist, push;
             if (pen > R(0)) push = std::min(pen * R(0.35) + r * R(0.01), r * R(0.25));
@@ -1771,17 +1988,26 @@ template <class R> void Solver<R>::self_collision() {
             for (int e = 0; e < cfg.edge_count; ++e) {
                 if (ehash && !edge_valid(e)) continue;
                 int accepted = 0;
+                SelfFilterState fs;
                 if (ehash) {
                     V3<R> mid = mul(pos[springs[e].i] + pos[springs[e].j], R(0.5));
                     int cx = cell_coord(mid.x, cs), cy = cell_coord(mid.y, cs), cz = cell_coord(mid.z, cs);
                     const auto& L = EH.lists[hash_cell(cx, cy, cz, EH.buckets)];
                     int lim = std::min(static_cast<int>(L.size()), kBucketSlots);
-                    for (int k = 0; k < lim && accepted < max_nb; ++k) if (ee_pair(e, L[k])) ++accepted;
+                    for (int k = 0; k < lim && accepted < max_nb; ++k) {
+                        if (!try_note_seen(&fs, L[k])) continue;
+                        if (ee_pair(e, L[k], &fs)) ++accepted;
+                    }
                 } else {
-                    for (int o = e + 1; o < cfg.edge_count && accepted < max_nb; ++o) if (ee_pair(e, o)) ++accepted;
+                    for (int o = e + 1; o < cfg.edge_count && accepted < max_nb; ++o) {
+                        if (!try_note_seen(nullptr, o)) continue;
+                        if (ee_pair(e, o, &fs)) ++accepted;
+                    }
                 }
             }
         }
+        // overlap islands on the final fast pass, A41:5226-5439
+        if (fast && pass == passes - 1 && cfg.triangle_count > 0 && thash) island_aggregate(TH, cs);
         // average + apply, A41:4953-4988
         const R max_delta = std::max(r * R(0.35), R(1.0e-4));
         for (int i = 0; i < n; ++i) {
@@ -2375,6 +2601,24 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
         diag.abi41_bending_wing_count = counts[kCountBendingWings] > 0 ? static_cast<long long>(counts[kCountBendingWings]) : wing_count;
         diag.abi41_tack_jitter_guarded = static_cast<long long>(counts[kCountTackGuards]);
         diag.abi41_bending_guarded = static_cast<long long>(counts[kCountBendingGuards]);
+        // A41:8128-8160
+        diag.fast_overlap_island_candidates = static_cast<long long>(counts[kCountIslandCandidates]);
+        diag.fast_overlap_island_clusters = static_cast<long long>(counts[kCountIslandClusters]);
+        diag.fast_overlap_island_vertex_refs = static_cast<long long>(counts[kCountIslandVertexRefs]);
+        diag.fast_overlap_island_applied_vertices = static_cast<long long>(counts[kCountIslandApplied]);
+        diag.fast_overlap_island_guarded = static_cast<long long>(counts[kCountIslandGuarded]);
+        diag.fast_overlap_island_max_delta = static_cast<float>(counts[kCountIslandMaxDeltaMicro]) * 1.0e-6f;
+        diag.self_filter_seen = static_cast<long long>(counts[kCountFilterSeen]);
+        diag.self_filter_accepted_vv = static_cast<long long>(counts[kCountFilterAcceptedVv]);
+        diag.self_filter_accepted_vt = static_cast<long long>(counts[kCountFilterAcceptedVt]);
+        diag.self_filter_accepted_ee = static_cast<long long>(counts[kCountFilterAcceptedEe]);
+        diag.self_filter_rejected_rest = static_cast<long long>(counts[kCountFilterRejectedRest]);
+        diag.self_filter_rejected_duplicate = static_cast<long long>(counts[kCountFilterRejectedDuplicate]);
+        diag.self_filter_rejected_ownership = static_cast<long long>(counts[kCountFilterRejectedOwnership]);
+        diag.self_filter_cache_hits = static_cast<long long>(counts[kCountFilterCacheHits]);
+        diag.self_filter_cache_misses = static_cast<long long>(counts[kCountFilterCacheMisses]);
+        diag.self_cluster_count = static_cast<long long>(counts[kCountClusterCount]);
+        diag.self_cluster_owned_contacts = static_cast<long long>(counts[kCountClusterOwnedContacts]);
     }
     if (cfg.stretch_optimization_enabled) {  // A41:7586-7624, 8159-8176
         diag.abi41_pcg_csr_nnz = inc_off[cfg.vertex_count];

@@ -34,7 +34,14 @@ constexpr double kFixedScale = 4294967296.0;  // 2^32: self-collision accumulato
 enum CountSlot {
     kCountSoft = 0, kCountExactImpulse = 1, kCountEdgeEdge = 2, kCountHardFallbacks = 3,
     kCountSelfCandidates = 9, kCountSelfOverflow = 10, kCountStaticSdf = 11, kCountLraTacks = 12,
-    kCountBendingWings = 13, kCountTackGuards = 14, kCountBendingGuards = 15, kCountFriction = 33
+    kCountBendingWings = 13, kCountTackGuards = 14, kCountBendingGuards = 15,
+    // fast self-collision mode (A41:56-72)
+    kCountIslandCandidates = 16, kCountIslandClusters = 17, kCountIslandVertexRefs = 18, kCountIslandApplied = 19,
+    kCountIslandGuarded = 20, kCountIslandMaxDeltaMicro = 21, kCountFilterSeen = 22, kCountFilterAcceptedVv = 23,
+    kCountFilterAcceptedVt = 24, kCountFilterAcceptedEe = 25, kCountFilterRejectedRest = 26, kCountFilterRejectedDuplicate = 27,
+    kCountFilterRejectedOwnership = 28, kCountFilterCacheHits = 29, kCountFilterCacheMisses = 30, kCountClusterCount = 31,
+    kCountClusterOwnedContacts = 32,
+    kCountFriction = 33
 };
 
 // Work of the fused per-vertex pass, as a bit mask.

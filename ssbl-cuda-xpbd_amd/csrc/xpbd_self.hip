@@ -327,34 +327,95 @@ __device__ inline bool vv_valid(const ClothView& c, const SelfView& sv, const fl
     return dot(cd, cd) < sv.onset * sv.onset || dot(pd, pd) < sv.onset * sv.onset;
 }
 
+// Fast-mode contact filters (A41:708-849).  Every contact a thread accepts
+// opens a cluster keyed by a coarse cell of its contact point, its normal
+// direction and its pair type; at most 16 clusters per thread and one contact
+// per cluster.  The cluster keys live in LDS, slot-major per thread.
+constexpr int kClusterMaxLocal = 16;
+constexpr int kIslandMaxContacts = 48;
+
+struct FilterCounts {
+    unsigned seen = 0, acc_vv = 0, acc_vt = 0, acc_ee = 0, rej_rest = 0, rej_dup = 0, rej_own = 0, clusters = 0;
+};
+
+__device__ __forceinline__ int filter_normal_bucket(float v) { return v > 0.45f ? 2 : (v < -0.45f ? 0 : 1); }
+
+// A41:742-760
+__device__ __forceinline__ unsigned filter_cluster_key(int type, F3 point, F3 normal, float cell) {
+    const float cs = fmaxf(cell, 1.0e-3f);
+    unsigned key = hash_cell(cell_coord(point.x, cs), cell_coord(point.y, cs), cell_coord(point.z, cs), 1 << 20);
+    const unsigned nk = static_cast<unsigned>(filter_normal_bucket(normal.x)) | (static_cast<unsigned>(filter_normal_bucket(normal.y)) << 2) |
+                        (static_cast<unsigned>(filter_normal_bucket(normal.z)) << 4);
+    key ^= nk * 2654435761u;
+    key ^= static_cast<unsigned>(type + 1) * 2246822519u;
+    return key;
+}
+
+struct ClusterSet {
+    unsigned* col;   // this thread's column of a [16][blockDim] LDS array
+    int stride;
+    int n;
+    // A41:793-829 with one contact per cluster: a key seen before is refused.
+    __device__ __forceinline__ bool try_accept(unsigned key, FilterCounts& fc) {
+        for (int i = 0; i < n; ++i)
+            if (col[i * stride] == key) { ++fc.rej_own; return false; }
+        if (n >= kClusterMaxLocal) { ++fc.rej_own; return false; }
+        col[n * stride] = key;
+        ++n;
+        ++fc.clusters;
+        return true;
+    }
+};
+
+__device__ __forceinline__ void flush_filter_counts(unsigned long long* counts, const FilterCounts& fc) {
+    auto add = [&](int slot, unsigned v) { if (v) atomicAdd(&counts[slot], static_cast<unsigned long long>(v)); };
+    add(kCountFilterSeen, fc.seen);
+    add(kCountFilterAcceptedVv, fc.acc_vv);
+    add(kCountFilterAcceptedVt, fc.acc_vt);
+    add(kCountFilterAcceptedEe, fc.acc_ee);
+    add(kCountFilterRejectedRest, fc.rej_rest);
+    add(kCountFilterRejectedDuplicate, fc.rej_dup);
+    add(kCountFilterRejectedOwnership, fc.rej_own);
+    add(kCountClusterCount, fc.clusters);
+    add(kCountClusterOwnedContacts, fc.clusters);   // one owned contact per cluster
+}
+
 // Vertex part: candidate table (A41:4738-4814) + vertex-vertex response
-// (A41:4816-4951) + vertex-triangle repulsion (A41:5109-5224).
-__device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const float4* P, int s, bool count) {
+// (A41:4816-4951) + vertex-triangle repulsion (A41:5109-5224).  The strict
+// mode builds the table on every pass; the fast mode on the first pass of a
+// substep only (`build`), and later passes read it back.
+template <bool kFast>
+__device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const float4* P, int s, bool count, bool build,
+                                   ClusterSet& cl, FilterCounts& fc) {
     const float cs = sv.cell;
     const F3 ps = f3(P[s]);
     unsigned acc = 0;
     unsigned* out = sv.nb_idx + static_cast<size_t>(s) * kNeighborSlots;
-    if (sv.use_vhash) {
-        const int cx = cell_coord(ps.x, cs), cy = cell_coord(ps.y, cs), cz = cell_coord(ps.z, cs);
-        for (int dz = -1; dz <= 1 && acc < kNeighborSlots; ++dz)
-            for (int dy = -1; dy <= 1 && acc < kNeighborSlots; ++dy)
-                for (int dx = -1; dx <= 1 && acc < kNeighborSlots; ++dx) {
-                    const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
-                    const int b = static_cast<int>(hash_cell(qx, qy, qz, sv.vh.buckets));
-                    const int base = sv.vh.off[b], lim = min(sv.vh.count[b], kBucketSlots);
-                    for (int k = 0; k < lim && acc < kNeighborSlots; ++k) {
-                        const int o = sv.vh.entries[base + k];
-                        const int4 oc = sv.vcell[o];
-                        if (oc.x != qx || oc.y != qy || oc.z != qz || !vv_valid(c, sv, P, s, o)) continue;
-                        out[acc++] = static_cast<unsigned>(o);
+    if (build) {
+        if (sv.use_vhash) {
+            const int cx = cell_coord(ps.x, cs), cy = cell_coord(ps.y, cs), cz = cell_coord(ps.z, cs);
+            for (int dz = -1; dz <= 1 && acc < kNeighborSlots; ++dz)
+                for (int dy = -1; dy <= 1 && acc < kNeighborSlots; ++dy)
+                    for (int dx = -1; dx <= 1 && acc < kNeighborSlots; ++dx) {
+                        const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
+                        const int b = static_cast<int>(hash_cell(qx, qy, qz, sv.vh.buckets));
+                        const int base = sv.vh.off[b], lim = min(sv.vh.count[b], kBucketSlots);
+                        for (int k = 0; k < lim && acc < kNeighborSlots; ++k) {
+                            const int o = sv.vh.entries[base + k];
+                            const int4 oc = sv.vcell[o];
+                            if (oc.x != qx || oc.y != qy || oc.z != qz || !vv_valid(c, sv, P, s, o)) continue;
+                            out[acc++] = static_cast<unsigned>(o);
+                        }
                     }
-                }
+        } else {
+            for (int o = 0; o < sv.nv && acc < kNeighborSlots; ++o)
+                if (vv_valid(c, sv, P, s, o)) out[acc++] = static_cast<unsigned>(o);
+        }
+        if (acc >= kNeighborSlots && count) atomicAdd(&c.counts[kCountSelfOverflow], 1ull);
+        sv.nb_count[s] = acc;
     } else {
-        for (int o = 0; o < sv.nv && acc < kNeighborSlots; ++o)
-            if (vv_valid(c, sv, P, s, o)) out[acc++] = static_cast<unsigned>(o);
+        acc = min(sv.nb_count[s], static_cast<unsigned>(kNeighborSlots));
     }
-    if (acc >= kNeighborSlots && count) atomicAdd(&c.counts[kCountSelfOverflow], 1ull);
-    sv.nb_count[s] = acc;
 
     const float r = sv.radius, onset = sv.onset;
     const float thickness = r * 0.5f + r * 0.5f;
@@ -363,7 +424,9 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
         const F3 pc = ps, pp = f3(c.prev[s]);
         for (unsigned k = 0; k < acc; ++k) {
             const int o = static_cast<int>(out[k]);
-            ++cand;
+            if (o < 0 || o >= sv.nv || o == s) continue;
+            if (kFast) ++fc.seen;
+            else ++cand;
             const F3 qc = f3(P[o]), qp = f3(c.prev[o]);
             if (thickness <= 0.0f) continue;
             F3 resp = f3(0.0f, 0.0f, 0.0f);
@@ -411,6 +474,13 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
                 const float ml = fmaxf(thickness * 0.35f, 1.0e-4f);
                 const float rl = len3(resp);
                 if (rl > ml) resp = mul(resp, ml / rl);
+                if (kFast) {
+                    const F3 nn = mul(resp, 1.0f / fmaxf(len3(resp), kEps));
+                    const F3 mid = mul(pc + qc, 0.5f);
+                    if (!cl.try_accept(filter_cluster_key(0, mid, nn, fmaxf(thickness * 2.5f, 1.0e-3f)), fc)) continue;
+                    ++fc.acc_vv;
+                    ++cand;
+                }
                 accumulate(sv, P, s, resp);
                 ++hits;
             }
@@ -418,17 +488,22 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
     }
     // vertex-triangle (A41:4990-5107 per pair)
     if (sv.nt > 0) {
+        if (kFast) cl.n = 0;
         auto pair = [&](int v, int t) -> bool {
             if (t < 0 || t >= sv.nt || r <= 0.0f || onset <= r) return false;
             const int4 tr = sv.tris[t];
             if (!tri_valid(sv, tr)) return false;
             const int ia = tr.x, ib = tr.y, ic = tr.z;
-            if (v == ia || v == ib || v == ic) return false;
-            {
+            bool rest_near = v == ia || v == ib || v == ic;   // A41:4615-4630
+            if (!rest_near) {
                 const F3 rv = f3(c.rest[v]);
                 const F3 rq = closest_point_on_triangle(rv, f3(c.rest[ia]), f3(c.rest[ib]), f3(c.rest[ic]));
                 const F3 rdd = rv - rq;
-                if (dot(rdd, rdd) < sv.rest_skip * sv.rest_skip) return false;
+                rest_near = dot(rdd, rdd) < sv.rest_skip * sv.rest_skip;
+            }
+            if (rest_near) {
+                if (kFast) ++fc.rej_rest;
+                return false;
             }
             const F3 p = f3(P[v]), a = f3(P[ia]), b = f3(P[ib]), cc = f3(P[ic]);
             const F3 q = closest_point_on_triangle(p, a, b, cc);
@@ -449,6 +524,10 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
                     if (rsq > kEps) nn = mul(rdl, rsqrt_exact(rsq));
                 }
                 dist = 0.0f;
+            }
+            if (kFast) {
+                if (!cl.try_accept(filter_cluster_key(1, q, f3(0.0f, 0.0f, 0.0f), fmaxf(onset * 12.0f, 1.0e-3f)), fc)) return true;
+                ++fc.acc_vt;
             }
             ++cand;
             float wa = 1.0f / 3.0f, wb = wa, wc = wa;
@@ -476,11 +555,25 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
             const int cx = cell_coord(ps.x, cs), cy = cell_coord(ps.y, cs), cz = cell_coord(ps.z, cs);
             const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.th.buckets));
             const int base = sv.th.off[b], lim = min(sv.th.count[b], kBucketSlots);
-            for (int k = 0; k < lim && accepted < sv.max_nb; ++k)
-                if (pair(s, sv.th.entries[base + k] / kMaxPrimCells)) ++accepted;
-        } else {
-            for (int t = 0; t < sv.nt && accepted < sv.max_nb; ++t)
+            // A bucket lists its keys (triangle * 64 + cell) in ascending order,
+            // so the copies of one triangle are adjacent and the seen-id list
+            // of A41:762-791 reduces to a comparison with the previous entry
+            // (a bucket holds at most 32 entries, below its 64 slots).
+            int prev_t = -1;
+            for (int k = 0; k < lim && accepted < sv.max_nb; ++k) {
+                const int t = sv.th.entries[base + k] / kMaxPrimCells;
+                if (kFast) {
+                    ++fc.seen;
+                    if (t == prev_t) { ++fc.rej_dup; continue; }
+                    prev_t = t;
+                }
                 if (pair(s, t)) ++accepted;
+            }
+        } else {
+            for (int t = 0; t < sv.nt && accepted < sv.max_nb; ++t) {
+                if (kFast) ++fc.seen;
+                if (pair(s, t)) ++accepted;
+            }
         }
     }
     if (count) {
@@ -491,7 +584,9 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
 }
 
 // Edge part: A41:5595-5844.
-__device__ inline void self_edge(const ClothView& c, const SelfView& sv, const float4* P, int e, bool count) {
+template <bool kFast>
+__device__ inline void self_edge(const ClothView& c, const SelfView& sv, const float4* P, int e, bool count, ClusterSet& cl,
+                                 FilterCounts& fc) {
     const float r = sv.radius, onset = sv.onset, cs = sv.cell;
     unsigned cand = 0, hits = 0;
     auto pair = [&](int ea, int eb) -> bool {
@@ -499,13 +594,17 @@ __device__ inline void self_edge(const ClothView& c, const SelfView& sv, const f
         const int2 A = sv.edges[ea], B = sv.edges[eb];
         if (!edge_valid(sv, A) || !edge_valid(sv, B)) return false;
         const int a0 = A.x, a1 = A.y, b0 = B.x, b1 = B.y;
-        if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
-        {
+        bool rest_near = a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1;   // A41:5507-5530
+        if (!rest_near) {
             const F3 r0 = f3(c.rest[a0]), r1 = f3(c.rest[a1]), r2 = f3(c.rest[b0]), r3 = f3(c.rest[b1]);
             float s = 0.0f, t = 0.0f;
             closest_segment_parameters(r0, r1, r2, r3, &s, &t);
             const F3 dd = (r0 + mul(r1 - r0, s)) - (r2 + mul(r3 - r2, t));
-            if (dot(dd, dd) < sv.rest_skip * sv.rest_skip) return false;
+            rest_near = dot(dd, dd) < sv.rest_skip * sv.rest_skip;
+        }
+        if (rest_near) {
+            if (kFast) ++fc.rej_rest;
+            return false;
         }
         const F3 p0 = f3(P[a0]), p1 = f3(P[a1]), p2 = f3(P[b0]), p3 = f3(P[b1]);
         float s = 0.0f, t = 0.0f;
@@ -530,6 +629,11 @@ __device__ inline void self_edge(const ClothView& c, const SelfView& sv, const f
                 if (rsq > kEps) nn = mul(rd, rsqrt_exact(rsq));
             }
             dist = 0.0f;
+        }
+        if (kFast) {
+            if (!cl.try_accept(filter_cluster_key(2, mul(pa + pb, 0.5f), f3(0.0f, 0.0f, 0.0f), fmaxf(onset * 12.0f, 1.0e-3f)), fc))
+                return true;
+            ++fc.acc_ee;
         }
         ++cand;
         const float pen = r - dist;
@@ -560,11 +664,21 @@ __device__ inline void self_edge(const ClothView& c, const SelfView& sv, const f
         const int cx = cell_coord(mid.x, cs), cy = cell_coord(mid.y, cs), cz = cell_coord(mid.z, cs);
         const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.eh.buckets));
         const int base = sv.eh.off[b], lim = min(sv.eh.count[b], kBucketSlots);
-        for (int k = 0; k < lim && accepted < sv.max_nb; ++k)
-            if (pair(e, sv.eh.entries[base + k] / kMaxPrimCells)) ++accepted;
-    } else {
-        for (int o = e + 1; o < sv.ne && accepted < sv.max_nb; ++o)
+        int prev_e = -1;   // ascending keys: copies of one edge are adjacent
+        for (int k = 0; k < lim && accepted < sv.max_nb; ++k) {
+            const int o = sv.eh.entries[base + k] / kMaxPrimCells;
+            if (kFast) {
+                ++fc.seen;
+                if (o == prev_e) { ++fc.rej_dup; continue; }
+                prev_e = o;
+            }
             if (pair(e, o)) ++accepted;
+        }
+    } else {
+        for (int o = e + 1; o < sv.ne && accepted < sv.max_nb; ++o) {
+            if (kFast) ++fc.seen;
+            if (pair(e, o)) ++accepted;
+        }
     }
     if (count) {
         if (cand) atomicAdd(&c.counts[kCountSelfCandidates], static_cast<unsigned long long>(cand));
@@ -574,12 +688,152 @@ __device__ inline void self_edge(const ClothView& c, const SelfView& sv, const f
 
 // Threads [0, V) do the vertex part, [V, V + E) the edge part; every read is
 // of positions that stay fixed until the apply kernel.
-__global__ __launch_bounds__(256) void k_self_contacts(const ClothView* __restrict__ cv, SelfView sv, int parity, int count) {
+template <bool kFast>
+__global__ __launch_bounds__(256) void k_self_contacts(const ClothView* __restrict__ cv, SelfView sv, int parity, int count, int build) {
     const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const ClothView& c = *cv;
     const float4* P = cur_pos(c, parity);
-    if (k < sv.nv) self_vertex(c, sv, P, k, count != 0);
-    else if (k < sv.nv + sv.ne) self_edge(c, sv, P, k - sv.nv, count != 0);
+    FilterCounts fc;
+    ClusterSet cl{nullptr, 256, 0};
+    if (kFast) {
+        __shared__ unsigned keys[kClusterMaxLocal * 256];
+        cl.col = keys + threadIdx.x;
+    }
+    if (k < sv.nv) self_vertex<kFast>(c, sv, P, k, count != 0, !kFast || build != 0, cl, fc);
+    else if (k < sv.nv + sv.ne) self_edge<kFast>(c, sv, P, k - sv.nv, count != 0, cl, fc);
+    if (kFast && count) flush_filter_counts(c.counts, fc);
+}
+
+// A41:5226-5439: a vertex that sits inside three or more nearby triangles
+// (searched in the 27 triangle-hash cells around it) is pushed out along
+// their penetration-weighted mean normal.  Runs on the last pass of a fast
+// substep, before the averaging.
+constexpr int kIslandBlock = 128;
+__global__ __launch_bounds__(kIslandBlock) void k_self_island(const ClothView* __restrict__ cv, SelfView sv, int parity, int count) {
+    __shared__ int accepted_tris[kIslandMaxContacts * kIslandBlock];
+    const int v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= sv.nv) return;
+    const ClothView& c = *cv;
+    const float4* P = cur_pos(c, parity);
+    if (P[v].w == 0.0f) return;
+    const float r = sv.radius;
+    if (r <= 0.0f) return;
+    int* seen = accepted_tris + threadIdx.x;
+    const F3 p = f3(P[v]), pp = f3(c.prev[v]);
+    unsigned guarded = 0;
+    if (!finite3(p) || !finite3(pp)) {
+        if (count) atomicAdd(&c.counts[kCountIslandGuarded], 1ull);
+        return;
+    }
+    const float cs = sv.cell;
+    const int max_contacts = sv.island_contacts;
+    const int scan_limit = max_contacts * 10;
+    const int cx = cell_coord(p.x, cs), cy = cell_coord(p.y, cs), cz = cell_coord(p.z, cs);
+    const F3 rv = f3(c.rest[v]);
+    int scanned = 0, contacts = 0;
+    float pen_sum = 0.0f, sample_sum = 0.0f, max_pen = 0.0f;
+    F3 axis = f3(0.0f, 0.0f, 0.0f), best = f3(0.0f, 0.0f, 1.0f);
+    for (int dz = -1; dz <= 1 && contacts < max_contacts && scanned < scan_limit; ++dz)
+        for (int dy = -1; dy <= 1 && contacts < max_contacts && scanned < scan_limit; ++dy)
+            for (int dx = -1; dx <= 1 && contacts < max_contacts && scanned < scan_limit; ++dx) {
+                const int b = static_cast<int>(hash_cell(cx + dx, cy + dy, cz + dz, sv.th.buckets));
+                const int base = sv.th.off[b], lim = min(sv.th.count[b], kBucketSlots);
+                for (int k = 0; k < lim && contacts < max_contacts && scanned < scan_limit; ++k) {
+                    ++scanned;
+                    const int t = sv.th.entries[base + k] / kMaxPrimCells;
+                    bool dup = false;
+                    for (int i = 0; i < contacts; ++i)
+                        if (seen[i * kIslandBlock] == t) { dup = true; break; }
+                    if (dup) continue;
+                    const int4 tr = sv.tris[t];
+                    if (!tri_valid(sv, tr)) continue;
+                    const int ia = tr.x, ib = tr.y, ic = tr.z;
+                    if (v == ia || v == ib || v == ic) continue;
+                    {
+                        const F3 rd = rv - closest_point_on_triangle(rv, f3(c.rest[ia]), f3(c.rest[ib]), f3(c.rest[ic]));
+                        if (dot(rd, rd) < sv.rest_skip * sv.rest_skip) continue;
+                    }
+                    const F3 a = f3(P[ia]), bb = f3(P[ib]), cc = f3(P[ic]);
+                    const F3 ap = f3(c.prev[ia]), bp = f3(c.prev[ib]), cp = f3(c.prev[ic]);
+                    if (!finite3(a) || !finite3(bb) || !finite3(cc) || !finite3(ap) || !finite3(bp) || !finite3(cp)) {
+                        ++guarded;
+                        continue;
+                    }
+                    const F3 q = closest_point_on_triangle(p, a, bb, cc);
+                    F3 d = p - q;
+                    const float dsq = dot(d, d);
+                    float dist = sqrtf(fmaxf(dsq, kEps));
+                    if (dist >= r) continue;
+                    F3 nn = f3(0.0f, 0.0f, 1.0f);
+                    if (dsq > kEps) nn = mul(d, 1.0f / dist);
+                    else {
+                        nn = cross(bb - a, cc - a);
+                        const float nsq = dot(nn, nn);
+                        if (nsq > kEps) nn = mul(nn, rsqrt_exact(nsq));
+                        else {
+                            const F3 rd = rv - closest_point_on_triangle(rv, f3(c.rest[ia]), f3(c.rest[ib]), f3(c.rest[ic]));
+                            const float rsq = dot(rd, rd);
+                            if (rsq > kEps) nn = mul(rd, rsqrt_exact(rsq));
+                        }
+                        d = mul(nn, dist);
+                        dist = 0.0f;
+                    }
+                    if (!finite3(nn)) { ++guarded; continue; }
+                    const F3 pd = pp - closest_point_on_triangle(pp, ap, bp, cp);
+                    if (dot(pd, nn) > r * 0.25f && dot(d, nn) > -r * 0.10f) continue;
+                    float wa = 1.0f / 3.0f, wb = wa, wc = wa;
+                    triangle_barycentric(q, a, bb, cc, &wa, &wb, &wc);
+                    const float pen = r - dist;
+                    if (pen <= 0.0f) continue;
+                    seen[contacts * kIslandBlock] = t;
+                    ++contacts;
+                    pen_sum += pen;
+                    sample_sum += c.rest[ia].w * wa + c.rest[ib].w * wb + c.rest[ic].w * wc;
+                    axis = axis + mul(nn, pen);
+                    if (pen > max_pen) { max_pen = pen; best = nn; }
+                }
+            }
+    if (contacts > 0) {
+        bool applied = false;
+        float dl = 0.0f;
+        F3 agg = best;
+        if (contacts < 3) ++guarded;
+        else {
+            const float avg = pen_sum / static_cast<float>(contacts);
+            if (avg < r * 0.04f) ++guarded;
+            else {
+                const float axis_len = len3(axis);
+                agg = axis_len > kEps ? mul(axis, 1.0f / axis_len) : best;
+                const float source_w = c.rest[v].w;
+                const float sample_w = sample_sum / static_cast<float>(contacts);
+                const float share = source_w / fmaxf(source_w + sample_w, kEps);
+                const float max_delta = fmaxf(r * 0.30f, 1.0e-5f);
+                dl = fminf(avg * 0.50f * share, max_delta);
+                if (!finite3(agg) || !isfinite(dl) || dl <= kEps) ++guarded;
+                else applied = true;
+            }
+        }
+        if (applied) accumulate(sv, P, v, mul(agg, dl));
+        if (count) {
+            atomicAdd(&c.counts[kCountIslandCandidates], static_cast<unsigned long long>(contacts));
+            if (applied) {
+                atomicAdd(&c.counts[kCountIslandClusters], 1ull);
+                atomicAdd(&c.counts[kCountIslandVertexRefs], static_cast<unsigned long long>(contacts));
+                atomicAdd(&c.counts[kCountIslandApplied], 1ull);
+                // A41:852-863
+                atomicMax(&c.counts[kCountIslandMaxDeltaMicro], static_cast<unsigned long long>(fminf(dl, 1024.0f) * 1000000.0f));
+            }
+        }
+    }
+    if (count && guarded) atomicAdd(&c.counts[kCountIslandGuarded], static_cast<unsigned long long>(guarded));
+}
+
+// Cache counters of a fast substep (A41:831-850): the table built once serves
+// every pass.
+__global__ void k_self_cache_counts(unsigned long long* counts, unsigned long long hits, unsigned long long misses) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (hits) atomicAdd(&counts[kCountFilterCacheHits], hits);
+    if (misses) atomicAdd(&counts[kCountFilterCacheMisses], misses);
 }
 
 // A41:4953-4988: average, clamp, apply; the accumulators are cleared for the
@@ -643,6 +897,12 @@ bool SelfCollision::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& e
     s.cell = std::max(r * 1.8f, 1.0e-3f);
     s.prim_margin = r * 1.8f;
     s.max_nb = std::max(1, std::min(cfg.max_self_collision_neighbors, kBucketSlots));
+    s.fast = cfg.self_collision_mode == 1 ? 1 : 0;   // A41:140
+    if (s.fast && s.max_nb > 16) s.max_nb = 16;       // A41:5133-5135, 5749-5751
+    {
+        const long long mn = std::max(cfg.max_self_collision_neighbors, 1);
+        s.island_contacts = static_cast<int>(std::max<long long>(std::min<long long>(2 * mn, kIslandMaxContacts), 3));
+    }
     s.counts = counts;
     s.use_vhash = nv >= 128 ? 1 : 0;
     s.use_thash = s.nt >= 64 ? 1 : 0;
@@ -727,11 +987,24 @@ bool SelfCollision::run_for(hipStream_t st, const ClothView* cv, int parity, boo
         if (s.use_thash) build_prim_hash(st, cv, s, s.th, 0, s.nt, parity, count);
         if (s.use_ehash) build_prim_hash(st, cv, s, s.eh, 1, s.ne, parity, count);
     }
+    const dim3 contacts_grid(grid_for(static_cast<long long>(s.nv) + s.ne, 256));
+    if (s.fast && count) {
+        // A41:9291-9301: the fast mode builds the table once per substep
+        launch_k(k_self_cache_counts, dim3(1), dim3(1), 0, st, s.counts,
+                 static_cast<unsigned long long>(s.nv) * static_cast<unsigned long long>(passes_ - 1),
+                 static_cast<unsigned long long>(s.nv));
+    }
     for (int pass = 0; pass < passes_; ++pass) {
         {
             TimedLaunch t("self_contacts", st);
-            launch_k(k_self_contacts, dim3(grid_for(static_cast<long long>(s.nv) + s.ne, 256)), dim3(256), 0, st, cv, s,
-                     parity, count ? 1 : 0);
+            if (s.fast)
+                launch_k(k_self_contacts<true>, contacts_grid, dim3(256), 0, st, cv, s, parity, count ? 1 : 0, pass == 0 ? 1 : 0);
+            else
+                launch_k(k_self_contacts<false>, contacts_grid, dim3(256), 0, st, cv, s, parity, count ? 1 : 0, 1);
+        }
+        if (s.fast && pass == passes_ - 1 && s.nt > 0 && s.use_thash) {
+            TimedLaunch t("self_island", st);
+            launch_k(k_self_island, dim3(grid_for(s.nv, kIslandBlock)), dim3(kIslandBlock), 0, st, cv, s, parity, count ? 1 : 0);
         }
         {
             TimedLaunch t("self_apply", st);

@@ -2,7 +2,8 @@
 //
 // Reference: A41:4542-5844 (hash builds, candidate table, vertex-vertex,
 // vertex-triangle and edge-edge responses, averaging), host set-up
-// A41:6784-6994, schedule A41:9255-9339 (strict mode).
+// A41:6784-6994, schedule A41:9255-9345; fast-mode filters A41:708-849,
+// overlap islands A41:5226-5439.
 #pragma once
 
 #include <string>
@@ -40,7 +41,9 @@ struct SelfView {
     int nv, nt, ne;
     int use_vhash, use_thash, use_ehash;
     float cell, radius, onset, rest_skip, prim_margin;
-    int max_nb;
+    int max_nb;                  // vertex-triangle / edge-edge candidates per query (16 at most in fast mode)
+    int fast;                    // fast mode: contact filters, table reuse, overlap islands
+    int island_contacts;         // overlap-island contact cap, clamp(2 * max neighbours, 3, 48)
     int4* vcell;                 // V: cell x, y, z, bucket (vertex hash)
     HashView vh, th, eh;
     unsigned* nb_count;          // V

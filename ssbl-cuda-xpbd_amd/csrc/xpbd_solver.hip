@@ -395,7 +395,6 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     cfg.stretch_optimization_strength = std::min(std::max(cfg.stretch_optimization_strength, 0.0f), 1.0f);
     cfg.stretch_optimization_enabled =
         (cfg.edge_count > 0 && cfg.stretch_optimization_enabled && cfg.stretch_optimization_strength > 0.0f) ? 1 : 0;
-    if (cfg.self_collision && cfg.self_collision_mode == 1) return fail("fast self-collision mode is not available in this build yet");
     pressure_accel_scale = std::max(1.0f, std::min(env_float("SSBL_ABI41_PRESSURE_UI_ACCEL_SCALE", 1792.0f), 8192.0f));
     pressure_impulse_rest_scale = std::max(0.05f, std::min(env_float("SSBL_ABI41_PRESSURE_IMPULSE_REST_SCALE", 0.85f), 8.0f));
     pressure_velocity_rest_scale = std::max(0.10f, std::min(env_float("SSBL_ABI41_PRESSURE_VELOCITY_REST_SCALE", 2.25f), 16.0f));
@@ -990,6 +989,24 @@ static bool fetch_counts(Solver& s) {  // A41:8059-8179
     d.abi41_bending_wing_count = c[kCountBendingWings] > 0 ? static_cast<long long>(c[kCountBendingWings]) : s.wing_count;
     d.abi41_tack_jitter_guarded = static_cast<long long>(c[kCountTackGuards]) + (s.cfg.lra_count > 0 ? s.lra_invalid : 0);
     d.abi41_bending_guarded = static_cast<long long>(c[kCountBendingGuards]);
+    // A41:8128-8160
+    d.fast_overlap_island_candidates = static_cast<long long>(c[kCountIslandCandidates]);
+    d.fast_overlap_island_clusters = static_cast<long long>(c[kCountIslandClusters]);
+    d.fast_overlap_island_vertex_refs = static_cast<long long>(c[kCountIslandVertexRefs]);
+    d.fast_overlap_island_applied_vertices = static_cast<long long>(c[kCountIslandApplied]);
+    d.fast_overlap_island_guarded = static_cast<long long>(c[kCountIslandGuarded]);
+    d.fast_overlap_island_max_delta = static_cast<float>(c[kCountIslandMaxDeltaMicro]) * 1.0e-6f;
+    d.self_filter_seen = static_cast<long long>(c[kCountFilterSeen]);
+    d.self_filter_accepted_vv = static_cast<long long>(c[kCountFilterAcceptedVv]);
+    d.self_filter_accepted_vt = static_cast<long long>(c[kCountFilterAcceptedVt]);
+    d.self_filter_accepted_ee = static_cast<long long>(c[kCountFilterAcceptedEe]);
+    d.self_filter_rejected_rest = static_cast<long long>(c[kCountFilterRejectedRest]);
+    d.self_filter_rejected_duplicate = static_cast<long long>(c[kCountFilterRejectedDuplicate]);
+    d.self_filter_rejected_ownership = static_cast<long long>(c[kCountFilterRejectedOwnership]);
+    d.self_filter_cache_hits = static_cast<long long>(c[kCountFilterCacheHits]);
+    d.self_filter_cache_misses = static_cast<long long>(c[kCountFilterCacheMisses]);
+    d.self_cluster_count = static_cast<long long>(c[kCountClusterCount]);
+    d.self_cluster_owned_contacts = static_cast<long long>(c[kCountClusterOwnedContacts]);
     return s.pcg.fetch(&d);
 }
 

@@ -95,17 +95,21 @@ def bend_pair(lib, compliance=0.0):
     return folded, r.positions(), rest
 
 
-# --- self collision: the reference smoke scenes (abi41_smoke.cpp:227-597) in
-# strict mode (self_collision_mode 2), with the smoke's own pass criteria.
-SELF_CFG = dict(self_collision=1, self_collision_mode=2, cloth_thickness=0.10, collision_margin=0.0,
-                max_self_collision_neighbors=32, fast_self_collision_passes=4)
+# --- self collision: the reference smoke scenes (abi41_smoke.cpp:227-597),
+# with the smoke's own pass criteria.  The smoke runs them in fast mode
+# (self_collision_mode 1) with one pass; mode 2 is the strict mode.
+SELF_CFG = dict(self_collision=1, cloth_thickness=0.10, collision_margin=0.0, max_self_collision_neighbors=32)
 
 
-def self_vv(lib):
+def _self_cfg(mode, passes):
+    return dict(SELF_CFG, self_collision_mode=mode, fast_self_collision_passes=passes)
+
+
+def self_vv(lib, mode=2, passes=4):
     n = 600
     rest = np.zeros((n, 3), np.float32)
     rest[:, 0] = np.arange(n)
-    r = Raw(lib, rest, np.ones(n, np.float32), **SELF_CFG)
+    r = Raw(lib, rest, np.ones(n, np.float32), **_self_cfg(mode, passes))
     close = rest.copy()
     close[n - 1, 0] = 0.03
     r.set_positions(close)
@@ -114,7 +118,7 @@ def self_vv(lib):
     return float(np.linalg.norm(close[0] - close[n - 1])), float(np.linalg.norm(out[0] - out[n - 1])), r.diag(), r
 
 
-def self_vt(lib):
+def self_vt(lib, mode=2, passes=4):
     n, ntri = 600, 64
     rest = np.zeros((n, 3), np.float32)
     rest[0], rest[1], rest[2] = (-1, -1, 0), (1, -1, 0), (0, 1, 0)
@@ -124,7 +128,7 @@ def self_vt(lib):
         rest[i] = (20.0 + i, 20.0 + (i % 7) * 0.25, (i % 5) * 0.20)
     rest[n - 1] = (5.0, 5.0, 1.0)
     tris = [[0, 1, 2]] + [[3 + t * 3, 4 + t * 3, 5 + t * 3] for t in range(1, ntri)]
-    r = Raw(lib, rest, inv, triangles=tris, **SELF_CFG)
+    r = Raw(lib, rest, inv, triangles=tris, **_self_cfg(mode, passes))
     close = rest.copy()
     close[n - 1] = (0.0, -0.2, 0.03)
     r.set_positions(close)
@@ -133,7 +137,7 @@ def self_vt(lib):
     return float(close[n - 1, 2]), float(out[n - 1, 2]), r.diag(), r
 
 
-def self_ee(lib):
+def self_ee(lib, mode=2, passes=4):
     n, ne = 600, 64
     rest = np.zeros((n, 3), np.float32)
     rest[:, 0] = 200.0 + np.arange(n) * 3.0
@@ -149,7 +153,7 @@ def self_ee(lib):
         edges.append([a, a + 1])
     edges = np.asarray(edges, np.int32)
     lengths = np.linalg.norm(rest[edges[:, 0]] - rest[edges[:, 1]], axis=1).astype(np.float32)
-    r = Raw(lib, rest, inv, edges=edges, edge_rest=lengths, **SELF_CFG)
+    r = Raw(lib, rest, inv, edges=edges, edge_rest=lengths, **_self_cfg(mode, passes))
     close = rest.copy()
     close[2] = (0.0, -1.0, 0.03)
     close[3] = (0.0, 1.0, 0.03)
@@ -159,17 +163,17 @@ def self_ee(lib):
     return float((close[2, 2] + close[3, 2]) / 2), float((out[2, 2] + out[3, 2]) / 2), r.diag(), r
 
 
-def crumple(scenes, n=40, frames=25, seed=1, self_distance=0.01 / 1.8):
+def crumple(scenes, n=40, frames=25, seed=1, self_distance=0.01 / 1.8, mode=2, max_neighbors=32):
     """A grid dropped onto a small sphere so it folds over itself."""
     cloth = scenes.grid_cloth(n, 0.8, 0.5, pin_corners=False, seed=seed)
     opts = scenes.SolverOptions(use_ground=True, use_sphere=True, sphere_center=(0.0, 0.0, 0.2), sphere_radius=0.12,
-                                contact_friction=0.3, cloth_thickness=0.005, self_collision=True, self_collision_mode=2,
+                                contact_friction=0.3, cloth_thickness=0.005, self_collision=True, self_collision_mode=mode,
                                 self_collision_distance=self_distance, fast_self_collision_passes=4,
-                                max_self_collision_neighbors=32)
+                                max_self_collision_neighbors=max_neighbors)
     return cloth, opts, frames
 
 
-def folded_cloth(scenes, n=30, size=0.6, gap=0.004, seed=2):
+def folded_cloth(scenes, n=30, size=0.6, gap=0.004, seed=2, mode=2, passes=1):
     """A grid whose upper half starts folded onto the lower half, `gap` above it."""
     cloth = scenes.grid_cloth(n, size, 0.3, pin_corners=False, seed=seed, jitter=0.0)
     p = cloth.positions_world.astype(np.float64).copy()
@@ -183,9 +187,92 @@ def folded_cloth(scenes, n=30, size=0.6, gap=0.004, seed=2):
     rng = np.random.Generator(np.random.PCG64(seed))
     cloth.start_positions = (p + rng.normal(0, 1e-4, p.shape)).astype(np.float32)
     opts = scenes.SolverOptions(use_ground=True, contact_friction=0.3, cloth_thickness=0.005, self_collision=True,
-                                self_collision_mode=2, self_collision_distance=0.01 / 1.8, fast_self_collision_passes=1,
+                                self_collision_mode=mode, self_collision_distance=0.01 / 1.8, fast_self_collision_passes=passes,
                                 max_self_collision_neighbors=32, gravity=(0.0, 0.0, 0.0))
     return cloth, opts
+
+
+def vv_clusters(lib, mode, same_cluster):
+    """One free vertex s with two pinned neighbours inside the contact
+    thickness (0.1 m).  Each neighbour pushes s by 0.035 m (the 0.35 x
+    thickness clamp) along their separation.  With same_cluster the two
+    contact midpoints share a 0.25 m cluster cell and normal bucket, so the
+    fast mode keeps only the first neighbour's push; otherwise (and in the
+    strict mode) s moves by the mean of both pushes (A41:4816-4951, 793-829)."""
+    start = np.array([[0.1, 0.1, 0.1], [0.15, 0.11, 0.1], [0.15, 0.09, 0.1] if same_cluster else [0.05, 0.1, 0.1]],
+                     np.float32)
+    # pinned vertices hold their rest positions; s rests far from both
+    rest = start.copy()
+    rest[0] = (5.0, 5.0, 5.0)
+    inv = np.array([1.0, 0.0, 0.0], np.float32)
+    r = Raw(lib, rest, inv, self_collision=1, self_collision_mode=mode, fast_self_collision_passes=1,
+            self_collision_distance=0.1, cloth_thickness=0.1, collision_margin=0.0, max_self_collision_neighbors=32,
+            gravity=(0.0, 0.0, 0.0))
+    r.set_positions(start)
+    r.step(1, 1)
+    return start, r.positions(), r.diag()
+
+
+def vv_clusters_expected(start, mode, same_cluster):
+    s = start[0].astype(np.float64)
+    pushes = []
+    for o in (1, 2):
+        d = s - start[o].astype(np.float64)
+        pushes.append(d / np.linalg.norm(d) * 0.035)
+    keep = pushes[:1] if (mode == 1 and same_cluster) else pushes
+    return s + np.mean(keep, axis=0)
+
+
+def overlap_island(lib, crossing, copies=3, r=0.03, drop=0.02, triangles=64):
+    """A free vertex falls through `copies` coincident triangles (0.6 m wide,
+    centroid at the origin, z = 0) in one substep of 1/60 s and ends `drop`
+    lower, inside the contact radius r below them.  Gravity makes the fall
+    exactly `drop` up to float rounding (x1 = x0 + g h^2, no damping).
+
+    The triangles span more than 64 cells of the 1.8 r triangle hash, so each
+    is hashed at its centroid cell only (A41:871-900): the overlap-island
+    search over the 27 cells around the vertex finds them, the
+    vertex-triangle query of the vertex's own cell (one cell lower) does not,
+    and the vertex-vertex table is empty, so only the island moves the
+    vertex.  With `crossing` it started above the triangles; otherwise below
+    and deeper than r/4, which the island test skips (A41:5371-5374).  The
+    remaining far-away triangles bring the count to the 64 that enable the
+    triangle hash."""
+    h = np.float32(np.float32(1.0) / np.float32(60.0))
+    g = -drop / float(h) ** 2
+    z0 = 0.004 if crossing else -0.008
+    tri = [(-0.3, -0.2, 0.0), (0.3, -0.2, 0.0), (0.0, 0.4, 0.0)]
+    pos = [(0.0, 0.0, z0)] + tri * copies
+    tris = [[1 + 3 * k, 2 + 3 * k, 3 + 3 * k] for k in range(copies)]
+    for t in range(copies, triangles):
+        b = len(pos)
+        pos += [(20.0 + 3 * t, 20.0, 0.0), (21.0 + 3 * t, 20.0, 0.0), (20.0 + 3 * t, 21.0, 0.0)]
+        tris.append([b, b + 1, b + 2])
+    pos = np.asarray(pos, np.float32)
+    rest = pos.copy()
+    rest[0] = (10.0, 10.0, 10.0)   # far from the triangles at rest: no rest-neighbour skip
+    inv = np.zeros(len(pos), np.float32)
+    inv[0] = 1.0
+    raw = Raw(lib, rest, inv, triangles=tris, self_collision=1, self_collision_mode=1, fast_self_collision_passes=1,
+              self_collision_distance=r, cloth_thickness=r, collision_margin=0.0, max_self_collision_neighbors=16,
+              gravity=(0.0, 0.0, g))
+    raw.set_positions(pos)
+    raw.step(1, 1)
+    return raw.positions(), raw.diag()
+
+
+def overlap_island_expected(crossing, copies=3, r=0.03, drop=0.02):
+    """Closed form of A41:5396-5417 for the scene above: every contact has
+    depth r - d along -z; the vertex moves by min(0.5 x depth x share, 0.3 r)
+    with share = w_v / (w_v + mean triangle weight at the contact point) = 1
+    (pinned triangles weigh 0)."""
+    z0 = 0.004 if crossing else -0.008
+    z1 = z0 - drop
+    if not crossing or copies < 3:
+        return z1, 0.0
+    depth = r - abs(z1)
+    dl = min(depth * 0.5, 0.3 * r)
+    return z1 - dl, dl
 
 
 # --- hard-stretch PCG

@@ -41,6 +41,10 @@ def test_grid_cloth_with_pcg(nb, hip, oracle, scenes, strength):
     assert dg.abi41_pcg_iterations == dr.abi41_pcg_iterations
 
 
+SMOKE_MARKERS = ("HARD_STRETCH_OK", "OVERPRESSURE_OK", "TACK_BENDING_OK", "STATIC_SDF_OK", "PIN_WEIGHT_OK", "NATIVE_OK",
+                 "SELF_COLLISION_OK", "SELF_VT_OK", "SELF_EE_OK")
+
+
 def test_reference_smoke_binary_on_hip():
     """The reference's own native smoke test, compiled unchanged against the
     HIP library (oracle/Makefile target `ref`)."""
@@ -50,5 +54,6 @@ def test_reference_smoke_binary_on_hip():
         pytest.skip("reference smoke binary not built (needs the reference tree at build time)")
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     print(out.stdout[-2000:], out.stderr[-2000:])
-    for marker in ("HARD_STRETCH_OK", "OVERPRESSURE_OK", "TACK_BENDING_OK", "STATIC_SDF_OK", "PIN_WEIGHT_OK", "NATIVE_OK"):
+    for marker in SMOKE_MARKERS:
         assert f"SSBL_ABI41_{marker}" in out.stdout, out.stdout
+    assert out.returncode == 0

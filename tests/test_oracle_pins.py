@@ -160,3 +160,19 @@ def test_fp32_oracle_close_to_fp64_on_plain_config(o64, nb, scenes):
     err = np.max(np.abs(d.download_positions().astype(np.float64) - f.download_positions()))
     print(f"FP32 vs FP64 oracle, plain 32x32, 1 frame: {err:.3e} m")
     assert err < 5e-5
+
+
+def test_reference_smoke_binary_on_oracle():
+    """The reference's own native smoke test, compiled unchanged against the
+    oracle (oracle/Makefile target `ref`): every section passes, including
+    the fast-mode self-collision scenes."""
+    import subprocess
+    from conftest import ROOT
+    exe = ROOT / "oracle" / "_ref" / "abi41_smoke_oracle"
+    if not exe.exists():
+        pytest.skip("reference smoke binary not built (needs the reference tree at build time)")
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    for marker in ("HARD_STRETCH_OK", "OVERPRESSURE_OK", "TACK_BENDING_OK", "STATIC_SDF_OK", "PIN_WEIGHT_OK", "NATIVE_OK",
+                   "SELF_COLLISION_OK", "SELF_VT_OK", "SELF_EE_OK"):
+        assert f"SSBL_ABI41_{marker}" in out.stdout, out.stdout + out.stderr
+    assert out.returncode == 0
