@@ -224,6 +224,53 @@ __global__ __launch_bounds__(256) void k_part_hash(const DynView* __restrict__ d
     if (count_diag) wave_count(counts, kCountDynParticleOverflow, over);
 }
 
+// Coarse particle summary (see DynView::coarse).  Each particle marks every
+// coarse cell its reach box overlaps; the box is widened by a relative slack
+// so that float rounding of the box bounds cannot leave out the coarse cell of
+// a point the particle reaches.
+__global__ __launch_bounds__(256) void k_coarse_fill(int2* t, int n, int* off) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) t[i] = make_int2(0x7fffffff, -2);
+    if (i == 0) *off = 0;
+}
+
+__global__ __launch_bounds__(256) void k_coarse_insert(const DynView* __restrict__ dv, int2* t, int* off) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const DynView& D = *dv;
+    if (i >= D.np) return;
+    const float4 p = D.pp[i];
+    if (!isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z)) { atomicExch(off, 1); return; }
+    const int src = D.psrc ? D.psrc[i] : -1;
+    const int layer = D.psrc ? (D.player ? D.player[i] : 0) : static_cast<int>(0x80000000u);
+    const float cc = D.coarse_cell, r = D.coarse_reach;
+    int lo[3], n[3];
+    const float c3[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float reach = r * 1.001f + fabsf(c3[a]) * 1.0e-5f + 1.0e-6f;
+        lo[a] = cell_coord(c3[a] - reach, cc);
+        const long long span = static_cast<long long>(cell_coord(c3[a] + reach, cc)) - lo[a] + 1;
+        // the box is under a coarse cell wide, so this fails only where the
+        // cell coordinates saturate: then the summary is not used at all
+        if (span < 1 || span > 3) { atomicExch(off, 1); return; }
+        n[a] = static_cast<int>(span);
+    }
+    for (int kz = 0; kz < n[2]; ++kz)
+        for (int ky = 0; ky < n[1]; ++ky)
+            for (int kx = 0; kx < n[0]; ++kx) {
+                int2* e = t + hash_cell(lo[0] + kx, lo[1] + ky, lo[2] + kz, D.coarse_buckets);
+                // plain coherent reads first: neighbouring particles mostly
+                // find their cells already marked and issue no atomic at all
+                if (__hip_atomic_load(&e->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > layer) atomicMin(&e->x, layer);
+                int cur = __hip_atomic_load(&e->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (cur != src && cur != -1) {
+                    const int prev = atomicCAS(&e->y, cur, cur == -2 ? src : -1);
+                    if (prev == cur) break;
+                    cur = prev;
+                }
+            }
+}
+
 // Scene gathers: a segment {source, begin, end} is a block's range.
 __global__ __launch_bounds__(256) void k_gather_particles(const GatherSource* __restrict__ src, const int4* __restrict__ segs,
                                                           float4* pp, float4* pprev, int* psrc, int* player, int keep_prev) {
@@ -362,6 +409,11 @@ bool DynamicColliders::reserve_particles(int n) {
     pksl_ = static_cast<int2*>(alloc(1, sizeof(int2) * m));
     pkcell_ = static_cast<int4*>(alloc(1, sizeof(int4) * m));
     const int buckets = next_pow2(std::max(kDynMinBuckets, n * 4));
+    // a particle marks about 3.4 coarse cells and a sheet puts about four
+    // particles in one, so 4n buckets keep unrelated cells mostly apart
+    coarse_buckets_ = next_pow2(std::max(kDynMinBuckets, n * 4));
+    coarse_ = static_cast<int2*>(alloc(1, sizeof(int2) * coarse_buckets_));
+    coarse_off_ = static_cast<int*>(alloc(1, sizeof(int)));
     pbin_ = static_cast<int4*>(alloc(1, sizeof(int4) * buckets));
     pcount_ = static_cast<int*>(alloc(1, sizeof(int) * buckets));
     poff_ = static_cast<int*>(alloc(1, sizeof(int) * (buckets + 1)));
@@ -370,7 +422,9 @@ bool DynamicColliders::reserve_particles(int n) {
         scan_tmp_ = static_cast<int*>(alloc(2, sizeof(int) * scan_tmp_ints(buckets)));
         scan_cap_ = scan_tmp_ ? buckets : 0;
     }
-    if (!pp_ || !pprev_ || !psrc_ || !player_ || !pcell_ || !pslot_ || !praw_ || !pkeys_ || !pksl_ || !pbin_ || !pkcell_ || !pcount_ || !poff_ || !scan_tmp_) return false;
+    if (!pp_ || !pprev_ || !psrc_ || !player_ || !pcell_ || !pslot_ || !praw_ || !pkeys_ || !pksl_ || !pbin_ || !pkcell_ || !pcount_ ||
+        !poff_ || !scan_tmp_ || !coarse_ || !coarse_off_)
+        return false;
     part_cap_ = n;
     part_buckets_ = buckets;
     return true;
@@ -401,6 +455,14 @@ bool DynamicColliders::upload_view(hipStream_t st) {
     v.pksl = part_filter_ ? pksl_ : nullptr;
     v.pbin = pbin_;
     v.pkcell = pkcell_;
+    if (v.part_hash) {
+        v.coarse = coarse_;
+        v.coarse_off = coarse_off_;
+        v.coarse_buckets = coarse_buckets_;
+        v.coarse_cell = 4.0f * part_cell_;
+        v.coarse_reach = part_cell_;
+        v.max_radius = max_radius_;
+    }
     if (v.tri_hash && v.tri_buckets > tri_buckets_) return fail("dynamic triangle hash capacity");
     if (v.part_hash && v.part_buckets > part_buckets_) return fail("dynamic particle hash capacity");
     launch_k(k_set_view, dim3(1), dim3(1), 0, st, d_view_, v);
@@ -454,6 +516,7 @@ bool DynamicColliders::set_particles(hipStream_t st, const float* pos, const flo
         clean[i] = std::max(r, 0.0f);
         max_radius = std::max(max_radius, clean[i]);
     }
+    max_radius_ = max_radius;
     if (part_cell_ <= 0.0f || clean != radii_seen_ || np_ != n) {
         part_cell_ = std::max(std::max(max_radius + std::max(margin, 0.0f), std::max(thickness, 0.0f)), 1.0e-3f);
         radii_seen_ = clean;
@@ -528,6 +591,7 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
     np_ = np;
     nt_ = nt;
     part_cell_ = std::max(std::max(max_radius + margin, thickness), 1.0e-3f);
+    max_radius_ = max_radius;
     radii_seen_.clear();
     span_margin_ = margin;
     tri_cell_ = 0.0f;   // computed on the device below
@@ -590,6 +654,8 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
         launch_k(k_bucket_bins, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
                  static_cast<const int*>(poff_), static_cast<const int*>(pcount_), buckets, kDynParticleSlots,
                  static_cast<const int2*>(part_filter_ ? pksl_ : nullptr), pbin_);
+        launch_k(k_coarse_fill, dim3(grid_for(coarse_buckets_, 256)), blk, 0, st, coarse_, coarse_buckets_, coarse_off_);
+        launch_k(k_coarse_insert, g, blk, 0, st, static_cast<const DynView*>(d_view_), coarse_, coarse_off_);
     }
     return hip_ok(hipGetLastError(), "dynamic collider hash");
 }

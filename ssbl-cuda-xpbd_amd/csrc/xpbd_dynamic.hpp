@@ -65,6 +65,15 @@ struct DynView {
     const int4* pbin;        // per bucket, over its visible entries: {offset, count, min layer, one source | -1 | -2}
     const int4* pkcell;      // the key's particle cell beside each key
     const int4* pcell;
+    // Coarse particle summary: per bucket of coarse cells (4 particle cells
+    // wide), {smallest layer, the one source | -1 mixed | -2 empty} over every
+    // particle whose reach box (position +- coarse_reach) overlaps the cell.
+    // A query whose own coarse cell has no particle its filter accepts cannot
+    // find a contact and skips the particle search.
+    const int2* coarse;      // null: no summary
+    const int* coarse_off;   // device flag: a particle position was not finite, do not use the summary
+    int coarse_buckets;
+    float coarse_cell, coarse_reach, max_radius;
 };
 
 // One contributing cloth of a device-gathered scene.
@@ -140,6 +149,10 @@ private:
     int2* pksl_ = nullptr;
     int4* pbin_ = nullptr;
     int4* pkcell_ = nullptr;
+    int2* coarse_ = nullptr;
+    int* coarse_off_ = nullptr;
+    int coarse_buckets_ = 0;
+    float max_radius_ = 0.0f;
     int* scan_tmp_ = nullptr;
     GatherSource* d_sources_ = nullptr;
     int sources_cap_ = 0;

@@ -258,67 +258,174 @@ __device__ __forceinline__ bool dyn_bucket_rejected(const ClothView& c, int4 bin
     return c.dyn_mode == 2 && bin.z >= c.dyn_target_layer;
 }
 
-__device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, float dt, bool count) {
+// One dynamic triangle against point p (A41:3999-4323 per candidate): counts
+// the candidate, the AABB and bounding-sphere rejections, and returns the
+// contact depth (0 when out of reach) with the push along the contact normal.
+__device__ __forceinline__ float dyn_triangle_depth(const DynView& D, int t, F3 p, float cm, unsigned& cand, unsigned& rej,
+                                                    F3& delta) {
+    ++cand;
+    const float4 mn = D.lo[t], mx = D.hi[t];
+    const F3 q0 = f3(fminf(fmaxf(p.x, mn.x), mx.x), fminf(fmaxf(p.y, mn.y), mx.y), fminf(fmaxf(p.z, mn.z), mx.z));
+    const F3 da = p - q0;
+    if (dot(da, da) > cm * cm) { ++rej; return 0.0f; }
+    const float4 ce = D.ctr[t];
+    const float rr = ce.w + cm;
+    const F3 dc = p - f3(ce);
+    if (dot(dc, dc) > rr * rr) { ++rej; return 0.0f; }
+    const F3 a = f3(D.tv[3 * t]), b = f3(D.tv[3 * t + 1]), cc = f3(D.tv[3 * t + 2]);
+    const F3 q = closest_point_on_triangle(p, a, b, cc);
+    const F3 d = p - q;
+    const float dsq = dot(d, d);
+    const float dist = sqrtf(fmaxf(dsq, kEps));
+    const F3 tn = cross(b - a, cc - a);
+    const float tl = sqrtf(fmaxf(dot(tn, tn), kEps));
+    const F3 tnorm = mul(tn, 1.0f / tl);
+    const F3 n = dsq > kEps * 4.0f ? mul(d, 1.0f / fmaxf(dist, kEps)) : tnorm;
+    if (dist >= cm) return 0.0f;
+    const float pen = cm - dist;
+    delta = mul(n, pen);
+    return pen;
+}
+
+// One dynamic particle against point p (A41:4354-4511 per candidate).
+__device__ __forceinline__ float dyn_particle_depth(const DynView& D, int j, F3 p, float margin, unsigned& cand, F3& delta) {
+    const float4 pj = D.pp[j];
+    const float radius = fmaxf(pj.w, 0.0f) + margin;
+    if (radius <= 0.0f) return 0.0f;
+    const F3 d = p - f3(pj);
+    const float dsq = dot(d, d);
+    const float dist = sqrtf(fmaxf(dsq, kEps));
+    if (dist >= radius) return 0.0f;
+    ++cand;
+    const F3 n = dist > kEps ? mul(d, 1.0f / dist) : f3(0.0f, 0.0f, 1.0f);
+    const float pen = radius - dist;
+    delta = mul(n, pen);
+    return pen;
+}
+
+// The deepest contact of a sequence visited in order is the first one of
+// greatest depth.  Depths are positive, so (depth bits, ~order) compares as
+// one unsigned 64-bit key whose wave maximum names that contact.
+__device__ __forceinline__ unsigned long long depth_key(float pen, unsigned order) {
+    return pen > 0.0f ? (static_cast<unsigned long long>(__float_as_uint(pen)) << 32) | (0xffffffffu - order) : 0ull;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
+    for (int d = 32; d > 0; d >>= 1) {
+        const unsigned long long o = __shfl_xor(k, d, 64);
+        k = o > k ? o : k;
+    }
+    return k;
+}
+
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned x) {
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+__device__ __forceinline__ F3 shfl3(F3 a, int lane) { return f3(__shfl(a.x, lane, 64), __shfl(a.y, lane, 64), __shfl(a.z, lane, 64)); }
+
+// The wave's best key and the push of the lane that holds it.
+__device__ __forceinline__ unsigned long long wave_best(unsigned long long key, F3 delta, F3& best_delta) {
+    const unsigned long long top = wave_max_u64(key);
+    if (top != 0ull) {
+        const int owner = __ffsll(static_cast<long long>(__ballot(key == top))) - 1;
+        best_delta = shfl3(delta, owner);
+    }
+    return top;
+}
+
+// Dynamic colliders, unsigned contacts: the deepest triangle contact
+// (A41:3999-4323) then the deepest particle contact (A41:4354-4511) push the
+// vertex by half their depth (A41:129).  Only the vertex's own position is
+// read and written, so running both here equals the reference's two
+// vertex-parallel launches.
+//
+// A vertex's hash probe is a serial chain of dependent loads, and only the
+// few vertices near another collider have one to make (the rest are turned
+// away by a bucket bin or the coarse summary with one load).  So the wave
+// serves those vertices one at a time with all 64 lanes: a lane per entry of
+// the triangle bucket, a lane per cell of the 27-cell particle search.  Each
+// lane keeps the first deepest contact of its own run and the wave takes the
+// deepest with the smallest visiting order, which is the contact the serial
+// visit in the reference's order keeps.  Every lane of the wave must call
+// this (`active` false for lanes without a vertex).
+__device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, bool active, float dt, bool count) {
     const DynView& D = *c.dyn;
+    const int lane = static_cast<int>(__lane_id());
     const float cm = fmaxf(c.collision_margin, 0.0f);
     const int nt = D.nt;
     if (nt > 0 && cm > 0.0f) {
-        const F3 p = s.p;
-        F3 best_delta = f3(0.0f, 0.0f, 0.0f);
         float best_pen = 0.0f;
+        F3 best_delta = f3(0.0f, 0.0f, 0.0f);
         unsigned cand = 0, rej = 0;
-        // the source filter has already passed when `filtered` (hash entries carry {source, layer})
-        auto consider = [&](int t, bool filtered) {
-            if (t < 0 || t >= nt) return;
-            if (!filtered && D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) return;
-            ++cand;
-            const float4 mn = D.lo[t], mx = D.hi[t];
-            const F3 q0 = f3(fminf(fmaxf(p.x, mn.x), mx.x), fminf(fmaxf(p.y, mn.y), mx.y), fminf(fmaxf(p.z, mn.z), mx.z));
-            const F3 da = p - q0;
-            if (dot(da, da) > cm * cm) { ++rej; return; }
-            const float4 ce = D.ctr[t];
-            const float rr = ce.w + cm;
-            const F3 dc = p - f3(ce);
-            if (dot(dc, dc) > rr * rr) { ++rej; return; }
-            const F3 a = f3(D.tv[3 * t]), b = f3(D.tv[3 * t + 1]), cc = f3(D.tv[3 * t + 2]);
-            const F3 q = closest_point_on_triangle(p, a, b, cc);
-            const F3 d = p - q;
-            const float dsq = dot(d, d);
-            const float dist = sqrtf(fmaxf(dsq, kEps));
-            const F3 tn = cross(b - a, cc - a);
-            const float tl = sqrtf(fmaxf(dot(tn, tn), kEps));
-            const F3 tnorm = mul(tn, 1.0f / tl);
-            const F3 n = dsq > kEps * 4.0f ? mul(d, 1.0f / fmaxf(dist, kEps)) : tnorm;
-            const float pen = cm - dist;
-            if (dist >= cm) return;
-            if (pen > best_pen) { best_pen = pen; best_delta = mul(n, pen); }
-        };
         if (D.tri_hash) {   // own cell only: the unsigned query radius is 0 (A41:3623-3633)
-            const int cx = cell_coord(p.x, D.tri_cell), cy = cell_coord(p.y, D.tri_cell), cz = cell_coord(p.z, D.tri_cell);
-            const int4 bin = D.tri_bin[hash_cell(cx, cy, cz, D.tri_buckets)];
-            const int e = dyn_bucket_rejected(c, bin) ? 0 : bin.x + bin.y;
-            for (int k = bin.x; k < e; ++k) {
-                // the source filter and the canonical-cell test (A41:3934-3982) are both
-                // side-effect free, so the cheaper one (beside the key) runs first
-                if (D.tri_ksl) {
-                    const int2 sl = D.tri_ksl[k];
-                    if (!dyn_enabled(c, sl.x, sl.y)) continue;
-                }
-                const int t = D.tri_keys[k] / kDynMaxHashCells;
-                const int4 lo = D.cmin[t], hi = D.cmax[t];
-                if (cx < lo.x || cx > hi.x || cy < lo.y || cy > hi.y || cz < lo.z || cz > hi.z) continue;
-                consider(t, true);
+            int cx = 0, cy = 0, cz = 0;
+            int4 bin = make_int4(0, 0, 0, -2);
+            if (active) {
+                cx = cell_coord(s.p.x, D.tri_cell); cy = cell_coord(s.p.y, D.tri_cell); cz = cell_coord(s.p.z, D.tri_cell);
+                bin = D.tri_bin[hash_cell(cx, cy, cz, D.tri_buckets)];
+                if (dyn_bucket_rejected(c, bin)) bin.y = 0;
             }
-            const int nl = *D.nlarge;
-            for (int k = 0; k < nl; ++k) consider(D.large[k], false);
-        } else {
-            for (int t = 0; t < nt; ++t) consider(t, false);
+            unsigned long long pending = __ballot(bin.y > 0);
+            while (pending) {
+                const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+                pending &= pending - 1;
+                const F3 lp = shfl3(s.p, leader);
+                const int lx = __shfl(cx, leader, 64), ly = __shfl(cy, leader, 64), lz = __shfl(cz, leader, 64);
+                const int b0 = __shfl(bin.x, leader, 64), nb = __shfl(bin.y, leader, 64);
+                unsigned long long key = 0ull;
+                F3 delta = f3(0.0f, 0.0f, 0.0f);
+                unsigned lc = 0, lr = 0;
+                for (int e = lane; e < nb; e += 64) {
+                    const int k = b0 + e;
+                    // the source filter and the canonical-cell test (A41:3934-3982) are
+                    // both side-effect free, so the cheaper one (beside the key) runs first
+                    if (D.tri_ksl) {
+                        const int2 sl = D.tri_ksl[k];
+                        if (!dyn_enabled(c, sl.x, sl.y)) continue;
+                    }
+                    const int t = D.tri_keys[k] / kDynMaxHashCells;
+                    if (t < 0 || t >= nt) continue;
+                    const int4 lo = D.cmin[t], hi = D.cmax[t];
+                    if (lx < lo.x || lx > hi.x || ly < lo.y || ly > hi.y || lz < lo.z || lz > hi.z) continue;
+                    F3 d;
+                    const unsigned long long kk = depth_key(dyn_triangle_depth(D, t, lp, cm, lc, lr, d), static_cast<unsigned>(e));
+                    if (kk > key) { key = kk; delta = d; }
+                }
+                F3 wd = f3(0.0f, 0.0f, 0.0f);
+                const unsigned long long top = wave_best(key, delta, wd);
+                const unsigned sc = wave_sum_u32(lc), sr = wave_sum_u32(lr);
+                if (lane == leader) {
+                    cand += sc;
+                    rej += sr;
+                    if (top != 0ull) { best_pen = __uint_as_float(static_cast<unsigned>(top >> 32)); best_delta = wd; }
+                }
+            }
+            if (active) {
+                const int nl = *D.nlarge;
+                for (int k = 0; k < nl; ++k) {
+                    const int t = D.large[k];
+                    if (t < 0 || t >= nt) continue;
+                    if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) continue;
+                    F3 d;
+                    const float pen = dyn_triangle_depth(D, t, s.p, cm, cand, rej, d);
+                    if (pen > best_pen) { best_pen = pen; best_delta = d; }
+                }
+            }
+        } else if (active) {
+            for (int t = 0; t < nt; ++t) {
+                if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) continue;
+                F3 d;
+                const float pen = dyn_triangle_depth(D, t, s.p, cm, cand, rej, d);
+                if (pen > best_pen) { best_pen = pen; best_delta = d; }
+            }
         }
         if (count && cand) atomicAdd(&c.counts[kCountDynTriCandidates], static_cast<unsigned long long>(cand));
         if (count && rej) atomicAdd(&c.counts[kCountDynTriAabbRejects], static_cast<unsigned long long>(rej));
-        if (best_pen > 0.0f) {
+        if (active && best_pen > 0.0f) {
             const F3 applied = mul(best_delta, 0.5f);
-            const F3 np = p + applied;
+            const F3 np = s.p + applied;
             F3 v = s.vel();
             const float vn = dot(v, applied);
             if (vn < 0.0f) {
@@ -337,55 +444,70 @@ __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, floa
     const int np_count = D.np;
     if (np_count > 0) {
         const float margin = fmaxf(c.collision_margin, 0.0f);
-        const F3 p = s.p;
-        F3 best_delta = f3(0.0f, 0.0f, 0.0f);
         float best_pen = 0.0f;
+        F3 best_delta = f3(0.0f, 0.0f, 0.0f);
         unsigned cand = 0;
-        auto consider = [&](int j, bool filtered) {
-            if (j < 0 || j >= np_count) return;
-            if (!filtered && D.psrc && !dyn_enabled(c, D.psrc[j], D.player ? D.player[j] : 0)) return;
-            const float4 pj = D.pp[j];
-            const float radius = fmaxf(pj.w, 0.0f) + margin;
-            if (radius <= 0.0f) return;
-            const F3 d = p - f3(pj);
-            const float dsq = dot(d, d);
-            const float dist = sqrtf(fmaxf(dsq, kEps));
-            if (dist >= radius) return;
-            ++cand;
-            const F3 n = dist > kEps ? mul(d, 1.0f / dist) : f3(0.0f, 0.0f, 1.0f);
-            const float pen = radius - dist;
-            if (pen > best_pen) { best_pen = pen; best_delta = mul(n, pen); }
-        };
         if (D.part_hash) {
-            const int cx = cell_coord(p.x, D.part_cell), cy = cell_coord(p.y, D.part_cell), cz = cell_coord(p.z, D.part_cell);
-            // one bin per bucket; the bins of a z plane are loaded together, then
-            // visited in the reference's order (z, then y, then x innermost)
-            for (int dz = -1; dz <= 1; ++dz) {
-                int4 bins[9];
-#pragma unroll
-                for (int k = 0; k < 9; ++k) bins[k] = D.pbin[hash_cell(cx + k % 3 - 1, cy + k / 3 - 1, cz + dz, D.part_buckets)];
-#pragma unroll
-                for (int k9 = 0; k9 < 9; ++k9) {
-                    if (dyn_bucket_rejected(c, bins[k9])) continue;
-                    const int qx = cx + k9 % 3 - 1, qy = cy + k9 / 3 - 1, qz = cz + dz;
-                    const int base = bins[k9].x, lim = min(bins[k9].y, kDynParticleSlots);
-                    for (int k = 0; k < lim; ++k) {
-                        const int4 pc = D.pkcell[base + k];
-                        if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
-                        if (D.pksl) {
-                            const int2 sl = D.pksl[base + k];
-                            if (!dyn_enabled(c, sl.x, sl.y)) continue;
+            // the coarse summary holds for this query when its reach box covers
+            // the query's reach (the largest radius plus this cloth's margin)
+            bool probe = active;
+            if (active && D.coarse && D.max_radius + margin <= D.coarse_reach && finite3(s.p) && *D.coarse_off == 0) {
+                const float cc = D.coarse_cell;
+                const int2 e = D.coarse[hash_cell(cell_coord(s.p.x, cc), cell_coord(s.p.y, cc), cell_coord(s.p.z, cc), D.coarse_buckets)];
+                probe = !(e.y == -2 || (c.dyn_mode != 0 && ((c.dyn_target_src != 0 && e.y == c.dyn_target_src) ||
+                                                            (c.dyn_mode == 2 && e.x >= c.dyn_target_layer))));
+            }
+            unsigned long long pending = __ballot(probe);
+            while (pending) {
+                const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+                pending &= pending - 1;
+                const F3 lp = shfl3(s.p, leader);
+                unsigned long long key = 0ull;
+                F3 delta = f3(0.0f, 0.0f, 0.0f);
+                unsigned lc = 0;
+                if (lane < 27) {
+                    // lane = 9 (dz + 1) + 3 (dy + 1) + (dx + 1): the reference's order, x innermost
+                    const int qx = cell_coord(lp.x, D.part_cell) + lane % 3 - 1;
+                    const int qy = cell_coord(lp.y, D.part_cell) + (lane / 3) % 3 - 1;
+                    const int qz = cell_coord(lp.z, D.part_cell) + lane / 9 - 1;
+                    const int4 bin = D.pbin[hash_cell(qx, qy, qz, D.part_buckets)];
+                    if (!dyn_bucket_rejected(c, bin)) {
+                        const int lim = min(bin.y, kDynParticleSlots);
+                        for (int k = 0; k < lim; ++k) {
+                            const int4 pc = D.pkcell[bin.x + k];
+                            if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
+                            if (D.pksl) {
+                                const int2 sl = D.pksl[bin.x + k];
+                                if (!dyn_enabled(c, sl.x, sl.y)) continue;
+                            }
+                            const int j = D.pkeys[bin.x + k];
+                            if (j < 0 || j >= np_count) continue;
+                            F3 d;
+                            const unsigned long long kk =
+                                depth_key(dyn_particle_depth(D, j, lp, margin, lc, d), static_cast<unsigned>(lane * kDynParticleSlots + k));
+                            if (kk > key) { key = kk; delta = d; }
                         }
-                        consider(D.pkeys[base + k], true);
                     }
                 }
+                F3 wd = f3(0.0f, 0.0f, 0.0f);
+                const unsigned long long top = wave_best(key, delta, wd);
+                const unsigned sc = wave_sum_u32(lc);
+                if (lane == leader) {
+                    cand += sc;
+                    if (top != 0ull) { best_pen = __uint_as_float(static_cast<unsigned>(top >> 32)); best_delta = wd; }
+                }
             }
-        } else {
-            for (int j = 0; j < np_count; ++j) consider(j, false);
+        } else if (active) {
+            for (int j = 0; j < np_count; ++j) {
+                if (D.psrc && !dyn_enabled(c, D.psrc[j], D.player ? D.player[j] : 0)) continue;
+                F3 d;
+                const float pen = dyn_particle_depth(D, j, s.p, margin, cand, d);
+                if (pen > best_pen) { best_pen = pen; best_delta = d; }
+            }
         }
         if (count && cand) atomicAdd(&c.counts[kCountDynParticleCandidates], static_cast<unsigned long long>(cand));
-        if (best_pen > 0.0f) {
-            s.p = p + mul(best_delta, 0.5f);
+        if (active && best_pen > 0.0f) {
+            s.p = s.p + mul(best_delta, 0.5f);
             if (count) {
                 atomicAdd(&c.counts[kCountExactImpulse], 1ull);
                 atomicAdd(&c.counts[kCountDynParticleContacts], 1ull);
@@ -671,15 +793,18 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
 __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
                                                         int ntiles, int parity, unsigned ops) {
     const Tile tile = tiles[xcd_tile(ntiles)];
-    const int v = tile.begin + static_cast<int>(threadIdx.x);
-    if (v >= tile.end) return;
     const ClothView& c = views[tile.cloth];
-    if (!c.dyn) return;
+    if (!c.dyn) return;   // uniform over the block
+    const int v = tile.begin + static_cast<int>(threadIdx.x);
+    // every lane stays for the wave-wide probes; lanes past the tile or on a
+    // pinned vertex take part without a vertex of their own
     float4* pos = parity ? c.pos1 : c.pos0;
-    const float4 p4 = pos[v];
-    if (p4.w == 0.0f) return;
+    const bool in_tile = v < tile.end;
+    const float4 p4 = in_tile ? pos[v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool active = in_tile && p4.w != 0.0f;
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
-    dynamic_contacts(c, s, c.dt_sub, (ops & kOpCount) != 0u);
+    dynamic_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
+    if (!active) return;
     if (s.p.x != p4.x || s.p.y != p4.y || s.p.z != p4.z) pos[v] = with_w(s.p, p4.w);
     if (s.q_dirty) c.prev[v] = with_w(s.q_, 0.0f);
     if (s.v_dirty) c.vel[v] = with_w(s.vel_, 0.0f);
