@@ -13,11 +13,15 @@ an optional moving sphere collider, and a number of frames to run. The server:
   Sessions that arrive while a tick runs join at the next tick. Finished
   sessions leave, and their memory frees slots for the queue;
 * records each session's per-frame latency (from the start of the tick's
-  input updates to the end of its step) and the aggregate vertex-substeps/s.
+  input updates to the end of its step) and the aggregate vertex-substeps/s;
+* writes a PC2 cache of every session that asks for one (``cache_path``):
+  the start positions, then one sample per frame, committed when the session
+  ends (the reference's realtime auto-cache, session_manager.py:4650-4728).
 
 The per-frame order follows the reference's session manager
 (session_manager.py:2520-2591): frame inputs of every slot, then the steps,
-then downloads (here only at a session's end).
+then downloads (every frame for cached sessions, otherwise at a session's
+end).
 
 Batched stepping gives each handle the same result as stepping it alone
 (ssbl_ext_step_batch), so a session's result does not depend on the mix it
@@ -33,6 +37,7 @@ from functools import lru_cache
 import numpy as np
 
 from . import native_backend as nb
+from .pc2 import Pc2Writer
 from .scenes import ClothBuildData, SolverOptions, build_cloth, grid_mesh, y_up_to_z_up
 
 
@@ -60,6 +65,8 @@ class SessionRequest:
     frames: int
     wind: dict | None = None            # SsblXpbdForceField fields of a wind field (type 1)
     sphere: SpherePath | None = None
+    cache_path: str | None = None       # PC2 file of the session's frames
+    start_frame: int = 1
 
     @property
     def vertex_count(self) -> int:
@@ -122,6 +129,7 @@ class SessionResult:
     admitted: float
     finished: float
     device_bytes: int
+    cache_path: str | None = None
 
 
 @dataclass
@@ -132,6 +140,7 @@ class _Active:
     admitted: float
     frame: int = 0
     frame_ms: list = field(default_factory=list)
+    cache: Pc2Writer | None = None
 
 
 @dataclass
@@ -208,7 +217,12 @@ class ClothServer:
             held = int(self.lib.ssbl_ext_device_bytes(solver.handle))
             if held < 0:                                # library without device accounting
                 held = estimate_device_bytes(req.cloth, req.options)
-            self.active.append(_Active(req, solver, held, now))
+            act = _Active(req, solver, held, now)
+            if req.cache_path is not None:
+                start = req.cloth.start_positions if req.cloth.start_positions is not None else req.cloth.positions_world
+                act.cache = Pc2Writer(req.cache_path, req.vertex_count, req.start_frame)
+                act.cache.add_sample(start)
+            self.active.append(act)
             self.used_bytes += held
             self.active_vertices += req.vertex_count
             self._batch_stale = True
@@ -244,6 +258,8 @@ class ClothServer:
         self.vertex_substeps += self.active_vertices * self.substeps
         done = []
         for s in self.active:
+            if s.cache is not None:
+                s.cache.add_sample(s.solver.download_positions())
             s.frame += 1
             s.frame_ms.append(ms)
             if s.frame >= s.req.frames:
@@ -258,18 +274,21 @@ class ClothServer:
 
     def _retire(self, s: _Active, now: float) -> None:
         pos = s.solver.download_positions()
+        path = s.cache.commit() if s.cache is not None else None
         s.solver.close()
         self.active.remove(s)
         self.used_bytes -= s.device_bytes
         self.active_vertices -= s.req.vertex_count
         self.results[s.req.sid] = SessionResult(s.req.sid, s.req.vertex_count, s.frame, pos, s.frame_ms, s.req.arrival,
-                                                s.admitted, now, s.device_bytes)
+                                                s.admitted, now, s.device_bytes, path)
 
     def close(self) -> None:
         if self._batch is not None:
             self._batch.close()
             self._batch = None
         for s in list(self.active):
+            if s.cache is not None:       # unfinished: the reference drops an uncommitted cache
+                s.cache.discard()
             s.solver.close()
         self.active.clear()
 
