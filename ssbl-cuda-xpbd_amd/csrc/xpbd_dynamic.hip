@@ -511,12 +511,13 @@ bool DynamicColliders::set_particles(hipStream_t st, const float* pos, const flo
     if (n == 0) { np_ = 0; return d_view_ ? upload_view(st) : true; }
     std::vector<float> clean(n);
     float max_radius = 0.0f;
+    max_radius_ = 0.0f;   // the largest radius a query sees, max(r, 0) of the raw value (inf stays inf)
     for (int i = 0; i < n; ++i) {
         const float r = std::isfinite(radii[i]) ? radii[i] : 0.0f;
         clean[i] = std::max(r, 0.0f);
         max_radius = std::max(max_radius, clean[i]);
+        if (radii[i] > max_radius_) max_radius_ = radii[i];
     }
-    max_radius_ = max_radius;
     if (part_cell_ <= 0.0f || clean != radii_seen_ || np_ != n) {
         part_cell_ = std::max(std::max(max_radius + std::max(margin, 0.0f), std::max(thickness, 0.0f)), 1.0e-3f);
         radii_seen_ = clean;
@@ -552,7 +553,7 @@ bool DynamicColliders::set_particles(hipStream_t st, const float* pos, const flo
 bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& sources_in, float margin, float thickness) {
     std::vector<GatherSource> sources = sources_in;
     int np = 0, nt = 0;
-    float max_radius = 0.0f;
+    float max_radius = 0.0f, reach_radius = 0.0f;
     for (GatherSource& s : sources) {
         s.pbase = np;
         s.tbase = nt;
@@ -560,6 +561,7 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
         nt += std::max(s.nt, 0);
         const float r = std::isfinite(s.radius) ? s.radius : 0.0f;
         max_radius = std::max(max_radius, std::max(r, 0.0f));
+        if (s.radius > reach_radius) reach_radius = s.radius;
     }
     margin = std::max(margin, 0.0f);
     thickness = std::max(thickness, 0.0f);
@@ -591,7 +593,7 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
     np_ = np;
     nt_ = nt;
     part_cell_ = std::max(std::max(max_radius + margin, thickness), 1.0e-3f);
-    max_radius_ = max_radius;
+    max_radius_ = reach_radius;
     radii_seen_.clear();
     span_margin_ = margin;
     tri_cell_ = 0.0f;   // computed on the device below

@@ -303,6 +303,18 @@ __device__ __forceinline__ float dyn_particle_depth(const DynView& D, int j, F3 
     return pen;
 }
 
+// False when no point of neighbour cell (cx+dx, cy+dy, cz+dz) of p's cell
+// lies within `reach` of p (with slack for the rounding of the cell bounds
+// and of the distance itself): particles there can neither touch p nor count
+// as candidates, so the search may pass the cell by.
+__device__ __forceinline__ bool cell_in_reach(F3 p, int cx, int cy, int cz, int dx, int dy, int dz, float cs, float reach) {
+    const float gx = dx < 0 ? p.x - static_cast<float>(cx) * cs : (dx > 0 ? static_cast<float>(cx + 1) * cs - p.x : 0.0f);
+    const float gy = dy < 0 ? p.y - static_cast<float>(cy) * cs : (dy > 0 ? static_cast<float>(cy + 1) * cs - p.y : 0.0f);
+    const float gz = dz < 0 ? p.z - static_cast<float>(cz) * cs : (dz > 0 ? static_cast<float>(cz + 1) * cs - p.z : 0.0f);
+    const float r = reach * 1.001f + (fabsf(p.x) + fabsf(p.y) + fabsf(p.z)) * 1.0e-6f + 1.0e-7f;
+    return !(gx * gx + gy * gy + gz * gz >= r * r);
+}
+
 // The deepest contact of a sequence visited in order is the first one of
 // greatest depth.  Depths are positive, so (depth bits, ~order) compares as
 // one unsigned 64-bit key whose wave maximum names that contact.
@@ -341,16 +353,26 @@ __device__ __forceinline__ unsigned long long wave_best(unsigned long long key, 
 // read and written, so running both here equals the reference's two
 // vertex-parallel launches.
 //
-// A vertex's hash probe is a serial chain of dependent loads, and only the
-// few vertices near another collider have one to make (the rest are turned
-// away by a bucket bin or the coarse summary with one load).  So the wave
-// serves those vertices one at a time with all 64 lanes: a lane per entry of
-// the triangle bucket, a lane per cell of the 27-cell particle search.  Each
-// lane keeps the first deepest contact of its own run and the wave takes the
+// A vertex's hash probe is a serial chain of dependent loads, and often only
+// a few vertices of a wave have one to make (the rest are turned away by a
+// bucket bin or the coarse summary with one load).  A wave with at most a few
+// probes serves them one at a time with all 64 lanes: a lane per entry of the
+// triangle bucket, a lane per cell of the 27-cell particle search.  Each lane
+// keeps the first deepest contact of its own run and the wave takes the
 // deepest with the smallest visiting order, which is the contact the serial
-// visit in the reference's order keeps.  Every lane of the wave must call
-// this (`active` false for lanes without a vertex).
-__device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, bool active, float dt, bool count) {
+// visit in the reference's order keeps.  A wave with more probes (cloths in
+// contact) lets every lane walk its own, in that order.  Both forms give the
+// same result.  Every lane of the wave must call this (`active` false for
+// lanes without a vertex).
+#ifndef SSBL_COOP_TRIANGLE_PROBES
+#define SSBL_COOP_TRIANGLE_PROBES 32
+#endif
+#ifndef SSBL_COOP_PARTICLE_PROBES
+#define SSBL_COOP_PARTICLE_PROBES 16
+#endif
+constexpr int kCoopTriangleProbes = SSBL_COOP_TRIANGLE_PROBES;
+constexpr int kCoopParticleProbes = SSBL_COOP_PARTICLE_PROBES;
+__device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState& s, bool active, float dt, bool count) {
     const DynView& D = *c.dyn;
     const int lane = static_cast<int>(__lane_id());
     const float cm = fmaxf(c.collision_margin, 0.0f);
@@ -368,6 +390,23 @@ __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, bool
                 if (dyn_bucket_rejected(c, bin)) bin.y = 0;
             }
             unsigned long long pending = __ballot(bin.y > 0);
+            if (__popcll(pending) > kCoopTriangleProbes) {
+                // many probes in this wave: each lane walks its own bucket
+                pending = 0ull;
+                for (int k = bin.x; k < bin.x + bin.y; ++k) {
+                    if (D.tri_ksl) {
+                        const int2 sl = D.tri_ksl[k];
+                        if (!dyn_enabled(c, sl.x, sl.y)) continue;
+                    }
+                    const int t = D.tri_keys[k] / kDynMaxHashCells;
+                    if (t < 0 || t >= nt) continue;
+                    const int4 lo = D.cmin[t], hi = D.cmax[t];
+                    if (cx < lo.x || cx > hi.x || cy < lo.y || cy > hi.y || cz < lo.z || cz > hi.z) continue;
+                    F3 d;
+                    const float pen = dyn_triangle_depth(D, t, s.p, cm, cand, rej, d);
+                    if (pen > best_pen) { best_pen = pen; best_delta = d; }
+                }
+            }
             while (pending) {
                 const int leader = __ffsll(static_cast<long long>(pending)) - 1;
                 pending &= pending - 1;
@@ -441,9 +480,15 @@ __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, bool
             }
         }
     }
+}
+
+__device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState& s, bool active, bool count) {
+    const DynView& D = *c.dyn;
+    const int lane = static_cast<int>(__lane_id());
     const int np_count = D.np;
     if (np_count > 0) {
         const float margin = fmaxf(c.collision_margin, 0.0f);
+        const float reach = D.max_radius + margin;   // no particle reaches further
         float best_pen = 0.0f;
         F3 best_delta = f3(0.0f, 0.0f, 0.0f);
         unsigned cand = 0;
@@ -458,6 +503,42 @@ __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, bool
                                                             (c.dyn_mode == 2 && e.x >= c.dyn_target_layer))));
             }
             unsigned long long pending = __ballot(probe);
+            if (__popcll(pending) > kCoopParticleProbes) {
+                // many probes in this wave: each lane searches its own 27 cells
+                pending = 0ull;
+                if (probe) {
+                    const int cx = cell_coord(s.p.x, D.part_cell), cy = cell_coord(s.p.y, D.part_cell), cz = cell_coord(s.p.z, D.part_cell);
+                    // the bins of a z plane are loaded together, then visited in
+                    // the reference's order (z, then y, then x innermost)
+                    for (int dz = -1; dz <= 1; ++dz) {
+                        int4 bins[9];
+#pragma unroll
+                        for (int k = 0; k < 9; ++k)
+                            bins[k] = cell_in_reach(s.p, cx, cy, cz, k % 3 - 1, k / 3 - 1, dz, D.part_cell, reach)
+                                          ? D.pbin[hash_cell(cx + k % 3 - 1, cy + k / 3 - 1, cz + dz, D.part_buckets)]
+                                          : make_int4(0, 0, 0, -2);
+#pragma unroll
+                        for (int k9 = 0; k9 < 9; ++k9) {
+                            if (dyn_bucket_rejected(c, bins[k9])) continue;
+                            const int qx = cx + k9 % 3 - 1, qy = cy + k9 / 3 - 1, qz = cz + dz;
+                            const int base = bins[k9].x, lim = min(bins[k9].y, kDynParticleSlots);
+                            for (int k = 0; k < lim; ++k) {
+                                const int4 pc = D.pkcell[base + k];
+                                if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
+                                if (D.pksl) {
+                                    const int2 sl = D.pksl[base + k];
+                                    if (!dyn_enabled(c, sl.x, sl.y)) continue;
+                                }
+                                const int j = D.pkeys[base + k];
+                                if (j < 0 || j >= np_count) continue;
+                                F3 d;
+                                const float pen = dyn_particle_depth(D, j, s.p, margin, cand, d);
+                                if (pen > best_pen) { best_pen = pen; best_delta = d; }
+                            }
+                        }
+                    }
+                }
+            }
             while (pending) {
                 const int leader = __ffsll(static_cast<long long>(pending)) - 1;
                 pending &= pending - 1;
@@ -465,11 +546,12 @@ __device__ inline void dynamic_contacts(const ClothView& c, VertexState& s, bool
                 unsigned long long key = 0ull;
                 F3 delta = f3(0.0f, 0.0f, 0.0f);
                 unsigned lc = 0;
-                if (lane < 27) {
-                    // lane = 9 (dz + 1) + 3 (dy + 1) + (dx + 1): the reference's order, x innermost
-                    const int qx = cell_coord(lp.x, D.part_cell) + lane % 3 - 1;
-                    const int qy = cell_coord(lp.y, D.part_cell) + (lane / 3) % 3 - 1;
-                    const int qz = cell_coord(lp.z, D.part_cell) + lane / 9 - 1;
+                // lane = 9 (dz + 1) + 3 (dy + 1) + (dx + 1): the reference's order, x innermost
+                const int lcx = cell_coord(lp.x, D.part_cell), lcy = cell_coord(lp.y, D.part_cell), lcz = cell_coord(lp.z, D.part_cell);
+                if (lane < 27 && cell_in_reach(lp, lcx, lcy, lcz, lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1, D.part_cell, reach)) {
+                    const int qx = lcx + lane % 3 - 1;
+                    const int qy = lcy + (lane / 3) % 3 - 1;
+                    const int qz = lcz + lane / 9 - 1;
                     const int4 bin = D.pbin[hash_cell(qx, qy, qz, D.part_buckets)];
                     if (!dyn_bucket_rejected(c, bin)) {
                         const int lim = min(bin.y, kDynParticleSlots);
@@ -788,8 +870,11 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
     c.vel[v] = with_w(s.vel_, 0.0f);
 }
 
-// Dynamic collider contacts of the last iteration (A41:9230-9252) as a pass
-// of their own: the hash probes want registers the constraint passes do not.
+// Dynamic collider contacts of the last iteration (A41:9230-9252) as passes
+// of their own (the hash probes want registers the constraint passes do
+// not): triangles (kind 0), then particles (kind 1), as the reference's two
+// launches.
+template <int kKind>
 __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
                                                         int ntiles, int parity, unsigned ops) {
     const Tile tile = tiles[xcd_tile(ntiles)];
@@ -803,7 +888,8 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     const float4 p4 = in_tile ? pos[v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const bool active = in_tile && p4.w != 0.0f;
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
-    dynamic_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
+    if (kKind == 0) dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
+    else dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u);
     if (!active) return;
     if (s.p.x != p4.x || s.p.y != p4.y || s.p.z != p4.z) pos[v] = with_w(s.p, p4.w);
     if (s.q_dirty) c.prev[v] = with_w(s.q_, 0.0f);
@@ -1169,7 +1255,8 @@ void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsi
 
 void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops) {
     TimedLaunch t("dynamic_contacts", c.stream);
-    launch_k(k_dynamic_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
+    launch_k(k_dynamic_pass<0>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
+    launch_k(k_dynamic_pass<1>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
 }
 
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops) {
