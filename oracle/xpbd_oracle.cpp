@@ -385,6 +385,19 @@ constexpr int kTriangleHashMinCount = 64;        // A41:39
 constexpr int kMinTriangleBuckets = 1024;        // A41:40
 constexpr int kDynamicMaxHashCells = 64;         // A41:79
 constexpr float kNeighborImpulseScale = 0.5f;    // A41:129
+// signed (indexed) dynamic triangles, A41:80-83, 130-137
+constexpr int kSignedInsidePasses = 6;
+constexpr float kSignedQueryMargin = 0.04f;
+constexpr float kSignedMaxQueryMargin = 0.08f;
+constexpr float kSignedInsideTargetMargin = -0.001f;
+constexpr float kSignedRelaxation = 1.0f / 3.0f;
+constexpr float kSignedOutwardRetention = 1.0f / 8.0f;
+constexpr float kSignedContactMarginScale = 0.75f;
+constexpr float kSignedInsideRelaxation = 1.0f / 3.0f;
+constexpr float kSignedInsideMaxDeltaScale = 16.0f;
+constexpr float kSignedInsideMaxDeltaFloor = 3.0e-3f;
+constexpr float kDynamicMaxDeltaScale = 3.0f;
+constexpr float kDynamicMaxDeltaFloor = 5.0e-4f;
 constexpr int kCrossOff = 0, kCrossLower = 2;    // A41:167-169
 enum DynCountSlot {
     kCountDynParticleCandidates = 4, kCountDynParticleContacts = 5, kCountDynParticleOverflow = 6, kCountTrianglePairs = 7,
@@ -430,6 +443,10 @@ struct DynamicSet {
     int part_buckets = 0;
     std::vector<int> pcell;                  // 3P
     std::vector<int> pcount, pidx;           // buckets, buckets * 16
+    // signed (indexed) triangles: A41:8433-8590
+    bool signed_set = false;
+    bool tri_motion = false, part_motion = false;   // previous centres / positions valid for one step
+    std::vector<float> prev_ctr;             // 3T
 
     // A41:8338-8420 (soup) and 6692-6722 (scene): cell size from the largest extent
     void set_triangles(const float* t, int n, const int* src, const int* layer, float margin, float thickness, bool fine) {
@@ -449,6 +466,65 @@ struct DynamicSet {
         }
         tri_cell = dynamic_triangle_cell_size(max_extent, std::max(margin, 0.0f), std::max(thickness, 0.0f), fine);
         span_margin = std::max(margin, 0.0f);
+        signed_set = false;
+        tri_motion = false;
+        compute_bounds();
+    }
+    // A41:3694-3718: bounds, centre and bounding radius of every triangle
+    void compute_bounds() {
+        lo.assign(3 * static_cast<size_t>(nt), 0.0f);
+        hi = lo;
+        ctr = lo;
+        rad.assign(nt, 0.0f);
+        for (int k = 0; k < nt; ++k) {
+            const float* T = &tv[9 * static_cast<size_t>(k)];
+            const V3<float> a = mk(T[0], T[1], T[2]), b = mk(T[3], T[4], T[5]), c = mk(T[6], T[7], T[8]);
+            const V3<float> cen = mul((a + b) + c, 1.0f / 3.0f);
+            lo[3 * k] = std::min(a.x, std::min(b.x, c.x)); lo[3 * k + 1] = std::min(a.y, std::min(b.y, c.y));
+            lo[3 * k + 2] = std::min(a.z, std::min(b.z, c.z));
+            hi[3 * k] = std::max(a.x, std::max(b.x, c.x)); hi[3 * k + 1] = std::max(a.y, std::max(b.y, c.y));
+            hi[3 * k + 2] = std::max(a.z, std::max(b.z, c.z));
+            ctr[3 * k] = cen.x; ctr[3 * k + 1] = cen.y; ctr[3 * k + 2] = cen.z;
+            rad[k] = std::sqrt(std::max(std::max(dot(a - cen, a - cen), dot(b - cen, b - cen)), dot(c - cen, c - cen)));
+        }
+    }
+    // A41:8433-8590: corners gathered from indexed vertices (triangles with an
+    // index out of range collapse to the origin), the fine cell from the
+    // largest extent, and the centres of the previous upload kept as the
+    // previous centres when the topology and count are unchanged.
+    void set_indexed(const float* verts, int nverts, const int* idx, int ntri, bool topology_changed, float margin, float thickness) {
+        nverts = std::max(nverts, 0);
+        ntri = std::max(ntri, 0);
+        if (nverts == 0 || ntri == 0) {
+            clear_triangles();
+            return;
+        }
+        float max_extent = 0.0f;
+        std::vector<float> soup(9 * static_cast<size_t>(ntri), 0.0f);
+        for (int t = 0; t < ntri; ++t) {
+            const int ia = idx[3 * t], ib = idx[3 * t + 1], ic = idx[3 * t + 2];
+            if (ia < 0 || ib < 0 || ic < 0 || ia >= nverts || ib >= nverts || ic >= nverts) continue;
+            const int id[3] = {ia, ib, ic};
+            for (int j = 0; j < 3; ++j)
+                for (int a = 0; a < 3; ++a) soup[9 * static_cast<size_t>(t) + 3 * j + a] = verts[3 * id[j] + a];
+            const float* T = &soup[9 * static_cast<size_t>(t)];
+            const float ext = std::max(std::max(T[0], std::max(T[3], T[6])) - std::min(T[0], std::min(T[3], T[6])),
+                                       std::max(std::max(T[1], std::max(T[4], T[7])) - std::min(T[1], std::min(T[4], T[7])),
+                                                std::max(T[2], std::max(T[5], T[8])) - std::min(T[2], std::min(T[5], T[8]))));
+            if (std::isfinite(ext) && ext >= 0.0f) max_extent = std::max(max_extent, ext);
+        }
+        const bool had_previous = !topology_changed && nt == ntri && ctr.size() == 3 * static_cast<size_t>(ntri);
+        const std::vector<float> old_ctr = ctr;
+        nt = ntri;
+        tv = soup;
+        tsrc.clear();
+        tlayer.clear();
+        tri_cell = dynamic_triangle_cell_size(max_extent, std::max(margin, 0.0f), std::max(thickness, 0.0f), true);
+        span_margin = std::max(margin, 0.0f);
+        compute_bounds();
+        prev_ctr = had_previous ? old_ctr : ctr;
+        tri_motion = had_previous;
+        signed_set = true;
     }
     // A41:8575-8697 (per cloth) and 6620-6668 (scene)
     void set_particles(const float* p, const float* prev, const float* r, const int* src, const int* layer, int n, float margin,
@@ -467,6 +543,7 @@ struct DynamicSet {
         }
         const bool had_previous = np == n && !pp.empty();
         std::vector<float> old = pp;
+        part_motion = had_previous;   // A41:8625-8690
         np = n;
         pp.assign(p, p + 3 * static_cast<size_t>(n));
         if (prev) pprev.assign(prev, prev + 3 * static_cast<size_t>(n));
@@ -477,8 +554,11 @@ struct DynamicSet {
         if (src) psrc.assign(src, src + n);
         if (layer) player.assign(layer, layer + n);
     }
-    void clear_triangles() { nt = 0; tv.clear(); tsrc.clear(); tlayer.clear(); tri_hash = false; large.clear(); max_occupancy = 0; }
-    void clear_particles() { np = 0; pp.clear(); pprev.clear(); pr.clear(); psrc.clear(); player.clear(); part_hash = false; }
+    void clear_triangles() {
+        nt = 0; tv.clear(); tsrc.clear(); tlayer.clear(); tri_hash = false; large.clear(); max_occupancy = 0;
+        ctr.clear(); signed_set = false; tri_motion = false;
+    }
+    void clear_particles() { np = 0; pp.clear(); pprev.clear(); pr.clear(); psrc.clear(); player.clear(); part_hash = false; part_motion = false; }
 
     // A41:3694-3760 bounds, 3762-3932 hash (compact), 4325-4352 particle hash
     void build(unsigned long long* counts) {
@@ -652,6 +732,8 @@ template <class R> struct Solver {
     template <class H> void island_aggregate(const H& TH, R cs);
     void self_accumulate(int v, V3<R> d);
     void dynamic_contacts(R dt);
+    void signed_contacts(R dt, bool inside);
+    void signed_response(int i, V3<R> orig, V3<R> raw, V3<R> cvel, R proxy_depth, R dt, bool inside);
     bool contact_response(Vec n, R proxy, Vec* vel_io);
     // helpers
     Vec normal_at(int v, const std::vector<Vec>& P, int* has) const;
@@ -2500,6 +2582,232 @@ template <class R> void Solver<R>::dynamic_contacts(R dt) {
     }
 }
 
+// A41:995-1068: the signed contact response.  The push is a third of the
+// (locally averaged) contact delta, clamped; the velocity keeps the
+// collider's normal speed, loses the inward part, takes the static-contact
+// friction and keeps an eighth of any outward speed.  The inside-only passes
+// just remove the inward part.
+template <class R>
+void Solver<R>::signed_response(int i, V3<R> orig, V3<R> raw, V3<R> cvel, R proxy_depth, R dt, bool inside) {
+    const R eps = R(kEpsF);
+    if (i < 0 || i >= V() || !finite(orig) || !finite(raw)) return;
+    const R raw_len = length(raw);
+    if (!std::isfinite(raw_len) || raw_len <= eps) return;
+    const V3<R> n = mul(raw, R(1) / raw_len);
+    V3<R> applied = mul(raw, R(inside ? kSignedInsideRelaxation : kSignedRelaxation));
+    {   // A41:951-968
+        const R basis = std::max(std::max(R(cfg.cloth_thickness), R(0)), std::max(R(cfg.collision_margin), R(0)));
+        const R max_len = inside ? std::max(R(kSignedInsideMaxDeltaFloor), basis * R(kSignedInsideMaxDeltaScale))
+                                 : std::max(R(kDynamicMaxDeltaFloor), basis * R(kDynamicMaxDeltaScale));
+        if (!finite(applied) || !std::isfinite(max_len) || max_len <= R(0)) applied = mk<R>(0, 0, 0);
+        else {
+            const R len = length(applied);
+            if (!std::isfinite(len) || len <= eps) applied = mk<R>(0, 0, 0);
+            else if (len > max_len) applied = mul(applied, max_len / len);
+        }
+    }
+    const R applied_len = length(applied);
+    if (!std::isfinite(applied_len) || applied_len <= eps) return;
+    const R sdt = std::max(dt, eps);
+    const V3<R> corrected = orig + applied;
+    V3<R> v = vel[i];
+    V3<R> base = mk<R>(0, 0, 0);
+    if (inside) v = finite(v) ? v : mk<R>(0, 0, 0);
+    if (finite(cvel)) {
+        const R cn = dot(cvel, n);
+        if (std::isfinite(cn)) base = mul(n, cn);
+    }
+    V3<R> rel = v - base;
+    if (inside) {
+        const R ns = dot(rel, n);
+        if (std::isfinite(ns) && ns < R(0)) {
+            rel = rel - mul(n, ns);
+            v = base + rel;
+        }
+    } else {
+        const R inward = std::max(-dot(rel, n), R(0));
+        const R proxy = std::max(std::max(proxy_depth, raw_len), applied_len);
+        contact_response(n, inward + proxy / sdt, &rel);
+        const R outward = dot(rel, n);
+        if (outward > R(0)) rel = rel - mul(n, outward * (R(1) - R(kSignedOutwardRetention)));
+        v = base + rel;
+    }
+    pos[i] = corrected;
+    vel[i] = finite(v) ? v : mk<R>(0, 0, 0);
+    prev[i] = corrected - mul(vel[i], sdt);
+}
+
+// A41:3600-3633, 3999-4323, 4354-4511, 4520-4540: contacts with the cloth's
+// own indexed (signed) triangles and its particles.  The triangle query
+// visits every hash cell within the query radius (one cell, or up to four
+// for the inside-only passes) and takes each triangle at its canonical cell.
+// The contact kept is the nearest triangle (the first of equal distance)
+// whose signed distance is under the contact margin (or, inside-only, under
+// 1 mm behind the face); its push direction is the depth-weighted mean of all
+// contacts scaled to its own depth.  The particle part (not inside-only) keeps
+// the deepest particle the same way.
+template <class R> void Solver<R>::signed_contacts(R dt, bool inside) {
+    const DynamicSet& D = dyn;
+    const R eps = R(kEpsF);
+    const R cdt = (std::isfinite(cfg.dt) && cfg.dt > kEpsF) ? R(cfg.dt) : std::max(dt, eps);   // A41:4513-4518
+    auto local_average = [&](V3<R> acc, R w, V3<R> fallback) {   // A41:978-993
+        if (w > R(0) && finite(acc)) {
+            const V3<R> avg = mul(acc, R(1) / w);
+            const R al = length(avg), fl = length(fallback);
+            if (finite(avg) && std::isfinite(al) && std::isfinite(fl) && al > eps && fl > eps) return mul(avg, fl / al);
+        }
+        return fallback;
+    };
+    const R cm = std::max(std::max(R(cfg.collision_margin), R(0)), std::max(R(cfg.cloth_thickness), R(0)) * R(kSignedContactMarginScale));
+    const R qm = std::min(std::max(std::max(std::max(cm * R(2), R(kSignedQueryMargin)), std::max(R(D.tri_cell), R(0)) * R(1.25)), cm),
+                          R(kSignedMaxQueryMargin));
+    if (D.nt > 0 && cm > R(0) && qm > R(0)) {
+        const R cs = std::max(R(D.tri_cell), R(1.0e-3));
+        const int radius = inside ? std::min(std::max(static_cast<int>(std::ceil(std::max(qm, R(0)) / cs)), 1), 4) : 1;
+        for (int i = 0; i < V(); ++i) {
+            if (w_zero(i)) continue;
+            const V3<R> p = pos[i];
+            V3<R> best_delta = mk<R>(0, 0, 0), best_vel = mk<R>(0, 0, 0), acc = mk<R>(0, 0, 0);
+            R best_pen = R(0), best_dist = R(3.402823466e+38f), accw = R(0);
+            unsigned long long cand = 0, rej = 0;
+            auto consider = [&](int t) {   // A41:3999-4113, signed branch
+                if (t < 0 || t >= D.nt) return;
+                ++cand;
+                const V3<R> mn = mk<R>(D.lo[3 * t], D.lo[3 * t + 1], D.lo[3 * t + 2]);
+                const V3<R> mx = mk<R>(D.hi[3 * t], D.hi[3 * t + 1], D.hi[3 * t + 2]);
+                const V3<R> q0 = mk(std::min(std::max(p.x, mn.x), mx.x), std::min(std::max(p.y, mn.y), mx.y), std::min(std::max(p.z, mn.z), mx.z));
+                const V3<R> da = p - q0;
+                if (dot(da, da) > qm * qm) { ++rej; return; }
+                const V3<R> cen = mk<R>(D.ctr[3 * t], D.ctr[3 * t + 1], D.ctr[3 * t + 2]);
+                const R rr = R(D.rad[t]) + qm;
+                const V3<R> dc = p - cen;
+                if (dot(dc, dc) > rr * rr) { ++rej; return; }
+                const float* T = &D.tv[9 * static_cast<size_t>(t)];
+                const V3<R> a = mk<R>(T[0], T[1], T[2]), b = mk<R>(T[3], T[4], T[5]), c = mk<R>(T[6], T[7], T[8]);
+                const V3<R> q = closest_point_on_triangle(p, a, b, c);
+                const V3<R> d = p - q;
+                const R dsq = dot(d, d);
+                const R dist = std::sqrt(std::max(dsq, eps));
+                const V3<R> tn = cross(b - a, c - a);
+                const R tl = std::sqrt(std::max(dot(tn, tn), eps));
+                const V3<R> tnorm = mul(tn, R(1) / tl);
+                const R sd = dot(d, tnorm);
+                R pen;
+                if (inside) {
+                    const R target = -R(kSignedInsideTargetMargin);
+                    if (sd >= target) return;
+                    pen = target - sd;
+                } else {
+                    pen = cm - sd;
+                }
+                if (pen <= R(0)) return;
+                const V3<R> delta = mul(tnorm, pen);
+                if (!inside && finite(delta)) {
+                    const R w = std::max(pen, eps);
+                    acc = acc + mul(delta, w);
+                    accw += w;
+                }
+                if (dist < best_dist) {
+                    V3<R> cv = mk<R>(0, 0, 0);
+                    if (D.tri_motion && cdt > eps) {
+                        const V3<R> c0 = mk<R>(D.prev_ctr[3 * t], D.prev_ctr[3 * t + 1], D.prev_ctr[3 * t + 2]);
+                        cv = mul(cen - c0, R(1) / cdt);
+                    }
+                    best_dist = dist;
+                    best_pen = pen;
+                    best_delta = delta;
+                    best_vel = cv;
+                }
+            };
+            if (D.tri_hash) {   // A41:4115-4249
+                const int cx = cell_coord<R>(p.x, R(D.tri_cell)), cy = cell_coord<R>(p.y, R(D.tri_cell)), cz = cell_coord<R>(p.z, R(D.tri_cell));
+                for (int dz = -radius; dz <= radius; ++dz)
+                    for (int dy = -radius; dy <= radius; ++dy)
+                        for (int dx = -radius; dx <= radius; ++dx) {
+                            const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
+                            const unsigned bk = hash_cell(qx, qy, qz, D.tri_buckets);
+                            for (int k = D.tri_off[bk]; k < D.tri_off[bk + 1]; ++k) {
+                                const int t = D.tri_idx[k];
+                                if (t < 0 || t >= D.nt) continue;
+                                // A41:3934-3982: the triangle's canonical cell for this vertex
+                                const int kx = std::min(std::max(cx, D.cmin[3 * t]), D.cmax[3 * t]);
+                                const int ky = std::min(std::max(cy, D.cmin[3 * t + 1]), D.cmax[3 * t + 1]);
+                                const int kz = std::min(std::max(cz, D.cmin[3 * t + 2]), D.cmax[3 * t + 2]);
+                                if (qx != kx || qy != ky || qz != kz) continue;
+                                consider(t);
+                            }
+                        }
+                for (int t : D.large) consider(t);
+            } else {
+                for (int t = 0; t < D.nt; ++t) consider(t);
+            }
+            count(kCountDynTriCandidates, cand);
+            count(kCountDynTriAabbRejects, rej);
+            if (best_pen > R(0)) {
+                count(kCountTrianglePairs);
+                signed_response(i, p, local_average(acc, accw, best_delta), best_vel, best_pen, dt, inside);
+                count(kCountExactImpulse);
+            }
+        }
+    }
+    if (inside || D.np <= 0) return;
+    const R margin = std::max(R(cfg.collision_margin), R(0));
+    for (int i = 0; i < V(); ++i) {
+        if (w_zero(i)) continue;
+        const V3<R> p = pos[i];
+        V3<R> best_delta = mk<R>(0, 0, 0), best_vel = mk<R>(0, 0, 0), acc = mk<R>(0, 0, 0);
+        R best_pen = R(0), accw = R(0);
+        auto consider = [&](int j) {   // A41:4354-4413, signed branch
+            if (j < 0 || j >= D.np) return;
+            const V3<R> cen = mk<R>(D.pp[3 * j], D.pp[3 * j + 1], D.pp[3 * j + 2]);
+            const R radius = R(std::fmax(D.pr[j], 0.0f)) + margin;
+            if (radius <= R(0)) return;
+            const V3<R> d = p - cen;
+            const R dsq = dot(d, d);
+            const R dist = std::sqrt(std::max(dsq, eps));
+            if (dist >= radius) return;
+            count(kCountDynParticleCandidates);
+            const V3<R> n = dist > eps ? mul(d, R(1) / dist) : mk<R>(0, 0, 1);
+            const R pen = radius - dist;
+            const V3<R> delta = mul(n, pen);
+            if (finite(delta)) {
+                const R w = std::max(pen, eps);
+                acc = acc + mul(delta, w);
+                accw += w;
+            }
+            if (pen > best_pen) {
+                V3<R> pv = mk<R>(0, 0, 0);
+                if (D.part_motion && cdt > eps) pv = mul(cen - mk<R>(D.pprev[3 * j], D.pprev[3 * j + 1], D.pprev[3 * j + 2]), R(1) / cdt);
+                best_pen = pen;
+                best_delta = delta;
+                best_vel = pv;
+            }
+        };
+        if (D.part_hash) {
+            const int cx = cell_coord<R>(p.x, R(D.part_cell)), cy = cell_coord<R>(p.y, R(D.part_cell)), cz = cell_coord<R>(p.z, R(D.part_cell));
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
+                        const unsigned bk = hash_cell(qx, qy, qz, D.part_buckets);
+                        const int lim = std::min(D.pcount[bk], kParticleBucketSlots);
+                        for (int k = 0; k < lim; ++k) {
+                            const int j = D.pidx[bk * kParticleBucketSlots + k];
+                            if (D.pcell[3 * j] != qx || D.pcell[3 * j + 1] != qy || D.pcell[3 * j + 2] != qz) continue;
+                            consider(j);
+                        }
+                    }
+        } else {
+            for (int j = 0; j < D.np; ++j) consider(j);
+        }
+        if (best_pen > R(0)) {
+            signed_response(i, p, local_average(acc, accw, best_delta), best_vel, best_pen, dt, false);
+            count(kCountExactImpulse);
+            count(kCountDynParticleContacts);
+        }
+    }
+}
+
 template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch) {
     auto t0 = std::chrono::steady_clock::now();
     diag = SsblXpbdDiagnostics{};
@@ -2514,10 +2822,11 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
         diag.global_dynamic_particle_count = scene->set.np;
         diag.global_dynamic_triangle_count = scene->set.nt;
         diag.global_dynamic_hash_overflow = scene->hash_overflow;
-    } else {
-        dyn.build(counts);                                                   // A41:9129-9136
     }
-    const bool dynamic_on = use_scene() ? (scene->set.nt > 0 || scene->set.np > 0) : (dyn.nt > 0 || dyn.np > 0);
+    if (!use_scene() || dyn.signed_set) dyn.build(counts);                  // A41:9129-9136
+    // a signed own set takes the place of the unsigned contacts (A41:9240-9252)
+    const bool dynamic_on = use_scene() ? (scene->set.nt > 0 || scene->set.np > 0) : (!dyn.signed_set && (dyn.nt > 0 || dyn.np > 0));
+    const bool signed_on = dyn.signed_set && (dyn.nt > 0 || dyn.np > 0);
     max_smoothed = 0.0f;
     pcg_guard = 0;
     pcg_runs = 0;
@@ -2559,7 +2868,10 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
             const bool run_self = scheduled && V() > 0 && r > R(0);
             if ((scheduled && !run_self) || (self_it && !scheduled)) diag.self_collision_skipped_launches += 1;
             if (run_self) self_collision();
+            if (last_it && signed_on) signed_contacts(sub_dt, false);            // A41:9322-9327
             if (run_pcg && final_c) final_caps(sub_dt);
+            if (final_c && signed_on)                                             // A41:9351-9359
+                for (int k = 0; k < kSignedInsidePasses; ++k) signed_contacts(sub_dt, true);
             // the reference's final pin polish also runs on every last iteration
             // because dynamic collision is scheduled there (A41:9230, 9360-9369)
             const bool polish = !pins.empty() && (analytic_on || sdf_ready || last_it || run_self);
@@ -2567,6 +2879,7 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
         }
         velocity_update(sub_dt);
     }
+    dyn.tri_motion = dyn.part_motion = false;   // A41:9389-9390
     if (fetch) {
         diag.abi41_soft_contact_count = static_cast<long long>(counts[kCountSoft]);
         diag.abi41_exact_impulse_contact_count = static_cast<long long>(counts[kCountExactImpulse]);
@@ -2805,8 +3118,14 @@ SSBL_API int ssbl_update_frame_inputs(void* handle, const SsblXpbdFrameInputs* i
         if (in->update_pin_targets && !upload_pins(s, in->pin_indices, in->pin_positions, in->pin_weights, in->pin_count)) return 0;
         if (in->update_runtime_colliders) s.colliders = in->runtime_colliders;
         if (in->update_dynamic_triangles && !set_dynamic_triangles(s, in->dynamic_triangles, in->dynamic_triangle_count)) return 0;
-        if (in->update_dynamic_indexed_triangles && in->dynamic_indexed_triangle_count > 0 && in->dynamic_triangle_vertex_count > 0)
-            return fail("oracle: signed indexed dynamic triangles are not restated") ? 1 : 0;
+        if (in->update_dynamic_indexed_triangles) {   // A41:8433-8590
+            const int nv = std::max(in->dynamic_triangle_vertex_count, 0), nt = std::max(in->dynamic_indexed_triangle_count, 0);
+            if (nv > 0 && nt > 0 && (!in->dynamic_triangle_vertices || !in->dynamic_triangle_indices))
+                return fail("dynamic indexed triangle positions and indices are required when count is nonzero") ? 1 : 0;
+            s.dyn.set_indexed(in->dynamic_triangle_vertices, nv, in->dynamic_triangle_indices, nt, in->dynamic_triangle_topology_changed != 0,
+                              s.cfg.collision_margin, s.cfg.cloth_thickness);
+            if (nv > 0 && nt > 0) s.dyn_tri_allocated = true;
+        }
         if (in->update_dynamic_particles &&
             !set_dynamic_particles(s, in->dynamic_particle_positions, in->dynamic_particle_radii, in->dynamic_particle_count))
             return 0;

@@ -133,7 +133,12 @@ struct ClothView {
     int dyn_mode;                 // 0 no filter, 1 every other source, 2 lower layers only
     int dyn_target_src;
     int dyn_target_layer;
-    int pad4;
+    // The cloth's own signed (indexed) dynamic triangles and its particles
+    // (null when none), contacts of A41:4520-4540; collider_dt turns their
+    // centre motion into a velocity (A41:4513-4518).
+    const DynView* sdyn;
+    float collider_dt;
+    int pad5;
 };
 
 struct Tile { int cloth, begin, end, pad; };
@@ -149,6 +154,9 @@ struct LaunchCtx {
 void launch_integrate(const LaunchCtx& c, int parity, unsigned ops, bool full);
 void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops);
 void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops);
+// Signed contacts: the pass after self collision (inside = false) and the
+// inside-only passes of a step's last iteration (A41:9322-9359).
+void launch_signed_pass(const LaunchCtx& c, int parity, bool inside, unsigned ops);
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops);
 void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float ox, float oy, float oz, float voxel,
                       const float* tris, int ntris, int signed_sdf);

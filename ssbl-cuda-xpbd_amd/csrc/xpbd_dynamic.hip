@@ -359,6 +359,8 @@ bool DynamicColliders::reserve_triangles(int n) {
     lo_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
     hi_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
     ctr_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
+    pctr_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
+    bounds_valid_ = false;
     cmin_ = static_cast<int4*>(alloc(0, sizeof(int4) * m));
     cmax_ = static_cast<int4*>(alloc(0, sizeof(int4) * m));
     large_flag_ = static_cast<int*>(alloc(0, sizeof(int) * m));
@@ -463,6 +465,11 @@ bool DynamicColliders::upload_view(hipStream_t st) {
         v.coarse_reach = part_cell_;
         v.max_radius = max_radius_;
     }
+    v.signed_set = signed_ ? 1 : 0;
+    v.tri_motion = tri_motion_ ? 1 : 0;
+    v.part_motion = part_motion_ ? 1 : 0;
+    v.pctr = pctr_;
+    v.pprev = pprev_;
     if (v.tri_hash && v.tri_buckets > tri_buckets_) return fail("dynamic triangle hash capacity");
     if (v.part_hash && v.part_buckets > part_buckets_) return fail("dynamic particle hash capacity");
     launch_k(k_set_view, dim3(1), dim3(1), 0, st, d_view_, v);
@@ -474,7 +481,9 @@ bool DynamicColliders::set_triangles(hipStream_t st, const float* soup, int n, c
                                      float thickness, bool fine) {
     n = std::max(n, 0);
     if (!sync_stream()) return false;
-    if (n == 0) { nt_ = 0; return d_view_ ? upload_view(st) : true; }
+    signed_ = false;
+    tri_motion_ = false;
+    if (n == 0) { nt_ = 0; bounds_valid_ = false; return d_view_ ? upload_view(st) : true; }
     if (!reserve_triangles(n)) return false;
     float max_extent = 0.0f;
     std::vector<float4> h(3 * static_cast<size_t>(n));
@@ -500,6 +509,67 @@ bool DynamicColliders::set_triangles(hipStream_t st, const float* soup, int n, c
     }
     if (!ok) return false;
     nt_ = n;
+    return upload_view(st) && compute_bounds(st);
+}
+
+// The bounds of the current corners, so that the centres always belong to the
+// last upload (the signed path keeps them as the previous centres).
+bool DynamicColliders::compute_bounds(hipStream_t st) {
+    if (nt_ <= 0) return true;
+    launch_k(k_bounds, dim3(grid_for(nt_, 256)), dim3(256), 0, st, static_cast<const DynView*>(d_view_), span_margin_, lo_, hi_, ctr_,
+             cmin_, cmax_, large_flag_, static_cast<unsigned long long*>(nullptr), 0);
+    bounds_valid_ = true;
+    return hip_ok(hipGetLastError(), "dynamic triangle bounds");
+}
+
+bool DynamicColliders::set_indexed(hipStream_t st, const float* verts, int nverts, const int* idx, int ntri, bool topology_changed,
+                                   float margin, float thickness) {
+    nverts = std::max(nverts, 0);
+    ntri = std::max(ntri, 0);
+    if (!sync_stream()) return false;
+    if (nverts == 0 || ntri == 0) {
+        nt_ = 0;
+        signed_ = tri_motion_ = bounds_valid_ = false;
+        return d_view_ ? upload_view(st) : true;
+    }
+    float max_extent = 0.0f;
+    std::vector<float4> h(3 * static_cast<size_t>(ntri), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (int t = 0; t < ntri; ++t) {
+        const int ia = idx[3 * t], ib = idx[3 * t + 1], ic = idx[3 * t + 2];
+        if (ia < 0 || ib < 0 || ic < 0 || ia >= nverts || ib >= nverts || ic >= nverts) continue;
+        const float* A = verts + 3 * static_cast<size_t>(ia);
+        const float* B = verts + 3 * static_cast<size_t>(ib);
+        const float* C = verts + 3 * static_cast<size_t>(ic);
+        h[3 * t] = make_float4(A[0], A[1], A[2], 0.0f);
+        h[3 * t + 1] = make_float4(B[0], B[1], B[2], 0.0f);
+        h[3 * t + 2] = make_float4(C[0], C[1], C[2], 0.0f);
+        const float ext = std::max(std::max(std::max(A[0], std::max(B[0], C[0])) - std::min(A[0], std::min(B[0], C[0])),
+                                            std::max(A[1], std::max(B[1], C[1])) - std::min(A[1], std::min(B[1], C[1]))),
+                                   std::max(A[2], std::max(B[2], C[2])) - std::min(A[2], std::min(B[2], C[2])));
+        if (std::isfinite(ext) && ext >= 0.0f) max_extent = std::max(max_extent, ext);
+    }
+    const bool had_previous = !topology_changed && nt_ == ntri && bounds_valid_ && tri_cap_ >= ntri;
+    if (!reserve_triangles(ntri)) return false;
+    if (had_previous && !hip_ok(hipMemcpy(pctr_, ctr_, sizeof(float4) * ntri, hipMemcpyDeviceToDevice), "dynamic previous centres"))
+        return false;
+    if (!hip_ok(hipMemcpy(tv_, h.data(), sizeof(float4) * h.size(), hipMemcpyHostToDevice), "dynamic indexed triangle upload"))
+        return false;
+    margin = std::max(margin, 0.0f);
+    tri_cell_ = tri_cell_size(max_extent, margin, std::max(thickness, 0.0f), true);
+    span_margin_ = margin;
+    tri_filter_ = false;
+    nt_ = ntri;
+    signed_ = true;
+    tri_motion_ = had_previous;
+    if (!upload_view(st) || !compute_bounds(st)) return false;
+    if (!had_previous && !hip_ok(hipMemcpyAsync(pctr_, ctr_, sizeof(float4) * ntri, hipMemcpyDeviceToDevice, st), "dynamic previous centres"))
+        return false;
+    return true;
+}
+
+bool DynamicColliders::clear_motion(hipStream_t st) {
+    if (!tri_motion_ && !part_motion_) return true;
+    tri_motion_ = part_motion_ = false;
     return upload_view(st);
 }
 
@@ -508,7 +578,7 @@ bool DynamicColliders::set_particles(hipStream_t st, const float* pos, const flo
                                      const int* layer, int n, float margin, float thickness) {
     n = std::max(n, 0);
     if (!sync_stream()) return false;
-    if (n == 0) { np_ = 0; return d_view_ ? upload_view(st) : true; }
+    if (n == 0) { np_ = 0; part_motion_ = false; return d_view_ ? upload_view(st) : true; }
     std::vector<float> clean(n);
     float max_radius = 0.0f;
     max_radius_ = 0.0f;   // the largest radius a query sees, max(r, 0) of the raw value (inf stays inf)
@@ -523,6 +593,7 @@ bool DynamicColliders::set_particles(hipStream_t st, const float* pos, const flo
         radii_seen_ = clean;
     }
     const bool had_previous = np_ == n && part_cap_ >= n;
+    part_motion_ = had_previous || prev != nullptr;   // A41:8625-8690
     if (!reserve_particles(n)) return false;
     std::vector<float4> h(n);
     for (int i = 0; i < n; ++i) h[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], radii[i]);
@@ -592,6 +663,7 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
         return false;
     np_ = np;
     nt_ = nt;
+    bounds_valid_ = false;
     part_cell_ = std::max(std::max(max_radius + margin, thickness), 1.0e-3f);
     max_radius_ = reach_radius;
     radii_seen_.clear();

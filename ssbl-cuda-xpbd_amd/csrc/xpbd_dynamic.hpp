@@ -74,6 +74,12 @@ struct DynView {
     const int* coarse_off;   // device flag: a particle position was not finite, do not use the summary
     int coarse_buckets;
     float coarse_cell, coarse_reach, max_radius;
+    // signed (indexed) triangles, A41:8433-8590: their contacts take the
+    // place of the unsigned ones; centres and particle positions of the
+    // previous upload give the colliders' velocities for one step
+    int signed_set, tri_motion, part_motion, pad5;
+    const float4* pctr;      // previous triangle centres
+    const float4* pprev;     // previous particle positions (.w = radius)
 };
 
 // One contributing cloth of a device-gathered scene.
@@ -97,6 +103,13 @@ public:
     // (A41:6590-6601).  Both sync the library stream before replacing data.
     bool set_triangles(hipStream_t st, const float* soup, int n, const int* src, const int* layer, float margin, float thickness,
                        bool fine);
+    // Signed triangles from indexed vertices (A41:8433-8590); an index out of
+    // range collapses its triangle to the origin.
+    bool set_indexed(hipStream_t st, const float* verts, int nverts, const int* idx, int ntri, bool topology_changed, float margin,
+                     float thickness);
+    // The colliders' velocities hold for the step after an upload only (A41:9389-9390).
+    bool clear_motion(hipStream_t st);
+    bool is_signed() const { return signed_; }
     bool set_particles(hipStream_t st, const float* pos, const float* prev, const float* radii, const int* src, const int* layer,
                        int n, float margin, float thickness);
     // Device gather of every source's current positions (particles) and
@@ -118,6 +131,7 @@ private:
     bool reserve_triangles(int n);
     bool reserve_particles(int n);
     bool upload_view(hipStream_t st);
+    bool compute_bounds(hipStream_t st);
     // allocation groups: 0 triangles, 1 particles, 2 scan scratch, 3 the view, 4 gather tables
     void* alloc(int group, size_t bytes);
     void release(int group);
@@ -149,6 +163,8 @@ private:
     int2* pksl_ = nullptr;
     int4* pbin_ = nullptr;
     int4* pkcell_ = nullptr;
+    float4* pctr_ = nullptr;
+    bool signed_ = false, tri_motion_ = false, part_motion_ = false, bounds_valid_ = false;
     int2* coarse_ = nullptr;
     int* coarse_off_ = nullptr;
     int coarse_buckets_ = 0;

@@ -896,6 +896,252 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     if (s.v_dirty) c.vel[v] = with_w(s.vel_, 0.0f);
 }
 
+// ------------------------------------------------- signed dynamic contacts
+// A41:80-83, 130-137
+constexpr int kSignedInsidePasses = 6;
+constexpr float kSignedQueryMargin = 0.04f;
+constexpr float kSignedMaxQueryMargin = 0.08f;
+constexpr float kSignedInsideTargetMargin = -0.001f;
+constexpr float kSignedRelaxation = 1.0f / 3.0f;
+constexpr float kSignedOutwardRetention = 1.0f / 8.0f;
+constexpr float kSignedContactMarginScale = 0.75f;
+constexpr float kSignedInsideRelaxation = 1.0f / 3.0f;
+constexpr float kSignedInsideMaxDeltaScale = 16.0f;
+constexpr float kSignedInsideMaxDeltaFloor = 3.0e-3f;
+constexpr float kDynamicMaxDeltaScale = 3.0f;
+constexpr float kDynamicMaxDeltaFloor = 5.0e-4f;
+
+// A41:978-993: the depth-weighted mean direction of all contacts at the
+// length of the kept one.
+__device__ inline F3 local_average_delta(F3 acc, float w, F3 fallback) {
+    if (w > 0.0f && finite3(acc)) {
+        const F3 avg = mul(acc, 1.0f / w);
+        const float al = len3(avg), fl = len3(fallback);
+        if (finite3(avg) && isfinite(al) && isfinite(fl) && al > kEps && fl > kEps) return mul(avg, fl / al);
+    }
+    return fallback;
+}
+
+// A41:995-1068
+__device__ inline void signed_response(const ClothView& c, VertexState& s, F3 orig, F3 raw, F3 cvel, float proxy_depth, float dt,
+                                       bool inside, bool count) {
+    if (!finite3(orig) || !finite3(raw)) return;
+    const float raw_len = len3(raw);
+    if (!isfinite(raw_len) || raw_len <= kEps) return;
+    const F3 n = mul(raw, 1.0f / raw_len);
+    F3 applied = mul(raw, inside ? kSignedInsideRelaxation : kSignedRelaxation);
+    {   // A41:951-968
+        const float basis = fmaxf(fmaxf(c.thickness, 0.0f), fmaxf(c.collision_margin, 0.0f));
+        const float max_len = inside ? fmaxf(kSignedInsideMaxDeltaFloor, basis * kSignedInsideMaxDeltaScale)
+                                     : fmaxf(kDynamicMaxDeltaFloor, basis * kDynamicMaxDeltaScale);
+        if (!finite3(applied) || !isfinite(max_len) || max_len <= 0.0f) applied = f3(0.0f, 0.0f, 0.0f);
+        else {
+            const float len = len3(applied);
+            if (!isfinite(len) || len <= kEps) applied = f3(0.0f, 0.0f, 0.0f);
+            else if (len > max_len) applied = mul(applied, max_len / len);
+        }
+    }
+    const float applied_len = len3(applied);
+    if (!isfinite(applied_len) || applied_len <= kEps) return;
+    const float sdt = fmaxf(dt, kEps);
+    const F3 corrected = orig + applied;
+    F3 v = s.vel();
+    F3 base = f3(0.0f, 0.0f, 0.0f);
+    if (inside) v = finite3(v) ? v : f3(0.0f, 0.0f, 0.0f);
+    if (finite3(cvel)) {
+        const float cn = dot(cvel, n);
+        if (isfinite(cn)) base = mul(n, cn);
+    }
+    F3 rel = v - base;
+    if (inside) {
+        const float ns = dot(rel, n);
+        if (isfinite(ns) && ns < 0.0f) {
+            rel = rel - mul(n, ns);
+            v = base + rel;
+        }
+    } else {
+        const float inward = fmaxf(-dot(rel, n), 0.0f);
+        const float proxy = fmaxf(fmaxf(proxy_depth, raw_len), applied_len);
+        contact_response(c, n, inward + proxy / sdt, rel, count);
+        const float outward = dot(rel, n);
+        if (outward > 0.0f) rel = rel - mul(n, outward * (1.0f - kSignedOutwardRetention));
+        v = base + rel;
+    }
+    s.p = corrected;
+    const F3 vf = finite3(v) ? v : f3(0.0f, 0.0f, 0.0f);
+    s.set_vel(vf);
+    s.set_q(corrected - mul(vf, sdt));
+}
+
+// A41:3600-3633, 3999-4323, 4354-4511, 4520-4540.  The triangle query visits
+// every hash cell within the query radius and takes each triangle at its
+// canonical cell; it keeps the nearest triangle (the first of equal distance)
+// whose signed distance is under the contact margin, or, inside-only, more
+// than 1 mm behind the face.  The particle part keeps the deepest particle.
+// Both push along the depth-weighted mean of all their contacts.  These are
+// the cloth's own colliders: no source filter.
+__device__ inline void signed_contacts(const ClothView& c, VertexState& s, bool inside, bool count) {
+    const DynView& D = *c.sdyn;
+    const float dt = c.dt_sub, cdt = c.collider_dt;
+    const float cm = fmaxf(fmaxf(c.collision_margin, 0.0f), fmaxf(c.thickness, 0.0f) * kSignedContactMarginScale);
+    const float qm = fminf(fmaxf(fmaxf(fmaxf(cm * 2.0f, kSignedQueryMargin), fmaxf(D.tri_cell, 0.0f) * 1.25f), cm), kSignedMaxQueryMargin);
+    const int nt = D.nt;
+    if (nt > 0 && cm > 0.0f && qm > 0.0f) {
+        const F3 p = s.p;
+        F3 best_delta = f3(0.0f, 0.0f, 0.0f), best_vel = f3(0.0f, 0.0f, 0.0f), acc = f3(0.0f, 0.0f, 0.0f);
+        float best_pen = 0.0f, best_dist = 3.402823466e+38f, accw = 0.0f;
+        unsigned cand = 0, rej = 0;
+        auto consider = [&](int t) {
+            if (t < 0 || t >= nt) return;
+            ++cand;
+            const float4 mn = D.lo[t], mx = D.hi[t];
+            const F3 q0 = f3(fminf(fmaxf(p.x, mn.x), mx.x), fminf(fmaxf(p.y, mn.y), mx.y), fminf(fmaxf(p.z, mn.z), mx.z));
+            const F3 da = p - q0;
+            if (dot(da, da) > qm * qm) { ++rej; return; }
+            const float4 ce = D.ctr[t];
+            const float rr = ce.w + qm;
+            const F3 dc = p - f3(ce);
+            if (dot(dc, dc) > rr * rr) { ++rej; return; }
+            const F3 a = f3(D.tv[3 * t]), b = f3(D.tv[3 * t + 1]), cc = f3(D.tv[3 * t + 2]);
+            const F3 q = closest_point_on_triangle(p, a, b, cc);
+            const F3 d = p - q;
+            const float dsq = dot(d, d);
+            const float dist = sqrtf(fmaxf(dsq, kEps));
+            const F3 tn = cross(b - a, cc - a);
+            const float tl = sqrtf(fmaxf(dot(tn, tn), kEps));
+            const F3 tnorm = mul(tn, 1.0f / tl);
+            const float sd = dot(d, tnorm);
+            float pen;
+            if (inside) {
+                const float target = -kSignedInsideTargetMargin;
+                if (sd >= target) return;
+                pen = target - sd;
+            } else {
+                pen = cm - sd;
+            }
+            if (pen <= 0.0f) return;
+            const F3 delta = mul(tnorm, pen);
+            if (!inside && finite3(delta)) {
+                const float w = fmaxf(pen, kEps);
+                acc = acc + mul(delta, w);
+                accw += w;
+            }
+            if (dist < best_dist) {
+                F3 cv = f3(0.0f, 0.0f, 0.0f);
+                if (D.tri_motion && cdt > kEps) cv = mul(f3(ce) - f3(D.pctr[t]), 1.0f / cdt);
+                best_dist = dist;
+                best_pen = pen;
+                best_delta = delta;
+                best_vel = cv;
+            }
+        };
+        if (D.tri_hash) {
+            const float cs = fmaxf(D.tri_cell, 1.0e-3f);
+            const int radius = inside ? min(max(static_cast<int>(ceilf(fmaxf(qm, 0.0f) / cs)), 1), 4) : 1;
+            const int cx = cell_coord(p.x, D.tri_cell), cy = cell_coord(p.y, D.tri_cell), cz = cell_coord(p.z, D.tri_cell);
+            for (int dz = -radius; dz <= radius; ++dz)
+                for (int dy = -radius; dy <= radius; ++dy)
+                    for (int dx = -radius; dx <= radius; ++dx) {
+                        const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
+                        const int4 bin = D.tri_bin[hash_cell(qx, qy, qz, D.tri_buckets)];
+                        for (int k = bin.x; k < bin.x + bin.y; ++k) {
+                            const int t = D.tri_keys[k] / kDynMaxHashCells;
+                            if (t < 0 || t >= nt) continue;
+                            // A41:3934-3982: the triangle's canonical cell for this vertex
+                            const int4 lo = D.cmin[t], hi = D.cmax[t];
+                            if (qx != min(max(cx, lo.x), hi.x) || qy != min(max(cy, lo.y), hi.y) || qz != min(max(cz, lo.z), hi.z)) continue;
+                            consider(t);
+                        }
+                    }
+            const int nl = *D.nlarge;
+            for (int k = 0; k < nl; ++k) consider(D.large[k]);
+        } else {
+            for (int t = 0; t < nt; ++t) consider(t);
+        }
+        if (count && cand) atomicAdd(&c.counts[kCountDynTriCandidates], static_cast<unsigned long long>(cand));
+        if (count && rej) atomicAdd(&c.counts[kCountDynTriAabbRejects], static_cast<unsigned long long>(rej));
+        if (best_pen > 0.0f) {
+            if (count) atomicAdd(&c.counts[kCountTrianglePairs], 1ull);
+            signed_response(c, s, p, local_average_delta(acc, accw, best_delta), best_vel, best_pen, dt, inside, count);
+            if (count) atomicAdd(&c.counts[kCountExactImpulse], 1ull);
+        }
+    }
+    const int np_count = D.np;
+    if (inside || np_count <= 0) return;
+    const float margin = fmaxf(c.collision_margin, 0.0f);
+    const F3 p = s.p;
+    F3 best_delta = f3(0.0f, 0.0f, 0.0f), best_vel = f3(0.0f, 0.0f, 0.0f), acc = f3(0.0f, 0.0f, 0.0f);
+    float best_pen = 0.0f, accw = 0.0f;
+    unsigned cand = 0;
+    auto consider = [&](int j) {
+        if (j < 0 || j >= np_count) return;
+        const float4 pj = D.pp[j];
+        const float radius = fmaxf(pj.w, 0.0f) + margin;
+        if (radius <= 0.0f) return;
+        const F3 d = p - f3(pj);
+        const float dsq = dot(d, d);
+        const float dist = sqrtf(fmaxf(dsq, kEps));
+        if (dist >= radius) return;
+        ++cand;
+        const F3 n = dist > kEps ? mul(d, 1.0f / dist) : f3(0.0f, 0.0f, 1.0f);
+        const float pen = radius - dist;
+        const F3 delta = mul(n, pen);
+        if (finite3(delta)) {
+            const float w = fmaxf(pen, kEps);
+            acc = acc + mul(delta, w);
+            accw += w;
+        }
+        if (pen > best_pen) {
+            F3 pv = f3(0.0f, 0.0f, 0.0f);
+            if (D.part_motion && cdt > kEps) pv = mul(f3(pj) - f3(D.pprev[j]), 1.0f / cdt);
+            best_pen = pen;
+            best_delta = delta;
+            best_vel = pv;
+        }
+    };
+    if (D.part_hash) {
+        const int cx = cell_coord(p.x, D.part_cell), cy = cell_coord(p.y, D.part_cell), cz = cell_coord(p.z, D.part_cell);
+        for (int k27 = 0; k27 < 27; ++k27) {   // z, then y, then x innermost
+            const int qx = cx + k27 % 3 - 1, qy = cy + (k27 / 3) % 3 - 1, qz = cz + k27 / 9 - 1;
+            const int4 bin = D.pbin[hash_cell(qx, qy, qz, D.part_buckets)];
+            const int lim = min(bin.y, kDynParticleSlots);
+            for (int k = 0; k < lim; ++k) {
+                const int4 pc = D.pkcell[bin.x + k];
+                if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
+                consider(D.pkeys[bin.x + k]);
+            }
+        }
+    } else {
+        for (int j = 0; j < np_count; ++j) consider(j);
+    }
+    if (count && cand) atomicAdd(&c.counts[kCountDynParticleCandidates], static_cast<unsigned long long>(cand));
+    if (best_pen > 0.0f) {
+        signed_response(c, s, p, local_average_delta(acc, accw, best_delta), best_vel, best_pen, dt, false, count);
+        if (count) {
+            atomicAdd(&c.counts[kCountExactImpulse], 1ull);
+            atomicAdd(&c.counts[kCountDynParticleContacts], 1ull);
+        }
+    }
+}
+
+template <bool kInside>
+__global__ __launch_bounds__(kTile) void k_signed_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles, int ntiles,
+                                                       int parity, unsigned ops) {
+    const Tile tile = tiles[xcd_tile(ntiles)];
+    const int v = tile.begin + static_cast<int>(threadIdx.x);
+    if (v >= tile.end) return;
+    const ClothView& c = views[tile.cloth];
+    if (!c.sdyn) return;
+    float4* pos = parity ? c.pos1 : c.pos0;
+    const float4 p4 = pos[v];
+    if (p4.w == 0.0f) return;
+    VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
+    signed_contacts(c, s, kInside, (ops & kOpCount) != 0u);
+    if (s.p.x != p4.x || s.p.y != p4.y || s.p.z != p4.z) pos[v] = with_w(s.p, p4.w);
+    if (s.q_dirty) c.prev[v] = with_w(s.q_, 0.0f);
+    if (s.v_dirty) c.vel[v] = with_w(s.vel_, 0.0f);
+}
+
 // ---------------------------------------------------------- vertex pass
 // Neighbour positions loaded together before any of their edges is
 // evaluated: one round of memory latency per chunk instead of one per edge.
@@ -1257,6 +1503,16 @@ void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops) {
     TimedLaunch t("dynamic_contacts", c.stream);
     launch_k(k_dynamic_pass<0>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
     launch_k(k_dynamic_pass<1>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
+}
+
+void launch_signed_pass(const LaunchCtx& c, int parity, bool inside, unsigned ops) {
+    TimedLaunch t("signed_contacts", c.stream);
+    if (inside) {
+        for (int k = 0; k < kSignedInsidePasses; ++k)
+            launch_k(k_signed_pass<true>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
+    } else {
+        launch_k(k_signed_pass<false>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
+    }
 }
 
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops) {

@@ -813,7 +813,12 @@ void Solver::fill_view(int substeps) {
     v.need_normal = (v.use_pressure || !fields.empty()) ? 1 : 0;
     v.direct_stretch = cfg.stretch_optimization_enabled ? 0 : 1;
     v.contacts_on = (analytic_on() || sdf_ready) ? 1 : 0;
-    v.dyn = dyn_on() ? dyn_set().device_view() : nullptr;
+    // a signed own set replaces the unsigned contacts (A41:9240-9252); an
+    // attached scene is never signed (A41:6561)
+    const bool own_signed = dyn.is_signed() && (dyn.triangles() > 0 || dyn.particles() > 0);
+    v.dyn = (dyn_on() && (use_scene() || !dyn.is_signed())) ? dyn_set().device_view() : nullptr;
+    v.sdyn = own_signed ? dyn.device_view() : nullptr;
+    v.collider_dt = (std::isfinite(cfg.dt) && cfg.dt > 1.0e-7f) ? cfg.dt : std::max(v.dt_sub, 1.0e-7f);   // A41:4513-4518
     v.dyn_mode = use_scene() ? cross_mode : 0;
     v.dyn_target_src = target_src;
     v.dyn_target_layer = target_layer;
@@ -825,7 +830,7 @@ void Solver::fill_view(int substeps) {
 // `dviews[m]` is member m's view inside that table.
 struct StepPlan {
     bool any_pins = false, any_edges = false, any_lra = false, any_bend = false, any_self = false, any_pcg = false;
-    bool any_bend_gather = false, any_forces = false, any_dyn = false;
+    bool any_bend_gather = false, any_forces = false, any_dyn = false, any_signed = false;
 };
 
 static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const int4* bend_segs_dev,
@@ -842,6 +847,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
         plan.any_bend_gather |= s->bend_gather;
         plan.any_forces |= s->view.need_normal != 0 || s->view.nfield > 0;
         plan.any_dyn |= s->view.dyn != nullptr;
+        plan.any_signed |= s->view.sdyn != nullptr;
     }
     const unsigned cnt = count ? kOpCount : 0u;
     const unsigned pin_ops = plan.any_pins ? (kOpPin | kOpPinPolish) : 0u;
@@ -866,6 +872,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             const bool final_c = last && sub == substeps - 1;
             const unsigned last_op = last ? kOpLastIter : 0u;
             const bool dyn_now = last && plan.any_dyn;   // A41:9230-9252
+            const bool signed_now = last && plan.any_signed;   // A41:9322-9359
             bool self_now = false;
             if (plan.any_self && last) {
                 for (Solver* m : members) {
@@ -874,7 +881,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
                 }
             }
             const bool run_pcg = plan.any_pcg && last;   // substep cadence 1 (A41:284-292)
-            const bool split = run_pcg || self_now || dyn_now || (final_c && (plan.any_lra || plan.any_bend));
+            const bool split = run_pcg || self_now || dyn_now || signed_now || (final_c && (plan.any_lra || plan.any_bend));
             if (!split) {
                 launch_vertex_pass(ctx, parity, parity ^ 1,
                                    stretch | pin_ops | kOpAnalytic | kOpSdf | last_op | (last ? kOpVelocity : 0u) | cnt);
@@ -904,16 +911,18 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
                     parity ^= 1;
                 }
             }
-            const bool tail = self_now || dyn_now || (run_pcg && final_c);
+            const bool tail = self_now || dyn_now || signed_now || (run_pcg && final_c);
             launch_vertex_pass(ctx, parity, parity,
                                (tail ? (pin_ops & ~kOpPinPolish) : (pin_ops | last_op | kOpVelocity)) | kOpAnalytic | kOpSdf | cnt);
             if (dyn_now) launch_dynamic_pass(ctx, parity, cnt);
             if (self_now) {
                 if (!self_batch || !self_batch->run(ctx, parity, sub, substeps, count)) return false;
             }
+            if (signed_now) launch_signed_pass(ctx, parity, false, cnt);
             if (run_pcg && final_c) {
                 per_pcg([&](Solver& s, const ClothView* cv) { return s.pcg.final_caps(st, cv, s.d_tiles.p, parity, s.view.dt_sub); });
             }
+            if (signed_now && final_c) launch_signed_pass(ctx, parity, true, cnt);
             if (tail) launch_vertex_pass(ctx, parity, parity, (pin_ops & kOpPinPolish) | last_op | kOpVelocity | cnt);
         }
     }
@@ -1019,7 +1028,8 @@ static bool prepare_member(Solver& s, int substeps, int iterations, bool count) 
         launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.max_smoothed.p), 0u, 1);
     }
     // the cloth's own dynamic colliders are re-hashed every step (A41:9129-9136)
-    if (!s.use_scene() && s.dyn_on() && !s.dyn.build(lib_stream(), s.counts.p, count, count)) return false;
+    const bool own = (!s.use_scene() || s.dyn.is_signed()) && (s.dyn.triangles() > 0 || s.dyn.particles() > 0);
+    if (own && !s.dyn.build(lib_stream(), s.counts.p, count, count)) return false;
     return true;
 }
 
@@ -1039,6 +1049,7 @@ static bool step_one(Solver& s, int substeps, int iterations, int fetch, int for
                           count))
             return false;
     }
+    if (!s.dyn.clear_motion(lib_stream())) return false;
     if (force_sync || fetch) {
         const auto ts = std::chrono::steady_clock::now();
         if (!sync_stream()) return false;
@@ -1111,6 +1122,8 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     SelfBatch* sb = b.self.get(b.members, b.h_views, b.d_views.p);
     if (!run_schedule(b.members, ctx, b.d_views.p, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false))
         return false;
+    for (Solver* s : b.members)
+        if (!s->dyn.clear_motion(lib_stream())) return false;
     return force_sync ? sync_stream() : true;
 }
 
@@ -1266,8 +1279,14 @@ SSBL_API int ssbl_update_frame_inputs(void* handle, const SsblXpbdFrameInputs* i
                                   s->cfg.cloth_thickness, false))
             return 0;
     }
-    if (in->update_dynamic_indexed_triangles && in->dynamic_indexed_triangle_count > 0 && in->dynamic_triangle_vertex_count > 0)
-        return fail("signed indexed dynamic triangle colliders are not available in this build yet") ? 1 : 0;
+    if (in->update_dynamic_indexed_triangles) {   // A41:8433-8590
+        const int nv = std::max(in->dynamic_triangle_vertex_count, 0), nt = std::max(in->dynamic_indexed_triangle_count, 0);
+        if (nv > 0 && nt > 0 && (!in->dynamic_triangle_vertices || !in->dynamic_triangle_indices))
+            return fail("dynamic indexed triangle positions and indices are required when count is nonzero") ? 1 : 0;
+        if (!s->dyn.set_indexed(lib_stream(), in->dynamic_triangle_vertices, nv, in->dynamic_triangle_indices, nt,
+                                in->dynamic_triangle_topology_changed != 0, s->cfg.collision_margin, s->cfg.cloth_thickness))
+            return 0;
+    }
     if (in->update_dynamic_particles) {   // A41:8575-8697
         const int c = std::max(in->dynamic_particle_count, 0);
         if (c > 0 && (!in->dynamic_particle_positions || !in->dynamic_particle_radii))

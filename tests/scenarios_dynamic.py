@@ -156,3 +156,112 @@ def cloth_on_sdf_grid(nb, lib, scenes, grid, frames=20):
         s.step(8, 4, diagnostics=True)
         out.append(s.download_positions())
     return out, s.diagnostics()
+
+
+# --- signed (indexed) dynamic triangles, A41:8433-8590 and 4520-4540
+def indexed_plane(z, half=1.0, n=1):
+    """An n x n grid of quads at height z as indexed vertices (normals +z)."""
+    s = np.linspace(-half, half, n + 1)
+    x, y = np.meshgrid(s, s, indexing="ij")
+    verts = np.stack([x.ravel(), y.ravel(), np.full(x.size, z)], 1).astype(np.float32)
+    idx = []
+    for i in range(n):
+        for j in range(n):
+            a, b, c, d = i * (n + 1) + j, (i + 1) * (n + 1) + j, (i + 1) * (n + 1) + j + 1, i * (n + 1) + j + 1
+            idx += [[a, b, c], [a, c, d]]
+    return verts, np.asarray(idx, np.int32)
+
+
+def upload_indexed(lib, r, verts, idx, topology_changed=0):
+    verts = np.ascontiguousarray(verts, np.float32)
+    idx = np.ascontiguousarray(idx, np.int32)
+    fi = abi.FrameInputs()
+    fi.update_dynamic_indexed_triangles = 1
+    fi.dynamic_triangle_vertices = verts.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    fi.dynamic_triangle_vertex_count = len(verts)
+    fi.dynamic_triangle_indices = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    fi.dynamic_indexed_triangle_count = len(idx)
+    fi.dynamic_triangle_topology_changed = topology_changed
+    r.check(lib.ssbl_update_frame_inputs(r.h, ctypes.byref(fi)))
+
+
+SIGNED_MARGIN, SIGNED_THICKNESS = 0.005, 0.004   # contact margin max(0.005, 0.75 x 0.004) = 0.005
+
+
+def signed_plane_contact(lib, height, quads_per_side, moved=None):
+    """A free vertex at `height` over an indexed plane at z = 0 (no gravity),
+    one substep of one iteration, so the pass after self collision and the
+    six inside-only passes all run.  With `moved`, the plane was uploaded
+    before at z = -moved, so it rises by `moved` over the frame."""
+    r = Raw(lib, [[0.01, 0.02, height]], [1.0], collision_margin=SIGNED_MARGIN, cloth_thickness=SIGNED_THICKNESS,
+            gravity=(0.0, 0.0, 0.0))
+    if moved is not None:
+        v0, idx = indexed_plane(-moved, 1.0, quads_per_side)
+        upload_indexed(lib, r, v0, idx)
+    verts, idx = indexed_plane(0.0, 1.0, quads_per_side)
+    upload_indexed(lib, r, verts, idx)
+    r.step(1, 1)
+    return r.state(), r.diag(), r
+
+
+def signed_plane_expected(height, cross_len=4.0):
+    """Closed form of A41:995-1068 for the scene above (plane normal +z): the
+    first pass pushes by a third of (margin - signed distance); the six
+    inside-only passes each push by a third of (1 mm - signed distance) while
+    the vertex is more than 1 mm below that.
+
+    `cross_len` is |(b - a) x (c - a)| of the plane's triangles.  The
+    reference divides the normal by sqrt(max(|n|^2, 1e-7)) (A41:4073-4075), so
+    for triangles with |n| < sqrt(1e-7) both the signed distance and the push
+    shrink by s = |n| / sqrt(1e-7)."""
+    sc = min(1.0, cross_len / np.sqrt(np.float32(1.0e-7)))
+    z = float(height)
+    pen = SIGNED_MARGIN - sc * z
+    if pen > 0.0:
+        z += sc * pen / 3.0
+    for _ in range(6):
+        pen = 0.001 - sc * z
+        if pen > 0.0:
+            z += sc * pen / 3.0
+    return z
+
+
+def cloth_over_indexed_collider(nb, lib, scenes, frames=20, particles=True, batched=False):
+    """A 24 x 24 cloth dropped onto a rising indexed dome (a bumpy grid) that
+    is re-uploaded every frame, plus a few dynamic particles: the signed
+    passes with collider velocity, local averaging, and inside-only passes."""
+    cloth = scenes.grid_cloth(24, 0.8, 0.25, pin_corners=False, seed=3)
+    opts = scenes.SolverOptions(collision_margin=SIGNED_MARGIN, cloth_thickness=SIGNED_THICKNESS, contact_friction=0.3)
+    s = nb.NativeXpbdSolver(cloth, opts, None, lib=lib)
+    n = 16
+    g = np.linspace(-0.7, 0.7, n + 1)
+    x, y = np.meshgrid(g, g, indexing="ij")
+    idx = indexed_plane(0.0, 0.7, n)[1]
+    p = np.stack([np.linspace(-0.2, 0.2, 5), np.zeros(5), np.full(5, 0.12)], 1).astype(np.float32)
+    rad = np.full(5, 0.03, np.float32)
+    out = []
+    for f in range(frames):
+        z = 0.05 + 0.004 * f + 0.04 * np.exp(-4.0 * (x ** 2 + y ** 2))
+        verts = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.float32)
+        fi = abi.FrameInputs()
+        fi.update_dynamic_indexed_triangles = 1
+        fi.dynamic_triangle_vertices = verts.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        fi.dynamic_triangle_vertex_count = len(verts)
+        fi.dynamic_triangle_indices = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        fi.dynamic_indexed_triangle_count = len(idx)
+        if particles:
+            pf = p + np.float32(0.002 * f) * np.array([1.0, 0.0, 0.0], np.float32)
+            fi.update_dynamic_particles = 1
+            fi.dynamic_particle_positions = pf.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            fi.dynamic_particle_radii = rad.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            fi.dynamic_particle_count = len(pf)
+        if not lib.ssbl_update_frame_inputs(s.handle, ctypes.byref(fi)):
+            raise nb.NativeSolverError(nb.last_error(lib))
+        if batched:
+            b = nb.SolverBatch([s])
+            b.step(4, 3, synchronize=True)
+            b.close()
+        else:
+            s.step(4, 3, diagnostics=True)
+        out.append(s.download_state() if hasattr(s, "download_state") else s.download_positions())
+    return s, out

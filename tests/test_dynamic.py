@@ -123,3 +123,64 @@ def test_analytic_box_grid_keeps_cloth_outside(nb, oracle, scenes):
     # without the SDF the cloth would fall through to the ground (30 cm inside);
     # corners of a 4 cm-voxel trilinear grid leave at most a few mm
     assert dist.min() > -0.01
+
+
+# --- signed (indexed) dynamic triangles
+@pytest.mark.parametrize("height", [0.002, -0.002, -0.004, 0.0045, 0.02])
+@pytest.mark.parametrize("quads", [1, 40])
+def test_signed_plane_closed_form(oracle, height, quads):
+    """One quad (two triangles, brute-force query) or 40 x 40 quads of 1 cm
+    (3,200 triangles, hash query over the 27 or 125 cells around the vertex,
+    whose normals are shortened by the reference's 1e-7 clamp)."""
+    half = 1.0 if quads == 1 else 0.2
+    r = sd.Raw(oracle, [[0.01, 0.02, height]], [1.0], collision_margin=sd.SIGNED_MARGIN, cloth_thickness=sd.SIGNED_THICKNESS,
+               gravity=(0.0, 0.0, 0.0))
+    verts, idx = sd.indexed_plane(0.0, half, quads)
+    sd.upload_indexed(oracle, r, verts, idx)
+    r.step(1, 1)
+    p, q, v = r.state()
+    d = r.diag()
+    side = 2.0 * half / quads
+    expect = sd.signed_plane_expected(height, cross_len=side * side)
+    assert abs(float(p[0, 2]) - expect) < 2e-7, (p[0], expect)
+    assert abs(p[0, 0] - 0.01) < 1e-7 and abs(p[0, 1] - 0.02) < 1e-7
+    touched = height < sd.SIGNED_MARGIN
+    assert (d.fast_triangle_pair_contacts > 0) == touched
+    assert d.dynamic_triangle_count == 2 * quads * quads
+
+
+def test_signed_collider_velocity(oracle):
+    """A plane that rose by 2 mm since its last upload moves at 2 mm / (1/60 s):
+    the vertex it pushes leaves with that normal speed (A41:1034-1066)."""
+    (p, q, v), d, r = sd.signed_plane_contact(oracle, 0.002, 1, moved=0.002)
+    assert abs(float(p[0, 2]) - sd.signed_plane_expected(0.002)) < 2e-7
+    assert abs(float(v[0, 2]) - 0.002 * 60.0) < 1e-4 and abs(v[0, 0]) < 1e-7 and abs(v[0, 1]) < 1e-7
+    (p2, q2, v2), _, _ = sd.signed_plane_contact(oracle, 0.002, 1)
+    assert abs(float(v2[0, 2])) < 1e-6
+
+
+def test_soup_upload_turns_signed_off(oracle):
+    """A soup upload after an indexed one uses the unsigned contact (half the
+    depth, A41:4310-4322) again."""
+    r = sd.Raw(oracle, [[0.01, 0.02, 0.002]], [1.0], collision_margin=sd.SIGNED_MARGIN, cloth_thickness=sd.SIGNED_THICKNESS,
+               gravity=(0.0, 0.0, 0.0))
+    verts, idx = sd.indexed_plane(0.0, 1.0, 1)
+    sd.upload_indexed(oracle, r, verts, idx)
+    soup = sd.plane_soup(0.0, 1.0, 1)
+    r.check(oracle.ssbl_update_dynamic_triangles(r.h, soup.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(soup)))
+    r.step(1, 1)
+    p = r.state()[0]
+    assert abs(float(p[0, 2]) - (0.002 + 0.5 * (sd.SIGNED_MARGIN - 0.002))) < 2e-7
+
+
+def test_cloth_over_indexed_collider_runs(nb, oracle, scenes):
+    s, out = sd.cloth_over_indexed_collider(nb, oracle, scenes, frames=12)
+    d = s.diagnostics()
+    assert d.fast_triangle_pair_contacts > 0 and d.dynamic_particle_contacts >= 0
+    pos = out[-1][0]
+    assert np.all(np.isfinite(pos))
+    # nothing ends more than 1 mm below the dome's surface under it
+    x, y = pos[:, 0], pos[:, 1]
+    inside = (np.abs(x) < 0.6) & (np.abs(y) < 0.6)
+    dome = 0.05 + 0.004 * 11 + 0.04 * np.exp(-4.0 * (x ** 2 + y ** 2))
+    assert np.min(pos[inside, 2] - dome[inside]) > -0.002
