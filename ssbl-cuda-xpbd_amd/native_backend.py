@@ -12,6 +12,7 @@ tests use that to drive the CPU oracle through the same class.
 from __future__ import annotations
 
 import ctypes
+import hashlib
 import os
 from pathlib import Path
 
@@ -52,6 +53,19 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
     abi.bind(lib, with_ext=True)
     _CACHE[path] = lib
     return lib
+
+
+def content_signature(arr) -> tuple:
+    """dtype, shape and a 16-byte BLAKE2b digest of an array's bytes: the key the
+    frame-input update uses to skip re-uploading unchanged colliders
+    (native_backend.py:644-677)."""
+    arr = np.ascontiguousarray(arr)
+    h = hashlib.blake2b(digest_size=16)
+    h.update(arr.dtype.str.encode("ascii", errors="ignore"))
+    h.update(np.asarray(arr.shape, dtype=np.int64).tobytes())
+    if arr.size:
+        h.update(arr.tobytes())
+    return (arr.dtype.str, tuple(int(v) for v in arr.shape), h.digest())
 
 
 def last_error(lib) -> str:
@@ -154,6 +168,8 @@ class NativeXpbdSolver:
         if not self._handle:
             raise NativeSolverError(last_error(self._lib))
         self._keep = None
+        self._dynamic_triangle_signature = None
+        self._dynamic_particle_signature = None
         if len(cloth.pin_indices) > 0:
             self.update_pin_targets(cloth.pin_indices, cloth.pin_targets_world, cloth.pin_weights)
         if cloth.start_positions is not None:
@@ -237,6 +253,92 @@ class NativeXpbdSolver:
         fi.dynamic_particle_radii = _fptr(r)
         fi.dynamic_particle_count = len(p)
         self._check(self._lib.ssbl_update_frame_inputs(self._handle, ctypes.byref(fi)))
+
+    def update_frame_inputs(self, *, pin_indices=None, pin_positions=None, pin_weights=None, update_pin: bool = False,
+                            options: SolverOptions | None = None, update_runtime: bool = False, static_triangles=None,
+                            update_static: bool = False, dynamic_triangles=None, update_dynamic: bool = False,
+                            dynamic_triangle_vertices=None, dynamic_triangle_indices=None,
+                            dynamic_triangle_topology_signature=None, dynamic_particles: dict | None = None,
+                            update_dynamic_particles: bool = False, force_fields: list[dict] | None = None,
+                            update_force_fields: bool = False) -> bool:
+        """All of a frame's inputs in one ssbl_update_frame_inputs call
+        (native_backend.py:1064-1279).
+
+        Indexed dynamic triangles (vertices + indices) are sent as signed
+        colliders.  Their topology counts as changed when the vertex count,
+        triangle count or index content differ from the last upload, unless
+        the caller passes its own signature.  A soup, or particles whose
+        content equals the last upload, is not sent again.  Returns False
+        when there was nothing to send."""
+        pin_idx = _i32(pin_indices if pin_indices is not None else np.empty(0, np.int32))
+        pin_pos = _f32(pin_positions if pin_positions is not None else np.empty((0, 3), np.float32), (-1, 3))
+        pin_w = np.ones(len(pin_idx), np.float32) if pin_weights is None else _f32(pin_weights)
+        if len(pin_idx) != len(pin_pos) or len(pin_idx) != len(pin_w):
+            raise NativeSolverError("Pin index, target, and weight arrays must have matching lengths.")
+        verts = _f32(dynamic_triangle_vertices if dynamic_triangle_vertices is not None else np.empty((0, 3), np.float32), (-1, 3))
+        idx = _i32(dynamic_triangle_indices if dynamic_triangle_indices is not None else np.empty((0, 3), np.int32)).reshape(-1, 3)
+        indexed = bool(update_dynamic and dynamic_triangle_vertices is not None and dynamic_triangle_indices is not None)
+        if update_dynamic and not indexed and dynamic_triangles is None and len(verts) and len(idx):
+            dynamic_triangles = verts[idx]
+        soup = (_f32(dynamic_triangles, (-1, 9)) if (dynamic_triangles is not None and not indexed)
+                else np.empty((0, 9), np.float32))
+        tri_count = len(idx) if indexed else len(soup)
+        signature, topology_changed = None, False
+        if update_dynamic and indexed:
+            signature = ((len(verts), tri_count, "provided", dynamic_triangle_topology_signature)
+                         if dynamic_triangle_topology_signature is not None
+                         else (len(verts), tri_count, content_signature(idx)))
+            topology_changed = self._dynamic_triangle_signature != signature
+        elif update_dynamic:
+            signature = (tri_count, content_signature(soup))
+            if signature == self._dynamic_triangle_signature:
+                update_dynamic = False
+        part = dynamic_particles or {}
+        ppos = _f32(part.get("positions", np.empty((0, 3), np.float32)), (-1, 3))
+        prad = _f32(part.get("radii", np.empty(0, np.float32))) if len(ppos) else np.empty(0, np.float32)
+        if len(ppos) and len(prad) != len(ppos):
+            raise NativeSolverError("Dynamic particle arrays must have matching lengths.")
+        psig = None
+        if update_dynamic_particles:
+            psig = (("provided", part["signature"]) if part.get("signature") is not None
+                    else (len(ppos), content_signature(ppos), content_signature(prad)))
+            if psig == self._dynamic_particle_signature:
+                update_dynamic_particles = False
+        if not (update_pin or update_runtime or update_static or update_dynamic or update_dynamic_particles or update_force_fields):
+            return False
+        static = _f32(static_triangles if static_triangles is not None else np.empty((0, 9), np.float32), (-1, 9))
+        fields = list(force_fields or [])
+        field_arr = (abi.ForceField * max(len(fields), 1))()
+        for k, f in enumerate(fields):
+            for key, value in f.items():
+                cur = getattr(field_arr[k], key)
+                setattr(field_arr[k], key, (ctypes.c_float * 3)(*value) if hasattr(cur, "__len__") else value)
+        fi = abi.FrameInputs()
+        fi.update_pin_targets = int(update_pin)
+        fi.pin_indices, fi.pin_positions, fi.pin_weights, fi.pin_count = _iptr(pin_idx), _fptr(pin_pos), _fptr(pin_w), len(pin_idx)
+        fi.update_runtime_colliders = int(update_runtime and options is not None)
+        if options is not None:
+            fi.runtime_colliders = runtime_colliders_from_options(options)
+        fi.update_static_triangles = int(update_static)
+        fi.static_triangles, fi.static_triangle_count = _fptr(static), len(static)
+        fi.update_dynamic_triangles = int(update_dynamic and not indexed)
+        fi.dynamic_triangles, fi.dynamic_triangle_count = _fptr(soup), 0 if indexed else tri_count
+        fi.update_dynamic_indexed_triangles = int(update_dynamic and indexed)
+        fi.dynamic_triangle_vertices, fi.dynamic_triangle_vertex_count = _fptr(verts), len(verts)
+        fi.dynamic_triangle_indices = _iptr(idx)
+        fi.dynamic_indexed_triangle_count = tri_count if indexed else 0
+        fi.dynamic_triangle_topology_changed = int(topology_changed)
+        fi.update_dynamic_particles = int(update_dynamic_particles)
+        fi.dynamic_particle_positions, fi.dynamic_particle_radii, fi.dynamic_particle_count = _fptr(ppos), _fptr(prad), len(ppos)
+        fi.update_force_fields = int(update_force_fields)
+        fi.force_fields = ctypes.cast(field_arr, ctypes.POINTER(abi.ForceField))
+        fi.force_field_count = len(fields)
+        self._check(self._lib.ssbl_update_frame_inputs(self._handle, ctypes.byref(fi)))
+        if update_dynamic:
+            self._dynamic_triangle_signature = signature
+        if update_dynamic_particles:
+            self._dynamic_particle_signature = psig
+        return True
 
     def attach_global_dynamic_scene(self, scene, target_source_id: int = 0, target_collision_layer: int = 0,
                                     cross_mode: int = 1) -> None:

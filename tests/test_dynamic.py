@@ -184,3 +184,35 @@ def test_cloth_over_indexed_collider_runs(nb, oracle, scenes):
     inside = (np.abs(x) < 0.6) & (np.abs(y) < 0.6)
     dome = 0.05 + 0.004 * 11 + 0.04 * np.exp(-4.0 * (x ** 2 + y ** 2))
     assert np.min(pos[inside, 2] - dome[inside]) > -0.002
+
+
+def test_bridge_frame_inputs_signatures(nb, oracle, scenes):
+    """NativeXpbdSolver.update_frame_inputs (native_backend.py:1064-1279): an
+    unchanged soup or particle set is not sent again; indexed triangles with
+    the same topology keep the previous centres (collider velocity), a new
+    topology signature drops them."""
+    def vertex_solver():
+        cloth = scenes.build_cloth(np.array([[0.01, 0.02, 0.002]], np.float32), np.zeros((0, 3), np.int32), np.zeros(1, bool))
+        opts = scenes.SolverOptions(collision_margin=sd.SIGNED_MARGIN, cloth_thickness=sd.SIGNED_THICKNESS, gravity=(0.0, 0.0, 0.0))
+        return nb.NativeXpbdSolver(cloth, opts, None, lib=oracle)
+
+    s = vertex_solver()
+    soup = sd.plane_soup(-0.5, 1.0, 1)
+    assert s.update_frame_inputs(dynamic_triangles=soup.reshape(-1, 3, 3), update_dynamic=True)
+    assert not s.update_frame_inputs(dynamic_triangles=soup.reshape(-1, 3, 3), update_dynamic=True)
+    parts = {"positions": np.array([[5.0, 5.0, 5.0]], np.float32), "radii": np.array([0.01], np.float32)}
+    assert s.update_frame_inputs(dynamic_particles=parts, update_dynamic_particles=True)
+    assert not s.update_frame_inputs(dynamic_particles=parts, update_dynamic_particles=True)
+
+    for signature, expect_v in ((None, 0.002 * 60.0), ("changed", 0.0)):
+        s = vertex_solver()
+        v0, idx = sd.indexed_plane(-0.002, 1.0, 1)
+        v1, _ = sd.indexed_plane(0.0, 1.0, 1)
+        assert s.update_frame_inputs(dynamic_triangle_vertices=v0, dynamic_triangle_indices=idx, update_dynamic=True,
+                                     dynamic_triangle_topology_signature=None if signature is None else "a")
+        assert s.update_frame_inputs(dynamic_triangle_vertices=v1, dynamic_triangle_indices=idx, update_dynamic=True,
+                                     dynamic_triangle_topology_signature=None if signature is None else "b")
+        s.step(1, 1)
+        p, q, v = s.download_state()
+        assert abs(float(p[0, 2]) - sd.signed_plane_expected(0.002)) < 2e-7
+        assert abs(float(v[0, 2]) - expect_v) < 1e-4
