@@ -360,3 +360,46 @@ def config3(seed: int = 0, count: int = 64, frames: int = 120, self_collision: b
     box = box_triangles((0.0, 0.0, 0.3), 0.3)
     return Scene("batched_64_uneven", cloths, opts, box, 10, 8, frames, layers if cross_cloth else None,
                  0.005 if cross_cloth else 0.0)
+
+
+def config4(seed: int = 0, subdivisions: int = 8, towel_n: int = 1000, frames: int = 120) -> Scene:
+    """Large closed-mesh stress case: an icosphere balloon (subdivision 8:
+    655,362 vertices, 1,310,720 triangles, r = 1 m) with volume pressure 1.2,
+    and a towel_n x towel_n grid towel (2 m square) twisted by its two end rows,
+    which are pinned and turn in opposite directions at 1 rev/s about the
+    towel's long axis (`towel_pin_targets`).  Strict self collision on both,
+    hard-stretch PCG at strength 1.0 (polish and caps active), 20 substeps x 10
+    iterations, seed-jittered starts."""
+    pos, tris = icosphere(subdivisions, 1.0)
+    pos = pos + np.array([3.0, 0.0, 1.5])
+    balloon = build_cloth(pos, tris, np.zeros(len(pos), bool), jitter=1.0e-4, seed=seed * 1000 + 1)
+    tpos, ttris = grid_mesh(towel_n, 2.0)
+    tpos[:, 1] += 1.5
+    rows = np.arange(towel_n * towel_n) // towel_n
+    ends = (rows == 0) | (rows == towel_n - 1)
+    towel = build_cloth(y_up_to_z_up(tpos), ttris, ends, jitter=1.0e-4, seed=seed * 1000 + 2)
+    common = dict(collision_margin=0.005, cloth_thickness=0.005, self_collision=True, self_collision_mode=2,
+                  self_collision_interval=1, fast_self_collision_passes=4, max_self_collision_neighbors=32,
+                  stretch_optimization_enabled=True, stretch_optimization_strength=1.0)
+    o_balloon = SolverOptions(use_volume_pressure=True, pressure_strength=1.2, **common)
+    o_towel = SolverOptions(**common)
+    return Scene("stress_balloon_towel", [balloon, towel], [o_balloon, o_towel], np.empty((0, 9), np.float32), 20, 10, frames)
+
+
+def towel_pin_targets(towel: ClothBuildData, t: float) -> np.ndarray:
+    """Targets of the towel's pinned end rows at time t: each row turns about
+    the towel's long axis (through the two rows' centres) at 1 rev/s, the two
+    rows in opposite directions."""
+    rest = towel.positions_world[towel.pin_indices].astype(np.float64)
+    centre = towel.positions_world.astype(np.float64).mean(axis=0)
+    axis = np.array([0.0, 1.0, 0.0])   # rows run along x; the long axis is y (z-up world)
+    sign = np.where(rest[:, 1] < centre[1], 1.0, -1.0)
+    ang = 2.0 * math.pi * t * sign
+    rel = rest - centre
+    c, s_ = np.cos(ang), np.sin(ang)
+    # rotation about the y axis
+    x = c * rel[:, 0] + s_ * rel[:, 2]
+    z = -s_ * rel[:, 0] + c * rel[:, 2]
+    out = np.stack([x, rel[:, 1], z], 1) + centre
+    return out.astype(np.float32)
+

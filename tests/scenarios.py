@@ -323,3 +323,26 @@ def pcg_exact_solution(start, inv_mass, rests, strength):
     x = np.zeros(3 * n)
     x[free] = np.linalg.solve(A[np.ix_(free, free)], b[free])
     return p + x.reshape(n, 3)
+
+
+# --- the large closed-mesh stress case (balloon + twisted towel)
+def run_stress(nb, lib, scenes, sc, frames, *, batched):
+    """Steps the balloon and the towel `frames` frames, turning the towel's end
+    rows before every frame; batched (one launch per phase for both) or each
+    handle alone.  Returns both cloths' states and diagnostics."""
+    solvers = [nb.NativeXpbdSolver(c, o, None, lib=lib) for c, o in zip(sc.cloths, sc.options)]
+    towel = sc.cloths[1]
+    batch = nb.SolverBatch(solvers) if batched else None
+    dt = sc.options[1].dt
+    for f in range(frames):
+        print(f"stress frame {f} ({'batched' if batched else 'alone'}, {sc.vertex_count} vertices)", flush=True)
+        solvers[1].update_pin_targets(towel.pin_indices, scenes.towel_pin_targets(towel, (f + 1) * dt), towel.pin_weights)
+        if batched:
+            batch.step(sc.substeps, sc.iterations, synchronize=True)
+        else:
+            for s in solvers:
+                s.step(sc.substeps, sc.iterations, diagnostics=False)
+    if batch is not None:
+        batch.close()
+    states = [s.download_state() for s in solvers]
+    return solvers, states

@@ -51,3 +51,24 @@ def test_reference_smoke_binary_on_oracle():
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120).stdout
     for marker in ("HARD_STRETCH_OK", "OVERPRESSURE_OK", "TACK_BENDING_OK", "STATIC_SDF_OK", "PIN_WEIGHT_OK", "NATIVE_OK"):
         assert f"SSBL_ABI41_{marker}" in out, out
+
+
+def test_stress_case_reduced_on_oracle(nb, oracle, scenes):
+    """The balloon + twisted towel scene at small size: the towel's pinned rows
+    sit on their turning targets, batched and alone agree, everything stays
+    finite and the balloon keeps its size (pressure against the hard-stretch
+    caps makes it pulse by about 15 %)."""
+    sc = scenes.config4(subdivisions=2, towel_n=12, frames=2)
+    solvers, alone = S.run_stress(nb, oracle, scenes, sc, 2, batched=False)
+    _, batched = S.run_stress(nb, oracle, scenes, sc, 2, batched=True)
+    for a, b in zip(alone, batched):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    towel = sc.cloths[1]
+    target = scenes.towel_pin_targets(towel, 2 * sc.options[1].dt)
+    assert np.array_equal(alone[1][0][towel.pin_indices], target)
+    assert all(np.all(np.isfinite(x)) for st in alone for x in st)
+    # the balloon keeps its size to within the pressure/stretch oscillation
+    balloon = alone[0][0]
+    r = np.linalg.norm(balloon - balloon.mean(axis=0), axis=1).mean()
+    assert 0.7 < r < 1.3
