@@ -48,8 +48,7 @@ __global__ void k_fill_i32(int* p, int value, int n) {
 
 // A41:3694-3760: AABB, centre, bounding radius and (hashed sets) the cell
 // span with the margin; a span of more than 64 cells goes to the large list.
-__global__ __launch_bounds__(256) void k_bounds(const DynView* __restrict__ dv, float span_margin, float4* lo, float4* hi,
-                                                float4* ctr, int4* cmin, int4* cmax, int* large_flag,
+__global__ __launch_bounds__(256) void k_bounds(const DynView* __restrict__ dv, float span_margin, TriBox* box, int* large_flag,
                                                 unsigned long long* counts, int count_diag) {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int nt = dv->nt;
@@ -60,9 +59,7 @@ __global__ __launch_bounds__(256) void k_bounds(const DynView* __restrict__ dv, 
         const F3 mx = f3(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)));
         const F3 cen = mul((a + b) + c, 1.0f / 3.0f);
         const float r = sqrtf(fmaxf(fmaxf(dot(a - cen, a - cen), dot(b - cen, b - cen)), dot(c - cen, c - cen)));
-        lo[t] = with_w(mn, 0.0f);
-        hi[t] = with_w(mx, 0.0f);
-        ctr[t] = with_w(cen, r);
+        int4 c0 = make_int4(0, 0, 0, 0), c1 = make_int4(0, 0, __float_as_int(mn.x), __float_as_int(mn.y));
         if (dv->tri_hash) {
             const float cs = fmaxf(dv->tri_cell, 1.0e-3f), m = span_margin;
             int x0 = cell_coord(mn.x - m, cs), y0 = cell_coord(mn.y - m, cs), z0 = cell_coord(mn.z - m, cs);
@@ -75,13 +72,25 @@ __global__ __launch_bounds__(256) void k_bounds(const DynView* __restrict__ dv, 
                 z0 = z1 = cell_coord(cen.z, cs);
                 cells = 1;
             }
-            cmin[t] = make_int4(x0, y0, z0, 0);
-            cmax[t] = make_int4(x1, y1, z1, 0);
+            c0 = make_int4(x0, y0, z0, x1);
+            c1.x = y1;
+            c1.y = z1;
             large = cells > kDynMaxHashCells;
         }
+        TriBox bx;
+        bx.c0 = c0;
+        bx.c1 = c1;
+        bx.b0 = make_float4(mn.z, mx.x, mx.y, mx.z);
+        bx.s = with_w(cen, r);
+        box[t] = bx;
         large_flag[t] = large ? 1 : 0;
     }
     if (count_diag) wave_count(counts, kCountDynTriLarge, large);
+}
+
+__global__ void k_box_centres(const TriBox* __restrict__ box, float4* ctr, int n) {
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t < n) ctr[t] = box[t].s;
 }
 
 __global__ void k_large_scatter(int nt, const int* __restrict__ flag, const int* __restrict__ off, int* large) {
@@ -102,27 +111,65 @@ __device__ __forceinline__ bool span_cell(int4 a, int4 b, int idx, int* x, int* 
 
 constexpr int kTriLanes = 16;   // lanes per triangle in the hash builds (cells beyond 16 loop)
 
-// kTriLanes lanes per triangle, lane l taking cells l, l + 16, ...  phase 0
-// counts (keeping the arrival slot), 1 scatters keys to raw[off[b] + slot].
-__global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv, const int4* __restrict__ cmin,
-                                                  const int4* __restrict__ cmax, const int* __restrict__ large_flag, int* count,
-                                                  const int* __restrict__ off, int* slot, int* raw, int* rawb, int phase) {
+// kTriLanes lanes per triangle, lane l taking cells l, l + 16, ... (16
+// triangles per block).  Phase 0 counts the entries of every bucket; phase 1
+// counts them down again, each entry taking the place its decrement returns
+// in the bucket's segment: raw[off[b] + slot] = {key, b}, in arrival order
+// until ranked.  Neighbouring triangles share most of their cells, so a
+// block first gathers its entries per bucket in an LDS table and makes one
+// global atomic per distinct bucket (device-scope atomics are the cost).
+constexpr int kAggSlots = 2048;   // >= twice the block's 16 x 64 entries: probing ends
+__global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv, const TriBox* __restrict__ box,
+                                                  const int* __restrict__ large_flag, int* count, const int* __restrict__ off,
+                                                  int2* raw, int phase) {
+    __shared__ int s_bucket[kAggSlots];
+    __shared__ int s_count[kAggSlots];
+    __shared__ int s_base[kAggSlots];
+    for (int i = static_cast<int>(threadIdx.x); i < kAggSlots; i += static_cast<int>(blockDim.x)) {
+        s_bucket[i] = -1;
+        s_count[i] = 0;
+    }
+    __syncthreads();
     const int t = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kTriLanes);
     const int lane = static_cast<int>(threadIdx.x % kTriLanes);
-    if (t >= dv->nt || large_flag[t]) return;
-    const int4 lo = cmin[t], hi = cmax[t];
-    for (int idx = lane; idx < kDynMaxHashCells; idx += kTriLanes) {
-        int x, y, z;
-        if (!span_cell(lo, hi, idx, &x, &y, &z)) return;
-        const int b = static_cast<int>(hash_cell(x, y, z, dv->tri_buckets));
-        const int key = t * kDynMaxHashCells + idx;
-        if (phase == 0) {
-            slot[key] = atomicAdd(&count[b], 1);
-        } else {
-            const int j = off[b] + slot[key];
-            raw[j] = key;
-            rawb[j] = b;
+    constexpr int kRounds = kDynMaxHashCells / kTriLanes;
+    int e_slot[kRounds], e_local[kRounds], e_key[kRounds], e_bucket[kRounds];
+    int n = 0;
+    if (t < dv->nt && !large_flag[t]) {
+        const TriBox bx = box[t];
+        const int4 lo = box_cmin(bx), hi = box_cmax(bx);
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            const int idx = lane + r * kTriLanes;
+            int x, y, z;
+            if (!span_cell(lo, hi, idx, &x, &y, &z)) break;
+            const int b = static_cast<int>(hash_cell(x, y, z, dv->tri_buckets));
+            int h = b & (kAggSlots - 1);
+            for (;;) {
+                const int old = atomicCAS(&s_bucket[h], -1, b);
+                if (old == -1 || old == b) break;
+                h = (h + 1) & (kAggSlots - 1);
+            }
+            e_slot[r] = h;
+            e_local[r] = atomicAdd(&s_count[h], 1);
+            e_key[r] = t * kDynMaxHashCells + idx;
+            e_bucket[r] = b;
+            n = r + 1;
         }
+    }
+    __syncthreads();
+    for (int i = static_cast<int>(threadIdx.x); i < kAggSlots; i += static_cast<int>(blockDim.x)) {
+        const int b = s_bucket[i];
+        if (b < 0) continue;
+        if (phase == 0) atomicAdd(&count[b], s_count[i]);
+        else s_base[i] = atomicSub(&count[b], s_count[i]) - s_count[i];
+    }
+    if (phase == 0) return;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        if (r >= n) break;
+        raw[off[e_bucket[r]] + s_base[e_slot[r]] + e_local[r]] = make_int2(e_key[r], e_bucket[r]);
     }
 }
 
@@ -136,25 +183,25 @@ __device__ __forceinline__ int block_max(int m) {
 }
 
 // Largest bucket (diagnostics): one atomic per block.
-__global__ __launch_bounds__(256) void k_bucket_max(const int* __restrict__ count, int buckets, int* out) {
+__global__ __launch_bounds__(256) void k_bucket_max(const int* __restrict__ off, int buckets, int* out) {
     int m = 0;
     for (int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); b < buckets; b += static_cast<int>(gridDim.x * blockDim.x))
-        m = max(m, count[b]);
+        m = max(m, off[b + 1] - off[b]);
     m = block_max(m);
     if (threadIdx.x == 0 && m > 0) atomicMax(out, m);
 }
 
 // Every scattered key lands at its rank among its bucket's keys, with its
 // triangle's {source, layer} beside it (grid-stride over the entries).
-__global__ __launch_bounds__(256) void k_tri_rank(const int* __restrict__ off, int buckets, const int* __restrict__ count,
-                                                  const int* __restrict__ raw, const int* __restrict__ rawb, int* keys,
+__global__ __launch_bounds__(256) void k_tri_rank(const int* __restrict__ off, int buckets, const int2* __restrict__ raw, int* keys,
                                                   const int* __restrict__ tsrc, const int* __restrict__ tlayer, int2* ksl) {
     const int total = off[buckets];
     for (int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); j < total; j += static_cast<int>(gridDim.x * blockDim.x)) {
-        const int b = rawb[j], key = raw[j];
-        const int base = off[b], n = count[b];
+        const int2 e = raw[j];
+        const int b = e.y, key = e.x;
+        const int base = off[b], n = off[b + 1] - base;
         int rank = 0;
-        for (int k = 0; k < n; ++k) rank += raw[base + k] < key ? 1 : 0;
+        for (int k = 0; k < n; ++k) rank += raw[base + k].x < key ? 1 : 0;
         keys[base + rank] = key;
         if (tsrc) {
             const int t = key / kDynMaxHashCells;
@@ -168,10 +215,10 @@ __global__ __launch_bounds__(256) void k_tri_rank(const int* __restrict__ off, i
 // differ, -2 when the bucket is empty)}.  A query whose filter rejects every
 // entry of a bucket skips it (A41:3984-3997 filters each entry the same way,
 // so results are equal).  Sets without sources get layer INT_MIN, source -1.
-__global__ __launch_bounds__(256) void k_bucket_bins(const int* __restrict__ off, const int* __restrict__ count, int buckets,
-                                                     int cap, const int2* __restrict__ ksl, int4* bins) {
+__global__ __launch_bounds__(256) void k_bucket_bins(const int* __restrict__ off, int buckets, int cap, const int2* __restrict__ ksl,
+                                                     int4* bins) {
     for (int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); b < buckets; b += static_cast<int>(gridDim.x * blockDim.x)) {
-        const int n = count[b], base = off[b];
+        const int base = off[b], n = off[b + 1] - base;
         int min_layer = ksl ? 0x7fffffff : static_cast<int>(0x80000000u), src = -2;
         if (ksl) {
             for (int k = 0; k < min(n, cap); ++k) {
@@ -356,22 +403,16 @@ bool DynamicColliders::reserve_triangles(int n) {
     tv_ = static_cast<float4*>(alloc(0, sizeof(float4) * 3 * m));
     tsrc_ = static_cast<int*>(alloc(0, sizeof(int) * m));
     tlayer_ = static_cast<int*>(alloc(0, sizeof(int) * m));
-    lo_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
-    hi_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
-    ctr_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
+    box_ = static_cast<TriBox*>(alloc(0, sizeof(TriBox) * m));
     pctr_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
     bounds_valid_ = false;
-    cmin_ = static_cast<int4*>(alloc(0, sizeof(int4) * m));
-    cmax_ = static_cast<int4*>(alloc(0, sizeof(int4) * m));
     large_flag_ = static_cast<int*>(alloc(0, sizeof(int) * m));
     large_off_ = static_cast<int*>(alloc(0, sizeof(int) * (m + 1)));
     large_ = static_cast<int*>(alloc(0, sizeof(int) * m));
     const int buckets = next_pow2(std::max(kDynMinBuckets, n * 4));
     tri_count_ = static_cast<int*>(alloc(0, sizeof(int) * buckets));
     tri_off_ = static_cast<int*>(alloc(0, sizeof(int) * (buckets + 1)));
-    tri_slot_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
-    tri_raw_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
-    tri_rawb_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
+    tri_raw_ = static_cast<int2*>(alloc(0, sizeof(int2) * m * kDynMaxHashCells));
     tri_keys_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
     tri_ksl_ = static_cast<int2*>(alloc(0, sizeof(int2) * m * kDynMaxHashCells));
     tri_bin_ = static_cast<int4*>(alloc(0, sizeof(int4) * buckets));
@@ -383,8 +424,8 @@ bool DynamicColliders::reserve_triangles(int n) {
         scan_tmp_ = static_cast<int*>(alloc(2, sizeof(int) * scan_tmp_ints(need_scan)));
         scan_cap_ = scan_tmp_ ? need_scan : 0;
     }
-    if (!tv_ || !tsrc_ || !tlayer_ || !lo_ || !hi_ || !ctr_ || !cmin_ || !cmax_ || !large_flag_ || !large_off_ || !large_ ||
-        !tri_count_ || !tri_off_ || !tri_slot_ || !tri_raw_ || !tri_rawb_ || !tri_keys_ || !tri_ksl_ || !tri_bin_ || !occupancy_ || !extent_bits_ || !scan_tmp_)
+    if (!tv_ || !tsrc_ || !tlayer_ || !box_ || !pctr_ || !large_flag_ || !large_off_ || !large_ || !tri_count_ || !tri_off_ ||
+        !tri_raw_ || !tri_keys_ || !tri_ksl_ || !tri_bin_ || !occupancy_ || !extent_bits_ || !scan_tmp_)
         return false;
     tri_cap_ = n;
     tri_buckets_ = buckets;
@@ -442,8 +483,8 @@ bool DynamicColliders::upload_view(hipStream_t st) {
     v.tv = tv_;
     v.tsrc = tri_filter_ ? tsrc_ : nullptr;
     v.tlayer = tri_filter_ ? tlayer_ : nullptr;
-    v.lo = lo_; v.hi = hi_; v.ctr = ctr_; v.cmin = cmin_; v.cmax = cmax_;
-    v.tri_off = tri_off_; v.tri_keys = tri_keys_; v.tri_ksl = tri_filter_ ? tri_ksl_ : nullptr; v.large = large_;
+    v.box = box_;
+    v.tri_keys = tri_keys_; v.tri_ksl = tri_filter_ ? tri_ksl_ : nullptr; v.large = large_;
     v.tri_bin = tri_bin_;
     v.nlarge = large_off_ ? large_off_ + nt_ : nullptr;
     v.np = np_;
@@ -516,8 +557,8 @@ bool DynamicColliders::set_triangles(hipStream_t st, const float* soup, int n, c
 // last upload (the signed path keeps them as the previous centres).
 bool DynamicColliders::compute_bounds(hipStream_t st) {
     if (nt_ <= 0) return true;
-    launch_k(k_bounds, dim3(grid_for(nt_, 256)), dim3(256), 0, st, static_cast<const DynView*>(d_view_), span_margin_, lo_, hi_, ctr_,
-             cmin_, cmax_, large_flag_, static_cast<unsigned long long*>(nullptr), 0);
+    launch_k(k_bounds, dim3(grid_for(nt_, 256)), dim3(256), 0, st, static_cast<const DynView*>(d_view_), span_margin_, box_, large_flag_,
+             static_cast<unsigned long long*>(nullptr), 0);
     bounds_valid_ = true;
     return hip_ok(hipGetLastError(), "dynamic triangle bounds");
 }
@@ -550,8 +591,7 @@ bool DynamicColliders::set_indexed(hipStream_t st, const float* verts, int nvert
     }
     const bool had_previous = !topology_changed && nt_ == ntri && bounds_valid_ && tri_cap_ >= ntri;
     if (!reserve_triangles(ntri)) return false;
-    if (had_previous && !hip_ok(hipMemcpy(pctr_, ctr_, sizeof(float4) * ntri, hipMemcpyDeviceToDevice), "dynamic previous centres"))
-        return false;
+    if (had_previous) launch_k(k_box_centres, dim3(grid_for(ntri, 256)), dim3(256), 0, st, static_cast<const TriBox*>(box_), pctr_, ntri);
     if (!hip_ok(hipMemcpy(tv_, h.data(), sizeof(float4) * h.size(), hipMemcpyHostToDevice), "dynamic indexed triangle upload"))
         return false;
     margin = std::max(margin, 0.0f);
@@ -562,9 +602,8 @@ bool DynamicColliders::set_indexed(hipStream_t st, const float* verts, int nvert
     signed_ = true;
     tri_motion_ = had_previous;
     if (!upload_view(st) || !compute_bounds(st)) return false;
-    if (!had_previous && !hip_ok(hipMemcpyAsync(pctr_, ctr_, sizeof(float4) * ntri, hipMemcpyDeviceToDevice, st), "dynamic previous centres"))
-        return false;
-    return true;
+    if (!had_previous) launch_k(k_box_centres, dim3(grid_for(ntri, 256)), dim3(256), 0, st, static_cast<const TriBox*>(box_), pctr_, ntri);
+    return hip_ok(hipGetLastError(), "dynamic previous centres");
 }
 
 bool DynamicColliders::clear_motion(hipStream_t st) {
@@ -686,7 +725,7 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
     TimedLaunch t("dynamic_hash", st);
     if (nt_ > 0) {
         const dim3 g(grid_for(nt_, 256)), blk(256);
-        launch_k(k_bounds, g, blk, 0, st, static_cast<const DynView*>(d_view_), span_margin_, lo_, hi_, ctr_, cmin_, cmax_, large_flag_,
+        launch_k(k_bounds, g, blk, 0, st, static_cast<const DynView*>(d_view_), span_margin_, box_, large_flag_,
                  counts, cd);
         exclusive_scan(st, large_flag_, nt_, large_off_, scan_tmp_);
         launch_k(k_large_scatter, g, blk, 0, st, nt_, static_cast<const int*>(large_flag_), static_cast<const int*>(large_off_), large_);
@@ -694,24 +733,21 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
             const int buckets = next_pow2(std::max(kDynMinBuckets, nt_ * 4));
             const dim3 gw(grid_for(static_cast<long long>(nt_) * kTriLanes, 256));
             launch_k(k_fill_i32, dim3(grid_for(buckets, 256)), blk, 0, st, tri_count_, 0, buckets);
-            launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const int4*>(cmin_),
-                     static_cast<const int4*>(cmax_), static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_),
-                     tri_slot_, tri_raw_, tri_rawb_, 0);
+            launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
+                     static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, 0);
             exclusive_scan(st, tri_count_, buckets, tri_off_, scan_tmp_);
-            launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const int4*>(cmin_),
-                     static_cast<const int4*>(cmax_), static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_),
-                     tri_slot_, tri_raw_, tri_rawb_, 1);
+            launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
+                     static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, 1);
             const dim3 gr(std::min<unsigned>(grid_for(static_cast<long long>(nt_) * 16, 256), 16384u));
-            launch_k(k_tri_rank, gr, blk, 0, st, static_cast<const int*>(tri_off_), buckets, static_cast<const int*>(tri_count_),
-                     static_cast<const int*>(tri_raw_), static_cast<const int*>(tri_rawb_), tri_keys_,
+            launch_k(k_tri_rank, gr, blk, 0, st, static_cast<const int*>(tri_off_), buckets, static_cast<const int2*>(tri_raw_), tri_keys_,
                      static_cast<const int*>(tri_filter_ ? tsrc_ : nullptr), static_cast<const int*>(tlayer_), tri_ksl_);
             launch_k(k_bucket_bins, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
-                     static_cast<const int*>(tri_off_), static_cast<const int*>(tri_count_), buckets, 0x7fffffff,
-                     static_cast<const int2*>(tri_filter_ ? tri_ksl_ : nullptr), tri_bin_);
+                     static_cast<const int*>(tri_off_), buckets, 0x7fffffff, static_cast<const int2*>(tri_filter_ ? tri_ksl_ : nullptr),
+                     tri_bin_);
             if (count_diag && occupancy) {
                 launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, occupancy_, 0, 1);
                 launch_k(k_bucket_max, dim3(std::min<unsigned>(grid_for(buckets, 256), 4096u)), blk, 0, st,
-                         static_cast<const int*>(tri_count_), buckets, occupancy_);
+                         static_cast<const int*>(tri_off_), buckets, occupancy_);
             }
         }
     }
@@ -726,8 +762,8 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
                      static_cast<const int*>(player_), counts, cd, phase);
         }
         launch_k(k_bucket_bins, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
-                 static_cast<const int*>(poff_), static_cast<const int*>(pcount_), buckets, kDynParticleSlots,
-                 static_cast<const int2*>(part_filter_ ? pksl_ : nullptr), pbin_);
+                 static_cast<const int*>(poff_), buckets, kDynParticleSlots, static_cast<const int2*>(part_filter_ ? pksl_ : nullptr),
+                 pbin_);
         launch_k(k_coarse_fill, dim3(grid_for(coarse_buckets_, 256)), blk, 0, st, coarse_, coarse_buckets_, coarse_off_);
         launch_k(k_coarse_insert, g, blk, 0, st, static_cast<const DynView*>(d_view_), coarse_, coarse_off_);
     }

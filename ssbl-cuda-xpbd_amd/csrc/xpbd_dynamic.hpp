@@ -33,6 +33,21 @@ enum DynCountSlot {
     kCountDynTriCandidates = 34, kCountDynTriBucketOverflow = 35, kCountDynTriLarge = 36, kCountDynTriAabbRejects = 37
 };
 
+// One triangle's bounds in one 64-byte record, so that a query's canonical
+// cell test, box test and sphere test read one memory segment: the hashed
+// cell span (cmin, cmax), the box (lo, hi), the centre and the bounding radius.
+struct __align__(64) TriBox {
+    int4 c0;     // cmin.x, cmin.y, cmin.z, cmax.x
+    int4 c1;     // cmax.y, cmax.z, lo.x bits, lo.y bits
+    float4 b0;   // lo.z, hi.x, hi.y, hi.z
+    float4 s;    // centre xyz, bounding radius
+};
+
+__device__ __forceinline__ int4 box_cmin(const TriBox& b) { return make_int4(b.c0.x, b.c0.y, b.c0.z, 0); }
+__device__ __forceinline__ int4 box_cmax(const TriBox& b) { return make_int4(b.c0.w, b.c1.x, b.c1.y, 0); }
+__device__ __forceinline__ float4 box_lo(const TriBox& b) { return make_float4(__int_as_float(b.c1.z), __int_as_float(b.c1.w), b.b0.x, 0.0f); }
+__device__ __forceinline__ float4 box_hi(const TriBox& b) { return make_float4(b.b0.y, b.b0.z, b.b0.w, 0.0f); }
+
 // What the contact pass reads; lives in device memory (one per collider set).
 struct DynView {
     int nt;
@@ -41,12 +56,7 @@ struct DynView {
     const float4* tv;        // 3 per triangle (.w unused)
     const int* tsrc;         // null: no source filter
     const int* tlayer;
-    const float4* lo;        // AABB (.w unused)
-    const float4* hi;
-    const float4* ctr;       // centre, .w = bounding radius
-    const int4* cmin;        // hashed cell span (hashed sets only)
-    const int4* cmax;
-    const int* tri_off;      // buckets + 1
+    const TriBox* box;       // bounds of every triangle (cell span only in hashed sets)
     const int* tri_keys;     // triangle * 64 + cell index, ascending per bucket
     const int2* tri_ksl;     // {source id, layer} of each key's triangle (filtered sets)
     const int4* tri_bin;     // per bucket: {offset, count, min layer, the one source of all entries | -1 mixed | -2 empty}
@@ -146,10 +156,10 @@ private:
     int* tsrc_ = nullptr;
     int* tlayer_ = nullptr;
     bool tri_filter_ = false;
-    float4 *lo_ = nullptr, *hi_ = nullptr, *ctr_ = nullptr;
-    int4 *cmin_ = nullptr, *cmax_ = nullptr;
+    TriBox* box_ = nullptr;
     int *large_flag_ = nullptr, *large_off_ = nullptr, *large_ = nullptr;
-    int *tri_count_ = nullptr, *tri_off_ = nullptr, *tri_slot_ = nullptr, *tri_raw_ = nullptr, *tri_rawb_ = nullptr, *tri_keys_ = nullptr;
+    int *tri_count_ = nullptr, *tri_off_ = nullptr, *tri_keys_ = nullptr;
+    int2* tri_raw_ = nullptr;   // {key, bucket} in arrival order
     int2* tri_ksl_ = nullptr;
     int4* tri_bin_ = nullptr;
     int* occupancy_ = nullptr;

@@ -261,14 +261,14 @@ __device__ __forceinline__ bool dyn_bucket_rejected(const ClothView& c, int4 bin
 // One dynamic triangle against point p (A41:3999-4323 per candidate): counts
 // the candidate, the AABB and bounding-sphere rejections, and returns the
 // contact depth (0 when out of reach) with the push along the contact normal.
-__device__ __forceinline__ float dyn_triangle_depth(const DynView& D, int t, F3 p, float cm, unsigned& cand, unsigned& rej,
-                                                    F3& delta) {
+__device__ __forceinline__ float dyn_triangle_depth(const DynView& D, int t, const TriBox& bx, F3 p, float cm, unsigned& cand,
+                                                    unsigned& rej, F3& delta) {
     ++cand;
-    const float4 mn = D.lo[t], mx = D.hi[t];
+    const float4 mn = box_lo(bx), mx = box_hi(bx);
     const F3 q0 = f3(fminf(fmaxf(p.x, mn.x), mx.x), fminf(fmaxf(p.y, mn.y), mx.y), fminf(fmaxf(p.z, mn.z), mx.z));
     const F3 da = p - q0;
     if (dot(da, da) > cm * cm) { ++rej; return 0.0f; }
-    const float4 ce = D.ctr[t];
+    const float4 ce = bx.s;
     const float rr = ce.w + cm;
     const F3 dc = p - f3(ce);
     if (dot(dc, dc) > rr * rr) { ++rej; return 0.0f; }
@@ -400,10 +400,11 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                     }
                     const int t = D.tri_keys[k] / kDynMaxHashCells;
                     if (t < 0 || t >= nt) continue;
-                    const int4 lo = D.cmin[t], hi = D.cmax[t];
+                    const TriBox bx = D.box[t];
+                    const int4 lo = box_cmin(bx), hi = box_cmax(bx);
                     if (cx < lo.x || cx > hi.x || cy < lo.y || cy > hi.y || cz < lo.z || cz > hi.z) continue;
                     F3 d;
-                    const float pen = dyn_triangle_depth(D, t, s.p, cm, cand, rej, d);
+                    const float pen = dyn_triangle_depth(D, t, bx, s.p, cm, cand, rej, d);
                     if (pen > best_pen) { best_pen = pen; best_delta = d; }
                 }
             }
@@ -426,10 +427,11 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                     }
                     const int t = D.tri_keys[k] / kDynMaxHashCells;
                     if (t < 0 || t >= nt) continue;
-                    const int4 lo = D.cmin[t], hi = D.cmax[t];
+                    const TriBox bx = D.box[t];
+                    const int4 lo = box_cmin(bx), hi = box_cmax(bx);
                     if (lx < lo.x || lx > hi.x || ly < lo.y || ly > hi.y || lz < lo.z || lz > hi.z) continue;
                     F3 d;
-                    const unsigned long long kk = depth_key(dyn_triangle_depth(D, t, lp, cm, lc, lr, d), static_cast<unsigned>(e));
+                    const unsigned long long kk = depth_key(dyn_triangle_depth(D, t, bx, lp, cm, lc, lr, d), static_cast<unsigned>(e));
                     if (kk > key) { key = kk; delta = d; }
                 }
                 F3 wd = f3(0.0f, 0.0f, 0.0f);
@@ -448,7 +450,7 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                     if (t < 0 || t >= nt) continue;
                     if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) continue;
                     F3 d;
-                    const float pen = dyn_triangle_depth(D, t, s.p, cm, cand, rej, d);
+                    const float pen = dyn_triangle_depth(D, t, D.box[t], s.p, cm, cand, rej, d);
                     if (pen > best_pen) { best_pen = pen; best_delta = d; }
                 }
             }
@@ -456,7 +458,7 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
             for (int t = 0; t < nt; ++t) {
                 if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) continue;
                 F3 d;
-                const float pen = dyn_triangle_depth(D, t, s.p, cm, cand, rej, d);
+                const float pen = dyn_triangle_depth(D, t, D.box[t], s.p, cm, cand, rej, d);
                 if (pen > best_pen) { best_pen = pen; best_delta = d; }
             }
         }
@@ -991,14 +993,14 @@ __device__ inline void signed_contacts(const ClothView& c, VertexState& s, bool 
         F3 best_delta = f3(0.0f, 0.0f, 0.0f), best_vel = f3(0.0f, 0.0f, 0.0f), acc = f3(0.0f, 0.0f, 0.0f);
         float best_pen = 0.0f, best_dist = 3.402823466e+38f, accw = 0.0f;
         unsigned cand = 0, rej = 0;
-        auto consider = [&](int t) {
+        auto consider = [&](int t, const TriBox& bx) {
             if (t < 0 || t >= nt) return;
             ++cand;
-            const float4 mn = D.lo[t], mx = D.hi[t];
+            const float4 mn = box_lo(bx), mx = box_hi(bx);
             const F3 q0 = f3(fminf(fmaxf(p.x, mn.x), mx.x), fminf(fmaxf(p.y, mn.y), mx.y), fminf(fmaxf(p.z, mn.z), mx.z));
             const F3 da = p - q0;
             if (dot(da, da) > qm * qm) { ++rej; return; }
-            const float4 ce = D.ctr[t];
+            const float4 ce = bx.s;
             const float rr = ce.w + qm;
             const F3 dc = p - f3(ce);
             if (dot(dc, dc) > rr * rr) { ++rej; return; }
@@ -1048,15 +1050,19 @@ __device__ inline void signed_contacts(const ClothView& c, VertexState& s, bool 
                             const int t = D.tri_keys[k] / kDynMaxHashCells;
                             if (t < 0 || t >= nt) continue;
                             // A41:3934-3982: the triangle's canonical cell for this vertex
-                            const int4 lo = D.cmin[t], hi = D.cmax[t];
+                            const TriBox bx = D.box[t];
+                            const int4 lo = box_cmin(bx), hi = box_cmax(bx);
                             if (qx != min(max(cx, lo.x), hi.x) || qy != min(max(cy, lo.y), hi.y) || qz != min(max(cz, lo.z), hi.z)) continue;
-                            consider(t);
+                            consider(t, bx);
                         }
                     }
             const int nl = *D.nlarge;
-            for (int k = 0; k < nl; ++k) consider(D.large[k]);
+            for (int k = 0; k < nl; ++k) {
+                const int t = D.large[k];
+                if (t >= 0 && t < nt) consider(t, D.box[t]);
+            }
         } else {
-            for (int t = 0; t < nt; ++t) consider(t);
+            for (int t = 0; t < nt; ++t) consider(t, D.box[t]);
         }
         if (count && cand) atomicAdd(&c.counts[kCountDynTriCandidates], static_cast<unsigned long long>(cand));
         if (count && rej) atomicAdd(&c.counts[kCountDynTriAabbRejects], static_cast<unsigned long long>(rej));
