@@ -115,6 +115,12 @@ struct ClothView {
     int sdf_dim[3];
     float sdf_origin[3];
     float sdf_voxel;
+    // 1 / voxel and 1 / max(voxel, eps), divided once on the host (IEEE
+    // single precision both sides, so bit-equal to the per-sample division),
+    // and the largest grid coordinate of each axis, dim - 1, as a float.
+    float sdf_inv;
+    float sdf_inv_clamped;
+    float sdf_top[3];
     int need_normal;
     int contacts_on;              // analytic colliders or a static SDF present
     // The first ell_width incidences of every vertex again, slot-major

@@ -64,12 +64,12 @@ struct VertexState {
     F3 q_, vel_;
     bool q_have, q_dirty, v_have, v_dirty;
     __device__ F3 q() {
-        if (!q_have) { q_ = f3(c->prev[v]); q_have = true; }
+        if (!q_have) { q_ = f3(gload(c->prev + v)); q_have = true; }
         return q_;
     }
     __device__ void set_q(F3 x) { q_ = x; q_have = true; q_dirty = true; }
     __device__ F3 vel() {
-        if (!v_have) { vel_ = f3(c->vel[v]); v_have = true; }
+        if (!v_have) { vel_ = f3(gload(c->vel + v)); v_have = true; }
         return vel_;
     }
     __device__ void set_vel(F3 x) { vel_ = x; v_have = true; v_dirty = true; }
@@ -119,7 +119,7 @@ __device__ inline void analytic_contacts(const ClothView& c, VertexState& s, flo
     if (col.use_sphere && col.sphere_radius > 0.0f) {
         const F3 center = f3(col.sphere_center[0], col.sphere_center[1], col.sphere_center[2]);
         const F3 d = p - center;
-        const float dist = sqrtf(fmaxf(dot(d, d), kEps));
+        const float dist = sqrt_eps(dot(d, d));
         const float target = col.sphere_radius + margin;
         if (dist < target) {
             if (!hit) v = s.vel();
@@ -134,7 +134,7 @@ __device__ inline void analytic_contacts(const ClothView& c, VertexState& s, flo
     if (col.use_wall) {
         const F3 o = f3(col.wall_origin[0], col.wall_origin[1], col.wall_origin[2]);
         F3 n = f3(col.wall_normal[0], col.wall_normal[1], col.wall_normal[2]);
-        const float nl = sqrtf(fmaxf(dot(n, n), kEps));
+        const float nl = sqrt_eps(dot(n, n));
         n = mul(n, 1.0f / nl);
         const float gap = dot(p - o, n);
         if (gap < margin) {
@@ -153,44 +153,42 @@ __device__ inline void analytic_contacts(const ClothView& c, VertexState& s, flo
     }
 }
 
-// A41:1645-1691 trilinear sample; false outside the grid.
-__device__ inline bool sdf_sample(const ClothView& c, F3 p, float* out) {
+// Trilinear interpolation at grid coordinates (gx, gy, gz) inside the grid,
+// the corner order and lerps of A41:1645-1691 and A41:1693-1723.  Element
+// offsets are 32-bit: a grid holds at most INT_MAX voxels.
+__device__ __forceinline__ float sdf_trilinear(const ClothView& c, float gx, float gy, float gz) {
     const int dx = c.sdf_dim[0], dy = c.sdf_dim[1], dz = c.sdf_dim[2];
-    if (dx < 2 || dy < 2 || dz < 2 || c.sdf_voxel <= 0.0f) return false;
-    const float inv = 1.0f / c.sdf_voxel;
-    const float gx = (p.x - c.sdf_origin[0]) * inv, gy = (p.y - c.sdf_origin[1]) * inv, gz = (p.z - c.sdf_origin[2]) * inv;
-    if (gx < 0.0f || gy < 0.0f || gz < 0.0f || gx > static_cast<float>(dx - 1) || gy > static_cast<float>(dy - 1) ||
-        gz > static_cast<float>(dz - 1))
-        return false;
     const int x0 = min(max(static_cast<int>(floorf(gx)), 0), dx - 1), y0 = min(max(static_cast<int>(floorf(gy)), 0), dy - 1),
               z0 = min(max(static_cast<int>(floorf(gz)), 0), dz - 1);
     const int x1 = min(x0 + 1, dx - 1), y1 = min(y0 + 1, dy - 1), z1 = min(z0 + 1, dz - 1);
     const float tx = gx - static_cast<float>(x0), ty = gy - static_cast<float>(y0), tz = gz - static_cast<float>(z0);
-    const float* s = c.sdf;
-    auto at = [&](int x, int y, int z) { return s[(static_cast<size_t>(z) * dy + y) * dx + x]; };
+    const unsigned r00 = static_cast<unsigned>((z0 * dy + y0) * dx), r10 = static_cast<unsigned>((z0 * dy + y1) * dx),
+                   r01 = static_cast<unsigned>((z1 * dy + y0) * dx), r11 = static_cast<unsigned>((z1 * dy + y1) * dx);
+    const float* sdf = c.sdf;
     auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
-    const float c00 = lerp(at(x0, y0, z0), at(x1, y0, z0), tx), c10 = lerp(at(x0, y1, z0), at(x1, y1, z0), tx);
-    const float c01 = lerp(at(x0, y0, z1), at(x1, y0, z1), tx), c11 = lerp(at(x0, y1, z1), at(x1, y1, z1), tx);
-    *out = lerp(lerp(c00, c10, ty), lerp(c01, c11, ty), tz);
+    auto at = [&](unsigned i) { return gload(sdf + i); };
+    const float c00 = lerp(at(r00 + x0), at(r00 + x1), tx), c10 = lerp(at(r10 + x0), at(r10 + x1), tx);
+    const float c01 = lerp(at(r01 + x0), at(r01 + x1), tx), c11 = lerp(at(r11 + x0), at(r11 + x1), tx);
+    return lerp(lerp(c00, c10, ty), lerp(c01, c11, ty), tz);
+}
+
+// A41:1645-1691 trilinear sample; false outside the grid.
+__device__ inline bool sdf_sample(const ClothView& c, F3 p, float* out) {
+    if (c.sdf_dim[0] < 2 || c.sdf_dim[1] < 2 || c.sdf_dim[2] < 2 || c.sdf_voxel <= 0.0f) return false;
+    const float inv = c.sdf_inv;
+    const float gx = (p.x - c.sdf_origin[0]) * inv, gy = (p.y - c.sdf_origin[1]) * inv, gz = (p.z - c.sdf_origin[2]) * inv;
+    if (gx < 0.0f || gy < 0.0f || gz < 0.0f || gx > c.sdf_top[0] || gy > c.sdf_top[1] || gz > c.sdf_top[2]) return false;
+    *out = sdf_trilinear(c, gx, gy, gz);
     return isfinite(*out);
 }
 
 // A41:1693-1723
 __device__ inline float sdf_sample_clamped(const ClothView& c, F3 p) {
-    const int dx = c.sdf_dim[0], dy = c.sdf_dim[1], dz = c.sdf_dim[2];
-    const float inv = 1.0f / fmaxf(c.sdf_voxel, kEps);
-    const float gx = fminf(fmaxf((p.x - c.sdf_origin[0]) * inv, 0.0f), static_cast<float>(dx - 1));
-    const float gy = fminf(fmaxf((p.y - c.sdf_origin[1]) * inv, 0.0f), static_cast<float>(dy - 1));
-    const float gz = fminf(fmaxf((p.z - c.sdf_origin[2]) * inv, 0.0f), static_cast<float>(dz - 1));
-    const int x0 = min(max(static_cast<int>(floorf(gx)), 0), dx - 1), y0 = min(max(static_cast<int>(floorf(gy)), 0), dy - 1),
-              z0 = min(max(static_cast<int>(floorf(gz)), 0), dz - 1);
-    const int x1 = min(x0 + 1, dx - 1), y1 = min(y0 + 1, dy - 1), z1 = min(z0 + 1, dz - 1);
-    const float tx = gx - static_cast<float>(x0), ty = gy - static_cast<float>(y0), tz = gz - static_cast<float>(z0);
-    const float* s = c.sdf;
-    auto at = [&](int x, int y, int z) { return s[(static_cast<size_t>(z) * dy + y) * dx + x]; };
-    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
-    return lerp(lerp(lerp(at(x0, y0, z0), at(x1, y0, z0), tx), lerp(at(x0, y1, z0), at(x1, y1, z0), tx), ty),
-                lerp(lerp(at(x0, y0, z1), at(x1, y0, z1), tx), lerp(at(x0, y1, z1), at(x1, y1, z1), tx), ty), tz);
+    const float inv = c.sdf_inv_clamped;
+    const float gx = fminf(fmaxf((p.x - c.sdf_origin[0]) * inv, 0.0f), c.sdf_top[0]);
+    const float gy = fminf(fmaxf((p.y - c.sdf_origin[1]) * inv, 0.0f), c.sdf_top[1]);
+    const float gz = fminf(fmaxf((p.z - c.sdf_origin[2]) * inv, 0.0f), c.sdf_top[2]);
+    return sdf_trilinear(c, gx, gy, gz);
 }
 
 // A41:1771-1785, falling back to the nearest static triangle (A41:1749-1769).
@@ -276,9 +274,9 @@ __device__ __forceinline__ float dyn_triangle_depth(const DynView& D, int t, con
     const F3 q = closest_point_on_triangle(p, a, b, cc);
     const F3 d = p - q;
     const float dsq = dot(d, d);
-    const float dist = sqrtf(fmaxf(dsq, kEps));
+    const float dist = sqrt_eps(dsq);
     const F3 tn = cross(b - a, cc - a);
-    const float tl = sqrtf(fmaxf(dot(tn, tn), kEps));
+    const float tl = sqrt_eps(dot(tn, tn));
     const F3 tnorm = mul(tn, 1.0f / tl);
     const F3 n = dsq > kEps * 4.0f ? mul(d, 1.0f / fmaxf(dist, kEps)) : tnorm;
     if (dist >= cm) return 0.0f;
@@ -294,7 +292,7 @@ __device__ __forceinline__ float dyn_particle_depth(const DynView& D, int j, F3 
     if (radius <= 0.0f) return 0.0f;
     const F3 d = p - f3(pj);
     const float dsq = dot(d, d);
-    const float dist = sqrtf(fmaxf(dsq, kEps));
+    const float dist = sqrt_eps(dsq);
     if (dist >= radius) return 0.0f;
     ++cand;
     const F3 n = dist > kEps ? mul(d, 1.0f / dist) : f3(0.0f, 0.0f, 1.0f);
@@ -1008,9 +1006,9 @@ __device__ inline void signed_contacts(const ClothView& c, VertexState& s, bool 
             const F3 q = closest_point_on_triangle(p, a, b, cc);
             const F3 d = p - q;
             const float dsq = dot(d, d);
-            const float dist = sqrtf(fmaxf(dsq, kEps));
+            const float dist = sqrt_eps(dsq);
             const F3 tn = cross(b - a, cc - a);
-            const float tl = sqrtf(fmaxf(dot(tn, tn), kEps));
+            const float tl = sqrt_eps(dot(tn, tn));
             const F3 tnorm = mul(tn, 1.0f / tl);
             const float sd = dot(d, tnorm);
             float pen;
@@ -1086,7 +1084,7 @@ __device__ inline void signed_contacts(const ClothView& c, VertexState& s, bool 
         if (radius <= 0.0f) return;
         const F3 d = p - f3(pj);
         const float dsq = dot(d, d);
-        const float dist = sqrtf(fmaxf(dsq, kEps));
+        const float dist = sqrt_eps(dsq);
         if (dist >= radius) return;
         ++cand;
         const F3 n = dist > kEps ? mul(d, 1.0f / dist) : f3(0.0f, 0.0f, 1.0f);
@@ -1161,7 +1159,7 @@ static_assert(kEllMax % kGatherChunk == 0, "the gather chunk must divide the slo
 // (a scalar base register and one vector register per load in flight).
 template <typename T>
 __device__ __forceinline__ T load_at(const T* __restrict__ base, unsigned i) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<unsigned>(sizeof(T)));
+    return gload(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<unsigned>(sizeof(T))));
 }
 
 // A41:1927-1980, one edge as seen from vertex s.v: the edge thread's
@@ -1173,14 +1171,15 @@ __device__ __forceinline__ void stretch_edge(int2 a, float4 o4, F3 p0, float w, 
     const float wsum = w + o4.w;
     if (wsum <= 0.0f) return;
     const F3 delta = f3(o4) - p0;
-    const float len = sqrtf(fmaxf(dot(delta, delta), kEps));
+    // len >= sqrt(kEps) > kEps, so max(len, eps) is len itself
+    const float len = sqrt_eps(dot(delta, delta));
     const float cval = len - restl;
     const float dl = -cval / (wsum + alpha);
-    F3 corr = mul(delta, 0.18f * dl / fmaxf(len, kEps));
-    const float cl = sqrtf(fmaxf(dot(corr, corr), 0.0f));
+    F3 corr = mul(delta, 0.18f * dl / len);
+    const float csq = fmaxf(dot(corr, corr), 0.0f);
     const float mc = fmaxf(0.0025f, fminf(fmaxf(restl, 0.0f) * 0.10f, 0.025f));
-    if (cl > mc) {
-        corr = mul(corr, mc / fmaxf(cl, kEps));
+    if (sqrt_greater(csq, mc)) {
+        corr = mul(corr, mc / fmaxf(sqrtf(csq), kEps));
         if (a.x < 0) ++clamps;
     }
     if (w > 0.0f) {
@@ -1190,17 +1189,24 @@ __device__ __forceinline__ void stretch_edge(int2 a, float4 o4, F3 p0, float w, 
     }
 }
 
+// The first ell_width incidences of vertex v.  All kEllMax slots are loaded
+// (those past the width load the last real slot again, from cache), so the
+// loads issue back to back with no branch between them; the gather ignores
+// slots at or past the width.  Requires ell_width > 0.
+__device__ __forceinline__ void ell_load(const ClothView& c, int v, int2 (&a)[kEllMax]) {
+    const int last = c.ell_width - 1;
+#pragma unroll
+    for (int j = 0; j < kEllMax; ++j) a[j] = load_at(c.ell, static_cast<unsigned>(min(j, last) * c.ell_pitch + v));
+}
+
 // Direct stretch gathered per vertex, incident edges in ascending edge order:
-// the slot-major first ell_width incidences, then any overflow from adj.
+// the slot-major first ell_width incidences (a, loaded by ell_load), then any
+// overflow from adj.
 __device__ inline void stretch_gather(const ClothView& c, const float4* __restrict__ src, VertexState& s, float w,
-                                      bool count) {
+                                      const int2 (&a)[kEllMax], bool count) {
     const int v = s.v;
     const int width = c.ell_width;
     const float alpha = c.stretch_alpha;
-    int2 a[kEllMax];
-#pragma unroll
-    for (int j = 0; j < kEllMax; ++j)
-        a[j] = j < width ? load_at(c.ell, static_cast<unsigned>(j * c.ell_pitch + v)) : make_int2(-1, 0);
     const F3 p0 = s.p;
     F3 p = p0;
     F3 q = w > 0.0f ? s.q() : p0;
@@ -1210,14 +1216,16 @@ __device__ inline void stretch_gather(const ClothView& c, const float4* __restri
     for (int j0 = 0; j0 < kEllMax; j0 += kGatherChunk) {
         if (j0 >= width) break;
         float4 o[kGatherChunk];
+        bool live[kGatherChunk];
 #pragma unroll
         for (int j = 0; j < kGatherChunk; ++j) {
             const int x = a[j0 + j].x;
-            o[j] = load_at(src, static_cast<unsigned>(x == -1 ? v : (x & 0x7fffffff)));
+            live[j] = j0 + j < width && x != -1;
+            o[j] = load_at(src, static_cast<unsigned>(live[j] ? (x & 0x7fffffff) : v));
         }
 #pragma unroll
         for (int j = 0; j < kGatherChunk; ++j)
-            if (a[j0 + j].x != -1) stretch_edge(a[j0 + j], o[j], p0, w, alpha, p, q, q_moved, clamps);
+            if (live[j]) stretch_edge(a[j0 + j], o[j], p0, w, alpha, p, q, q_moved, clamps);
     }
     if (c.ell_overflow) {
         const int e = c.adj_off[v + 1];
@@ -1340,14 +1348,36 @@ __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __res
     float4* dst = dst_parity ? c.pos1 : c.pos0;
     const float dt = c.dt_sub;
     const bool count = (ops & kOpCount) != 0u;
-    const float4 p4 = src[v];
+    // Everything the stretch gather reads about the vertex itself (its
+    // incidences, position and previous position) is requested before any of
+    // it is used: one memory round trip, then one per chunk of neighbours.
+    bool gather = false;
+    int2 ell[kEllMax];
+    float4 q4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (kGather == kGatherStretch) {
+        gather = c.direct_stretch != 0;
+        if (gather) {
+            if (c.ell_width > 0) {
+                ell_load(c, v, ell);
+            } else {
+#pragma unroll
+                for (int j = 0; j < kEllMax; ++j) ell[j] = make_int2(-1, 0);   // never read: width 0
+            }
+            q4 = gload(c.prev + v);
+        }
+    }
+    const float4 p4 = gload(src + v);
     const float w = p4.w;
     // .w holds 0 exactly for the vertices the reference flags as pinned
     // (inverse mass <= 0, A41:8861-8865)
     const bool pinned = w == 0.0f;
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
     if constexpr (kGather == kGatherStretch) {
-        if (c.direct_stretch) stretch_gather(c, src, s, w, count);
+        if (gather) {
+            s.q_ = f3(q4);
+            s.q_have = true;
+            stretch_gather(c, src, s, w, ell, count);
+        }
     } else if constexpr (kGather == kGatherLra) {
         lra_gather(c, src, s, w, dt, count);
     } else if constexpr (kGather == kGatherBend) {
@@ -1363,9 +1393,9 @@ __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __res
         if (pinned) s.set_vel(f3(0.0f, 0.0f, 0.0f));
         else velocity_update(c, s, dt);
     }
-    dst[v] = with_w(s.p, w);
-    if (s.q_dirty) c.prev[v] = with_w(s.q_, 0.0f);
-    if (s.v_dirty) c.vel[v] = with_w(s.vel_, 0.0f);
+    gstore(dst + v, with_w(s.p, w));
+    if (s.q_dirty) gstore(c.prev + v, with_w(s.q_, 0.0f));
+    if (s.v_dirty) gstore(c.vel + v, with_w(s.vel_, 0.0f));
 }
 
 // -------------------------------------------------------------- bending

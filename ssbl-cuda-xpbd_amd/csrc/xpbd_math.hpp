@@ -31,7 +31,56 @@ __device__ __forceinline__ float len3(F3 a) { return sqrtf(fmaxf(dot(a, a), 0.0f
 __device__ __forceinline__ bool finite3(F3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
 __device__ __forceinline__ float clamp01f(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 __device__ __forceinline__ float rsqrt_exact(float v) { return 1.0f / sqrtf(v); }
+
+// sqrtf(fmaxf(x, kEps)), bit for bit, in fewer instructions.  The correctly
+// rounded root of a float y >= 2^-96 is the hardware root s (within 1 ulp)
+// moved to a neighbour s' = s -+ 1 ulp when the residual fma(-s', s, y) says
+// the root lies beyond the midpoint: the sequence the compiler emits for
+// sqrtf, minus its rescaling of inputs below 2^-96 and its pass-through of
+// +-0 and +inf, neither of which a value >= kEps needs (+inf comes out as
+// +inf: both residuals are NaN).  NaN x gives the root of kEps, as fmaxf does.
+__device__ __forceinline__ float sqrt_eps(float x) {
+    const float y = fmaxf(x, kEps);
+    const float s = __builtin_amdgcn_sqrtf(y);
+    const float down = __int_as_float(__float_as_int(s) - 1), up = __int_as_float(__float_as_int(s) + 1);
+    float r = __builtin_fmaf(-down, s, y) <= 0.0f ? down : s;
+    r = __builtin_fmaf(-up, s, y) > 0.0f ? up : r;
+    return r;
+}
+
+// sqrtf(x) > m for a float x >= 0 (or NaN or +inf) and a positive normal
+// float m, without taking the root.  sqrtf(x) > m exactly when sqrt(x) lies
+// above the midpoint h between m and the next float; h has 25 significant
+// bits, so h and h*h (at most 50 bits) are exact in double, and sqrt(x) can
+// never equal h (h*h needs more than the 24 bits of a float).
+__device__ __forceinline__ bool sqrt_greater(float x, float m) {
+    const double h = (static_cast<double>(m) + static_cast<double>(__int_as_float(__float_as_int(m) + 1))) * 0.5;
+    return static_cast<double>(x) > h * h;
+}
 __device__ __forceinline__ float4 with_w(F3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// Loads and stores through the global address space.  The kernels take
+// their array pointers from a table in device memory, so the compiler sees
+// generic pointers and emits flat accesses, which also count against the
+// scalar-load counter and make every scalar wait drain the vector loads in
+// flight; these helpers emit global_load / global_store instead.
+#define SSBL_GLOBAL __attribute__((address_space(1)))
+typedef float gv4f __attribute__((ext_vector_type(4)));
+typedef int gv2i __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float gload(const float* p) { return *(const SSBL_GLOBAL float*)(p); }
+__device__ __forceinline__ int2 gload(const int2* p) {
+    const gv2i x = *(const SSBL_GLOBAL gv2i*)(p);
+    return make_int2(x.x, x.y);
+}
+__device__ __forceinline__ float4 gload(const float4* p) {
+    const gv4f x = *(const SSBL_GLOBAL gv4f*)(p);
+    return make_float4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ void gstore(float4* p, float4 v) {
+    gv4f x;
+    x.x = v.x; x.y = v.y; x.z = v.z; x.w = v.w;
+    *(SSBL_GLOBAL gv4f*)(p) = x;
+}
 
 __device__ __forceinline__ F3 limit_delta(F3 d, float max_len) {
     if (!finite3(d) || !isfinite(max_len) || max_len <= 0.0f) return f3(0.0f, 0.0f, 0.0f);

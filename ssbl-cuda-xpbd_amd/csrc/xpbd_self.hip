@@ -459,7 +459,7 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
                 const float dsq = dot(d, d);
                 if (dsq >= thickness * thickness) continue;
                 F3 nn = f3(0.0f, 0.0f, 1.0f);
-                float dist = sqrtf(fmaxf(dsq, kEps));
+                float dist = sqrt_eps(dsq);
                 if (dsq > kEps) nn = mul(d, 1.0f / dist);
                 else {
                     const F3 rd = f3(c.rest[s]) - f3(c.rest[o]);
@@ -509,7 +509,7 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
             const F3 q = closest_point_on_triangle(p, a, b, cc);
             const F3 d = p - q;
             const float dsq = dot(d, d);
-            float dist = sqrtf(fmaxf(dsq, kEps));
+            float dist = sqrt_eps(dsq);
             if (dist >= onset) return false;
             F3 nn = f3(0.0f, 0.0f, 1.0f);
             if (dsq > kEps) nn = mul(d, 1.0f / dist);
@@ -612,7 +612,7 @@ __device__ inline void self_edge(const ClothView& c, const SelfView& sv, const f
         const F3 pa = p0 + mul(p1 - p0, s), pb = p2 + mul(p3 - p2, t);
         const F3 d = pa - pb;
         const float dsq = dot(d, d);
-        float dist = sqrtf(fmaxf(dsq, kEps));
+        float dist = sqrt_eps(dsq);
         if (dist >= onset) return false;
         F3 nn = f3(0.0f, 0.0f, 1.0f);
         if (dsq > kEps) nn = mul(d, 1.0f / dist);
@@ -762,7 +762,7 @@ __global__ __launch_bounds__(kIslandBlock) void k_self_island(const ClothView* _
                     const F3 q = closest_point_on_triangle(p, a, bb, cc);
                     F3 d = p - q;
                     const float dsq = dot(d, d);
-                    float dist = sqrtf(fmaxf(dsq, kEps));
+                    float dist = sqrt_eps(dsq);
                     if (dist >= r) continue;
                     F3 nn = f3(0.0f, 0.0f, 1.0f);
                     if (dsq > kEps) nn = mul(d, 1.0f / dist);

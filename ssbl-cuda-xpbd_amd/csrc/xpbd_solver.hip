@@ -810,6 +810,9 @@ void Solver::fill_view(int substeps) {
     v.sdf_ready = sdf_ready;
     for (int k = 0; k < 3; ++k) { v.sdf_dim[k] = sdf_dim[k]; v.sdf_origin[k] = sdf_origin[k]; }
     v.sdf_voxel = sdf_voxel;
+    v.sdf_inv = sdf_voxel > 0.0f ? 1.0f / sdf_voxel : 0.0f;
+    v.sdf_inv_clamped = 1.0f / std::max(sdf_voxel, 1.0e-7f);   // the kernels' eps
+    for (int k = 0; k < 3; ++k) v.sdf_top[k] = static_cast<float>(sdf_dim[k] - 1);
     v.need_normal = (v.use_pressure || !fields.empty()) ? 1 : 0;
     v.direct_stretch = cfg.stretch_optimization_enabled ? 0 : 1;
     v.contacts_on = (analytic_on() || sdf_ready) ? 1 : 0;
