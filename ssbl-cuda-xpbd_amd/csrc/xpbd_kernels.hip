@@ -153,24 +153,37 @@ __device__ inline void analytic_contacts(const ClothView& c, VertexState& s, flo
     }
 }
 
-// Trilinear interpolation at grid coordinates (gx, gy, gz) inside the grid,
-// the corner order and lerps of A41:1645-1691 and A41:1693-1723.  Element
-// offsets are 32-bit: a grid holds at most INT_MAX voxels.
-__device__ __forceinline__ float sdf_trilinear(const ClothView& c, float gx, float gy, float gz) {
-    const int dx = c.sdf_dim[0], dy = c.sdf_dim[1], dz = c.sdf_dim[2];
-    const int x0 = min(max(static_cast<int>(floorf(gx)), 0), dx - 1), y0 = min(max(static_cast<int>(floorf(gy)), 0), dy - 1),
-              z0 = min(max(static_cast<int>(floorf(gz)), 0), dz - 1);
-    const int x1 = min(x0 + 1, dx - 1), y1 = min(y0 + 1, dy - 1), z1 = min(z0 + 1, dz - 1);
-    const float tx = gx - static_cast<float>(x0), ty = gy - static_cast<float>(y0), tz = gz - static_cast<float>(z0);
-    const unsigned r00 = static_cast<unsigned>((z0 * dy + y0) * dx), r10 = static_cast<unsigned>((z0 * dy + y1) * dx),
-                   r01 = static_cast<unsigned>((z1 * dy + y0) * dx), r11 = static_cast<unsigned>((z1 * dy + y1) * dx);
-    const float* sdf = c.sdf;
-    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
-    auto at = [&](unsigned i) { return gload(sdf + i); };
-    const float c00 = lerp(at(r00 + x0), at(r00 + x1), tx), c10 = lerp(at(r10 + x0), at(r10 + x1), tx);
-    const float c01 = lerp(at(r01 + x0), at(r01 + x1), tx), c11 = lerp(at(r11 + x0), at(r11 + x1), tx);
-    return lerp(lerp(c00, c10, ty), lerp(c01, c11, ty), tz);
-}
+// One trilinear sample (A41:1645-1691, A41:1693-1723) split in two halves:
+// corners() issues the eight loads of the cell containing grid coordinates
+// (gx, gy, gz) inside the grid and keeps the weights; blend() interpolates
+// once they arrive.  Callers with several samples issue all their loads
+// first.  Element offsets are 32-bit: a grid holds at most INT_MAX voxels.
+struct SdfCell {
+    float k[8];
+    float tx, ty, tz;
+    __device__ __forceinline__ void corners(const ClothView& c, float gx, float gy, float gz) {
+        const int dx = c.sdf_dim[0], dy = c.sdf_dim[1], dz = c.sdf_dim[2];
+        const int x0 = min(max(static_cast<int>(floorf(gx)), 0), dx - 1), y0 = min(max(static_cast<int>(floorf(gy)), 0), dy - 1),
+                  z0 = min(max(static_cast<int>(floorf(gz)), 0), dz - 1);
+        const int x1 = min(x0 + 1, dx - 1), y1 = min(y0 + 1, dy - 1), z1 = min(z0 + 1, dz - 1);
+        tx = gx - static_cast<float>(x0);
+        ty = gy - static_cast<float>(y0);
+        tz = gz - static_cast<float>(z0);
+        const unsigned r00 = static_cast<unsigned>((z0 * dy + y0) * dx), r10 = static_cast<unsigned>((z0 * dy + y1) * dx),
+                       r01 = static_cast<unsigned>((z1 * dy + y0) * dx), r11 = static_cast<unsigned>((z1 * dy + y1) * dx);
+        const float* sdf = c.sdf;
+        k[0] = gload(sdf + (r00 + x0)); k[1] = gload(sdf + (r00 + x1));
+        k[2] = gload(sdf + (r10 + x0)); k[3] = gload(sdf + (r10 + x1));
+        k[4] = gload(sdf + (r01 + x0)); k[5] = gload(sdf + (r01 + x1));
+        k[6] = gload(sdf + (r11 + x0)); k[7] = gload(sdf + (r11 + x1));
+    }
+    __device__ __forceinline__ float blend() const {
+        auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+        const float c00 = lerp(k[0], k[1], tx), c10 = lerp(k[2], k[3], tx);
+        const float c01 = lerp(k[4], k[5], tx), c11 = lerp(k[6], k[7], tx);
+        return lerp(lerp(c00, c10, ty), lerp(c01, c11, ty), tz);
+    }
+};
 
 // A41:1645-1691 trilinear sample; false outside the grid.
 __device__ inline bool sdf_sample(const ClothView& c, F3 p, float* out) {
@@ -178,25 +191,36 @@ __device__ inline bool sdf_sample(const ClothView& c, F3 p, float* out) {
     const float inv = c.sdf_inv;
     const float gx = (p.x - c.sdf_origin[0]) * inv, gy = (p.y - c.sdf_origin[1]) * inv, gz = (p.z - c.sdf_origin[2]) * inv;
     if (gx < 0.0f || gy < 0.0f || gz < 0.0f || gx > c.sdf_top[0] || gy > c.sdf_top[1] || gz > c.sdf_top[2]) return false;
-    *out = sdf_trilinear(c, gx, gy, gz);
+    SdfCell cell;
+    cell.corners(c, gx, gy, gz);
+    *out = cell.blend();
     return isfinite(*out);
 }
 
-// A41:1693-1723
-__device__ inline float sdf_sample_clamped(const ClothView& c, F3 p) {
+// A41:1693-1723, the loads only (blend() gives the sample).
+__device__ __forceinline__ void sdf_clamped_corners(const ClothView& c, F3 p, SdfCell& cell) {
     const float inv = c.sdf_inv_clamped;
     const float gx = fminf(fmaxf((p.x - c.sdf_origin[0]) * inv, 0.0f), c.sdf_top[0]);
     const float gy = fminf(fmaxf((p.y - c.sdf_origin[1]) * inv, 0.0f), c.sdf_top[1]);
     const float gz = fminf(fmaxf((p.z - c.sdf_origin[2]) * inv, 0.0f), c.sdf_top[2]);
-    return sdf_trilinear(c, gx, gy, gz);
+    cell.corners(c, gx, gy, gz);
+}
+
+// Central difference of the clamped samples at p + d and p - d, both cells'
+// loads in flight together.
+__device__ __forceinline__ float sdf_difference(const ClothView& c, F3 p, F3 d, F3 md) {
+    SdfCell hi, lo;
+    sdf_clamped_corners(c, p + d, hi);
+    sdf_clamped_corners(c, p + md, lo);
+    return hi.blend() - lo.blend();
 }
 
 // A41:1771-1785, falling back to the nearest static triangle (A41:1749-1769).
 __device__ inline F3 sdf_normal(const ClothView& c, F3 p) {
     const float h = fmaxf(c.sdf_voxel, 1.0e-5f);
-    const float dx = sdf_sample_clamped(c, p + f3(h, 0.0f, 0.0f)) - sdf_sample_clamped(c, p + f3(-h, 0.0f, 0.0f));
-    const float dy = sdf_sample_clamped(c, p + f3(0.0f, h, 0.0f)) - sdf_sample_clamped(c, p + f3(0.0f, -h, 0.0f));
-    const float dz = sdf_sample_clamped(c, p + f3(0.0f, 0.0f, h)) - sdf_sample_clamped(c, p + f3(0.0f, 0.0f, -h));
+    const float dx = sdf_difference(c, p, f3(h, 0.0f, 0.0f), f3(-h, 0.0f, 0.0f));
+    const float dy = sdf_difference(c, p, f3(0.0f, h, 0.0f), f3(0.0f, -h, 0.0f));
+    const float dz = sdf_difference(c, p, f3(0.0f, 0.0f, h), f3(0.0f, 0.0f, -h));
     const F3 n = f3(dx, dy, dz);
     const float lsq = dot(n, n);
     if (isfinite(lsq) && lsq > 1.0e-12f) return mul(n, rsqrt_exact(lsq));
