@@ -109,67 +109,86 @@ __device__ __forceinline__ bool span_cell(int4 a, int4 b, int idx, int* x, int* 
     return true;
 }
 
-constexpr int kTriLanes = 16;   // lanes per triangle in the hash builds (cells beyond 16 loop)
+constexpr int kTriLanes = 4;      // lanes per triangle in the hash builds: lane l takes cells l, l + 4, ...
+constexpr int kAggSlots = 1024;   // LDS table of the distinct buckets a block's entries fall in
+constexpr int kAggProbes = 32;    // a bucket with no slot within this many probes counts globally, entry by entry
 
-// kTriLanes lanes per triangle, lane l taking cells l, l + 16, ... (16
-// triangles per block).  Phase 0 counts the entries of every bucket; phase 1
-// counts them down again, each entry taking the place its decrement returns
-// in the bucket's segment: raw[off[b] + slot] = {key, b}, in arrival order
-// until ranked.  Neighbouring triangles share most of their cells, so a
-// block first gathers its entries per bucket in an LDS table and makes one
-// global atomic per distinct bucket (device-scope atomics are the cost).
-constexpr int kAggSlots = 2048;   // >= twice the block's 16 x 64 entries: probing ends
+// Slot of bucket b in a block's LDS table (linear probing from b's low
+// bits), inserting b when `insert`; -1 when the probes run out.  Slots are
+// never freed, so a bucket inserted once is found again at the same place,
+// and one that found no slot never finds one later in the launch.
+__device__ __forceinline__ int agg_slot(int* s_bucket, int* s_list, int* s_n, int b, bool insert) {
+    int h = b & (kAggSlots - 1);
+    for (int k = 0; k < kAggProbes; ++k) {
+        const int old = insert ? atomicCAS(&s_bucket[h], -1, b) : s_bucket[h];
+        if (old == b) return h;
+        if (old == -1) {
+            if (!insert) return -1;
+            s_list[atomicAdd(s_n, 1)] = h;
+            return h;
+        }
+        h = (h + 1) & (kAggSlots - 1);
+    }
+    return -1;
+}
+
+// kTriLanes lanes per triangle (64 triangles per block).  Phase 0 counts the
+// entries of every bucket; phase 1 counts them down again, each entry taking
+// the place its decrement returns in the bucket's segment:
+// raw[off[b] + slot] = {key, b}, in arrival order until ranked.  Neighbouring
+// triangles share most of their cells, so a block first gathers its entries
+// per bucket in an LDS table and makes one global atomic per distinct bucket
+// (device-scope atomics are the cost); phase 1 then walks the cells again and
+// hands out the places of each bucket's block-wide range.
 __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv, const TriBox* __restrict__ box,
                                                   const int* __restrict__ large_flag, int* count, const int* __restrict__ off,
                                                   int2* raw, int phase) {
     __shared__ int s_bucket[kAggSlots];
     __shared__ int s_count[kAggSlots];
     __shared__ int s_base[kAggSlots];
+    __shared__ int s_list[kAggSlots];
+    __shared__ int s_n;
     for (int i = static_cast<int>(threadIdx.x); i < kAggSlots; i += static_cast<int>(blockDim.x)) {
         s_bucket[i] = -1;
         s_count[i] = 0;
     }
+    if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const int t = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kTriLanes);
     const int lane = static_cast<int>(threadIdx.x % kTriLanes);
-    constexpr int kRounds = kDynMaxHashCells / kTriLanes;
-    int e_slot[kRounds], e_local[kRounds], e_key[kRounds], e_bucket[kRounds];
-    int n = 0;
-    if (t < dv->nt && !large_flag[t]) {
+    const bool live = t < dv->nt && !large_flag[t];   // a span of at most 64 cells
+    int4 lo = make_int4(0, 0, 0, 0), hi = make_int4(-1, -1, -1, 0);
+    if (live) {
         const TriBox bx = box[t];
-        const int4 lo = box_cmin(bx), hi = box_cmax(bx);
-#pragma unroll
-        for (int r = 0; r < kRounds; ++r) {
-            const int idx = lane + r * kTriLanes;
-            int x, y, z;
-            if (!span_cell(lo, hi, idx, &x, &y, &z)) break;
-            const int b = static_cast<int>(hash_cell(x, y, z, dv->tri_buckets));
-            int h = b & (kAggSlots - 1);
-            for (;;) {
-                const int old = atomicCAS(&s_bucket[h], -1, b);
-                if (old == -1 || old == b) break;
-                h = (h + 1) & (kAggSlots - 1);
-            }
-            e_slot[r] = h;
-            e_local[r] = atomicAdd(&s_count[h], 1);
-            e_key[r] = t * kDynMaxHashCells + idx;
-            e_bucket[r] = b;
-            n = r + 1;
-        }
+        lo = box_cmin(bx);
+        hi = box_cmax(bx);
+    }
+    const int buckets = dv->tri_buckets;
+    int x, y, z;
+    for (int idx = lane; live && span_cell(lo, hi, idx, &x, &y, &z); idx += kTriLanes) {
+        const int b = static_cast<int>(hash_cell(x, y, z, buckets));
+        const int h = agg_slot(s_bucket, s_list, &s_n, b, true);
+        if (h >= 0) atomicAdd(&s_count[h], 1);
+        else if (phase == 0) atomicAdd(&count[b], 1);
     }
     __syncthreads();
-    for (int i = static_cast<int>(threadIdx.x); i < kAggSlots; i += static_cast<int>(blockDim.x)) {
-        const int b = s_bucket[i];
-        if (b < 0) continue;
-        if (phase == 0) atomicAdd(&count[b], s_count[i]);
-        else s_base[i] = atomicSub(&count[b], s_count[i]) - s_count[i];
+    const int n = s_n;
+    for (int i = static_cast<int>(threadIdx.x); i < n; i += static_cast<int>(blockDim.x)) {
+        const int h = s_list[i], b = s_bucket[h], k = s_count[h];
+        if (phase == 0) {
+            atomicAdd(&count[b], k);
+        } else {
+            s_base[h] = atomicSub(&count[b], k) - k;
+            s_count[h] = 0;
+        }
     }
     if (phase == 0) return;
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kRounds; ++r) {
-        if (r >= n) break;
-        raw[off[e_bucket[r]] + s_base[e_slot[r]] + e_local[r]] = make_int2(e_key[r], e_bucket[r]);
+    for (int idx = lane; live && span_cell(lo, hi, idx, &x, &y, &z); idx += kTriLanes) {
+        const int b = static_cast<int>(hash_cell(x, y, z, buckets));
+        const int h = agg_slot(s_bucket, s_list, &s_n, b, false);
+        const int slot = h >= 0 ? s_base[h] + atomicAdd(&s_count[h], 1) : atomicSub(&count[b], 1) - 1;
+        raw[off[b] + slot] = make_int2(t * kDynMaxHashCells + idx, b);
     }
 }
 

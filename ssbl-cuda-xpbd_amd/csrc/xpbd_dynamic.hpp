@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "xpbd_device.hpp"
+#include "xpbd_math.hpp"
 
 namespace ssbl {
 
@@ -42,6 +43,15 @@ struct __align__(64) TriBox {
     float4 b0;   // lo.z, hi.x, hi.y, hi.z
     float4 s;    // centre xyz, bounding radius
 };
+
+__device__ __forceinline__ TriBox gload(const TriBox* p) {
+    TriBox b;
+    b.c0 = gload(&p->c0);
+    b.c1 = gload(&p->c1);
+    b.b0 = gload(&p->b0);
+    b.s = gload(&p->s);
+    return b;
+}
 
 __device__ __forceinline__ int4 box_cmin(const TriBox& b) { return make_int4(b.c0.x, b.c0.y, b.c0.z, 0); }
 __device__ __forceinline__ int4 box_cmax(const TriBox& b) { return make_int4(b.c0.w, b.c1.x, b.c1.y, 0); }

@@ -283,7 +283,7 @@ __device__ __forceinline__ bool dyn_bucket_rejected(const ClothView& c, int4 bin
 // One dynamic triangle against point p (A41:3999-4323 per candidate): counts
 // the candidate, the AABB and bounding-sphere rejections, and returns the
 // contact depth (0 when out of reach) with the push along the contact normal.
-__device__ __forceinline__ float dyn_triangle_depth(const DynView& D, int t, const TriBox& bx, F3 p, float cm, unsigned& cand,
+__device__ __forceinline__ float dyn_triangle_depth(const SSBL_CONST DynView& D, int t, const TriBox& bx, F3 p, float cm, unsigned& cand,
                                                     unsigned& rej, F3& delta) {
     ++cand;
     const float4 mn = box_lo(bx), mx = box_hi(bx);
@@ -294,7 +294,7 @@ __device__ __forceinline__ float dyn_triangle_depth(const DynView& D, int t, con
     const float rr = ce.w + cm;
     const F3 dc = p - f3(ce);
     if (dot(dc, dc) > rr * rr) { ++rej; return 0.0f; }
-    const F3 a = f3(D.tv[3 * t]), b = f3(D.tv[3 * t + 1]), cc = f3(D.tv[3 * t + 2]);
+    const F3 a = f3(gload(D.tv + 3 * t)), b = f3(gload(D.tv + 3 * t + 1)), cc = f3(gload(D.tv + 3 * t + 2));
     const F3 q = closest_point_on_triangle(p, a, b, cc);
     const F3 d = p - q;
     const float dsq = dot(d, d);
@@ -310,8 +310,8 @@ __device__ __forceinline__ float dyn_triangle_depth(const DynView& D, int t, con
 }
 
 // One dynamic particle against point p (A41:4354-4511 per candidate).
-__device__ __forceinline__ float dyn_particle_depth(const DynView& D, int j, F3 p, float margin, unsigned& cand, F3& delta) {
-    const float4 pj = D.pp[j];
+__device__ __forceinline__ float dyn_particle_depth(const SSBL_CONST DynView& D, int j, F3 p, float margin, unsigned& cand, F3& delta) {
+    const float4 pj = gload(D.pp + j);
     const float radius = fmaxf(pj.w, 0.0f) + margin;
     if (radius <= 0.0f) return 0.0f;
     const F3 d = p - f3(pj);
@@ -395,7 +395,7 @@ __device__ __forceinline__ unsigned long long wave_best(unsigned long long key, 
 constexpr int kCoopTriangleProbes = SSBL_COOP_TRIANGLE_PROBES;
 constexpr int kCoopParticleProbes = SSBL_COOP_PARTICLE_PROBES;
 __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState& s, bool active, float dt, bool count) {
-    const DynView& D = *c.dyn;
+    const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
     const int lane = static_cast<int>(__lane_id());
     const float cm = fmaxf(c.collision_margin, 0.0f);
     const int nt = D.nt;
@@ -408,7 +408,7 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
             int4 bin = make_int4(0, 0, 0, -2);
             if (active) {
                 cx = cell_coord(s.p.x, D.tri_cell); cy = cell_coord(s.p.y, D.tri_cell); cz = cell_coord(s.p.z, D.tri_cell);
-                bin = D.tri_bin[hash_cell(cx, cy, cz, D.tri_buckets)];
+                bin = gload(D.tri_bin + hash_cell(cx, cy, cz, D.tri_buckets));
                 if (dyn_bucket_rejected(c, bin)) bin.y = 0;
             }
             unsigned long long pending = __ballot(bin.y > 0);
@@ -417,12 +417,12 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                 pending = 0ull;
                 for (int k = bin.x; k < bin.x + bin.y; ++k) {
                     if (D.tri_ksl) {
-                        const int2 sl = D.tri_ksl[k];
+                        const int2 sl = gload(D.tri_ksl + k);
                         if (!dyn_enabled(c, sl.x, sl.y)) continue;
                     }
-                    const int t = D.tri_keys[k] / kDynMaxHashCells;
+                    const int t = gload(D.tri_keys + k) / kDynMaxHashCells;
                     if (t < 0 || t >= nt) continue;
-                    const TriBox bx = D.box[t];
+                    const TriBox bx = gload(D.box + t);
                     const int4 lo = box_cmin(bx), hi = box_cmax(bx);
                     if (cx < lo.x || cx > hi.x || cy < lo.y || cy > hi.y || cz < lo.z || cz > hi.z) continue;
                     F3 d;
@@ -444,12 +444,12 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                     // the source filter and the canonical-cell test (A41:3934-3982) are
                     // both side-effect free, so the cheaper one (beside the key) runs first
                     if (D.tri_ksl) {
-                        const int2 sl = D.tri_ksl[k];
+                        const int2 sl = gload(D.tri_ksl + k);
                         if (!dyn_enabled(c, sl.x, sl.y)) continue;
                     }
-                    const int t = D.tri_keys[k] / kDynMaxHashCells;
+                    const int t = gload(D.tri_keys + k) / kDynMaxHashCells;
                     if (t < 0 || t >= nt) continue;
-                    const TriBox bx = D.box[t];
+                    const TriBox bx = gload(D.box + t);
                     const int4 lo = box_cmin(bx), hi = box_cmax(bx);
                     if (lx < lo.x || lx > hi.x || ly < lo.y || ly > hi.y || lz < lo.z || lz > hi.z) continue;
                     F3 d;
@@ -507,7 +507,7 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
 }
 
 __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState& s, bool active, bool count) {
-    const DynView& D = *c.dyn;
+    const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
     const int lane = static_cast<int>(__lane_id());
     const int np_count = D.np;
     if (np_count > 0) {
@@ -522,7 +522,7 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
             bool probe = active;
             if (active && D.coarse && D.max_radius + margin <= D.coarse_reach && finite3(s.p) && *D.coarse_off == 0) {
                 const float cc = D.coarse_cell;
-                const int2 e = D.coarse[hash_cell(cell_coord(s.p.x, cc), cell_coord(s.p.y, cc), cell_coord(s.p.z, cc), D.coarse_buckets)];
+                const int2 e = gload(D.coarse + hash_cell(cell_coord(s.p.x, cc), cell_coord(s.p.y, cc), cell_coord(s.p.z, cc), D.coarse_buckets));
                 probe = !(e.y == -2 || (c.dyn_mode != 0 && ((c.dyn_target_src != 0 && e.y == c.dyn_target_src) ||
                                                             (c.dyn_mode == 2 && e.x >= c.dyn_target_layer))));
             }
@@ -539,7 +539,7 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
 #pragma unroll
                         for (int k = 0; k < 9; ++k)
                             bins[k] = cell_in_reach(s.p, cx, cy, cz, k % 3 - 1, k / 3 - 1, dz, D.part_cell, reach)
-                                          ? D.pbin[hash_cell(cx + k % 3 - 1, cy + k / 3 - 1, cz + dz, D.part_buckets)]
+                                          ? gload(D.pbin + hash_cell(cx + k % 3 - 1, cy + k / 3 - 1, cz + dz, D.part_buckets))
                                           : make_int4(0, 0, 0, -2);
 #pragma unroll
                         for (int k9 = 0; k9 < 9; ++k9) {
@@ -547,13 +547,13 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                             const int qx = cx + k9 % 3 - 1, qy = cy + k9 / 3 - 1, qz = cz + dz;
                             const int base = bins[k9].x, lim = min(bins[k9].y, kDynParticleSlots);
                             for (int k = 0; k < lim; ++k) {
-                                const int4 pc = D.pkcell[base + k];
+                                const int4 pc = gload(D.pkcell + base + k);
                                 if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
                                 if (D.pksl) {
-                                    const int2 sl = D.pksl[base + k];
+                                    const int2 sl = gload(D.pksl + base + k);
                                     if (!dyn_enabled(c, sl.x, sl.y)) continue;
                                 }
-                                const int j = D.pkeys[base + k];
+                                const int j = gload(D.pkeys + base + k);
                                 if (j < 0 || j >= np_count) continue;
                                 F3 d;
                                 const float pen = dyn_particle_depth(D, j, s.p, margin, cand, d);
@@ -576,17 +576,17 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                     const int qx = lcx + lane % 3 - 1;
                     const int qy = lcy + (lane / 3) % 3 - 1;
                     const int qz = lcz + lane / 9 - 1;
-                    const int4 bin = D.pbin[hash_cell(qx, qy, qz, D.part_buckets)];
+                    const int4 bin = gload(D.pbin + hash_cell(qx, qy, qz, D.part_buckets));
                     if (!dyn_bucket_rejected(c, bin)) {
                         const int lim = min(bin.y, kDynParticleSlots);
                         for (int k = 0; k < lim; ++k) {
-                            const int4 pc = D.pkcell[bin.x + k];
+                            const int4 pc = gload(D.pkcell + bin.x + k);
                             if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
                             if (D.pksl) {
-                                const int2 sl = D.pksl[bin.x + k];
+                                const int2 sl = gload(D.pksl + bin.x + k);
                                 if (!dyn_enabled(c, sl.x, sl.y)) continue;
                             }
-                            const int j = D.pkeys[bin.x + k];
+                            const int j = gload(D.pkeys + bin.x + k);
                             if (j < 0 || j >= np_count) continue;
                             F3 d;
                             const unsigned long long kk =
@@ -909,15 +909,15 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     // pinned vertex take part without a vertex of their own
     float4* pos = parity ? c.pos1 : c.pos0;
     const bool in_tile = v < tile.end;
-    const float4 p4 = in_tile ? pos[v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 p4 = in_tile ? gload(pos + v) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const bool active = in_tile && p4.w != 0.0f;
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
     if (kKind == 0) dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
     else dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u);
     if (!active) return;
-    if (s.p.x != p4.x || s.p.y != p4.y || s.p.z != p4.z) pos[v] = with_w(s.p, p4.w);
-    if (s.q_dirty) c.prev[v] = with_w(s.q_, 0.0f);
-    if (s.v_dirty) c.vel[v] = with_w(s.vel_, 0.0f);
+    if (s.p.x != p4.x || s.p.y != p4.y || s.p.z != p4.z) gstore(pos + v, with_w(s.p, p4.w));
+    if (s.q_dirty) gstore(c.prev + v, with_w(s.q_, 0.0f));
+    if (s.v_dirty) gstore(c.vel + v, with_w(s.vel_, 0.0f));
 }
 
 // ------------------------------------------------- signed dynamic contacts
@@ -1005,7 +1005,7 @@ __device__ inline void signed_response(const ClothView& c, VertexState& s, F3 or
 // Both push along the depth-weighted mean of all their contacts.  These are
 // the cloth's own colliders: no source filter.
 __device__ inline void signed_contacts(const ClothView& c, VertexState& s, bool inside, bool count) {
-    const DynView& D = *c.sdyn;
+    const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.sdyn);
     const float dt = c.dt_sub, cdt = c.collider_dt;
     const float cm = fmaxf(fmaxf(c.collision_margin, 0.0f), fmaxf(c.thickness, 0.0f) * kSignedContactMarginScale);
     const float qm = fminf(fmaxf(fmaxf(fmaxf(cm * 2.0f, kSignedQueryMargin), fmaxf(D.tri_cell, 0.0f) * 1.25f), cm), kSignedMaxQueryMargin);

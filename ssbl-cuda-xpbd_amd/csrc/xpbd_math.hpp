@@ -65,6 +65,9 @@ __device__ __forceinline__ float4 with_w(F3 a, float w) { return make_float4(a.x
 // scalar-load counter and make every scalar wait drain the vector loads in
 // flight; these helpers emit global_load / global_store instead.
 #define SSBL_GLOBAL __attribute__((address_space(1)))
+// A table that no kernel writes (a collider set's view), read through the
+// constant address space: its uniform fields come in with scalar loads.
+#define SSBL_CONST __attribute__((address_space(4)))
 typedef float gv4f __attribute__((ext_vector_type(4)));
 typedef int gv2i __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float gload(const float* p) { return *(const SSBL_GLOBAL float*)(p); }
@@ -75,6 +78,12 @@ __device__ __forceinline__ int2 gload(const int2* p) {
 __device__ __forceinline__ float4 gload(const float4* p) {
     const gv4f x = *(const SSBL_GLOBAL gv4f*)(p);
     return make_float4(x.x, x.y, x.z, x.w);
+}
+typedef int gv4i __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int gload(const int* p) { return *(const SSBL_GLOBAL int*)(p); }
+__device__ __forceinline__ int4 gload(const int4* p) {
+    const gv4i x = *(const SSBL_GLOBAL gv4i*)(p);
+    return make_int4(x.x, x.y, x.z, x.w);
 }
 __device__ __forceinline__ void gstore(float4* p, float4 v) {
     gv4f x;
