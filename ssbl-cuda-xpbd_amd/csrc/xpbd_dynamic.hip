@@ -134,15 +134,18 @@ __device__ __forceinline__ int agg_slot(int* s_bucket, int* s_list, int* s_n, in
 
 // kTriLanes lanes per triangle (64 triangles per block).  Phase 0 counts the
 // entries of every bucket; phase 1 counts them down again, each entry taking
-// the place its decrement returns in the bucket's segment:
-// raw[off[b] + slot] = {key, b}, in arrival order until ranked.  Neighbouring
+// the place its decrement returns in the bucket's segment, in arrival order:
+// ranked sets write raw[off[b] + slot] = {key, b} for k_tri_rank to sort;
+// the others, whose queries take the deepest contact with the smallest key
+// whatever the order, write ent[off[b] + slot] = {key, source, layer, 0}.  Neighbouring
 // triangles share most of their cells, so a block first gathers its entries
 // per bucket in an LDS table and makes one global atomic per distinct bucket
 // (device-scope atomics are the cost); phase 1 then walks the cells again and
 // hands out the places of each bucket's block-wide range.
 __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv, const TriBox* __restrict__ box,
                                                   const int* __restrict__ large_flag, int* count, const int* __restrict__ off,
-                                                  int2* raw, int phase) {
+                                                  int2* raw, int4* ent, const int* __restrict__ tsrc,
+                                                  const int* __restrict__ tlayer, int ranked, int phase) {
     __shared__ int s_bucket[kAggSlots];
     __shared__ int s_count[kAggSlots];
     __shared__ int s_base[kAggSlots];
@@ -184,11 +187,15 @@ __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv
     }
     if (phase == 0) return;
     __syncthreads();
+    int2 sl = make_int2(0, 0);
+    if (live && !ranked && tsrc) sl = make_int2(tsrc[t], tlayer[t]);
     for (int idx = lane; live && span_cell(lo, hi, idx, &x, &y, &z); idx += kTriLanes) {
         const int b = static_cast<int>(hash_cell(x, y, z, buckets));
         const int h = agg_slot(s_bucket, s_list, &s_n, b, false);
         const int slot = h >= 0 ? s_base[h] + atomicAdd(&s_count[h], 1) : atomicSub(&count[b], 1) - 1;
-        raw[off[b] + slot] = make_int2(t * kDynMaxHashCells + idx, b);
+        const int at = off[b] + slot, key = t * kDynMaxHashCells + idx;
+        if (ranked) raw[at] = make_int2(key, b);
+        else ent[at] = make_int4(key, sl.x, sl.y, 0);
     }
 }
 
@@ -234,14 +241,17 @@ __global__ __launch_bounds__(256) void k_tri_rank(const int* __restrict__ off, i
 // differ, -2 when the bucket is empty)}.  A query whose filter rejects every
 // entry of a bucket skips it (A41:3984-3997 filters each entry the same way,
 // so results are equal).  Sets without sources get layer INT_MIN, source -1.
-__global__ __launch_bounds__(256) void k_bucket_bins(const int* __restrict__ off, int buckets, int cap, const int2* __restrict__ ksl,
+__device__ __forceinline__ int2 entry_source_layer(int2 e) { return e; }
+__device__ __forceinline__ int2 entry_source_layer(int4 e) { return make_int2(e.y, e.z); }
+template <typename R>
+__global__ __launch_bounds__(256) void k_bucket_bins(const int* __restrict__ off, int buckets, int cap, const R* __restrict__ rec,
                                                      int4* bins) {
     for (int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); b < buckets; b += static_cast<int>(gridDim.x * blockDim.x)) {
         const int base = off[b], n = off[b + 1] - base;
-        int min_layer = ksl ? 0x7fffffff : static_cast<int>(0x80000000u), src = -2;
-        if (ksl) {
+        int min_layer = rec ? 0x7fffffff : static_cast<int>(0x80000000u), src = -2;
+        if (rec) {
             for (int k = 0; k < min(n, cap); ++k) {
-                const int2 e = ksl[base + k];
+                const int2 e = entry_source_layer(rec[base + k]);
                 min_layer = min(min_layer, e.y);
                 src = (src == -2 || src == e.x) ? e.x : -1;
             }
@@ -434,6 +444,7 @@ bool DynamicColliders::reserve_triangles(int n) {
     tri_raw_ = static_cast<int2*>(alloc(0, sizeof(int2) * m * kDynMaxHashCells));
     tri_keys_ = static_cast<int*>(alloc(0, sizeof(int) * m * kDynMaxHashCells));
     tri_ksl_ = static_cast<int2*>(alloc(0, sizeof(int2) * m * kDynMaxHashCells));
+    tri_ent_ = static_cast<int4*>(alloc(0, sizeof(int4) * m * kDynMaxHashCells));
     tri_bin_ = static_cast<int4*>(alloc(0, sizeof(int4) * buckets));
     occupancy_ = static_cast<int*>(alloc(0, sizeof(int)));
     extent_bits_ = static_cast<int*>(alloc(0, sizeof(int)));
@@ -444,7 +455,7 @@ bool DynamicColliders::reserve_triangles(int n) {
         scan_cap_ = scan_tmp_ ? need_scan : 0;
     }
     if (!tv_ || !tsrc_ || !tlayer_ || !box_ || !pctr_ || !large_flag_ || !large_off_ || !large_ || !tri_count_ || !tri_off_ ||
-        !tri_raw_ || !tri_keys_ || !tri_ksl_ || !tri_bin_ || !occupancy_ || !extent_bits_ || !scan_tmp_)
+        !tri_raw_ || !tri_keys_ || !tri_ksl_ || !tri_ent_ || !tri_bin_ || !occupancy_ || !extent_bits_ || !scan_tmp_)
         return false;
     tri_cap_ = n;
     tri_buckets_ = buckets;
@@ -505,6 +516,7 @@ bool DynamicColliders::upload_view(hipStream_t st) {
     v.box = box_;
     v.tri_keys = tri_keys_; v.tri_ksl = tri_filter_ ? tri_ksl_ : nullptr; v.large = large_;
     v.tri_bin = tri_bin_;
+    v.tri_ent = signed_ ? nullptr : tri_ent_;
     v.nlarge = large_off_ ? large_off_ + nt_ : nullptr;
     v.np = np_;
     v.part_hash = np_ >= kDynParticleHashMin ? 1 : 0;
@@ -728,6 +740,7 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
     span_margin_ = margin;
     tri_cell_ = 0.0f;   // computed on the device below
     tri_filter_ = part_filter_ = true;
+    signed_ = tri_motion_ = false;   // a gathered scene is never signed (A41:6561)
     if (!upload_view(st)) return false;
     TimedLaunch t("scene_gather", st);
     if (npseg > 0) launch_k(k_gather_particles, dim3(npseg), dim3(256), 0, st, d_sources_, d_segments_, pp_, pprev_, psrc_, player_, keep_prev ? 1 : 0);
@@ -752,17 +765,29 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
             const int buckets = next_pow2(std::max(kDynMinBuckets, nt_ * 4));
             const dim3 gw(grid_for(static_cast<long long>(nt_) * kTriLanes, 256));
             launch_k(k_fill_i32, dim3(grid_for(buckets, 256)), blk, 0, st, tri_count_, 0, buckets);
+            // signed queries average their contacts in visiting order, so only
+            // their buckets are sorted by key
+            const int ranked = signed_ ? 1 : 0;
+            const int* fsrc = tri_filter_ ? tsrc_ : nullptr;
             launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
-                     static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, 0);
+                     static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, tri_ent_,
+                     fsrc, static_cast<const int*>(tlayer_), ranked, 0);
             exclusive_scan(st, tri_count_, buckets, tri_off_, scan_tmp_);
             launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
-                     static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, 1);
-            const dim3 gr(std::min<unsigned>(grid_for(static_cast<long long>(nt_) * 16, 256), 16384u));
-            launch_k(k_tri_rank, gr, blk, 0, st, static_cast<const int*>(tri_off_), buckets, static_cast<const int2*>(tri_raw_), tri_keys_,
-                     static_cast<const int*>(tri_filter_ ? tsrc_ : nullptr), static_cast<const int*>(tlayer_), tri_ksl_);
-            launch_k(k_bucket_bins, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
-                     static_cast<const int*>(tri_off_), buckets, 0x7fffffff, static_cast<const int2*>(tri_filter_ ? tri_ksl_ : nullptr),
-                     tri_bin_);
+                     static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, tri_ent_,
+                     fsrc, static_cast<const int*>(tlayer_), ranked, 1);
+            if (ranked) {
+                const dim3 gr(std::min<unsigned>(grid_for(static_cast<long long>(nt_) * 16, 256), 16384u));
+                launch_k(k_tri_rank, gr, blk, 0, st, static_cast<const int*>(tri_off_), buckets, static_cast<const int2*>(tri_raw_),
+                         tri_keys_, fsrc, static_cast<const int*>(tlayer_), tri_ksl_);
+            }
+            const dim3 gb(std::min<unsigned>(grid_for(buckets, 256), 16384u));
+            if (ranked)
+                launch_k(k_bucket_bins<int2>, gb, blk, 0, st, static_cast<const int*>(tri_off_), buckets, 0x7fffffff,
+                         static_cast<const int2*>(tri_filter_ ? tri_ksl_ : nullptr), tri_bin_);
+            else
+                launch_k(k_bucket_bins<int4>, gb, blk, 0, st, static_cast<const int*>(tri_off_), buckets, 0x7fffffff,
+                         static_cast<const int4*>(tri_filter_ ? tri_ent_ : nullptr), tri_bin_);
             if (count_diag && occupancy) {
                 launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, occupancy_, 0, 1);
                 launch_k(k_bucket_max, dim3(std::min<unsigned>(grid_for(buckets, 256), 4096u)), blk, 0, st,
@@ -780,7 +805,7 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
                      pslot_, praw_, pkeys_, pksl_, pkcell_, static_cast<const int*>(part_filter_ ? psrc_ : nullptr),
                      static_cast<const int*>(player_), counts, cd, phase);
         }
-        launch_k(k_bucket_bins, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
+        launch_k(k_bucket_bins<int2>, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
                  static_cast<const int*>(poff_), buckets, kDynParticleSlots, static_cast<const int2*>(part_filter_ ? pksl_ : nullptr),
                  pbin_);
         launch_k(k_coarse_fill, dim3(grid_for(coarse_buckets_, 256)), blk, 0, st, coarse_, coarse_buckets_, coarse_off_);

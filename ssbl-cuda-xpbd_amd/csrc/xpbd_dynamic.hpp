@@ -70,6 +70,7 @@ struct DynView {
     const int* tri_keys;     // triangle * 64 + cell index, ascending per bucket
     const int2* tri_ksl;     // {source id, layer} of each key's triangle (filtered sets)
     const int4* tri_bin;     // per bucket: {offset, count, min layer, the one source of all entries | -1 mixed | -2 empty}
+    const int4* tri_ent;     // unranked sets: {key, source, layer, 0} per entry, in no particular order
     const int* large;        // triangles spanning more than 64 cells, ascending
     const int* nlarge;       // their count (device)
     int np;
@@ -171,6 +172,7 @@ private:
     int *tri_count_ = nullptr, *tri_off_ = nullptr, *tri_keys_ = nullptr;
     int2* tri_raw_ = nullptr;   // {key, bucket} in arrival order
     int2* tri_ksl_ = nullptr;
+    int4* tri_ent_ = nullptr;   // {key, source, layer, 0} in arrival order (sets not ranked)
     int4* tri_bin_ = nullptr;
     int* occupancy_ = nullptr;
     int* extent_bits_ = nullptr;

@@ -384,8 +384,11 @@ __device__ __forceinline__ unsigned long long wave_best(unsigned long long key, 
 // deepest with the smallest visiting order, which is the contact the serial
 // visit in the reference's order keeps.  A wave with more probes (cloths in
 // contact) lets every lane walk its own, in that order.  Both forms give the
-// same result.  Every lane of the wave must call this (`active` false for
-// lanes without a vertex).
+// same result.  The entries of an unsigned triangle bucket are stored in no
+// particular order, so there the visiting order is the key (triangle * 64 +
+// cell index): the deepest contact with the smallest key is the one a walk
+// of the bucket in ascending key order keeps.  Every lane of the wave must
+// call this (`active` false for lanes without a vertex).
 #ifndef SSBL_COOP_TRIANGLE_PROBES
 #define SSBL_COOP_TRIANGLE_PROBES 32
 #endif
@@ -415,20 +418,21 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
             if (__popcll(pending) > kCoopTriangleProbes) {
                 // many probes in this wave: each lane walks its own bucket
                 pending = 0ull;
+                unsigned long long best_key = 0ull;
                 for (int k = bin.x; k < bin.x + bin.y; ++k) {
-                    if (D.tri_ksl) {
-                        const int2 sl = gload(D.tri_ksl + k);
-                        if (!dyn_enabled(c, sl.x, sl.y)) continue;
-                    }
-                    const int t = gload(D.tri_keys + k) / kDynMaxHashCells;
+                    const int4 en = gload(D.tri_ent + k);
+                    if (D.tsrc && !dyn_enabled(c, en.y, en.z)) continue;
+                    const int key = en.x;
+                    const int t = key / kDynMaxHashCells;
                     if (t < 0 || t >= nt) continue;
                     const TriBox bx = gload(D.box + t);
                     const int4 lo = box_cmin(bx), hi = box_cmax(bx);
                     if (cx < lo.x || cx > hi.x || cy < lo.y || cy > hi.y || cz < lo.z || cz > hi.z) continue;
                     F3 d;
-                    const float pen = dyn_triangle_depth(D, t, bx, s.p, cm, cand, rej, d);
-                    if (pen > best_pen) { best_pen = pen; best_delta = d; }
+                    const unsigned long long kk = depth_key(dyn_triangle_depth(D, t, bx, s.p, cm, cand, rej, d), static_cast<unsigned>(key));
+                    if (kk > best_key) { best_key = kk; best_delta = d; }
                 }
+                if (best_key != 0ull) best_pen = __uint_as_float(static_cast<unsigned>(best_key >> 32));
             }
             while (pending) {
                 const int leader = __ffsll(static_cast<long long>(pending)) - 1;
@@ -442,18 +446,17 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                 for (int e = lane; e < nb; e += 64) {
                     const int k = b0 + e;
                     // the source filter and the canonical-cell test (A41:3934-3982) are
-                    // both side-effect free, so the cheaper one (beside the key) runs first
-                    if (D.tri_ksl) {
-                        const int2 sl = gload(D.tri_ksl + k);
-                        if (!dyn_enabled(c, sl.x, sl.y)) continue;
-                    }
-                    const int t = gload(D.tri_keys + k) / kDynMaxHashCells;
+                    // both side-effect free, so the cheaper one (in the entry) runs first
+                    const int4 en = gload(D.tri_ent + k);
+                    if (D.tsrc && !dyn_enabled(c, en.y, en.z)) continue;
+                    const int tkey = en.x;
+                    const int t = tkey / kDynMaxHashCells;
                     if (t < 0 || t >= nt) continue;
                     const TriBox bx = gload(D.box + t);
                     const int4 lo = box_cmin(bx), hi = box_cmax(bx);
                     if (lx < lo.x || lx > hi.x || ly < lo.y || ly > hi.y || lz < lo.z || lz > hi.z) continue;
                     F3 d;
-                    const unsigned long long kk = depth_key(dyn_triangle_depth(D, t, bx, lp, cm, lc, lr, d), static_cast<unsigned>(e));
+                    const unsigned long long kk = depth_key(dyn_triangle_depth(D, t, bx, lp, cm, lc, lr, d), static_cast<unsigned>(tkey));
                     if (kk > key) { key = kk; delta = d; }
                 }
                 F3 wd = f3(0.0f, 0.0f, 0.0f);
