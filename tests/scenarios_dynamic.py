@@ -38,6 +38,36 @@ def plane_contact(lib, quads_per_side, depth=0.004, margin=0.01):
     return r.state(), r.diag()
 
 
+def valley_soup(left_first, fillers=64):
+    """A V-shaped valley along y with its crease on x = 0: a left face rising
+    to x = -0.1, z = 0.1 and its mirror image, two triangles each (the left
+    ones first when `left_first`), then far-away filler triangles so that the
+    set is hashed (64 triangles or more)."""
+    def face(sx):
+        a, b, c, d = (sx * 0.1, -0.3, 0.1), (0.0, -0.3, 0.0), (0.0, 0.3, 0.0), (sx * 0.1, 0.3, 0.1)
+        return [[a, b, c], [a, c, d]]
+    faces = face(-1.0) + face(1.0) if left_first else face(1.0) + face(-1.0)
+    fill = [[(50.0 + k, 0.0, 0.0), (50.1 + k, 0.0, 0.0), (50.0 + k, 0.1, 0.0)] for k in range(fillers)]
+    return np.asarray(faces + fill, np.float32).reshape(-1, 9)
+
+
+def valley_tie(lib, count, left_first, height=0.008, margin=0.01):
+    """`count` free vertices on the valley's mirror plane x = 0 at `height`.
+    Each is exactly as deep in both faces (mirror images give the same
+    distance bit for bit) with pushes of opposite x: the tie goes to the
+    triangle first in the set, so every vertex moves to +x when the left face
+    comes first (its push is along (1, 0, 1) / sqrt 2) and to -x otherwise.
+    Four vertices make the few probes a wave serves with all its lanes, 64
+    make every lane walk its own bucket."""
+    ys = np.linspace(-0.2, 0.2, count)
+    pts = [[0.0, float(y), height] for y in ys]
+    r = Raw(lib, pts, [1.0] * count, collision_margin=margin, cloth_thickness=0.005)
+    soup = valley_soup(left_first)
+    r.check(lib.ssbl_update_dynamic_triangles(r.h, soup.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(soup)))
+    r.step(1, 1)
+    return r.state(), r.diag()
+
+
 def particle_contact(lib, far_particles=0, distance=0.01, radius=0.008, margin=0.005):
     """A free vertex over one particle (plus far-away ones): pushed up by half of
     radius + margin - distance (A41:4504-4507)."""

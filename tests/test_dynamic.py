@@ -21,6 +21,19 @@ def test_plane_contact_closed_form(oracle, quads):
     assert d.dynamic_triangle_candidate_count >= 4
 
 
+@pytest.mark.parametrize("left_first", [True, False])
+@pytest.mark.parametrize("count", [4, 64])
+def test_valley_tie_goes_to_first_triangle(oracle, count, left_first):
+    """Equal depths in two mirrored faces: the first triangle of the set wins."""
+    (p, _, _), d = sd.valley_tie(oracle, count, left_first)
+    sign = 1.0 if left_first else -1.0
+    assert np.all(np.sign(p[:, 0]) == sign), p[:, 0]
+    # half the depth m - h / sqrt 2 along (1, 0, 1) / sqrt 2 (no gravity here)
+    push = 0.5 * (0.01 - 0.008 / np.sqrt(2.0)) / np.sqrt(2.0)
+    assert np.allclose(p[:, 0], sign * push, atol=1.0e-6)
+    assert d.fast_triangle_pair_contacts == count
+
+
 @pytest.mark.parametrize("far", [0, 200])         # 1 particle: scan; 201: hashed
 def test_particle_contact_closed_form(oracle, far):
     (p, q, v), d = sd.particle_contact(oracle, far_particles=far)

@@ -23,6 +23,19 @@ def test_plane_contact_closed_form_gpu(hip, oracle, quads):
         assert getattr(d, k) == getattr(do, k), k
 
 
+@pytest.mark.parametrize("left_first", [True, False])
+@pytest.mark.parametrize("count", [4, 64])          # 4: wave-cooperative probes; 64: per-lane walks
+def test_valley_tie_gpu(hip, oracle, count, left_first):
+    """Unsigned buckets are stored unordered on the GPU: the depth tie between
+    two mirrored faces must still go to the set's first triangle."""
+    (p, q, v), d = sd.valley_tie(hip, count, left_first)
+    (po, qo, vo), do = sd.valley_tie(oracle, count, left_first)
+    assert np.array_equal(p, po) and np.array_equal(q, qo) and np.array_equal(v, vo)
+    assert np.all(np.sign(p[:, 0]) == (1.0 if left_first else -1.0))
+    for k in ("fast_triangle_pair_contacts", "dynamic_triangle_candidate_count", "dynamic_triangle_aabb_reject_count"):
+        assert getattr(d, k) == getattr(do, k), k
+
+
 @pytest.mark.parametrize("far", [0, 200])
 def test_particle_contact_closed_form_gpu(hip, oracle, far):
     (p, _, _), d = sd.particle_contact(hip, far_particles=far)
