@@ -98,38 +98,54 @@ __global__ void k_large_scatter(int nt, const int* __restrict__ flag, const int*
     if (t < nt && flag[t]) large[off[t]] = t;
 }
 
-// Cell `idx` of triangle t's span (z outer, then y, then x: the reference's
-// loop order); false past the span.
-__device__ __forceinline__ bool span_cell(int4 a, int4 b, int idx, int* x, int* y, int* z) {
-    const int sx = b.x - a.x + 1, sy = b.y - a.y + 1, sz = b.z - a.z + 1;
-    if (sx <= 0 || sy <= 0 || sz <= 0 || idx >= sx * sy * sz) return false;
-    *x = a.x + idx % sx;
-    *y = a.y + (idx / sx) % sy;
-    *z = a.z + idx / (sx * sy);
-    return true;
-}
+// A triangle's cell span as extents and their reciprocals: cell idx of the
+// span (z outer, then y, then x: the reference's loop order) decoded without
+// integer division.  Spans hold at most 64 cells, so for idx < 64 and
+// divisors up to 64 the quotient (idx + 0.5) / d lies at least 1/128 from an
+// integer, far beyond the error of a hardware reciprocal, and truncation gives
+// the exact integer quotient.
+struct CellSpan {
+    int4 lo;
+    int sx, sxy, n;
+    float rsx, rsxy;
+    __device__ __forceinline__ void init(int4 a, int4 b) {
+        lo = a;
+        sx = b.x - a.x + 1;
+        const int sy = b.y - a.y + 1, sz = b.z - a.z + 1;
+        sxy = sx * sy;
+        n = (sx > 0 && sy > 0 && sz > 0) ? sxy * sz : 0;
+        rsx = __builtin_amdgcn_rcpf(static_cast<float>(max(sx, 1)));
+        rsxy = __builtin_amdgcn_rcpf(static_cast<float>(max(sxy, 1)));
+    }
+    __device__ __forceinline__ void cell(int idx, int* x, int* y, int* z) const {
+        const int q = static_cast<int>((static_cast<float>(idx) + 0.5f) * rsxy);
+        const int r = idx - q * sxy;
+        const int w = static_cast<int>((static_cast<float>(r) + 0.5f) * rsx);
+        *x = lo.x + (r - w * sx);
+        *y = lo.y + w;
+        *z = lo.z + q;
+    }
+};
 
 constexpr int kTriLanes = 4;      // lanes per triangle in the hash builds: lane l takes cells l, l + 4, ...
+constexpr int kTriPerBlock = 256 / kTriLanes;
 constexpr int kAggSlots = 1024;   // LDS table of the distinct buckets a block's entries fall in
 constexpr int kAggProbes = 32;    // a bucket with no slot within this many probes counts globally, entry by entry
 
 // Slot of bucket b in a block's LDS table (linear probing from b's low
-// bits), inserting b when `insert`; -1 when the probes run out.  Slots are
-// never freed, so a bucket inserted once is found again at the same place,
-// and one that found no slot never finds one later in the launch.
-__device__ __forceinline__ int agg_slot(int* s_bucket, int* s_list, int* s_n, int b, bool insert) {
+// bits), inserting b; -1 when the probes run out.  Slots are never freed, so
+// a bucket inserted once is found again at the same place, and one that
+// found no slot never finds one later in the launch.
+__device__ __forceinline__ int agg_slot(int* s_bucket, int* s_list, int* s_n, int b) {
     int h = b & (kAggSlots - 1);
-    for (int k = 0; k < kAggProbes; ++k) {
-        const int old = insert ? atomicCAS(&s_bucket[h], -1, b) : s_bucket[h];
-        if (old == b) return h;
-        if (old == -1) {
-            if (!insert) return -1;
-            s_list[atomicAdd(s_n, 1)] = h;
-            return h;
-        }
+    int old = atomicCAS(&s_bucket[h], -1, b);
+    for (int k = 1; old != -1 && old != b; ++k) {
+        if (k == kAggProbes) return -1;
         h = (h + 1) & (kAggSlots - 1);
+        old = atomicCAS(&s_bucket[h], -1, b);
     }
-    return -1;
+    if (old == -1) s_list[atomicAdd(s_n, 1)] = h;
+    return h;
 }
 
 // kTriLanes lanes per triangle (64 triangles per block).  Phase 0 counts the
@@ -137,11 +153,12 @@ __device__ __forceinline__ int agg_slot(int* s_bucket, int* s_list, int* s_n, in
 // the place its decrement returns in the bucket's segment, in arrival order:
 // ranked sets write raw[off[b] + slot] = {key, b} for k_tri_rank to sort;
 // the others, whose queries take the deepest contact with the smallest key
-// whatever the order, write ent[off[b] + slot] = {key, source, layer, 0}.  Neighbouring
-// triangles share most of their cells, so a block first gathers its entries
-// per bucket in an LDS table and makes one global atomic per distinct bucket
-// (device-scope atomics are the cost); phase 1 then walks the cells again and
-// hands out the places of each bucket's block-wide range.
+// whatever the order, write ent[off[b] + slot] = {key, source, layer, 0}.
+// Neighbouring triangles share most of their cells, so a block first gathers
+// its entries per bucket in an LDS table and makes one global atomic per
+// distinct bucket (device-scope atomics are the cost); phase 1 keeps each
+// entry's table slot in LDS and then hands out the places of each bucket's
+// block-wide range.
 __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv, const TriBox* __restrict__ box,
                                                   const int* __restrict__ large_flag, int* count, const int* __restrict__ off,
                                                   int2* raw, int4* ent, const int* __restrict__ tsrc,
@@ -150,6 +167,7 @@ __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv
     __shared__ int s_count[kAggSlots];
     __shared__ int s_base[kAggSlots];
     __shared__ int s_list[kAggSlots];
+    __shared__ short s_slot[kTriPerBlock * kDynMaxHashCells];
     __shared__ int s_n;
     for (int i = static_cast<int>(threadIdx.x); i < kAggSlots; i += static_cast<int>(blockDim.x)) {
         s_bucket[i] = -1;
@@ -157,22 +175,26 @@ __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv
     }
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    const int t = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kTriLanes);
-    const int lane = static_cast<int>(threadIdx.x % kTriLanes);
+    const int tl = static_cast<int>(threadIdx.x) / kTriLanes;
+    const int t = static_cast<int>(blockIdx.x) * kTriPerBlock + tl;
+    const int lane = static_cast<int>(threadIdx.x) % kTriLanes;
     const bool live = t < dv->nt && !large_flag[t];   // a span of at most 64 cells
-    int4 lo = make_int4(0, 0, 0, 0), hi = make_int4(-1, -1, -1, 0);
+    CellSpan sp;
+    sp.init(make_int4(0, 0, 0, 0), make_int4(-1, -1, -1, 0));
     if (live) {
         const TriBox bx = box[t];
-        lo = box_cmin(bx);
-        hi = box_cmax(bx);
+        sp.init(box_cmin(bx), box_cmax(bx));
     }
     const int buckets = dv->tri_buckets;
-    int x, y, z;
-    for (int idx = lane; live && span_cell(lo, hi, idx, &x, &y, &z); idx += kTriLanes) {
+    short* my_slot = s_slot + tl * kDynMaxHashCells;
+    for (int idx = lane; idx < sp.n; idx += kTriLanes) {
+        int x, y, z;
+        sp.cell(idx, &x, &y, &z);
         const int b = static_cast<int>(hash_cell(x, y, z, buckets));
-        const int h = agg_slot(s_bucket, s_list, &s_n, b, true);
+        const int h = agg_slot(s_bucket, s_list, &s_n, b);
         if (h >= 0) atomicAdd(&s_count[h], 1);
         else if (phase == 0) atomicAdd(&count[b], 1);
+        my_slot[idx] = static_cast<short>(h);
     }
     __syncthreads();
     const int n = s_n;
@@ -189,10 +211,18 @@ __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv
     __syncthreads();
     int2 sl = make_int2(0, 0);
     if (live && !ranked && tsrc) sl = make_int2(tsrc[t], tlayer[t]);
-    for (int idx = lane; live && span_cell(lo, hi, idx, &x, &y, &z); idx += kTriLanes) {
-        const int b = static_cast<int>(hash_cell(x, y, z, buckets));
-        const int h = agg_slot(s_bucket, s_list, &s_n, b, false);
-        const int slot = h >= 0 ? s_base[h] + atomicAdd(&s_count[h], 1) : atomicSub(&count[b], 1) - 1;
+    for (int idx = lane; idx < sp.n; idx += kTriLanes) {
+        const int h = my_slot[idx];
+        int b, slot;
+        if (h >= 0) {
+            b = s_bucket[h];
+            slot = s_base[h] + atomicAdd(&s_count[h], 1);
+        } else {
+            int x, y, z;
+            sp.cell(idx, &x, &y, &z);
+            b = static_cast<int>(hash_cell(x, y, z, buckets));
+            slot = atomicSub(&count[b], 1) - 1;
+        }
         const int at = off[b] + slot, key = t * kDynMaxHashCells + idx;
         if (ranked) raw[at] = make_int2(key, b);
         else ent[at] = make_int4(key, sl.x, sl.y, 0);
@@ -250,10 +280,19 @@ __global__ __launch_bounds__(256) void k_bucket_bins(const int* __restrict__ off
         const int base = off[b], n = off[b + 1] - base;
         int min_layer = rec ? 0x7fffffff : static_cast<int>(0x80000000u), src = -2;
         if (rec) {
-            for (int k = 0; k < min(n, cap); ++k) {
-                const int2 e = entry_source_layer(rec[base + k]);
-                min_layer = min(min_layer, e.y);
-                src = (src == -2 || src == e.x) ? e.x : -1;
+            // four entries in flight per round (the merge is order-free: a
+            // minimum, and whether all sources agree)
+            const int m = min(n, cap);
+            for (int k = 0; k < m; k += 4) {
+                int2 e[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) e[u] = k + u < m ? entry_source_layer(rec[base + k + u]) : make_int2(-2, 0x7fffffff);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (k + u >= m) break;
+                    min_layer = min(min_layer, e[u].y);
+                    src = (src == -2 || src == e[u].x) ? e[u].x : -1;
+                }
             }
         } else if (n > 0) {
             src = -1;
