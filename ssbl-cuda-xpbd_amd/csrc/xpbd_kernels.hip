@@ -339,35 +339,21 @@ __device__ __forceinline__ bool cell_in_reach(F3 p, int cx, int cy, int cz, int 
 
 // The deepest contact of a sequence visited in order is the first one of
 // greatest depth.  Depths are positive, so (depth bits, ~order) compares as
-// one unsigned 64-bit key whose wave maximum names that contact.
+// one unsigned 64-bit key whose maximum names that contact.
 __device__ __forceinline__ unsigned long long depth_key(float pen, unsigned order) {
     return pen > 0.0f ? (static_cast<unsigned long long>(__float_as_uint(pen)) << 32) | (0xffffffffu - order) : 0ull;
 }
 
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
-    for (int d = 32; d > 0; d >>= 1) {
-        const unsigned long long o = __shfl_xor(k, d, 64);
-        k = o > k ? o : k;
-    }
-    return k;
-}
-
-__device__ __forceinline__ unsigned wave_sum_u32(unsigned x) {
-    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-    return x;
+// Orders a wave's LDS writes before its later LDS reads of other lanes'
+// slots (a wave's LDS operations complete in issue order; this keeps the
+// compiler from moving them across, and waits for those in flight).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
 }
 
 __device__ __forceinline__ F3 shfl3(F3 a, int lane) { return f3(__shfl(a.x, lane, 64), __shfl(a.y, lane, 64), __shfl(a.z, lane, 64)); }
-
-// The wave's best key and the push of the lane that holds it.
-__device__ __forceinline__ unsigned long long wave_best(unsigned long long key, F3 delta, F3& best_delta) {
-    const unsigned long long top = wave_max_u64(key);
-    if (top != 0ull) {
-        const int owner = __ffsll(static_cast<long long>(__ballot(key == top))) - 1;
-        best_delta = shfl3(delta, owner);
-    }
-    return top;
-}
 
 // Dynamic colliders, unsigned contacts: the deepest triangle contact
 // (A41:3999-4323) then the deepest particle contact (A41:4354-4511) push the
@@ -375,28 +361,25 @@ __device__ __forceinline__ unsigned long long wave_best(unsigned long long key, 
 // read and written, so running both here equals the reference's two
 // vertex-parallel launches.
 //
-// A vertex's hash probe is a serial chain of dependent loads, and often only
-// a few vertices of a wave have one to make (the rest are turned away by a
-// bucket bin or the coarse summary with one load).  A wave with at most a few
-// probes serves them one at a time with all 64 lanes: a lane per entry of the
-// triangle bucket, a lane per cell of the 27-cell particle search.  Each lane
-// keeps the first deepest contact of its own run and the wave takes the
-// deepest with the smallest visiting order, which is the contact the serial
-// visit in the reference's order keeps.  A wave with more probes (cloths in
-// contact) lets every lane walk its own, in that order.  Both forms give the
-// same result.  The entries of an unsigned triangle bucket are stored in no
-// particular order, so there the visiting order is the key (triangle * 64 +
-// cell index): the deepest contact with the smallest key is the one a walk
-// of the bucket in ascending key order keeps.  Every lane of the wave must
-// call this (`active` false for lanes without a vertex).
-#ifndef SSBL_COOP_TRIANGLE_PROBES
-#define SSBL_COOP_TRIANGLE_PROBES 32
-#endif
-#ifndef SSBL_COOP_PARTICLE_PROBES
-#define SSBL_COOP_PARTICLE_PROBES 16
-#endif
-constexpr int kCoopTriangleProbes = SSBL_COOP_TRIANGLE_PROBES;
-constexpr int kCoopParticleProbes = SSBL_COOP_PARTICLE_PROBES;
+// A vertex's hash probe is a chain of dependent loads, its length set by
+// the vertex's bucket; vertices of one wave have buckets of very different
+// sizes (most are turned away by a bucket bin or the coarse summary with one
+// load).  So a wave pools its probes into one list of items -- (vertex,
+// bucket entry) for triangles, (vertex, neighbour cell) for particles -- and
+// its lanes take 64 items at a time.  Each item's (depth, ~visiting order)
+// goes into its vertex's maximum in LDS: the deepest contact, the first in
+// the reference's visiting order among equals.  The vertex then evaluates
+// its winner once more for the push.  The entries of an unsigned triangle
+// bucket are stored in no particular order, so there the visiting order is
+// the key (triangle * 64 + cell index), the order of a walk of the bucket in
+// ascending key order.  Every lane of the wave must call these (`active`
+// false for lanes without a vertex).
+// Above this many probing vertices a wave's particle searches run one per
+// lane (27 cells each, a z plane of bins loaded together): the pooled list
+// pays its owner lookups on every item, and with most lanes probing the
+// per-lane walks are already balanced.
+constexpr int kPooledParticleProbes = 16;
+
 __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState& s, bool active, float dt, bool count) {
     const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
     const int lane = static_cast<int>(__lane_id());
@@ -414,58 +397,63 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                 bin = gload(D.tri_bin + hash_cell(cx, cy, cz, D.tri_buckets));
                 if (dyn_bucket_rejected(c, bin)) bin.y = 0;
             }
-            unsigned long long pending = __ballot(bin.y > 0);
-            if (__popcll(pending) > kCoopTriangleProbes) {
-                // many probes in this wave: each lane walks its own bucket
-                pending = 0ull;
-                unsigned long long best_key = 0ull;
-                for (int k = bin.x; k < bin.x + bin.y; ++k) {
-                    const int4 en = gload(D.tri_ent + k);
-                    if (D.tsrc && !dyn_enabled(c, en.y, en.z)) continue;
-                    const int key = en.x;
-                    const int t = key / kDynMaxHashCells;
-                    if (t < 0 || t >= nt) continue;
-                    const TriBox bx = gload(D.box + t);
-                    const int4 lo = box_cmin(bx), hi = box_cmax(bx);
-                    if (cx < lo.x || cx > hi.x || cy < lo.y || cy > hi.y || cz < lo.z || cz > hi.z) continue;
-                    F3 d;
-                    const unsigned long long kk = depth_key(dyn_triangle_depth(D, t, bx, s.p, cm, cand, rej, d), static_cast<unsigned>(key));
-                    if (kk > best_key) { best_key = kk; best_delta = d; }
-                }
-                if (best_key != 0ull) best_pen = __uint_as_float(static_cast<unsigned>(best_key >> 32));
+            // The wave's probes as one list of (vertex, entry) items, 64 at a
+            // time: lane i owns items [start_i, start_i + n_i); item w belongs
+            // to the last lane whose start is at most w.  Each item's
+            // (depth, ~key) goes into its vertex's LDS maximum; the winner's
+            // push is recomputed by the vertex afterwards.
+            const int n_i = bin.y;
+            int incl = n_i;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int o = __shfl_up(incl, d, 64);
+                if (lane >= d) incl += o;
             }
-            while (pending) {
-                const int leader = __ffsll(static_cast<long long>(pending)) - 1;
-                pending &= pending - 1;
-                const F3 lp = shfl3(s.p, leader);
-                const int lx = __shfl(cx, leader, 64), ly = __shfl(cy, leader, 64), lz = __shfl(cz, leader, 64);
-                const int b0 = __shfl(bin.x, leader, 64), nb = __shfl(bin.y, leader, 64);
-                unsigned long long key = 0ull;
-                F3 delta = f3(0.0f, 0.0f, 0.0f);
-                unsigned lc = 0, lr = 0;
-                for (int e = lane; e < nb; e += 64) {
-                    const int k = b0 + e;
-                    // the source filter and the canonical-cell test (A41:3934-3982) are
-                    // both side-effect free, so the cheaper one (in the entry) runs first
-                    const int4 en = gload(D.tri_ent + k);
-                    if (D.tsrc && !dyn_enabled(c, en.y, en.z)) continue;
-                    const int tkey = en.x;
-                    const int t = tkey / kDynMaxHashCells;
-                    if (t < 0 || t >= nt) continue;
-                    const TriBox bx = gload(D.box + t);
-                    const int4 lo = box_cmin(bx), hi = box_cmax(bx);
-                    if (lx < lo.x || lx > hi.x || ly < lo.y || ly > hi.y || lz < lo.z || lz > hi.z) continue;
-                    F3 d;
-                    const unsigned long long kk = depth_key(dyn_triangle_depth(D, t, bx, lp, cm, lc, lr, d), static_cast<unsigned>(tkey));
-                    if (kk > key) { key = kk; delta = d; }
+            const int start = incl - n_i;
+            const int total = __shfl(incl, 63, 64);
+            if (total > 0) {   // uniform over the wave
+                __shared__ unsigned long long s_best[kTile];
+                unsigned long long* best = s_best + (threadIdx.x & ~63u);
+                best[lane] = 0ull;
+                wave_lds_sync();
+                const int first = bin.x - start;   // entry of item w is first + w in the owner's bucket
+                for (int base = 0; base < total; base += 64) {
+                    const int w = base + lane;
+                    int own = 0;
+                    for (int step = 32; step > 0; step >>= 1) {
+                        const int probe = own + step;
+                        const int sp = __shfl(start, min(probe, 63), 64);
+                        if (probe < 64 && sp <= w) own = probe;
+                    }
+                    const int ofirst = __shfl(first, own, 64);
+                    const int ox = __shfl(cx, own, 64), oy = __shfl(cy, own, 64), oz = __shfl(cz, own, 64);
+                    const F3 op = shfl3(s.p, own);
+                    if (w < total) {
+                        const int k = ofirst + w;
+                        const int4 en = gload(D.tri_ent + k);
+                        const int t = en.x / kDynMaxHashCells;
+                        // the source filter and the canonical-cell test (A41:3934-3982)
+                        // are both side-effect free, so the one in the entry runs first
+                        if ((!D.tsrc || dyn_enabled(c, en.y, en.z)) && t >= 0 && t < nt) {
+                            const TriBox bx = gload(D.box + t);
+                            const int4 lo = box_cmin(bx), hi = box_cmax(bx);
+                            if (ox >= lo.x && ox <= hi.x && oy >= lo.y && oy <= hi.y && oz >= lo.z && oz <= hi.z) {
+                                F3 d;
+                                const unsigned long long kk =
+                                    depth_key(dyn_triangle_depth(D, t, bx, op, cm, cand, rej, d), static_cast<unsigned>(en.x));
+                                if (kk != 0ull) atomicMax(best + own, kk);
+                            }
+                        }
+                    }
                 }
-                F3 wd = f3(0.0f, 0.0f, 0.0f);
-                const unsigned long long top = wave_best(key, delta, wd);
-                const unsigned sc = wave_sum_u32(lc), sr = wave_sum_u32(lr);
-                if (lane == leader) {
-                    cand += sc;
-                    rej += sr;
-                    if (top != 0ull) { best_pen = __uint_as_float(static_cast<unsigned>(top >> 32)); best_delta = wd; }
+                wave_lds_sync();
+                const unsigned long long top = best[lane];
+                if (top != 0ull) {
+                    // the winning entry again, for its push (same inputs, same result)
+                    const int t = static_cast<int>(0xffffffffu - static_cast<unsigned>(top)) / kDynMaxHashCells;
+                    unsigned dc = 0, dr = 0;
+                    F3 d;
+                    best_pen = dyn_triangle_depth(D, t, gload(D.box + t), s.p, cm, dc, dr, d);
+                    best_delta = d;
                 }
             }
             if (active) {
@@ -529,10 +517,15 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                 probe = !(e.y == -2 || (c.dyn_mode != 0 && ((c.dyn_target_src != 0 && e.y == c.dyn_target_src) ||
                                                             (c.dyn_mode == 2 && e.x >= c.dyn_target_layer))));
             }
-            unsigned long long pending = __ballot(probe);
-            if (__popcll(pending) > kCoopParticleProbes) {
+            // Otherwise the wave's searches as one list of (vertex, cell)
+            // items, 27 per probing vertex, 64 at a time (the owner lookup of
+            // the triangle query).  Cell c = 9 (dz + 1) + 3 (dy + 1) + (dx + 1) is the
+            // reference's visiting order, x innermost; an item's entries keep
+            // their bucket order k, so (depth, ~(16 c + k)) orders contacts as
+            // the serial search meets them.
+            const unsigned long long probes = __ballot(probe);
+            if (__popcll(probes) > kPooledParticleProbes) {
                 // many probes in this wave: each lane searches its own 27 cells
-                pending = 0ull;
                 if (probe) {
                     const int cx = cell_coord(s.p.x, D.part_cell), cy = cell_coord(s.p.y, D.part_cell), cz = cell_coord(s.p.z, D.part_cell);
                     // the bins of a z plane are loaded together, then visited in
@@ -566,44 +559,72 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                     }
                 }
             }
-            while (pending) {
-                const int leader = __ffsll(static_cast<long long>(pending)) - 1;
-                pending &= pending - 1;
-                const F3 lp = shfl3(s.p, leader);
-                unsigned long long key = 0ull;
-                F3 delta = f3(0.0f, 0.0f, 0.0f);
-                unsigned lc = 0;
-                // lane = 9 (dz + 1) + 3 (dy + 1) + (dx + 1): the reference's order, x innermost
-                const int lcx = cell_coord(lp.x, D.part_cell), lcy = cell_coord(lp.y, D.part_cell), lcz = cell_coord(lp.z, D.part_cell);
-                if (lane < 27 && cell_in_reach(lp, lcx, lcy, lcz, lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1, D.part_cell, reach)) {
-                    const int qx = lcx + lane % 3 - 1;
-                    const int qy = lcy + (lane / 3) % 3 - 1;
-                    const int qz = lcz + lane / 9 - 1;
-                    const int4 bin = gload(D.pbin + hash_cell(qx, qy, qz, D.part_buckets));
-                    if (!dyn_bucket_rejected(c, bin)) {
-                        const int lim = min(bin.y, kDynParticleSlots);
-                        for (int k = 0; k < lim; ++k) {
-                            const int4 pc = gload(D.pkcell + bin.x + k);
-                            if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
-                            if (D.pksl) {
-                                const int2 sl = gload(D.pksl + bin.x + k);
-                                if (!dyn_enabled(c, sl.x, sl.y)) continue;
+            const int n_i = probe && __popcll(probes) <= kPooledParticleProbes ? 27 : 0;
+            int incl = n_i;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int o = __shfl_up(incl, d, 64);
+                if (lane >= d) incl += o;
+            }
+            const int start = incl - n_i;
+            const int total = __shfl(incl, 63, 64);
+            if (total > 0) {   // uniform over the wave
+                __shared__ unsigned long long s_best[kTile];
+                unsigned long long* best = s_best + (threadIdx.x & ~63u);
+                best[lane] = 0ull;
+                wave_lds_sync();
+                int cx = 0, cy = 0, cz = 0;
+                if (probe) { cx = cell_coord(s.p.x, D.part_cell); cy = cell_coord(s.p.y, D.part_cell); cz = cell_coord(s.p.z, D.part_cell); }
+                for (int base = 0; base < total; base += 64) {
+                    const int w = base + lane;
+                    int own = 0;
+                    for (int step = 32; step > 0; step >>= 1) {
+                        const int probe_lane = own + step;
+                        const int sp = __shfl(start, min(probe_lane, 63), 64);
+                        if (probe_lane < 64 && sp <= w) own = probe_lane;
+                    }
+                    const int ostart = __shfl(start, own, 64);
+                    const int ox = __shfl(cx, own, 64), oy = __shfl(cy, own, 64), oz = __shfl(cz, own, 64);
+                    const F3 op = shfl3(s.p, own);
+                    if (w < total) {
+                        const int cell = w - ostart;
+                        const int dx = cell % 3 - 1, dy = (cell / 3) % 3 - 1, dz = cell / 9 - 1;
+                        if (cell_in_reach(op, ox, oy, oz, dx, dy, dz, D.part_cell, reach)) {
+                            const int qx = ox + dx, qy = oy + dy, qz = oz + dz;
+                            const int4 bin = gload(D.pbin + hash_cell(qx, qy, qz, D.part_buckets));
+                            if (!dyn_bucket_rejected(c, bin)) {
+                                unsigned long long key = 0ull;
+                                const int lim = min(bin.y, kDynParticleSlots);
+                                for (int k = 0; k < lim; ++k) {
+                                    const int4 pc = gload(D.pkcell + bin.x + k);
+                                    if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
+                                    if (D.pksl) {
+                                        const int2 sl = gload(D.pksl + bin.x + k);
+                                        if (!dyn_enabled(c, sl.x, sl.y)) continue;
+                                    }
+                                    const int j = gload(D.pkeys + bin.x + k);
+                                    if (j < 0 || j >= np_count) continue;
+                                    F3 d;
+                                    const unsigned long long kk = depth_key(dyn_particle_depth(D, j, op, margin, cand, d),
+                                                                            static_cast<unsigned>(cell * kDynParticleSlots + k));
+                                    key = kk > key ? kk : key;
+                                }
+                                if (key != 0ull) atomicMax(best + own, key);
                             }
-                            const int j = gload(D.pkeys + bin.x + k);
-                            if (j < 0 || j >= np_count) continue;
-                            F3 d;
-                            const unsigned long long kk =
-                                depth_key(dyn_particle_depth(D, j, lp, margin, lc, d), static_cast<unsigned>(lane * kDynParticleSlots + k));
-                            if (kk > key) { key = kk; delta = d; }
                         }
                     }
                 }
-                F3 wd = f3(0.0f, 0.0f, 0.0f);
-                const unsigned long long top = wave_best(key, delta, wd);
-                const unsigned sc = wave_sum_u32(lc);
-                if (lane == leader) {
-                    cand += sc;
-                    if (top != 0ull) { best_pen = __uint_as_float(static_cast<unsigned>(top >> 32)); best_delta = wd; }
+                wave_lds_sync();
+                const unsigned long long top = best[lane];
+                if (top != 0ull) {
+                    // the winning particle again, for its push (same inputs, same result)
+                    const unsigned order = 0xffffffffu - static_cast<unsigned>(top);
+                    const int cell = static_cast<int>(order) / kDynParticleSlots, k = static_cast<int>(order) % kDynParticleSlots;
+                    const int qx = cx + cell % 3 - 1, qy = cy + (cell / 3) % 3 - 1, qz = cz + cell / 9 - 1;
+                    const int4 bin = gload(D.pbin + hash_cell(qx, qy, qz, D.part_buckets));
+                    unsigned dc = 0;
+                    F3 d;
+                    best_pen = dyn_particle_depth(D, gload(D.pkeys + bin.x + k), s.p, margin, dc, d);
+                    best_delta = d;
                 }
             }
         } else if (active) {
