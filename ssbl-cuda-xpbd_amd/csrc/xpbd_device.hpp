@@ -159,7 +159,9 @@ struct LaunchCtx {
 
 void launch_integrate(const LaunchCtx& c, int parity, unsigned ops, bool full);
 void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops);
-void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops);
+// tail_ops: the closing pin polish and velocity update, fused into the
+// particle launch when nothing else runs between (0: none)
+void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops);
 // Signed contacts: the pass after self collision (inside = false) and the
 // inside-only passes of a step's last iteration (A41:9322-9359).
 void launch_signed_pass(const LaunchCtx& c, int parity, bool inside, unsigned ops);

@@ -921,13 +921,18 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
 // Dynamic collider contacts of the last iteration (A41:9230-9252) as passes
 // of their own (the hash probes want registers the constraint passes do
 // not): triangles (kind 0), then particles (kind 1), as the reference's two
-// launches.
+// launches.  `tail_ops` (kind 1 only) are the iteration's closing vertex-local
+// steps when nothing else runs before them: the second pin projection and
+// the velocity update, exactly as the in-place vertex pass would apply them,
+// for every vertex of every cloth in the launch.
 template <int kKind>
 __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
-                                                        int ntiles, int parity, unsigned ops) {
+                                                        int ntiles, int parity, unsigned ops, unsigned tail_ops) {
     const Tile tile = tiles[xcd_tile(ntiles)];
     const ClothView& c = views[tile.cloth];
-    if (!c.dyn) return;   // uniform over the block
+    const bool has_dyn = c.dyn != nullptr;
+    const bool tail = (tail_ops & kOpVelocity) != 0u;
+    if (!has_dyn && !tail) return;   // uniform over the block
     const int v = tile.begin + static_cast<int>(threadIdx.x);
     // every lane stays for the wave-wide probes; lanes past the tile or on a
     // pinned vertex take part without a vertex of their own
@@ -936,9 +941,20 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     const float4 p4 = in_tile ? gload(pos + v) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const bool active = in_tile && p4.w != 0.0f;
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
-    if (kKind == 0) dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
-    else dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u);
-    if (!active) return;
+    if (has_dyn) {
+        if (kKind == 0) dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
+        else dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u);
+    }
+    if (!in_tile) return;
+    if (tail) {
+        const float dt = c.dt_sub;
+        const bool pinned = p4.w == 0.0f;
+        if ((tail_ops & kOpPinPolish) && ((tail_ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt);
+        if (pinned) s.set_vel(f3(0.0f, 0.0f, 0.0f));
+        else velocity_update(c, s, dt);
+    } else if (!active) {
+        return;
+    }
     if (s.p.x != p4.x || s.p.y != p4.y || s.p.z != p4.z) gstore(pos + v, with_w(s.p, p4.w));
     if (s.q_dirty) gstore(c.prev + v, with_w(s.q_, 0.0f));
     if (s.v_dirty) gstore(c.vel + v, with_w(s.vel_, 0.0f));
@@ -1583,10 +1599,10 @@ void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsi
         launch_k(k_vertex_pass<kGatherNone>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
 }
 
-void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops) {
+void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops) {
     TimedLaunch t("dynamic_contacts", c.stream);
-    launch_k(k_dynamic_pass<0>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
-    launch_k(k_dynamic_pass<1>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
+    launch_k(k_dynamic_pass<0>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops, 0u);
+    launch_k(k_dynamic_pass<1>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops, tail_ops);
 }
 
 void launch_signed_pass(const LaunchCtx& c, int parity, bool inside, unsigned ops) {

@@ -894,30 +894,42 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             // A41:9178-9369 order: stretch (direct or PCG + polish + caps),
             // final LRA and bends, pins, analytic and SDF contacts, self
             // collision, final caps, second pin projection
-            launch_vertex_pass(ctx, parity, parity ^ 1, stretch | cnt);
-            parity ^= 1;
-            if (run_pcg) {
-                per_pcg([&](Solver& s, const ClothView* cv) {
-                    s.pcg.solve(st, cv, s.d_tiles.p, parity);
-                    return s.pcg.polish(st, cv, s.d_tiles.p, parity, s.view.dt_sub);
-                });
-            }
-            if (final_c && plan.any_lra) {
-                launch_vertex_pass(ctx, parity, parity ^ 1, kOpLra | cnt);
+            const bool tail = self_now || dyn_now || signed_now || (run_pcg && final_c);
+            const unsigned contact_ops =
+                (tail ? (pin_ops & ~kOpPinPolish) : (pin_ops | last_op | kOpVelocity)) | kOpAnalytic | kOpSdf | cnt;
+            if (!run_pcg && !(final_c && (plan.any_lra || plan.any_bend))) {
+                // nothing runs between the stretch gather and the pins and
+                // contacts: one pass does both, as the unsplit iteration does
+                launch_vertex_pass(ctx, parity, parity ^ 1, stretch | contact_ops);
                 parity ^= 1;
-            }
-            if (final_c && plan.any_bend) {
-                for (size_t c = 0; c < seg_first.size(); ++c)
-                    launch_bend_color(ctx, bend_segs_dev + seg_first[c], seg_count[c], parity, cnt);
-                if (plan.any_bend_gather) {
-                    launch_vertex_pass(ctx, parity, parity ^ 1, kOpBend | cnt);
+            } else {
+                launch_vertex_pass(ctx, parity, parity ^ 1, stretch | cnt);
+                parity ^= 1;
+                if (run_pcg) {
+                    per_pcg([&](Solver& s, const ClothView* cv) {
+                        s.pcg.solve(st, cv, s.d_tiles.p, parity);
+                        return s.pcg.polish(st, cv, s.d_tiles.p, parity, s.view.dt_sub);
+                    });
+                }
+                if (final_c && plan.any_lra) {
+                    launch_vertex_pass(ctx, parity, parity ^ 1, kOpLra | cnt);
                     parity ^= 1;
                 }
+                if (final_c && plan.any_bend) {
+                    for (size_t c = 0; c < seg_first.size(); ++c)
+                        launch_bend_color(ctx, bend_segs_dev + seg_first[c], seg_count[c], parity, cnt);
+                    if (plan.any_bend_gather) {
+                        launch_vertex_pass(ctx, parity, parity ^ 1, kOpBend | cnt);
+                        parity ^= 1;
+                    }
+                }
+                launch_vertex_pass(ctx, parity, parity, contact_ops);
             }
-            const bool tail = self_now || dyn_now || signed_now || (run_pcg && final_c);
-            launch_vertex_pass(ctx, parity, parity,
-                               (tail ? (pin_ops & ~kOpPinPolish) : (pin_ops | last_op | kOpVelocity)) | kOpAnalytic | kOpSdf | cnt);
-            if (dyn_now) launch_dynamic_pass(ctx, parity, cnt);
+            const unsigned tail_ops = (pin_ops & kOpPinPolish) | last_op | kOpVelocity | cnt;
+            // the dynamic contacts end the iteration: their particle launch
+            // applies the closing steps itself
+            const bool tail_fused = dyn_now && !self_now && !signed_now && !(run_pcg && final_c);
+            if (dyn_now) launch_dynamic_pass(ctx, parity, cnt, tail_fused ? tail_ops : 0u);
             if (self_now) {
                 if (!self_batch || !self_batch->run(ctx, parity, sub, substeps, count)) return false;
             }
@@ -926,7 +938,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
                 per_pcg([&](Solver& s, const ClothView* cv) { return s.pcg.final_caps(st, cv, s.d_tiles.p, parity, s.view.dt_sub); });
             }
             if (signed_now && final_c) launch_signed_pass(ctx, parity, true, cnt);
-            if (tail) launch_vertex_pass(ctx, parity, parity, (pin_ops & kOpPinPolish) | last_op | kOpVelocity | cnt);
+            if (tail && !tail_fused) launch_vertex_pass(ctx, parity, parity, tail_ops);
         }
     }
     if (parity != 0) launch_vertex_pass(ctx, 1, 0, 0u);
