@@ -746,8 +746,6 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
     margin = std::max(margin, 0.0f);
     thickness = std::max(thickness, 0.0f);
     const bool keep_prev = np == np_ && np > 0;
-    if (!sync_stream()) return false;
-    if (!reserve_particles(std::max(np, 1)) || !reserve_triangles(std::max(nt, 1))) return false;
     // segment tables: {source, begin, end} per 256 items
     std::vector<int4> segs;
     for (size_t k = 0; k < sources.size(); ++k)
@@ -755,6 +753,23 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
     const int npseg = static_cast<int>(segs.size());
     for (size_t k = 0; k < sources.size(); ++k)
         for (int b = 0; b < sources[k].nt; b += 256) segs.push_back(make_int4(static_cast<int>(k), b, std::min(sources[k].nt, b + 256), 1));
+    // The same cloths as last time (the common case: a scene refreshed every
+    // frame): the device tables already hold these sources, so no upload and
+    // no wait for the kernels still reading them.
+    auto same_source = [](const GatherSource& x, const GatherSource& y) {
+        return x.pos == y.pos && x.tris == y.tris && x.nv == y.nv && x.nt == y.nt && x.pbase == y.pbase && x.tbase == y.tbase &&
+               std::memcmp(&x.radius, &y.radius, sizeof(float)) == 0 && x.source_id == y.source_id && x.layer == y.layer;
+    };
+    bool same = np == np_ && nt == nt_ && sources.size() == gathered_.size() && d_sources_ && d_segments_;
+    for (size_t k = 0; same && k < sources.size(); ++k) same = same_source(sources[k], gathered_[k]);
+    if (same) {
+        np_ = np;
+        nt_ = nt;
+        return launch_gather(st, npseg, static_cast<int>(segs.size()) - npseg, keep_prev, max_radius, reach_radius, margin, thickness);
+    }
+    if (!sync_stream()) return false;
+    gathered_.clear();
+    if (!reserve_particles(std::max(np, 1)) || !reserve_triangles(std::max(nt, 1))) return false;
     if (static_cast<int>(sources.size()) > sources_cap_ || static_cast<int>(segs.size()) > segments_cap_) {
         release(4);
         const size_t ns = std::max<size_t>(sources.size(), static_cast<size_t>(sources_cap_));
@@ -770,8 +785,14 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
         return false;
     if (!segs.empty() && !hip_ok(hipMemcpy(d_segments_, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice), "scene sources"))
         return false;
+    gathered_ = sources;
     np_ = np;
     nt_ = nt;
+    return launch_gather(st, npseg, static_cast<int>(segs.size()) - npseg, keep_prev, max_radius, reach_radius, margin, thickness);
+}
+
+bool DynamicColliders::launch_gather(hipStream_t st, int npseg, int ntseg, bool keep_prev, float max_radius, float reach_radius,
+                                     float margin, float thickness) {
     bounds_valid_ = false;
     part_cell_ = std::max(std::max(max_radius + margin, thickness), 1.0e-3f);
     max_radius_ = reach_radius;
@@ -784,7 +805,6 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
     TimedLaunch t("scene_gather", st);
     if (npseg > 0) launch_k(k_gather_particles, dim3(npseg), dim3(256), 0, st, d_sources_, d_segments_, pp_, pprev_, psrc_, player_, keep_prev ? 1 : 0);
     launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, extent_bits_, 0, 1);
-    const int ntseg = static_cast<int>(segs.size()) - npseg;
     if (ntseg > 0) launch_k(k_gather_triangles, dim3(ntseg), dim3(256), 0, st, d_sources_, d_segments_ + npseg, tv_, tsrc_, tlayer_, extent_bits_);
     launch_k(k_scene_cell, dim3(1), dim3(1), 0, st, d_view_, extent_bits_, margin, thickness);
     return hip_ok(hipGetLastError(), "scene gather");

@@ -152,6 +152,9 @@ private:
     bool reserve_triangles(int n);
     bool reserve_particles(int n);
     bool upload_view(hipStream_t st);
+    // the gather kernels over the uploaded source and segment tables
+    bool launch_gather(hipStream_t st, int npseg, int ntseg, bool keep_prev, float max_radius, float reach_radius, float margin,
+                       float thickness);
     bool compute_bounds(hipStream_t st);
     // allocation groups: 0 triangles, 1 particles, 2 scan scratch, 3 the view, 4 gather tables
     void* alloc(int group, size_t bytes);
@@ -192,6 +195,7 @@ private:
     int coarse_buckets_ = 0;
     float max_radius_ = 0.0f;
     int* scan_tmp_ = nullptr;
+    std::vector<GatherSource> gathered_;   // the sources the device tables hold (empty: none valid)
     GatherSource* d_sources_ = nullptr;
     int sources_cap_ = 0;
     int4* d_segments_ = nullptr;
