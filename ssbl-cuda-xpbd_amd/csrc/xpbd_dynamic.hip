@@ -349,41 +349,83 @@ __global__ __launch_bounds__(256) void k_coarse_fill(int2* t, int n, int* off) {
     if (i == 0) *off = 0;
 }
 
+// Merges a {smallest layer, one source | -1 mixed | -2 empty} summary into
+// coarse entry e: order-free, so a block may merge its own summary first.
+__device__ __forceinline__ void coarse_merge(int2* e, int layer, int src) {
+    // plain coherent reads first: most entries already hold the outcome
+    if (__hip_atomic_load(&e->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > layer) atomicMin(&e->x, layer);
+    int cur = __hip_atomic_load(&e->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        const int want = cur == -2 ? src : (cur == src ? cur : -1);
+        if (want == cur) break;
+        const int prev = atomicCAS(&e->y, cur, want);
+        if (prev == cur) break;
+        cur = prev;
+    }
+}
+
+// Every particle's reach box marks the coarse cells it overlaps.  The
+// particles of a block (neighbours on a cloth) share most of their cells, so
+// the block merges its marks per bucket in LDS first and then merges each
+// distinct bucket into the table once.
 __global__ __launch_bounds__(256) void k_coarse_insert(const DynView* __restrict__ dv, int2* t, int* off) {
+    __shared__ int s_bucket[kAggSlots];
+    __shared__ int s_layer[kAggSlots];
+    __shared__ int s_src[kAggSlots];
+    __shared__ int s_list[kAggSlots];
+    __shared__ int s_n;
+    for (int k = static_cast<int>(threadIdx.x); k < kAggSlots; k += static_cast<int>(blockDim.x)) {
+        s_bucket[k] = -1;
+        s_layer[k] = 0x7fffffff;
+        s_src[k] = -2;
+    }
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const DynView& D = *dv;
-    if (i >= D.np) return;
-    const float4 p = D.pp[i];
-    if (!isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z)) { atomicExch(off, 1); return; }
-    const int src = D.psrc ? D.psrc[i] : -1;
-    const int layer = D.psrc ? (D.player ? D.player[i] : 0) : static_cast<int>(0x80000000u);
+    bool live = i < D.np;
+    const float4 p = live ? D.pp[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (live && (!isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z))) { atomicExch(off, 1); live = false; }
+    const int src = D.psrc && live ? D.psrc[i] : -1;
+    const int layer = D.psrc ? (D.player && live ? D.player[i] : 0) : static_cast<int>(0x80000000u);
     const float cc = D.coarse_cell, r = D.coarse_reach;
-    int lo[3], n[3];
+    int lo[3] = {0, 0, 0}, n[3] = {0, 0, 0};
     const float c3[3] = {p.x, p.y, p.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
+        if (!live) break;
         const float reach = r * 1.001f + fabsf(c3[a]) * 1.0e-5f + 1.0e-6f;
         lo[a] = cell_coord(c3[a] - reach, cc);
         const long long span = static_cast<long long>(cell_coord(c3[a] + reach, cc)) - lo[a] + 1;
         // the box is under a coarse cell wide, so this fails only where the
         // cell coordinates saturate: then the summary is not used at all
-        if (span < 1 || span > 3) { atomicExch(off, 1); return; }
+        if (span < 1 || span > 3) { atomicExch(off, 1); live = false; break; }
         n[a] = static_cast<int>(span);
     }
-    for (int kz = 0; kz < n[2]; ++kz)
-        for (int ky = 0; ky < n[1]; ++ky)
-            for (int kx = 0; kx < n[0]; ++kx) {
-                int2* e = t + hash_cell(lo[0] + kx, lo[1] + ky, lo[2] + kz, D.coarse_buckets);
-                // plain coherent reads first: neighbouring particles mostly
-                // find their cells already marked and issue no atomic at all
-                if (__hip_atomic_load(&e->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > layer) atomicMin(&e->x, layer);
-                int cur = __hip_atomic_load(&e->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                while (cur != src && cur != -1) {
-                    const int prev = atomicCAS(&e->y, cur, cur == -2 ? src : -1);
-                    if (prev == cur) break;
-                    cur = prev;
+    if (live) {
+        for (int kz = 0; kz < n[2]; ++kz)
+            for (int ky = 0; ky < n[1]; ++ky)
+                for (int kx = 0; kx < n[0]; ++kx) {
+                    const int b = static_cast<int>(hash_cell(lo[0] + kx, lo[1] + ky, lo[2] + kz, D.coarse_buckets));
+                    const int h = agg_slot(s_bucket, s_list, &s_n, b);
+                    if (h < 0) { coarse_merge(t + b, layer, src); continue; }
+                    atomicMin(&s_layer[h], layer);
+                    int cur = s_src[h];
+                    for (;;) {
+                        const int want = cur == -2 ? src : (cur == src ? cur : -1);
+                        if (want == cur) break;
+                        const int prev = atomicCAS(&s_src[h], cur, want);
+                        if (prev == cur) break;
+                        cur = prev;
+                    }
                 }
-            }
+    }
+    __syncthreads();
+    const int nb = s_n;
+    for (int k = static_cast<int>(threadIdx.x); k < nb; k += static_cast<int>(blockDim.x)) {
+        const int h = s_list[k];
+        coarse_merge(t + s_bucket[h], s_layer[h], s_src[h]);
+    }
 }
 
 // Scene gathers: a segment {source, begin, end} is a block's range.
