@@ -56,6 +56,8 @@ enum PassOps : unsigned {
     kOpCount = 1u << 7,       // count diagnostics
     kOpLastIter = 1u << 8,    // last iteration of the substep (pin polish always runs)
     kOpBend = 1u << 9,        // uncoloured bend gather (src -> dst), instead of stretch
+    kOpIntegrate = 1u << 10,  // the next substep's gravity-only integration (parity -> parity ^ 1)
+    kOpIntegratePin = 1u << 11,   // ... followed by its pin projection
 };
 
 struct DynView;

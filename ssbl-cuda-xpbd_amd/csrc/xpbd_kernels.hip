@@ -849,6 +849,26 @@ __device__ inline F3 vertex_normal(const ClothView& c, const float4* src, int v,
     return mul(acc, rsqrt_exact(asq));
 }
 
+// A41:1869-1925 without forces, for a vertex whose state s.p / velocity are
+// in registers: the previous position becomes s.p, a pinned vertex goes to
+// its rest position with zero velocity, a free one moves by its damped,
+// gravity-accelerated velocity.  Leaves s ready to be stored, all dirty.
+__device__ __forceinline__ void integrate_plain(const ClothView& c, VertexState& s, bool pinned, F3 vel, float dt) {
+    s.q_ = s.p;
+    s.q_have = s.q_dirty = true;
+    s.v_have = s.v_dirty = true;
+    if (pinned) {
+        s.p = f3(gload(c.rest + s.v));
+        s.vel_ = f3(0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const F3 acc = f3(c.gravity[0], c.gravity[1], c.gravity[2]);
+    F3 nv = vel + mul(acc, dt);
+    nv = mul(nv, c.damping);
+    s.vel_ = nv;
+    s.p = s.p + mul(nv, dt);
+}
+
 // kFull: normals, overpressure and force fields are compiled in (the plain
 // variant keeps the register footprint, and so the occupancy, of the common
 // gravity-only case).
@@ -870,9 +890,9 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
         s.p = f3(c.rest[v]);
     } else {
         const F3 vel = f3(c.vel[v]);
-        F3 acc = f3(c.gravity[0], c.gravity[1], c.gravity[2]);
-        F3 nv;
         if constexpr (kFull) {
+            F3 acc = f3(c.gravity[0], c.gravity[1], c.gravity[2]);
+            F3 nv;
             const F3 p = s.p;
             bool has = false;
             const F3 n = c.need_normal ? vertex_normal(c, src, v, &has) : f3(0.0f, 0.0f, 0.0f);
@@ -905,12 +925,11 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
                     if (isfinite(out) && out > max_out) nv = nv - mul(n, out - max_out);
                 }
             }
+            s.vel_ = nv;
+            s.p = s.p + mul(nv, dt);
         } else {
-            nv = vel + mul(acc, dt);
-            nv = mul(nv, c.damping);
+            integrate_plain(c, s, false, vel, dt);
         }
-        s.vel_ = nv;
-        s.p = s.p + mul(nv, dt);
     }
     if (ops & kOpPin) apply_pins(c, s, pinned, dt);
     dst[v] = with_w(s.p, p4.w);
@@ -952,6 +971,18 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
         if ((tail_ops & kOpPinPolish) && ((tail_ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt);
         if (pinned) s.set_vel(f3(0.0f, 0.0f, 0.0f));
         else velocity_update(c, s, dt);
+        if (tail_ops & kOpIntegrate) {
+            // the next substep's integration, as k_integrate<false> would read
+            // this state back: the substep's final position stays in this
+            // buffer, the integrated one goes to the other
+            gstore(pos + v, with_w(s.p, p4.w));
+            integrate_plain(c, s, pinned, s.vel_, dt);
+            if (tail_ops & kOpIntegratePin) apply_pins(c, s, pinned, dt);
+            gstore((parity ? c.pos0 : c.pos1) + v, with_w(s.p, p4.w));
+            gstore(c.prev + v, with_w(s.q_, 0.0f));
+            gstore(c.vel + v, with_w(s.vel_, 0.0f));
+            return;
+        }
     } else if (!active) {
         return;
     }

@@ -867,9 +867,13 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             if (got != parity) s->pcg.copy(st, dviews + m, s->d_tiles.p, got);
         }
     };
+    bool integrated = false;   // the previous substep's last launch integrated this one
     for (int sub = 0; sub < substeps; ++sub) {
-        launch_integrate(ctx, parity, plan.any_pins ? kOpPin : 0u, plan.any_forces);
-        parity ^= 1;
+        if (!integrated) {
+            launch_integrate(ctx, parity, plan.any_pins ? kOpPin : 0u, plan.any_forces);
+            parity ^= 1;
+        }
+        integrated = false;
         for (int it = 0; it < iterations; ++it) {
             const bool last = it == iterations - 1;
             const bool final_c = last && sub == substeps - 1;
@@ -927,9 +931,17 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             }
             const unsigned tail_ops = (pin_ops & kOpPinPolish) | last_op | kOpVelocity | cnt;
             // the dynamic contacts end the iteration: their particle launch
-            // applies the closing steps itself
+            // applies the closing steps itself, and (gravity only) the next
+            // substep's integration as well
             const bool tail_fused = dyn_now && !self_now && !signed_now && !(run_pcg && final_c);
-            if (dyn_now) launch_dynamic_pass(ctx, parity, cnt, tail_fused ? tail_ops : 0u);
+            const bool integrate_next = tail_fused && sub + 1 < substeps && !plan.any_forces;
+            const unsigned fused_ops =
+                tail_ops | (integrate_next ? (kOpIntegrate | (plan.any_pins ? kOpIntegratePin : 0u)) : 0u);
+            if (dyn_now) launch_dynamic_pass(ctx, parity, cnt, tail_fused ? fused_ops : 0u);
+            if (integrate_next) {
+                parity ^= 1;
+                integrated = true;
+            }
             if (self_now) {
                 if (!self_batch || !self_batch->run(ctx, parity, sub, substeps, count)) return false;
             }
