@@ -309,9 +309,9 @@ __device__ __forceinline__ float dyn_triangle_depth(const SSBL_CONST DynView& D,
     return pen;
 }
 
-// One dynamic particle against point p (A41:4354-4511 per candidate).
-__device__ __forceinline__ float dyn_particle_depth(const SSBL_CONST DynView& D, int j, F3 p, float margin, unsigned& cand, F3& delta) {
-    const float4 pj = gload(D.pp + j);
+// One dynamic particle (position and radius pj) against point p
+// (A41:4354-4511 per candidate).
+__device__ __forceinline__ float dyn_particle_depth_at(float4 pj, F3 p, float margin, unsigned& cand, F3& delta) {
     const float radius = fmaxf(pj.w, 0.0f) + margin;
     if (radius <= 0.0f) return 0.0f;
     const F3 d = p - f3(pj);
@@ -323,6 +323,10 @@ __device__ __forceinline__ float dyn_particle_depth(const SSBL_CONST DynView& D,
     const float pen = radius - dist;
     delta = mul(n, pen);
     return pen;
+}
+
+__device__ __forceinline__ float dyn_particle_depth(const SSBL_CONST DynView& D, int j, F3 p, float margin, unsigned& cand, F3& delta) {
+    return dyn_particle_depth_at(gload(D.pp + j), p, margin, cand, delta);
 }
 
 // False when no point of neighbour cell (cx+dx, cy+dy, cz+dz) of p's cell
@@ -543,16 +547,12 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                             const int qx = cx + k9 % 3 - 1, qy = cy + k9 / 3 - 1, qz = cz + dz;
                             const int base = bins[k9].x, lim = min(bins[k9].y, kDynParticleSlots);
                             for (int k = 0; k < lim; ++k) {
-                                const int4 pc = gload(D.pkcell + base + k);
-                                if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
-                                if (D.pksl) {
-                                    const int2 sl = gload(D.pksl + base + k);
-                                    if (!dyn_enabled(c, sl.x, sl.y)) continue;
-                                }
-                                const int j = gload(D.pkeys + base + k);
-                                if (j < 0 || j >= np_count) continue;
+                                const PartEntry e = gload(D.pent + base + k);
+                                if (e.ck.x != qx || e.ck.y != qy || e.ck.z != qz) continue;
+                                if (D.pksl && !dyn_enabled(c, e.sl.x, e.sl.y)) continue;
+                                if (e.ck.w < 0 || e.ck.w >= np_count) continue;
                                 F3 d;
-                                const float pen = dyn_particle_depth(D, j, s.p, margin, cand, d);
+                                const float pen = dyn_particle_depth_at(e.p, s.p, margin, cand, d);
                                 if (pen > best_pen) { best_pen = pen; best_delta = d; }
                             }
                         }
@@ -595,16 +595,12 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                                 unsigned long long key = 0ull;
                                 const int lim = min(bin.y, kDynParticleSlots);
                                 for (int k = 0; k < lim; ++k) {
-                                    const int4 pc = gload(D.pkcell + bin.x + k);
-                                    if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
-                                    if (D.pksl) {
-                                        const int2 sl = gload(D.pksl + bin.x + k);
-                                        if (!dyn_enabled(c, sl.x, sl.y)) continue;
-                                    }
-                                    const int j = gload(D.pkeys + bin.x + k);
-                                    if (j < 0 || j >= np_count) continue;
+                                    const PartEntry e = gload(D.pent + bin.x + k);
+                                    if (e.ck.x != qx || e.ck.y != qy || e.ck.z != qz) continue;
+                                    if (D.pksl && !dyn_enabled(c, e.sl.x, e.sl.y)) continue;
+                                    if (e.ck.w < 0 || e.ck.w >= np_count) continue;
                                     F3 d;
-                                    const unsigned long long kk = depth_key(dyn_particle_depth(D, j, op, margin, cand, d),
+                                    const unsigned long long kk = depth_key(dyn_particle_depth_at(e.p, op, margin, cand, d),
                                                                             static_cast<unsigned>(cell * kDynParticleSlots + k));
                                     key = kk > key ? kk : key;
                                 }
@@ -623,7 +619,7 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                     const int4 bin = gload(D.pbin + hash_cell(qx, qy, qz, D.part_buckets));
                     unsigned dc = 0;
                     F3 d;
-                    best_pen = dyn_particle_depth(D, gload(D.pkeys + bin.x + k), s.p, margin, dc, d);
+                    best_pen = dyn_particle_depth_at(gload(D.pent + bin.x + k).p, s.p, margin, dc, d);
                     best_delta = d;
                 }
             }
