@@ -420,20 +420,31 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                 best[lane] = 0ull;
                 wave_lds_sync();
                 const int first = bin.x - start;   // entry of item w is first + w in the owner's bucket
-                for (int base = 0; base < total; base += 64) {
-                    const int w = base + lane;
+                // owner lane of item w: the last lane whose start is at most w
+                auto owner_of = [&](int w) {
                     int own = 0;
                     for (int step = 32; step > 0; step >>= 1) {
                         const int probe = own + step;
                         const int sp = __shfl(start, min(probe, 63), 64);
                         if (probe < 64 && sp <= w) own = probe;
                     }
-                    const int ofirst = __shfl(first, own, 64);
+                    return own;
+                };
+                auto entry_of = [&](int w, int own) {
+                    const int k = __shfl(first, own, 64) + w;
+                    return w < total ? gload(D.tri_ent + k) : make_int4(-1, 0, 0, 0);
+                };
+                // each round requests the next round's entry records before it
+                // works through its own
+                int own = owner_of(lane);
+                int4 en = entry_of(lane, own);
+                for (int base = 0; base < total; base += 64) {
+                    const int w = base + lane;
                     const int ox = __shfl(cx, own, 64), oy = __shfl(cy, own, 64), oz = __shfl(cz, own, 64);
                     const F3 op = shfl3(s.p, own);
+                    const int own_next = owner_of(w + 64);
+                    const int4 en_next = entry_of(w + 64, own_next);
                     if (w < total) {
-                        const int k = ofirst + w;
-                        const int4 en = gload(D.tri_ent + k);
                         const int t = en.x / kDynMaxHashCells;
                         // the source filter and the canonical-cell test (A41:3934-3982)
                         // are both side-effect free, so the one in the entry runs first
@@ -448,6 +459,8 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                             }
                         }
                     }
+                    own = own_next;
+                    en = en_next;
                 }
                 wave_lds_sync();
                 const unsigned long long top = best[lane];
@@ -461,9 +474,9 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                 }
             }
             if (active) {
-                const int nl = *D.nlarge;
+                const int nl = gload(D.nlarge);
                 for (int k = 0; k < nl; ++k) {
-                    const int t = D.large[k];
+                    const int t = gload(D.large + k);
                     if (t < 0 || t >= nt) continue;
                     if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) continue;
                     F3 d;
