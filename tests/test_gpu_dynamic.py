@@ -54,6 +54,42 @@ def test_cloth_over_soup_and_particles_gpu(nb, hip, oracle, scenes):
     assert dg["fast_triangle_pair_contacts"] > 0 and dg["dynamic_particle_contacts"] > 0
 
 
+def run_mixed(nb, lib, scenes, *, batched, frames=12):
+    """Cloth A over its own dynamic triangles and particles beside cloth B,
+    which has no dynamic colliders, stepped as one batch or one by one."""
+    a = sd.cloth_over_colliders(nb, lib, scenes)
+    opts = scenes.SolverOptions(collision_margin=0.01, cloth_thickness=0.005)
+    b = nb.NativeXpbdSolver(scenes.grid_cloth(12, 0.6, 1.2, pin_corners=True, seed=7), opts, None, lib=lib)
+    batch = nb.SolverBatch([a, b]) if batched else None
+    out = []
+    for _ in range(frames):
+        if batch is not None:
+            batch.step(6, 4, synchronize=True)
+        else:
+            a.step(6, 4, synchronize=True)
+            b.step(6, 4, synchronize=True)
+        out.append((a.download_positions(), b.download_positions()))
+    if batch is not None:
+        batch.close()
+    a.close()
+    b.close()
+    return out
+
+
+def test_mixed_batch_with_and_without_colliders_gpu(nb, hip, oracle, scenes):
+    """In a batch where only some cloths have dynamic colliders, the contact
+    launches also close the substep (pin polish, velocity, next integration)
+    for the cloths without any: batched equals alone equals the oracle."""
+    batched = run_mixed(nb, hip, scenes, batched=True)
+    alone = run_mixed(nb, hip, scenes, batched=False)
+    ref = run_mixed(nb, oracle, scenes, batched=False)
+    for (ab, bb), (aa, ba), (ao, bo) in zip(batched, alone, ref):
+        assert np.array_equal(ab, aa) and np.array_equal(bb, ba)
+        assert np.array_equal(aa, ao) and np.array_equal(ba, bo)
+    # cloth B moved under gravity, which the fused integration applies
+    assert not np.array_equal(batched[0][1], batched[-1][1])
+
+
 def run_scene(nb, lib, scenes, *, gather, batched, frames=24):
     a, b, scene = sd.two_cloth_scene(nb, lib, scenes, nb.GlobalDynamicScene.CROSS_LOWER_LAYERS)
     batch = nb.SolverBatch([a, b]) if batched else None
