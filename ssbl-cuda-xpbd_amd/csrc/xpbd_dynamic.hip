@@ -272,6 +272,7 @@ __global__ __launch_bounds__(256) void k_tri_rank(const int* __restrict__ off, i
 // entry of a bucket skips it (A41:3984-3997 filters each entry the same way,
 // so results are equal).  Sets without sources get layer INT_MIN, source -1.
 __device__ __forceinline__ int2 entry_source_layer(int2 e) { return e; }
+__device__ __forceinline__ int2 entry_source_layer(const PartEntry& e) { return make_int2(e.sl.x, e.sl.y); }
 __device__ __forceinline__ int2 entry_source_layer(int4 e) { return make_int2(e.y, e.z); }
 template <typename R>
 __global__ __launch_bounds__(256) void k_bucket_bins(const int* __restrict__ off, int buckets, int cap, const R* __restrict__ rec,
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(256) void k_bucket_bins(const int* __restrict__ off
 // A41:4325-4352 with rank placement: phase 0 counts, 1 scatters, 2 ranks
 // (the first 16 particles of a bucket by index are visible to queries).
 __global__ __launch_bounds__(256) void k_part_hash(const DynView* __restrict__ dv, int4* pcell, int* pcount, const int* __restrict__ poff,
-                                                   int* pslot, int* praw, int* pkeys, int2* pksl, int4* pkcell, PartEntry* pent,
+                                                   int* pslot, int* praw, PartEntry* pent,
                                                    const int* __restrict__ psrc, const int* __restrict__ player,
                                                    unsigned long long* counts, int count_diag, int phase) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -331,9 +332,6 @@ __global__ __launch_bounds__(256) void k_part_hash(const DynView* __restrict__ d
         if (rank < kDynParticleSlots) {
             const int4 cell = pcell[i];
             const int2 sl = psrc ? make_int2(psrc[i], player[i]) : make_int2(0, 0);
-            pkeys[base + rank] = i;
-            pkcell[base + rank] = cell;
-            if (psrc) pksl[base + rank] = sl;
             PartEntry& e = pent[base + rank];
             e.ck = make_int4(cell.x, cell.y, cell.z, i);
             e.sl = make_int4(sl.x, sl.y, 0, 0);
@@ -565,9 +563,6 @@ bool DynamicColliders::reserve_particles(int n) {
     pcell_ = static_cast<int4*>(alloc(1, sizeof(int4) * m));
     pslot_ = static_cast<int*>(alloc(1, sizeof(int) * m));
     praw_ = static_cast<int*>(alloc(1, sizeof(int) * m));
-    pkeys_ = static_cast<int*>(alloc(1, sizeof(int) * m));
-    pksl_ = static_cast<int2*>(alloc(1, sizeof(int2) * m));
-    pkcell_ = static_cast<int4*>(alloc(1, sizeof(int4) * m));
     pent_ = static_cast<PartEntry*>(alloc(1, sizeof(PartEntry) * m));
     const int buckets = next_pow2(std::max(kDynMinBuckets, n * 4));
     // a particle marks about 3.4 coarse cells and a sheet puts about four
@@ -583,7 +578,7 @@ bool DynamicColliders::reserve_particles(int n) {
         scan_tmp_ = static_cast<int*>(alloc(2, sizeof(int) * scan_tmp_ints(buckets)));
         scan_cap_ = scan_tmp_ ? buckets : 0;
     }
-    if (!pp_ || !pprev_ || !psrc_ || !player_ || !pcell_ || !pslot_ || !praw_ || !pkeys_ || !pksl_ || !pbin_ || !pkcell_ || !pent_ || !pcount_ ||
+    if (!pp_ || !pprev_ || !psrc_ || !player_ || !pcell_ || !pslot_ || !praw_ || !pbin_ || !pent_ || !pcount_ ||
         !poff_ || !scan_tmp_ || !coarse_ || !coarse_off_)
         return false;
     part_cap_ = n;
@@ -613,10 +608,8 @@ bool DynamicColliders::upload_view(hipStream_t st) {
     v.pp = pp_;
     v.psrc = part_filter_ ? psrc_ : nullptr;
     v.player = part_filter_ ? player_ : nullptr;
-    v.pcount = pcount_; v.poff = poff_; v.pkeys = pkeys_; v.pcell = pcell_;
-    v.pksl = part_filter_ ? pksl_ : nullptr;
+    v.pcount = pcount_; v.poff = poff_; v.pcell = pcell_;
     v.pbin = pbin_;
-    v.pkcell = pkcell_;
     v.pent = pent_;
     if (v.part_hash) {
         v.coarse = coarse_;
@@ -911,11 +904,11 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
         for (int phase = 0; phase < 3; ++phase) {
             if (phase == 1) exclusive_scan(st, pcount_, buckets, poff_, scan_tmp_);
             launch_k(k_part_hash, g, blk, 0, st, static_cast<const DynView*>(d_view_), pcell_, pcount_, static_cast<const int*>(poff_),
-                     pslot_, praw_, pkeys_, pksl_, pkcell_, pent_, static_cast<const int*>(part_filter_ ? psrc_ : nullptr),
+                     pslot_, praw_, pent_, static_cast<const int*>(part_filter_ ? psrc_ : nullptr),
                      static_cast<const int*>(player_), counts, cd, phase);
         }
-        launch_k(k_bucket_bins<int2>, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
-                 static_cast<const int*>(poff_), buckets, kDynParticleSlots, static_cast<const int2*>(part_filter_ ? pksl_ : nullptr),
+        launch_k(k_bucket_bins<PartEntry>, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
+                 static_cast<const int*>(poff_), buckets, kDynParticleSlots, static_cast<const PartEntry*>(part_filter_ ? pent_ : nullptr),
                  pbin_);
         launch_k(k_coarse_fill, dim3(grid_for(coarse_buckets_, 256)), blk, 0, st, coarse_, coarse_buckets_, coarse_off_);
         launch_k(k_coarse_insert, g, blk, 0, st, static_cast<const DynView*>(d_view_), coarse_, coarse_off_);

@@ -58,9 +58,10 @@ __device__ __forceinline__ int4 box_cmax(const TriBox& b) { return make_int4(b.c
 __device__ __forceinline__ float4 box_lo(const TriBox& b) { return make_float4(__int_as_float(b.c1.z), __int_as_float(b.c1.w), b.b0.x, 0.0f); }
 __device__ __forceinline__ float4 box_hi(const TriBox& b) { return make_float4(b.b0.y, b.b0.z, b.b0.w, 0.0f); }
 
-// A visible particle bucket entry as the unsigned query reads it, in one
-// 48-byte record: its particle cell and index, {source, layer}, and the
-// particle's position and radius (fixed while the set's hash is current).
+// A visible particle bucket entry (the first 16 particles of a bucket by
+// index) in one 48-byte record: its particle cell and index, {source, layer},
+// and the particle's position and radius (fixed while the set's hash is
+// current), so a query reads an entry in one round trip.
 struct PartEntry {
     int4 ck;     // cell x, y, z, particle index
     int4 sl;     // source id, layer, 0, 0
@@ -98,11 +99,8 @@ struct DynView {
     const int* player;
     const int* pcount;       // buckets
     const int* poff;         // buckets + 1
-    const int* pkeys;        // first min(count, 16) particles of a bucket, ascending
-    const int2* pksl;        // {source id, layer} beside each key (filtered sets)
     const int4* pbin;        // per bucket, over its visible entries: {offset, count, min layer, one source | -1 | -2}
-    const int4* pkcell;      // the key's particle cell beside each key
-    const PartEntry* pent;   // the same entries packed with source, layer and position
+    const PartEntry* pent;   // first min(count, 16) particles of a bucket by index: cell, index, source, layer, position
     const int4* pcell;
     // Coarse particle summary: per bucket of coarse cells (4 particle cells
     // wide), {smallest layer, the one source | -1 mixed | -2 empty} over every
@@ -202,10 +200,8 @@ private:
     int *psrc_ = nullptr, *player_ = nullptr;
     bool part_filter_ = false;
     int4* pcell_ = nullptr;
-    int *pslot_ = nullptr, *pcount_ = nullptr, *poff_ = nullptr, *praw_ = nullptr, *pkeys_ = nullptr;
-    int2* pksl_ = nullptr;
+    int *pslot_ = nullptr, *pcount_ = nullptr, *poff_ = nullptr, *praw_ = nullptr;
     int4* pbin_ = nullptr;
-    int4* pkcell_ = nullptr;
     PartEntry* pent_ = nullptr;
     float4* pctr_ = nullptr;
     bool signed_ = false, tri_motion_ = false, part_motion_ = false, bounds_valid_ = false;

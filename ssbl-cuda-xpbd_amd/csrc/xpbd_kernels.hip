@@ -565,7 +565,7 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                             for (int k = 0; k < lim; ++k) {
                                 const PartEntry e = gload(D.pent + base + k);
                                 if (e.ck.x != qx || e.ck.y != qy || e.ck.z != qz) continue;
-                                if (D.pksl && !dyn_enabled(c, e.sl.x, e.sl.y)) continue;
+                                if (D.psrc && !dyn_enabled(c, e.sl.x, e.sl.y)) continue;
                                 if (e.ck.w < 0 || e.ck.w >= np_count) continue;
                                 F3 d;
                                 const float pen = dyn_particle_depth_at(e.p, s.p, margin, cand, d);
@@ -613,7 +613,7 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                                 for (int k = 0; k < lim; ++k) {
                                     const PartEntry e = gload(D.pent + bin.x + k);
                                     if (e.ck.x != qx || e.ck.y != qy || e.ck.z != qz) continue;
-                                    if (D.pksl && !dyn_enabled(c, e.sl.x, e.sl.y)) continue;
+                                    if (D.psrc && !dyn_enabled(c, e.sl.x, e.sl.y)) continue;
                                     if (e.ck.w < 0 || e.ck.w >= np_count) continue;
                                     F3 d;
                                     const unsigned long long kk = depth_key(dyn_particle_depth_at(e.p, op, margin, cand, d),
@@ -1214,12 +1214,12 @@ __device__ inline void signed_contacts(const ClothView& c, VertexState& s, bool 
         const int cx = cell_coord(p.x, D.part_cell), cy = cell_coord(p.y, D.part_cell), cz = cell_coord(p.z, D.part_cell);
         for (int k27 = 0; k27 < 27; ++k27) {   // z, then y, then x innermost
             const int qx = cx + k27 % 3 - 1, qy = cy + (k27 / 3) % 3 - 1, qz = cz + k27 / 9 - 1;
-            const int4 bin = D.pbin[hash_cell(qx, qy, qz, D.part_buckets)];
+            const int4 bin = gload(D.pbin + hash_cell(qx, qy, qz, D.part_buckets));
             const int lim = min(bin.y, kDynParticleSlots);
             for (int k = 0; k < lim; ++k) {
-                const int4 pc = D.pkcell[bin.x + k];
-                if (pc.x != qx || pc.y != qy || pc.z != qz) continue;
-                consider(D.pkeys[bin.x + k]);
+                const int4 ck = gload(&D.pent[bin.x + k].ck);
+                if (ck.x != qx || ck.y != qy || ck.z != qz) continue;
+                consider(ck.w);
             }
         }
     } else {
