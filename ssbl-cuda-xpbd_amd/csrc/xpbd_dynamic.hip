@@ -434,7 +434,9 @@ __global__ __launch_bounds__(256) void k_coarse_insert(const DynView* __restrict
 
 // Scene gathers: a segment {source, begin, end} is a block's range.
 __global__ __launch_bounds__(256) void k_gather_particles(const GatherSource* __restrict__ src, const int4* __restrict__ segs,
-                                                          float4* pp, float4* pprev, int* psrc, int* player, int keep_prev) {
+                                                          float4* pp, float4* pprev, int* psrc, int* player, int keep_prev,
+                                                          int* extent_bits) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *extent_bits = 0;   // the triangle gather's running maximum
     const int4 sg = segs[blockIdx.x];
     const int v = sg.y + static_cast<int>(threadIdx.x);
     if (v >= sg.z) return;
@@ -626,7 +628,13 @@ bool DynamicColliders::upload_view(hipStream_t st) {
     v.pprev = pprev_;
     if (v.tri_hash && v.tri_buckets > tri_buckets_) return fail("dynamic triangle hash capacity");
     if (v.part_hash && v.part_buckets > part_buckets_) return fail("dynamic particle hash capacity");
+    // the device copy already holds this view (a scene gathered again from
+    // the same cloths); its one device-written field, the scene's triangle
+    // cell, is recomputed after every gather anyway
+    if (uploaded_valid_ && std::memcmp(&uploaded_, &v, sizeof(DynView)) == 0) return true;
     launch_k(k_set_view, dim3(1), dim3(1), 0, st, d_view_, v);
+    uploaded_ = v;
+    uploaded_valid_ = true;
     return hip_ok(hipGetLastError(), "dynamic collider view");
 }
 
@@ -846,8 +854,11 @@ bool DynamicColliders::launch_gather(hipStream_t st, int npseg, int ntseg, bool 
     signed_ = tri_motion_ = false;   // a gathered scene is never signed (A41:6561)
     if (!upload_view(st)) return false;
     TimedLaunch t("scene_gather", st);
-    if (npseg > 0) launch_k(k_gather_particles, dim3(npseg), dim3(256), 0, st, d_sources_, d_segments_, pp_, pprev_, psrc_, player_, keep_prev ? 1 : 0);
-    launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, extent_bits_, 0, 1);
+    if (npseg > 0)
+        launch_k(k_gather_particles, dim3(npseg), dim3(256), 0, st, d_sources_, d_segments_, pp_, pprev_, psrc_, player_, keep_prev ? 1 : 0,
+                 extent_bits_);
+    else
+        launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, extent_bits_, 0, 1);
     if (ntseg > 0) launch_k(k_gather_triangles, dim3(ntseg), dim3(256), 0, st, d_sources_, d_segments_ + npseg, tv_, tsrc_, tlayer_, extent_bits_);
     launch_k(k_scene_cell, dim3(1), dim3(1), 0, st, d_view_, extent_bits_, margin, thickness);
     return hip_ok(hipGetLastError(), "scene gather");

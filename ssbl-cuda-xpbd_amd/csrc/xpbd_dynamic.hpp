@@ -217,6 +217,8 @@ private:
     int segments_cap_ = 0;
     DynView* d_view_ = nullptr;
     DynView view_{};
+    DynView uploaded_{};            // the last view copied to d_view_
+    bool uploaded_valid_ = false;
 };
 
 // Triangle hash cell of a soup whose largest triangle extent is max_extent.
