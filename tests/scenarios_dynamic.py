@@ -57,8 +57,8 @@ def valley_tie(lib, count, left_first, height=0.008, margin=0.01):
     distance bit for bit) with pushes of opposite x: the tie goes to the
     triangle first in the set, so every vertex moves to +x when the left face
     comes first (its push is along (1, 0, 1) / sqrt 2) and to -x otherwise.
-    Four vertices make the few probes a wave serves with all its lanes, 64
-    make every lane walk its own bucket."""
+    Four vertices give a wave one round of pooled (vertex, entry) items; 64
+    give it several rounds, each requesting the next round's entries early."""
     ys = np.linspace(-0.2, 0.2, count)
     pts = [[0.0, float(y), height] for y in ys]
     r = Raw(lib, pts, [1.0] * count, collision_margin=margin, cloth_thickness=0.005)

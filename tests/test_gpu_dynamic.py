@@ -24,7 +24,7 @@ def test_plane_contact_closed_form_gpu(hip, oracle, quads):
 
 
 @pytest.mark.parametrize("left_first", [True, False])
-@pytest.mark.parametrize("count", [4, 64])          # 4: wave-cooperative probes; 64: per-lane walks
+@pytest.mark.parametrize("count", [4, 64])          # 4: one round of pooled items; 64: several rounds
 def test_valley_tie_gpu(hip, oracle, count, left_first):
     """Unsigned buckets are stored unordered on the GPU: the depth tie between
     two mirrored faces must still go to the set's first triangle."""
