@@ -437,33 +437,47 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                     const int k = __shfl(first, own, 64) + w;
                     return w < total ? gload(D.tri_ent + k) : make_int4(-1, 0, 0, 0);
                 };
-                // each round requests the next round's entry records before it
-                // works through its own
+                // Items flow through three stages: a round's entry records are
+                // requested two rounds ahead and its triangles' bounds one round
+                // ahead, so a round's own work waits only for its corners.  The
+                // source filter (in the entry) runs before the bounds are read;
+                // it and the canonical-cell test (A41:3934-3982) are both
+                // side-effect free.
+                auto accepted = [&](int4 e) {
+                    return e.x >= 0 && e.x / kDynMaxHashCells < nt && (!D.tsrc || dyn_enabled(c, e.y, e.z));
+                };
+                auto bounds_of = [&](int4 e) {
+                    TriBox bx{};
+                    if (accepted(e)) bx = gload(D.box + e.x / kDynMaxHashCells);
+                    return bx;
+                };
                 int own = owner_of(lane);
                 int4 en = entry_of(lane, own);
+                int own1 = owner_of(lane + 64);
+                int4 en1 = entry_of(lane + 64, own1);
+                TriBox bx = bounds_of(en);
                 for (int base = 0; base < total; base += 64) {
                     const int w = base + lane;
                     const int ox = __shfl(cx, own, 64), oy = __shfl(cy, own, 64), oz = __shfl(cz, own, 64);
                     const F3 op = shfl3(s.p, own);
-                    const int own_next = owner_of(w + 64);
-                    const int4 en_next = entry_of(w + 64, own_next);
-                    if (w < total) {
+                    const int own2 = owner_of(w + 128);
+                    const int4 en2 = entry_of(w + 128, own2);
+                    const TriBox bx1 = bounds_of(en1);
+                    if (w < total && accepted(en)) {
                         const int t = en.x / kDynMaxHashCells;
-                        // the source filter and the canonical-cell test (A41:3934-3982)
-                        // are both side-effect free, so the one in the entry runs first
-                        if ((!D.tsrc || dyn_enabled(c, en.y, en.z)) && t >= 0 && t < nt) {
-                            const TriBox bx = gload(D.box + t);
-                            const int4 lo = box_cmin(bx), hi = box_cmax(bx);
-                            if (ox >= lo.x && ox <= hi.x && oy >= lo.y && oy <= hi.y && oz >= lo.z && oz <= hi.z) {
-                                F3 d;
-                                const unsigned long long kk =
-                                    depth_key(dyn_triangle_depth(D, t, bx, op, cm, cand, rej, d), static_cast<unsigned>(en.x));
-                                if (kk != 0ull) atomicMax(best + own, kk);
-                            }
+                        const int4 lo = box_cmin(bx), hi = box_cmax(bx);
+                        if (ox >= lo.x && ox <= hi.x && oy >= lo.y && oy <= hi.y && oz >= lo.z && oz <= hi.z) {
+                            F3 d;
+                            const unsigned long long kk =
+                                depth_key(dyn_triangle_depth(D, t, bx, op, cm, cand, rej, d), static_cast<unsigned>(en.x));
+                            if (kk != 0ull) atomicMax(best + own, kk);
                         }
                     }
-                    own = own_next;
-                    en = en_next;
+                    own = own1;
+                    en = en1;
+                    bx = bx1;
+                    own1 = own2;
+                    en1 = en2;
                 }
                 wave_lds_sync();
                 const unsigned long long top = best[lane];
