@@ -81,6 +81,19 @@ SSBL_API long long ssbl_ext_device_bytes(void* handle);
 /* Waits for all work the library queued. */
 SSBL_API int ssbl_ext_synchronize(void);
 
+/* Graph replay of the substep loop (on by default).  ssbl_step_solver_ex and
+ * ssbl_ext_step_batch capture the launches of a step into a HIP graph the
+ * first time and replay it on later steps with the same substeps,
+ * iterations and set of active passes; steps with self collision or the
+ * hard-stretch solve, and steps while kernel timing is on, launch directly.
+ * Results are the same either way.  mode 0 turns replay off, nonzero on;
+ * returns the previous mode.  No counterpart in the reference, which launches
+ * every kernel of a substep from the host (ssbl_xpbd_cuda_abi41.cu:9087-9398). */
+SSBL_API int ssbl_ext_set_graph_mode(int mode);
+/* Captures and replays so far of one handle's graph (batch NULL) or one
+ * batch's (handle NULL). */
+SSBL_API int ssbl_ext_graph_stats(void* handle, void* batch, long long* captures, long long* replays);
+
 /* Device-side kernel timing.  While enabled, every launch carries a start and
  * a stop event stamped by its own dispatch, on the stream it runs on.  ssbl_ext_kernel_stats
  * writes a JSON object {"kernel": {"launches": n, "total_ms": t}, ...} and

@@ -3361,6 +3361,14 @@ SSBL_API int ssbl_ext_set_static_sdf(void* handle, const int* dims, const float*
 SSBL_API long long ssbl_ext_device_bytes(void*) { fail("oracle: no device memory"); return -1; }
 SSBL_API int ssbl_ext_time_batch_pass(void*, int, float*) { return fail("oracle: no device timing") ? 1 : 0; }
 SSBL_API int ssbl_ext_synchronize(void) { return 1; }
+// The oracle launches nothing, so it has no graphs to replay.
+SSBL_API int ssbl_ext_set_graph_mode(int) { return 0; }
+SSBL_API int ssbl_ext_graph_stats(void*, void*, long long* captures, long long* replays) {
+    if (!captures || !replays) return fail("invalid graph stats request") ? 1 : 0;
+    *captures = 0;
+    *replays = 0;
+    return 1;
+}
 SSBL_API int ssbl_ext_kernel_timing(int) { return 1; }
 SSBL_API int ssbl_ext_kernel_stats(char* out, int cap) {
     if (out && cap > 2) std::snprintf(out, static_cast<size_t>(cap), "{}");

@@ -190,6 +190,64 @@ int next_pow2(int v) {
 }
 
 // ----------------------------------------------------------------- solver
+// ---------------------------------------------------------------- graphs
+std::atomic<int> g_graph_mode{1};
+
+// The launches of a step's substep loop, captured once into a HIP graph and
+// replayed while the launch sequence is the same: the key holds everything
+// the sequence depends on (substeps, iterations, which passes run), and the
+// launches themselves take only buffers the owner keeps for its lifetime, so
+// a replay issues exactly the launches a fresh capture would.  A capture the
+// runtime refuses turns the holder off; it then launches directly.
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<long long> key;
+    bool refused = false;
+    long long captures = 0, replays = 0;
+    StepGraph() = default;
+    StepGraph(const StepGraph&) = delete;
+    StepGraph& operator=(const StepGraph&) = delete;
+    ~StepGraph() { reset(); }
+    void reset() {
+        if (exec) hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+    template <class F>
+    bool run(hipStream_t st, const std::vector<long long>& k, F&& record) {
+        if (refused) return record();
+        if (exec && k == key) {
+            ++replays;
+            return hip_ok(hipGraphLaunch(exec, st), "step graph launch");
+        }
+        reset();
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            refused = true;
+            return record();
+        }
+        const std::string error_before = g_error;
+        const bool ok = record();
+        hipGraph_t g = nullptr;
+        const hipError_t end = hipStreamEndCapture(st, &g);
+        hipGraphExec_t e = nullptr;
+        const hipError_t inst = (ok && end == hipSuccess) ? hipGraphInstantiate(&e, g, nullptr, nullptr, 0) : end;
+        if (g) hipGraphDestroy(g);
+        if (!ok || inst != hipSuccess) {
+            // nothing captured ran: launch the same sequence directly
+            if (e) hipGraphExecDestroy(e);
+            (void)hipGetLastError();
+            g_error = error_before;
+            refused = true;
+            return record();
+        }
+        exec = e;
+        key = k;
+        ++captures;
+        return hip_ok(hipGraphLaunch(exec, st), "step graph launch");
+    }
+};
+
 struct HostPin { int index; float target[3]; float weight; };
 
 // A global dynamic scene (A41:309-340): the colliders every attached cloth
@@ -255,6 +313,7 @@ struct Solver {
     ClothView view{};
     SsblXpbdDiagnostics diag{};
     bool counts_pending = false;
+    StepGraph graph;
 
     bool create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh);
     bool build_sdf(const float* triangles, int count);
@@ -836,11 +895,9 @@ struct StepPlan {
     bool any_bend_gather = false, any_forces = false, any_dyn = false, any_signed = false;
 };
 
-static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const int4* bend_segs_dev,
-                         const std::vector<int>& seg_first, const std::vector<int>& seg_count, SelfBatch* self_batch, int substeps,
-                         int iterations, bool count) {
+static StepPlan plan_of(const std::vector<Solver*>& members) {
     StepPlan plan;
-    for (Solver* s : members) {
+    for (const Solver* s : members) {
         plan.any_pins |= !s->pins.empty();
         plan.any_edges |= s->cfg.edge_count > 0 && !s->cfg.stretch_optimization_enabled;
         plan.any_lra |= s->cfg.lra_count > 0;
@@ -852,6 +909,23 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
         plan.any_dyn |= s->view.dyn != nullptr;
         plan.any_signed |= s->view.sdyn != nullptr;
     }
+    return plan;
+}
+
+// The graph key of a step: the launch sequence of run_schedule is a function
+// of these values alone, for plans without self collision or the hard-stretch
+// solve (whose launches depend on more host state and are never captured).
+static bool graph_key(const StepPlan& plan, int substeps, int iterations, bool count, std::vector<long long>* key) {
+    if (g_graph_mode.load(std::memory_order_relaxed) == 0 || timing_enabled() || plan.any_self || plan.any_pcg) return false;
+    *key = {substeps, iterations, count ? 1 : 0, plan.any_pins, plan.any_edges, plan.any_lra, plan.any_bend,
+            plan.any_bend_gather, plan.any_forces, plan.any_dyn, plan.any_signed};
+    return true;
+}
+
+static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const int4* bend_segs_dev,
+                         const std::vector<int>& seg_first, const std::vector<int>& seg_count, SelfBatch* self_batch, int substeps,
+                         int iterations, bool count) {
+    const StepPlan plan = plan_of(members);
     const unsigned cnt = count ? kOpCount : 0u;
     const unsigned pin_ops = plan.any_pins ? (kOpPin | kOpPinPolish) : 0u;
     const unsigned stretch = plan.any_edges ? kOpStretch : 0u;
@@ -1072,9 +1146,13 @@ static bool step_one(Solver& s, int substeps, int iterations, int fetch, int for
         LaunchCtx ctx{lib_stream(), s.d_view.p, s.d_tiles.p, s.ntiles};
         std::vector<Solver*> one{&s};
         SelfBatch* sb = s.cfg.self_collision ? s.self.single_batch(s.view, s.d_view.p) : nullptr;
-        if (!run_schedule(one, ctx, s.d_view.p, s.bend_segs.p, s.bend_seg_first, s.bend_seg_count, sb, substeps, iterations,
-                          count))
-            return false;
+        auto record = [&] {
+            return run_schedule(one, ctx, s.d_view.p, s.bend_segs.p, s.bend_seg_first, s.bend_seg_count, sb, substeps, iterations,
+                                count);
+        };
+        std::vector<long long> key;
+        const bool graphed = graph_key(plan_of(one), substeps, iterations, count, &key);
+        if (!(graphed ? s.graph.run(lib_stream(), key, record) : record())) return false;
     }
     if (!s.dyn.clear_motion(lib_stream())) return false;
     if (force_sync || fetch) {
@@ -1102,6 +1180,7 @@ struct Batch {
     Staging staging;
     int ntiles = 0;
     SelfBatchHolder self;
+    StepGraph graph;
 };
 
 static bool build_batch(Batch& b) {
@@ -1147,8 +1226,12 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
     LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles};
     SelfBatch* sb = b.self.get(b.members, b.h_views, b.d_views.p);
-    if (!run_schedule(b.members, ctx, b.d_views.p, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false))
-        return false;
+    auto record = [&] {
+        return run_schedule(b.members, ctx, b.d_views.p, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false);
+    };
+    std::vector<long long> key;
+    const bool graphed = graph_key(plan_of(b.members), substeps, iterations, false, &key);
+    if (!(graphed ? b.graph.run(lib_stream(), key, record) : record())) return false;
     for (Solver* s : b.members)
         if (!s->dyn.clear_motion(lib_stream())) return false;
     return force_sync ? sync_stream() : true;
@@ -1564,6 +1647,18 @@ SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms) 
     return time_batch_pass(*static_cast<Batch*>(batch), launches, avg_ms) ? 1 : 0;
 }
 SSBL_API int ssbl_ext_synchronize(void) { g_error.clear(); return sync_stream() ? 1 : 0; }
+SSBL_API int ssbl_ext_set_graph_mode(int mode) {
+    g_error.clear();
+    return g_graph_mode.exchange(mode != 0 ? 1 : 0);
+}
+SSBL_API int ssbl_ext_graph_stats(void* handle, void* batch, long long* captures, long long* replays) {
+    g_error.clear();
+    if ((handle == nullptr) == (batch == nullptr) || !captures || !replays) return fail("invalid graph stats request") ? 1 : 0;
+    const StepGraph& g = handle ? static_cast<Solver*>(handle)->graph : static_cast<Batch*>(batch)->graph;
+    *captures = g.captures;
+    *replays = g.replays;
+    return 1;
+}
 SSBL_API int ssbl_ext_kernel_timing(int enable) {
     g_error.clear();
     if (!enable) {

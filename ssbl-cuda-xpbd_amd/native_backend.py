@@ -469,6 +469,10 @@ class SolverBatch:
         if not self._lib.ssbl_ext_step_batch(self._handle, int(substeps), int(iterations), int(bool(synchronize))):
             raise NativeSolverError(last_error(self._lib))
 
+    def graph_stats(self) -> tuple[int, int]:
+        """(captures, replays) of the batch's step graph."""
+        return graph_stats(self._lib, batch=self._handle)
+
     def time_pass(self, launches: int = 64) -> float:
         """Mean ms of `launches` back-to-back fused constraint passes (advances the state)."""
         out = np.zeros(1, np.float32)
@@ -492,3 +496,16 @@ def synchronize(lib=None) -> None:
     lib = lib or load_library()
     if not lib.ssbl_ext_synchronize():
         raise NativeSolverError(last_error(lib))
+
+
+def set_graph_mode(lib, enabled: bool) -> bool:
+    """Turns graph replay of the substep loop on or off; returns the previous mode."""
+    return bool(lib.ssbl_ext_set_graph_mode(int(bool(enabled))))
+
+
+def graph_stats(lib, handle=None, batch=None) -> tuple[int, int]:
+    """(captures, replays) of one handle's step graph or one batch's."""
+    captures, replays = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    if not lib.ssbl_ext_graph_stats(handle, batch, ctypes.byref(captures), ctypes.byref(replays)):
+        raise NativeSolverError(last_error(lib))
+    return int(captures.value), int(replays.value)

@@ -1,0 +1,105 @@
+"""Graph replay of the substep loop: a step replayed from its captured HIP
+graph must equal the same step launched directly, bit for bit, on every
+replay, with the program's own eager work (parameter uploads, scene gathers
+and hashes, downloads, synchronisations) between the replays."""
+import numpy as np
+import pytest
+
+import scenarios_dynamic as sd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def graph_mode(nb, hip):
+    prev = nb.set_graph_mode(hip, True)
+    yield lambda on: nb.set_graph_mode(hip, on)
+    nb.set_graph_mode(hip, prev)
+
+
+def expected_captures(keys):
+    """One graph is kept per handle or batch: a step whose key differs from
+    the previous step's captures again."""
+    return sum(1 for i, k in enumerate(keys) if i == 0 or k != keys[i - 1])
+
+
+def test_solo_replays_equal_direct_launches(nb, hip, scenes, graph_mode):
+    sc = scenes.config1(frames=12)
+    direct = nb.NativeXpbdSolver(sc.cloths[0], sc.options[0], sc.static_triangles, lib=hip)
+    graphed = nb.NativeXpbdSolver(sc.cloths[0], sc.options[0], sc.static_triangles, lib=hip)
+    # a change of iterations and the diagnostics steps change the launch
+    # sequence, so they are captured anew
+    schedule = [(sc.substeps, sc.iterations, f == 3) for f in range(5)] + [(4, 3, False)] * 3
+    schedule += [(sc.substeps, sc.iterations, f == 2) for f in range(4)]
+    for f, (sub, it, diag) in enumerate(schedule):
+        graph_mode(False)
+        direct.step(sub, it, diagnostics=diag, synchronize=True)
+        graph_mode(True)
+        graphed.step(sub, it, diagnostics=diag, synchronize=True)
+        assert np.array_equal(direct.download_positions(), graphed.download_positions()), f
+        if diag:
+            dd, dg = direct.diagnostics(), graphed.diagnostics()
+            for name in ("abi41_soft_contact_count", "abi41_exact_impulse_contact_count", "finite_flag"):
+                assert getattr(dd, name) == getattr(dg, name), (f, name)
+    assert nb.graph_stats(hip, handle=direct.handle) == (0, 0)
+    captures = expected_captures(schedule)
+    assert nb.graph_stats(hip, handle=graphed.handle) == (captures, len(schedule) - captures)
+
+
+def run_two_cloth_batch(nb, hip, scenes, graphed, frames):
+    a, b, scene = sd.two_cloth_scene(nb, hip, scenes, nb.GlobalDynamicScene.CROSS_LOWER_LAYERS)
+    batch = nb.SolverBatch([a, b])
+    nb.set_graph_mode(hip, graphed)
+    out = []
+    for _ in range(frames):
+        sd.refresh_scene(scene, [a, b], device_gather=True)
+        batch.step(6, 4, synchronize=True)
+        out.append((a.download_positions(), b.download_positions()))
+    stats = batch.graph_stats()
+    batch.close()
+    return out, stats
+
+
+def test_batch_with_global_scene_replays_equal_direct(nb, hip, scenes, graph_mode):
+    """The batched frame of the cross-cloth workload: scene gather and hashes
+    run eagerly before each replay of the substep graph (dynamic contact
+    passes with the fused closing steps and next integration)."""
+    frames = 16
+    direct, sd_ = run_two_cloth_batch(nb, hip, scenes, False, frames)
+    graphed, sg = run_two_cloth_batch(nb, hip, scenes, True, frames)
+    assert sd_ == (0, 0)
+    assert sg == (1, frames - 1)
+    for f, ((ad, bd), (ag, bg)) in enumerate(zip(direct, graphed)):
+        assert np.array_equal(ad, ag) and np.array_equal(bd, bg), f
+    # the cloths met: B rests on A, so the contact passes did work
+    assert not np.array_equal(graphed[0][1], graphed[-1][1])
+
+
+def test_batch_of_mixed_cloths_replays_equal_direct(nb, hip, scenes, graph_mode):
+    sc = scenes.config3(seed=2, count=6, frames=6)
+    sc.static_triangles = np.empty((0, 9), np.float32)
+    runs = []
+    for graphed in (False, True):
+        graph_mode(graphed)
+        members = [nb.NativeXpbdSolver(c, o, sc.static_triangles, lib=hip) for c, o in zip(sc.cloths, sc.options)]
+        batch = nb.SolverBatch(members)
+        frames = []
+        for _ in range(sc.frames):
+            batch.step(sc.substeps, sc.iterations, synchronize=True)
+            frames.append([m.download_positions() for m in members])
+        runs.append((frames, batch.graph_stats()))
+        batch.close()
+    (direct, sd_), (graphed, sg) = runs
+    assert sd_ == (0, 0) and sg == (1, sc.frames - 1)
+    for f, (xd, xg) in enumerate(zip(direct, graphed)):
+        for k, (pd, pg) in enumerate(zip(xd, xg)):
+            assert np.array_equal(pd, pg), (f, k)
+
+
+def test_self_collision_steps_launch_directly(nb, hip, scenes, graph_mode):
+    cloth = scenes.grid_cloth(12, 0.4, 0.3, seed=5)
+    opts = scenes.SolverOptions(self_collision=True, cloth_thickness=0.005)
+    s = nb.NativeXpbdSolver(cloth, opts, None, lib=hip)
+    for _ in range(3):
+        s.step(4, 4, diagnostics=False, synchronize=True)
+    assert nb.graph_stats(hip, handle=s.handle) == (0, 0)
