@@ -309,6 +309,8 @@ def main():
             "frame_ms_p50_plain": float(np.percentile(lat_p, 50)),
             "frame_ms_p99_plain": float(np.percentile(lat_p, 99)),
             "kernels": stats,
+            "tuned_pool_limit": batch.tuned_pool_limit() if batch is not None else None,
+            "step_graph_captures_replays": list(batch.graph_stats()) if batch is not None else None,
             "selfcol_10k": secondary,
             "serving": serving,
         },

@@ -94,6 +94,20 @@ SSBL_API int ssbl_ext_set_graph_mode(int mode);
  * batch's (handle NULL). */
 SSBL_API int ssbl_ext_graph_stats(void* handle, void* batch, long long* captures, long long* replays);
 
+/* Launch tuning (off by default).  A handle or batch whose steps run the
+ * dynamic collider contacts times the first of those steps and every 16th
+ * after it, with the particle pass's pooling limit (the most probing
+ * vertices of a wave whose searches are pooled into one list) varied over
+ * {0, 8, 16, 32, 64} from substep to substep; a later step reads the timings
+ * once they are complete (no wait) and keeps the fastest limit until the
+ * next timed step.  Every limit gives the same contacts.  mode 0 keeps the
+ * default limit 16 (or the last choice); returns the previous mode.  No
+ * counterpart in the reference, whose launch shapes are fixed. */
+SSBL_API int ssbl_ext_set_autotune(int mode);
+/* The limit chosen for one handle (batch NULL) or one batch (handle NULL):
+ * -1 while not yet tuned, -2 on invalid arguments. */
+SSBL_API int ssbl_ext_tuned_pool_limit(void* handle, void* batch);
+
 /* Device-side kernel timing.  While enabled, every launch carries a start and
  * a stop event stamped by its own dispatch, on the stream it runs on.  ssbl_ext_kernel_stats
  * writes a JSON object {"kernel": {"launches": n, "total_ms": t}, ...} and

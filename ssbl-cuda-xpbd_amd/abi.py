@@ -132,6 +132,8 @@ EXT_SIGNATURES = {
     "ssbl_ext_update_global_scene_from_handles": (_i, [_vp, ctypes.POINTER(_vp), _ip, _ip, _fp, _i, ctypes.c_float, ctypes.c_float]),
     "ssbl_ext_synchronize": (_i, []),
     "ssbl_ext_set_graph_mode": (_i, [_i]),
+    "ssbl_ext_set_autotune": (_i, [_i]),
+    "ssbl_ext_tuned_pool_limit": (_i, [_vp, _vp]),
     "ssbl_ext_graph_stats": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "ssbl_ext_kernel_timing": (_i, [_i]),
     "ssbl_ext_kernel_stats": (_i, [ctypes.c_char_p, _i]),

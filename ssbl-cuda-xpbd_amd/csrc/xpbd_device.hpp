@@ -58,6 +58,9 @@ enum PassOps : unsigned {
     kOpBend = 1u << 9,        // uncoloured bend gather (src -> dst), instead of stretch
     kOpIntegrate = 1u << 10,  // the next substep's gravity-only integration (parity -> parity ^ 1)
     kOpIntegratePin = 1u << 11,   // ... followed by its pin projection
+    // bits 16-22 of a particle contact launch's ops: the most probing
+    // vertices a wave pools into one item list (more search per lane)
+    kOpPoolShift = 16,
 };
 
 struct DynView;
@@ -163,7 +166,11 @@ void launch_integrate(const LaunchCtx& c, int parity, unsigned ops, bool full);
 void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops);
 // tail_ops: the closing pin polish and velocity update, fused into the
 // particle launch when nothing else runs between (0: none)
-void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops);
+// The particle pass pools the searches of a wave with at most `pool_max`
+// probing vertices (0..64); the events, when given, bracket that launch.
+constexpr int kDefaultPoolMax = 16;
+void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops, int pool_max = kDefaultPoolMax,
+                         hipEvent_t begin = nullptr, hipEvent_t end = nullptr);
 // Signed contacts: the pass after self collision (inside = false) and the
 // inside-only passes of a step's last iteration (A41:9322-9359).
 void launch_signed_pass(const LaunchCtx& c, int parity, bool inside, unsigned ops);

@@ -381,11 +381,11 @@ __device__ __forceinline__ F3 shfl3(F3 a, int lane) { return f3(__shfl(a.x, lane
 // the key (triangle * 64 + cell index), the order of a walk of the bucket in
 // ascending key order.  Every lane of the wave must call these (`active`
 // false for lanes without a vertex).
-// Above this many probing vertices a wave's particle searches run one per
+// Above `pool_max` probing vertices a wave's particle searches run one per
 // lane (27 cells each, a z plane of bins loaded together): the pooled list
 // pays its owner lookups on every item, and with most lanes probing the
-// per-lane walks are already balanced.
-constexpr int kPooledParticleProbes = 16;
+// per-lane walks are already balanced.  The launch takes the limit
+// (kDefaultPoolMax unless tuned); every value gives the same contacts.
 
 __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState& s, bool active, float dt, bool count) {
     const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
@@ -531,7 +531,7 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
     }
 }
 
-__device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState& s, bool active, bool count) {
+__device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState& s, bool active, bool count, int pool_max) {
     const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
     const int lane = static_cast<int>(__lane_id());
     const int np_count = D.np;
@@ -558,7 +558,7 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
             // their bucket order k, so (depth, ~(16 c + k)) orders contacts as
             // the serial search meets them.
             const unsigned long long probes = __ballot(probe);
-            if (__popcll(probes) > kPooledParticleProbes) {
+            if (__popcll(probes) > pool_max) {
                 // many probes in this wave: each lane searches its own 27 cells
                 if (probe) {
                     const int cx = cell_coord(s.p.x, D.part_cell), cy = cell_coord(s.p.y, D.part_cell), cz = cell_coord(s.p.z, D.part_cell);
@@ -589,7 +589,7 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
                     }
                 }
             }
-            const int n_i = probe && __popcll(probes) <= kPooledParticleProbes ? 27 : 0;
+            const int n_i = probe && __popcll(probes) <= pool_max ? 27 : 0;
             int incl = n_i;
             for (int d = 1; d < 64; d <<= 1) {
                 const int o = __shfl_up(incl, d, 64);
@@ -988,7 +988,7 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
     if (has_dyn) {
         if (kKind == 0) dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
-        else dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u);
+        else dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u, static_cast<int>((ops >> kOpPoolShift) & 0x7fu));
     }
     if (!in_tile) return;
     if (tail) {
@@ -1656,10 +1656,14 @@ void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsi
         launch_k(k_vertex_pass<kGatherNone>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
 }
 
-void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops) {
+void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops, int pool_max, hipEvent_t begin,
+                         hipEvent_t end) {
     TimedLaunch t("dynamic_contacts", c.stream);
+    const unsigned pops = ops | (static_cast<unsigned>(std::min(std::max(pool_max, 0), 64)) << kOpPoolShift);
     launch_k(k_dynamic_pass<0>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops, 0u);
-    launch_k(k_dynamic_pass<1>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops, tail_ops);
+    if (begin) hipEventRecord(begin, c.stream);
+    launch_k(k_dynamic_pass<1>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, pops, tail_ops);
+    if (end) hipEventRecord(end, c.stream);
 }
 
 void launch_signed_pass(const LaunchCtx& c, int parity, bool inside, unsigned ops) {

@@ -194,33 +194,45 @@ int next_pow2(int v) {
 std::atomic<int> g_graph_mode{1};
 
 // The launches of a step's substep loop, captured once into a HIP graph and
-// replayed while the launch sequence is the same: the key holds everything
-// the sequence depends on (substeps, iterations, which passes run), and the
-// launches themselves take only buffers the owner keeps for its lifetime, so
-// a replay issues exactly the launches a fresh capture would.  A capture the
+// replayed whenever the launch sequence is one already seen: the key holds
+// everything the sequence depends on (substeps, iterations, which passes run,
+// the tuned launch limit), and the launches themselves take only buffers the
+// owner keeps for its lifetime, so a replay issues exactly the launches a
+// fresh capture would.  A capture the
 // runtime refuses turns the holder off; it then launches directly.
 struct StepGraph {
-    hipGraphExec_t exec = nullptr;
-    std::vector<long long> key;
+    static constexpr size_t kSlots = 4;   // launch sequences kept (least recently used goes)
+    struct Slot { std::vector<long long> key; hipGraphExec_t exec = nullptr; long long used = 0; };
+    std::vector<Slot> slots;
     bool refused = false;
-    long long captures = 0, replays = 0;
+    long long captures = 0, replays = 0, clock = 0;
     StepGraph() = default;
     StepGraph(const StepGraph&) = delete;
     StepGraph& operator=(const StepGraph&) = delete;
-    ~StepGraph() { reset(); }
-    void reset() {
-        if (exec) hipGraphExecDestroy(exec);
-        exec = nullptr;
-        key.clear();
+    // the owner waits for the stream before it goes
+    ~StepGraph() {
+        for (Slot& sl : slots)
+            if (sl.exec) hipGraphExecDestroy(sl.exec);
     }
     template <class F>
     bool run(hipStream_t st, const std::vector<long long>& k, F&& record) {
         if (refused) return record();
-        if (exec && k == key) {
+        ++clock;
+        for (Slot& sl : slots) {
+            if (sl.key != k) continue;
+            sl.used = clock;
             ++replays;
-            return hip_ok(hipGraphLaunch(exec, st), "step graph launch");
+            return hip_ok(hipGraphLaunch(sl.exec, st), "step graph launch");
         }
-        reset();
+        if (slots.size() == kSlots) {
+            // an evicted graph may still run: wait for the stream first
+            size_t old = 0;
+            for (size_t i = 1; i < slots.size(); ++i)
+                if (slots[i].used < slots[old].used) old = i;
+            if (!hip_ok(hipStreamSynchronize(st), "step graph eviction")) return false;
+            hipGraphExecDestroy(slots[old].exec);
+            slots.erase(slots.begin() + static_cast<long>(old));
+        }
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             (void)hipGetLastError();
             refused = true;
@@ -241,12 +253,106 @@ struct StepGraph {
             refused = true;
             return record();
         }
-        exec = e;
-        key = k;
+        slots.push_back(Slot{k, e, clock});
         ++captures;
-        return hip_ok(hipGraphLaunch(exec, st), "step graph launch");
+        return hip_ok(hipGraphLaunch(e, st), "step graph launch");
     }
 };
+
+// ---------------------------------------------------------------- tuning
+std::atomic<int> g_autotune{0};
+
+// Picks the particle contact pass's pooling limit (kOpPoolShift) for one
+// handle or batch by timing the candidates on its own launches.  Every
+// kEvery-th step with dynamic contacts (the first included) is a timed step:
+// substep k runs candidate (k + round) mod kCount, its particle launch
+// bracketed by events; a later step reads the events once they have
+// completed (no wait) and keeps the candidate with the least mean time, unless
+// the one in use is within 3 % of it.  Contacts change over a run (cloths meet
+// and pile up), and with them the best limit, hence the repeats.  Every
+// candidate gives the same contacts, so tuning changes timing only.  Timed
+// steps are never captured into a graph.
+struct PoolTuner {
+    static constexpr int kCount = 5;
+    static constexpr int kCandidates[kCount] = {0, 8, 16, 32, 64};
+    static constexpr int kEvery = 16;
+    int chosen = -1;
+    long long step = 0;    // steps with dynamic contacts so far
+    int round = 0;         // timed steps so far
+    bool pending = false;  // a timed step whose events are not read yet
+    double total_ms[kCount] = {};
+    int samples[kCount] = {};
+    std::vector<hipEvent_t> events;   // begin/end per substep of the timed step
+    std::vector<int> which;           // candidate index per substep, -1 when not launched
+    PoolTuner() = default;
+    PoolTuner(const PoolTuner&) = delete;
+    PoolTuner& operator=(const PoolTuner&) = delete;
+    ~PoolTuner() {
+        for (hipEvent_t e : events)
+            if (e) hipEventDestroy(e);
+    }
+    bool tuning() const { return g_autotune.load(std::memory_order_relaxed) != 0 && step % kEvery == 0; }
+    int value() const { return chosen >= 0 ? chosen : kDefaultPoolMax; }
+    bool begin_step(int substeps) {
+        for (int k = 0; k < kCount; ++k) {
+            total_ms[k] = 0.0;
+            samples[k] = 0;
+        }
+        while (events.size() < 2 * static_cast<size_t>(substeps)) {
+            hipEvent_t e = nullptr;
+            if (!hip_ok(hipEventCreate(&e), "tuning event")) return false;
+            events.push_back(e);
+        }
+        which.assign(static_cast<size_t>(substeps), -1);
+        return true;
+    }
+    // The limit substep `sub` of the timed step runs, and its events.
+    int candidate(int sub, hipEvent_t* begin, hipEvent_t* end) {
+        const int k = (sub + round) % kCount;
+        which[static_cast<size_t>(sub)] = k;
+        *begin = events[2 * static_cast<size_t>(sub)];
+        *end = events[2 * static_cast<size_t>(sub) + 1];
+        return kCandidates[k];
+    }
+    void end_step() { pending = true; }
+    // Reads the last timed step's events when they have completed, or waits
+    // for them when `wait`, and updates the choice.
+    bool poll(bool wait) {
+        if (!pending) return true;
+        int last = -1;
+        for (size_t sub = 0; sub < which.size(); ++sub)
+            if (which[sub] >= 0) last = static_cast<int>(sub);
+        if (last >= 0 && !wait) {
+            const hipError_t q = hipEventQuery(events[2 * static_cast<size_t>(last) + 1]);
+            if (q == hipErrorNotReady) {
+                (void)hipGetLastError();
+                return true;
+            }
+            if (!hip_ok(q, "tuning event")) return false;
+        }
+        pending = false;
+        for (size_t sub = 0; sub < which.size(); ++sub) {
+            const int k = which[sub];
+            if (k < 0) continue;
+            float ms = 0.0f;
+            if (!hip_ok(hipEventSynchronize(events[2 * sub + 1]), "tuning event") ||
+                !hip_ok(hipEventElapsedTime(&ms, events[2 * sub], events[2 * sub + 1]), "tuning event"))
+                return false;
+            total_ms[k] += ms;
+            samples[k] += 1;
+        }
+        ++round;
+        int best = -1, cur = -1;
+        for (int k = 0; k < kCount; ++k) {
+            if (samples[k] == 0) continue;
+            if (kCandidates[k] == chosen) cur = k;
+            if (best < 0 || total_ms[k] / samples[k] < total_ms[best] / samples[best]) best = k;
+        }
+        if (best >= 0 && (cur < 0 || total_ms[best] / samples[best] < 0.97 * (total_ms[cur] / samples[cur]))) chosen = kCandidates[best];
+        return true;
+    }
+};
+constexpr int PoolTuner::kCandidates[PoolTuner::kCount];
 
 struct HostPin { int index; float target[3]; float weight; };
 
@@ -314,6 +420,7 @@ struct Solver {
     SsblXpbdDiagnostics diag{};
     bool counts_pending = false;
     StepGraph graph;
+    PoolTuner tuner;
 
     bool create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh);
     bool build_sdf(const float* triangles, int count);
@@ -915,16 +1022,18 @@ static StepPlan plan_of(const std::vector<Solver*>& members) {
 // The graph key of a step: the launch sequence of run_schedule is a function
 // of these values alone, for plans without self collision or the hard-stretch
 // solve (whose launches depend on more host state and are never captured).
-static bool graph_key(const StepPlan& plan, int substeps, int iterations, bool count, std::vector<long long>* key) {
+static bool graph_key(const StepPlan& plan, int substeps, int iterations, bool count, const PoolTuner& tuner,
+                      std::vector<long long>* key) {
     if (g_graph_mode.load(std::memory_order_relaxed) == 0 || timing_enabled() || plan.any_self || plan.any_pcg) return false;
+    if (plan.any_dyn && tuner.tuning()) return false;
     *key = {substeps, iterations, count ? 1 : 0, plan.any_pins, plan.any_edges, plan.any_lra, plan.any_bend,
-            plan.any_bend_gather, plan.any_forces, plan.any_dyn, plan.any_signed};
+            plan.any_bend_gather, plan.any_forces, plan.any_dyn, plan.any_signed, tuner.value()};
     return true;
 }
 
 static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const int4* bend_segs_dev,
                          const std::vector<int>& seg_first, const std::vector<int>& seg_count, SelfBatch* self_batch, int substeps,
-                         int iterations, bool count) {
+                         int iterations, bool count, PoolTuner& tuner, bool tune) {
     const StepPlan plan = plan_of(members);
     const unsigned cnt = count ? kOpCount : 0u;
     const unsigned pin_ops = plan.any_pins ? (kOpPin | kOpPinPolish) : 0u;
@@ -1011,7 +1120,11 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             const bool integrate_next = tail_fused && sub + 1 < substeps && !plan.any_forces;
             const unsigned fused_ops =
                 tail_ops | (integrate_next ? (kOpIntegrate | (plan.any_pins ? kOpIntegratePin : 0u)) : 0u);
-            if (dyn_now) launch_dynamic_pass(ctx, parity, cnt, tail_fused ? fused_ops : 0u);
+            if (dyn_now) {
+                hipEvent_t tb = nullptr, te = nullptr;
+                const int pool = tune ? tuner.candidate(sub, &tb, &te) : tuner.value();
+                launch_dynamic_pass(ctx, parity, cnt, tail_fused ? fused_ops : 0u, pool, tb, te);
+            }
             if (integrate_next) {
                 parity ^= 1;
                 integrated = true;
@@ -1146,13 +1259,19 @@ static bool step_one(Solver& s, int substeps, int iterations, int fetch, int for
         LaunchCtx ctx{lib_stream(), s.d_view.p, s.d_tiles.p, s.ntiles};
         std::vector<Solver*> one{&s};
         SelfBatch* sb = s.cfg.self_collision ? s.self.single_batch(s.view, s.d_view.p) : nullptr;
+        const StepPlan plan = plan_of(one);
+        if (plan.any_dyn && !s.tuner.poll(s.tuner.tuning())) return false;
+        const bool tune = plan.any_dyn && s.tuner.tuning();
+        if (tune && !s.tuner.begin_step(substeps)) return false;
         auto record = [&] {
             return run_schedule(one, ctx, s.d_view.p, s.bend_segs.p, s.bend_seg_first, s.bend_seg_count, sb, substeps, iterations,
-                                count);
+                                count, s.tuner, tune);
         };
         std::vector<long long> key;
-        const bool graphed = graph_key(plan_of(one), substeps, iterations, count, &key);
+        const bool graphed = graph_key(plan, substeps, iterations, count, s.tuner, &key);
         if (!(graphed ? s.graph.run(lib_stream(), key, record) : record())) return false;
+        if (tune) s.tuner.end_step();
+        if (plan.any_dyn) ++s.tuner.step;
     }
     if (!s.dyn.clear_motion(lib_stream())) return false;
     if (force_sync || fetch) {
@@ -1181,6 +1300,7 @@ struct Batch {
     int ntiles = 0;
     SelfBatchHolder self;
     StepGraph graph;
+    PoolTuner tuner;
 };
 
 static bool build_batch(Batch& b) {
@@ -1226,12 +1346,19 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
     LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles};
     SelfBatch* sb = b.self.get(b.members, b.h_views, b.d_views.p);
+    const StepPlan plan = plan_of(b.members);
+    if (plan.any_dyn && !b.tuner.poll(b.tuner.tuning())) return false;
+    const bool tune = plan.any_dyn && b.tuner.tuning();
+    if (tune && !b.tuner.begin_step(substeps)) return false;
     auto record = [&] {
-        return run_schedule(b.members, ctx, b.d_views.p, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false);
+        return run_schedule(b.members, ctx, b.d_views.p, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false,
+                            b.tuner, tune);
     };
     std::vector<long long> key;
-    const bool graphed = graph_key(plan_of(b.members), substeps, iterations, false, &key);
+    const bool graphed = graph_key(plan, substeps, iterations, false, b.tuner, &key);
     if (!(graphed ? b.graph.run(lib_stream(), key, record) : record())) return false;
+    if (tune) b.tuner.end_step();
+    if (plan.any_dyn) ++b.tuner.step;
     for (Solver* s : b.members)
         if (!s->dyn.clear_motion(lib_stream())) return false;
     return force_sync ? sync_stream() : true;
@@ -1650,6 +1777,19 @@ SSBL_API int ssbl_ext_synchronize(void) { g_error.clear(); return sync_stream() 
 SSBL_API int ssbl_ext_set_graph_mode(int mode) {
     g_error.clear();
     return g_graph_mode.exchange(mode != 0 ? 1 : 0);
+}
+SSBL_API int ssbl_ext_set_autotune(int mode) {
+    g_error.clear();
+    return g_autotune.exchange(mode != 0 ? 1 : 0);
+}
+SSBL_API int ssbl_ext_tuned_pool_limit(void* handle, void* batch) {
+    g_error.clear();
+    if ((handle == nullptr) == (batch == nullptr)) {
+        fail("invalid tuning request");
+        return -2;
+    }
+    const PoolTuner& t = handle ? static_cast<Solver*>(handle)->tuner : static_cast<Batch*>(batch)->tuner;
+    return t.chosen;
 }
 SSBL_API int ssbl_ext_graph_stats(void* handle, void* batch, long long* captures, long long* replays) {
     g_error.clear();

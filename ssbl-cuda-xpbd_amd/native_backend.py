@@ -473,6 +473,10 @@ class SolverBatch:
         """(captures, replays) of the batch's step graph."""
         return graph_stats(self._lib, batch=self._handle)
 
+    def tuned_pool_limit(self) -> int:
+        """The particle pass's pooling limit the batch's tuning chose (-1: not yet)."""
+        return int(self._lib.ssbl_ext_tuned_pool_limit(None, self._handle))
+
     def time_pass(self, launches: int = 64) -> float:
         """Mean ms of `launches` back-to-back fused constraint passes (advances the state)."""
         out = np.zeros(1, np.float32)
@@ -509,3 +513,8 @@ def graph_stats(lib, handle=None, batch=None) -> tuple[int, int]:
     if not lib.ssbl_ext_graph_stats(handle, batch, ctypes.byref(captures), ctypes.byref(replays)):
         raise NativeSolverError(last_error(lib))
     return int(captures.value), int(replays.value)
+
+
+def set_autotune(lib, enabled: bool) -> bool:
+    """Turns launch tuning on or off; returns the previous mode."""
+    return bool(lib.ssbl_ext_set_autotune(int(bool(enabled))))

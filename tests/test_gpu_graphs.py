@@ -12,15 +12,19 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def graph_mode(nb, hip):
-    prev = nb.set_graph_mode(hip, True)
+    # launch tuning keeps its timed steps out of graphs; these tests count
+    # captures and replays without it
+    prev, prev_tune = nb.set_graph_mode(hip, True), nb.set_autotune(hip, False)
     yield lambda on: nb.set_graph_mode(hip, on)
     nb.set_graph_mode(hip, prev)
+    nb.set_autotune(hip, prev_tune)
 
 
 def expected_captures(keys):
-    """One graph is kept per handle or batch: a step whose key differs from
-    the previous step's captures again."""
-    return sum(1 for i, k in enumerate(keys) if i == 0 or k != keys[i - 1])
+    """A handle or batch keeps the graphs of up to four launch sequences: a
+    step captures when its key is new."""
+    assert len(set(keys)) <= 4
+    return len(set(keys))
 
 
 def test_solo_replays_equal_direct_launches(nb, hip, scenes, graph_mode):
