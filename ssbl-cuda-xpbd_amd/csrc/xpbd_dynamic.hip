@@ -127,7 +127,7 @@ struct CellSpan {
     }
 };
 
-constexpr int kTriLanes = 4;      // lanes per triangle in the hash builds: lane l takes cells l, l + 4, ...
+constexpr int kTriLanes = 8;      // lanes per triangle in the hash builds: lane l takes cells l, l + 8, ...
 constexpr int kTriPerBlock = 256 / kTriLanes;
 constexpr int kAggSlots = 1024;   // LDS table of the distinct buckets a block's entries fall in
 constexpr int kAggProbes = 32;    // a bucket with no slot within this many probes counts globally, entry by entry
@@ -148,7 +148,7 @@ __device__ __forceinline__ int agg_slot(int* s_bucket, int* s_list, int* s_n, in
     return h;
 }
 
-// kTriLanes lanes per triangle (64 triangles per block).  Phase 0 counts the
+// kTriLanes lanes per triangle (32 triangles per block).  Phase 0 counts the
 // entries of every bucket; phase 1 counts them down again, each entry taking
 // the place its decrement returns in the bucket's segment, in arrival order:
 // ranked sets write raw[off[b] + slot] = {key, b} for k_tri_rank to sort;
