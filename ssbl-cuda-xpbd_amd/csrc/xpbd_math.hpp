@@ -48,14 +48,27 @@ __device__ __forceinline__ float sqrt_eps(float x) {
     return r;
 }
 
-// sqrtf(x) > m for a float x >= 0 (or NaN or +inf) and a positive normal
-// float m, without taking the root.  sqrtf(x) > m exactly when sqrt(x) lies
-// above the midpoint h between m and the next float; h has 25 significant
-// bits, so h and h*h (at most 50 bits) are exact in double, and sqrt(x) can
-// never equal h (h*h needs more than the 24 bits of a float).
+// sqrtf(x) > m for a float x >= 0 (or NaN or +inf) and a float m in
+// [2^-60, 2^60], without taking the root.  sqrtf(x) > m exactly when sqrt(x)
+// lies above the midpoint h between m and the next float; h has 25
+// significant bits, so h and h*h (at most 50 bits) are exact in double, and
+// sqrt(x) can never equal h (h*h needs more than the 24 bits of a float).
+// Most x are far from h*h, which lies in (m^2, m^2 (1 + 2^-23 + 2^-48)]:
+// with t = m*m rounded, t (1 + 2^-20) rounded is at least
+// m^2 (1 + 7 * 2^-23) and t (1 - 2^-20) rounded is below m^2, so x above the
+// first is greater, x at or below the second is not, and only x between them
+// (NaN included) takes the comparison in double.  That branch is taken by the
+// wave as a whole, only when one of its lanes needs it.
 __device__ __forceinline__ bool sqrt_greater(float x, float m) {
-    const double h = (static_cast<double>(m) + static_cast<double>(__int_as_float(__float_as_int(m) + 1))) * 0.5;
-    return static_cast<double>(x) > h * h;
+    const float t = m * m;
+    const bool above = x > t * 1.00000095367431640625f;
+    const bool band = !above && !(x <= t * 0.99999904632568359375f);
+    bool r = above;
+    if (__ballot(band) != 0ull) {
+        const double h = (static_cast<double>(m) + static_cast<double>(__int_as_float(__float_as_int(m) + 1))) * 0.5;
+        if (band) r = static_cast<double>(x) > h * h;
+    }
+    return r;
 }
 __device__ __forceinline__ float4 with_w(F3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
