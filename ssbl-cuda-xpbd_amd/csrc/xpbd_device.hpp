@@ -58,7 +58,7 @@ enum PassOps : unsigned {
     kOpBend = 1u << 9,        // uncoloured bend gather (src -> dst), instead of stretch
     kOpIntegrate = 1u << 10,  // the next substep's gravity-only integration (parity -> parity ^ 1)
     kOpIntegratePin = 1u << 11,   // ... followed by its pin projection
-    // bits 16-22 of a particle contact launch's ops: the most probing
+    // bits 16-22 of a dynamic contact launch's ops: the most probing
     // vertices a wave pools into one item list (more search per lane)
     kOpPoolShift = 16,
 };
@@ -166,8 +166,9 @@ void launch_integrate(const LaunchCtx& c, int parity, unsigned ops, bool full);
 void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops);
 // tail_ops: the closing pin polish and velocity update, fused into the
 // particle launch when nothing else runs between (0: none)
-// The particle pass pools the searches of a wave with at most `pool_max`
-// probing vertices (0..64); the events, when given, bracket that launch.
+// The dynamic contact pass (triangles, then particles) pools the particle
+// searches of a wave with at most `pool_max` probing vertices (0..64); the
+// events, when given, bracket the launch.
 constexpr int kDefaultPoolMax = 16;
 void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops, int pool_max = kDefaultPoolMax,
                          hipEvent_t begin = nullptr, hipEvent_t end = nullptr);

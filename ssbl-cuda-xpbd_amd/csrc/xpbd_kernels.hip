@@ -963,14 +963,15 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
     c.vel[v] = with_w(s.vel_, 0.0f);
 }
 
-// Dynamic collider contacts of the last iteration (A41:9230-9252) as passes
+// Dynamic collider contacts of the last iteration (A41:9230-9252) in a pass
 // of their own (the hash probes want registers the constraint passes do
-// not): triangles (kind 0), then particles (kind 1), as the reference's two
-// launches.  `tail_ops` (kind 1 only) are the iteration's closing vertex-local
-// steps when nothing else runs before them: the second pin projection and
-// the velocity update, exactly as the in-place vertex pass would apply them,
-// for every vertex of every cloth in the launch.
-template <int kKind>
+// not): the triangle contacts, then the particle contacts on the updated
+// position, as the reference's two vertex-parallel launches do; only the
+// vertex's own state is read and written, so one launch equals two.
+// `tail_ops` are the iteration's closing vertex-local steps when nothing
+// else runs before them: the second pin projection and the velocity update,
+// exactly as the in-place vertex pass would apply them, for every vertex of
+// every cloth in the launch.
 __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
                                                         int ntiles, int parity, unsigned ops, unsigned tail_ops) {
     const Tile tile = tiles[xcd_tile(ntiles)];
@@ -987,8 +988,8 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     const bool active = in_tile && p4.w != 0.0f;
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
     if (has_dyn) {
-        if (kKind == 0) dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
-        else dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u, static_cast<int>((ops >> kOpPoolShift) & 0x7fu));
+        dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
+        dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u, static_cast<int>((ops >> kOpPoolShift) & 0x7fu));
     }
     if (!in_tile) return;
     if (tail) {
@@ -1660,9 +1661,8 @@ void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned 
                          hipEvent_t end) {
     TimedLaunch t("dynamic_contacts", c.stream);
     const unsigned pops = ops | (static_cast<unsigned>(std::min(std::max(pool_max, 0), 64)) << kOpPoolShift);
-    launch_k(k_dynamic_pass<0>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops, 0u);
     if (begin) hipEventRecord(begin, c.stream);
-    launch_k(k_dynamic_pass<1>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, pops, tail_ops);
+    launch_k(k_dynamic_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, pops, tail_ops);
     if (end) hipEventRecord(end, c.stream);
 }
 

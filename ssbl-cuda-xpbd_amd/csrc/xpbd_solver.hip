@@ -262,10 +262,10 @@ struct StepGraph {
 // ---------------------------------------------------------------- tuning
 std::atomic<int> g_autotune{0};
 
-// Picks the particle contact pass's pooling limit (kOpPoolShift) for one
+// Picks the dynamic contact pass's particle pooling limit (kOpPoolShift) for one
 // handle or batch by timing the candidates on its own launches.  Every
 // kEvery-th step with dynamic contacts (the first included) is a timed step:
-// substep k runs candidate (k + round) mod kCount, its particle launch
+// substep k runs candidate (k + round) mod kCount, its contact launch
 // bracketed by events; a later step reads the events once they have
 // completed (no wait) and keeps the candidate with the least mean time, unless
 // the one in use is within 3 % of it.  Contacts change over a run (cloths meet
@@ -1113,7 +1113,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
                 launch_vertex_pass(ctx, parity, parity, contact_ops);
             }
             const unsigned tail_ops = (pin_ops & kOpPinPolish) | last_op | kOpVelocity | cnt;
-            // the dynamic contacts end the iteration: their particle launch
+            // the dynamic contacts end the iteration: their launch
             // applies the closing steps itself, and (gravity only) the next
             // substep's integration as well
             const bool tail_fused = dyn_now && !self_now && !signed_now && !(run_pcg && final_c);

@@ -46,9 +46,10 @@ def test_tuned_batch_equals_fixed_limit(nb, hip, scenes, modes):
     assert limits[0] == -1 and all(v in CANDIDATES for v in limits[1:])
     assert len(set(limits[1:EVERY + 1])) == 1 and len(set(limits[EVERY + 1:])) == 1
     # the timed steps (0 and 16) launch directly; the others replay a graph,
-    # captured again when a timed step changed the limit
+    # captured again when the second timed step changed the limit (read at
+    # the start of step 17)
     captures, replays = stats
-    assert captures == (2 if limits[EVERY] != limits[0] else 1)
+    assert captures == (2 if limits[EVERY + 1] != limits[1] else 1)
     assert captures + replays == frames - 2
 
 
