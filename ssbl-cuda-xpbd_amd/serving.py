@@ -341,6 +341,18 @@ def run_alone(req: SessionRequest, substeps: int, iterations: int, static_triang
 
 
 # ------------------------------------------------------ served workload
+# The reference's default material hardness (xpbd_core.py:20) turns the
+# hard-stretch PCG on with strength 0.4 (xpbd_core.py:238-239).
+DEFAULT_HARDNESS = 0.4
+
+
+def session_options(hardness: float = DEFAULT_HARDNESS) -> SolverOptions:
+    """Solver options of a served session: ground with friction 0.3, 5 mm
+    margin and thickness, and the PCG stretch solve at the given hardness."""
+    return SolverOptions(use_ground=True, contact_friction=0.3, collision_margin=0.005, cloth_thickness=0.005,
+                         stretch_optimization_enabled=hardness > 0.0, stretch_optimization_strength=float(hardness))
+
+
 @lru_cache(maxsize=64)
 def _grid_template(side: int, spacing: float) -> ClothBuildData:
     pos, tris = grid_mesh(side, (side - 1) * spacing)
@@ -384,7 +396,7 @@ def served_requests(*, seed: int = 0, rate: float = 200.0, window_s: float = 1.0
         sphere = SpherePath(center=(0.0, 0.0, 0.9), amplitude=(0.6 * half, 0.6 * half, 0.2),
                             frequency=tuple(float(x) for x in rng.uniform(0.1, 0.5, size=3)),
                             phase=tuple(float(x) for x in rng.uniform(0.0, 2.0 * math.pi, size=3)), radius=0.3)
-        opts = SolverOptions(use_ground=True, contact_friction=0.3, collision_margin=0.005, cloth_thickness=0.005)
+        opts = session_options()
         reqs.append(SessionRequest(sid, t if arrival_unit == "s" else float(int(t)), session_cloth(side, seed * 100003 + sid),
                                    opts, frames, wind, sphere))
         sid += 1

@@ -1,6 +1,8 @@
-"""Serving on the GPU: sessions served in a continuously batched mix give
-exactly the result of the same session stepped alone (tolerance 0: batched
-stepping is bit-identical to single stepping), and the admission estimate
+"""Serving on the GPU: sessions served in a continuously batched mix (PCG
+stretch solve on, wind, a moving sphere) give exactly the result of the same
+session stepped alone, and that equals the oracle's result for the session
+(tolerance 0: batched stepping is bit-identical to single stepping, and the
+HIP path reproduces the oracle's FP32 operations); the admission estimate
 bounds the device memory a handle really holds."""
 from dataclasses import replace
 
@@ -18,7 +20,7 @@ def serving():
     return s
 
 
-def test_mix_equals_alone_on_gpu(serving, scenes, hip):
+def test_mix_equals_alone_on_gpu(serving, scenes, hip, oracle):
     reqs = small_requests(serving, scenes, sides=[40, 100, 24, 64, 32, 80], frames=[8, 5, 12, 6, 10, 4],
                           arrivals=[0, 0, 1, 2, 2, 5])
     est = sorted(serving.estimate_device_bytes(r.cloth, r.options) for r in reqs)
@@ -30,12 +32,17 @@ def test_mix_equals_alone_on_gpu(serving, scenes, hip):
     assert rep.peak_device_bytes <= budget
     assert rep.peak_sessions >= 3                   # sessions overlapped in one batch
     for r in reqs:
+        assert r.options.stretch_optimization_enabled      # PCG on, as served sessions run
         alone = serving.run_alone(r, 10, 8, lib=hip)
+        ref = serving.run_alone(r, 10, 8, lib=oracle)
         got = rep.results[r.sid].positions
         assert np.isfinite(got).all()
         diff = float(np.max(np.abs(got - alone)))
-        print(f"session {r.sid}: {r.vertex_count} vertices, {r.frames} frames, max |mix - alone| = {diff:.3e}")
+        dref = float(np.max(np.abs(alone - ref)))
+        print(f"session {r.sid}: {r.vertex_count} vertices, {r.frames} frames, max |mix - alone| = {diff:.3e}, "
+              f"max |alone - oracle| = {dref:.3e}")
         assert diff == 0.0
+        assert dref == 0.0
 
 
 def test_device_bytes_within_estimate(serving, scenes, hip, nb):

@@ -11,7 +11,7 @@ def small_requests(serving, scenes, sides, frames, arrivals):
     reqs = []
     for sid, (side, nf, t) in enumerate(zip(sides, frames, arrivals)):
         cloth = serving.session_cloth(side, seed=sid, spacing=0.05)
-        opts = scenes.SolverOptions(use_ground=True, contact_friction=0.3, collision_margin=0.005, cloth_thickness=0.005)
+        opts = serving.session_options()
         wind = serving.wind_field((1.0, 0.3, 0.2), 5.0 + sid)
         half = 0.5 * (side - 1) * 0.05
         sphere = serving.SpherePath((0.0, 0.0, 1.2), (0.5 * half, 0.5 * half, 0.1), (0.3, 0.4, 0.2), (0.0, 1.0, 2.0), 0.2)
