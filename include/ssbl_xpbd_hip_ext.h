@@ -115,6 +115,22 @@ SSBL_API int ssbl_ext_tuned_pool_limit(void* handle, void* batch);
 SSBL_API int ssbl_ext_kernel_timing(int enable);
 SSBL_API int ssbl_ext_kernel_stats(char* json_out, int capacity);
 
+/* Allocation-free variant of ssbl_step_solver_ex (same arguments, same
+ * results).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory mid-run (serving loops, harnesses that forbid it,
+ * code around graph capture).  It calls no hipMalloc, hipFree or other
+ * allocating routine: the hard-stretch PCG solve takes its scratch (iteration
+ * vectors, 3x3 blocks, reduction partials, scalars) from a static device array
+ * of 96 MiB per GPU, held until the process ends, instead of the handle's own
+ * arrays, and the step launches directly (no graph capture or instantiation,
+ * no launch tuning).  A handle whose solve needs more scratch than that
+ * (grid cloths above about 330k vertices) is rejected with an error.  The
+ * array is shared by every call on one GPU and keeps the last call's
+ * contents; calls are ordered on the library's stream, and calls that
+ * overlap on one GPU from several host threads race.  Everything a step may
+ * allocate besides is sized when the handle and its pins are uploaded. */
+SSBL_API int ssbl_ext_step_solver_ex_noalloc(void* handle, int substeps, int iterations, int fetch_diagnostics, int force_sync);
+
 /* Implementation name and device description, e.g. "hip gfx950 ...". */
 SSBL_API const char* ssbl_ext_backend_name(void);
 

@@ -3148,6 +3148,11 @@ SSBL_API int ssbl_step_solver_ex(void* handle, int substeps, int iterations, int
 
 SSBL_API int ssbl_step_solver(void* handle, int substeps, int iterations) { return ssbl_step_solver_ex(handle, substeps, iterations, 1, 1); }
 
+// The allocation-free step variant of the GPU library; on the CPU it is the step itself.
+SSBL_API int ssbl_ext_step_solver_ex_noalloc(void* handle, int substeps, int iterations, int fetch_diagnostics, int force_sync) {
+    return ssbl_step_solver_ex(handle, substeps, iterations, fetch_diagnostics, force_sync);
+}
+
 SSBL_API int ssbl_download_positions(void* handle, float* out, int max_floats) {
     g_error.clear();
     if (!handle || !out) return fail("invalid position download") ? 1 : 0;

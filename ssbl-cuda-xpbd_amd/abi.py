@@ -138,6 +138,7 @@ EXT_SIGNATURES = {
     "ssbl_ext_kernel_timing": (_i, [_i]),
     "ssbl_ext_kernel_stats": (_i, [ctypes.c_char_p, _i]),
     "ssbl_ext_backend_name": (ctypes.c_char_p, []),
+    "ssbl_ext_step_solver_ex_noalloc": (_i, [_vp, _i, _i, _i, _i]),
 }
 
 

@@ -21,9 +21,37 @@
 #include "xpbd_timing.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace ssbl {
+
+// The static scratch of the allocation-free step (ssbl_ext_step_solver_ex_noalloc):
+// one solve's vectors, blocks, partials and scalars.  In host code `scratch`
+// names a host-side stand-in, not the device array: kernels only ever get the
+// device address looked up below.
+static __device__ __align__(256) unsigned char scratch[kPcgScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_base[kScratchDevices] = {};
+}  // namespace
+
+bool pcg_static_scratch(void** base, size_t* bytes) {
+    int dev = -1;
+    if (!hip_ok(hipGetDevice(&dev), "current device")) return false;
+    if (dev < 0 || dev >= kScratchDevices) return fail("device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_base[dev]) {
+        void* p = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "static scratch address") || !p) return false;
+        g_scratch_base[dev] = p;
+    }
+    *base = g_scratch_base[dev];
+    *bytes = kPcgScratchBytes;
+    return true;
+}
 
 namespace {
 
@@ -523,14 +551,61 @@ bool PcgPath::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, 
         if (!segs.empty() && !hip_ok(hipMemcpy(segs_, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice), "PCG colours"))
             return false;
     }
+    own_ = pv_;
     ready_ = true;
     return true;
 }
 
+// Scratch layout: six float4 vectors and two 3x3 symmetric blocks per vertex,
+// one block per directed edge, two 64-bit words per tile, the scalars; each
+// part starts on a 256-byte boundary.
+namespace {
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+}  // namespace
+
+size_t PcgPath::scratch_bytes() const {
+    const size_t nv = static_cast<size_t>(own_.nv);
+    return 6 * align256(sizeof(float4) * nv) + 2 * align256(sizeof(float) * 6 * nv) +
+           align256(sizeof(float) * 6 * static_cast<size_t>(nnz_)) + align256(sizeof(unsigned long long) * 2 * static_cast<size_t>(ntiles_)) +
+           align256(sizeof(PcgScalars)) + 2 * align256(sizeof(unsigned long long));
+}
+
+bool PcgPath::use_scratch(void* base, size_t bytes) {
+    if (!ready_) return true;
+    const size_t need = scratch_bytes();
+    if (need > bytes)
+        return fail("the hard-stretch solve of this cloth needs " + std::to_string(need) + " bytes of scratch; the static scratch holds " +
+                    std::to_string(bytes) + " bytes");
+    char* p = static_cast<char*>(base);
+    auto take = [&](size_t n) { char* q = p; p += align256(n); return q; };
+    PcgView v = own_;
+    const size_t nv = static_cast<size_t>(own_.nv);
+    v.rhs = reinterpret_cast<float4*>(take(sizeof(float4) * nv));
+    v.sol = reinterpret_cast<float4*>(take(sizeof(float4) * nv));
+    v.res = reinterpret_cast<float4*>(take(sizeof(float4) * nv));
+    v.z = reinterpret_cast<float4*>(take(sizeof(float4) * nv));
+    v.dir = reinterpret_cast<float4*>(take(sizeof(float4) * nv));
+    v.adir = reinterpret_cast<float4*>(take(sizeof(float4) * nv));
+    v.diag = reinterpret_cast<float*>(take(sizeof(float) * 6 * nv));
+    v.pinv = reinterpret_cast<float*>(take(sizeof(float) * 6 * nv));
+    v.off = reinterpret_cast<float*>(take(sizeof(float) * 6 * static_cast<size_t>(nnz_)));
+    v.partials = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 2 * static_cast<size_t>(ntiles_)));
+    v.sc = reinterpret_cast<PcgScalars*>(take(sizeof(PcgScalars)));
+    v.guard = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long)));
+    v.max_delta = reinterpret_cast<float*>(take(sizeof(unsigned long long)));
+    pv_ = v;
+    return true;
+}
+
+// Clears the scalars and the guard / max-delta counters with fill launches
+// (no memset nodes, so a caller may capture the step).
 bool PcgPath::reset_diagnostics(hipStream_t st) {
     if (!ready_) return true;
-    return hip_ok(hipMemsetAsync(pv_.sc, 0, sizeof(PcgScalars), st), "PCG reset") &&
-           hip_ok(hipMemsetAsync(pv_.guard, 0, 8, st), "PCG reset") && hip_ok(hipMemsetAsync(pv_.max_delta, 0, 4, st), "PCG reset");
+    static_assert(sizeof(PcgScalars) % 4 == 0, "PcgScalars is filled as 32-bit words");
+    launch_fill_u32(st, reinterpret_cast<unsigned*>(pv_.sc), 0u, sizeof(PcgScalars) / 4);
+    launch_fill_u32(st, reinterpret_cast<unsigned*>(pv_.guard), 0u, 2);
+    launch_fill_u32(st, reinterpret_cast<unsigned*>(pv_.max_delta), 0u, 1);
+    return hip_ok(hipGetLastError(), "PCG reset");
 }
 
 void PcgPath::solve(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity) {

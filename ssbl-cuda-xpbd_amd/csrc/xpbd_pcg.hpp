@@ -34,6 +34,16 @@ struct PcgView {
     unsigned long long* counts;
 };
 
+// Size of the library's static PCG scratch array (per GPU, held until the
+// process ends): the iteration vectors, block diagonals, off-diagonal blocks,
+// reduction partials and scalars of one hard-stretch solve, for cloths up to
+// about 330k grid vertices.
+constexpr size_t kPcgScratchBytes = size_t(96) << 20;
+
+// Device address and size of the static scratch array on the current device
+// (looked up once per device).
+bool pcg_static_scratch(void** base, size_t* bytes);
+
 class PcgPath {
 public:
     PcgPath() = default;
@@ -49,6 +59,13 @@ public:
     int final_caps(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity, float dt);
     int copy(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity) { return gather(st, cv, tiles, parity, -1, 0.0f); }
     bool fetch(SsblXpbdDiagnostics* d) const;
+    // Bytes of scratch one solve of this cloth needs.
+    size_t scratch_bytes() const;
+    // Points the solve's scratch (vectors, blocks, partials, scalars) into
+    // [base, base + bytes) instead of the handle's own arrays; fails when it
+    // does not fit.  use_own() switches back.
+    bool use_scratch(void* base, size_t bytes);
+    void use_own() { pv_ = own_; }
 
 private:
     int gather(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity, int kind, float dt);
@@ -57,7 +74,8 @@ private:
     bool ready_ = false;
     bool colored_ = false;
     int nnz_ = 0, ntiles_ = 0;
-    PcgView pv_{};
+    PcgView pv_{};    // the view the launches use
+    PcgView own_{};   // the view over the handle's own arrays
     int4* segs_ = nullptr;
     std::vector<int> seg_first_, seg_count_;
     std::vector<std::pair<void*, size_t>> allocations_;

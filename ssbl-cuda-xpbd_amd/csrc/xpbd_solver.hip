@@ -164,15 +164,20 @@ struct Staging {
         }
         bytes = 0;
     }
-    bool upload(void* dst, const void* src, size_t n, hipStream_t s) {
-        if (n > bytes) {
-            release();
-            for (int i = 0; i < kRing; ++i) {
-                if (!hip_ok(hipHostMalloc(&host[i], n), "staging allocation")) return false;
-                if (!hip_ok(hipEventCreateWithFlags(&done[i], hipEventDisableTiming), "staging event")) return false;
-            }
-            bytes = n;
+    // Sizes the ring for uploads of up to n bytes (done at set-up, so that a
+    // step allocates nothing).
+    bool reserve(size_t n) {
+        if (n <= bytes) return true;
+        release();
+        for (int i = 0; i < kRing; ++i) {
+            if (!hip_ok(hipHostMalloc(&host[i], n), "staging allocation")) return false;
+            if (!hip_ok(hipEventCreateWithFlags(&done[i], hipEventDisableTiming), "staging event")) return false;
         }
+        bytes = n;
+        return true;
+    }
+    bool upload(void* dst, const void* src, size_t n, hipStream_t s) {
+        if (!reserve(n)) return false;
         const int slot = next;
         next = (next + 1) % kRing;
         if (!hip_ok(hipEventSynchronize(done[slot]), "staging wait")) return false;
@@ -749,7 +754,7 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
               rest_scale.upload(h_scale, "pressure scale allocation") && pin_off.upload(h_pin_off, "pin allocation") &&
               counts.alloc(kCountSlots, "diagnostic allocation") && max_smoothed.alloc(1, "diagnostic allocation") &&
               d_view.alloc(1, "view allocation") && d_tiles.upload(h_tiles, "tile allocation");
-    if (!ok) return false;
+    if (!ok || !staging.reserve(sizeof(ClothView))) return false;
     if (!hip_ok(hipMemset(counts.p, 0, sizeof(unsigned long long) * kCountSlots), "diagnostic reset")) return false;
     if (!hip_ok(hipMemset(max_smoothed.p, 0, sizeof(float)), "diagnostic reset")) return false;
     if (cfg.self_collision && !self.prepare(cfg, edges, tri, n, counts.p)) return false;
@@ -895,7 +900,14 @@ bool Solver::upload_pins(const int* idx, const float* p, const float* w, int cou
     pins.clear();
     for (int k = 0; k < count; ++k) pins.push_back(HostPin{idx[k], {p[3 * k], p[3 * k + 1], p[3 * k + 2]}, w ? w[k] : 1.0f});
     pins_iterations = -1;
-    return true;
+    // the device table is sized here, so that refreshing it in a step (a new
+    // iteration count changes the relaxation) allocates nothing
+    size_t in_range = 0;
+    for (const HostPin& pin : pins) in_range += (pin.index >= 0 && pin.index < nv) ? 1 : 0;
+    if (pin_off.n == static_cast<size_t>(nv) + 1 && pin_ids.n == in_range && d_pins.n == pins.size()) return true;
+    if (!sync_stream()) return false;
+    return pin_off.alloc(static_cast<size_t>(nv) + 1, "pin allocation") && pin_ids.alloc(in_range, "pin allocation") &&
+           d_pins.alloc(pins.size(), "pin allocation");
 }
 
 bool Solver::refresh_pins(int iterations) {
@@ -1247,7 +1259,9 @@ static bool prepare_member(Solver& s, int substeps, int iterations, bool count) 
     return true;
 }
 
-static bool step_one(Solver& s, int substeps, int iterations, int fetch, int force_sync) {
+// no_alloc: the allocation-free variant -- launches directly (no graph
+// capture or instantiation) and never times launch candidates.
+static bool step_one(Solver& s, int substeps, int iterations, int fetch, int force_sync, bool no_alloc = false) {
     const auto t0 = std::chrono::steady_clock::now();
     reset_diag(s);
     substeps = std::max(substeps, 1);
@@ -1260,18 +1274,18 @@ static bool step_one(Solver& s, int substeps, int iterations, int fetch, int for
         std::vector<Solver*> one{&s};
         SelfBatch* sb = s.cfg.self_collision ? s.self.single_batch(s.view, s.d_view.p) : nullptr;
         const StepPlan plan = plan_of(one);
-        if (plan.any_dyn && !s.tuner.poll(s.tuner.tuning())) return false;
-        const bool tune = plan.any_dyn && s.tuner.tuning();
+        if (plan.any_dyn && !no_alloc && !s.tuner.poll(s.tuner.tuning())) return false;
+        const bool tune = plan.any_dyn && !no_alloc && s.tuner.tuning();
         if (tune && !s.tuner.begin_step(substeps)) return false;
         auto record = [&] {
             return run_schedule(one, ctx, s.d_view.p, s.bend_segs.p, s.bend_seg_first, s.bend_seg_count, sb, substeps, iterations,
                                 count, s.tuner, tune);
         };
         std::vector<long long> key;
-        const bool graphed = graph_key(plan, substeps, iterations, count, s.tuner, &key);
+        const bool graphed = !no_alloc && graph_key(plan, substeps, iterations, count, s.tuner, &key);
         if (!(graphed ? s.graph.run(lib_stream(), key, record) : record())) return false;
         if (tune) s.tuner.end_step();
-        if (plan.any_dyn) ++s.tuner.step;
+        if (plan.any_dyn && !no_alloc) ++s.tuner.step;
     }
     if (!s.dyn.clear_motion(lib_stream())) return false;
     if (force_sync || fetch) {
@@ -1548,6 +1562,23 @@ SSBL_API int ssbl_step_solver_ex(void* handle, int substeps, int iterations, int
     Solver* s = static_cast<Solver*>(handle);
     if (!s) return fail("invalid solver handle") ? 1 : 0;
     return step_one(*s, substeps, iterations, fetch_diagnostics, force_sync) ? 1 : 0;
+}
+
+// The allocation-free variant of ssbl_step_solver_ex: the hard-stretch solve
+// takes its scratch from the library's static array instead of the handle's
+// own arrays, and the step launches directly; nothing is allocated or freed.
+SSBL_API int ssbl_ext_step_solver_ex_noalloc(void* handle, int substeps, int iterations, int fetch_diagnostics, int force_sync) {
+    g_error.clear();
+    Solver* s = static_cast<Solver*>(handle);
+    if (!s) return fail("invalid solver handle") ? 1 : 0;
+    if (s->pcg.enabled()) {
+        void* base = nullptr;
+        size_t bytes = 0;
+        if (!pcg_static_scratch(&base, &bytes) || !s->pcg.use_scratch(base, bytes)) return 0;
+    }
+    const bool ok = step_one(*s, substeps, iterations, fetch_diagnostics, force_sync, true);
+    s->pcg.use_own();
+    return ok ? 1 : 0;
 }
 
 SSBL_API int ssbl_step_solver(void* handle, int substeps, int iterations) {
