@@ -24,7 +24,12 @@ extern "C" {
  * pass (ssbl_xpbd_cuda_abi41.cu:1927-1980) made deterministic as a Jacobi
  * sweep -- every edge reads the positions from before the pass and each vertex
  * sums its corrections in ascending edge order.  1: graph-coloured
- * Gauss-Seidel, one pass per edge colour in colour order. */
+ * Gauss-Seidel, one pass per edge colour in colour order (the colouring of
+ * the mesh's edge_color_offsets, as the reference's per-colour bend and cap
+ * launches use theirs, ssbl_xpbd_cuda_abi41.cu:9207-9214), each edge reading
+ * the positions its colour's predecessors left.  Mode 1 fails unless the
+ * handle has edge colour offsets with no vertex twice in one colour.  The
+ * members of a batch must share the mode when the batch is created. */
 enum { SSBL_EXT_STRETCH_JACOBI = 0, SSBL_EXT_STRETCH_COLORED = 1 };
 SSBL_API int ssbl_ext_set_stretch_mode(void* handle, int mode);
 

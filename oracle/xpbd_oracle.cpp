@@ -1342,14 +1342,14 @@ template <class R> void Solver<R>::stretch_jacobi(R dt) {
 }
 
 // Coloured Gauss-Seidel form of the same projection: colours in order, each
-// constraint reading the current positions.
+// constraint reading the current positions (the mode requires a valid
+// colouring, so a colour's constraints touch disjoint vertices).
 template <class R> void Solver<R>::stretch_colored(R dt) {
     const int n = V();
     const R alpha = std::max(R(cfg.stretch_compliance), R(0)) / std::max(dt * dt, R(kEpsF));
-    const int ncol = edge_colors.empty() ? 1 : cfg.edge_color_count;
-    for (int c = 0; c < ncol; ++c) {
-        int lo = edge_colors.empty() ? 0 : std::max(edge_colors[c], 0);
-        int hi = edge_colors.empty() ? cfg.edge_count : std::min(edge_colors[c + 1], cfg.edge_count);
+    for (int c = 0; c < cfg.edge_color_count; ++c) {
+        int lo = std::max(edge_colors[c], 0);
+        int hi = std::min(edge_colors[c + 1], cfg.edge_count);
         for (int e = lo; e < hi; ++e) {
             const Edge& s = springs[e];
             int i = s.i, j = s.j;
@@ -3230,7 +3230,12 @@ SSBL_API const char* ssbl_last_error(void) { return g_error.c_str(); }
 SSBL_API int ssbl_ext_set_stretch_mode(void* handle, int mode) {
     g_error.clear();
     if (!handle || (mode != SSBL_EXT_STRETCH_JACOBI && mode != SSBL_EXT_STRETCH_COLORED)) return fail("invalid stretch mode") ? 1 : 0;
-    return with(handle, [&](auto& s) { s.stretch_mode = mode; return 1; });
+    return with(handle, [&](auto& s) {
+        if (mode == SSBL_EXT_STRETCH_COLORED && !s.edge_coloring_valid)
+            return fail("coloured stretch needs a valid edge colouring (edge color offsets, no vertex twice in one colour)") ? 1 : 0;
+        s.stretch_mode = mode;
+        return 1;
+    });
 }
 
 SSBL_API int ssbl_ext_download_state(void* handle, float* p, float* q, float* v, int max_floats) {

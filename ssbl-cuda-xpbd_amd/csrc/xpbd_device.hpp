@@ -150,6 +150,9 @@ struct ClothView {
     const DynView* sdyn;
     float collider_dt;
     int pad5;
+    // Stretch edges {i, j, rest length bits, 0} in input (colour-sorted)
+    // order, for the coloured Gauss-Seidel stretch mode (null otherwise).
+    const int4* sedges;
 };
 
 struct Tile { int cloth, begin, end, pad; };
@@ -176,6 +179,8 @@ void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned 
 // inside-only passes of a step's last iteration (A41:9322-9359).
 void launch_signed_pass(const LaunchCtx& c, int parity, bool inside, unsigned ops);
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops);
+// One colour of the coloured stretch mode; segments {cloth, begin, end}.
+void launch_stretch_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops);
 void launch_sdf_build(hipStream_t s, float* sdf, int dx, int dy, int dz, float ox, float oy, float oz, float voxel,
                       const float* tris, int ntris, int signed_sdf);
 void launch_fill_u32(hipStream_t s, unsigned* p, unsigned value, size_t n);

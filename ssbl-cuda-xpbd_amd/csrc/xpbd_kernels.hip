@@ -1520,6 +1520,54 @@ __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __res
     if (s.v_dirty) gstore(c.vel + v, with_w(s.vel_, 0.0f));
 }
 
+// ------------------------------------------------------ coloured stretch
+// Graph-coloured Gauss-Seidel form of the direct stretch projection
+// (A41:1927-1980): one launch per edge colour over the colour's segments of
+// every cloth ({cloth, begin, end}, 256 edges each), one thread per edge,
+// positions and previous positions updated in place -- race free because no
+// vertex appears twice in a colour, so the result equals a sequential sweep
+// of the colour.  Each edge is evaluated once for both endpoints with the
+// arithmetic of stretch_edge; the record {i, j, rest bits, 0} is one 16-byte
+// load.  Consecutive segments (neighbouring edges of one cloth) go to the
+// same XCD.
+__global__ __launch_bounds__(kTile) void k_stretch_color(const ClothView* __restrict__ views, const int4* __restrict__ segs,
+                                                         int nsegs, int parity, unsigned ops) {
+    const int4 seg = segs[xcd_tile(nsegs)];
+    const int k = seg.y + static_cast<int>(threadIdx.x);
+    if (k >= seg.z) return;
+    const ClothView& c = views[seg.x];
+    float4* pos = parity ? c.pos1 : c.pos0;
+    const int4 e = c.sedges[k];
+    const int i = e.x, j = e.y;
+    if (i >= c.nv || j >= c.nv) return;
+    const float4 pi4 = pos[i], pj4 = pos[j];
+    const float wi = pi4.w, wj = pj4.w;
+    const float w = wi + wj;
+    if (w <= 0.0f) return;
+    const F3 pi = f3(pi4), pj = f3(pj4);
+    const F3 delta = pj - pi;
+    const float len = sqrt_eps(dot(delta, delta));
+    const float restl = __int_as_float(e.z);
+    const float cval = len - restl;
+    const float dl = -cval / (w + c.stretch_alpha);
+    F3 corr = mul(delta, 0.18f * dl / len);
+    const float csq = fmaxf(dot(corr, corr), 0.0f);
+    const float mc = fmaxf(0.0025f, fminf(fmaxf(restl, 0.0f) * 0.10f, 0.025f));
+    const bool clamped = sqrt_greater(csq, mc);
+    if (clamped) corr = mul(corr, mc / fmaxf(sqrtf(csq), kEps));
+    if (wi > 0.0f) {
+        const F3 d = mul(corr, -wi);
+        pos[i] = with_w(pi + d, wi);
+        if (cval > 0.0f) c.prev[i] = with_w(f3(c.prev[i]) + mul(d, 0.08f), 0.0f);
+    }
+    if (wj > 0.0f) {
+        const F3 d = mul(corr, wj);
+        pos[j] = with_w(pj + d, wj);
+        if (cval > 0.0f) c.prev[j] = with_w(f3(c.prev[j]) + mul(d, 0.08f), 0.0f);
+    }
+    if (ops & kOpCount) wave_count(c.counts, kCountHardFallbacks, clamped);
+}
+
 // -------------------------------------------------------------- bending
 // A41:2886-2970, one launch per colour; a segment is {cloth, begin, end}.
 __global__ __launch_bounds__(kTile) void k_bend_color(const ClothView* __restrict__ views, const int4* __restrict__ segs,
@@ -1674,6 +1722,12 @@ void launch_signed_pass(const LaunchCtx& c, int parity, bool inside, unsigned op
     } else {
         launch_k(k_signed_pass<false>, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, ops);
     }
+}
+
+void launch_stretch_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops) {
+    if (nseg_blocks <= 0) return;
+    TimedLaunch t("stretch_color", c.stream);
+    launch_k(k_stretch_color, dim3(nseg_blocks), dim3(kTile), 0, c.stream, c.views, segments, nseg_blocks, parity, ops);
 }
 
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops) {

@@ -410,6 +410,14 @@ struct Solver {
     Dev<Tile> d_tiles;
     std::vector<int> bend_seg_first, bend_seg_count;   // per colour, into bend_segs
     std::vector<int4> h_bend_segs;
+    // coloured stretch mode: the edges as uploaded, whether their colouring
+    // is valid (no vertex twice in a colour), and, once the mode is chosen,
+    // the device edge records and per-colour segments
+    std::vector<int4> h_sedges;
+    bool edge_coloring_valid = false;
+    Dev<int4> sedges, stretch_segs;
+    std::vector<int4> h_stretch_segs;
+    std::vector<int> stretch_seg_first, stretch_seg_count;
     int ntiles = 0;
     Staging staging;
     SelfCollision self;
@@ -433,6 +441,10 @@ struct Solver {
     bool upload_pins(const int* idx, const float* p, const float* w, int count);
     bool refresh_pins(int iterations);
     bool upload_fields(const SsblXpbdForceField* f, int count, int unsupported);
+    bool set_stretch_mode(int mode);
+    bool colored_stretch() const {
+        return stretch_mode == SSBL_EXT_STRETCH_COLORED && cfg.edge_count > 0 && !cfg.stretch_optimization_enabled;
+    }
     void fill_view(int substeps);
     float self_radius() const {
         const float req = std::max(cfg.self_collision_distance, 0.0f);
@@ -632,6 +644,25 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     if (cfg.edge_color_count > 0) {
         if (!mesh->edge_color_offsets) return fail("ABI41 edge color offsets are required");
         edge_colors.assign(mesh->edge_color_offsets, mesh->edge_color_offsets + cfg.edge_color_count + 1);
+    }
+    h_sedges.resize(cfg.edge_count);
+    for (int e = 0; e < cfg.edge_count; ++e) {
+        int rb;
+        std::memcpy(&rb, &edge_rest[e], 4);
+        h_sedges[e] = make_int4(edges[2 * e], edges[2 * e + 1], rb, 0);
+    }
+    edge_coloring_valid = !edge_colors.empty();
+    if (edge_coloring_valid) {
+        std::vector<int> stamp(n, -1);
+        for (int c = 0; c < cfg.edge_color_count && edge_coloring_valid; ++c) {
+            const int lo = std::max(edge_colors[c], 0), hi = std::min(edge_colors[c + 1], cfg.edge_count);
+            for (int e = lo; e < hi; ++e) {
+                const int i = edges[2 * e], j = edges[2 * e + 1];
+                if (i >= n || j >= n) continue;
+                if (stamp[i] == c || stamp[j] == c) { edge_coloring_valid = false; break; }
+                stamp[i] = c; stamp[j] = c;
+            }
+        }
     }
     if (cfg.bend_count > 0 && (!mesh->bends || !mesh->bend_rest_lengths))
         return fail("ABI41 bend constraints require bend pairs and rest lengths");
@@ -953,6 +984,33 @@ bool Solver::upload_fields(const SsblXpbdForceField* f, int count, int unsupport
     return d_fields.upload(fields, "force field upload");
 }
 
+// Mode 1 (coloured Gauss-Seidel) needs a valid edge colouring: its one
+// launch per colour updates both endpoints in place.
+bool Solver::set_stretch_mode(int mode) {
+    if (mode == SSBL_EXT_STRETCH_COLORED) {
+        if (!edge_coloring_valid)
+            return fail("coloured stretch needs a valid edge colouring (edge color offsets, no vertex twice in one colour)");
+        // cloths on the hard-stretch solve keep the mode but run no
+        // direct stretch: no segments, so no colour launch reaches them
+        if (!cfg.stretch_optimization_enabled && sedges.n != h_sedges.size()) {
+            h_stretch_segs.clear();
+            stretch_seg_first.clear();
+            stretch_seg_count.clear();
+            for (int c = 0; c < cfg.edge_color_count; ++c) {
+                const int lo = std::max(edge_colors[c], 0), hi = std::min(edge_colors[c + 1], cfg.edge_count);
+                stretch_seg_first.push_back(static_cast<int>(h_stretch_segs.size()));
+                for (int e = lo; e < hi; e += kTile) h_stretch_segs.push_back(make_int4(0, e, std::min(hi, e + kTile), 0));
+                stretch_seg_count.push_back(static_cast<int>(h_stretch_segs.size()) - stretch_seg_first.back());
+            }
+            if (!sync_stream() || !sedges.upload(h_sedges, "stretch edge allocation") ||
+                !stretch_segs.upload(h_stretch_segs, "stretch segment allocation"))
+                return false;
+        }
+    }
+    stretch_mode = mode;
+    return true;
+}
+
 void Solver::fill_view(int substeps) {
     ClothView& v = view;
     v = ClothView{};
@@ -1003,6 +1061,7 @@ void Solver::fill_view(int substeps) {
     v.dyn_mode = use_scene() ? cross_mode : 0;
     v.dyn_target_src = target_src;
     v.dyn_target_layer = target_layer;
+    v.sedges = colored_stretch() ? sedges.p : nullptr;
 }
 
 // ------------------------------------------------------------ schedule
@@ -1012,6 +1071,7 @@ void Solver::fill_view(int substeps) {
 struct StepPlan {
     bool any_pins = false, any_edges = false, any_lra = false, any_bend = false, any_self = false, any_pcg = false;
     bool any_bend_gather = false, any_forces = false, any_dyn = false, any_signed = false;
+    bool colored = false;   // direct stretch as coloured Gauss-Seidel (every member shares the mode)
 };
 
 static StepPlan plan_of(const std::vector<Solver*>& members) {
@@ -1027,6 +1087,7 @@ static StepPlan plan_of(const std::vector<Solver*>& members) {
         plan.any_forces |= s->view.need_normal != 0 || s->view.nfield > 0;
         plan.any_dyn |= s->view.dyn != nullptr;
         plan.any_signed |= s->view.sdyn != nullptr;
+        plan.colored |= s->colored_stretch();
     }
     return plan;
 }
@@ -1039,19 +1100,39 @@ static bool graph_key(const StepPlan& plan, int substeps, int iterations, bool c
     if (g_graph_mode.load(std::memory_order_relaxed) == 0 || timing_enabled() || plan.any_self || plan.any_pcg) return false;
     if (plan.any_dyn && tuner.tuning()) return false;
     *key = {substeps, iterations, count ? 1 : 0, plan.any_pins, plan.any_edges, plan.any_lra, plan.any_bend,
-            plan.any_bend_gather, plan.any_forces, plan.any_dyn, plan.any_signed, tuner.value()};
+            plan.any_bend_gather, plan.any_forces, plan.any_dyn, plan.any_signed, plan.colored, tuner.value()};
     return true;
 }
 
-static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const int4* bend_segs_dev,
-                         const std::vector<int>& seg_first, const std::vector<int>& seg_count, SelfBatch* self_batch, int substeps,
-                         int iterations, bool count, PoolTuner& tuner, bool tune) {
+// Per-colour segment tables ({cloth, begin, end}) of a handle or batch.
+struct ColorSegs {
+    const int4* dev;
+    const std::vector<int>* first;
+    const std::vector<int>* count;
+};
+
+static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const ColorSegs& bends,
+                         const ColorSegs& stretch_colors, SelfBatch* self_batch, int substeps, int iterations, bool count,
+                         PoolTuner& tuner, bool tune) {
     const StepPlan plan = plan_of(members);
     const unsigned cnt = count ? kOpCount : 0u;
     const unsigned pin_ops = plan.any_pins ? (kOpPin | kOpPinPolish) : 0u;
     const unsigned stretch = plan.any_edges ? kOpStretch : 0u;
     hipStream_t st = ctx.stream;
     int parity = 0;
+    // The direct stretch projection followed by the vertex-local work `ops`:
+    // the Jacobi gather fused with them in one pass (src -> dst), or, in the
+    // coloured mode, one in-place launch per colour and then `ops` in place.
+    auto stretch_then = [&](unsigned ops) {
+        if (plan.colored && plan.any_edges) {
+            for (size_t c = 0; c < stretch_colors.first->size(); ++c)
+                launch_stretch_color(ctx, stretch_colors.dev + (*stretch_colors.first)[c], (*stretch_colors.count)[c], parity, cnt);
+            if (ops & ~kOpCount) launch_vertex_pass(ctx, parity, parity, ops);
+            return;
+        }
+        launch_vertex_pass(ctx, parity, parity ^ 1, stretch | ops);
+        parity ^= 1;
+    };
     // the hard-stretch passes run per member; gathers flip a member's buffers,
     // so each member is realigned to the common parity afterwards
     auto per_pcg = [&](auto&& fn) {
@@ -1085,9 +1166,7 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             const bool run_pcg = plan.any_pcg && last;   // substep cadence 1 (A41:284-292)
             const bool split = run_pcg || self_now || dyn_now || signed_now || (final_c && (plan.any_lra || plan.any_bend));
             if (!split) {
-                launch_vertex_pass(ctx, parity, parity ^ 1,
-                                   stretch | pin_ops | kOpAnalytic | kOpSdf | last_op | (last ? kOpVelocity : 0u) | cnt);
-                parity ^= 1;
+                stretch_then(pin_ops | kOpAnalytic | kOpSdf | last_op | (last ? kOpVelocity : 0u) | cnt);
                 continue;
             }
             // A41:9178-9369 order: stretch (direct or PCG + polish + caps),
@@ -1099,11 +1178,9 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             if (!run_pcg && !(final_c && (plan.any_lra || plan.any_bend))) {
                 // nothing runs between the stretch gather and the pins and
                 // contacts: one pass does both, as the unsplit iteration does
-                launch_vertex_pass(ctx, parity, parity ^ 1, stretch | contact_ops);
-                parity ^= 1;
+                stretch_then(contact_ops);
             } else {
-                launch_vertex_pass(ctx, parity, parity ^ 1, stretch | cnt);
-                parity ^= 1;
+                stretch_then(cnt);
                 if (run_pcg) {
                     per_pcg([&](Solver& s, const ClothView* cv) {
                         s.pcg.solve(st, cv, s.d_tiles.p, parity);
@@ -1115,8 +1192,8 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
                     parity ^= 1;
                 }
                 if (final_c && plan.any_bend) {
-                    for (size_t c = 0; c < seg_first.size(); ++c)
-                        launch_bend_color(ctx, bend_segs_dev + seg_first[c], seg_count[c], parity, cnt);
+                    for (size_t c = 0; c < bends.first->size(); ++c)
+                        launch_bend_color(ctx, bends.dev + (*bends.first)[c], (*bends.count)[c], parity, cnt);
                     if (plan.any_bend_gather) {
                         launch_vertex_pass(ctx, parity, parity ^ 1, kOpBend | cnt);
                         parity ^= 1;
@@ -1277,9 +1354,10 @@ static bool step_one(Solver& s, int substeps, int iterations, int fetch, int for
         if (plan.any_dyn && !no_alloc && !s.tuner.poll(s.tuner.tuning())) return false;
         const bool tune = plan.any_dyn && !no_alloc && s.tuner.tuning();
         if (tune && !s.tuner.begin_step(substeps)) return false;
+        const ColorSegs bends{s.bend_segs.p, &s.bend_seg_first, &s.bend_seg_count};
+        const ColorSegs stretch_colors{s.stretch_segs.p, &s.stretch_seg_first, &s.stretch_seg_count};
         auto record = [&] {
-            return run_schedule(one, ctx, s.d_view.p, s.bend_segs.p, s.bend_seg_first, s.bend_seg_count, sb, substeps, iterations,
-                                count, s.tuner, tune);
+            return run_schedule(one, ctx, s.d_view.p, bends, stretch_colors, sb, substeps, iterations, count, s.tuner, tune);
         };
         std::vector<long long> key;
         const bool graphed = !no_alloc && graph_key(plan, substeps, iterations, count, s.tuner, &key);
@@ -1304,11 +1382,12 @@ static bool step_one(Solver& s, int substeps, int iterations, int fetch, int for
 // ----------------------------------------------------------------- batch
 struct Batch {
     std::vector<Solver*> members;
+    int stretch_mode = SSBL_EXT_STRETCH_JACOBI;   // shared by every member
     Dev<ClothView> d_views;
     Dev<Tile> d_tiles;
-    Dev<int4> d_bend_segs;
-    std::vector<int4> h_bend_segs;
-    std::vector<int> seg_first, seg_count;
+    Dev<int4> d_bend_segs, d_stretch_segs;
+    std::vector<int4> h_bend_segs, h_stretch_segs;
+    std::vector<int> seg_first, seg_count, sseg_first, sseg_count;
     std::vector<ClothView> h_views;
     Staging staging;
     int ntiles = 0;
@@ -1317,40 +1396,57 @@ struct Batch {
     PoolTuner tuner;
 };
 
+// Colour c of the batch: colour c's segments of every member, in member order,
+// with the member index in the segment's cloth field.
+static void merge_color_segs(const std::vector<Solver*>& members, std::vector<int4> Solver::*segs, std::vector<int> Solver::*first,
+                             std::vector<int> Solver::*count, std::vector<int4>& out, std::vector<int>& out_first,
+                             std::vector<int>& out_count) {
+    size_t colors = 0;
+    for (const Solver* s : members) colors = std::max(colors, (s->*first).size());
+    out.clear();
+    out_first.clear();
+    out_count.clear();
+    for (size_t c = 0; c < colors; ++c) {
+        out_first.push_back(static_cast<int>(out.size()));
+        for (size_t m = 0; m < members.size(); ++m) {
+            const Solver* s = members[m];
+            if (c >= (s->*first).size()) continue;
+            for (int k = 0; k < (s->*count)[c]; ++k) {
+                int4 seg = (s->*segs)[(s->*first)[c] + k];
+                seg.x = static_cast<int>(m);
+                out.push_back(seg);
+            }
+        }
+        out_count.push_back(static_cast<int>(out.size()) - out_first.back());
+    }
+}
+
 static bool build_batch(Batch& b) {
     std::vector<Tile> tiles;
-    int max_colors = 0;
+    b.stretch_mode = b.members[0]->stretch_mode;
     for (size_t m = 0; m < b.members.size(); ++m) {
         Solver* s = b.members[m];
         for (int t = 0; t < s->ntiles; ++t) tiles.push_back(Tile{static_cast<int>(m), t * kTile, std::min(s->nv, (t + 1) * kTile), 0});
-        max_colors = std::max(max_colors, static_cast<int>(s->bend_seg_first.size()));
-        if (s->stretch_mode != b.members[0]->stretch_mode) return fail("batch members must share the stretch mode");
+        if (s->stretch_mode != b.stretch_mode) return fail("batch members must share the stretch mode");
     }
-    b.h_bend_segs.clear();
-    b.seg_first.clear();
-    b.seg_count.clear();
-    for (int c = 0; c < max_colors; ++c) {
-        b.seg_first.push_back(static_cast<int>(b.h_bend_segs.size()));
-        for (size_t m = 0; m < b.members.size(); ++m) {
-            Solver* s = b.members[m];
-            if (c >= static_cast<int>(s->bend_seg_first.size())) continue;
-            for (int k = 0; k < s->bend_seg_count[c]; ++k) {
-                int4 seg = s->h_bend_segs[s->bend_seg_first[c] + k];
-                seg.x = static_cast<int>(m);
-                b.h_bend_segs.push_back(seg);
-            }
-        }
-        b.seg_count.push_back(static_cast<int>(b.h_bend_segs.size()) - b.seg_first.back());
-    }
+    merge_color_segs(b.members, &Solver::h_bend_segs, &Solver::bend_seg_first, &Solver::bend_seg_count, b.h_bend_segs, b.seg_first,
+                     b.seg_count);
+    if (b.stretch_mode == SSBL_EXT_STRETCH_COLORED)
+        merge_color_segs(b.members, &Solver::h_stretch_segs, &Solver::stretch_seg_first, &Solver::stretch_seg_count, b.h_stretch_segs,
+                         b.sseg_first, b.sseg_count);
     b.ntiles = static_cast<int>(tiles.size());
     b.h_views.resize(b.members.size());
     return b.d_views.alloc(b.members.size(), "batch view allocation") && b.d_tiles.upload(tiles, "batch tile allocation") &&
-           b.d_bend_segs.upload(b.h_bend_segs, "batch bend allocation");
+           b.d_bend_segs.upload(b.h_bend_segs, "batch bend allocation") &&
+           b.d_stretch_segs.upload(b.h_stretch_segs, "batch stretch allocation") &&
+           b.staging.reserve(sizeof(ClothView) * b.members.size());
 }
 
 static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     substeps = std::max(substeps, 1);
     iterations = std::max(iterations, 1);
+    for (const Solver* s : b.members)
+        if (s->stretch_mode != b.stretch_mode) return fail("a batch member changed its stretch mode; create the batch again");
     for (size_t m = 0; m < b.members.size(); ++m) {
         Solver& s = *b.members[m];
         reset_diag(s);
@@ -1364,9 +1460,10 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     if (plan.any_dyn && !b.tuner.poll(b.tuner.tuning())) return false;
     const bool tune = plan.any_dyn && b.tuner.tuning();
     if (tune && !b.tuner.begin_step(substeps)) return false;
+    const ColorSegs bends{b.d_bend_segs.p, &b.seg_first, &b.seg_count};
+    const ColorSegs stretch_colors{b.d_stretch_segs.p, &b.sseg_first, &b.sseg_count};
     auto record = [&] {
-        return run_schedule(b.members, ctx, b.d_views.p, b.d_bend_segs.p, b.seg_first, b.seg_count, sb, substeps, iterations, false,
-                            b.tuner, tune);
+        return run_schedule(b.members, ctx, b.d_views.p, bends, stretch_colors, sb, substeps, iterations, false, b.tuner, tune);
     };
     std::vector<long long> key;
     const bool graphed = graph_key(plan, substeps, iterations, false, b.tuner, &key);
@@ -1686,9 +1783,8 @@ SSBL_API int ssbl_ext_set_stretch_mode(void* handle, int mode) {
     g_error.clear();
     Solver* s = static_cast<Solver*>(handle);
     if (!s || (mode != SSBL_EXT_STRETCH_JACOBI && mode != SSBL_EXT_STRETCH_COLORED)) return fail("invalid stretch mode") ? 1 : 0;
-    if (mode == SSBL_EXT_STRETCH_COLORED) return fail("coloured stretch mode is not available in this build yet") ? 1 : 0;
-    s->stretch_mode = mode;
-    return 1;
+    ByteScope bytes(s);
+    return s->set_stretch_mode(mode) ? 1 : 0;
 }
 
 SSBL_API int ssbl_ext_download_state(void* handle, float* p, float* q, float* v, int max_floats) {

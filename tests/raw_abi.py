@@ -17,7 +17,7 @@ def _ip(a):
 
 
 class Raw:
-    def __init__(self, lib, positions, inv_mass, *, edges=None, edge_rest=None, bends=None, bend_rest=None,
+    def __init__(self, lib, positions, inv_mass, *, edges=None, edge_rest=None, edge_offsets=None, bends=None, bend_rest=None,
                  bend_offsets=None, triangles=None, static_triangles=None, **cfg_fields):
         self.lib = lib
         self.p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
@@ -33,6 +33,10 @@ class Raw:
             keep["er"] = np.ascontiguousarray(edge_rest, np.float32)
             cfg.edge_count = len(keep["e"])
             mesh.edges, mesh.edge_rest_lengths = _ip(keep["e"]), _fp(keep["er"])
+            if edge_offsets is not None:
+                keep["eo"] = np.ascontiguousarray(edge_offsets, np.int32)
+                cfg.edge_color_count = len(keep["eo"]) - 1
+                mesh.edge_color_offsets = _ip(keep["eo"])
         if bends is not None:
             keep["b"] = np.ascontiguousarray(bends, np.int32).reshape(-1, 2)
             keep["br"] = np.ascontiguousarray(bend_rest, np.float32)

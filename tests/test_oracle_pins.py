@@ -176,3 +176,48 @@ def test_reference_smoke_binary_on_oracle():
                    "SELF_COLLISION_OK", "SELF_VT_OK", "SELF_EE_OK"):
         assert f"SSBL_ABI41_{marker}" in out.stdout, out.stdout + out.stderr
     assert out.returncode == 0
+
+
+def _chain(lib, order, mode):
+    """Three unit-mass vertices on the x axis at 0, 1.1, 2.2 with two edges of
+    rest length 1 in colours 0 and 1 (`order` lists the edges in colour
+    order), no gravity, one substep of one iteration."""
+    from raw_abi import Raw
+    edges = np.array(order, np.int32)
+    r = Raw(lib, [[0, 0, 0], [1.1, 0, 0], [2.2, 0, 0]], [1, 1, 1], edges=edges, edge_rest=[1.0, 1.0], edge_offsets=[0, 1, 2],
+            gravity=(0.0, 0.0, 0.0))
+    r.check(lib.ssbl_ext_set_stretch_mode(r.h, mode))
+    r.step(1, 1)
+    return r.positions()[:, 0].astype(np.float64)
+
+
+def _edge(x, i, j):
+    """One edge's XPBD projection in 1D (unit masses, compliance 0): each end
+    moves 18 % of half the constraint error toward the rest length."""
+    c = (x[j] - x[i]) - 1.0
+    x[i] += 0.18 * c / 2.0
+    x[j] -= 0.18 * c / 2.0
+
+
+@pytest.mark.parametrize("order", [[[0, 1], [1, 2]], [[1, 2], [0, 1]]])
+def test_colored_stretch_is_gauss_seidel_in_colour_order(oracle, order):
+    x = np.array([0.0, 1.1, 2.2])
+    for i, j in order:                  # colour 0, then colour 1, each seeing the other's update
+        _edge(x, i, j)
+    got = _chain(oracle, order, 1)
+    assert np.allclose(got, x, atol=2e-7), (got, x)
+    jac = np.array([0.0, 1.1, 2.2])
+    jac[0] += 0.009; jac[2] -= 0.009    # Jacobi: both edges read the start, the middle vertex's pushes cancel
+    assert np.allclose(_chain(oracle, order, 0), jac, atol=2e-7)
+    assert abs(got[1] - 1.1) > 5e-4     # the two modes differ (GS: 1.10081, Jacobi: 1.1)
+
+
+def test_colored_mode_requires_a_valid_colouring(oracle):
+    from raw_abi import Raw
+    r = Raw(oracle, [[0, 0, 0], [1, 0, 0], [2, 0, 0]], [1, 1, 1], edges=np.array([[0, 1], [1, 2]], np.int32),
+            edge_rest=[1.0, 1.0], edge_offsets=[0, 2])            # vertex 1 twice in colour 0
+    assert oracle.ssbl_ext_set_stretch_mode(r.h, 1) == 0
+    assert b"valid edge colouring" in oracle.ssbl_last_error()
+    r2 = Raw(oracle, [[0, 0, 0], [1, 0, 0]], [1, 1], edges=np.array([[0, 1]], np.int32), edge_rest=[1.0])   # no colours
+    assert oracle.ssbl_ext_set_stretch_mode(r2.h, 1) == 0
+    assert oracle.ssbl_ext_set_stretch_mode(r2.h, 0) == 1
