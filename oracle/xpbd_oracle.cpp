@@ -655,6 +655,7 @@ template <class R> struct Solver {
     SsblXpbdConfig cfg{};
     SsblXpbdRuntimeColliders colliders{};
     int stretch_mode = SSBL_EXT_STRETCH_JACOBI;
+    int pcg_cadence = 1;   // SSBL_ABI41_PCG_SUBSTEP_CADENCE when set above 0 (A41:284-292)
     std::vector<Vec> pos, prev, vel, rest;
     std::vector<float> inv_mass;       // as uploaded
     std::vector<unsigned char> pinned; // inv_mass <= 0 (A41:8861-8865)
@@ -902,6 +903,12 @@ template <class R> bool Solver<R>::create(const SsblXpbdConfig* config, const Ss
         float v = std::strtof(raw, &end);
         return (end == raw || !std::isfinite(v)) ? def : v;
     };
+    // A41:266-292: an integer after strtol, used when above 0, default cadence 1
+    if (const char* raw = std::getenv("SSBL_ABI41_PCG_SUBSTEP_CADENCE")) {
+        char* end = nullptr;
+        const long v = std::strtol(raw, &end, 10);
+        pcg_cadence = (end != raw && v > 0 && v <= std::numeric_limits<int>::max()) ? static_cast<int>(v) : 1;
+    }
     pressure_accel_scale = std::max(1.0f, std::min(envf("SSBL_ABI41_PRESSURE_UI_ACCEL_SCALE", 1792.0f), 8192.0f));
     pressure_impulse_rest_scale = std::max(0.05f, std::min(envf("SSBL_ABI41_PRESSURE_IMPULSE_REST_SCALE", 0.85f), 8.0f));
     pressure_velocity_rest_scale = std::max(0.10f, std::min(envf("SSBL_ABI41_PRESSURE_VELOCITY_REST_SCALE", 2.25f), 16.0f));
@@ -2849,7 +2856,9 @@ template <class R> bool Solver<R>::step(int substeps, int iterations, int fetch)
                 if (stretch_mode == SSBL_EXT_STRETCH_COLORED) stretch_colored(sub_dt);
                 else stretch_jacobi(sub_dt);
             }
-            const bool run_pcg = pcg_on && last_it;   // substep cadence 1 (A41:284-292)
+            // A41:9158-9177: every cadence-th substep and always the last
+            const int cadence = std::max(1, std::min(pcg_cadence, substeps));
+            const bool run_pcg = pcg_on && last_it && (((s + 1) % cadence) == 0 || s == substeps - 1);
             if (run_pcg) {
                 pcg_run(sub_dt);
                 polish(sub_dt);

@@ -65,6 +65,30 @@ enum PassOps : unsigned {
 
 struct DynView;
 
+// Scalars of one cloth's hard-stretch PCG solve (device resident).
+struct PcgScalars {
+    double rz, dad, rznext, initial;
+    float beta;
+    int status, dir_ok, runs;
+};
+
+// One cloth's hard-stretch PCG arrays (null / zero when the cloth has none).
+struct PcgView {
+    int nv, ne, hard, ntiles;
+    float strength_cfg;   // configured strength (polish / cap thresholds)
+    float strength;       // clamp01(strength * 2.5), the system stiffness
+    float thick, wb_limit;
+    float4 *rhs, *sol, *res, *z, *dir, *adir;
+    float *diag, *pinv, *off;   // 6 floats per block
+    unsigned long long* partials;   // two 64-bit words per tile of the cloth
+    PcgScalars* sc;
+    unsigned long long* guard;
+    float* max_delta;
+    const int2* edges;
+    const float* edge_rest;
+    unsigned long long* counts;
+};
+
 // Everything a kernel needs to know about one cloth.  Lives in device memory
 // (a table of these for a batch); host keeps a mirror.
 struct ClothView {
@@ -153,6 +177,9 @@ struct ClothView {
     // Stretch edges {i, j, rest length bits, 0} in input (colour-sorted)
     // order, for the coloured Gauss-Seidel stretch mode (null otherwise).
     const int4* sedges;
+    // The hard-stretch solve's arrays: the cloth's own, or the static
+    // scratch of the allocation-free step.
+    PcgView pcg;
 };
 
 struct Tile { int cloth, begin, end, pad; };

@@ -15,6 +15,11 @@
 // term becomes a 2^-96 fixed-point __int128, partial sums per workgroup, one
 // final 128-bit sum rounded to double -- the result does not depend on the
 // order of the additions.
+//
+// Batching: every kernel reads the cloth it works on from the step's view
+// table (the tile's or segment's cloth index), so one launch covers all the
+// cloths of a group; a cloth's workgroup partials sit at its own tile
+// indices, and the reduction runs one workgroup per cloth.
 #include "xpbd_pcg.hpp"
 
 #include "xpbd_math.hpp"
@@ -86,8 +91,8 @@ __device__ inline Sym sym_inverse(Sym m, unsigned long long* guard) {
 
 __device__ __forceinline__ __int128 to_fixed(float t) { return static_cast<__int128>(rint(static_cast<double>(t) * kFix)); }
 
-// Workgroup sum of one fixed-point term per thread -> partials[blockIdx.x].
-__device__ inline void block_partial(__int128 v, unsigned long long* partials) {
+// Workgroup sum of one fixed-point term per thread -> partials[slot].
+__device__ inline void block_partial(__int128 v, unsigned long long* partials, int slot) {
     __shared__ unsigned long long lo[kTile], hi[kTile];
     lo[threadIdx.x] = static_cast<unsigned long long>(v);
     hi[threadIdx.x] = static_cast<unsigned long long>(v >> 64);
@@ -103,8 +108,8 @@ __device__ inline void block_partial(__int128 v, unsigned long long* partials) {
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = lo[0];
-        partials[2 * blockIdx.x + 1] = hi[0];
+        partials[2 * slot] = lo[0];
+        partials[2 * slot + 1] = hi[0];
     }
 }
 
@@ -119,13 +124,14 @@ __device__ __forceinline__ float4* cur(const ClothView& c, int parity) { return 
 
 // A41:3032-3069 reset + A41:3085-3186 system build (gathered per vertex in
 // ascending edge order) + A41:3188-3226 preconditioner.
-__global__ __launch_bounds__(kTile) void k_pcg_build(const ClothView* __restrict__ cv, const Tile* __restrict__ tiles, int ntiles,
-                                                     int parity, PcgView pv) {
+__global__ __launch_bounds__(kTile) void k_pcg_build(const ClothView* __restrict__ views, const Tile* __restrict__ tiles, int ntiles,
+                                                     int parity) {
     const Tile tile = tiles[tile_of_block(ntiles)];
+    const ClothView& c = views[tile.cloth];
+    const PcgView& pv = c.pcg;
     const int v = tile.begin + static_cast<int>(threadIdx.x);
     __int128 term = 0;
     if (v < tile.end) {
-        const ClothView& c = *cv;
         const float4* P = cur(c, parity);
         const float4 p4 = P[v];
         const float w = p4.w;
@@ -191,21 +197,26 @@ __global__ __launch_bounds__(kTile) void k_pcg_build(const ClothView* __restrict
             term = to_fixed(dot(rhs, z));
         }
     }
-    block_partial(term, pv.partials);
+    block_partial(term, pv.partials, tile.begin / kTile);
 }
 
-// Final sum of the workgroup partials.  mode 0: initial rz; 1: dAd;
-// 2: rz_next, then the search-direction factor and the advance
-// (A41:3421-3464) when another iteration follows.
-__global__ __launch_bounds__(1024) void k_pcg_reduce(PcgView pv, int nparts, int mode, int more) {
-    __shared__ unsigned long long lo[1024], hi[1024];
+// Final sum of one cloth's workgroup partials, a workgroup per cloth of the
+// group (members[blockIdx.x]).  mode 0: initial rz; 1: dAd; 2: rz_next, then
+// the search-direction factor and the advance (A41:3421-3464) when another
+// iteration follows.
+constexpr int kReduceBlock = 256;
+__global__ __launch_bounds__(kReduceBlock) void k_pcg_reduce(const ClothView* __restrict__ views, const int* __restrict__ members,
+                                                             int mode, int more) {
+    const PcgView& pv = views[members[blockIdx.x]].pcg;
+    const int nparts = pv.ntiles;
+    __shared__ unsigned long long lo[kReduceBlock], hi[kReduceBlock];
     __int128 acc = 0;
-    for (int i = static_cast<int>(threadIdx.x); i < nparts; i += 1024)
+    for (int i = static_cast<int>(threadIdx.x); i < nparts; i += kReduceBlock)
         acc += (static_cast<__int128>(pv.partials[2 * i + 1]) << 64) | pv.partials[2 * i];
     lo[threadIdx.x] = static_cast<unsigned long long>(acc);
     hi[threadIdx.x] = static_cast<unsigned long long>(acc >> 64);
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
+    for (int s = kReduceBlock / 2; s > 0; s >>= 1) {
         if (static_cast<int>(threadIdx.x) < s) {
             const __int128 a = (static_cast<__int128>(hi[threadIdx.x]) << 64) | lo[threadIdx.x];
             const __int128 b = (static_cast<__int128>(hi[threadIdx.x + s]) << 64) | lo[threadIdx.x + s];
@@ -236,14 +247,15 @@ __global__ __launch_bounds__(1024) void k_pcg_reduce(PcgView pv, int nparts, int
 }
 
 // A41:3228-3267: A d (block diagonal + CSR off-diagonal blocks).
-__global__ __launch_bounds__(kTile) void k_pcg_ad(const ClothView* __restrict__ cv, const Tile* __restrict__ tiles, int ntiles,
-                                                  int parity, PcgView pv) {
+__global__ __launch_bounds__(kTile) void k_pcg_ad(const ClothView* __restrict__ views, const Tile* __restrict__ tiles, int ntiles,
+                                                  int parity) {
     const Tile tile = tiles[tile_of_block(ntiles)];
+    const ClothView& c = views[tile.cloth];
+    const PcgView& pv = c.pcg;
     const int v = tile.begin + static_cast<int>(threadIdx.x);
     __int128 term = 0;
     const bool ok = pv.sc->status == 0;
     if (v < tile.end && ok) {
-        const ClothView& c = *cv;
         if (cur(c, parity)[v].w == 0.0f) {
             pv.adir[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         } else {
@@ -258,13 +270,15 @@ __global__ __launch_bounds__(kTile) void k_pcg_ad(const ClothView* __restrict__ 
             term = to_fixed(dot(d, q));
         }
     }
-    block_partial(term, pv.partials);
+    block_partial(term, pv.partials, tile.begin / kTile);
 }
 
 // A41:3320-3405 (device alpha) with the write-back of the step to the positions.
-__global__ __launch_bounds__(kTile) void k_pcg_update(const ClothView* __restrict__ cv, const Tile* __restrict__ tiles, int ntiles,
-                                                      int parity, PcgView pv) {
+__global__ __launch_bounds__(kTile) void k_pcg_update(const ClothView* __restrict__ views, const Tile* __restrict__ tiles, int ntiles,
+                                                      int parity) {
     const Tile tile = tiles[tile_of_block(ntiles)];
+    const ClothView& c = views[tile.cloth];
+    const PcgView& pv = c.pcg;
     const int v = tile.begin + static_cast<int>(threadIdx.x);
     const PcgScalars& s = *pv.sc;
     int status = 0;
@@ -278,7 +292,6 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const ClothView* __restric
     }
     __int128 term = 0;
     if (status == 0 && v < tile.end) {
-        const ClothView& c = *cv;
         float4* P = cur(c, parity);
         const float4 p4 = P[v];
         const F3 zero = f3(0.0f, 0.0f, 0.0f);
@@ -308,21 +321,24 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const ClothView* __restric
             if (dl > 0.0f) atomicMax(reinterpret_cast<int*>(pv.max_delta), __float_as_int(dl));
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // every workgroup of the cloth derives the same status; its first tile's records it
+    if (tile.begin == 0 && threadIdx.x == 0) {
         pv.sc->status = status;
         if (status != 0 && status != 2) atomicAdd(pv.guard, 1ull);
     }
-    block_partial(term, pv.partials);
+    block_partial(term, pv.partials, tile.begin / kTile);
 }
 
 // A41:3421-3450 with the factor computed by k_pcg_reduce.
-__global__ __launch_bounds__(kTile) void k_pcg_dir(const ClothView* __restrict__ cv, const Tile* __restrict__ tiles, int ntiles,
-                                                   int parity, PcgView pv) {
-    if (!pv.sc->dir_ok) return;
+__global__ __launch_bounds__(kTile) void k_pcg_dir(const ClothView* __restrict__ views, const Tile* __restrict__ tiles, int ntiles,
+                                                   int parity) {
     const Tile tile = tiles[tile_of_block(ntiles)];
+    const ClothView& c = views[tile.cloth];
+    const PcgView& pv = c.pcg;
+    if (!pv.sc->dir_ok) return;
     const int v = tile.begin + static_cast<int>(threadIdx.x);
     if (v >= tile.end) return;
-    if (cur(*cv, parity)[v].w == 0.0f) { pv.dir[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); return; }
+    if (cur(c, parity)[v].w == 0.0f) { pv.dir[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); return; }
     const float beta = pv.sc->beta;
     const F3 d = f3(pv.dir[v]), z = f3(pv.z[v]);
     pv.dir[v] = with_w(f3(__builtin_fmaf(d.x, beta, z.x), __builtin_fmaf(d.y, beta, z.y), __builtin_fmaf(d.z, beta, z.z)), 0.0f);
@@ -402,12 +418,14 @@ __device__ inline bool edge_kind(int kind, F3 pi, F3 pj, F3 qi, F3 qj, float wi,
 // Jacobi gather of one kind over every incident edge (src -> dst): caps add
 // each edge's result in ascending edge order; the polish / tiny-edge kinds
 // average their contributions (A41:2082-2112, 2468-2494).
-__global__ __launch_bounds__(kTile) void k_edge_gather(const ClothView* __restrict__ cv, const Tile* __restrict__ tiles, int ntiles,
-                                                       int src_parity, int kind, float dt, PcgView pv) {
+__global__ __launch_bounds__(kTile) void k_edge_gather(const ClothView* __restrict__ views, const Tile* __restrict__ tiles, int ntiles,
+                                                       int src_parity, int kind) {
     const Tile tile = tiles[tile_of_block(ntiles)];
     const int v = tile.begin + static_cast<int>(threadIdx.x);
     if (v >= tile.end) return;
-    const ClothView& c = *cv;
+    const ClothView& c = views[tile.cloth];
+    const PcgView& pv = c.pcg;
+    const float dt = c.dt_sub;
     const float4* S = src_parity ? c.pos1 : c.pos0;
     float4* D = src_parity ? c.pos0 : c.pos1;
     const float4 p4 = S[v];
@@ -447,12 +465,14 @@ __global__ __launch_bounds__(kTile) void k_edge_gather(const ClothView* __restri
 
 // One colour of a cap kind, one thread per edge, in place (race free for a
 // valid colouring).  A segment is {cloth, first edge, end edge}.
-__global__ __launch_bounds__(kTile) void k_edge_color(const ClothView* __restrict__ cv, const int4* __restrict__ segs, int parity,
-                                                      int kind, float dt, PcgView pv) {
-    const int4 seg = segs[blockIdx.x];
+__global__ __launch_bounds__(kTile) void k_edge_color(const ClothView* __restrict__ views, const int4* __restrict__ segs, int nsegs,
+                                                      int parity, int kind) {
+    const int4 seg = segs[tile_of_block(nsegs)];
     const int k = seg.y + static_cast<int>(threadIdx.x);
     if (k >= seg.z) return;
-    const ClothView& c = *cv;
+    const ClothView& c = views[seg.x];
+    const PcgView& pv = c.pcg;
+    const float dt = c.dt_sub;
     float4* P = cur(c, parity);
     const int2 ed = pv.edges[k];
     const int i = ed.x, j = ed.y;
@@ -465,6 +485,16 @@ __global__ __launch_bounds__(kTile) void k_edge_color(const ClothView* __restric
     const bool aj = edge_kind(kind, f3(pj4), f3(pi4), qj, qi, pj4.w, pi4.w, rest, pv.thick, pv.strength_cfg, dt, &oj) && oj.apply;
     if (ai) { P[i] = with_w(f3(pi4) + oi.d, pi4.w); c.prev[i] = with_w(qi + oi.q, 0.0f); }
     if (aj) { P[j] = with_w(f3(pj4) + oj.d, pj4.w); c.prev[j] = with_w(qj + oj.q, 0.0f); }
+}
+
+// Zeroes the scalars, guard and max-delta of every cloth of the group.
+__global__ void k_pcg_reset(const ClothView* __restrict__ views, const int* __restrict__ members, int n) {
+    const int m = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (m >= n) return;
+    const PcgView& pv = views[members[m]].pcg;
+    *pv.sc = PcgScalars{0.0, 0.0, 0.0, 0.0, 0.0f, 0, 0, 0};
+    *pv.guard = 0ull;
+    *pv.max_delta = 0.0f;
 }
 
 }  // namespace
@@ -494,6 +524,7 @@ bool PcgPath::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, 
     v.nv = nv;
     v.ne = ne;
     v.strength_cfg = cfg.stretch_optimization_strength;
+    v.ntiles = ntiles;
     v.strength = std::min(std::max(cfg.stretch_optimization_strength * 2.5f, 0.0f), 1.0f);
     v.thick = cfg.cloth_thickness;
     v.hard = (cfg.stretch_optimization_strength >= 0.95f && std::isfinite(cfg.cloth_thickness) && cfg.cloth_thickness > 1.0e-7f) ? 1 : 0;
@@ -522,34 +553,31 @@ bool PcgPath::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, 
         !hip_ok(hipMemset(v.max_delta, 0, 4), "PCG reset"))
         return false;
     // colour segments for the coloured cap passes; a colouring that puts a
-    // vertex twice in one colour cannot use the race-free per-colour form
+    // vertex twice in one colour cannot use the race-free per-colour form.
+    // Without colours the caps are one Jacobi gather over every edge.
     colored_ = !colors.empty();
+    segs_host_.clear();
+    seg_first_.clear();
+    seg_count_.clear();
     if (colored_) {
         std::vector<int> stamp(nv, -1);
         const int ncol = static_cast<int>(colors.size()) - 1;
-        for (int c = 0; c < ncol && colored_; ++c) {
+        for (int c = 0; c < ncol; ++c) {
             const int lo = std::max(colors[c], 0), hi = std::min(colors[c + 1], ne);
             for (int e = lo; e < hi; ++e) {
                 const int i = edges[2 * e], j = edges[2 * e + 1];
                 if (i >= nv || j >= nv) continue;
-                if (stamp[i] == c || stamp[j] == c) { colored_ = false; break; }
+                if (stamp[i] == c || stamp[j] == c)
+                    return fail("the hard-stretch caps need a valid edge colouring (no vertex twice in one colour)");
                 stamp[i] = c; stamp[j] = c;
             }
         }
-        if (!colored_) return fail("the hard-stretch caps need a valid edge colouring (no vertex twice in one colour)");
-        std::vector<int4> segs;
-        seg_first_.clear();
-        seg_count_.clear();
         for (int c = 0; c < ncol; ++c) {
             const int lo = std::max(colors[c], 0), hi = std::min(colors[c + 1], ne);
-            seg_first_.push_back(static_cast<int>(segs.size()));
-            for (int e = lo; e < hi; e += kTile) segs.push_back(make_int4(0, e, std::min(hi, e + kTile), 0));
-            seg_count_.push_back(static_cast<int>(segs.size()) - seg_first_.back());
+            seg_first_.push_back(static_cast<int>(segs_host_.size()));
+            for (int e = lo; e < hi; e += kTile) segs_host_.push_back(make_int4(0, e, std::min(hi, e + kTile), 0));
+            seg_count_.push_back(static_cast<int>(segs_host_.size()) - seg_first_.back());
         }
-        segs_ = static_cast<int4*>(alloc_bytes(sizeof(int4) * std::max<size_t>(segs.size(), 1)));
-        if (!segs_) return false;
-        if (!segs.empty() && !hip_ok(hipMemcpy(segs_, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice), "PCG colours"))
-            return false;
     }
     own_ = pv_;
     ready_ = true;
@@ -597,75 +625,142 @@ bool PcgPath::use_scratch(void* base, size_t bytes) {
     return true;
 }
 
-// Clears the scalars and the guard / max-delta counters with fill launches
-// (no memset nodes, so a caller may capture the step).
-bool PcgPath::reset_diagnostics(hipStream_t st) {
-    if (!ready_) return true;
-    static_assert(sizeof(PcgScalars) % 4 == 0, "PcgScalars is filled as 32-bit words");
-    launch_fill_u32(st, reinterpret_cast<unsigned*>(pv_.sc), 0u, sizeof(PcgScalars) / 4);
-    launch_fill_u32(st, reinterpret_cast<unsigned*>(pv_.guard), 0u, 2);
-    launch_fill_u32(st, reinterpret_cast<unsigned*>(pv_.max_delta), 0u, 1);
-    return hip_ok(hipGetLastError(), "PCG reset");
+// ------------------------------------------------------------- groups
+template <class T> bool PcgGroup::Buf<T>::upload(const std::vector<T>& h, const char* what) {
+    device_free(p, sizeof(T) * n);
+    p = nullptr;
+    n = 0;
+    if (h.empty()) return true;
+    p = static_cast<T*>(device_alloc(sizeof(T) * h.size(), what));
+    if (!p) return false;
+    n = h.size();
+    return hip_ok(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice), what);
 }
 
-void PcgPath::solve(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity) {
+bool PcgGroup::build(const std::vector<PcgMember>& members, int cadence) {
+    cadence_ = std::max(cadence, 1);
+    std::vector<Tile> tiles, hard_c, hard_u;
+    std::vector<int> idx;
+    std::vector<int4> segs;
+    size_t colors = 0;
+    for (const PcgMember& m : members) {
+        idx.push_back(m.index);
+        const bool polishes = m.path->polishes();
+        for (int t = 0; t * kTile < m.nv; ++t) {
+            const Tile tile{m.index, t * kTile, std::min(m.nv, (t + 1) * kTile), 0};
+            tiles.push_back(tile);
+            if (polishes) (m.path->colored() ? hard_c : hard_u).push_back(tile);
+        }
+        if (polishes && m.path->colored()) colors = std::max(colors, m.path->cap_first().size());
+    }
+    // colour c of the caps: colour c's segments of every coloured cloth that polishes
+    cap_first_.clear();
+    cap_count_.clear();
+    for (size_t c = 0; c < colors; ++c) {
+        cap_first_.push_back(static_cast<int>(segs.size()));
+        for (const PcgMember& m : members) {
+            if (!m.path->polishes() || !m.path->colored() || c >= m.path->cap_first().size()) continue;
+            for (int k = 0; k < m.path->cap_count()[c]; ++k) {
+                int4 seg = m.path->cap_segments()[m.path->cap_first()[c] + k];
+                seg.x = m.index;
+                segs.push_back(seg);
+            }
+        }
+        cap_count_.push_back(static_cast<int>(segs.size()) - cap_first_.back());
+    }
+    ntiles_ = static_cast<int>(tiles.size());
+    nmembers_ = static_cast<int>(idx.size());
+    colored_.colored = true;
+    colored_.ntiles = static_cast<int>(hard_c.size());
+    plain_.colored = false;
+    plain_.ntiles = static_cast<int>(hard_u.size());
+    return tiles_.upload(tiles, "PCG tile table") && colored_.tiles.upload(hard_c, "PCG tile table") &&
+           plain_.tiles.upload(hard_u, "PCG tile table") && members_.upload(idx, "PCG member table") &&
+           cap_segs_.upload(segs, "PCG cap segments");
+}
+
+void PcgGroup::reset(hipStream_t st, const ClothView* views) const {
+    if (nmembers_ == 0) return;
     TimedLaunch t("pcg", st);
-    const dim3 grid(ntiles_), block(kTile);
-    launch_k(k_pcg_build, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
-    launch_k(k_pcg_reduce, dim3(1), dim3(1024), 0, st, pv_, ntiles_, 0, 0);
+    launch_k(k_pcg_reset, dim3((nmembers_ + 63) / 64), dim3(64), 0, st, views, members_.p, nmembers_);
+}
+
+void PcgGroup::solve(hipStream_t st, const ClothView* views, int parity) const {
+    if (ntiles_ == 0) return;
+    TimedLaunch t("pcg", st);
+    const dim3 grid(ntiles_), block(kTile), rgrid(nmembers_), rblock(kReduceBlock);
+    launch_k(k_pcg_build, grid, block, 0, st, views, tiles_.p, ntiles_, parity);
+    launch_k(k_pcg_reduce, rgrid, rblock, 0, st, views, members_.p, 0, 0);
     for (int it = 0; it < kPcgIterations; ++it) {
         const int more = it + 1 < kPcgIterations ? 1 : 0;
-        launch_k(k_pcg_ad, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
-        launch_k(k_pcg_reduce, dim3(1), dim3(1024), 0, st, pv_, ntiles_, 1, more);
-        launch_k(k_pcg_update, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
-        launch_k(k_pcg_reduce, dim3(1), dim3(1024), 0, st, pv_, ntiles_, 2, more);
-        if (more) launch_k(k_pcg_dir, grid, block, 0, st, cv, tiles, ntiles_, parity, pv_);
+        launch_k(k_pcg_ad, grid, block, 0, st, views, tiles_.p, ntiles_, parity);
+        launch_k(k_pcg_reduce, rgrid, rblock, 0, st, views, members_.p, 1, more);
+        launch_k(k_pcg_update, grid, block, 0, st, views, tiles_.p, ntiles_, parity);
+        launch_k(k_pcg_reduce, rgrid, rblock, 0, st, views, members_.p, 2, more);
+        if (more) launch_k(k_pcg_dir, grid, block, 0, st, views, tiles_.p, ntiles_, parity);
     }
 }
 
-// Returns the new parity (gathers flip it).
-int PcgPath::gather(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity, int kind, float dt) {
+// A gather over one polishing set's tiles (src -> dst); returns the new parity.
+int PcgGroup::gather(hipStream_t st, const ClothView* views, const Polishing& set, int parity, int kind) const {
     TimedLaunch t("edge_gather", st);
-    launch_k(k_edge_gather, dim3(ntiles_), dim3(kTile), 0, st, cv, tiles, ntiles_, parity, kind, dt, pv_);
+    launch_k(k_edge_gather, dim3(set.ntiles), dim3(kTile), 0, st, views, set.tiles.p, set.ntiles, parity, kind);
     return parity ^ 1;
 }
 
-int PcgPath::caps(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity, int kind, int passes, bool reverse_too, float dt) {
+// Caps of one kind, `passes` times: for coloured cloths each colour in place,
+// forward and, when asked, backward (A41:2249-2362, 2496-2708); for the
+// others one Jacobi gather over every edge per pass.  Returns the new parity.
+int PcgGroup::caps(hipStream_t st, const ClothView* views, const Polishing& set, int parity, int kind, int passes,
+                   bool reverse_too) const {
+    if (!set.colored) {
+        for (int p = 0; p < passes; ++p) parity = gather(st, views, set, parity, kind);
+        return parity;
+    }
+    TimedLaunch t("edge_color", st);
+    const int ncol = static_cast<int>(cap_first_.size());
     for (int p = 0; p < passes; ++p) {
-        if (!colored_) { parity = gather(st, cv, tiles, parity, kind, dt); continue; }
-        TimedLaunch t("edge_color", st);
-        const int ncol = static_cast<int>(seg_first_.size());
         for (int c = 0; c < ncol; ++c)
-            if (seg_count_[c] > 0)
-                launch_k(k_edge_color, dim3(seg_count_[c]), dim3(kTile), 0, st, cv, segs_ + seg_first_[c], parity, kind, dt, pv_);
+            if (cap_count_[c] > 0)
+                launch_k(k_edge_color, dim3(cap_count_[c]), dim3(kTile), 0, st, views, cap_segs_.p + cap_first_[c], cap_count_[c],
+                         parity, kind);
         if (reverse_too)
             for (int c = ncol - 1; c >= 0; --c)
-                if (seg_count_[c] > 0)
-                    launch_k(k_edge_color, dim3(seg_count_[c]), dim3(kTile), 0, st, cv, segs_ + seg_first_[c], parity, kind, dt, pv_);
+                if (cap_count_[c] > 0)
+                    launch_k(k_edge_color, dim3(cap_count_[c]), dim3(kTile), 0, st, views, cap_segs_.p + cap_first_[c], cap_count_[c],
+                             parity, kind);
     }
     return parity;
 }
 
-// A41:7823-7857 polish sequence, then the extreme caps (A41:7974-7995).
-int PcgPath::polish(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity, float dt) {
-    if (pv_.strength_cfg < 0.95f || pv_.ne <= 0) return parity;
-    for (int p = 0; p < 2; ++p) {
-        parity = gather(st, cv, tiles, parity, 3, dt);
-        parity = caps(st, cv, tiles, parity, 0, 2, true, dt);
-        for (int t = 0; t < 2; ++t) {
-            parity = gather(st, cv, tiles, parity, 4, dt);
-            parity = caps(st, cv, tiles, parity, 1, 3, false, dt);
+// A41:7823-7857 polish sequence, then the extreme caps (A41:7974-7995).  The
+// gathers flip the polishing cloths' buffers; an odd count is copied back.
+void PcgGroup::polish(hipStream_t st, const ClothView* views, int parity) const {
+    for (const Polishing* set : {&colored_, &plain_}) {
+        if (set->ntiles == 0) continue;
+        int p = parity;
+        for (int k = 0; k < 2; ++k) {
+            p = gather(st, views, *set, p, 3);
+            p = caps(st, views, *set, p, 0, 2, true);
+            for (int t = 0; t < 2; ++t) {
+                p = gather(st, views, *set, p, 4);
+                p = caps(st, views, *set, p, 1, 3, false);
+            }
         }
+        p = caps(st, views, *set, p, 2, 3, false);
+        if (p != parity) gather(st, views, *set, p, -1);
     }
-    return caps(st, cv, tiles, parity, 2, 3, false, dt);
 }
 
 // A41:7997-8027
-int PcgPath::final_caps(hipStream_t st, const ClothView* cv, const Tile* tiles, int parity, float dt) {
-    if (pv_.strength_cfg < 0.95f || pv_.ne <= 0) return parity;
-    parity = caps(st, cv, tiles, parity, 0, 2, true, dt);
-    parity = caps(st, cv, tiles, parity, 2, 3, false, dt);
-    return caps(st, cv, tiles, parity, 1, 3, false, dt);
+void PcgGroup::final_caps(hipStream_t st, const ClothView* views, int parity) const {
+    for (const Polishing* set : {&colored_, &plain_}) {
+        if (set->ntiles == 0) continue;
+        int p = caps(st, views, *set, parity, 0, 2, true);
+        p = caps(st, views, *set, p, 2, 3, false);
+        p = caps(st, views, *set, p, 1, 3, false);
+        if (p != parity) gather(st, views, *set, p, -1);
+    }
 }
 
 bool PcgPath::fetch(SsblXpbdDiagnostics* d) const {

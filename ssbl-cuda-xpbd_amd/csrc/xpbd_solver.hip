@@ -376,6 +376,7 @@ struct Solver {
     SsblXpbdConfig cfg{};
     SsblXpbdRuntimeColliders colliders{};
     int stretch_mode = SSBL_EXT_STRETCH_JACOBI;
+    int pcg_cadence = 1;   // SSBL_ABI41_PCG_SUBSTEP_CADENCE when set above 0 (A41:284-292)
     int nv = 0;
     // host mirrors needed after create
     std::vector<float4> h_rest;          // w = inverse mass as uploaded
@@ -422,6 +423,7 @@ struct Solver {
     Staging staging;
     SelfCollision self;
     PcgPath pcg;
+    PcgGroup pcg_group;   // the solve's launches over this cloth alone
     DynamicColliders dyn;              // the cloth's own dynamic colliders
     GlobalScene* scene = nullptr;      // attached global scene
     int target_src = 0, target_layer = 0, cross_mode = 0;
@@ -581,6 +583,13 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     pressure_accel_scale = std::max(1.0f, std::min(env_float("SSBL_ABI41_PRESSURE_UI_ACCEL_SCALE", 1792.0f), 8192.0f));
     pressure_impulse_rest_scale = std::max(0.05f, std::min(env_float("SSBL_ABI41_PRESSURE_IMPULSE_REST_SCALE", 0.85f), 8.0f));
     pressure_velocity_rest_scale = std::max(0.10f, std::min(env_float("SSBL_ABI41_PRESSURE_VELOCITY_REST_SCALE", 2.25f), 16.0f));
+    // A41:266-292: an integer after strtol, used when above 0, default cadence 1
+    pcg_cadence = 1;
+    if (const char* raw = std::getenv("SSBL_ABI41_PCG_SUBSTEP_CADENCE")) {
+        char* end = nullptr;
+        const long v = std::strtol(raw, &end, 10);
+        if (end != raw && v > 0 && v <= std::numeric_limits<int>::max()) pcg_cadence = static_cast<int>(v);
+    }
     colliders.use_ground = cfg.use_ground;
     colliders.ground_height = cfg.ground_height;
     colliders.use_wall = cfg.use_wall;
@@ -791,6 +800,7 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     if (cfg.self_collision && !self.prepare(cfg, edges, tri, n, counts.p)) return false;
     if (cfg.stretch_optimization_enabled && !pcg.prepare(cfg, edges, edge_rest, edge_colors, n, h_adj_off[n], ntiles, counts.p))
         return false;
+    if (pcg.enabled() && !pcg_group.build({PcgMember{0, &pcg, n}}, pcg_cadence)) return false;
     // A41:7193-7302: bending "wings" (an edge met again with another opposite
     // vertex); the reference builds them for its diagnostics only.
     wing_count = 0;
@@ -1062,6 +1072,7 @@ void Solver::fill_view(int substeps) {
     v.dyn_target_src = target_src;
     v.dyn_target_layer = target_layer;
     v.sedges = colored_stretch() ? sedges.p : nullptr;
+    v.pcg = pcg.enabled() ? pcg.view() : PcgView{};
 }
 
 // ------------------------------------------------------------ schedule
@@ -1097,10 +1108,14 @@ static StepPlan plan_of(const std::vector<Solver*>& members) {
 // solve (whose launches depend on more host state and are never captured).
 static bool graph_key(const StepPlan& plan, int substeps, int iterations, bool count, const PoolTuner& tuner,
                       std::vector<long long>* key) {
-    if (g_graph_mode.load(std::memory_order_relaxed) == 0 || timing_enabled() || plan.any_self || plan.any_pcg) return false;
+    if (g_graph_mode.load(std::memory_order_relaxed) == 0 || timing_enabled()) return false;
     if (plan.any_dyn && tuner.tuning()) return false;
+    // the self-collision and hard-stretch launches are functions of the
+    // members' fixed set-up (intervals, passes, cadences, tile tables) and of
+    // the flags below
     *key = {substeps, iterations, count ? 1 : 0, plan.any_pins, plan.any_edges, plan.any_lra, plan.any_bend,
-            plan.any_bend_gather, plan.any_forces, plan.any_dyn, plan.any_signed, plan.colored, tuner.value()};
+            plan.any_bend_gather, plan.any_forces, plan.any_dyn, plan.any_signed, plan.colored, plan.any_self, plan.any_pcg,
+            tuner.value()};
     return true;
 }
 
@@ -1112,8 +1127,8 @@ struct ColorSegs {
 };
 
 static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, const ClothView* dviews, const ColorSegs& bends,
-                         const ColorSegs& stretch_colors, SelfBatch* self_batch, int substeps, int iterations, bool count,
-                         PoolTuner& tuner, bool tune) {
+                         const ColorSegs& stretch_colors, const std::vector<const PcgGroup*>& pcg_groups, SelfBatch* self_batch,
+                         int substeps, int iterations, bool count, PoolTuner& tuner, bool tune) {
     const StepPlan plan = plan_of(members);
     const unsigned cnt = count ? kOpCount : 0u;
     const unsigned pin_ops = plan.any_pins ? (kOpPin | kOpPinPolish) : 0u;
@@ -1133,16 +1148,9 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
         launch_vertex_pass(ctx, parity, parity ^ 1, stretch | ops);
         parity ^= 1;
     };
-    // the hard-stretch passes run per member; gathers flip a member's buffers,
-    // so each member is realigned to the common parity afterwards
-    auto per_pcg = [&](auto&& fn) {
-        for (size_t m = 0; m < members.size(); ++m) {
-            Solver* s = members[m];
-            if (!s->pcg.enabled()) continue;
-            const int got = fn(*s, dviews + m);
-            if (got != parity) s->pcg.copy(st, dviews + m, s->d_tiles.p, got);
-        }
-    };
+    // the hard-stretch scalars of the step start at zero (A41:7586-7624)
+    if (plan.any_pcg)
+        for (const PcgGroup* g : pcg_groups) g->reset(st, dviews);
     bool integrated = false;   // the previous substep's last launch integrated this one
     for (int sub = 0; sub < substeps; ++sub) {
         if (!integrated) {
@@ -1163,7 +1171,11 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
                     if (m->cfg.self_collision && (((sub + 1) % interval) == 0 || sub == substeps - 1)) self_now = true;
                 }
             }
-            const bool run_pcg = plan.any_pcg && last;   // substep cadence 1 (A41:284-292)
+            // A41:9158-9177: each group's solve runs on every cadence-th
+            // substep and always on the last
+            bool run_pcg = false;
+            if (plan.any_pcg && last)
+                for (const PcgGroup* g : pcg_groups) run_pcg |= g->due(sub, substeps);
             const bool split = run_pcg || self_now || dyn_now || signed_now || (final_c && (plan.any_lra || plan.any_bend));
             if (!split) {
                 stretch_then(pin_ops | kOpAnalytic | kOpSdf | last_op | (last ? kOpVelocity : 0u) | cnt);
@@ -1182,10 +1194,11 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
             } else {
                 stretch_then(cnt);
                 if (run_pcg) {
-                    per_pcg([&](Solver& s, const ClothView* cv) {
-                        s.pcg.solve(st, cv, s.d_tiles.p, parity);
-                        return s.pcg.polish(st, cv, s.d_tiles.p, parity, s.view.dt_sub);
-                    });
+                    for (const PcgGroup* g : pcg_groups) {
+                        if (!g->due(sub, substeps)) continue;
+                        g->solve(st, dviews, parity);
+                        g->polish(st, dviews, parity);
+                    }
                 }
                 if (final_c && plan.any_lra) {
                     launch_vertex_pass(ctx, parity, parity ^ 1, kOpLra | cnt);
@@ -1222,9 +1235,8 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
                 if (!self_batch || !self_batch->run(ctx, parity, sub, substeps, count)) return false;
             }
             if (signed_now) launch_signed_pass(ctx, parity, false, cnt);
-            if (run_pcg && final_c) {
-                per_pcg([&](Solver& s, const ClothView* cv) { return s.pcg.final_caps(st, cv, s.d_tiles.p, parity, s.view.dt_sub); });
-            }
+            if (run_pcg && final_c)
+                for (const PcgGroup* g : pcg_groups) g->final_caps(st, dviews, parity);
             if (signed_now && final_c) launch_signed_pass(ctx, parity, true, cnt);
             if (tail && !tail_fused) launch_vertex_pass(ctx, parity, parity, tail_ops);
         }
@@ -1325,7 +1337,6 @@ static bool fetch_counts(Solver& s) {  // A41:8059-8179
 static bool prepare_member(Solver& s, int substeps, int iterations, bool count) {
     if (!s.pins.empty() && !s.refresh_pins(iterations)) return false;
     s.fill_view(substeps);
-    if (s.pcg.enabled() && !s.pcg.reset_diagnostics(lib_stream())) return false;
     if (count) {
         launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.counts.p), 0u, 2 * static_cast<size_t>(kCountSlots));
         launch_fill_u32(lib_stream(), reinterpret_cast<unsigned*>(s.max_smoothed.p), 0u, 1);
@@ -1356,8 +1367,10 @@ static bool step_one(Solver& s, int substeps, int iterations, int fetch, int for
         if (tune && !s.tuner.begin_step(substeps)) return false;
         const ColorSegs bends{s.bend_segs.p, &s.bend_seg_first, &s.bend_seg_count};
         const ColorSegs stretch_colors{s.stretch_segs.p, &s.stretch_seg_first, &s.stretch_seg_count};
+        std::vector<const PcgGroup*> groups;
+        if (s.pcg.enabled()) groups.push_back(&s.pcg_group);
         auto record = [&] {
-            return run_schedule(one, ctx, s.d_view.p, bends, stretch_colors, sb, substeps, iterations, count, s.tuner, tune);
+            return run_schedule(one, ctx, s.d_view.p, bends, stretch_colors, groups, sb, substeps, iterations, count, s.tuner, tune);
         };
         std::vector<long long> key;
         const bool graphed = !no_alloc && graph_key(plan, substeps, iterations, count, s.tuner, &key);
@@ -1392,6 +1405,7 @@ struct Batch {
     Staging staging;
     int ntiles = 0;
     SelfBatchHolder self;
+    std::vector<PcgGroup> pcg_groups;   // members on the hard-stretch solve, one group per cadence
     StepGraph graph;
     PoolTuner tuner;
 };
@@ -1436,6 +1450,17 @@ static bool build_batch(Batch& b) {
                          b.sseg_first, b.sseg_count);
     b.ntiles = static_cast<int>(tiles.size());
     b.h_views.resize(b.members.size());
+    // members on the hard-stretch solve, grouped by substep cadence
+    std::map<int, std::vector<PcgMember>> by_cadence;
+    for (size_t m = 0; m < b.members.size(); ++m) {
+        Solver* s = b.members[m];
+        if (s->pcg.enabled()) by_cadence[s->pcg_cadence].push_back(PcgMember{static_cast<int>(m), &s->pcg, s->nv});
+    }
+    b.pcg_groups.clear();
+    for (auto& kv : by_cadence) {
+        b.pcg_groups.emplace_back();
+        if (!b.pcg_groups.back().build(kv.second, kv.first)) return false;
+    }
     return b.d_views.alloc(b.members.size(), "batch view allocation") && b.d_tiles.upload(tiles, "batch tile allocation") &&
            b.d_bend_segs.upload(b.h_bend_segs, "batch bend allocation") &&
            b.d_stretch_segs.upload(b.h_stretch_segs, "batch stretch allocation") &&
@@ -1462,8 +1487,10 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     if (tune && !b.tuner.begin_step(substeps)) return false;
     const ColorSegs bends{b.d_bend_segs.p, &b.seg_first, &b.seg_count};
     const ColorSegs stretch_colors{b.d_stretch_segs.p, &b.sseg_first, &b.sseg_count};
+    std::vector<const PcgGroup*> groups;
+    for (const PcgGroup& g : b.pcg_groups) groups.push_back(&g);
     auto record = [&] {
-        return run_schedule(b.members, ctx, b.d_views.p, bends, stretch_colors, sb, substeps, iterations, false, b.tuner, tune);
+        return run_schedule(b.members, ctx, b.d_views.p, bends, stretch_colors, groups, sb, substeps, iterations, false, b.tuner, tune);
     };
     std::vector<long long> key;
     const bool graphed = graph_key(plan, substeps, iterations, false, b.tuner, &key);

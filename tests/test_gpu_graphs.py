@@ -100,10 +100,84 @@ def test_batch_of_mixed_cloths_replays_equal_direct(nb, hip, scenes, graph_mode)
             assert np.array_equal(pd, pg), (f, k)
 
 
-def test_self_collision_steps_launch_directly(nb, hip, scenes, graph_mode):
-    cloth = scenes.grid_cloth(12, 0.4, 0.3, seed=5)
-    opts = scenes.SolverOptions(self_collision=True, cloth_thickness=0.005)
-    s = nb.NativeXpbdSolver(cloth, opts, None, lib=hip)
-    for _ in range(3):
-        s.step(4, 4, diagnostics=False, synchronize=True)
-    assert nb.graph_stats(hip, handle=s.handle) == (0, 0)
+def _replay_vs_direct(nb, hip, graph_mode, make, frames, step):
+    """Builds the handles twice, runs `frames` frames with graph replay off
+    and on, and returns the per-frame positions and the graph statistics."""
+    runs = []
+    for graphed in (False, True):
+        graph_mode(graphed)
+        solvers, owner = make()
+        out = []
+        for _ in range(frames):
+            step(solvers, owner)
+            nb.synchronize(hip)
+            out.append([s.download_positions() for s in solvers])
+        stats = owner.graph_stats() if isinstance(owner, nb.SolverBatch) else nb.graph_stats(hip, handle=owner.handle)
+        runs.append((out, stats))
+        if isinstance(owner, nb.SolverBatch):
+            owner.close()
+    return runs
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_self_collision_steps_replay_equal_direct(nb, hip, scenes, graph_mode, mode):
+    """Strict (2) and fast (1) self collision: hashes, candidate tables and
+    responses captured with the substep loop."""
+    def make():
+        cloth = scenes.grid_cloth(24, 0.5, 0.15, pin_corners=False, seed=5)
+        opts = scenes.SolverOptions(use_ground=True, self_collision=True, self_collision_mode=mode, cloth_thickness=0.005,
+                                    self_collision_distance=0.006)
+        s = nb.NativeXpbdSolver(cloth, opts, None, lib=hip)
+        return [s], s
+    frames = 10
+    (direct, sd_), (graphed, sg) = _replay_vs_direct(nb, hip, graph_mode, make, frames,
+                                                     lambda ss, o: o.step(4, 4, diagnostics=False, synchronize=False))
+    assert sd_ == (0, 0) and sg == (1, frames - 1)
+    for f, (a, b) in enumerate(zip(direct, graphed)):
+        assert np.array_equal(a[0], b[0]), f
+
+
+def pcg_batch_cloths(scenes):
+    """Hard-stretch cloths of different sizes: strength 1.0 (polish and caps)
+    coloured and uncoloured, strength 0.4, and one cloth on the direct
+    stretch, on a ground."""
+    out = []
+    for side, strength, colored, seed in ((20, 1.0, True, 1), (12, 1.0, False, 2), (28, 0.4, True, 3), (16, 0.0, True, 4)):
+        cloth = scenes.grid_cloth(side, 0.03 * (side - 1), 0.6, seed=seed)
+        if not colored:
+            cloth.edge_color_offsets = np.empty(0, np.int32)
+        opts = scenes.SolverOptions(use_ground=True, stretch_optimization_enabled=strength > 0.0,
+                                    stretch_optimization_strength=strength, cloth_thickness=0.005)
+        out.append((cloth, opts))
+    return out
+
+
+def test_pcg_batch_replays_equal_direct(nb, hip, scenes, graph_mode):
+    def make():
+        solvers = [nb.NativeXpbdSolver(c, o, None, lib=hip) for c, o in pcg_batch_cloths(scenes)]
+        return solvers, nb.SolverBatch(solvers)
+    frames = 6
+    (direct, sd_), (graphed, sg) = _replay_vs_direct(nb, hip, graph_mode, make, frames, lambda ss, b: b.step(5, 3))
+    assert sd_ == (0, 0) and sg == (1, frames - 1)
+    for f, (xd, xg) in enumerate(zip(direct, graphed)):
+        for k, (pd, pg) in enumerate(zip(xd, xg)):
+            assert np.array_equal(pd, pg), (f, k)
+
+
+def test_many_handles_first_captures_back_to_back(nb, hip, scenes, graph_mode):
+    """The per-handle loop of the 64-cloth workload (SDF, ground, pins): every
+    handle captures its first graph while the previous handles' graphs still
+    run on the library stream (no synchronisation between steps)."""
+    sc = scenes.config3(seed=0, cross_cloth=False)
+    def make():
+        solvers = [nb.NativeXpbdSolver(c, o, sc.static_triangles, lib=hip) for c, o in zip(sc.cloths, sc.options)]
+        return solvers, solvers[-1]
+    def step(solvers, _):
+        for s in solvers:
+            s.step(sc.substeps, sc.iterations, diagnostics=False, synchronize=False)
+    frames = 3
+    (direct, _), (graphed, sg) = _replay_vs_direct(nb, hip, graph_mode, make, frames, step)
+    assert sg == (1, frames - 1)
+    for f, (xd, xg) in enumerate(zip(direct, graphed)):
+        for k, (pd, pg) in enumerate(zip(xd, xg)):
+            assert np.array_equal(pd, pg), (f, k)

@@ -103,3 +103,34 @@ def test_stress_case_full_size(nb, hip, scenes):
     towel = sc.cloths[1]
     assert np.array_equal(alone[1][0][towel.pin_indices], scenes.towel_pin_targets(towel, 2 * sc.options[1].dt))
     assert all(np.all(np.isfinite(x)) for st in alone for x in st)
+
+
+@pytest.mark.parametrize("cadence", [2, 4])
+def test_pcg_substep_cadence_gpu(nb, hip, oracle, scenes, cadence):
+    from test_pcg import _cadence_run
+    g, rg = _cadence_run(nb, hip, scenes, cadence)
+    r, rr = _cadence_run(nb, oracle, scenes, cadence)
+    assert rg == rr
+    for x, y in zip(g.download_state(), r.download_state()):
+        assert np.array_equal(x, y)
+
+
+def test_pcg_batch_equals_oracle(nb, hip, oracle, scenes):
+    """Hard-stretch cloths of several sizes and strengths (coloured and
+    uncoloured caps) and a direct-stretch cloth stepped as one batch: the
+    solve is one launch per phase over all of them, the reductions one
+    workgroup per cloth.  Each equals the oracle's run of it bit for bit."""
+    from test_gpu_graphs import pcg_batch_cloths
+    cloths = pcg_batch_cloths(scenes)
+    members = [nb.NativeXpbdSolver(c, o, None, lib=hip) for c, o in cloths]
+    refs = [nb.NativeXpbdSolver(c, o, None, lib=oracle) for c, o in cloths]
+    batch = nb.SolverBatch(members)
+    for f in range(8):
+        batch.step(5, 3)
+        for r in refs:
+            r.step(5, 3, diagnostics=False, synchronize=True)
+        nb.synchronize(hip)
+        for k, (m, r) in enumerate(zip(members, refs)):
+            for x, y in zip(m.download_state(), r.download_state()):
+                assert np.array_equal(x, y), (f, k)
+    batch.close()

@@ -72,3 +72,36 @@ def test_stress_case_reduced_on_oracle(nb, oracle, scenes):
     balloon = alone[0][0]
     r = np.linalg.norm(balloon - balloon.mean(axis=0), axis=1).mean()
     assert 0.7 < r < 1.3
+
+
+def _cadence_run(nb, lib, scenes, cadence, frames=3, substeps=6):
+    """A 16x16 cloth on the hard-stretch solve, created with
+    SSBL_ABI41_PCG_SUBSTEP_CADENCE set (read when the handle is created)."""
+    import os
+    old = os.environ.get("SSBL_ABI41_PCG_SUBSTEP_CADENCE")
+    os.environ["SSBL_ABI41_PCG_SUBSTEP_CADENCE"] = str(cadence)
+    try:
+        cloth = scenes.grid_cloth(16, 0.6, 1.0, seed=8)
+        opts = scenes.SolverOptions(use_ground=True, stretch_optimization_enabled=True, stretch_optimization_strength=1.0)
+        s = nb.NativeXpbdSolver(cloth, opts, None, lib=lib)
+    finally:
+        if old is None:
+            del os.environ["SSBL_ABI41_PCG_SUBSTEP_CADENCE"]
+        else:
+            os.environ["SSBL_ABI41_PCG_SUBSTEP_CADENCE"] = old
+    runs = []
+    for _ in range(frames):
+        s.step(substeps, 3, diagnostics=True, synchronize=True)
+        runs.append(s.diagnostics().abi41_pcg_iterations // 8)
+    return s, runs
+
+
+@pytest.mark.parametrize("cadence,solves", [(1, 6), (2, 3), (4, 2), (8, 1), (0, 6), (-3, 6)])
+def test_pcg_substep_cadence(nb, oracle, scenes, cadence, solves):
+    # A41:284-292, 9158-9177: the solve runs on substeps s with (s + 1) % c == 0
+    # and always on the last; c = min(env, substeps), unset or <= 0 means 1
+    s, runs = _cadence_run(nb, oracle, scenes, cadence)
+    assert runs == [solves] * len(runs)
+    if cadence == 2:
+        base, _ = _cadence_run(nb, oracle, scenes, 1)
+        assert not np.array_equal(s.download_positions(), base.download_positions())
