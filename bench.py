@@ -1,25 +1,35 @@
 """Benchmark: cloth vertex-substeps/s of the XPBD substep path on one MI355X.
 
 A step is one frame: every cloth of the workload advances by one
-ssbl_step_solver_ex(substeps, iterations) worth of work.  Two forms run on the
-same GPU and the same inputs:
+ssbl_step_solver_ex(substeps, iterations) worth of work.  The headline
+workload is the 64-cloth uneven batch (492,741 vertices, 10 substeps x 8
+iterations, ground + 128^3 box SDF, cross-cloth collision over four layers).
+Two forms run on the same GPU and the same inputs:
   plain   -- one ABI41 handle per cloth, stepped one after another (the
-             reference's per-slot loop, session_manager.py:2528-2591);
+             reference's per-slot loop, session_manager.py:2528-2591), with
+             the cross-cloth scene packed on the host every frame;
   batched -- the same handles stepped together through ssbl_ext_step_batch
-             (one launch per solver phase covers every cloth).
-`value` is the batched form's whole-job vertex-substeps/s with all state
-resident in HBM.  Per-frame latency percentiles come from a separate pass that
-synchronises after every frame.
+             (one launch per solver phase covers every cloth), the scene
+             gathered on the device.
+`value` is the batched form's whole-job vertex-substeps/s over all `--steps`
+frames (default: the workload's 120) from the workload's start state, with all
+state resident in HBM: the warm-up frames run first, then every handle is put
+back to its start state (ssbl_reset_solver + ssbl_update_positions) before the
+timed frames.  Per-frame latency percentiles, the per-kernel roofline entries
+and the CPU baseline come from separate passes over the same frames.
 
 Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
+import math
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -34,6 +44,11 @@ from ssbl_xpbd_amd import cross_cloth, native_backend as nb, scenes  # noqa: E40
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg: str) -> None:
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------- bytes
 def pass_bytes_per_vertex(scene) -> float:
     """Algorithmic bytes of one fused constraint/contact pass per vertex.
 
@@ -50,6 +65,50 @@ def pass_bytes_per_vertex(scene) -> float:
     return 12 + 4 + 12 + 12.0 * e / v + 24
 
 
+def scene_hash_sizes(scene):
+    """Triangle and particle counts of the cross-cloth scene and the bucket
+    counts of its two hashes (next power of two of 4n, at least 1024)."""
+    nt = sum(len(c.triangles) for c in scene.cloths)
+    npart = scene.vertex_count
+    buckets = lambda n: 1 << max(10, math.ceil(math.log2(max(4 * n, 1))))  # noqa: E731
+    return nt, npart, buckets(nt), buckets(npart)
+
+
+def kernel_bytes(name: str, scene, frames: int, entries: float, counters: dict) -> float | None:
+    """Algorithmic bytes of all launches of one timed scope over `frames`
+    frames of the cross-cloth workload (scene rebuilt once per frame; the
+    dynamic contact pass runs once per substep):
+      tri_hash_count   per triangle its 64-byte span record and 4-byte large
+                       flag; per bucket a 4-byte count written and read back;
+      tri_hash_scatter per triangle the record, flag and 8-byte source/layer;
+                       per entry (triangle, cell) a 16-byte record written;
+                       per bucket its offset and count (12 bytes);
+      tri_bins         per bucket the offsets (4) and a 16-byte bin written;
+                       every entry's 16-byte record read;
+      part_hash        per particle its 16-byte position, 4-byte source, layer,
+                       cell (16) and 48-byte packed entry; per bucket 12 bytes;
+      dynamic_contacts per vertex and launch position, previous position and
+                       velocity read and written (72), inverse mass (4) and
+                       its 16-byte triangle bin; per probed triangle entry 16
+                       bytes plus the 64-byte triangle record, per probed
+                       particle entry 48 bytes (counts from the diagnostics of
+                       the same frames)."""
+    nt, npart, tb, pb = scene_hash_sizes(scene)
+    if name == "tri_hash_count":
+        return frames * (nt * 68.0 + tb * 8.0)
+    if name == "tri_hash_scatter":
+        return frames * (nt * 76.0 + entries * 16.0 + tb * 12.0)
+    if name == "tri_bins":
+        return frames * (tb * 20.0 + entries * 16.0)
+    if name == "part_hash":
+        return frames * (npart * 88.0 + pb * 12.0)
+    if name == "dynamic_contacts":
+        launches = frames * scene.substeps
+        return (launches * scene.vertex_count * 92.0 + counters["dynamic_triangle_candidate_count"] * 80.0
+                + counters["dynamic_particle_candidate_count"] * 48.0)
+    return None
+
+
 def measured_traffic(workload: str):
     """HBM bytes per launch of the timed kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes over this workload (scaled by the copy
@@ -64,69 +123,24 @@ def measured_traffic(workload: str):
     return 1024.0 * (rec["fetch_kib_per_dispatch"] * rec["fetch_scale"] + rec["write_kib_per_dispatch"] * rec["write_scale"])
 
 
-def make_workload(name: str, seed: int):
-    if name == "plain_32x32":
-        return scenes.config1(seed=seed)
-    if name == "batched_64_uneven":
-        return scenes.config3(seed=seed)
-    if name == "selfcol_100x100":
-        return scenes.config2(seed=seed)
-    raise SystemExit(f"unknown workload {name}")
-
-
-def single_cloth_line(name: str, seed: int, frames: int, warmup: int, lib) -> dict:
-    """Throughput and per-frame latency of one cloth stepped alone."""
-    sc = make_workload(name, seed)
-    (s,) = create(sc)
-    run_plain([s], sc, warmup)
-    t = timed(lambda: run_plain([s], sc, frames), lib)
-    lat = latencies(lambda: run_plain([s], sc, 1), lib, frames)
-    d = s.diagnostics()
-    return {"workload": sc.name, "vertices": sc.vertex_count, "substeps": sc.substeps, "iterations": sc.iterations,
-            "vertex_substeps_per_s": frames * sc.vertex_count * sc.substeps / t,
-            "frame_ms_p50": float(np.percentile(lat, 50)), "frame_ms_p99": float(np.percentile(lat, 99)),
-            "self_collision": bool(sc.options[0].self_collision), "finite": int(d.finite_flag)}
-
-
-def serving_line(seed: int, window_s: float, max_s: float, lib) -> dict:
-    """Served cloth sessions: Poisson(200/s) arrivals over `window_s` seconds,
-    sizes log-uniform in [1k, 250k] vertices, 60-600 frames each, wind and a
-    Lissajous sphere per session, admitted against a 200 GB HBM budget and
-    stepped by continuous batching (10 substeps x 8 iterations)."""
-    from ssbl_xpbd_amd import serving
-    t0 = time.perf_counter()
-    reqs = serving.served_requests(seed=seed, rate=200.0, window_s=window_s)
-    build_s = time.perf_counter() - t0
-    srv = serving.ClothServer(hbm_budget_bytes=200 * 10**9, substeps=10, iterations=8, max_sessions=512,
-                              max_batch_vertices=16 * 10**6, lib=lib)
-    rep = srv.run(reqs, clock="wall", max_seconds=max_s)
-    out = rep.summary()
-    out.update({"workload": "served_sessions", "arrival_rate_per_s": 200.0, "arrival_window_s": window_s,
-                "sessions_requested": len(reqs), "request_build_s": build_s, "substeps": 10, "iterations": 8,
-                "hbm_budget_bytes": 200 * 10**9})
-    print(f"# serving: {len(rep.results)}/{len(reqs)} sessions in {rep.wall_s:.1f} s, "
-          f"p50 {rep.frame_ms_p50:.2f} ms p99 {rep.frame_ms_p99:.2f} ms", file=sys.stderr, flush=True)
-    return out
-
-
+# ---------------------------------------------------------------- drivers
 def create(scene, lib=None):
     return [nb.NativeXpbdSolver(c, o, scene.static_triangles, lib=lib) for c, o in zip(scene.cloths, scene.options)]
+
+
+def restart(solvers, scene):
+    """Every handle back to the workload's start state: rest positions, zero
+    velocity, then the seed-jittered start (what creation left)."""
+    for s, c in zip(solvers, scene.cloths):
+        s.reset()
+        if c.start_positions is not None:
+            s.update_positions(c.start_positions)
 
 
 def cross_scene(solvers, scene, lib=None):
     if not scene.layers or len(solvers) < 2:
         return None
     return cross_cloth.CrossClothScene(solvers, scene.layers[:len(solvers)], contact_distance=scene.contact_distance, lib=lib)
-
-
-def cross_contacts(solvers, scene):
-    """Dynamic (cross-cloth) contacts of one frame, summed over the cloths."""
-    total = 0
-    for s in solvers:
-        s.step(scene.substeps, scene.iterations, diagnostics=True, synchronize=True)
-        d = s.diagnostics()
-        total += int(d.fast_triangle_pair_contacts + d.dynamic_particle_contacts)
-    return total
 
 
 def run_plain(solvers, scene, frames, cross=None):
@@ -138,11 +152,22 @@ def run_plain(solvers, scene, frames, cross=None):
             s.step(scene.substeps, scene.iterations, diagnostics=False, synchronize=False)
 
 
-def run_batched(batch, scene, frames, cross=None):
+def run_batched(batch, scene, frames, cross=None, diagnostics=False, counters=None):
+    """Batched frames; with `counters`, every member's diagnostics are summed
+    into it after each frame, and "scene_entries" gets the scene's triangle
+    hash entries of each frame."""
     for _ in range(frames):
         if cross is not None:
             cross.refresh(device=True)
-        batch.step(scene.substeps, scene.iterations)
+            if counters is not None:
+                counters["scene_entries"] += int(cross.lib.ssbl_ext_global_scene_entries(cross.scene.handle))
+        batch.step(scene.substeps, scene.iterations, diagnostics=diagnostics)
+        if counters is not None:
+            for s in batch.solvers:
+                d = s.diagnostics()
+                for k in counters:
+                    if k != "scene_entries":
+                        counters[k] += int(getattr(d, k))
 
 
 def timed(fn, lib):
@@ -169,53 +194,164 @@ def kernel_stats(lib, fn):
     fn()
     nb.synchronize(lib)
     lib.ssbl_ext_kernel_timing(0)
-    buf = __import__("ctypes").create_string_buffer(1 << 16)
+    buf = ctypes.create_string_buffer(1 << 16)
     if not lib.ssbl_ext_kernel_stats(buf, len(buf)):
         return {}
     return json.loads(buf.value.decode())
 
 
+def pct(x, q):
+    return float(np.percentile(x, q)) if len(x) else 0.0
+
+
+# ------------------------------------------------------------ CPU baseline
 def cpu_baseline(scene, budget_s: float):
-    """The CPU oracle (FP32, single thread) on a bounded sample of the workload."""
+    """The CPU oracle (FP32) on the first frames of the same workload: every
+    cloth, the cross-cloth scene packed every frame, stepped once on one thread
+    and once with the cloths spread over the host's cores (each cloth's step
+    is independent of the others', so the threads change nothing in the
+    results).  Frames: as many as the single-thread pass fits in `budget_s`
+    (at least one); the all-core pass runs the same frames."""
     lib = nb.load_library(str(ROOT / "oracle" / "liboracle_abi41.so"))
-    if scene.layers and len(scene.cloths) >= 8:
-        picked = list(range(8))            # the first two stacks of four (cross-cloth collision within them)
-    else:
-        order = np.argsort([len(c.positions_world) for c in scene.cloths])
-        picked = [int(k) for k in order[::max(1, len(order) // 8)]]
-    solvers = [nb.NativeXpbdSolver(scene.cloths[k], scene.options[k], scene.static_triangles, lib=lib) for k in picked]
-    cross = None
-    if scene.layers and len(solvers) > 1:
-        cross = cross_cloth.CrossClothScene(solvers, [scene.layers[k] for k in picked], contact_distance=scene.contact_distance, lib=lib)
-    verts = sum(s.vertex_count for s in solvers)
-    frames = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s and frames < scene.frames:
+    host_cores = len(os.sched_getaffinity(0))
+    threads = max(1, min(host_cores, 16))
+    solvers = create(scene, lib=lib)
+    cross = cross_scene(solvers, scene, lib=lib)
+    work_frame = scene.vertex_count * scene.substeps
+
+    def frame(pool=None):
         if cross is not None:
             cross.refresh(device=True)
-        for s in solvers:
-            s.step(scene.substeps, scene.iterations, diagnostics=False, synchronize=False)
+        if pool is None:
+            for s in solvers:
+                s.step(scene.substeps, scene.iterations, diagnostics=False, synchronize=False)
+        else:
+            list(pool.map(lambda s: s.step(scene.substeps, scene.iterations, diagnostics=False, synchronize=False),
+                          sorted(solvers, key=lambda s: -s.vertex_count)))
+
+    frames = 0
+    t0 = time.perf_counter()
+    while frames < 1 or (time.perf_counter() - t0 < budget_s and frames < scene.frames):
+        frame()
         frames += 1
-    dt = time.perf_counter() - t0
-    work = verts * scene.substeps * frames
-    return {"value": work / dt, "unit": "vertex-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP32 single thread, {len(solvers)} of {len(scene.cloths)} cloths ({verts} vertices) x {frames} frames"
-                      + (", cross-cloth scene packed every frame" if cross is not None else "")}
+    t_single = time.perf_counter() - t0
+    single_pos = [s.download_positions() for s in solvers]
+    restart(solvers, scene)
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            frame(pool)
+        t_multi = time.perf_counter() - t0
+    same = all(np.array_equal(a, s.download_positions()) for a, s in zip(single_pos, solvers))
+    if cross is not None:
+        cross.close()
+    for s in solvers:
+        s.close()
+    return {"value": frames * work_frame / t_multi, "unit": "vertex-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32, all {len(scene.cloths)} cloths ({scene.vertex_count} vertices), frames 1-{frames} of the "
+                      f"workload, cross-cloth scene packed every frame; cloths spread over {threads} threads",
+            "single_thread_value": frames * work_frame / t_single, "host_cores": host_cores,
+            "threads_equal_single_thread": bool(same)}
 
 
+# ---------------------------------------------------------- secondary lines
+def single_scene_line(scene, frames: int, warmup: int, lib, *, pins=None, batched=True) -> dict:
+    """Throughput and per-frame latency of a small scene stepped from its start
+    state: batched when it has several cloths, else the handle alone.  `pins`
+    updates pin targets before frame f (a moving boundary)."""
+    solvers = create(scene)
+    batch = nb.SolverBatch(solvers) if batched and len(solvers) > 1 else None
+
+    def frame(f):
+        if pins is not None:
+            pins(solvers, f)
+        if batch is not None:
+            batch.step(scene.substeps, scene.iterations)
+        else:
+            for s in solvers:
+                s.step(scene.substeps, scene.iterations, diagnostics=False, synchronize=False)
+
+    for f in range(warmup):
+        frame(f)
+    restart(solvers, scene)
+    t = timed(lambda: [frame(f) for f in range(frames)], lib)
+    restart(solvers, scene)
+    lat = []
+    for f in range(frames):
+        nb.synchronize(lib)
+        t0 = time.perf_counter()
+        frame(f)
+        nb.synchronize(lib)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    finite = all(np.isfinite(s.download_positions()).all() for s in solvers)
+    if batch is not None:
+        batch.close()
+    for s in solvers:
+        s.close()
+    return {"workload": scene.name, "cloths": len(scene.cloths), "vertices": scene.vertex_count, "substeps": scene.substeps,
+            "iterations": scene.iterations, "frames": frames, "of_frames": scene.frames,
+            "vertex_substeps_per_s": frames * scene.vertex_count * scene.substeps / t, "ms_per_frame": t / frames * 1e3,
+            "frame_ms_p50": pct(lat, 50), "frame_ms_p99": pct(lat, 99), "form": "batched" if batch is not None else "alone",
+            "finite": bool(finite)}
+
+
+def stress_line(frames: int, warmup: int, lib) -> dict:
+    """The closed-mesh stress case (655,362-vertex balloon with pressure and a
+    1,000,000-vertex towel twisted by its turning end rows; strict self
+    collision and PCG on both; 20 x 10): the first `frames` frames."""
+    sc = scenes.config5(frames=max(frames, 1))
+    towel = sc.cloths[1]
+    dt = sc.options[1].dt
+
+    def pins(solvers, f):
+        solvers[1].update_pin_targets(towel.pin_indices, scenes.towel_pin_targets(towel, (f + 1) * dt), towel.pin_weights)
+
+    return single_scene_line(sc, frames, warmup, lib, pins=pins)
+
+
+def serving_line(reqs, window_s: float, max_s: float, budget: int, capacity: int, builders: int, lib) -> dict:
+    """Served cloth sessions: Poisson(200/s) arrivals over `window_s` seconds,
+    sizes log-uniform in [1k, 250k] vertices, 60-600 frames each, wind and a
+    Lissajous sphere per session, PCG on; admitted against `budget` bytes of
+    HBM by `builders` threads beside the tick loop, stepped by continuous
+    batching (10 substeps x 8 iterations) with at most `capacity` vertices per
+    tick; the run stops at `max_s` seconds.  Percentiles are over every frame
+    served; "10k" is the 7k-14k-vertex sessions."""
+    from ssbl_xpbd_amd import serving
+    srv = serving.ClothServer(hbm_budget_bytes=budget, substeps=10, iterations=8, max_batch_vertices=capacity, lib=lib,
+                              builders=builders)
+    rep = srv.run(reqs, clock="wall", max_seconds=max_s)
+    out = rep.summary()
+    out.update({"workload": "served_sessions", "arrival_rate_per_s": 200.0, "arrival_window_s": window_s, "run_cap_s": max_s,
+                "sessions_requested": len(reqs), "substeps": 10, "iterations": 8, "hbm_budget_bytes": budget,
+                "tick_vertex_capacity": capacity, "builder_threads": builders, "frames_served": len(srv.frame_log),
+                "pcg": "on (strength 0.4, the reference's default hardness)"})
+    log(f"serving (budget {budget / 1e9:.0f} GB): {len(rep.results)}/{len(reqs)} sessions finished in {rep.wall_s:.1f} s, "
+        f"{len(srv.frame_log)} frames, step p50 {rep.frame_ms_p50:.2f} ms p99 {rep.frame_ms_p99:.2f} ms, "
+        f"10k p99 {rep.step_ms_10k[1]:.2f} ms, peak {rep.peak_device_bytes / 1e9:.1f} GB, budget waits {rep.budget_waits}")
+    return out
+
+
+# ------------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30, help="timed frames")
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default=os.environ.get("SSBL_BENCH_WORKLOAD", "batched_64_uneven"))
+    ap.add_argument("--steps", type=int, default=0, help="timed frames (0: the workload's 120)")
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--latency-frames", type=int, default=20)
+    ap.add_argument("--plain-latency-frames", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serving-window", type=float, default=0.5,
-                    help="seconds of Poisson(200/s) session arrivals in the served-sessions run (0 = skip it)")
-    ap.add_argument("--serving-max-s", type=float, default=25.0, help="wall-clock cap of the served-sessions run")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the other workloads' lines")
+    ap.add_argument("--stress-frames", type=int, default=6, help="frames of the 1.66M-vertex stress case (0 = skip)")
+    ap.add_argument("--serving-window", type=float, default=6.0,
+                    help="seconds of Poisson(200/s) session arrivals in the served-sessions runs (0 = skip them)")
+    ap.add_argument("--serving-max-s", type=float, default=15.0, help="wall-clock cap of each served-sessions run")
+    ap.add_argument("--serving-capacity", type=int, default=1_000_000, help="most vertices stepped in one served tick")
+    ap.add_argument("--serving-builders", type=int, default=4, help="threads creating served sessions' handles")
+    ap.add_argument("--serving-small-budget-gb", type=float, default=16.0,
+                    help="HBM budget of the second served run, small enough to bind")
+    ap.add_argument("--graph", type=int, default=1, help="graph replay of the substep loop (0 = direct launches)")
     ap.add_argument("--batched-only", action="store_true",
                     help="only step the batched form (warmup + steps frames) and exit: for counter passes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -235,59 +371,76 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         raise SystemExit(2)
 
-    scene = make_workload(args.workload, args.seed)
+    scene = scenes.config3(seed=args.seed)
+    steps = args.steps if args.steps > 0 else scene.frames
     lib = nb.load_library()
-    print(f"# workload {scene.name}: {len(scene.cloths)} cloths, {scene.vertex_count} vertices, "
-          f"{scene.substeps}x{scene.iterations}; device {lib.ssbl_ext_backend_name().decode()}", file=sys.stderr, flush=True)
+    nb.set_graph_mode(lib, bool(args.graph))
+    log(f"workload {scene.name}: {len(scene.cloths)} cloths, {scene.vertex_count} vertices, "
+        f"{scene.substeps}x{scene.iterations}, {steps} frames; device {lib.ssbl_ext_backend_name().decode()}")
     if args.batched_only:
         members = create(scene)
         batch = nb.SolverBatch(members)
-        run_batched(batch, scene, args.warmup + args.steps, cross_scene(members, scene))
+        run_batched(batch, scene, args.warmup + steps, cross_scene(members, scene))
         nb.synchronize(lib)
-        print(json.dumps({"batched_only_frames": args.warmup + args.steps, "workload": scene.name}), flush=True)
+        print(json.dumps({"batched_only_frames": args.warmup + steps, "workload": scene.name}), flush=True)
         return
     plain = create(scene)
     members = create(scene)
     cross_p, cross_b = cross_scene(plain, scene), cross_scene(members, scene)
-    batch = nb.SolverBatch(members) if len(members) > 1 else None
+    batch = nb.SolverBatch(members)
     work_per_frame = scene.vertex_count * scene.substeps
 
+    # plain form: warm up, back to the start, all timed frames
     run_plain(plain, scene, args.warmup, cross_p)
-    t_plain = timed(lambda: run_plain(plain, scene, args.steps, cross_p), lib)
-    print(f"# plain: {args.steps} frames in {t_plain:.3f} s", file=sys.stderr, flush=True)
-    if batch is not None:
-        run_batched(batch, scene, args.warmup, cross_b)
-        t_batch = timed(lambda: run_batched(batch, scene, args.steps, cross_b), lib)
-        step_frame = lambda: run_batched(batch, scene, 1, cross_b)  # noqa: E731
-    else:
-        t_batch = t_plain
-        step_frame = lambda: run_plain(plain, scene, 1, cross_p)  # noqa: E731
-    print(f"# batched: {args.steps} frames in {t_batch:.3f} s", file=sys.stderr, flush=True)
-    lat_b = latencies(step_frame, lib, args.latency_frames)
-    lat_p = latencies(lambda: run_plain(plain, scene, 1, cross_p), lib, args.latency_frames)
-    contacts = cross_contacts(members, scene) if cross_b is not None else None
+    restart(plain, scene)
+    t_plain = timed(lambda: run_plain(plain, scene, steps, cross_p), lib)
+    log(f"plain: {steps} frames in {t_plain:.3f} s")
+    lat_p = latencies(lambda: run_plain(plain, scene, 1, cross_p), lib, args.plain_latency_frames)
 
-    secondary = single_cloth_line("selfcol_100x100", args.seed, 10, 2, lib) if args.workload != "selfcol_100x100" else None
-    serving = serving_line(args.seed, args.serving_window, args.serving_max_s, lib) if args.serving_window > 0 else None
-    stats = kernel_stats(lib, step_frame)
-    # the dominant kernel's mean duration: back-to-back launches between two
-    # events on the library stream (per-launch instrumentation slows each
-    # launch, so the per-kernel stats above are only a breakdown)
-    timing_batch = batch if batch is not None else nb.SolverBatch(plain)
-    avg_ms = timing_batch.time_pass(64)
+    # batched form: the value
+    run_batched(batch, scene, args.warmup, cross_b)
+    restart(members, scene)
+    t_batch = timed(lambda: run_batched(batch, scene, steps, cross_b), lib)
+    log(f"batched: {steps} frames in {t_batch:.3f} s")
+    # per-frame latency over the same frames (a synchronisation after each)
+    restart(members, scene)
+    lat_b = latencies(lambda: run_batched(batch, scene, 1, cross_b), lib, steps)
+    # the same frames once more with every launch timed by its own dispatch
+    # and every member's counters read back after each frame
+    restart(members, scene)
+    counters = {"dynamic_triangle_candidate_count": 0, "dynamic_particle_candidate_count": 0, "fast_triangle_pair_contacts": 0,
+                "dynamic_particle_contacts": 0, "static_sdf_contact_count": 0, "scene_entries": 0}
+    stats = kernel_stats(lib, lambda: run_batched(batch, scene, steps, cross_b, diagnostics=True, counters=counters))
+    # back-to-back fused passes on the final state (dispatch gaps included)
+    avg_ms = batch.time_pass(64)
+    kernels = {}
+    for name, rec in stats.items():
+        per = rec["total_ms"] / max(rec["launches"], 1)
+        kernels[name] = {"launches": rec["launches"], "total_ms": rec["total_ms"], "avg_us": per * 1e3}
+    roof_more = []
+    for name in ("tri_hash_count", "tri_hash_scatter", "tri_bins", "part_hash", "dynamic_contacts"):
+        if name not in kernels:
+            continue
+        b = kernel_bytes(name, scene, steps, counters["scene_entries"] / steps, counters)
+        k = kernels[name]
+        achieved = b / (k["total_ms"] * 1e-3) / 1e9
+        roof_more.append({"kernel": name, "bound": "hbm", "algorithmic_bytes_per_launch": b / k["launches"],
+                          "avg_launch_us": k["avg_us"], "achieved": achieved, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS})
+    vp = kernels.get("vertex_pass_stretch")
     bytes_per_launch = pass_bytes_per_vertex(scene) * scene.vertex_count
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic = measured_traffic(scene.name)
-    value = args.steps * work_per_frame / t_batch
-    plain_value = args.steps * work_per_frame / t_plain
+    vp_ms = vp["total_ms"] / vp["launches"] if vp else avg_ms
+    achieved = bytes_per_launch / (vp_ms * 1e-3) / 1e9
+    contacts = (counters["fast_triangle_pair_contacts"] + counters["dynamic_particle_contacts"]) / steps
+    value = steps * work_per_frame / t_batch
+    plain_value = steps * work_per_frame / t_plain
     line = {
         "metric": "cloth vertex-substeps/s, batched vs plain on same GPU; p50/p99 step latency",
         "value": value,
         "unit": "vertex-substeps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": t_batch / args.steps * 1e3,
+        "ms_per_step": t_batch / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -299,20 +452,20 @@ def main():
             "vertices": scene.vertex_count,
             "substeps": scene.substeps,
             "iterations": scene.iterations,
-            "cross_cloth_collision": cross_b is not None,
-            "cross_cloth_layers": 4 if cross_b is not None else 0,
+            "frames": f"1-{steps} of {scene.frames}, from the start state",
+            "graph_replay": bool(args.graph),
+            "cross_cloth_collision": True,
+            "cross_cloth_layers": 4,
             "cross_cloth_contacts_per_frame": contacts,
             "plain_vertex_substeps_per_s": plain_value,
             "batched_over_plain": value / plain_value,
-            "frame_ms_p50_batched": float(np.percentile(lat_b, 50)),
-            "frame_ms_p99_batched": float(np.percentile(lat_b, 99)),
-            "frame_ms_p50_plain": float(np.percentile(lat_p, 50)),
-            "frame_ms_p99_plain": float(np.percentile(lat_p, 99)),
-            "kernels": stats,
-            "tuned_pool_limit": batch.tuned_pool_limit() if batch is not None else None,
-            "step_graph_captures_replays": list(batch.graph_stats()) if batch is not None else None,
-            "selfcol_10k": secondary,
-            "serving": serving,
+            "frame_ms_p50_batched": pct(lat_b, 50),
+            "frame_ms_p99_batched": pct(lat_b, 99),
+            "frame_ms_p50_plain": pct(lat_p, 50),
+            "frame_ms_p99_plain": pct(lat_p, 99),
+            "kernels": kernels,
+            "scene_hash_entries_per_frame": counters["scene_entries"] / steps,
+            "step_graph_captures_replays": list(batch.graph_stats()),
         },
         "roofline": {
             "bound": "hbm",
@@ -321,11 +474,37 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
+            "traffic": measured_traffic(scene.name),
             "algorithmic_bytes_per_launch": bytes_per_launch,
-            "avg_launch_ms": avg_ms,
+            "avg_launch_ms": vp_ms,
+            "avg_launch_ms_back_to_back_final_state": avg_ms,
+            "other_kernels": roof_more,
         },
     }
+    batch.close()
+    for c in (cross_p, cross_b):
+        c.close()
+    for s in plain + members:
+        s.close()
+    if not args.no_secondary:
+        sec = {}
+        sec["plain_32x32"] = single_scene_line(scenes.config1(seed=args.seed), 120, 10, lib)
+        log(f"plain_32x32: {sec['plain_32x32']['ms_per_frame']:.2f} ms/frame")
+        sec["selfcol_100x100"] = single_scene_line(scenes.config2(seed=args.seed), 240, 5, lib)
+        log(f"selfcol_100x100: {sec['selfcol_100x100']['ms_per_frame']:.2f} ms/frame")
+        if args.stress_frames > 0:
+            sec["stress_balloon_towel"] = stress_line(args.stress_frames, 1, lib)
+            log(f"stress: {sec['stress_balloon_towel']['ms_per_frame']:.1f} ms/frame")
+        if args.serving_window > 0:
+            from ssbl_xpbd_amd import serving
+            t0 = time.perf_counter()
+            reqs = serving.served_requests(seed=args.seed, rate=200.0, window_s=args.serving_window)
+            log(f"served requests: {len(reqs)} sessions, {sum(r.vertex_count for r in reqs)} vertices, "
+                f"built in {time.perf_counter() - t0:.1f} s")
+            for key, budget in (("serving", 200 * 10**9), ("serving_budget_bound", args.serving_small_budget_gb * 10**9)):
+                sec[key] = serving_line(reqs, args.serving_window, args.serving_max_s, budget, args.serving_capacity,
+                                        args.serving_builders, lib)
+        line["config"]["other_workloads"] = sec
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget)
     print(json.dumps(line), flush=True)

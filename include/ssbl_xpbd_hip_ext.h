@@ -59,6 +59,11 @@ SSBL_API int ssbl_ext_set_static_sdf(void* handle, const int* dims, const float*
 SSBL_API void* ssbl_ext_create_batch(void* const* handles, int count);
 SSBL_API int ssbl_ext_destroy_batch(void* batch);
 SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int force_sync);
+/* The same step with every member's diagnostics counted and read back when
+ * fetch_diagnostics is nonzero (ssbl_get_diagnostics on each handle
+ * afterwards), as ssbl_step_solver_ex does for one handle; waits for the
+ * step then. */
+SSBL_API int ssbl_ext_step_batch_ex(void* batch, int substeps, int iterations, int fetch_diagnostics, int force_sync);
 
 /* Runs `launches` (rounded up to even) back-to-back fused constraint passes of
  * the batch (stretch gather, pins, analytic and SDF contacts) between two
@@ -77,6 +82,11 @@ SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms);
 SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const* handles, const int* source_ids,
                                                       const int* collision_layers, const float* particle_radii, int count,
                                                       float collision_margin, float cloth_thickness);
+
+/* (triangle, cell) entries of a global scene's triangle hash after its last
+ * build, for measurement (the algorithmic bytes of the hash build); waits for
+ * the library's queued work.  -1 for an invalid scene. */
+SSBL_API long long ssbl_ext_global_scene_entries(void* scene);
 
 /* Device memory (bytes) the handle holds: state, constraint tables, SDF,
  * self-collision hashes and PCG arrays.  -1 for an invalid handle.  A

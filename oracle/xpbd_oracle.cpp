@@ -3324,6 +3324,19 @@ SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int 
         if (!ssbl_step_solver_ex(h, substeps, iterations, 0, force_sync)) return 0;
     return 1;
 }
+SSBL_API long long ssbl_ext_global_scene_entries(void* scene) {
+    g_error.clear();
+    if (!scene) { fail("invalid global dynamic scene handle"); return -1; }
+    const DynamicSet& d = static_cast<Scene*>(scene)->set;
+    return d.tri_hash && !d.tri_off.empty() ? d.tri_off.back() : 0;
+}
+SSBL_API int ssbl_ext_step_batch_ex(void* batch, int substeps, int iterations, int fetch_diagnostics, int force_sync) {
+    g_error.clear();
+    if (!batch) return fail("invalid batch") ? 1 : 0;
+    for (void* h : static_cast<Batch*>(batch)->handles)
+        if (!ssbl_step_solver_ex(h, substeps, iterations, fetch_diagnostics, force_sync)) return 0;
+    return 1;
+}
 SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const* handles, const int* source_ids, const int* layers,
                                                       const float* radii, int count, float margin, float thickness) {
     // the host packing of session_manager.py:3242-3300 from each handle's positions

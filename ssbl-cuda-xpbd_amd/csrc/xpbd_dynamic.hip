@@ -882,19 +882,26 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
             // their buckets are sorted by key
             const int ranked = signed_ ? 1 : 0;
             const int* fsrc = tri_filter_ ? tsrc_ : nullptr;
-            launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
-                     static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, tri_ent_,
-                     fsrc, static_cast<const int*>(tlayer_), ranked, 0);
+            {
+                TimedLaunch tc("tri_hash_count", st);
+                launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
+                         static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, tri_ent_,
+                         fsrc, static_cast<const int*>(tlayer_), ranked, 0);
+            }
             exclusive_scan(st, tri_count_, buckets, tri_off_, scan_tmp_);
-            launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
-                     static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, tri_ent_,
-                     fsrc, static_cast<const int*>(tlayer_), ranked, 1);
+            {
+                TimedLaunch ts("tri_hash_scatter", st);
+                launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
+                         static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, tri_ent_,
+                         fsrc, static_cast<const int*>(tlayer_), ranked, 1);
+            }
             if (ranked) {
                 const dim3 gr(std::min<unsigned>(grid_for(static_cast<long long>(nt_) * 16, 256), 16384u));
                 launch_k(k_tri_rank, gr, blk, 0, st, static_cast<const int*>(tri_off_), buckets, static_cast<const int2*>(tri_raw_),
                          tri_keys_, fsrc, static_cast<const int*>(tlayer_), tri_ksl_);
             }
             const dim3 gb(std::min<unsigned>(grid_for(buckets, 256), 16384u));
+            TimedLaunch tb("tri_bins", st);
             if (ranked)
                 launch_k(k_bucket_bins<int2>, gb, blk, 0, st, static_cast<const int*>(tri_off_), buckets, 0x7fffffff,
                          static_cast<const int2*>(tri_filter_ ? tri_ksl_ : nullptr), tri_bin_);
@@ -909,6 +916,7 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
         }
     }
     if (np_ >= kDynParticleHashMin) {
+        TimedLaunch tp("part_hash", st);
         const int buckets = next_pow2(std::max(kDynMinBuckets, np_ * 4));
         const dim3 g(grid_for(np_, 256)), blk(256);
         launch_k(k_fill_i32, dim3(grid_for(buckets, 256)), blk, 0, st, pcount_, 0, buckets);
@@ -925,6 +933,14 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
         launch_k(k_coarse_insert, g, blk, 0, st, static_cast<const DynView*>(d_view_), coarse_, coarse_off_);
     }
     return hip_ok(hipGetLastError(), "dynamic collider hash");
+}
+
+long long DynamicColliders::hash_entries() const {
+    if (!tri_off_ || nt_ < kDynTriangleHashMin) return 0;
+    const int buckets = next_pow2(std::max(kDynMinBuckets, nt_ * 4));
+    int n = 0;
+    if (!hip_ok(hipMemcpy(&n, tri_off_ + buckets, sizeof(int), hipMemcpyDeviceToHost), "dynamic hash entries")) return -1;
+    return n;
 }
 
 int DynamicColliders::large_count() const {

@@ -161,6 +161,9 @@ public:
     int particles() const { return np_; }
     bool triangles_allocated() const { return tri_cap_ > 0; }
     // Synchronous reads for diagnostics (the caller has synchronised).
+    // (triangle, cell) entries of the triangle hash after the last build
+    // (waits for the stream's work first); -1 on a copy error
+    long long hash_entries() const;
     int large_count() const;
     int max_occupancy() const;
 

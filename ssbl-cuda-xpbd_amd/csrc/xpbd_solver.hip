@@ -106,22 +106,25 @@ void timing_push(const char* name, hipEvent_t begin, hipEvent_t end) {
 }
 
 // ------------------------------------------------------- device buffers
+// Bytes allocated minus bytes freed by the calling thread: a handle's
+// creation and updates are measured on the thread that makes them, so
+// handles created on one thread while another steps are accounted exactly.
 namespace {
-std::atomic<long long> g_device_bytes{0};
+thread_local long long t_device_bytes = 0;
 }  // namespace
 
 void* device_alloc(size_t bytes, const char* what) {
     void* p = nullptr;
     if (!hip_ok(hipMalloc(&p, bytes), what)) return nullptr;
-    g_device_bytes.fetch_add(static_cast<long long>(bytes), std::memory_order_relaxed);
+    t_device_bytes += static_cast<long long>(bytes);
     return p;
 }
 void device_free(void* p, size_t bytes) {
     if (!p) return;
     (void)hipFree(p);
-    g_device_bytes.fetch_sub(static_cast<long long>(bytes), std::memory_order_relaxed);
+    t_device_bytes -= static_cast<long long>(bytes);
 }
-long long device_bytes_in_use() { return g_device_bytes.load(std::memory_order_relaxed); }
+long long device_bytes_in_use() { return t_device_bytes; }
 
 template <class T> struct Dev {
     T* p = nullptr;
@@ -1467,7 +1470,10 @@ static bool build_batch(Batch& b) {
            b.staging.reserve(sizeof(ClothView) * b.members.size());
 }
 
-static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
+// fetch: count diagnostics and read every member's counters afterwards (as
+// ssbl_step_solver_ex with fetch_diagnostics does for one handle).
+static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, bool fetch = false) {
+    const auto t0 = std::chrono::steady_clock::now();
     substeps = std::max(substeps, 1);
     iterations = std::max(iterations, 1);
     for (const Solver* s : b.members)
@@ -1475,7 +1481,7 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     for (size_t m = 0; m < b.members.size(); ++m) {
         Solver& s = *b.members[m];
         reset_diag(s);
-        if (!prepare_member(s, substeps, iterations, false)) return false;
+        if (!prepare_member(s, substeps, iterations, fetch)) return false;
         b.h_views[m] = s.view;
     }
     if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
@@ -1490,16 +1496,25 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync) {
     std::vector<const PcgGroup*> groups;
     for (const PcgGroup& g : b.pcg_groups) groups.push_back(&g);
     auto record = [&] {
-        return run_schedule(b.members, ctx, b.d_views.p, bends, stretch_colors, groups, sb, substeps, iterations, false, b.tuner, tune);
+        return run_schedule(b.members, ctx, b.d_views.p, bends, stretch_colors, groups, sb, substeps, iterations, fetch, b.tuner, tune);
     };
     std::vector<long long> key;
-    const bool graphed = graph_key(plan, substeps, iterations, false, b.tuner, &key);
+    const bool graphed = graph_key(plan, substeps, iterations, fetch, b.tuner, &key);
     if (!(graphed ? b.graph.run(lib_stream(), key, record) : record())) return false;
     if (tune) b.tuner.end_step();
     if (plan.any_dyn) ++b.tuner.step;
     for (Solver* s : b.members)
         if (!s->dyn.clear_motion(lib_stream())) return false;
-    return force_sync ? sync_stream() : true;
+    if (!fetch) return force_sync ? sync_stream() : true;
+    if (!sync_stream()) return false;
+    const float ms = static_cast<float>(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    for (Solver* s : b.members) {
+        if (!fetch_counts(*s)) return false;
+        s->diag.step_ms = s->diag.constraints_ms = ms;
+        s->diag.resolved_contacts = s->diag.abi41_soft_contact_count + s->diag.abi41_exact_impulse_contact_count +
+                                    s->diag.abi41_edge_edge_contact_count + s->diag.static_sdf_contact_count;
+    }
+    return true;
 }
 
 // Runs `launches` (even) back-to-back fused constraint passes of a batch --
@@ -1878,6 +1893,11 @@ SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int 
     if (!batch) return fail("invalid batch") ? 1 : 0;
     return step_batch(*static_cast<Batch*>(batch), substeps, iterations, force_sync) ? 1 : 0;
 }
+SSBL_API int ssbl_ext_step_batch_ex(void* batch, int substeps, int iterations, int fetch_diagnostics, int force_sync) {
+    g_error.clear();
+    if (!batch) return fail("invalid batch") ? 1 : 0;
+    return step_batch(*static_cast<Batch*>(batch), substeps, iterations, force_sync, fetch_diagnostics != 0) ? 1 : 0;
+}
 SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const* handles, const int* source_ids,
                                                       const int* collision_layers, const float* particle_radii, int count,
                                                       float collision_margin, float cloth_thickness) {
@@ -1907,6 +1927,13 @@ SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const*
     if (!sc->set.build(st, sc->counts.p, true, false)) return 0;
     sc->overflow_pending = true;
     return 1;
+}
+
+SSBL_API long long ssbl_ext_global_scene_entries(void* scene) {
+    g_error.clear();
+    if (!scene) { fail("invalid global dynamic scene handle"); return -1; }
+    if (!sync_stream()) return -1;
+    return static_cast<GlobalScene*>(scene)->set.hash_entries();
 }
 
 SSBL_API int ssbl_ext_set_static_sdf(void* handle, const int* dims, const float* origin_and_voxel, const float* values, int count) {

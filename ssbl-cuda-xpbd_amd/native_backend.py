@@ -465,8 +465,14 @@ class SolverBatch:
         if not self._handle:
             raise NativeSolverError(last_error(self._lib))
 
-    def step(self, substeps: int, iterations: int, synchronize: bool = False) -> None:
-        if not self._lib.ssbl_ext_step_batch(self._handle, int(substeps), int(iterations), int(bool(synchronize))):
+    def step(self, substeps: int, iterations: int, synchronize: bool = False, diagnostics: bool = False) -> None:
+        """One batched step; with diagnostics, every member's counters are
+        read back afterwards (each solver's diagnostics() then holds them)."""
+        if diagnostics:
+            ok = self._lib.ssbl_ext_step_batch_ex(self._handle, int(substeps), int(iterations), 1, 1)
+        else:
+            ok = self._lib.ssbl_ext_step_batch(self._handle, int(substeps), int(iterations), int(bool(synchronize)))
+        if not ok:
             raise NativeSolverError(last_error(self._lib))
 
     def graph_stats(self) -> tuple[int, int]:

@@ -362,7 +362,7 @@ def config3(seed: int = 0, count: int = 64, frames: int = 120, self_collision: b
                  0.005 if cross_cloth else 0.0)
 
 
-def config4(seed: int = 0, subdivisions: int = 8, towel_n: int = 1000, frames: int = 120) -> Scene:
+def config5(seed: int = 0, subdivisions: int = 8, towel_n: int = 1000, frames: int = 120) -> Scene:
     """Large closed-mesh stress case: an icosphere balloon (subdivision 8:
     655,362 vertices, 1,310,720 triangles, r = 1 m) with volume pressure 1.2,
     and a towel_n x towel_n grid towel (2 m square) twisted by its two end rows,

@@ -3,17 +3,20 @@
 A session is one cloth advanced frame by frame. It has a wind force field,
 an optional moving sphere collider, and a number of frames to run. The server:
 
-* admits waiting sessions in arrival order, while the HBM budget, the session
-  slots and the per-tick vertex capacity allow;
+* admits waiting sessions in arrival order while the HBM budget and the
+  session slots allow, building their handles (host set-up and uploads) on
+  builder threads beside the tick loop;
 * keeps every admitted session's state resident on the device (one ABI41
   handle each) until its last frame;
-* steps all active sessions together each tick (continuous batching).
-  A tick is one frame of every active session, run as one batched step
+* steps resident sessions together each tick (continuous batching): a tick
+  is one frame of each session it takes, least recently stepped first, up
+  to a per-tick vertex capacity, run as one batched step
   (ssbl_ext_step_batch), so one launch per solver phase covers them all.
   Sessions that arrive while a tick runs join at the next tick. Finished
-  sessions leave, and their memory frees slots for the queue;
-* records each session's per-frame latency (from the start of the tick's
-  input updates to the end of its step) and the aggregate vertex-substeps/s;
+  sessions leave, and their memory frees room for the queue;
+* records each session's per-frame step latency (from the start of the
+  tick's input updates to the end of its step), its frame interval (from the
+  end of its previous frame) and the aggregate vertex-substeps/s;
 * writes a PC2 cache of every session that asks for one (``cache_path``):
   the start positions, then one sample per frame, committed when the session
   ends (the reference's realtime auto-cache, session_manager.py:4650-4728).
@@ -124,7 +127,8 @@ class SessionResult:
     vertex_count: int
     frames: int
     positions: np.ndarray
-    frame_ms: list
+    frame_ms: list          # step latency of each frame: the duration of the batched step that advanced it
+    frame_gap_ms: list      # time from the previous frame's end (admission for the first) to this frame's end
     arrival: float
     admitted: float
     finished: float
@@ -138,9 +142,17 @@ class _Active:
     solver: nb.NativeXpbdSolver
     device_bytes: int
     admitted: float
+    order: int                       # admission order (ties of the round robin)
+    last_done: float = 0.0           # wall time of the last frame's end (admission before the first)
+    last_tick: int = -1              # last tick that stepped the session
     frame: int = 0
     frame_ms: list = field(default_factory=list)
+    frame_gap_ms: list = field(default_factory=list)
     cache: Pc2Writer | None = None
+
+
+# sessions counted as "10k-vertex" in the report: the size classes between 7k and 14k vertices
+TEN_K = (7000, 14000)
 
 
 @dataclass
@@ -152,10 +164,15 @@ class ServeReport:
     frame_ms_p50: float
     frame_ms_p99: float
     peak_device_bytes: int
-    peak_active_vertices: int
-    peak_sessions: int
+    peak_active_vertices: int     # most vertices stepped in one tick
+    peak_sessions: int            # most sessions resident at once
     mean_queue_wait: float
     unfinished: int
+    budget_waits: int = 0         # times an arrived session waited for device memory
+    step_ms_10k: tuple = (0.0, 0.0)   # percentiles over every frame served (finished sessions or not)
+    gap_ms: tuple = (0.0, 0.0)
+    gap_ms_10k: tuple = (0.0, 0.0)
+    sessions_10k: int = 0             # finished 10k-vertex sessions
 
     @property
     def vertex_substeps_per_s(self) -> float:
@@ -165,15 +182,44 @@ class ServeReport:
         return {"sessions_completed": len(self.results), "sessions_unfinished": self.unfinished, "ticks": self.ticks,
                 "wall_s": self.wall_s, "vertex_substeps_per_s": self.vertex_substeps_per_s,
                 "frame_ms_p50": self.frame_ms_p50, "frame_ms_p99": self.frame_ms_p99,
-                "peak_device_bytes": self.peak_device_bytes, "peak_active_vertices": self.peak_active_vertices,
-                "peak_sessions": self.peak_sessions, "mean_queue_wait": self.mean_queue_wait}
+                "frame_ms_p50_10k": self.step_ms_10k[0], "frame_ms_p99_10k": self.step_ms_10k[1],
+                "sessions_10k": self.sessions_10k,
+                "frame_interval_ms_p50": self.gap_ms[0], "frame_interval_ms_p99": self.gap_ms[1],
+                "frame_interval_ms_p50_10k": self.gap_ms_10k[0], "frame_interval_ms_p99_10k": self.gap_ms_10k[1],
+                "peak_device_bytes": self.peak_device_bytes, "peak_tick_vertices": self.peak_active_vertices,
+                "peak_resident_sessions": self.peak_sessions, "mean_queue_wait": self.mean_queue_wait,
+                "budget_waits": self.budget_waits}
+
+
+def _pcts(values) -> tuple:
+    return (float(np.percentile(values, 50)), float(np.percentile(values, 99))) if len(values) else (0.0, 0.0)
 
 
 class ClothServer:
-    """Admission + continuous batching over ABI41 handles of one library."""
+    """Admission by HBM budget and continuous batching over ABI41 handles of one library.
 
-    def __init__(self, *, hbm_budget_bytes: int, substeps: int, iterations: int, max_sessions: int = 1024,
-                 max_batch_vertices: int = 1 << 62, static_triangles: np.ndarray | None = None, lib=None):
+    Two limits, as in request-serving systems that keep state resident:
+      residency  -- sessions are admitted in arrival order while the device
+                    memory of the resident and building sessions (their
+                    estimates until built, then what the handle holds) stays
+                    within `hbm_budget_bytes` and at most `max_sessions` are
+                    resident; a finished session frees its memory for the
+                    queue;
+      scheduling -- each tick steps resident sessions, least recently
+                    stepped first, until `max_batch_vertices` vertices are in
+                    the tick (at least one session), as one batched step.  The
+                    cap bounds a tick's duration, which is every stepped
+                    session's step latency; with more resident work than one
+                    tick takes, the sessions take turns.
+    With `builders` > 0 (wall clock only), handles are created by that many
+    threads beside the tick loop: host set-up and uploads of a new session
+    never delay a tick.  Graph replay is turned off while serving: the
+    batch's members change from tick to tick, so a capture would rarely be
+    replayed.
+    """
+
+    def __init__(self, *, hbm_budget_bytes: int, substeps: int, iterations: int, max_sessions: int = 4096,
+                 max_batch_vertices: int = 1 << 62, static_triangles: np.ndarray | None = None, lib=None, builders: int = 0):
         self.lib = lib if lib is not None else nb.load_library()
         self.budget = int(hbm_budget_bytes)
         self.substeps = int(substeps)
@@ -181,18 +227,23 @@ class ClothServer:
         self.max_sessions = int(max_sessions)
         self.max_batch_vertices = int(max_batch_vertices)
         self.static_triangles = static_triangles
+        self.builders = int(builders)
         self.pending: list[SessionRequest] = []
-        self.active: list[_Active] = []
+        self.building: list = []                  # (request, estimate, future) in admission order
+        self.resident: list[_Active] = []
         self.results: dict[int, SessionResult] = {}
-        self.used_bytes = 0
-        self.active_vertices = 0
+        self.used_bytes = 0                       # resident sessions' bytes plus building sessions' estimates
         self.ticks = 0
         self.vertex_substeps = 0
-        self._batch: nb.SolverBatch | None = None
-        self._batch_stale = True
         self.peak_bytes = 0
         self.peak_vertices = 0
         self.peak_sessions = 0
+        self.budget_waits = 0
+        self._order = 0
+        self.frame_log: list = []                 # (vertex count, step ms, interval ms) of every frame served
+        self._pool = None
+        self._batch: nb.SolverBatch | None = None
+        self._batch_members: list = []
 
     # -- admission
     def submit(self, req: SessionRequest) -> None:
@@ -201,38 +252,67 @@ class ClothServer:
         self.pending.append(req)
         self.pending.sort(key=lambda r: (r.arrival, r.sid))
 
-    def _fits(self, req: SessionRequest) -> bool:
-        return (len(self.active) < self.max_sessions
-                and self.active_vertices + req.vertex_count <= self.max_batch_vertices
-                and self.used_bytes + estimate_device_bytes(req.cloth, req.options) <= self.budget)
+    def _build(self, req: SessionRequest):
+        solver = nb.NativeXpbdSolver(req.cloth, req.options, self.static_triangles, lib=self.lib)
+        if req.wind is not None:
+            solver.update_force_fields([req.wind])
+        held = int(self.lib.ssbl_ext_device_bytes(solver.handle))
+        if held < 0:                                # library without device accounting
+            held = estimate_device_bytes(req.cloth, req.options)
+        return solver, held
 
-    def admit(self, now: float) -> int:
-        """Admits arrived sessions in arrival order while they fit; returns how many."""
-        n = 0
-        while self.pending and self.pending[0].arrival <= now and self._fits(self.pending[0]):
-            req = self.pending.pop(0)
-            solver = nb.NativeXpbdSolver(req.cloth, req.options, self.static_triangles, lib=self.lib)
-            if req.wind is not None:
-                solver.update_force_fields([req.wind])
-            held = int(self.lib.ssbl_ext_device_bytes(solver.handle))
-            if held < 0:                                # library without device accounting
-                held = estimate_device_bytes(req.cloth, req.options)
-            act = _Active(req, solver, held, now)
+    def _start_builds(self, now: float) -> None:
+        """Starts building arrived sessions in arrival order while they fit."""
+        while self.pending and self.pending[0].arrival <= now:
+            req = self.pending[0]
+            est = estimate_device_bytes(req.cloth, req.options)
+            if len(self.resident) + len(self.building) >= self.max_sessions:
+                break
+            if self.used_bytes + est > self.budget:
+                self.budget_waits += 1
+                break
+            self.pending.pop(0)
+            self.used_bytes += est
+            fut = self._pool.submit(self._build, req) if self._pool is not None else self._build(req)
+            self.building.append((req, est, fut))
+
+    def _collect_built(self, now: float, wait: bool = False) -> None:
+        """Makes built sessions resident (in admission order)."""
+        while self.building:
+            req, est, fut = self.building[0]
+            if self._pool is not None:
+                if not wait and not fut.done():
+                    break
+                solver, held = fut.result()
+            else:
+                solver, held = fut
+            self.building.pop(0)
+            self.used_bytes += held - est
+            act = _Active(req, solver, held, now, self._order, last_done=time.perf_counter())
+            self._order += 1
             if req.cache_path is not None:
                 start = req.cloth.start_positions if req.cloth.start_positions is not None else req.cloth.positions_world
                 act.cache = Pc2Writer(req.cache_path, req.vertex_count, req.start_frame)
                 act.cache.add_sample(start)
-            self.active.append(act)
-            self.used_bytes += held
-            self.active_vertices += req.vertex_count
-            self._batch_stale = True
-            n += 1
+            self.resident.append(act)
         self.peak_bytes = max(self.peak_bytes, self.used_bytes)
-        self.peak_vertices = max(self.peak_vertices, self.active_vertices)
-        self.peak_sessions = max(self.peak_sessions, len(self.active))
-        return n
+        self.peak_sessions = max(self.peak_sessions, len(self.resident))
 
-    # -- one frame of every active session
+    def admit(self, now: float, wait: bool = False) -> None:
+        self._start_builds(now)
+        self._collect_built(now, wait)
+
+    # -- scheduling
+    def _choose(self) -> list:
+        order = sorted(self.resident, key=lambda s: (s.last_tick, s.order))
+        chosen, verts = [], 0
+        for s in order:
+            if chosen and verts + s.req.vertex_count > self.max_batch_vertices:
+                continue
+            chosen.append(s)
+            verts += s.req.vertex_count
+        return chosen
+
     def _frame_inputs(self, s: _Active) -> None:
         if s.req.sphere is None:
             return
@@ -241,33 +321,42 @@ class ClothServer:
                                                   sphere_radius=s.req.sphere.radius))
 
     def tick(self, now: float) -> bool:
+        """One frame of the sessions chosen for this tick, as one batched step."""
         self.admit(now)
-        if not self.active:
+        if not self.resident:
             return False
+        chosen = self._choose()
         t0 = time.perf_counter()
-        for s in self.active:
+        for s in chosen:
             self._frame_inputs(s)
-        if self._batch_stale or self._batch is None:
+        if self._batch is None or self._batch_members != [id(s) for s in chosen]:
             if self._batch is not None:
                 self._batch.close()
-            self._batch = nb.SolverBatch([s.solver for s in self.active])
-            self._batch_stale = False
+            self._batch = nb.SolverBatch([s.solver for s in chosen])
+            self._batch_members = [id(s) for s in chosen]
         self._batch.step(self.substeps, self.iterations, synchronize=True)
-        ms = (time.perf_counter() - t0) * 1.0e3
-        self.ticks += 1
-        self.vertex_substeps += self.active_vertices * self.substeps
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1.0e3
+        verts = sum(s.req.vertex_count for s in chosen)
+        self.peak_vertices = max(self.peak_vertices, verts)
+        self.vertex_substeps += verts * self.substeps
         done = []
-        for s in self.active:
+        for s in chosen:
             if s.cache is not None:
                 s.cache.add_sample(s.solver.download_positions())
             s.frame += 1
             s.frame_ms.append(ms)
+            s.frame_gap_ms.append((t1 - s.last_done) * 1.0e3)
+            self.frame_log.append((s.req.vertex_count, ms, s.frame_gap_ms[-1]))
+            s.last_done = t1
+            s.last_tick = self.ticks
             if s.frame >= s.req.frames:
                 done.append(s)
+        self.ticks += 1
         if done:
             self._batch.close()
             self._batch = None
-            self._batch_stale = True
+            self._batch_members = []
             for s in done:
                 self._retire(s, now)
         return True
@@ -276,51 +365,67 @@ class ClothServer:
         pos = s.solver.download_positions()
         path = s.cache.commit() if s.cache is not None else None
         s.solver.close()
-        self.active.remove(s)
+        self.resident.remove(s)
         self.used_bytes -= s.device_bytes
-        self.active_vertices -= s.req.vertex_count
-        self.results[s.req.sid] = SessionResult(s.req.sid, s.req.vertex_count, s.frame, pos, s.frame_ms, s.req.arrival,
-                                                s.admitted, now, s.device_bytes, path)
+        self.results[s.req.sid] = SessionResult(s.req.sid, s.req.vertex_count, s.frame, pos, s.frame_ms, s.frame_gap_ms,
+                                                s.req.arrival, s.admitted, now, s.device_bytes, path)
 
     def close(self) -> None:
         if self._batch is not None:
             self._batch.close()
             self._batch = None
-        for s in list(self.active):
+        if self._pool is not None:
+            self._collect_built(0.0, wait=True)
+        for s in list(self.resident):
             if s.cache is not None:       # unfinished: the reference drops an uncommitted cache
                 s.cache.discard()
             s.solver.close()
-        self.active.clear()
+        self.resident.clear()
 
     # -- driver
     def run(self, requests, *, clock: str = "wall", max_seconds: float | None = None, max_ticks: int | None = None) -> ServeReport:
         """Serves `requests` until all finish or a limit is reached.
 
         clock "wall": arrivals are seconds after the start; "tick": arrivals
-        are tick indices (deterministic, for tests)."""
+        are tick indices and handles are built in the tick loop
+        (deterministic, for tests)."""
         for r in requests:
             self.submit(r)
+        threaded = clock == "wall" and self.builders > 0
+        if threaded:
+            from concurrent.futures import ThreadPoolExecutor
+            self._pool = ThreadPoolExecutor(max_workers=self.builders)
+        graph_before = nb.set_graph_mode(self.lib, False)
         start = time.perf_counter()
         tick_index = 0
-        while self.pending or self.active:
-            now = (time.perf_counter() - start) if clock == "wall" else float(tick_index)
-            if max_seconds is not None and clock == "wall" and now > max_seconds:
-                break
-            if max_ticks is not None and tick_index >= max_ticks:
-                break
-            ran = self.tick(now)
-            tick_index += 1
-            if not ran and self.pending and clock == "wall":
-                time.sleep(min(max(self.pending[0].arrival - now, 0.0), 0.01))
-        wall = time.perf_counter() - start
-        unfinished = len(self.active) + len(self.pending)
-        self.close()
-        lat = [m for r in self.results.values() for m in r.frame_ms]
+        try:
+            while self.pending or self.building or self.resident:
+                now = (time.perf_counter() - start) if clock == "wall" else float(tick_index)
+                if max_seconds is not None and clock == "wall" and now > max_seconds:
+                    break
+                if max_ticks is not None and tick_index >= max_ticks:
+                    break
+                ran = self.tick(now)
+                tick_index += 1
+                if not ran and clock == "wall":
+                    time.sleep(0.001 if self.building else min(max((self.pending[0].arrival - now) if self.pending else 0.0, 0.0), 0.01))
+            wall = time.perf_counter() - start
+            unfinished = len(self.resident) + len(self.building) + len(self.pending)
+        finally:
+            self.close()
+            if self._pool is not None:
+                self._pool.shutdown(wait=True)
+                self._pool = None
+            nb.set_graph_mode(self.lib, graph_before)
+        # every frame served, of finished and unfinished sessions alike
+        log = np.asarray(self.frame_log, np.float64).reshape(-1, 3)
+        tenk = (log[:, 0] >= TEN_K[0]) & (log[:, 0] <= TEN_K[1])
         waits = [r.admitted - r.arrival for r in self.results.values()]
-        return ServeReport(dict(self.results), self.ticks, wall, self.vertex_substeps,
-                           float(np.percentile(lat, 50)) if lat else 0.0, float(np.percentile(lat, 99)) if lat else 0.0,
-                           self.peak_bytes, self.peak_vertices, self.peak_sessions,
-                           float(np.mean(waits)) if waits else 0.0, unfinished)
+        p50, p99 = _pcts(log[:, 1])
+        n10k = len({r.sid for r in self.results.values() if TEN_K[0] <= r.vertex_count <= TEN_K[1]})
+        return ServeReport(dict(self.results), self.ticks, wall, self.vertex_substeps, p50, p99, self.peak_bytes,
+                           self.peak_vertices, self.peak_sessions, float(np.mean(waits)) if waits else 0.0, unfinished,
+                           self.budget_waits, _pcts(log[tenk, 1]), _pcts(log[:, 2]), _pcts(log[tenk, 2]), n10k)
 
 
 def run_alone(req: SessionRequest, substeps: int, iterations: int, static_triangles=None, lib=None) -> np.ndarray:

@@ -7,7 +7,8 @@
   on it) for 32 frames, past the layers' first contact (about frame 8) and
   the box contact (about frame 18).  The batched HIP form, the per-handle
   HIP form and the oracle are compared bit for bit: positions every 8
-  frames, the dynamic-contact and SDF counters every frame.  Then all 64
+  frames, the dynamic-contact and SDF counters (the batch's read back with
+  its step) every frame.  Then all 64
   cloths for 12 frames: batched equals per-handle bit for bit, all finite.
 * The 10k-vertex strict self-collision cloth (100x100, 20 substeps x 5
   iterations, sphere, box SDF): frames where the oracle stays within a few
@@ -51,15 +52,16 @@ def test_config3_stacks_vs_oracle(nb, hip, oracle, scenes):
     print(f"config 3 stacks {STACKS}: {len(picked)} cloths, {sum(s.vertex_count for s in ref)} vertices", flush=True)
     for f in range(frames):
         cross_b.refresh(device=True)
-        batch.step(sc.substeps, sc.iterations)
+        batch.step(sc.substeps, sc.iterations, diagnostics=True)     # counters of every member read back
         cross_o.refresh(device=True)
         cross_r.refresh(device=True)
-        for g, r in zip(one, ref):
+        for b, g, r in zip(bat, one, ref):
             g.step(sc.substeps, sc.iterations, diagnostics=True, synchronize=False)
             r.step(sc.substeps, sc.iterations, diagnostics=True, synchronize=True)
-            dg, dr = g.diagnostics(), r.diagnostics()
+            db, dg, dr = b.diagnostics(), g.diagnostics(), r.diagnostics()
             for k in DYN_COUNTERS:
                 assert getattr(dg, k) == getattr(dr, k), (f, k, getattr(dg, k), getattr(dr, k))
+                assert getattr(db, k) == getattr(dr, k), ("batched", f, k, getattr(db, k), getattr(dr, k))
                 totals[k] += getattr(dr, k)
         if (f + 1) % 8 == 0:
             nb.synchronize(hip)

@@ -63,7 +63,7 @@ def test_stress_case_reduced_parity(nb, hip, oracle, scenes):
     """Balloon (subdivision 3, 642 vertices) + 24 x 24 twisted towel: strict
     self collision, PCG at strength 1, pressure, turning pins; three frames of
     20 x 10, bit-exact against the oracle, alone and batched."""
-    sc = scenes.config4(subdivisions=3, towel_n=24, frames=3)
+    sc = scenes.config5(subdivisions=3, towel_n=24, frames=3)
     _, hip_alone = S.run_stress(nb, hip, scenes, sc, 3, batched=False)
     _, hip_batch = S.run_stress(nb, hip, scenes, sc, 3, batched=True)
     _, ora = S.run_stress(nb, oracle, scenes, sc, 3, batched=False)
@@ -84,7 +84,7 @@ def test_stress_case_full_size(nb, hip, scenes):
     1.2 against the stretch caps doubles the balloon's radius within two
     frames; the reduced case above pins those dynamics against the oracle.)"""
     import time
-    sc = scenes.config4(frames=2)
+    sc = scenes.config5(frames=2)
     t0 = time.perf_counter()
     solvers, batched = S.run_stress(nb, hip, scenes, sc, 2, batched=True)
     t_batch = time.perf_counter() - t0

@@ -58,7 +58,7 @@ def test_stress_case_reduced_on_oracle(nb, oracle, scenes):
     sit on their turning targets, batched and alone agree, everything stays
     finite and the balloon keeps its size (pressure against the hard-stretch
     caps makes it pulse by about 15 %)."""
-    sc = scenes.config4(subdivisions=2, towel_n=12, frames=2)
+    sc = scenes.config5(subdivisions=2, towel_n=12, frames=2)
     solvers, alone = S.run_stress(nb, oracle, scenes, sc, 2, batched=False)
     _, batched = S.run_stress(nb, oracle, scenes, sc, 2, batched=True)
     for a, b in zip(alone, batched):

@@ -45,12 +45,39 @@ def test_budget_admission_and_mix_equals_alone(serving, scenes, oracle):
 
 def test_vertex_capacity_and_slots(serving, scenes, oracle):
     reqs = small_requests(serving, scenes, sides=[6, 6, 6], frames=[2, 2, 2], arrivals=[0, 0, 0])
+    # all three resident, two stepped per tick (72 vertices): 3 ticks for 6 frames
     srv = serving.ClothServer(hbm_budget_bytes=1 << 40, substeps=1, iterations=1, max_batch_vertices=72, lib=oracle)
     rep = srv.run(reqs, clock="tick")
-    assert rep.peak_active_vertices <= 72 and rep.peak_sessions == 2 and len(rep.results) == 3
+    assert rep.peak_active_vertices == 72 and rep.peak_sessions == 3 and len(rep.results) == 3 and rep.ticks == 3
     srv = serving.ClothServer(hbm_budget_bytes=1 << 40, substeps=1, iterations=1, max_sessions=1, lib=oracle)
     rep = srv.run(reqs, clock="tick")
     assert rep.peak_sessions == 1 and rep.ticks == 6
+
+
+def test_sessions_take_turns_under_the_tick_capacity(serving, scenes, oracle):
+    """More resident work than a tick takes: the least recently stepped go
+    first, every session still gets all its frames, results equal alone, and
+    a waiting session's frame interval exceeds its step latency."""
+    reqs = small_requests(serving, scenes, sides=[6, 7, 5, 6], frames=[4, 3, 5, 2], arrivals=[0, 0, 0, 1])
+    srv = serving.ClothServer(hbm_budget_bytes=1 << 40, substeps=2, iterations=2, max_batch_vertices=60, lib=oracle)
+    rep = srv.run(reqs, clock="tick")
+    assert rep.unfinished == 0 and rep.peak_sessions == 4 and rep.peak_active_vertices <= 60
+    for r in reqs:
+        got = rep.results[r.sid]
+        assert got.frames == r.frames and len(got.frame_gap_ms) == r.frames
+        assert np.array_equal(got.positions, serving.run_alone(r, 2, 2, lib=oracle)), r.sid
+    assert any(g > m for r in rep.results.values() for g, m in zip(r.frame_gap_ms, r.frame_ms))
+
+
+def test_builder_threads_equal_inline_builds(serving, scenes, oracle):
+    """Handles created on builder threads beside the tick loop (wall clock):
+    every session's result equals the session alone."""
+    reqs = small_requests(serving, scenes, sides=[6, 8, 7, 5, 6], frames=[3, 2, 4, 2, 3], arrivals=[0, 0, 0.01, 0.02, 0.02])
+    srv = serving.ClothServer(hbm_budget_bytes=1 << 40, substeps=2, iterations=2, lib=oracle, builders=3)
+    rep = srv.run(reqs, clock="wall", max_seconds=60)
+    assert rep.unfinished == 0 and len(rep.results) == len(reqs)
+    for r in reqs:
+        assert np.array_equal(rep.results[r.sid].positions, serving.run_alone(r, 2, 2, lib=oracle)), r.sid
 
 
 def test_oversized_session_is_rejected(serving, scenes, oracle):
