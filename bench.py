@@ -406,13 +406,20 @@ def main():
     restart(members, scene)
     lat_b = latencies(lambda: run_batched(batch, scene, 1, cross_b), lib, steps)
     # the same frames once more with every launch timed by its own dispatch
-    # and every member's counters read back after each frame
+    restart(members, scene)
+    stats = kernel_stats(lib, lambda: run_batched(batch, scene, steps, cross_b))
+    # and once more with every member's counters read back after each frame;
+    # at five points of the run the fused pass is timed back to back (the
+    # measurement restores the state, so the frames are those of the timed run)
     restart(members, scene)
     counters = {"dynamic_triangle_candidate_count": 0, "dynamic_particle_candidate_count": 0, "fast_triangle_pair_contacts": 0,
                 "dynamic_particle_contacts": 0, "static_sdf_contact_count": 0, "scene_entries": 0}
-    stats = kernel_stats(lib, lambda: run_batched(batch, scene, steps, cross_b, diagnostics=True, counters=counters))
-    # back-to-back fused passes on the final state (dispatch gaps included)
-    avg_ms = batch.time_pass(64)
+    probes, done = [], 0
+    for at in sorted({0, steps // 4, steps // 2, (3 * steps) // 4, steps}):
+        run_batched(batch, scene, at - done, cross_b, diagnostics=True, counters=counters)
+        done = at
+        probes.append(batch.time_pass(32))
+    avg_ms = float(np.mean(probes))
     kernels = {}
     for name, rec in stats.items():
         per = rec["total_ms"] / max(rec["launches"], 1)
@@ -428,8 +435,7 @@ def main():
                           "avg_launch_us": k["avg_us"], "achieved": achieved, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS})
     vp = kernels.get("vertex_pass_stretch")
     bytes_per_launch = pass_bytes_per_vertex(scene) * scene.vertex_count
-    vp_ms = vp["total_ms"] / vp["launches"] if vp else avg_ms
-    achieved = bytes_per_launch / (vp_ms * 1e-3) / 1e9
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     contacts = (counters["fast_triangle_pair_contacts"] + counters["dynamic_particle_contacts"]) / steps
     value = steps * work_per_frame / t_batch
     plain_value = steps * work_per_frame / t_plain
@@ -476,8 +482,9 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(scene.name),
             "algorithmic_bytes_per_launch": bytes_per_launch,
-            "avg_launch_ms": vp_ms,
-            "avg_launch_ms_back_to_back_final_state": avg_ms,
+            "avg_launch_ms": avg_ms,
+            "avg_launch_ms_at": {f"frame_{a}": p for a, p in zip(sorted({0, steps // 4, steps // 2, (3 * steps) // 4, steps}), probes)},
+            "avg_launch_ms_dispatch_stamped": vp["total_ms"] / vp["launches"] if vp else None,
             "other_kernels": roof_more,
         },
     }

@@ -58,6 +58,11 @@ SSBL_API int ssbl_ext_set_static_sdf(void* handle, const int* dims, const float*
  * produces.  The batch refers to the handles; it does not own them. */
 SSBL_API void* ssbl_ext_create_batch(void* const* handles, int count);
 SSBL_API int ssbl_ext_destroy_batch(void* batch);
+/* Points an existing batch at new members (same rules as create): its device
+ * tables and pinned staging are reused, grown only when needed, and its
+ * captured graphs dropped.  Waits for the library's queued work.  For loops
+ * whose set of stepped handles changes from step to step (serving). */
+SSBL_API int ssbl_ext_batch_set_members(void* batch, void* const* handles, int count);
 SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int force_sync);
 /* The same step with every member's diagnostics counted and read back when
  * fetch_diagnostics is nonzero (ssbl_get_diagnostics on each handle
@@ -67,8 +72,9 @@ SSBL_API int ssbl_ext_step_batch_ex(void* batch, int substeps, int iterations, i
 
 /* Runs `launches` (rounded up to even) back-to-back fused constraint passes of
  * the batch (stretch gather, pins, analytic and SDF contacts) between two
- * events on the library stream and writes their mean duration in ms.  Advances
- * the members' state by that many Jacobi iterations: a measurement call. */
+ * events on the library stream and writes their mean duration in ms.  The
+ * members' state is copied aside first and restored afterwards, so the call
+ * leaves it unchanged: a measurement call (it allocates the copy). */
 SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms);
 
 /* Global dynamic scene from cloth handles, gathered on the device: particle k

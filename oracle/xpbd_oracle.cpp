@@ -3317,6 +3317,12 @@ SSBL_API void* ssbl_ext_create_batch(void* const* handles, int count) {
     return b;
 }
 SSBL_API int ssbl_ext_destroy_batch(void* batch) { delete static_cast<Batch*>(batch); return 1; }
+SSBL_API int ssbl_ext_batch_set_members(void* batch, void* const* handles, int count) {
+    g_error.clear();
+    if (!batch || !handles || count <= 0) return fail("invalid batch member update") ? 1 : 0;
+    static_cast<Batch*>(batch)->handles.assign(handles, handles + count);
+    return 1;
+}
 SSBL_API int ssbl_ext_step_batch(void* batch, int substeps, int iterations, int force_sync) {
     g_error.clear();
     if (!batch) return fail("invalid batch") ? 1 : 0;

@@ -126,6 +126,7 @@ EXT_SIGNATURES = {
     "ssbl_ext_set_static_sdf": (_i, [_vp, _ip, _fp, _fp, _i]),
     "ssbl_ext_create_batch": (_vp, [ctypes.POINTER(_vp), _i]),
     "ssbl_ext_destroy_batch": (_i, [_vp]),
+    "ssbl_ext_batch_set_members": (_i, [_vp, ctypes.POINTER(_vp), _i]),
     "ssbl_ext_step_batch": (_i, [_vp, _i, _i, _i]),
     "ssbl_ext_step_batch_ex": (_i, [_vp, _i, _i, _i, _i]),
     "ssbl_ext_time_batch_pass": (_i, [_vp, _i, _fp]),

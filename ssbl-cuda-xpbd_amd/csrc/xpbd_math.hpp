@@ -164,4 +164,13 @@ __device__ __forceinline__ void wave_count(unsigned long long* counts, int slot,
     }
 }
 
+// Wave64 maximum of non-negative floats into *dst (compared as their bit
+// patterns, which order non-negative floats), one atomic per wavefront.
+// Every lane of the wave must call it.
+__device__ __forceinline__ void wave_max_nonneg(float* dst, float v) {
+    int m = __float_as_int(fmaxf(v, 0.0f));
+    for (int d = 32; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63u) == 0u && m > 0) atomicMax(reinterpret_cast<int*>(dst), m);
+}
+
 }  // namespace ssbl

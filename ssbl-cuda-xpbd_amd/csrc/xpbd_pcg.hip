@@ -291,6 +291,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const ClothView* __restric
         if (!isfinite(alpha)) status = 4;
     }
     __int128 term = 0;
+    bool clamped = false;
+    float moved = 0.0f;
     if (status == 0 && v < tile.end) {
         float4* P = cur(c, parity);
         const float4 p4 = P[v];
@@ -312,15 +314,19 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const ClothView* __restric
             const float l = len3(delta);
             if (l > pv.wb_limit) {
                 delta = mul(delta, pv.wb_limit / fmaxf(l, kEps));
-                atomicAdd(&c.counts[kCountHardFallbacks], 1ull);
+                clamped = true;
             }
             P[v] = with_w(f3(p4) + delta, p4.w);
             const float4 q = c.prev[v];
             c.prev[v] = with_w(f3(q) + mul(delta, 0.08f), 0.0f);
             const float dl = len3(delta);
-            if (dl > 0.0f) atomicMax(reinterpret_cast<int*>(pv.max_delta), __float_as_int(dl));
+            if (dl > 0.0f) moved = dl;
         }
     }
+    // the clamp count and the largest step (both order-free) with one atomic
+    // per wave instead of one per vertex on a single address
+    wave_count(c.counts, kCountHardFallbacks, clamped);
+    wave_max_nonneg(pv.max_delta, moved);
     // every workgroup of the cloth derives the same status; its first tile's records it
     if (tile.begin == 0 && threadIdx.x == 0) {
         pv.sc->status = status;
@@ -626,14 +632,15 @@ bool PcgPath::use_scratch(void* base, size_t bytes) {
 }
 
 // ------------------------------------------------------------- groups
+// Grow-only: a group rebuilt for new members reuses its arrays when they fit.
 template <class T> bool PcgGroup::Buf<T>::upload(const std::vector<T>& h, const char* what) {
-    device_free(p, sizeof(T) * n);
-    p = nullptr;
-    n = 0;
     if (h.empty()) return true;
-    p = static_cast<T*>(device_alloc(sizeof(T) * h.size(), what));
-    if (!p) return false;
-    n = h.size();
+    if (h.size() > n) {
+        device_free(p, sizeof(T) * n);
+        p = static_cast<T*>(device_alloc(sizeof(T) * h.size(), what));
+        n = p ? h.size() : 0;
+        if (!p) return false;
+    }
     return hip_ok(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice), what);
 }
 

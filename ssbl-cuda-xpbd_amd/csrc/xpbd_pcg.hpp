@@ -106,7 +106,7 @@ private:
         Buf(Buf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
         Buf& operator=(Buf&& o) noexcept { std::swap(p, o.p); std::swap(n, o.n); return *this; }
         ~Buf() { device_free(p, sizeof(T) * n); }
-        bool upload(const std::vector<T>& h, const char* what);
+        bool upload(const std::vector<T>& h, const char* what);   // grow-only
     };
     // the tiles of the cloths that polish, coloured or not
     struct Polishing {

@@ -840,29 +840,31 @@ __global__ void k_self_cache_counts(unsigned long long* counts, unsigned long lo
 // next pass.
 __global__ __launch_bounds__(256) void k_self_apply(const ClothView* __restrict__ cv, SelfView sv, int parity) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= sv.nv) return;
     const ClothView& c = *cv;
-    const int w = sv.accw[i];
-    if (w <= 0) return;
-    float4* P = parity ? c.pos1 : c.pos0;
-    const double inv = 1.0 / kFixedScale;
-    const F3 sum = f3(static_cast<float>(static_cast<double>(static_cast<long long>(sv.acc[3 * i + 0])) * inv),
-                      static_cast<float>(static_cast<double>(static_cast<long long>(sv.acc[3 * i + 1])) * inv),
-                      static_cast<float>(static_cast<double>(static_cast<long long>(sv.acc[3 * i + 2])) * inv));
-    sv.acc[3 * i + 0] = 0ull;
-    sv.acc[3 * i + 1] = 0ull;
-    sv.acc[3 * i + 2] = 0ull;
-    sv.accw[i] = 0;
-    F3 avg = mul(sum, 1.0f / static_cast<float>(w));
-    const float max_delta = fmaxf(sv.radius * 0.35f, 1.0e-4f);
-    const float dl = len3(avg);
-    if (dl > max_delta) avg = mul(avg, max_delta / dl);
-    const float alen = len3(avg);
-    if (alen > 0.0f) atomicMax(reinterpret_cast<int*>(c.max_smoothed), __float_as_int(alen));
-    const float4 p = P[i];
-    P[i] = with_w(f3(p) + avg, p.w);
-    const float4 q = c.prev[i];
-    c.prev[i] = with_w(f3(q) + mul(avg, 0.55f), 0.0f);
+    float alen = 0.0f;
+    const int w = i < sv.nv ? sv.accw[i] : 0;
+    if (w > 0) {
+        float4* P = parity ? c.pos1 : c.pos0;
+        const double inv = 1.0 / kFixedScale;
+        const F3 sum = f3(static_cast<float>(static_cast<double>(static_cast<long long>(sv.acc[3 * i + 0])) * inv),
+                          static_cast<float>(static_cast<double>(static_cast<long long>(sv.acc[3 * i + 1])) * inv),
+                          static_cast<float>(static_cast<double>(static_cast<long long>(sv.acc[3 * i + 2])) * inv));
+        sv.acc[3 * i + 0] = 0ull;
+        sv.acc[3 * i + 1] = 0ull;
+        sv.acc[3 * i + 2] = 0ull;
+        sv.accw[i] = 0;
+        F3 avg = mul(sum, 1.0f / static_cast<float>(w));
+        const float max_delta = fmaxf(sv.radius * 0.35f, 1.0e-4f);
+        const float dl = len3(avg);
+        if (dl > max_delta) avg = mul(avg, max_delta / dl);
+        alen = len3(avg);
+        const float4 p = P[i];
+        P[i] = with_w(f3(p) + avg, p.w);
+        const float4 q = c.prev[i];
+        c.prev[i] = with_w(f3(q) + mul(avg, 0.55f), 0.0f);
+    }
+    // the largest averaged delta (order-free): one atomic per wave
+    wave_max_nonneg(c.max_smoothed, alen);
 }
 
 }  // namespace

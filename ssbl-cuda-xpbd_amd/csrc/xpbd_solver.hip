@@ -147,6 +147,13 @@ template <class T> struct Dev {
         if (h.empty()) return true;
         return hip_ok(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice), what);
     }
+    // Grow-only forms: reallocate only when more elements are needed.
+    bool reserve(size_t count, const char* what) { return count <= n || alloc(count, what); }
+    bool upload_grow(const std::vector<T>& h, const char* what) {
+        if (!reserve(h.size(), what)) return false;
+        if (h.empty()) return true;
+        return hip_ok(hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice), what);
+    }
 };
 
 // Pinned staging ring for per-step parameter uploads that are ordered on the
@@ -221,6 +228,16 @@ struct StepGraph {
     ~StepGraph() {
         for (Slot& sl : slots)
             if (sl.exec) hipGraphExecDestroy(sl.exec);
+    }
+    // Drops every kept graph (the owner's launch tables changed); waits for
+    // the stream first, since a dropped graph may still run.
+    bool clear(hipStream_t st) {
+        if (slots.empty()) return true;
+        if (!hip_ok(hipStreamSynchronize(st), "step graph reset")) return false;
+        for (Slot& sl : slots)
+            if (sl.exec) hipGraphExecDestroy(sl.exec);
+        slots.clear();
+        return true;
     }
     template <class F>
     bool run(hipStream_t st, const std::vector<long long>& k, F&& record) {
@@ -1459,15 +1476,24 @@ static bool build_batch(Batch& b) {
         Solver* s = b.members[m];
         if (s->pcg.enabled()) by_cadence[s->pcg_cadence].push_back(PcgMember{static_cast<int>(m), &s->pcg, s->nv});
     }
-    b.pcg_groups.clear();
-    for (auto& kv : by_cadence) {
-        b.pcg_groups.emplace_back();
-        if (!b.pcg_groups.back().build(kv.second, kv.first)) return false;
-    }
-    return b.d_views.alloc(b.members.size(), "batch view allocation") && b.d_tiles.upload(tiles, "batch tile allocation") &&
-           b.d_bend_segs.upload(b.h_bend_segs, "batch bend allocation") &&
-           b.d_stretch_segs.upload(b.h_stretch_segs, "batch stretch allocation") &&
+    b.pcg_groups.resize(by_cadence.size());   // existing groups keep their (grow-only) tables
+    size_t g = 0;
+    for (auto& kv : by_cadence)
+        if (!b.pcg_groups[g++].build(kv.second, kv.first)) return false;
+    // grow-only: a batch given new members keeps its device tables and staging
+    return b.d_views.reserve(b.members.size(), "batch view allocation") && b.d_tiles.upload_grow(tiles, "batch tile allocation") &&
+           b.d_bend_segs.upload_grow(b.h_bend_segs, "batch bend allocation") &&
+           b.d_stretch_segs.upload_grow(b.h_stretch_segs, "batch stretch allocation") &&
            b.staging.reserve(sizeof(ClothView) * b.members.size());
+}
+
+// New members for an existing batch: its tables are rebuilt in place (no
+// allocation unless they grow) and its graphs dropped.  Waits for the
+// library's queued work, which may still read the old tables.
+static bool retarget_batch(Batch& b, const std::vector<Solver*>& members) {
+    if (!sync_stream() || !b.graph.clear(lib_stream())) return false;
+    b.members = members;
+    return build_batch(b);
 }
 
 // fetch: count diagnostics and read every member's counters afterwards (as
@@ -1522,8 +1548,29 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, b
 // analytic and SDF contacts -- between two events on the library stream, and
 // returns the mean duration.  Consecutive launches on one stream: the mean is
 // the kernel time plus the dispatch gap, with no per-launch instrumentation.
+// The members' positions, previous positions and velocities are copied aside
+// before and back afterwards, so the measurement leaves the state as it was.
 static bool time_batch_pass(Batch& b, int launches, float* avg_ms) {
     launches = std::max(2, launches + (launches & 1));
+    size_t total = 0;
+    for (Solver* s : b.members) total += 4 * static_cast<size_t>(s->nv);
+    Dev<float4> saved;
+    if (!sync_stream() || !saved.alloc(std::max<size_t>(total, 1), "timing snapshot")) return false;
+    auto copy_state = [&](bool save) {
+        size_t at = 0;
+        for (Solver* s : b.members) {
+            for (float4* a : {s->pos0.p, s->pos1.p, s->prev.p, s->vel.p}) {
+                const size_t n = static_cast<size_t>(s->nv);
+                if (n == 0) continue;
+                if (!hip_ok(hipMemcpyAsync(save ? saved.p + at : a, save ? a : saved.p + at, sizeof(float4) * n, hipMemcpyDeviceToDevice,
+                                           lib_stream()), "timing snapshot"))
+                    return false;
+                at += n;
+            }
+        }
+        return true;
+    };
+    if (!copy_state(true)) return false;
     bool pins = false, edges = false;
     for (Solver* s : b.members) {
         pins |= !s->pins.empty();
@@ -1540,6 +1587,7 @@ static bool time_batch_pass(Batch& b, int launches, float* avg_ms) {
     ok = ok && hip_ok(hipEventElapsedTime(&ms, e0, e1), "timing event");
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    ok = ok && copy_state(false) && sync_stream();
     if (ok) *avg_ms = ms / static_cast<float>(launches);
     return ok;
 }
@@ -1882,6 +1930,16 @@ SSBL_API void* ssbl_ext_create_batch(void* const* handles, int count) {
     }
     if (!build_batch(*b)) { const std::string e = g_error; delete b; g_error = e; return nullptr; }
     return b;
+}
+SSBL_API int ssbl_ext_batch_set_members(void* batch, void* const* handles, int count) {
+    g_error.clear();
+    if (!batch || !handles || count <= 0) return fail("invalid batch member update") ? 1 : 0;
+    std::vector<Solver*> members;
+    for (int k = 0; k < count; ++k) {
+        if (!handles[k]) return fail("invalid batch member") ? 1 : 0;
+        members.push_back(static_cast<Solver*>(handles[k]));
+    }
+    return retarget_batch(*static_cast<Batch*>(batch), members) ? 1 : 0;
 }
 SSBL_API int ssbl_ext_destroy_batch(void* batch) {
     g_error.clear();

@@ -465,6 +465,15 @@ class SolverBatch:
         if not self._handle:
             raise NativeSolverError(last_error(self._lib))
 
+    def set_members(self, solvers: list[NativeXpbdSolver]) -> None:
+        """Points the batch at new members, reusing its device tables."""
+        if not solvers:
+            raise ValueError("empty batch")
+        handles = (ctypes.c_void_p * len(solvers))(*[s.handle for s in solvers])
+        if not self._lib.ssbl_ext_batch_set_members(self._handle, handles, len(solvers)):
+            raise NativeSolverError(last_error(self._lib))
+        self.solvers = list(solvers)
+
     def step(self, substeps: int, iterations: int, synchronize: bool = False, diagnostics: bool = False) -> None:
         """One batched step; with diagnostics, every member's counters are
         read back afterwards (each solver's diagnostics() then holds them)."""

@@ -329,10 +329,11 @@ class ClothServer:
         t0 = time.perf_counter()
         for s in chosen:
             self._frame_inputs(s)
-        if self._batch is None or self._batch_members != [id(s) for s in chosen]:
-            if self._batch is not None:
-                self._batch.close()
+        if self._batch is None:
             self._batch = nb.SolverBatch([s.solver for s in chosen])
+            self._batch_members = [id(s) for s in chosen]
+        elif self._batch_members != [id(s) for s in chosen]:
+            self._batch.set_members([s.solver for s in chosen])   # the batch's tables are reused
             self._batch_members = [id(s) for s in chosen]
         self._batch.step(self.substeps, self.iterations, synchronize=True)
         t1 = time.perf_counter()
@@ -354,9 +355,7 @@ class ClothServer:
                 done.append(s)
         self.ticks += 1
         if done:
-            self._batch.close()
-            self._batch = None
-            self._batch_members = []
+            self._batch_members = []     # the next tick points the batch at its new members
             for s in done:
                 self._retire(s, now)
         return True

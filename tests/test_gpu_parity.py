@@ -5,6 +5,8 @@ relative or 1e-5 m.  The vertex-local path is built so that it reproduces the
 oracle's FP32 operation order; the printed max difference shows how close it
 is (0.0 means bit-exact).
 """
+from dataclasses import replace
+
 import numpy as np
 import pytest
 
@@ -134,3 +136,29 @@ def test_batch_equals_single(nb, hip, oracle, scenes):
     for _ in range(sc.frames):
         ref.step(sc.substeps, sc.iterations)
     assert state_diff(members[0], ref) <= TOL
+
+
+def test_batch_set_members_equals_fresh_batch(nb, hip, scenes):
+    """A batch pointed at new members (a serving loop's changing set of
+    stepped sessions) steps them exactly as a batch created for them and as
+    each handle alone: first cloths 0-3, then 2-5 (two new, one PCG), then 0 and 5."""
+    sc = scenes.config3(seed=4, count=6, frames=3, cross_cloth=False)
+    sc.options[3] = replace(sc.options[3], stretch_optimization_enabled=True, stretch_optimization_strength=1.0)
+    sc.options[5] = replace(sc.options[5], stretch_optimization_enabled=True, stretch_optimization_strength=0.4)
+    make = lambda: [nb.NativeXpbdSolver(c, o, sc.static_triangles, lib=hip) for c, o in zip(sc.cloths, sc.options)]  # noqa: E731
+    a, b, c = make(), make(), make()
+    moving = nb.SolverBatch([a[k] for k in (0, 1, 2, 3)])
+    for members in ((0, 1, 2, 3), (2, 3, 4, 5), (0, 5)):
+        moving.set_members([a[k] for k in members])
+        fresh = nb.SolverBatch([b[k] for k in members])
+        for _ in range(2):
+            moving.step(sc.substeps, sc.iterations)
+            fresh.step(sc.substeps, sc.iterations)
+            for k in members:
+                c[k].step(sc.substeps, sc.iterations, diagnostics=False, synchronize=False)
+        fresh.close()
+        nb.synchronize(hip)
+        for k in members:
+            pa, pb, pc = a[k].download_positions(), b[k].download_positions(), c[k].download_positions()
+            assert np.array_equal(pa, pb) and np.array_equal(pb, pc), (members, k)
+    moving.close()
