@@ -332,6 +332,44 @@ def serving_line(reqs, window_s: float, max_s: float, budget: int, capacity: int
     return out
 
 
+def step_only(which: str, scene, args, lib) -> None:
+    """Steps one workload without timing it, for a profiler or counter pass."""
+    if which == "serving":
+        from ssbl_xpbd_amd import serving
+        reqs = serving.served_requests(seed=args.seed, rate=200.0, window_s=args.serving_window)
+        out = serving_line(reqs, args.serving_window, args.serving_max_s, 200 * 10**9, args.serving_capacity,
+                           args.serving_builders, lib)
+        print(json.dumps({"only": which, "frames_served": out["frames_served"]}), flush=True)
+        return
+    if which == "batched":
+        sc = scene
+    elif which == "selfcol":
+        sc = scenes.config2(seed=args.seed)
+    else:
+        sc = scenes.config5(frames=max(args.warmup + args.steps, 1))
+    frames = args.warmup + (args.steps if args.steps > 0 else sc.frames)
+    solvers = create(sc)
+    batch = nb.SolverBatch(solvers) if len(solvers) > 1 else None
+    cross = cross_scene(solvers, sc) if which == "batched" else None
+    towel = sc.cloths[1] if which == "stress" else None
+    for f in range(frames):
+        if cross is not None:
+            cross.refresh(device=True)
+        if towel is not None:
+            solvers[1].update_pin_targets(towel.pin_indices, scenes.towel_pin_targets(towel, (f + 1) * sc.options[1].dt),
+                                          towel.pin_weights)
+        if batch is not None:
+            batch.step(sc.substeps, sc.iterations)
+        else:
+            for s in solvers:
+                s.step(sc.substeps, sc.iterations, diagnostics=False, synchronize=False)
+        if f % 10 == 9:
+            nb.synchronize(lib)
+            log(f"{which}: {f + 1} of {frames} frames")
+    nb.synchronize(lib)
+    print(json.dumps({"only": which, "workload": sc.name, "frames": frames}), flush=True)
+
+
 # ------------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -352,8 +390,11 @@ def main():
     ap.add_argument("--serving-small-budget-gb", type=float, default=16.0,
                     help="HBM budget of the second served run, small enough to bind")
     ap.add_argument("--graph", type=int, default=1, help="graph replay of the substep loop (0 = direct launches)")
-    ap.add_argument("--batched-only", action="store_true",
-                    help="only step the batched form (warmup + steps frames) and exit: for counter passes")
+    ap.add_argument("--only", choices=("batched", "selfcol", "stress", "serving"), default=None,
+                    help="only step one workload, untimed, and exit (for profiler and counter passes): the batched "
+                         "64-cloth form, the 10k-vertex self-collision cloth or the stress case for warmup + steps "
+                         "frames, or one served-sessions run")
+    ap.add_argument("--batched-only", action="store_true", help="the same as --only batched")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -378,11 +419,9 @@ def main():
     log(f"workload {scene.name}: {len(scene.cloths)} cloths, {scene.vertex_count} vertices, "
         f"{scene.substeps}x{scene.iterations}, {steps} frames; device {lib.ssbl_ext_backend_name().decode()}")
     if args.batched_only:
-        members = create(scene)
-        batch = nb.SolverBatch(members)
-        run_batched(batch, scene, args.warmup + steps, cross_scene(members, scene))
-        nb.synchronize(lib)
-        print(json.dumps({"batched_only_frames": args.warmup + steps, "workload": scene.name}), flush=True)
+        args.only = "batched"
+    if args.only:
+        step_only(args.only, scene, args, lib)
         return
     plain = create(scene)
     members = create(scene)

@@ -348,17 +348,6 @@ __device__ __forceinline__ unsigned long long depth_key(float pen, unsigned orde
     return pen > 0.0f ? (static_cast<unsigned long long>(__float_as_uint(pen)) << 32) | (0xffffffffu - order) : 0ull;
 }
 
-// Orders a wave's LDS writes before its later LDS reads of other lanes'
-// slots (a wave's LDS operations complete in issue order; this keeps the
-// compiler from moving them across, and waits for those in flight).
-// s_waitcnt operand of gfx9: lgkmcnt (bits 11:8) 0, vmcnt (bits 3:0, 15:14)
-// and expcnt (bits 6:4) at their maxima, i.e. left alone.
-constexpr unsigned kWaitLgkmZero = 0xc07fu;
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_s_waitcnt(kWaitLgkmZero);
-    __builtin_amdgcn_wave_barrier();
-}
 
 __device__ __forceinline__ F3 shfl3(F3 a, int lane) { return f3(__shfl(a.x, lane, 64), __shfl(a.y, lane, 64), __shfl(a.z, lane, 64)); }
 

@@ -164,6 +164,25 @@ __device__ __forceinline__ void wave_count(unsigned long long* counts, int slot,
     }
 }
 
+// Orders a wave's LDS writes before its later LDS reads of other lanes'
+// slots (a wave's LDS operations complete in issue order; this keeps the
+// compiler from moving them across, and waits for those in flight).
+// s_waitcnt operand of gfx9: lgkmcnt (bits 11:8) 0, vmcnt (bits 3:0, 15:14)
+// and expcnt (bits 6:4) at their maxima, i.e. left alone.
+constexpr unsigned kWaitLgkmZero = 0xc07fu;
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_s_waitcnt(kWaitLgkmZero);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Wave64 sum of a per-lane count added to counts[slot] by one lane.  Every
+// lane of the wave must call it.
+__device__ __forceinline__ void wave_add(unsigned long long* counts, int slot, unsigned v) {
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63u) == 0u && v != 0u) atomicAdd(&counts[slot], static_cast<unsigned long long>(v));
+}
+
 // Wave64 maximum of non-negative floats into *dst (compared as their bit
 // patterns, which order non-negative floats), one atomic per wavefront.
 // Every lane of the wave must call it.

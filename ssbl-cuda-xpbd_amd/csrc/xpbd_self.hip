@@ -232,50 +232,61 @@ __device__ inline bool edge_box(const ClothView& c, const SelfView& sv, int pari
     return true;
 }
 
-// One wave per primitive, one lane per cell of its box (at most 64 cells:
-// larger boxes are reduced to one cell by the guard).  kind 0 = triangles,
-// 1 = edges; phase 0 counts, 1 scatters.  Key = primitive * 64 + the cell's
-// index in the box loop (z outer, then y, then x).
+// Half a wave per primitive, one lane per cell of its box and a second round
+// for boxes over 32 cells (at most 64 cells: larger boxes are reduced to one
+// cell by the guard; a box is usually 2-3 cells a side, as the margin is one
+// cell).  kind 0 = triangles, 1 = edges; phase 0 counts, 1 scatters.
+// Key = primitive * 64 + the cell's index in the box loop (z outer, then y,
+// then x).
+constexpr int kPrimLanes = 32;
 __global__ __launch_bounds__(256) void k_prim_hash(const ClothView* __restrict__ cv, SelfView sv, int parity, int kind,
                                                    int phase, int count_diag) {
-    const int k = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int idx = static_cast<int>(threadIdx.x & 63u);
+    const int k = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kPrimLanes);
+    const int lane = static_cast<int>(threadIdx.x % kPrimLanes);
     const int n = kind == 0 ? sv.nt : sv.ne;
     if (k >= n) return;
     int b0[3], b1[3];
     bool guarded = false;
     const bool ok = kind == 0 ? tri_box(*cv, sv, parity, k, b0, b1, &guarded) : edge_box(*cv, sv, parity, k, b0, b1, &guarded);
     if (!ok) return;
-    if (phase == 0 && guarded && count_diag && idx == 0) atomicAdd(&cv->counts[kCountSelfOverflow], 1ull);
+    if (phase == 0 && guarded && count_diag && lane == 0) atomicAdd(&cv->counts[kCountSelfOverflow], 1ull);
     const int sx = b1[0] - b0[0] + 1, sy = b1[1] - b0[1] + 1, sz = b1[2] - b0[2] + 1;
-    if (sx <= 0 || sy <= 0 || sz <= 0 || idx >= sx * sy * sz) return;
-    const int x = b0[0] + idx % sx, y = b0[1] + (idx / sx) % sy, z = b0[2] + idx / (sx * sy);
+    if (sx <= 0 || sy <= 0 || sz <= 0) return;
     const HashView& h = kind == 0 ? sv.th : sv.eh;
-    const int b = static_cast<int>(hash_cell(x, y, z, h.buckets));
-    const int key = k * kMaxPrimCells + idx;
-    if (phase == 0) {
-        h.slot[key] = atomicAdd(&h.count[b], 1);
-    } else {
-        const int j = h.off[b] + h.slot[key];
-        h.raw[j] = key;
-        h.rawb[j] = b;
+    for (int idx = lane; idx < sx * sy * sz; idx += kPrimLanes) {
+        const int x = b0[0] + idx % sx, y = b0[1] + (idx / sx) % sy, z = b0[2] + idx / (sx * sy);
+        const int b = static_cast<int>(hash_cell(x, y, z, h.buckets));
+        const int key = k * kMaxPrimCells + idx;
+        if (phase == 0) {
+            h.slot[key] = atomicAdd(&h.count[b], 1);
+        } else {
+            const int j = h.off[b] + h.slot[key];
+            h.raw[j] = key;
+            h.rawb[j] = b;
+        }
     }
 }
 
 // Every scattered item of a primitive hash finds its rank in its bucket and
 // lands there if it is among the first 32 (the others count as overflow).
+// A grid-stride loop over the items actually scattered (their total is the
+// scan's last offset); every lane runs the same number of rounds.
 __global__ __launch_bounds__(256) void k_hash_rank(HashView h, unsigned long long* counts, int count_diag) {
-    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int total = h.off[h.buckets];
-    bool over = false;
-    if (j < total) {
-        const int b = h.rawb[j], key = h.raw[j];
-        const int base = h.off[b];
-        const int rank = bucket_rank(h.raw, base, h.count[b], key);
-        if (rank < kBucketSlots) h.entries[base + rank] = key;
-        else over = true;
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    const int rounds = (total + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        const int j = r * stride + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+        bool over = false;
+        if (j < total) {
+            const int b = h.rawb[j], key = h.raw[j];
+            const int base = h.off[b];
+            const int rank = bucket_rank(h.raw, base, h.count[b], key);
+            if (rank < kBucketSlots) h.entries[base + rank] = key;
+            else over = true;
+        }
+        if (count_diag) wave_count(counts, kCountSelfOverflow, over);
     }
-    if (count_diag) wave_count(counts, kCountSelfOverflow, over);
 }
 
 // ------------------------------------------------------------- responses
@@ -327,6 +338,204 @@ __device__ inline bool vv_valid(const ClothView& c, const SelfView& sv, const fl
     return dot(cd, cd) < sv.onset * sv.onset || dot(pd, pd) < sv.onset * sv.onset;
 }
 
+// ------------------------------------------------------- pair evaluations
+// One vertex-triangle (A41:4990-5107) or edge-edge (A41:5595-5722) candidate
+// evaluated without side effects: rest = skipped by the rest-distance test,
+// accepted = what the pair test returns (a contact within onset), point = the
+// contact point that keys the fast mode's clusters, and when apply the
+// response delta * w[k] goes to vertex id[k] for each w[k] != 0.
+struct PairEval {
+    bool rest = false, accepted = false, apply = false;
+    int id[4] = {0, 0, 0, 0};
+    float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    F3 delta, point;
+};
+
+__device__ __forceinline__ float contact_push(float r, float onset, float dist) {
+    const float pen = r - dist;
+    if (pen > 0.0f) return fminf(pen * 0.35f + r * 0.01f, r * 0.25f);
+    const float x = 1.0f - fminf(dist / onset, 1.0f);
+    return fminf(x * x * r * 0.06f, r * 0.04f);
+}
+
+__device__ inline PairEval vt_eval(const ClothView& c, const SelfView& sv, const float4* P, int v, int t) {
+    PairEval out;
+    const float r = sv.radius, onset = sv.onset;
+    if (t < 0 || t >= sv.nt || r <= 0.0f || onset <= r) return out;
+    const int4 tr = sv.tris[t];
+    if (!tri_valid(sv, tr)) return out;
+    const int ia = tr.x, ib = tr.y, ic = tr.z;
+    bool rest_near = v == ia || v == ib || v == ic;   // A41:4615-4630
+    if (!rest_near) {
+        const F3 rv = f3(c.rest[v]);
+        const F3 rq = closest_point_on_triangle(rv, f3(c.rest[ia]), f3(c.rest[ib]), f3(c.rest[ic]));
+        const F3 rdd = rv - rq;
+        rest_near = dot(rdd, rdd) < sv.rest_skip * sv.rest_skip;
+    }
+    if (rest_near) {
+        out.rest = true;
+        return out;
+    }
+    const F3 p = f3(P[v]), a = f3(P[ia]), b = f3(P[ib]), cc = f3(P[ic]);
+    const F3 q = closest_point_on_triangle(p, a, b, cc);
+    const F3 d = p - q;
+    const float dsq = dot(d, d);
+    float dist = sqrt_eps(dsq);
+    if (dist >= onset) return out;
+    F3 nn = f3(0.0f, 0.0f, 1.0f);
+    if (dsq > kEps) nn = mul(d, 1.0f / dist);
+    else {
+        nn = cross(b - a, cc - a);
+        const float nsq = dot(nn, nn);
+        if (nsq > kEps) nn = mul(nn, rsqrt_exact(nsq));
+        else {
+            const F3 rv = f3(c.rest[v]);
+            const F3 rdl = rv - closest_point_on_triangle(rv, f3(c.rest[ia]), f3(c.rest[ib]), f3(c.rest[ic]));
+            const float rsq = dot(rdl, rdl);
+            if (rsq > kEps) nn = mul(rdl, rsqrt_exact(rsq));
+        }
+        dist = 0.0f;
+    }
+    out.accepted = true;
+    out.point = q;
+    float wa = 1.0f / 3.0f, wb = wa, wc = wa;
+    triangle_barycentric(q, a, b, cc, &wa, &wb, &wc);
+    const float push = contact_push(r, onset, dist);
+    const float wv = c.rest[v].w, wta = c.rest[ia].w * wa, wtb = c.rest[ib].w * wb, wtc = c.rest[ic].w * wc;
+    const float total = wv + wta + wtb + wtc;
+    if (push <= 0.0f || total <= 0.0f) return out;
+    out.apply = true;
+    out.delta = mul(nn, push / total);
+    out.id[0] = v; out.id[1] = ia; out.id[2] = ib; out.id[3] = ic;
+    out.w[0] = wv > 0.0f ? wv : 0.0f;
+    out.w[1] = wta > 0.0f ? -wta : 0.0f;
+    out.w[2] = wtb > 0.0f ? -wtb : 0.0f;
+    out.w[3] = wtc > 0.0f ? -wtc : 0.0f;
+    return out;
+}
+
+__device__ inline PairEval ee_eval(const ClothView& c, const SelfView& sv, const float4* P, int ea, int eb) {
+    PairEval out;
+    const float r = sv.radius, onset = sv.onset;
+    if (eb <= ea || ea < 0 || eb >= sv.ne || r <= 0.0f || onset <= r) return out;
+    const int2 A = sv.edges[ea], B = sv.edges[eb];
+    if (!edge_valid(sv, A) || !edge_valid(sv, B)) return out;
+    const int a0 = A.x, a1 = A.y, b0 = B.x, b1 = B.y;
+    bool rest_near = a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1;   // A41:5507-5530
+    if (!rest_near) {
+        const F3 r0 = f3(c.rest[a0]), r1 = f3(c.rest[a1]), r2 = f3(c.rest[b0]), r3 = f3(c.rest[b1]);
+        float s = 0.0f, t = 0.0f;
+        closest_segment_parameters(r0, r1, r2, r3, &s, &t);
+        const F3 dd = (r0 + mul(r1 - r0, s)) - (r2 + mul(r3 - r2, t));
+        rest_near = dot(dd, dd) < sv.rest_skip * sv.rest_skip;
+    }
+    if (rest_near) {
+        out.rest = true;
+        return out;
+    }
+    const F3 p0 = f3(P[a0]), p1 = f3(P[a1]), p2 = f3(P[b0]), p3 = f3(P[b1]);
+    float s = 0.0f, t = 0.0f;
+    closest_segment_parameters(p0, p1, p2, p3, &s, &t);
+    const F3 pa = p0 + mul(p1 - p0, s), pb = p2 + mul(p3 - p2, t);
+    const F3 d = pa - pb;
+    const float dsq = dot(d, d);
+    float dist = sqrt_eps(dsq);
+    if (dist >= onset) return out;
+    F3 nn = f3(0.0f, 0.0f, 1.0f);
+    if (dsq > kEps) nn = mul(d, 1.0f / dist);
+    else {
+        nn = cross(p1 - p0, p3 - p2);
+        const float nsq = dot(nn, nn);
+        if (nsq > kEps) nn = mul(nn, rsqrt_exact(nsq));
+        else {
+            const F3 r0 = f3(c.rest[a0]), r1 = f3(c.rest[a1]), r2 = f3(c.rest[b0]), r3 = f3(c.rest[b1]);
+            float rs = 0.0f, rt = 0.0f;
+            closest_segment_parameters(r0, r1, r2, r3, &rs, &rt);
+            const F3 rd = (r0 + mul(r1 - r0, rs)) - (r2 + mul(r3 - r2, rt));
+            const float rsq = dot(rd, rd);
+            if (rsq > kEps) nn = mul(rd, rsqrt_exact(rsq));
+        }
+        dist = 0.0f;
+    }
+    out.accepted = true;
+    out.point = mul(pa + pb, 0.5f);
+    const float push = contact_push(r, onset, dist);
+    const float w0 = 1.0f - s, w1 = s, w2 = 1.0f - t, w3 = t;
+    const float ma0 = c.rest[a0].w, ma1 = c.rest[a1].w, mb0 = c.rest[b0].w, mb1 = c.rest[b1].w;
+    const float wa0 = ma0 * w0, wa1 = ma1 * w1, wb0 = mb0 * w2, wb1 = mb1 * w3;
+    const float total = ma0 * w0 * w0 + ma1 * w1 * w1 + mb0 * w2 * w2 + mb1 * w3 * w3;
+    if (push <= 0.0f || total <= 0.0f) return out;
+    out.apply = true;
+    out.delta = mul(nn, push / total);
+    out.id[0] = a0; out.id[1] = a1; out.id[2] = b0; out.id[3] = b1;
+    out.w[0] = wa0 > 0.0f ? wa0 : 0.0f;
+    out.w[1] = wa1 > 0.0f ? wa1 : 0.0f;
+    out.w[2] = wb0 > 0.0f ? -wb0 : 0.0f;
+    out.w[3] = wb1 > 0.0f ? -wb1 : 0.0f;
+    return out;
+}
+
+__device__ __forceinline__ void apply_eval(const SelfView& sv, const float4* P, const PairEval& e) {
+    for (int k = 0; k < 4; ++k)
+        if (e.w[k] != 0.0f) accumulate(sv, P, e.id[k], mul(e.delta, e.w[k]));
+}
+
+// Vertex-vertex response of vertex s to table entry o (A41:4816-4951): the
+// continuous time of impact of the two thick points over the substep, else
+// the static overlap push, clamped to 0.35 thickness.  Returns whether a
+// response is to be applied (in *resp).
+__device__ inline bool vv_response(const ClothView& c, const float4* P, int s, int o, F3 pc, F3 pp, float thickness, F3* resp_out) {
+    if (thickness <= 0.0f) return false;
+    const F3 qc = f3(P[o]), qp = f3(c.prev[o]);
+    F3 resp = f3(0.0f, 0.0f, 0.0f);
+    bool hit = false;
+    const F3 dref = pp - qp;
+    const float cco = dot(dref, dref) - thickness * thickness;
+    const F3 rel = (pc - pp) - (qc - qp);
+    const float aco = dot(rel, rel), bco = dot(dref, rel);
+    const bool other_pinned = P[o].w == 0.0f;
+    const float scale = other_pinned ? 1.0f : 0.5f;
+    if (cco >= 0.0f && bco < 0.0f && aco > 1.0e-5f) {
+        const float disc = bco * bco - aco * cco;
+        if (disc >= 0.0f) {
+            const float t = (-bco - sqrtf(disc)) / aco;
+            if (t > 0.0f && t <= 1.0f) {
+                const float sv1 = 1.0f - t;
+                const F3 ph = mul(pp, sv1) + mul(pc, t), qh = mul(qp, sv1) + mul(qc, t);
+                F3 nn = ph - qh;
+                const float nsq = dot(nn, nn);
+                if (nsq > kEps) {
+                    nn = mul(nn, rsqrt_exact(nsq));
+                    const float vn = dot((pc - ph) - (qc - qh), nn);
+                    if (vn > 0.0f) { resp = mul(nn, -vn * scale); hit = true; }
+                }
+            }
+        }
+    }
+    if (!hit) {
+        const F3 d = pc - qc;
+        const float dsq = dot(d, d);
+        if (dsq >= thickness * thickness) return false;
+        F3 nn = f3(0.0f, 0.0f, 1.0f);
+        float dist = sqrt_eps(dsq);
+        if (dsq > kEps) nn = mul(d, 1.0f / dist);
+        else {
+            const F3 rd = f3(c.rest[s]) - f3(c.rest[o]);
+            const float rsq = dot(rd, rd);
+            if (rsq > kEps) nn = mul(rd, rsqrt_exact(rsq));
+            dist = 0.0f;
+        }
+        resp = mul(nn, fminf((thickness - dist) * scale, thickness * 0.35f));
+    }
+    if (dot(resp, resp) <= kEps) return false;
+    const float ml = fmaxf(thickness * 0.35f, 1.0e-4f);
+    const float rl = len3(resp);
+    if (rl > ml) resp = mul(resp, ml / rl);
+    *resp_out = resp;
+    return true;
+}
+
+// ----------------------------------------------------- fast mode, per thread
 // Fast-mode contact filters (A41:708-849).  Every contact a thread accepts
 // opens a cluster keyed by a coarse cell of its contact point, its normal
 // direction and its pair type; at most 16 clusters per thread and one contact
@@ -381,11 +590,10 @@ __device__ __forceinline__ void flush_filter_counts(unsigned long long* counts, 
 }
 
 // Vertex part: candidate table (A41:4738-4814) + vertex-vertex response
-// (A41:4816-4951) + vertex-triangle repulsion (A41:5109-5224).  The strict
-// mode builds the table on every pass; the fast mode on the first pass of a
-// substep only (`build`), and later passes read it back.
-template <bool kFast>
-__device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const float4* P, int s, bool count, bool build,
+// (A41:4816-4951) + vertex-triangle repulsion (A41:5109-5224).  The table is
+// built on the first pass of a substep only (`build`); later passes read it
+// back.
+__device__ inline void fast_vertex(const ClothView& c, const SelfView& sv, const float4* P, int s, bool count, bool build,
                                    ClusterSet& cl, FilterCounts& fc) {
     const float cs = sv.cell;
     const F3 ps = f3(P[s]);
@@ -425,129 +633,31 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
         for (unsigned k = 0; k < acc; ++k) {
             const int o = static_cast<int>(out[k]);
             if (o < 0 || o >= sv.nv || o == s) continue;
-            if (kFast) ++fc.seen;
-            else ++cand;
-            const F3 qc = f3(P[o]), qp = f3(c.prev[o]);
-            if (thickness <= 0.0f) continue;
-            F3 resp = f3(0.0f, 0.0f, 0.0f);
-            bool hit = false;
-            const F3 dref = pp - qp;
-            const float cco = dot(dref, dref) - thickness * thickness;
-            const F3 rel = (pc - pp) - (qc - qp);
-            const float aco = dot(rel, rel), bco = dot(dref, rel);
-            const bool other_pinned = P[o].w == 0.0f;
-            const float scale = other_pinned ? 1.0f : 0.5f;
-            if (cco >= 0.0f && bco < 0.0f && aco > 1.0e-5f) {
-                const float disc = bco * bco - aco * cco;
-                if (disc >= 0.0f) {
-                    const float t = (-bco - sqrtf(disc)) / aco;
-                    if (t > 0.0f && t <= 1.0f) {
-                        const float sv1 = 1.0f - t;
-                        const F3 ph = mul(pp, sv1) + mul(pc, t), qh = mul(qp, sv1) + mul(qc, t);
-                        F3 nn = ph - qh;
-                        const float nsq = dot(nn, nn);
-                        if (nsq > kEps) {
-                            nn = mul(nn, rsqrt_exact(nsq));
-                            const float vn = dot((pc - ph) - (qc - qh), nn);
-                            if (vn > 0.0f) { resp = mul(nn, -vn * scale); hit = true; }
-                        }
-                    }
-                }
-            }
-            if (!hit) {
-                const F3 d = pc - qc;
-                const float dsq = dot(d, d);
-                if (dsq >= thickness * thickness) continue;
-                F3 nn = f3(0.0f, 0.0f, 1.0f);
-                float dist = sqrt_eps(dsq);
-                if (dsq > kEps) nn = mul(d, 1.0f / dist);
-                else {
-                    const F3 rd = f3(c.rest[s]) - f3(c.rest[o]);
-                    const float rsq = dot(rd, rd);
-                    if (rsq > kEps) nn = mul(rd, rsqrt_exact(rsq));
-                    dist = 0.0f;
-                }
-                resp = mul(nn, fminf((thickness - dist) * scale, thickness * 0.35f));
-                hit = true;
-            }
-            if (hit && dot(resp, resp) > kEps) {
-                const float ml = fmaxf(thickness * 0.35f, 1.0e-4f);
-                const float rl = len3(resp);
-                if (rl > ml) resp = mul(resp, ml / rl);
-                if (kFast) {
-                    const F3 nn = mul(resp, 1.0f / fmaxf(len3(resp), kEps));
-                    const F3 mid = mul(pc + qc, 0.5f);
-                    if (!cl.try_accept(filter_cluster_key(0, mid, nn, fmaxf(thickness * 2.5f, 1.0e-3f)), fc)) continue;
-                    ++fc.acc_vv;
-                    ++cand;
-                }
-                accumulate(sv, P, s, resp);
-                ++hits;
-            }
+            ++fc.seen;
+            F3 resp;
+            if (!vv_response(c, P, s, o, pc, pp, thickness, &resp)) continue;
+            const F3 nn = mul(resp, 1.0f / fmaxf(len3(resp), kEps));
+            const F3 mid = mul(pc + f3(P[o]), 0.5f);
+            if (!cl.try_accept(filter_cluster_key(0, mid, nn, fmaxf(thickness * 2.5f, 1.0e-3f)), fc)) continue;
+            ++fc.acc_vv;
+            ++cand;
+            accumulate(sv, P, s, resp);
+            ++hits;
         }
     }
-    // vertex-triangle (A41:4990-5107 per pair)
     if (sv.nt > 0) {
-        if (kFast) cl.n = 0;
-        auto pair = [&](int v, int t) -> bool {
-            if (t < 0 || t >= sv.nt || r <= 0.0f || onset <= r) return false;
-            const int4 tr = sv.tris[t];
-            if (!tri_valid(sv, tr)) return false;
-            const int ia = tr.x, ib = tr.y, ic = tr.z;
-            bool rest_near = v == ia || v == ib || v == ic;   // A41:4615-4630
-            if (!rest_near) {
-                const F3 rv = f3(c.rest[v]);
-                const F3 rq = closest_point_on_triangle(rv, f3(c.rest[ia]), f3(c.rest[ib]), f3(c.rest[ic]));
-                const F3 rdd = rv - rq;
-                rest_near = dot(rdd, rdd) < sv.rest_skip * sv.rest_skip;
-            }
-            if (rest_near) {
-                if (kFast) ++fc.rej_rest;
-                return false;
-            }
-            const F3 p = f3(P[v]), a = f3(P[ia]), b = f3(P[ib]), cc = f3(P[ic]);
-            const F3 q = closest_point_on_triangle(p, a, b, cc);
-            const F3 d = p - q;
-            const float dsq = dot(d, d);
-            float dist = sqrt_eps(dsq);
-            if (dist >= onset) return false;
-            F3 nn = f3(0.0f, 0.0f, 1.0f);
-            if (dsq > kEps) nn = mul(d, 1.0f / dist);
-            else {
-                nn = cross(b - a, cc - a);
-                const float nsq = dot(nn, nn);
-                if (nsq > kEps) nn = mul(nn, rsqrt_exact(nsq));
-                else {
-                    const F3 rv = f3(c.rest[v]);
-                    const F3 rdl = rv - closest_point_on_triangle(rv, f3(c.rest[ia]), f3(c.rest[ib]), f3(c.rest[ic]));
-                    const float rsq = dot(rdl, rdl);
-                    if (rsq > kEps) nn = mul(rdl, rsqrt_exact(rsq));
-                }
-                dist = 0.0f;
-            }
-            if (kFast) {
-                if (!cl.try_accept(filter_cluster_key(1, q, f3(0.0f, 0.0f, 0.0f), fmaxf(onset * 12.0f, 1.0e-3f)), fc)) return true;
-                ++fc.acc_vt;
-            }
+        cl.n = 0;
+        auto pair = [&](int t) -> bool {
+            const PairEval e = vt_eval(c, sv, P, s, t);
+            if (e.rest) ++fc.rej_rest;
+            if (!e.accepted) return false;
+            if (!cl.try_accept(filter_cluster_key(1, e.point, f3(0.0f, 0.0f, 0.0f), fmaxf(onset * 12.0f, 1.0e-3f)), fc)) return true;
+            ++fc.acc_vt;
             ++cand;
-            float wa = 1.0f / 3.0f, wb = wa, wc = wa;
-            triangle_barycentric(q, a, b, cc, &wa, &wb, &wc);
-            const float pen = r - dist;
-            float push;
-            if (pen > 0.0f) push = fminf(pen * 0.35f + r * 0.01f, r * 0.25f);
-            else {
-                const float x = 1.0f - fminf(dist / onset, 1.0f);
-                push = fminf(x * x * r * 0.06f, r * 0.04f);
+            if (e.apply) {
+                apply_eval(sv, P, e);
+                ++soft;
             }
-            const float wv = c.rest[v].w, wta = c.rest[ia].w * wa, wtb = c.rest[ib].w * wb, wtc = c.rest[ic].w * wc;
-            const float total = wv + wta + wtb + wtc;
-            if (push <= 0.0f || total <= 0.0f) return true;
-            const F3 delta = mul(nn, push / total);
-            if (wv > 0.0f) accumulate(sv, P, v, mul(delta, wv));
-            if (wta > 0.0f) accumulate(sv, P, ia, mul(delta, -wta));
-            if (wtb > 0.0f) accumulate(sv, P, ib, mul(delta, -wtb));
-            if (wtc > 0.0f) accumulate(sv, P, ic, mul(delta, -wtc));
-            ++soft;
             return true;
         };
         int accepted = 0;
@@ -562,17 +672,15 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
             int prev_t = -1;
             for (int k = 0; k < lim && accepted < sv.max_nb; ++k) {
                 const int t = sv.th.entries[base + k] / kMaxPrimCells;
-                if (kFast) {
-                    ++fc.seen;
-                    if (t == prev_t) { ++fc.rej_dup; continue; }
-                    prev_t = t;
-                }
-                if (pair(s, t)) ++accepted;
+                ++fc.seen;
+                if (t == prev_t) { ++fc.rej_dup; continue; }
+                prev_t = t;
+                if (pair(t)) ++accepted;
             }
         } else {
             for (int t = 0; t < sv.nt && accepted < sv.max_nb; ++t) {
-                if (kFast) ++fc.seen;
-                if (pair(s, t)) ++accepted;
+                ++fc.seen;
+                if (pair(t)) ++accepted;
             }
         }
     }
@@ -584,76 +692,21 @@ __device__ inline void self_vertex(const ClothView& c, const SelfView& sv, const
 }
 
 // Edge part: A41:5595-5844.
-template <bool kFast>
-__device__ inline void self_edge(const ClothView& c, const SelfView& sv, const float4* P, int e, bool count, ClusterSet& cl,
+__device__ inline void fast_edge(const ClothView& c, const SelfView& sv, const float4* P, int e, bool count, ClusterSet& cl,
                                  FilterCounts& fc) {
-    const float r = sv.radius, onset = sv.onset, cs = sv.cell;
+    const float onset = sv.onset, cs = sv.cell;
     unsigned cand = 0, hits = 0;
-    auto pair = [&](int ea, int eb) -> bool {
-        if (eb <= ea || ea < 0 || eb >= sv.ne || r <= 0.0f || onset <= r) return false;
-        const int2 A = sv.edges[ea], B = sv.edges[eb];
-        if (!edge_valid(sv, A) || !edge_valid(sv, B)) return false;
-        const int a0 = A.x, a1 = A.y, b0 = B.x, b1 = B.y;
-        bool rest_near = a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1;   // A41:5507-5530
-        if (!rest_near) {
-            const F3 r0 = f3(c.rest[a0]), r1 = f3(c.rest[a1]), r2 = f3(c.rest[b0]), r3 = f3(c.rest[b1]);
-            float s = 0.0f, t = 0.0f;
-            closest_segment_parameters(r0, r1, r2, r3, &s, &t);
-            const F3 dd = (r0 + mul(r1 - r0, s)) - (r2 + mul(r3 - r2, t));
-            rest_near = dot(dd, dd) < sv.rest_skip * sv.rest_skip;
-        }
-        if (rest_near) {
-            if (kFast) ++fc.rej_rest;
-            return false;
-        }
-        const F3 p0 = f3(P[a0]), p1 = f3(P[a1]), p2 = f3(P[b0]), p3 = f3(P[b1]);
-        float s = 0.0f, t = 0.0f;
-        closest_segment_parameters(p0, p1, p2, p3, &s, &t);
-        const F3 pa = p0 + mul(p1 - p0, s), pb = p2 + mul(p3 - p2, t);
-        const F3 d = pa - pb;
-        const float dsq = dot(d, d);
-        float dist = sqrt_eps(dsq);
-        if (dist >= onset) return false;
-        F3 nn = f3(0.0f, 0.0f, 1.0f);
-        if (dsq > kEps) nn = mul(d, 1.0f / dist);
-        else {
-            nn = cross(p1 - p0, p3 - p2);
-            const float nsq = dot(nn, nn);
-            if (nsq > kEps) nn = mul(nn, rsqrt_exact(nsq));
-            else {
-                const F3 r0 = f3(c.rest[a0]), r1 = f3(c.rest[a1]), r2 = f3(c.rest[b0]), r3 = f3(c.rest[b1]);
-                float rs = 0.0f, rt = 0.0f;
-                closest_segment_parameters(r0, r1, r2, r3, &rs, &rt);
-                const F3 rd = (r0 + mul(r1 - r0, rs)) - (r2 + mul(r3 - r2, rt));
-                const float rsq = dot(rd, rd);
-                if (rsq > kEps) nn = mul(rd, rsqrt_exact(rsq));
-            }
-            dist = 0.0f;
-        }
-        if (kFast) {
-            if (!cl.try_accept(filter_cluster_key(2, mul(pa + pb, 0.5f), f3(0.0f, 0.0f, 0.0f), fmaxf(onset * 12.0f, 1.0e-3f)), fc))
-                return true;
-            ++fc.acc_ee;
-        }
+    auto pair = [&](int o) -> bool {
+        const PairEval ev = ee_eval(c, sv, P, e, o);
+        if (ev.rest) ++fc.rej_rest;
+        if (!ev.accepted) return false;
+        if (!cl.try_accept(filter_cluster_key(2, ev.point, f3(0.0f, 0.0f, 0.0f), fmaxf(onset * 12.0f, 1.0e-3f)), fc)) return true;
+        ++fc.acc_ee;
         ++cand;
-        const float pen = r - dist;
-        float push;
-        if (pen > 0.0f) push = fminf(pen * 0.35f + r * 0.01f, r * 0.25f);
-        else {
-            const float x = 1.0f - fminf(dist / onset, 1.0f);
-            push = fminf(x * x * r * 0.06f, r * 0.04f);
+        if (ev.apply) {
+            apply_eval(sv, P, ev);
+            ++hits;
         }
-        const float w0 = 1.0f - s, w1 = s, w2 = 1.0f - t, w3 = t;
-        const float ma0 = c.rest[a0].w, ma1 = c.rest[a1].w, mb0 = c.rest[b0].w, mb1 = c.rest[b1].w;
-        const float wa0 = ma0 * w0, wa1 = ma1 * w1, wb0 = mb0 * w2, wb1 = mb1 * w3;
-        const float total = ma0 * w0 * w0 + ma1 * w1 * w1 + mb0 * w2 * w2 + mb1 * w3 * w3;
-        if (push <= 0.0f || total <= 0.0f) return true;
-        const F3 delta = mul(nn, push / total);
-        if (wa0 > 0.0f) accumulate(sv, P, a0, mul(delta, wa0));
-        if (wa1 > 0.0f) accumulate(sv, P, a1, mul(delta, wa1));
-        if (wb0 > 0.0f) accumulate(sv, P, b0, mul(delta, -wb0));
-        if (wb1 > 0.0f) accumulate(sv, P, b1, mul(delta, -wb1));
-        ++hits;
         return true;
     };
     int accepted = 0;
@@ -667,17 +720,15 @@ __device__ inline void self_edge(const ClothView& c, const SelfView& sv, const f
         int prev_e = -1;   // ascending keys: copies of one edge are adjacent
         for (int k = 0; k < lim && accepted < sv.max_nb; ++k) {
             const int o = sv.eh.entries[base + k] / kMaxPrimCells;
-            if (kFast) {
-                ++fc.seen;
-                if (o == prev_e) { ++fc.rej_dup; continue; }
-                prev_e = o;
-            }
-            if (pair(e, o)) ++accepted;
+            ++fc.seen;
+            if (o == prev_e) { ++fc.rej_dup; continue; }
+            prev_e = o;
+            if (pair(o)) ++accepted;
         }
     } else {
         for (int o = e + 1; o < sv.ne && accepted < sv.max_nb; ++o) {
-            if (kFast) ++fc.seen;
-            if (pair(e, o)) ++accepted;
+            ++fc.seen;
+            if (pair(o)) ++accepted;
         }
     }
     if (count) {
@@ -688,20 +739,164 @@ __device__ inline void self_edge(const ClothView& c, const SelfView& sv, const f
 
 // Threads [0, V) do the vertex part, [V, V + E) the edge part; every read is
 // of positions that stay fixed until the apply kernel.
-template <bool kFast>
-__global__ __launch_bounds__(256) void k_self_contacts(const ClothView* __restrict__ cv, SelfView sv, int parity, int count, int build) {
+__global__ __launch_bounds__(256) void k_self_fast(const ClothView* __restrict__ cv, SelfView sv, int parity, int count, int build) {
+    __shared__ unsigned keys[kClusterMaxLocal * 256];
     const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const ClothView& c = *cv;
     const float4* P = cur_pos(c, parity);
     FilterCounts fc;
-    ClusterSet cl{nullptr, 256, 0};
-    if (kFast) {
-        __shared__ unsigned keys[kClusterMaxLocal * 256];
-        cl.col = keys + threadIdx.x;
+    ClusterSet cl{keys + threadIdx.x, 256, 0};
+    if (k < sv.nv) fast_vertex(c, sv, P, k, count != 0, build != 0, cl, fc);
+    else if (k < sv.nv + sv.ne) fast_edge(c, sv, P, k - sv.nv, count != 0, cl, fc);
+    if (count) flush_filter_counts(c.counts, fc);
+}
+
+// ------------------------------------------------ strict mode, lane per entry
+// The strict mode's candidate work (A41:4738-4814 table, 4816-4951 VV,
+// 4990-5224 VT, 5595-5844 EE) with 32 lanes per primitive: lane l takes entry
+// l of every bucket its vertex or edge queries (a bucket shows at most 32
+// entries), instead of one thread walking them all.  Results equal the
+// sequential walk: the neighbour table is filled in the walk's order (a
+// ballot and a prefix count place each cell's valid entries after the ones
+// already taken, the first 32 kept), the responses are fixed-point sums
+// (order-free), and the cap on accepted VT / EE candidates is applied by the
+// same prefix count (entry l runs only when fewer than max_nb entries before
+// it were accepted, which is when the walk would have reached it).
+
+// This lane's half-wave bits of a ballot.
+__device__ __forceinline__ unsigned half_ballot(bool pred) {
+    return static_cast<unsigned>(__ballot(pred) >> (threadIdx.x & 32u));
+}
+
+// Threads 32p .. 32p + 31 serve primitive p: vertices [0, V), then edges.
+constexpr int kStrictHalves = 256 / 32;
+__global__ __launch_bounds__(256) void k_self_strict(const ClothView* __restrict__ cv, SelfView sv, int parity, int count) {
+    __shared__ unsigned s_tab[kStrictHalves][kNeighborSlots];
+    const int p = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 5);
+    const int l = static_cast<int>(threadIdx.x & 31u);
+    unsigned* tab = s_tab[threadIdx.x >> 5];
+    const unsigned below = (1u << l) - 1u;
+    const ClothView& c = *cv;
+    const float4* P = cur_pos(c, parity);
+    unsigned cand = 0, hits = 0, soft = 0, ee_hits = 0, overflow = 0;
+    // Takes entry l of a walk's chunk when accepted and fewer than max_nb
+    // entries before it were.
+    auto capped = [&](const PairEval& e, bool in_range, int taken, unsigned& applied) {
+        const unsigned m = half_ballot(e.accepted);
+        if (in_range && e.accepted && taken + __popc(m & below) < sv.max_nb) {
+            ++cand;
+            if (e.apply) {
+                apply_eval(sv, P, e);
+                ++applied;
+            }
+        }
+        return __popc(m);
+    };
+    if (p < sv.nv) {
+        const int s = p;
+        const float cs = sv.cell;
+        const F3 ps = f3(P[s]);
+        unsigned acc = 0;
+        unsigned* out = sv.nb_idx + static_cast<size_t>(s) * kNeighborSlots;
+        // candidate table, A41:4738-4788 (hash) / 4790-4814 (brute force)
+        auto take = [&](bool ok, int o) {
+            const unsigned m = half_ballot(ok);
+            const unsigned pos = acc + static_cast<unsigned>(__popc(m & below));
+            if (ok && pos < kNeighborSlots) {
+                tab[pos] = static_cast<unsigned>(o);
+                out[pos] = static_cast<unsigned>(o);
+            }
+            acc = min(acc + static_cast<unsigned>(__popc(m)), static_cast<unsigned>(kNeighborSlots));
+        };
+        if (sv.use_vhash) {
+            const int cx = cell_coord(ps.x, cs), cy = cell_coord(ps.y, cs), cz = cell_coord(ps.z, cs);
+            for (int dz = -1; dz <= 1 && acc < kNeighborSlots; ++dz)
+                for (int dy = -1; dy <= 1 && acc < kNeighborSlots; ++dy)
+                    for (int dx = -1; dx <= 1 && acc < kNeighborSlots; ++dx) {
+                        const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
+                        const int b = static_cast<int>(hash_cell(qx, qy, qz, sv.vh.buckets));
+                        const int base = sv.vh.off[b], lim = min(sv.vh.count[b], kBucketSlots);
+                        bool ok = false;
+                        int o = 0;
+                        if (l < lim) {
+                            o = sv.vh.entries[base + l];
+                            const int4 oc = sv.vcell[o];
+                            ok = oc.x == qx && oc.y == qy && oc.z == qz && vv_valid(c, sv, P, s, o);
+                        }
+                        take(ok, o);
+                    }
+        } else {
+            for (int o0 = 0; o0 < sv.nv && acc < kNeighborSlots; o0 += 32) {
+                const int o = o0 + l;
+                take(o < sv.nv && vv_valid(c, sv, P, s, o), o);
+            }
+        }
+        if (l == 0) {
+            sv.nb_count[s] = acc;
+            overflow = acc >= kNeighborSlots ? 1u : 0u;
+        }
+        wave_lds_sync();
+        // vertex-vertex: lane l takes table entry l
+        const float r = sv.radius;
+        const float thickness = r * 0.5f + r * 0.5f;
+        if (P[s].w != 0.0f && static_cast<unsigned>(l) < acc) {
+            const int o = static_cast<int>(tab[l]);
+            if (o >= 0 && o < sv.nv && o != s) {
+                ++cand;
+                F3 resp;
+                if (vv_response(c, P, s, o, ps, f3(c.prev[s]), thickness, &resp)) {
+                    accumulate(sv, P, s, resp);
+                    ++hits;
+                }
+            }
+        }
+        // vertex-triangle: the own triangle bucket (A41:5109-5182) or every triangle
+        if (sv.nt > 0) {
+            if (sv.use_thash) {
+                const int cx = cell_coord(ps.x, cs), cy = cell_coord(ps.y, cs), cz = cell_coord(ps.z, cs);
+                const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.th.buckets));
+                const int base = sv.th.off[b], lim = min(sv.th.count[b], kBucketSlots);
+                PairEval e;
+                if (l < lim) e = vt_eval(c, sv, P, s, sv.th.entries[base + l] / kMaxPrimCells);
+                capped(e, l < lim, 0, soft);
+            } else {
+                for (int t0 = 0, taken = 0; t0 < sv.nt && taken < sv.max_nb; t0 += 32) {
+                    PairEval e;
+                    if (t0 + l < sv.nt) e = vt_eval(c, sv, P, s, t0 + l);
+                    taken += capped(e, t0 + l < sv.nt, taken, soft);
+                }
+            }
+        }
+    } else if (p < sv.nv + sv.ne) {
+        // edge-edge: the own edge bucket (A41:5724-5802) or every later edge
+        const int e0 = p - sv.nv;
+        if (sv.use_ehash) {
+            const int2 E = sv.edges[e0];
+            if (edge_valid(sv, E)) {
+                const float cs = sv.cell;
+                const F3 mid = mul(f3(P[E.x]) + f3(P[E.y]), 0.5f);
+                const int cx = cell_coord(mid.x, cs), cy = cell_coord(mid.y, cs), cz = cell_coord(mid.z, cs);
+                const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.eh.buckets));
+                const int base = sv.eh.off[b], lim = min(sv.eh.count[b], kBucketSlots);
+                PairEval e;
+                if (l < lim) e = ee_eval(c, sv, P, e0, sv.eh.entries[base + l] / kMaxPrimCells);
+                capped(e, l < lim, 0, ee_hits);
+            }
+        } else {
+            for (int o0 = e0 + 1, taken = 0; o0 < sv.ne && taken < sv.max_nb; o0 += 32) {
+                PairEval e;
+                if (o0 + l < sv.ne) e = ee_eval(c, sv, P, e0, o0 + l);
+                taken += capped(e, o0 + l < sv.ne, taken, ee_hits);
+            }
+        }
     }
-    if (k < sv.nv) self_vertex<kFast>(c, sv, P, k, count != 0, !kFast || build != 0, cl, fc);
-    else if (k < sv.nv + sv.ne) self_edge<kFast>(c, sv, P, k - sv.nv, count != 0, cl, fc);
-    if (kFast && count) flush_filter_counts(c.counts, fc);
+    if (count) {
+        wave_add(c.counts, kCountSelfCandidates, cand);
+        wave_add(c.counts, kCountExactImpulse, hits);
+        wave_add(c.counts, kCountSoft, soft);
+        wave_add(c.counts, kCountEdgeEdge, ee_hits);
+        wave_add(c.counts, kCountSelfOverflow, overflow);
+    }
 }
 
 // A41:5226-5439: a vertex that sits inside three or more nearby triangles
@@ -965,12 +1160,14 @@ bool SelfCollision::run(const LaunchCtx& ctx, int parity, int substep, int subst
 
 static void build_prim_hash(hipStream_t st, const ClothView* cv, const SelfView& s, const HashView& h, int kind, int n,
                             int parity, bool count) {
-    const dim3 grid(grid_for(static_cast<long long>(n) * kMaxPrimCells, 256)), block(256);
+    const dim3 grid(grid_for(static_cast<long long>(n) * kPrimLanes, 256)), block(256);
     launch_k(k_fill_i32, dim3(grid_for(h.buckets, 256)), block, 0, st, h.count, 0, h.buckets);
     launch_k(k_prim_hash, grid, block, 0, st, cv, s, parity, kind, 0, count ? 1 : 0);
     exclusive_scan(st, h.count, h.buckets, h.off, s.scan_tmp);
     launch_k(k_prim_hash, grid, block, 0, st, cv, s, parity, kind, 1, 0);
-    launch_k(k_hash_rank, dim3(grid_for(h.capacity, 256)), block, 0, st, h, s.counts, count ? 1 : 0);
+    // about 27 items per primitive (3 cells a side): one thread each
+    const long long rank_threads = std::min<long long>(h.capacity, static_cast<long long>(n) * 32);
+    launch_k(k_hash_rank, dim3(grid_for(rank_threads, 256)), block, 0, st, h, s.counts, count ? 1 : 0);
 }
 
 bool SelfCollision::run_for(hipStream_t st, const ClothView* cv, int parity, bool count) {
@@ -1000,9 +1197,10 @@ bool SelfCollision::run_for(hipStream_t st, const ClothView* cv, int parity, boo
         {
             TimedLaunch t("self_contacts", st);
             if (s.fast)
-                launch_k(k_self_contacts<true>, contacts_grid, dim3(256), 0, st, cv, s, parity, count ? 1 : 0, pass == 0 ? 1 : 0);
+                launch_k(k_self_fast, contacts_grid, dim3(256), 0, st, cv, s, parity, count ? 1 : 0, pass == 0 ? 1 : 0);
             else
-                launch_k(k_self_contacts<false>, contacts_grid, dim3(256), 0, st, cv, s, parity, count ? 1 : 0, 1);
+                launch_k(k_self_strict, dim3(grid_for((static_cast<long long>(s.nv) + s.ne) * 32, 256)), dim3(256), 0, st, cv, s,
+                         parity, count ? 1 : 0);
         }
         if (s.fast && pass == passes_ - 1 && s.nt > 0 && s.use_thash) {
             TimedLaunch t("self_island", st);
