@@ -1424,6 +1424,7 @@ struct Batch {
     std::vector<ClothView> h_views;
     Staging staging;
     int ntiles = 0;
+    int stepped_substeps = 0;   // substeps of the last step since the tables were built (0: none; d_views unfilled)
     SelfBatchHolder self;
     std::vector<PcgGroup> pcg_groups;   // members on the hard-stretch solve, one group per cadence
     StepGraph graph;
@@ -1469,6 +1470,7 @@ static bool build_batch(Batch& b) {
         merge_color_segs(b.members, &Solver::h_stretch_segs, &Solver::stretch_seg_first, &Solver::stretch_seg_count, b.h_stretch_segs,
                          b.sseg_first, b.sseg_count);
     b.ntiles = static_cast<int>(tiles.size());
+    b.stepped_substeps = 0;
     b.h_views.resize(b.members.size());
     // members on the hard-stretch solve, grouped by substep cadence
     std::map<int, std::vector<PcgMember>> by_cadence;
@@ -1511,6 +1513,7 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, b
         b.h_views[m] = s.view;
     }
     if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
+    b.stepped_substeps = substeps;
     LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles};
     SelfBatch* sb = b.self.get(b.members, b.h_views, b.d_views.p);
     const StepPlan plan = plan_of(b.members);
@@ -1550,7 +1553,16 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, b
 // the kernel time plus the dispatch gap, with no per-launch instrumentation.
 // The members' positions, previous positions and velocities are copied aside
 // before and back afterwards, so the measurement leaves the state as it was.
+// The batch must have been stepped since its tables were built: the members'
+// views (refilled here, with that step's substep count) describe a prepared
+// step only then.
 static bool time_batch_pass(Batch& b, int launches, float* avg_ms) {
+    if (b.stepped_substeps <= 0 || b.ntiles <= 0) return fail("time a batch's pass after stepping it once");
+    for (size_t m = 0; m < b.members.size(); ++m) {
+        b.members[m]->fill_view(b.stepped_substeps);
+        b.h_views[m] = b.members[m]->view;
+    }
+    if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
     launches = std::max(2, launches + (launches & 1));
     size_t total = 0;
     for (Solver* s : b.members) total += 4 * static_cast<size_t>(s->nv);

@@ -493,7 +493,9 @@ class SolverBatch:
         return int(self._lib.ssbl_ext_tuned_pool_limit(None, self._handle))
 
     def time_pass(self, launches: int = 64) -> float:
-        """Mean ms of `launches` back-to-back fused constraint passes (advances the state)."""
+        """Mean ms of `launches` back-to-back fused constraint passes; the
+        members' state is restored afterwards.  The batch must have been
+        stepped since it was created or given new members."""
         out = np.zeros(1, np.float32)
         if not self._lib.ssbl_ext_time_batch_pass(self._handle, int(launches), _fptr(out)):
             raise NativeSolverError(last_error(self._lib))

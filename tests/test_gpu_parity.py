@@ -162,3 +162,25 @@ def test_batch_set_members_equals_fresh_batch(nb, hip, scenes):
             pa, pb, pc = a[k].download_positions(), b[k].download_positions(), c[k].download_positions()
             assert np.array_equal(pa, pb) and np.array_equal(pb, pc), (members, k)
     moving.close()
+
+
+def test_batch_pass_timing_needs_a_step_and_keeps_the_state(nb, hip, scenes):
+    """The pass timing refuses a batch that has not been stepped since it was
+    built or re-targeted (its device view table is filled by a step); after
+    a step it leaves every member's state as it was."""
+    sc = scenes.config3(seed=1, count=4, frames=2, cross_cloth=False)
+    members = [nb.NativeXpbdSolver(c, o, sc.static_triangles, lib=hip) for c, o in zip(sc.cloths, sc.options)]
+    batch = nb.SolverBatch(members)
+    with pytest.raises(nb.NativeSolverError, match="stepping it once"):
+        batch.time_pass(8)
+    batch.step(sc.substeps, sc.iterations)
+    nb.synchronize(hip)
+    before = [m.download_state() for m in members]
+    assert batch.time_pass(8) > 0.0
+    for m, b in zip(members, before):
+        for x, y in zip(m.download_state(), b):
+            assert np.array_equal(x, y)
+    batch.set_members(members[:2])
+    with pytest.raises(nb.NativeSolverError, match="stepping it once"):
+        batch.time_pass(8)
+    batch.close()
