@@ -74,7 +74,9 @@ SSBL_API int ssbl_ext_step_batch_ex(void* batch, int substeps, int iterations, i
  * the batch (stretch gather, pins, analytic and SDF contacts) between two
  * events on the library stream and writes their mean duration in ms.  The
  * members' state is copied aside first and restored afterwards, so the call
- * leaves it unchanged: a measurement call (it allocates the copy). */
+ * leaves it unchanged: a measurement call (it allocates the copy).  Fails
+ * for a batch not stepped since it was created or given new members (the
+ * pass needs a prepared step's member views). */
 SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms);
 
 /* Global dynamic scene from cloth handles, gathered on the device: particle k
