@@ -150,37 +150,23 @@ __device__ __forceinline__ int bucket_rank(const int* __restrict__ raw, int base
     return rank;
 }
 
-// A41:4550-4575: vertex cell and bucket.  phase 0 counts, 1 scatters, 2 ranks
-// (items beyond the first 32 of a bucket only count as overflow: A41:4570-4574
-// drops them).
-__global__ __launch_bounds__(256) void k_vhash(const ClothView* __restrict__ cv, SelfView sv, int parity, int phase, int count_diag) {
-    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const bool active = i < sv.nv;
+// A41:4550-4575: vertex cell and bucket.  phase 0 counts, 1 scatters (the
+// rank pass then keeps the first 32 of a bucket; A41:4570-4574 drops the
+// others).
+__device__ __forceinline__ void vertex_hash_item(const ClothView& c, const SelfView& sv, int parity, int i, int phase) {
+    if (i >= sv.nv) return;
     if (phase == 0) {
-        if (!active) return;
-        const float4 p = cur_pos(*cv, parity)[i];
+        const float4 p = cur_pos(c, parity)[i];
         const int cx = cell_coord(p.x, sv.cell), cy = cell_coord(p.y, sv.cell), cz = cell_coord(p.z, sv.cell);
         const int b = static_cast<int>(hash_cell(cx, cy, cz, sv.vh.buckets));
         sv.vcell[i] = make_int4(cx, cy, cz, b);
         sv.vh.slot[i] = atomicAdd(&sv.vh.count[b], 1);
-        return;
-    }
-    if (phase == 1) {
-        if (active) {
-            const int b = sv.vcell[i].w;
-            sv.vh.raw[sv.vh.off[b] + sv.vh.slot[i]] = i;
-        }
-        return;
-    }
-    bool over = false;
-    if (active) {
+    } else {
         const int b = sv.vcell[i].w;
-        const int base = sv.vh.off[b];
-        const int rank = bucket_rank(sv.vh.raw, base, sv.vh.count[b], i);
-        if (rank < kBucketSlots) sv.vh.entries[base + rank] = i;
-        else over = true;
+        const int j = sv.vh.off[b] + sv.vh.slot[i];
+        sv.vh.raw[j] = i;
+        sv.vh.rawb[j] = sv.vh.base + b;
     }
-    if (count_diag) wave_count(sv.counts, kCountSelfOverflow, over);
 }
 
 // Primitive box in cells (A41:4632-4694, 5532-5593) with the >64-cell guard
@@ -237,19 +223,19 @@ __device__ inline bool edge_box(const ClothView& c, const SelfView& sv, int pari
 // cell by the guard; a box is usually 2-3 cells a side, as the margin is one
 // cell).  kind 0 = triangles, 1 = edges; phase 0 counts, 1 scatters.
 // Key = primitive * 64 + the cell's index in the box loop (z outer, then y,
-// then x).
+// then x).  `t` is the thread's index within its hash's part of the launch.
 constexpr int kPrimLanes = 32;
-__global__ __launch_bounds__(256) void k_prim_hash(const ClothView* __restrict__ cv, SelfView sv, int parity, int kind,
-                                                   int phase, int count_diag) {
-    const int k = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / kPrimLanes);
-    const int lane = static_cast<int>(threadIdx.x % kPrimLanes);
+__device__ __forceinline__ void prim_hash_item(const ClothView& c, const SelfView& sv, int parity, int kind, int t, int phase,
+                                               int count_diag) {
+    const int k = t / kPrimLanes;
+    const int lane = t % kPrimLanes;
     const int n = kind == 0 ? sv.nt : sv.ne;
     if (k >= n) return;
     int b0[3], b1[3];
     bool guarded = false;
-    const bool ok = kind == 0 ? tri_box(*cv, sv, parity, k, b0, b1, &guarded) : edge_box(*cv, sv, parity, k, b0, b1, &guarded);
+    const bool ok = kind == 0 ? tri_box(c, sv, parity, k, b0, b1, &guarded) : edge_box(c, sv, parity, k, b0, b1, &guarded);
     if (!ok) return;
-    if (phase == 0 && guarded && count_diag && lane == 0) atomicAdd(&cv->counts[kCountSelfOverflow], 1ull);
+    if (phase == 0 && guarded && count_diag && lane == 0) atomicAdd(&c.counts[kCountSelfOverflow], 1ull);
     const int sx = b1[0] - b0[0] + 1, sy = b1[1] - b0[1] + 1, sz = b1[2] - b0[2] + 1;
     if (sx <= 0 || sy <= 0 || sz <= 0) return;
     const HashView& h = kind == 0 ? sv.th : sv.eh;
@@ -262,30 +248,51 @@ __global__ __launch_bounds__(256) void k_prim_hash(const ClothView* __restrict__
         } else {
             const int j = h.off[b] + h.slot[key];
             h.raw[j] = key;
-            h.rawb[j] = b;
+            h.rawb[j] = h.base + b;
         }
     }
 }
 
-// Every scattered item of a primitive hash finds its rank in its bucket and
+// One phase of all three hash builds in one launch: blocks [0, vh_blocks)
+// serve the vertex hash, the next th_blocks the triangle hash, the rest the
+// edge hash (each part uniform over its blocks).
+__global__ __launch_bounds__(256) void k_self_hash(const ClothView* __restrict__ cv, SelfView sv, int parity, int phase,
+                                                   int count_diag) {
+    int blk = static_cast<int>(blockIdx.x);
+    const int tid = static_cast<int>(threadIdx.x);
+    if (blk < sv.vh_blocks) {
+        vertex_hash_item(*cv, sv, parity, blk * 256 + tid, phase);
+        return;
+    }
+    blk -= sv.vh_blocks;
+    if (blk < sv.th_blocks) {
+        prim_hash_item(*cv, sv, parity, 0, blk * 256 + tid, phase, count_diag);
+        return;
+    }
+    blk -= sv.th_blocks;
+    if (sv.use_ehash) prim_hash_item(*cv, sv, parity, 1, blk * 256 + tid, phase, count_diag);
+}
+
+// Every scattered item of the shared table finds its rank in its bucket and
 // lands there if it is among the first 32 (the others count as overflow).
 // A grid-stride loop over the items actually scattered (their total is the
 // scan's last offset); every lane runs the same number of rounds.
-__global__ __launch_bounds__(256) void k_hash_rank(HashView h, unsigned long long* counts, int count_diag) {
-    const int total = h.off[h.buckets];
+__global__ __launch_bounds__(256) void k_self_rank(SelfView sv, const int* __restrict__ raw, const int* __restrict__ rawb,
+                                                   int* __restrict__ entries, int count_diag) {
+    const int total = sv.all_off[sv.all_buckets];
     const int stride = static_cast<int>(gridDim.x * blockDim.x);
     const int rounds = (total + stride - 1) / stride;
     for (int r = 0; r < rounds; ++r) {
         const int j = r * stride + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
         bool over = false;
         if (j < total) {
-            const int b = h.rawb[j], key = h.raw[j];
-            const int base = h.off[b];
-            const int rank = bucket_rank(h.raw, base, h.count[b], key);
-            if (rank < kBucketSlots) h.entries[base + rank] = key;
+            const int b = rawb[j], key = raw[j];
+            const int base = sv.all_off[b];
+            const int rank = bucket_rank(raw, base, sv.all_count[b], key);
+            if (rank < kBucketSlots) entries[base + rank] = key;
             else over = true;
         }
-        if (count_diag) wave_count(counts, kCountSelfOverflow, over);
+        if (count_diag) wave_count(sv.counts, kCountSelfOverflow, over);
     }
 }
 
@@ -1106,23 +1113,45 @@ bool SelfCollision::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& e
     s.use_ehash = s.ne >= 64 ? 1 : 0;
     passes_ = std::max(1, std::min(cfg.fast_self_collision_passes, 8));
     interval_ = std::max(cfg.self_collision_interval, 1);
-    auto make_hash = [&](HashView& h, int n, int capacity) -> bool {
+    // the shared bucket table: vertex, triangle, edge buckets in that order,
+    // item slots likewise
+    long long buckets = 0, items = 0;
+    std::vector<std::pair<HashView*, long long>> used;   // hash, first item slot
+    auto plan = [&](HashView& h, int n, long long capacity) {
         h.buckets = next_pow2(std::max(1024, n * 4));
-        h.capacity = capacity;
-        const size_t cap = static_cast<size_t>(std::max(capacity, 1));
-        h.count = static_cast<int*>(alloc_bytes(sizeof(int) * h.buckets));
-        h.off = static_cast<int*>(alloc_bytes(sizeof(int) * (h.buckets + 1)));
-        h.slot = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
-        h.raw = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
-        h.rawb = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
-        h.entries = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
-        return h.count && h.off && h.slot && h.raw && h.rawb && h.entries;
+        h.capacity = static_cast<int>(capacity);
+        h.base = static_cast<int>(buckets);
+        used.emplace_back(&h, items);
+        buckets += h.buckets;
+        items += capacity;
     };
-    if (s.use_vhash && !make_hash(s.vh, nv, nv)) return false;
-    if (s.use_thash && !make_hash(s.th, s.nt, s.nt * kMaxPrimCells)) return false;
-    if (s.use_ehash && !make_hash(s.eh, s.ne, s.ne * kMaxPrimCells)) return false;
-    int max_buckets = std::max(s.vh.buckets, std::max(s.th.buckets, s.eh.buckets));
-    s.scan_tmp = static_cast<int*>(alloc_bytes(sizeof(int) * scan_tmp_ints(max_buckets)));
+    if (s.use_vhash) plan(s.vh, nv, nv);
+    if (s.use_thash) plan(s.th, s.nt, static_cast<long long>(s.nt) * kMaxPrimCells);
+    if (s.use_ehash) plan(s.eh, s.ne, static_cast<long long>(s.ne) * kMaxPrimCells);
+    if (buckets >= (1ll << 30) || items >= (1ll << 31)) return fail("self-collision hashes too large");
+    s.all_buckets = static_cast<int>(buckets);
+    s.hash_items = static_cast<int>(items);
+    if (!used.empty()) {
+        const size_t cap = static_cast<size_t>(std::max<long long>(items, 1));
+        s.all_count = static_cast<int*>(alloc_bytes(sizeof(int) * buckets));
+        s.all_off = static_cast<int*>(alloc_bytes(sizeof(int) * (buckets + 1)));
+        int* slot = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
+        int* raw = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
+        int* rawb = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
+        int* entries = static_cast<int*>(alloc_bytes(sizeof(int) * cap));
+        if (!s.all_count || !s.all_off || !slot || !raw || !rawb || !entries) return false;
+        for (auto& [h, first_item] : used) {
+            h->count = s.all_count + h->base;
+            h->off = s.all_off + h->base;
+            h->slot = slot + first_item;
+            h->raw = raw;
+            h->rawb = rawb;
+            h->entries = entries;
+        }
+    }
+    s.vh_blocks = s.use_vhash ? static_cast<int>(grid_for(nv, 256)) : 0;
+    s.th_blocks = s.use_thash ? static_cast<int>(grid_for(static_cast<long long>(s.nt) * kPrimLanes, 256)) : 0;
+    s.scan_tmp = static_cast<int*>(alloc_bytes(sizeof(int) * scan_tmp_ints(std::max(s.all_buckets, 1))));
     s.vcell = static_cast<int4*>(alloc_bytes(sizeof(int4) * nv));
     s.nb_count = static_cast<unsigned*>(alloc_bytes(sizeof(unsigned) * nv));
     s.nb_idx = static_cast<unsigned*>(alloc_bytes(sizeof(unsigned) * static_cast<size_t>(nv) * kNeighborSlots));
@@ -1158,33 +1187,25 @@ bool SelfCollision::run(const LaunchCtx& ctx, int parity, int substep, int subst
     return run_for(ctx.stream, dview_, parity, count);
 }
 
-static void build_prim_hash(hipStream_t st, const ClothView* cv, const SelfView& s, const HashView& h, int kind, int n,
-                            int parity, bool count) {
-    const dim3 grid(grid_for(static_cast<long long>(n) * kPrimLanes, 256)), block(256);
-    launch_k(k_fill_i32, dim3(grid_for(h.buckets, 256)), block, 0, st, h.count, 0, h.buckets);
-    launch_k(k_prim_hash, grid, block, 0, st, cv, s, parity, kind, 0, count ? 1 : 0);
-    exclusive_scan(st, h.count, h.buckets, h.off, s.scan_tmp);
-    launch_k(k_prim_hash, grid, block, 0, st, cv, s, parity, kind, 1, 0);
-    // about 27 items per primitive (3 cells a side): one thread each
-    const long long rank_threads = std::min<long long>(h.capacity, static_cast<long long>(n) * 32);
-    launch_k(k_hash_rank, dim3(grid_for(rank_threads, 256)), block, 0, st, h, s.counts, count ? 1 : 0);
-}
-
 bool SelfCollision::run_for(hipStream_t st, const ClothView* cv, int parity, bool count) {
     if (!ready_) return true;
     const SelfView& s = sv_;
-    {
+    if (s.all_buckets > 0) {
+        // the three hashes in one shared table: fill, count, scan, scatter, rank
         TimedLaunch t("self_hash", st);
-        if (s.use_vhash) {
-            const dim3 grid(grid_for(s.nv, 256)), block(256);
-            launch_k(k_fill_i32, dim3(grid_for(s.vh.buckets, 256)), block, 0, st, s.vh.count, 0, s.vh.buckets);
-            launch_k(k_vhash, grid, block, 0, st, cv, s, parity, 0, 0);
-            exclusive_scan(st, s.vh.count, s.vh.buckets, s.vh.off, s.scan_tmp);
-            launch_k(k_vhash, grid, block, 0, st, cv, s, parity, 1, 0);
-            launch_k(k_vhash, grid, block, 0, st, cv, s, parity, 2, count ? 1 : 0);
-        }
-        if (s.use_thash) build_prim_hash(st, cv, s, s.th, 0, s.nt, parity, count);
-        if (s.use_ehash) build_prim_hash(st, cv, s, s.eh, 1, s.ne, parity, count);
+        const dim3 block(256);
+        const unsigned eh_blocks = s.use_ehash ? grid_for(static_cast<long long>(s.ne) * kPrimLanes, 256) : 0u;
+        const dim3 grid(static_cast<unsigned>(s.vh_blocks + s.th_blocks) + eh_blocks);
+        const HashView& any = s.use_vhash ? s.vh : (s.use_thash ? s.th : s.eh);
+        launch_k(k_fill_i32, dim3(grid_for(s.all_buckets, 256)), block, 0, st, s.all_count, 0, s.all_buckets);
+        launch_k(k_self_hash, grid, block, 0, st, cv, s, parity, 0, count ? 1 : 0);
+        exclusive_scan(st, s.all_count, s.all_buckets, s.all_off, s.scan_tmp);
+        launch_k(k_self_hash, grid, block, 0, st, cv, s, parity, 1, 0);
+        // about one item per vertex and 27 per primitive (3 cells a side): one thread each
+        const long long rank_threads = std::min<long long>(
+            s.hash_items, (s.use_vhash ? s.nv : 0) + 32ll * ((s.use_thash ? s.nt : 0) + (s.use_ehash ? s.ne : 0)));
+        launch_k(k_self_rank, dim3(grid_for(std::max<long long>(rank_threads, 1), 256)), block, 0, st, s, any.raw, any.rawb,
+                 any.entries, count ? 1 : 0);
     }
     const dim3 contacts_grid(grid_for(static_cast<long long>(s.nv) + s.ne, 256));
     if (s.fast && count) {

@@ -26,15 +26,21 @@ struct Solver;
 // bucket b in ascending order (key = vertex, or primitive * 64 + cell index in
 // the primitive's box loop), which is the order queries visit.  The ranks do
 // not depend on the order the atomics landed in.
+//
+// A cloth's three hashes (vertex, triangle, edge) share one table: their
+// buckets are consecutive ranges of one count / offset array, scanned
+// together, so an offset is an index into one raw / rawb / entries array, and
+// each build phase is one launch for all three.
 struct HashView {
     int buckets;
     int capacity;
-    int* count;      // buckets
-    int* off;        // buckets + 1
+    int base;        // first bucket of this hash in the shared table
+    int* count;      // buckets (a range of the shared counts)
+    int* off;        // buckets + 1 (a range of the shared offsets)
     int* slot;       // capacity: arrival slot of item (vertex, or prim * 64 + cell)
-    int* raw;        // capacity: keys in arrival order
-    int* rawb;       // capacity: bucket of raw[j]
-    int* entries;    // capacity: first min(count, 32) keys of a bucket, ascending
+    int* raw;        // shared: keys in arrival order
+    int* rawb;       // shared: shared-table bucket of raw[j]
+    int* entries;    // shared: first min(count, 32) keys of a bucket, ascending
 };
 
 struct SelfView {
@@ -46,6 +52,11 @@ struct SelfView {
     int island_contacts;         // overlap-island contact cap, clamp(2 * max neighbours, 3, 48)
     int4* vcell;                 // V: cell x, y, z, bucket (vertex hash)
     HashView vh, th, eh;
+    int* all_count;              // the shared table's counts (all_buckets)
+    int* all_off;                // and offsets (all_buckets + 1)
+    int all_buckets;
+    int hash_items;              // largest number of items the three hashes hold together
+    int vh_blocks, th_blocks;    // blocks of the vertex and triangle parts of a build launch
     unsigned* nb_count;          // V
     unsigned* nb_idx;            // V * 32
     unsigned long long* acc;     // 3V fixed-point 2^-32 m
