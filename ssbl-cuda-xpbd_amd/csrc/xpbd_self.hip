@@ -816,22 +816,44 @@ __global__ __launch_bounds__(256) void k_self_strict(const ClothView* __restrict
             acc = min(acc + static_cast<unsigned>(__popc(m)), static_cast<unsigned>(kNeighborSlots));
         };
         if (sv.use_vhash) {
+            // The walk's 27 cells (dz outer, then dy, then dx) at once: lane l
+            // < 27 takes cell l and its bucket's first min(count, 32) entries;
+            // a prefix sum lays them out in walk order (cell, then entry), and
+            // the half-wave takes that list 32 items at a time until the table
+            // is full.  Item w belongs to the last lane whose start is <= w.
             const int cx = cell_coord(ps.x, cs), cy = cell_coord(ps.y, cs), cz = cell_coord(ps.z, cs);
-            for (int dz = -1; dz <= 1 && acc < kNeighborSlots; ++dz)
-                for (int dy = -1; dy <= 1 && acc < kNeighborSlots; ++dy)
-                    for (int dx = -1; dx <= 1 && acc < kNeighborSlots; ++dx) {
-                        const int qx = cx + dx, qy = cy + dy, qz = cz + dz;
-                        const int b = static_cast<int>(hash_cell(qx, qy, qz, sv.vh.buckets));
-                        const int base = sv.vh.off[b], lim = min(sv.vh.count[b], kBucketSlots);
-                        bool ok = false;
-                        int o = 0;
-                        if (l < lim) {
-                            o = sv.vh.entries[base + l];
-                            const int4 oc = sv.vcell[o];
-                            ok = oc.x == qx && oc.y == qy && oc.z == qz && vv_valid(c, sv, P, s, o);
-                        }
-                        take(ok, o);
-                    }
+            int qx = 0, qy = 0, qz = 0, first = 0, n = 0;
+            if (l < 27) {
+                qx = cx + l % 3 - 1;
+                qy = cy + (l / 3) % 3 - 1;
+                qz = cz + l / 9 - 1;
+                const int b = static_cast<int>(hash_cell(qx, qy, qz, sv.vh.buckets));
+                first = sv.vh.off[b];
+                n = min(sv.vh.count[b], kBucketSlots);
+            }
+            int incl = n;
+            for (int d = 1; d < 32; d <<= 1) {
+                const int o = __shfl_up(incl, d, 32);
+                if (l >= d) incl += o;
+            }
+            const int start = incl - n;
+            const int total = __shfl(incl, 31, 32);
+            for (int base = 0; base < total && acc < kNeighborSlots; base += 32) {
+                const int w = base + l;
+                int own = 0;
+                for (int step = 16; step > 0; step >>= 1)
+                    if (__shfl(start, own + step, 32) <= w) own += step;
+                const int at = __shfl(first, own, 32) + (w - __shfl(start, own, 32));
+                const int ox = __shfl(qx, own, 32), oy = __shfl(qy, own, 32), oz = __shfl(qz, own, 32);
+                bool ok = false;
+                int o = 0;
+                if (w < total) {
+                    o = sv.vh.entries[at];
+                    const int4 oc = sv.vcell[o];
+                    ok = oc.x == ox && oc.y == oy && oc.z == oz && vv_valid(c, sv, P, s, o);
+                }
+                take(ok, o);
+            }
         } else {
             for (int o0 = 0; o0 < sv.nv && acc < kNeighborSlots; o0 += 32) {
                 const int o = o0 + l;
