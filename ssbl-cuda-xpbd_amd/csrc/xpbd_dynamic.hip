@@ -132,20 +132,8 @@ constexpr int kTriPerBlock = 256 / kTriLanes;
 constexpr int kAggSlots = 1024;   // LDS table of the distinct buckets a block's entries fall in
 constexpr int kAggProbes = 32;    // a bucket with no slot within this many probes counts globally, entry by entry
 
-// Slot of bucket b in a block's LDS table (linear probing from b's low
-// bits), inserting b; -1 when the probes run out.  Slots are never freed, so
-// a bucket inserted once is found again at the same place, and one that
-// found no slot never finds one later in the launch.
 __device__ __forceinline__ int agg_slot(int* s_bucket, int* s_list, int* s_n, int b) {
-    int h = b & (kAggSlots - 1);
-    int old = atomicCAS(&s_bucket[h], -1, b);
-    for (int k = 1; old != -1 && old != b; ++k) {
-        if (k == kAggProbes) return -1;
-        h = (h + 1) & (kAggSlots - 1);
-        old = atomicCAS(&s_bucket[h], -1, b);
-    }
-    if (old == -1) s_list[atomicAdd(s_n, 1)] = h;
-    return h;
+    return lds_bucket_slot<kAggSlots, kAggProbes>(s_bucket, s_list, s_n, b);
 }
 
 // kTriLanes lanes per triangle (32 triangles per block).  Phase 0 counts the

@@ -164,6 +164,25 @@ __device__ __forceinline__ void wave_count(unsigned long long* counts, int slot,
     }
 }
 
+// Slot of bucket b in a block's LDS table of kSlots slots (a power of two;
+// linear probing from b's low bits), inserting b and appending the slot to
+// s_list when it is new; -1 when kProbes probes find no slot.  Slots are
+// never freed, so a bucket inserted once is found again at the same place,
+// and one that found no slot never finds one later in the launch.  s_bucket
+// starts filled with -1 and *s_n with 0.
+template <int kSlots, int kProbes>
+__device__ __forceinline__ int lds_bucket_slot(int* s_bucket, int* s_list, int* s_n, int b) {
+    int h = b & (kSlots - 1);
+    int old = atomicCAS(&s_bucket[h], -1, b);
+    for (int k = 1; old != -1 && old != b; ++k) {
+        if (k == kProbes) return -1;
+        h = (h + 1) & (kSlots - 1);
+        old = atomicCAS(&s_bucket[h], -1, b);
+    }
+    if (old == -1) s_list[atomicAdd(s_n, 1)] = h;
+    return h;
+}
+
 // Orders a wave's LDS writes before its later LDS reads of other lanes'
 // slots (a wave's LDS operations complete in issue order; this keeps the
 // compiler from moving them across, and waits for those in flight).

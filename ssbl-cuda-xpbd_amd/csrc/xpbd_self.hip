@@ -223,34 +223,82 @@ __device__ inline bool edge_box(const ClothView& c, const SelfView& sv, int pari
 // cell by the guard; a box is usually 2-3 cells a side, as the margin is one
 // cell).  kind 0 = triangles, 1 = edges; phase 0 counts, 1 scatters.
 // Key = primitive * 64 + the cell's index in the box loop (z outer, then y,
-// then x).  `t` is the thread's index within its hash's part of the launch.
+// then x).  `blk` is the block's index within its hash's part of the launch;
+// every thread of the block calls this (phase 0 has barriers).
+//
+// Neighbouring primitives share most of their cells, so phase 0 first counts
+// the block's items per bucket in an LDS table and then makes one device
+// atomic per distinct bucket: each item's slot is the block's base in its
+// bucket plus its place among the block's items there (a bucket the table
+// has no room for counts item by item).  Slots only need to be distinct
+// within a bucket: the rank pass orders the bucket by key.
 constexpr int kPrimLanes = 32;
-__device__ __forceinline__ void prim_hash_item(const ClothView& c, const SelfView& sv, int parity, int kind, int t, int phase,
-                                               int count_diag) {
+constexpr int kSelfAggSlots = 1024;
+constexpr int kSelfAggProbes = 32;
+__device__ __forceinline__ void prim_hash_block(const ClothView& c, const SelfView& sv, int parity, int kind, int blk, int phase,
+                                                int count_diag) {
+    const int t = blk * 256 + static_cast<int>(threadIdx.x);
     const int k = t / kPrimLanes;
     const int lane = t % kPrimLanes;
     const int n = kind == 0 ? sv.nt : sv.ne;
-    if (k >= n) return;
-    int b0[3], b1[3];
+    int b0[3] = {0, 0, 0}, b1[3] = {-1, -1, -1};
     bool guarded = false;
-    const bool ok = kind == 0 ? tri_box(c, sv, parity, k, b0, b1, &guarded) : edge_box(c, sv, parity, k, b0, b1, &guarded);
-    if (!ok) return;
-    if (phase == 0 && guarded && count_diag && lane == 0) atomicAdd(&c.counts[kCountSelfOverflow], 1ull);
+    bool ok = false;
+    if (k < n) ok = kind == 0 ? tri_box(c, sv, parity, k, b0, b1, &guarded) : edge_box(c, sv, parity, k, b0, b1, &guarded);
+    if (ok && phase == 0 && guarded && count_diag && lane == 0) atomicAdd(&c.counts[kCountSelfOverflow], 1ull);
     const int sx = b1[0] - b0[0] + 1, sy = b1[1] - b0[1] + 1, sz = b1[2] - b0[2] + 1;
-    if (sx <= 0 || sy <= 0 || sz <= 0) return;
+    const int cells = (ok && sx > 0 && sy > 0 && sz > 0) ? sx * sy * sz : 0;
     const HashView& h = kind == 0 ? sv.th : sv.eh;
-    for (int idx = lane; idx < sx * sy * sz; idx += kPrimLanes) {
+    auto bucket_of = [&](int idx) {
         const int x = b0[0] + idx % sx, y = b0[1] + (idx / sx) % sy, z = b0[2] + idx / (sx * sy);
-        const int b = static_cast<int>(hash_cell(x, y, z, h.buckets));
-        const int key = k * kMaxPrimCells + idx;
-        if (phase == 0) {
-            h.slot[key] = atomicAdd(&h.count[b], 1);
-        } else {
+        return static_cast<int>(hash_cell(x, y, z, h.buckets));
+    };
+    if (phase == 1) {
+        for (int idx = lane; idx < cells; idx += kPrimLanes) {
+            const int b = bucket_of(idx);
+            const int key = k * kMaxPrimCells + idx;
             const int j = h.off[b] + h.slot[key];
             h.raw[j] = key;
             h.rawb[j] = h.base + b;
         }
+        return;
     }
+    __shared__ int s_bucket[kSelfAggSlots];
+    __shared__ int s_count[kSelfAggSlots];
+    __shared__ int s_base[kSelfAggSlots];
+    __shared__ int s_list[kSelfAggSlots];
+    __shared__ int s_n;
+    for (int i = static_cast<int>(threadIdx.x); i < kSelfAggSlots; i += static_cast<int>(blockDim.x)) {
+        s_bucket[i] = -1;
+        s_count[i] = 0;
+    }
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    int hs[2] = {-1, -1}, place[2] = {0, 0};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int idx = lane + r * kPrimLanes;
+        if (idx < cells) {
+            const int b = bucket_of(idx);
+            const int slot = lds_bucket_slot<kSelfAggSlots, kSelfAggProbes>(s_bucket, s_list, &s_n, b);
+            if (slot >= 0) {
+                hs[r] = slot;
+                place[r] = atomicAdd(&s_count[slot], 1);
+            } else {
+                h.slot[k * kMaxPrimCells + idx] = atomicAdd(&h.count[b], 1);
+            }
+        }
+    }
+    __syncthreads();
+    const int distinct = s_n;
+    for (int i = static_cast<int>(threadIdx.x); i < distinct; i += static_cast<int>(blockDim.x)) {
+        const int slot = s_list[i];
+        s_base[slot] = atomicAdd(&h.count[s_bucket[slot]], s_count[slot]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+        if (hs[r] >= 0) h.slot[k * kMaxPrimCells + lane + r * kPrimLanes] = s_base[hs[r]] + place[r];
 }
 
 // One phase of all three hash builds in one launch: blocks [0, vh_blocks)
@@ -266,11 +314,11 @@ __global__ __launch_bounds__(256) void k_self_hash(const ClothView* __restrict__
     }
     blk -= sv.vh_blocks;
     if (blk < sv.th_blocks) {
-        prim_hash_item(*cv, sv, parity, 0, blk * 256 + tid, phase, count_diag);
+        prim_hash_block(*cv, sv, parity, 0, blk, phase, count_diag);
         return;
     }
     blk -= sv.th_blocks;
-    if (sv.use_ehash) prim_hash_item(*cv, sv, parity, 1, blk * 256 + tid, phase, count_diag);
+    if (sv.use_ehash) prim_hash_block(*cv, sv, parity, 1, blk, phase, count_diag);
 }
 
 // Every scattered item of the shared table finds its rank in its bucket and
