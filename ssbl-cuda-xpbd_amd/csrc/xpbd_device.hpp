@@ -80,10 +80,19 @@ struct PcgView {
     float thick, wb_limit;
     float4 *rhs, *sol, *res, *z, *dir, *adir;
     float *diag, *pinv, *off;   // 6 floats per block
+    // 1: off holds the block of incidence j of vertex v at 6 * (j * ell_pitch
+    // + v), slot-major like ClothView::ell (every vertex's incidences fit its
+    // slots); 0: at 6 * k for CSR incidence k
+    int ell_off;
+    // previous positions as they were before an uncoloured cap gather (the
+    // gather reads its neighbours' from here while it rewrites its own);
+    // null unless the cloth polishes without an edge colouring
+    float4* qsnap;
     unsigned long long* partials;   // two 64-bit words per tile of the cloth
     PcgScalars* sc;
     unsigned long long* guard;
     float* max_delta;
+    float* tile_max;   // per tile of the cloth: the largest step of the last update
     const int2* edges;
     const float* edge_rest;
     unsigned long long* counts;

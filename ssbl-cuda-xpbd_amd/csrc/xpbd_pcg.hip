@@ -65,6 +65,10 @@ constexpr int kPcgIterations = 8;                           // A41:152
 
 struct Sym { float m11, m12, m13, m22, m23, m33; };
 
+// Incidence slots whose loads a gather issues together.
+constexpr int kPcgChunk = 4;
+static_assert(kEllMax % kPcgChunk == 0, "the chunk must divide the slot count");
+
 __device__ __forceinline__ Sym load_sym(const float* p) { return Sym{p[0], p[1], p[2], p[3], p[4], p[5]}; }
 __device__ __forceinline__ void store_sym(float* p, Sym s) { p[0] = s.m11; p[1] = s.m12; p[2] = s.m13; p[3] = s.m22; p[4] = s.m23; p[5] = s.m33; }
 
@@ -90,6 +94,19 @@ __device__ inline Sym sym_inverse(Sym m, unsigned long long* guard) {
 }
 
 __device__ __forceinline__ __int128 to_fixed(float t) { return static_cast<__int128>(rint(static_cast<double>(t) * kFix)); }
+
+// Workgroup maximum of non-negative floats (compared as bit patterns, which
+// order them), returned to every thread.  Every thread must call it.
+__device__ inline float block_max_nonneg(float v) {
+    __shared__ int wave_m[kTile / 64];
+    int m = __float_as_int(fmaxf(v, 0.0f));
+    for (int d = 32; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63u) == 0u) wave_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    int r = wave_m[0];
+    for (int w = 1; w < kTile / 64; ++w) r = max(r, wave_m[w]);
+    return __int_as_float(r);
+}
 
 // Workgroup sum of one fixed-point term per thread -> partials[slot].
 __device__ inline void block_partial(__int128 v, unsigned long long* partials, int slot) {
@@ -138,12 +155,8 @@ __global__ __launch_bounds__(kTile) void k_pcg_build(const ClothView* __restrict
         const bool pinned = w == 0.0f;
         Sym d{1.0f, 0.0f, 0.0f, 1.0f, 0.0f, 1.0f};
         F3 rhs = f3(0.0f, 0.0f, 0.0f);
-        const int b = c.adj_off[v], e = c.adj_off[v + 1];
-        for (int k = b; k < e; ++k) {
-            const int2 a = c.adj[k];
-            const int o = a.x & 0x7fffffff;
+        auto incidence = [&](int2 a, float4 o4, size_t slot) {
             const float rest = __int_as_float(a.y);
-            const float4 o4 = P[o];
             const float wo = o4.w;
             Sym off{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             const float wsum = w + wo;
@@ -179,7 +192,30 @@ __global__ __launch_bounds__(kTile) void k_pcg_build(const ClothView* __restrict
                     if (w > 0.0f && wo > 0.0f) off = Sym{-blk.m11, -blk.m12, -blk.m13, -blk.m22, -blk.m23, -blk.m33};
                 }
             }
-            store_sym(pv.off + 6 * static_cast<size_t>(k), off);
+            store_sym(pv.off + 6 * slot, off);
+        };
+        if (pv.ell_off) {
+            // slot-major incidences, each chunk's loads issued together
+#pragma unroll
+            for (int j0 = 0; j0 < kEllMax; j0 += kPcgChunk) {
+                if (j0 >= c.ell_width) break;
+                int2 a[kPcgChunk];
+                float4 o4[kPcgChunk];
+#pragma unroll
+                for (int j = 0; j < kPcgChunk; ++j) {
+                    a[j] = j0 + j < c.ell_width ? c.ell[static_cast<size_t>(j0 + j) * c.ell_pitch + v] : make_int2(-1, 0);
+                    o4[j] = P[a[j].x != -1 ? (a[j].x & 0x7fffffff) : v];
+                }
+#pragma unroll
+                for (int j = 0; j < kPcgChunk; ++j)
+                    if (a[j].x != -1) incidence(a[j], o4[j], static_cast<size_t>(j0 + j) * c.ell_pitch + v);
+            }
+        } else {
+            const int b = c.adj_off[v], e = c.adj_off[v + 1];
+            for (int k = b; k < e; ++k) {
+                const int2 a = c.adj[k];
+                incidence(a, P[a.x & 0x7fffffff], static_cast<size_t>(k));
+            }
         }
         store_sym(pv.diag + 6 * static_cast<size_t>(v), d);
         const F3 zero = f3(0.0f, 0.0f, 0.0f);
@@ -210,9 +246,14 @@ __global__ __launch_bounds__(kReduceBlock) void k_pcg_reduce(const ClothView* __
     const PcgView& pv = views[members[blockIdx.x]].pcg;
     const int nparts = pv.ntiles;
     __shared__ unsigned long long lo[kReduceBlock], hi[kReduceBlock];
+    __shared__ int top[kReduceBlock];
     __int128 acc = 0;
-    for (int i = static_cast<int>(threadIdx.x); i < nparts; i += kReduceBlock)
+    int m = 0;   // mode 2 (after an update): the largest step over the tiles, as bits
+    for (int i = static_cast<int>(threadIdx.x); i < nparts; i += kReduceBlock) {
         acc += (static_cast<__int128>(pv.partials[2 * i + 1]) << 64) | pv.partials[2 * i];
+        if (mode == 2) m = max(m, __float_as_int(pv.tile_max[i]));
+    }
+    top[threadIdx.x] = m;
     lo[threadIdx.x] = static_cast<unsigned long long>(acc);
     hi[threadIdx.x] = static_cast<unsigned long long>(acc >> 64);
     __syncthreads();
@@ -223,10 +264,12 @@ __global__ __launch_bounds__(kReduceBlock) void k_pcg_reduce(const ClothView* __
             const __int128 c = a + b;
             lo[threadIdx.x] = static_cast<unsigned long long>(c);
             hi[threadIdx.x] = static_cast<unsigned long long>(c >> 64);
+            top[threadIdx.x] = max(top[threadIdx.x], top[threadIdx.x + s]);
         }
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
+    if (mode == 2 && top[0] > __float_as_int(*pv.max_delta)) *pv.max_delta = __int_as_float(top[0]);
     const __int128 total = (static_cast<__int128>(hi[0]) << 64) | lo[0];
     const double value = static_cast<double>(total) * (1.0 / kFix);
     PcgScalars& s = *pv.sc;
@@ -261,10 +304,32 @@ __global__ __launch_bounds__(kTile) void k_pcg_ad(const ClothView* __restrict__ 
         } else {
             const F3 d = f3(pv.dir[v]);
             F3 q = sym_madd(load_sym(pv.diag + 6 * static_cast<size_t>(v)), d, f3(0.0f, 0.0f, 0.0f));
-            const int b = c.adj_off[v], e = c.adj_off[v + 1];
-            for (int k = b; k < e; ++k) {
-                const int o = c.adj[k].x & 0x7fffffff;
-                q = sym_madd(load_sym(pv.off + 6 * static_cast<size_t>(k)), f3(pv.dir[o]), q);
+            if (pv.ell_off) {
+                // slot-major incidences and blocks, each chunk's loads issued
+                // together; the sum runs in incidence order as below
+#pragma unroll
+                for (int j0 = 0; j0 < kEllMax; j0 += kPcgChunk) {
+                    if (j0 >= c.ell_width) break;
+                    int x[kPcgChunk];
+                    float4 dj[kPcgChunk];
+                    Sym m[kPcgChunk];
+#pragma unroll
+                    for (int j = 0; j < kPcgChunk; ++j) {
+                        const size_t slot = static_cast<size_t>(j0 + j) * c.ell_pitch + v;
+                        x[j] = j0 + j < c.ell_width ? c.ell[slot].x : -1;
+                        dj[j] = pv.dir[x[j] != -1 ? (x[j] & 0x7fffffff) : v];
+                        m[j] = load_sym(pv.off + 6 * (x[j] != -1 ? slot : static_cast<size_t>(v)));
+                    }
+#pragma unroll
+                    for (int j = 0; j < kPcgChunk; ++j)
+                        if (x[j] != -1) q = sym_madd(m[j], f3(dj[j]), q);
+                }
+            } else {
+                const int b = c.adj_off[v], e = c.adj_off[v + 1];
+                for (int k = b; k < e; ++k) {
+                    const int o = c.adj[k].x & 0x7fffffff;
+                    q = sym_madd(load_sym(pv.off + 6 * static_cast<size_t>(k)), f3(pv.dir[o]), q);
+                }
             }
             pv.adir[v] = with_w(q, 0.0f);
             term = to_fixed(dot(d, q));
@@ -323,10 +388,13 @@ __global__ __launch_bounds__(kTile) void k_pcg_update(const ClothView* __restric
             if (dl > 0.0f) moved = dl;
         }
     }
-    // the clamp count and the largest step (both order-free) with one atomic
-    // per wave instead of one per vertex on a single address
+    // the clamp count with one atomic per wave; the largest step per
+    // workgroup, merged into the cloth's maximum by the reduction that
+    // follows (one atomic per wave on the cloth's one address serialises
+    // on large cloths)
     wave_count(c.counts, kCountHardFallbacks, clamped);
-    wave_max_nonneg(pv.max_delta, moved);
+    const float block_moved = block_max_nonneg(moved);
+    if (threadIdx.x == 0) pv.tile_max[tile.begin / kTile] = block_moved;
     // every workgroup of the cloth derives the same status; its first tile's records it
     if (tile.begin == 0 && threadIdx.x == 0) {
         pv.sc->status = status;
@@ -434,22 +502,49 @@ __global__ __launch_bounds__(kTile) void k_edge_gather(const ClothView* __restri
     const float dt = c.dt_sub;
     const float4* S = src_parity ? c.pos1 : c.pos0;
     float4* D = src_parity ? c.pos0 : c.pos1;
+    if (kind == -2) { pv.qsnap[v] = c.prev[v]; return; }   // snapshot of the previous positions
     const float4 p4 = S[v];
     if (kind < 0) { D[v] = p4; return; }   // plain copy (parity realignment)
     const F3 p0 = f3(p4), q0 = f3(c.prev[v]);
+    // kinds 0-2 use the neighbours' previous positions: from the snapshot,
+    // since this launch rewrites them (kinds 3-4 only test them for being
+    // finite, which the rewrite keeps)
+    const float4* Qn = kind <= 2 && pv.qsnap ? pv.qsnap : c.prev;
     const float w = p4.w;
     F3 p = p0, q = q0, sum = f3(0.0f, 0.0f, 0.0f);
     int cnt = 0;
-    const int b = c.adj_off[v], e = c.adj_off[v + 1];
-    for (int k = b; k < e; ++k) {
-        const int2 a = c.adj[k];
-        const int o = a.x & 0x7fffffff;
-        const float4 o4 = S[o];
+    auto incidence = [&](int2 a, float4 o4, float4 qo) {
         EdgeOut out;
-        if (!edge_kind(kind, p0, f3(o4), q0, f3(c.prev[o]), w, o4.w, __int_as_float(a.y), pv.thick, pv.strength_cfg, dt, &out)) continue;
-        if (!out.apply) continue;
+        if (!edge_kind(kind, p0, f3(o4), q0, f3(qo), w, o4.w, __int_as_float(a.y), pv.thick, pv.strength_cfg, dt, &out)) return;
+        if (!out.apply) return;
         if (kind >= 3) { sum = sum + out.d; ++cnt; }
         else { p = p + out.d; q = q + out.q; }
+    };
+    if (pv.ell_off) {
+        // slot-major incidences (the same order), each chunk's loads issued together
+#pragma unroll
+        for (int j0 = 0; j0 < kEllMax; j0 += kPcgChunk) {
+            if (j0 >= c.ell_width) break;
+            int2 a[kPcgChunk];
+            float4 o4[kPcgChunk], qo[kPcgChunk];
+#pragma unroll
+            for (int j = 0; j < kPcgChunk; ++j) {
+                a[j] = j0 + j < c.ell_width ? c.ell[static_cast<size_t>(j0 + j) * c.ell_pitch + v] : make_int2(-1, 0);
+                const int o = a[j].x != -1 ? (a[j].x & 0x7fffffff) : v;
+                o4[j] = S[o];
+                qo[j] = Qn[o];
+            }
+#pragma unroll
+            for (int j = 0; j < kPcgChunk; ++j)
+                if (a[j].x != -1) incidence(a[j], o4[j], qo[j]);
+        }
+    } else {
+        const int b = c.adj_off[v], e = c.adj_off[v + 1];
+        for (int k = b; k < e; ++k) {
+            const int2 a = c.adj[k];
+            const int o = a.x & 0x7fffffff;
+            incidence(a, S[o], Qn[o]);
+        }
     }
     if (kind >= 3 && cnt > 0 && w != 0.0f) {
         const float md = kind == 3 ? fmaxf(5.0e-4f, fminf(fmaxf(pv.thick * 0.20f, 1.0e-3f), 0.005f))
@@ -518,7 +613,8 @@ void* PcgPath::alloc_bytes(size_t bytes) {
 }
 
 bool PcgPath::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, const std::vector<float>& edge_rest,
-                      const std::vector<int>& colors, int nv, int nadj, int ntiles, unsigned long long* counts) {
+                      const std::vector<int>& colors, int nv, int nadj, int ntiles, int ell_width, int ell_pitch,
+                      int ell_overflow, unsigned long long* counts) {
     ready_ = false;
     if (!cfg.stretch_optimization_enabled) return true;
     if (nadj <= 0) return fail("PCG stretch optimization found no valid CSR edges");
@@ -536,18 +632,22 @@ bool PcgPath::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, 
     v.hard = (cfg.stretch_optimization_strength >= 0.95f && std::isfinite(cfg.cloth_thickness) && cfg.cloth_thickness > 1.0e-7f) ? 1 : 0;
     v.wb_limit = v.hard ? std::max(2.5e-4f, std::min(cfg.cloth_thickness * 0.035f, 9.0e-4f)) : 0.25f;   // A41:1091-1105
     v.counts = counts;
+    v.ell_off = (ell_overflow == 0 && ell_width > 0) ? 1 : 0;
+    off_blocks_ = v.ell_off ? static_cast<size_t>(ell_width) * static_cast<size_t>(ell_pitch) : static_cast<size_t>(nadj);
     auto f4 = [&](float4** p) { *p = static_cast<float4*>(alloc_bytes(sizeof(float4) * nv)); return *p != nullptr; };
     bool ok = f4(&v.rhs) && f4(&v.sol) && f4(&v.res) && f4(&v.z) && f4(&v.dir) && f4(&v.adir);
     v.diag = static_cast<float*>(alloc_bytes(sizeof(float) * 6 * static_cast<size_t>(nv)));
     v.pinv = static_cast<float*>(alloc_bytes(sizeof(float) * 6 * static_cast<size_t>(nv)));
-    v.off = static_cast<float*>(alloc_bytes(sizeof(float) * 6 * static_cast<size_t>(nadj)));
+    v.off = static_cast<float*>(alloc_bytes(sizeof(float) * 6 * off_blocks_));
     v.partials = static_cast<unsigned long long*>(alloc_bytes(sizeof(unsigned long long) * 2 * static_cast<size_t>(ntiles)));
     v.sc = static_cast<PcgScalars*>(alloc_bytes(sizeof(PcgScalars)));
     v.guard = static_cast<unsigned long long*>(alloc_bytes(sizeof(unsigned long long)));
     v.max_delta = static_cast<float*>(alloc_bytes(sizeof(float)));
+    v.tile_max = static_cast<float*>(alloc_bytes(sizeof(float) * static_cast<size_t>(ntiles)));
     int2* d_edges = static_cast<int2*>(alloc_bytes(sizeof(int2) * std::max(ne, 1)));
     float* d_rest = static_cast<float*>(alloc_bytes(sizeof(float) * std::max(ne, 1)));
-    if (!ok || !v.diag || !v.pinv || !v.off || !v.partials || !v.sc || !v.guard || !v.max_delta || !d_edges || !d_rest) return false;
+    if (!ok || !v.diag || !v.pinv || !v.off || !v.partials || !v.sc || !v.guard || !v.max_delta || !v.tile_max || !d_edges || !d_rest)
+        return false;
     std::vector<int2> he(ne);
     for (int e = 0; e < ne; ++e) he[e] = make_int2(edges[2 * e], edges[2 * e + 1]);
     if (ne > 0 && (!hip_ok(hipMemcpy(d_edges, he.data(), sizeof(int2) * ne, hipMemcpyHostToDevice), "PCG edge upload") ||
@@ -585,6 +685,10 @@ bool PcgPath::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, 
             seg_count_.push_back(static_cast<int>(segs_host_.size()) - seg_first_.back());
         }
     }
+    if (!colored_ && v.strength_cfg >= 0.95f && ne > 0) {
+        v.qsnap = static_cast<float4*>(alloc_bytes(sizeof(float4) * nv));
+        if (!v.qsnap) return false;
+    }
     own_ = pv_;
     ready_ = true;
     return true;
@@ -600,8 +704,9 @@ size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 size_t PcgPath::scratch_bytes() const {
     const size_t nv = static_cast<size_t>(own_.nv);
     return 6 * align256(sizeof(float4) * nv) + 2 * align256(sizeof(float) * 6 * nv) +
-           align256(sizeof(float) * 6 * static_cast<size_t>(nnz_)) + align256(sizeof(unsigned long long) * 2 * static_cast<size_t>(ntiles_)) +
-           align256(sizeof(PcgScalars)) + 2 * align256(sizeof(unsigned long long));
+           align256(sizeof(float) * 6 * off_blocks_) + align256(sizeof(unsigned long long) * 2 * static_cast<size_t>(ntiles_)) +
+           align256(sizeof(PcgScalars)) + 2 * align256(sizeof(unsigned long long)) + align256(sizeof(float) * static_cast<size_t>(ntiles_)) +
+           (own_.qsnap ? align256(sizeof(float4) * nv) : 0);
 }
 
 bool PcgPath::use_scratch(void* base, size_t bytes) {
@@ -622,11 +727,13 @@ bool PcgPath::use_scratch(void* base, size_t bytes) {
     v.adir = reinterpret_cast<float4*>(take(sizeof(float4) * nv));
     v.diag = reinterpret_cast<float*>(take(sizeof(float) * 6 * nv));
     v.pinv = reinterpret_cast<float*>(take(sizeof(float) * 6 * nv));
-    v.off = reinterpret_cast<float*>(take(sizeof(float) * 6 * static_cast<size_t>(nnz_)));
+    v.off = reinterpret_cast<float*>(take(sizeof(float) * 6 * off_blocks_));
     v.partials = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 2 * static_cast<size_t>(ntiles_)));
     v.sc = reinterpret_cast<PcgScalars*>(take(sizeof(PcgScalars)));
     v.guard = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long)));
     v.max_delta = reinterpret_cast<float*>(take(sizeof(unsigned long long)));
+    v.tile_max = reinterpret_cast<float*>(take(sizeof(float) * static_cast<size_t>(ntiles_)));
+    if (own_.qsnap) v.qsnap = reinterpret_cast<float4*>(take(sizeof(float4) * nv));
     pv_ = v;
     return true;
 }
@@ -721,7 +828,13 @@ int PcgGroup::gather(hipStream_t st, const ClothView* views, const Polishing& se
 int PcgGroup::caps(hipStream_t st, const ClothView* views, const Polishing& set, int parity, int kind, int passes,
                    bool reverse_too) const {
     if (!set.colored) {
-        for (int p = 0; p < passes; ++p) parity = gather(st, views, set, parity, kind);
+        for (int p = 0; p < passes; ++p) {
+            if (kind <= 2) {
+                TimedLaunch t("edge_gather", st);
+                launch_k(k_edge_gather, dim3(set.ntiles), dim3(kTile), 0, st, views, set.tiles.p, set.ntiles, parity, -2);
+            }
+            parity = gather(st, views, set, parity, kind);
+        }
         return parity;
     }
     TimedLaunch t("edge_color", st);

@@ -32,7 +32,8 @@ public:
     PcgPath& operator=(const PcgPath&) = delete;
     ~PcgPath();
     bool prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, const std::vector<float>& edge_rest,
-                 const std::vector<int>& colors, int nv, int nadj, int ntiles, unsigned long long* counts);
+                 const std::vector<int>& colors, int nv, int nadj, int ntiles, int ell_width, int ell_pitch, int ell_overflow,
+                 unsigned long long* counts);
     bool enabled() const { return ready_; }
     // polish and caps run for this cloth (strength >= 0.95, A41:7823-7857)
     bool polishes() const { return ready_ && pv_.strength_cfg >= 0.95f && pv_.ne > 0; }
@@ -56,6 +57,7 @@ private:
     bool ready_ = false;
     bool colored_ = false;
     int nnz_ = 0, ntiles_ = 0;
+    size_t off_blocks_ = 0;   // off-diagonal blocks stored (slot-major slots, or the CSR incidences)
     PcgView pv_{};    // the view the launches use
     PcgView own_{};   // the view over the handle's own arrays
     std::vector<int4> segs_host_;

@@ -818,7 +818,8 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     if (!hip_ok(hipMemset(counts.p, 0, sizeof(unsigned long long) * kCountSlots), "diagnostic reset")) return false;
     if (!hip_ok(hipMemset(max_smoothed.p, 0, sizeof(float)), "diagnostic reset")) return false;
     if (cfg.self_collision && !self.prepare(cfg, edges, tri, n, counts.p)) return false;
-    if (cfg.stretch_optimization_enabled && !pcg.prepare(cfg, edges, edge_rest, edge_colors, n, h_adj_off[n], ntiles, counts.p))
+    if (cfg.stretch_optimization_enabled && !pcg.prepare(cfg, edges, edge_rest, edge_colors, n, h_adj_off[n], ntiles, ell_width, ell_pitch,
+                                                           ell_overflow, counts.p))
         return false;
     if (pcg.enabled() && !pcg_group.build({PcgMember{0, &pcg, n}}, pcg_cadence)) return false;
     // A41:7193-7302: bending "wings" (an edge met again with another opposite
