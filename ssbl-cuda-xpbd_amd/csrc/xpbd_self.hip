@@ -321,24 +321,90 @@ __global__ __launch_bounds__(256) void k_self_hash(const ClothView* __restrict__
     if (sv.use_ehash) prim_hash_block(*cv, sv, parity, 1, blk, phase, count_diag);
 }
 
+constexpr int kBigBucket = 64;   // buckets with more items are ranked by one wave each
+
+// Ascending bitonic sort of one value per lane across the wave.
+__device__ __forceinline__ int wave_sort64(int v) {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int o = __shfl_xor(v, j, 64);
+            v = (((lane & j) == 0) == ((lane & k) == 0)) ? min(v, o) : max(v, o);
+        }
+    }
+    return v;
+}
+
+// The 32 smallest keys of every bucket of more than kBigBucket items, one
+// wave per bucket: each wave scans a group of 64 buckets and takes its big
+// ones in turn.  Each 64-key chunk is sorted across the lanes and merged
+// into the running 32 smallest (ascending in lanes 0-31, the chunk's 32
+// smallest descending in lanes 32-63: a bitonic sequence, merged ascending).
+// The same entries, in the same order, as the per-item ranks (keys are
+// distinct); the other n - 32 items count as overflow.
+__device__ inline void big_bucket_ranks(const SelfView& sv, const int* __restrict__ raw, int* __restrict__ entries, int count_diag,
+                                        int wave, int nwaves) {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    for (int g = wave * 64; g < sv.all_buckets; g += nwaves * 64) {
+        const int mine = g + lane < sv.all_buckets ? sv.all_count[g + lane] : 0;
+        unsigned long long big = __ballot(mine > kBigBucket);
+        while (big != 0ull) {
+            const int at = __ffsll(static_cast<long long>(big)) - 1;
+            big &= big - 1ull;
+            const int b = g + at;
+            const int base = sv.all_off[b], n = __shfl(mine, at, 64);
+            int top = 0x7fffffff;
+            for (int c0 = 0; c0 < n; c0 += 64) {
+                const int v = wave_sort64(c0 + lane < n ? raw[base + c0 + lane] : 0x7fffffff);
+                // every lane takes part in the shuffle (a lane reading from
+                // one that skips it would not get that lane's value)
+                const int reversed = __shfl(v, 63 - lane, 64);
+                int m = lane < 32 ? top : reversed;
+#pragma unroll
+                for (int j = 32; j > 0; j >>= 1) {
+                    const int o = __shfl_xor(m, j, 64);
+                    m = (lane & j) == 0 ? min(m, o) : max(m, o);
+                }
+                top = lane < 32 ? m : 0x7fffffff;
+            }
+            if (lane < kBucketSlots) entries[base + lane] = top;
+            if (count_diag && lane == 0) atomicAdd(&sv.counts[kCountSelfOverflow], static_cast<unsigned long long>(n - kBucketSlots));
+        }
+    }
+}
+
 // Every scattered item of the shared table finds its rank in its bucket and
 // lands there if it is among the first 32 (the others count as overflow).
 // A grid-stride loop over the items actually scattered (their total is the
-// scan's last offset); every lane runs the same number of rounds.
+// scan's last offset) by the first item_blocks blocks; every lane runs the
+// same number of rounds.  Buckets of more than kBigBucket items are left to
+// the other blocks (big_bucket_ranks): the two parts touch different buckets.
 __global__ __launch_bounds__(256) void k_self_rank(SelfView sv, const int* __restrict__ raw, const int* __restrict__ rawb,
-                                                   int* __restrict__ entries, int count_diag) {
+                                                   int* __restrict__ entries, int count_diag, int item_blocks) {
+    if (static_cast<int>(blockIdx.x) >= item_blocks) {
+        const int waves = (static_cast<int>(gridDim.x) - item_blocks) * (static_cast<int>(blockDim.x) >> 6);
+        big_bucket_ranks(sv, raw, entries, count_diag,
+                         (static_cast<int>(blockIdx.x) - item_blocks) * (static_cast<int>(blockDim.x) >> 6) +
+                             static_cast<int>(threadIdx.x >> 6),
+                         waves);
+        return;
+    }
     const int total = sv.all_off[sv.all_buckets];
-    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    const int stride = item_blocks * static_cast<int>(blockDim.x);
     const int rounds = (total + stride - 1) / stride;
     for (int r = 0; r < rounds; ++r) {
         const int j = r * stride + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
         bool over = false;
         if (j < total) {
             const int b = rawb[j], key = raw[j];
-            const int base = sv.all_off[b];
-            const int rank = bucket_rank(raw, base, sv.all_count[b], key);
-            if (rank < kBucketSlots) entries[base + rank] = key;
-            else over = true;
+            const int base = sv.all_off[b], n = sv.all_count[b];
+            if (n <= kBigBucket) {
+                const int rank = bucket_rank(raw, base, n, key);
+                if (rank < kBucketSlots) entries[base + rank] = key;
+                else over = true;
+            }
         }
         if (count_diag) wave_count(sv.counts, kCountSelfOverflow, over);
     }
@@ -1274,8 +1340,11 @@ bool SelfCollision::run_for(hipStream_t st, const ClothView* cv, int parity, boo
         // about one item per vertex and 27 per primitive (3 cells a side): one thread each
         const long long rank_threads = std::min<long long>(
             s.hash_items, (s.use_vhash ? s.nv : 0) + 32ll * ((s.use_thash ? s.nt : 0) + (s.use_ehash ? s.ne : 0)));
-        launch_k(k_self_rank, dim3(grid_for(std::max<long long>(rank_threads, 1), 256)), block, 0, st, s, any.raw, any.rawb,
-                 any.entries, count ? 1 : 0);
+        // and a wave per group of 64 buckets for the big ones, four waves per block
+        const unsigned item_blocks = grid_for(std::max<long long>(rank_threads, 1), 256);
+        const unsigned group_blocks = grid_for((static_cast<long long>(s.all_buckets) + 63) / 64, 4);
+        launch_k(k_self_rank, dim3(item_blocks + group_blocks), block, 0, st, s, any.raw, any.rawb, any.entries, count ? 1 : 0,
+                 static_cast<int>(item_blocks));
     }
     const dim3 contacts_grid(grid_for(static_cast<long long>(s.nv) + s.ne, 256));
     if (s.fast && count) {
