@@ -171,6 +171,12 @@ struct ClothView {
     int ell_pitch;
     int ell_overflow;
     int pad3;
+    // The same slots in 6 bytes for the stretch gather, when every
+    // neighbour of the cloth lies within 16383 indices (else null, and ell
+    // is used): ell16 = (other - v) * 2 + first-endpoint bit (0: unused),
+    // ell_rest = the rest length.
+    const short* ell16;
+    const float* ell_rest;
     // Dynamic colliders (null when none): the cloth's own, or an attached
     // global scene filtered by source and layer (A41:3984-3997).
     const DynView* dyn;

@@ -1306,6 +1306,24 @@ __device__ __forceinline__ void stretch_edge(int2 a, float4 o4, F3 p0, float w, 
 // slots at or past the width.  Requires ell_width > 0.
 __device__ __forceinline__ void ell_load(const ClothView& c, int v, int2 (&a)[kEllMax]) {
     const int last = c.ell_width - 1;
+    if (c.ell16) {
+        // 6-byte slots, expanded to the 8-byte form
+        short d[kEllMax];
+        float r[kEllMax];
+#pragma unroll
+        for (int j = 0; j < kEllMax; ++j) {
+            const unsigned slot = static_cast<unsigned>(min(j, last) * c.ell_pitch + v);
+            d[j] = load_at(c.ell16, slot);
+            r[j] = load_at(c.ell_rest, slot);
+        }
+#pragma unroll
+        for (int j = 0; j < kEllMax; ++j) {
+            const int code = d[j];
+            const unsigned other = static_cast<unsigned>(v + (code >> 1)) | ((code & 1) ? 0x80000000u : 0u);
+            a[j] = code == 0 ? make_int2(-1, 0) : make_int2(static_cast<int>(other), __float_as_int(r[j]));
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < kEllMax; ++j) a[j] = load_at(c.ell, static_cast<unsigned>(min(j, last) * c.ell_pitch + v));
 }

@@ -94,6 +94,7 @@ __device__ __forceinline__ float4 gload(const float4* p) {
 }
 typedef int gv4i __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ int gload(const int* p) { return *(const SSBL_GLOBAL int*)(p); }
+__device__ __forceinline__ short gload(const short* p) { return *(const SSBL_GLOBAL short*)(p); }
 __device__ __forceinline__ int4 gload(const int4* p) {
     const gv4i x = *(const SSBL_GLOBAL gv4i*)(p);
     return make_int4(x.x, x.y, x.z, x.w);

@@ -420,6 +420,8 @@ struct Solver {
     Dev<float4> pos0, pos1, prev, vel, rest;
     Dev<int> adj_off, lra_off, tri_off, tri_inc, pin_off, pin_ids, bend_off;
     Dev<int2> adj, bends, ell;
+    Dev<short> ell16;
+    Dev<float> ell_rest;
     int ell_width = 0, ell_pitch = 0, ell_overflow = 0;
     Dev<int4> lra_adj, tris, bend_segs, bend_adj;
     Dev<float> bend_rest, area_share, rest_scale, sdf, static_tris;
@@ -669,6 +671,30 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < std::min(h_adj_off[i + 1] - h_adj_off[i], ell_width); ++j)
             h_ell[static_cast<size_t>(j) * ell_pitch + i] = h_adj[h_adj_off[i] + j];
+    // 6-byte slots (ClothView::ell16, ell_rest) for a cloth on the direct
+    // stretch whose neighbours all lie within 16383 indices; the 8-byte table
+    // is then not needed on the device
+    bool compact_ell = !cfg.stretch_optimization_enabled && ell_width > 0;
+    std::vector<short> h_ell16;
+    std::vector<float> h_ell_rest;
+    if (compact_ell) {
+        h_ell16.assign(h_ell.size(), 0);
+        h_ell_rest.assign(h_ell.size(), 0.0f);
+        for (size_t k = 0; k < h_ell.size() && compact_ell; ++k) {
+            const int2 a = h_ell[k];
+            if (a.x == -1) continue;
+            const int v = static_cast<int>(k % static_cast<size_t>(ell_pitch));
+            const int d = (a.x & 0x7fffffff) - v;
+            if (d < -16384 || d > 16383) { compact_ell = false; break; }
+            h_ell16[k] = static_cast<short>(d * 2 + (a.x < 0 ? 1 : 0));
+            std::memcpy(&h_ell_rest[k], &a.y, 4);
+        }
+        if (!compact_ell) {
+            h_ell16.clear();
+            h_ell_rest.clear();
+        }
+    }
+    if (compact_ell) h_ell.clear();
     edge_colors.clear();
     if (cfg.edge_color_count > 0) {
         if (!mesh->edge_color_offsets) return fail("ABI41 edge color offsets are required");
@@ -805,7 +831,8 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     bool ok = pos0.upload(h_pos, "position allocation") && pos1.upload(h_pos, "position allocation") &&
               prev.upload(h_prev, "previous position allocation") && vel.upload(h_zero, "velocity allocation") &&
               rest.upload(h_rest, "rest position allocation") && adj_off.upload(h_adj_off, "edge incidence allocation") &&
-              adj.upload(h_adj, "edge incidence allocation") && ell.upload(h_ell, "edge incidence allocation") && lra_off.upload(h_lra_off, "LRA incidence allocation") &&
+              adj.upload(h_adj, "edge incidence allocation") && ell.upload(h_ell, "edge incidence allocation") &&
+              ell16.upload(h_ell16, "edge incidence allocation") && ell_rest.upload(h_ell_rest, "edge incidence allocation") && lra_off.upload(h_lra_off, "LRA incidence allocation") &&
               lra_adj.upload(h_lra, "LRA incidence allocation") && tri_off.upload(h_tri_off, "triangle incidence allocation") &&
               tri_inc.upload(h_tri_inc, "triangle incidence allocation") && tris.upload(h_tris, "triangle allocation") &&
               bends.upload(h_bends, "bend allocation") && bend_rest.upload(h_bend_rest, "bend rest allocation") &&
@@ -1049,7 +1076,7 @@ void Solver::fill_view(int substeps) {
     v.npin = static_cast<int>(pins.size()); v.nfield = static_cast<int>(fields.size());
     v.pos0 = pos0.p; v.pos1 = pos1.p; v.prev = prev.p; v.vel = vel.p; v.rest = rest.p;
     v.adj_off = adj_off.p; v.adj = adj.p;
-    v.ell = ell.p; v.ell_width = ell_width; v.ell_pitch = ell_pitch; v.ell_overflow = ell_overflow; v.lra_off = lra_off.p; v.lra_adj = lra_adj.p;
+    v.ell = ell.p; v.ell16 = ell16.p; v.ell_rest = ell_rest.p; v.ell_width = ell_width; v.ell_pitch = ell_pitch; v.ell_overflow = ell_overflow; v.lra_off = lra_off.p; v.lra_adj = lra_adj.p;
     v.tri_off = tri_off.p; v.tri_inc = tri_inc.p; v.tris = tris.p; v.bends = bends.p; v.bend_rest = bend_rest.p;
     v.bend_off = bend_off.p; v.bend_adj = bend_adj.p;
     v.pin_off = pin_off.p; v.pin_ids = pin_ids.p; v.pins = d_pins.p;
