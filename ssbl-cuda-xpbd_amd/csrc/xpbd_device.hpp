@@ -29,6 +29,7 @@ constexpr int kNeighborSlots = 32;      // vertex-vertex candidates per vertex
 constexpr int kBucketSlots = 32;        // entries a query sees per hash bucket
 constexpr int kMaxForceFields = 64;
 constexpr int kEllMax = 8;              // incidence slots held slot-major per vertex
+constexpr int kStageHaloMax = 384;      // widest neighbour distance a staged stretch gather covers
 constexpr double kFixedScale = 4294967296.0;  // 2^32: self-collision accumulators
 
 enum CountSlot {
@@ -170,7 +171,11 @@ struct ClothView {
     int ell_width;
     int ell_pitch;
     int ell_overflow;
-    int pad3;
+    // Stretch gathers of this cloth read their neighbours from LDS: a
+    // workgroup first copies the source positions of its tile and of the
+    // stage_halo indices either side (every ELL neighbour of the cloth lies
+    // within that distance, kStageHaloMax at most).  0: gathered from HBM.
+    int stage_halo;
     // The same slots in 6 bytes for the stretch gather, when every
     // neighbour of the cloth lies within 16383 indices (else null, and ell
     // is used): ell16 = (other - v) * 2 + first-endpoint bit (0: unused),

@@ -1316,11 +1316,13 @@ __device__ __forceinline__ void ell_load(const ClothView& c, int v, int2 (&a)[kE
             d[j] = load_at(c.ell16, slot);
             r[j] = load_at(c.ell_rest, slot);
         }
+        // an unused slot holds code 0 and rest length 0, so the rest length
+        // is taken as loaded (no load waits on its slot's code)
 #pragma unroll
         for (int j = 0; j < kEllMax; ++j) {
             const int code = d[j];
             const unsigned other = static_cast<unsigned>(v + (code >> 1)) | ((code & 1) ? 0x80000000u : 0u);
-            a[j] = code == 0 ? make_int2(-1, 0) : make_int2(static_cast<int>(other), __float_as_int(r[j]));
+            a[j] = make_int2(code == 0 ? -1 : static_cast<int>(other), __float_as_int(r[j]));
         }
         return;
     }
@@ -1328,11 +1330,38 @@ __device__ __forceinline__ void ell_load(const ClothView& c, int v, int2 (&a)[kE
     for (int j = 0; j < kEllMax; ++j) a[j] = load_at(c.ell, static_cast<unsigned>(min(j, last) * c.ell_pitch + v));
 }
 
+// Neighbour positions of a gather: from the pass's source buffer in HBM, or
+// from the workgroup's LDS copy of it (StagedSource, indices lo .. lo + n).
+struct GlobalSource {
+    const float4* src;
+    __device__ __forceinline__ float4 operator()(unsigned i) const { return load_at(src, i); }
+};
+struct StagedSource {
+    const float4* stage;
+    int lo;
+    __device__ __forceinline__ float4 operator()(unsigned i) const { return stage[static_cast<int>(i) - lo]; }
+};
+
+// Copies src[lo, lo + n) (n <= kStageMax) to stage[0, n).  Each wave
+// instruction moves 64 consecutive positions from global memory straight
+// into LDS, with no registers in between; lanes past n load position n - 1
+// again into slots no gather reads.
+constexpr int kStageMax = kTile + 2 * kStageHaloMax;
+__device__ __forceinline__ void stage_positions(const float4* __restrict__ src, int lo, int n, float4* stage) {
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    for (int ch = static_cast<int>(threadIdx.x) >> 6; ch * 64 < n; ch += kTile / 64) {
+        const int i = min(ch * 64 + lane, n - 1);
+        __builtin_amdgcn_global_load_lds(src + lo + i, (__attribute__((address_space(3))) void*)(stage + ch * 64),
+                                         16, 0, 0);
+    }
+}
+
 // Direct stretch gathered per vertex, incident edges in ascending edge order:
 // the slot-major first ell_width incidences (a, loaded by ell_load), then any
-// overflow from adj.
-__device__ inline void stretch_gather(const ClothView& c, const float4* __restrict__ src, VertexState& s, float w,
-                                      const int2 (&a)[kEllMax], bool count) {
+// overflow from adj (read from src in HBM).
+template <typename Source>
+__device__ inline void stretch_gather(const ClothView& c, Source fetch, const float4* __restrict__ src, VertexState& s,
+                                      float w, const int2 (&a)[kEllMax], bool count) {
     const int v = s.v;
     const int width = c.ell_width;
     const float alpha = c.stretch_alpha;
@@ -1350,7 +1379,7 @@ __device__ inline void stretch_gather(const ClothView& c, const float4* __restri
         for (int j = 0; j < kGatherChunk; ++j) {
             const int x = a[j0 + j].x;
             live[j] = j0 + j < width && x != -1;
-            o[j] = load_at(src, static_cast<unsigned>(live[j] ? (x & 0x7fffffff) : v));
+            o[j] = fetch(static_cast<unsigned>(live[j] ? (x & 0x7fffffff) : v));
         }
 #pragma unroll
         for (int j = 0; j < kGatherChunk; ++j)
@@ -1464,54 +1493,10 @@ __device__ inline void bend_gather(const ClothView& c, const float4* __restrict_
 
 enum GatherKind { kGatherNone = 0, kGatherStretch = 1, kGatherLra = 2, kGatherBend = 3 };
 
-// One solver iteration for every vertex of the tile: the constraint gather
-// selected at compile time, then the vertex-local steps selected by `ops`.
-template <int kGather>
-__global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
-                                                       int ntiles, int src_parity, int dst_parity, unsigned ops) {
-    const Tile tile = tiles[xcd_tile(ntiles)];
-    const int v = tile.begin + static_cast<int>(threadIdx.x);
-    if (v >= tile.end) return;
-    const ClothView& c = views[tile.cloth];
-    const float4* src = src_parity ? c.pos1 : c.pos0;
-    float4* dst = dst_parity ? c.pos1 : c.pos0;
-    const float dt = c.dt_sub;
-    const bool count = (ops & kOpCount) != 0u;
-    // Everything the stretch gather reads about the vertex itself (its
-    // incidences, position and previous position) is requested before any of
-    // it is used: one memory round trip, then one per chunk of neighbours.
-    bool gather = false;
-    int2 ell[kEllMax];
-    float4 q4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if constexpr (kGather == kGatherStretch) {
-        gather = c.direct_stretch != 0;
-        if (gather) {
-            if (c.ell_width > 0) {
-                ell_load(c, v, ell);
-            } else {
-#pragma unroll
-                for (int j = 0; j < kEllMax; ++j) ell[j] = make_int2(-1, 0);   // never read: width 0
-            }
-            q4 = gload(c.prev + v);
-        }
-    }
-    const float4 p4 = gload(src + v);
-    const float w = p4.w;
-    // .w holds 0 exactly for the vertices the reference flags as pinned
-    // (inverse mass <= 0, A41:8861-8865)
-    const bool pinned = w == 0.0f;
-    VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
-    if constexpr (kGather == kGatherStretch) {
-        if (gather) {
-            s.q_ = f3(q4);
-            s.q_have = true;
-            stretch_gather(c, src, s, w, ell, count);
-        }
-    } else if constexpr (kGather == kGatherLra) {
-        lra_gather(c, src, s, w, dt, count);
-    } else if constexpr (kGather == kGatherBend) {
-        bend_gather(c, src, s, w, count);
-    }
+// The vertex-local steps of a pass after its constraint gather, then the
+// vertex's writes.
+__device__ __forceinline__ void vertex_tail(const ClothView& c, VertexState& s, float w, bool pinned, float dt, float4* dst,
+                                            unsigned ops, bool count) {
     if (ops & kOpPin) apply_pins(c, s, pinned, dt);
     if ((ops & kOpAnalytic) && !pinned) analytic_contacts(c, s, dt, count);
     if ((ops & kOpSdf) && c.sdf_ready && !pinned) sdf_contact(c, s, dt, count);
@@ -1522,9 +1507,97 @@ __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __res
         if (pinned) s.set_vel(f3(0.0f, 0.0f, 0.0f));
         else velocity_update(c, s, dt);
     }
-    gstore(dst + v, with_w(s.p, w));
-    if (s.q_dirty) gstore(c.prev + v, with_w(s.q_, 0.0f));
-    if (s.v_dirty) gstore(c.vel + v, with_w(s.vel_, 0.0f));
+    gstore(dst + s.v, with_w(s.p, w));
+    if (s.q_dirty) gstore(c.prev + s.v, with_w(s.q_, 0.0f));
+    if (s.v_dirty) gstore(c.vel + s.v, with_w(s.vel_, 0.0f));
+}
+
+// The stretch pass of a cloth with stage_halo > 0.  The workgroup copies the
+// source positions of its tile and of stage_halo indices either side into
+// LDS while each vertex's incidence slots and previous position load; after
+// one barrier the vertex's own position and every ELL neighbour come from
+// LDS, so the pass makes one round trip to memory before its arithmetic.
+// Same arithmetic, same order as the gather from HBM.
+__device__ inline void vertex_pass_staged(const ClothView& c, const Tile& tile, const float4* __restrict__ src,
+                                          float4* __restrict__ dst, unsigned ops, float4* stage) {
+    const int v = tile.begin + static_cast<int>(threadIdx.x);
+    const bool live = v < tile.end;
+    const int lo = max(tile.begin - c.stage_halo, 0);
+    const int hi = min(tile.end + c.stage_halo, c.nv);
+    stage_positions(src, lo, hi - lo, stage);
+    const int vl = live ? v : tile.begin;
+    const F3 q = gload_xyz(c.prev + vl);
+    int2 ell[kEllMax];
+    ell_load(c, vl, ell);
+    __syncthreads();
+    if (!live) return;
+    const bool count = (ops & kOpCount) != 0u;
+    const float dt = c.dt_sub;
+    const float4 p4 = stage[v - lo];
+    const float w = p4.w;
+    const bool pinned = w == 0.0f;
+    VertexState s{&c, v, f3(p4), q, f3(0.0f, 0.0f, 0.0f), true, false, false, false};
+    stretch_gather(c, StagedSource{stage, lo}, src, s, w, ell, count);
+    vertex_tail(c, s, w, pinned, dt, dst, ops, count);
+}
+
+// One solver iteration for every vertex of the tile: the constraint gather
+// selected at compile time, then the vertex-local steps selected by `ops`.
+template <int kGather>
+__global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
+                                                       int ntiles, int src_parity, int dst_parity, unsigned ops) {
+    const Tile tile = tiles[xcd_tile(ntiles)];
+    const ClothView& c = views[tile.cloth];
+    const float4* src = src_parity ? c.pos1 : c.pos0;
+    float4* dst = dst_parity ? c.pos1 : c.pos0;
+    if constexpr (kGather == kGatherStretch) {
+        // uniform over the workgroup: every thread reaches the barrier
+        if (c.direct_stretch != 0 && c.stage_halo > 0) {
+            __shared__ float4 stage[kStageMax];
+            vertex_pass_staged(c, tile, src, dst, ops, stage);
+            return;
+        }
+    }
+    const int v = tile.begin + static_cast<int>(threadIdx.x);
+    if (v >= tile.end) return;
+    const float dt = c.dt_sub;
+    const bool count = (ops & kOpCount) != 0u;
+    // Everything the stretch gather reads about the vertex itself (its
+    // incidences, position and previous position) is requested before any of
+    // it is used.
+    bool gather = false;
+    int2 ell[kEllMax];
+    F3 q = f3(0.0f, 0.0f, 0.0f);
+    const float4 p4 = gload(src + v);
+    if constexpr (kGather == kGatherStretch) {
+        gather = c.direct_stretch != 0;
+        if (gather) {
+            q = gload_xyz(c.prev + v);
+            if (c.ell_width > 0) {
+                ell_load(c, v, ell);
+            } else {
+#pragma unroll
+                for (int j = 0; j < kEllMax; ++j) ell[j] = make_int2(-1, 0);   // never read: width 0
+            }
+        }
+    }
+    const float w = p4.w;
+    // .w holds 0 exactly for the vertices the reference flags as pinned
+    // (inverse mass <= 0, A41:8861-8865)
+    const bool pinned = w == 0.0f;
+    VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
+    if constexpr (kGather == kGatherStretch) {
+        if (gather) {
+            s.q_ = q;
+            s.q_have = true;
+            stretch_gather(c, GlobalSource{src}, src, s, w, ell, count);
+        }
+    } else if constexpr (kGather == kGatherLra) {
+        lra_gather(c, src, s, w, dt, count);
+    } else if constexpr (kGather == kGatherBend) {
+        bend_gather(c, src, s, w, count);
+    }
+    vertex_tail(c, s, w, pinned, dt, dst, ops, count);
 }
 
 // ------------------------------------------------------ coloured stretch

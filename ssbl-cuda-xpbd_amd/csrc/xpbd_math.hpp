@@ -92,6 +92,14 @@ __device__ __forceinline__ float4 gload(const float4* p) {
     const gv4f x = *(const SSBL_GLOBAL gv4f*)(p);
     return make_float4(x.x, x.y, x.z, x.w);
 }
+typedef float gv3f __attribute__((ext_vector_type(3)));
+// x, y, z of a float4 array element with one 12-byte load: a 16-byte load
+// whose .w is never read leaves a register the compiler reuses, and it then
+// waits for the load to land before the reuse.
+__device__ __forceinline__ F3 gload_xyz(const float4* p) {
+    const gv3f x = *(const SSBL_GLOBAL gv3f*)(p);
+    return f3(x.x, x.y, x.z);
+}
 typedef int gv4i __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ int gload(const int* p) { return *(const SSBL_GLOBAL int*)(p); }
 __device__ __forceinline__ short gload(const short* p) { return *(const SSBL_GLOBAL short*)(p); }

@@ -422,7 +422,7 @@ struct Solver {
     Dev<int2> adj, bends, ell;
     Dev<short> ell16;
     Dev<float> ell_rest;
-    int ell_width = 0, ell_pitch = 0, ell_overflow = 0;
+    int ell_width = 0, ell_pitch = 0, ell_overflow = 0, stage_halo = 0;
     Dev<int4> lra_adj, tris, bend_segs, bend_adj;
     Dev<float> bend_rest, area_share, rest_scale, sdf, static_tris;
     Dev<float4> d_pins;
@@ -693,6 +693,14 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
             h_ell16.clear();
             h_ell_rest.clear();
         }
+    }
+    // the widest neighbour distance of the 6-byte slots decides whether the
+    // stretch gather stages its tile's neighbourhood in LDS
+    stage_halo = 0;
+    if (compact_ell) {
+        int halo = 0;
+        for (short code : h_ell16) halo = std::max(halo, std::abs(code >> 1));
+        stage_halo = halo <= kStageHaloMax ? halo : 0;
     }
     if (compact_ell) h_ell.clear();
     edge_colors.clear();
@@ -1076,7 +1084,7 @@ void Solver::fill_view(int substeps) {
     v.npin = static_cast<int>(pins.size()); v.nfield = static_cast<int>(fields.size());
     v.pos0 = pos0.p; v.pos1 = pos1.p; v.prev = prev.p; v.vel = vel.p; v.rest = rest.p;
     v.adj_off = adj_off.p; v.adj = adj.p;
-    v.ell = ell.p; v.ell16 = ell16.p; v.ell_rest = ell_rest.p; v.ell_width = ell_width; v.ell_pitch = ell_pitch; v.ell_overflow = ell_overflow; v.lra_off = lra_off.p; v.lra_adj = lra_adj.p;
+    v.ell = ell.p; v.ell16 = ell16.p; v.ell_rest = ell_rest.p; v.ell_width = ell_width; v.ell_pitch = ell_pitch; v.ell_overflow = ell_overflow; v.stage_halo = stage_halo; v.lra_off = lra_off.p; v.lra_adj = lra_adj.p;
     v.tri_off = tri_off.p; v.tri_inc = tri_inc.p; v.tris = tris.p; v.bends = bends.p; v.bend_rest = bend_rest.p;
     v.bend_off = bend_off.p; v.bend_adj = bend_adj.p;
     v.pin_off = pin_off.p; v.pin_ids = pin_ids.p; v.pins = d_pins.p;
