@@ -150,7 +150,8 @@ __device__ __forceinline__ int agg_slot(int* s_bucket, int* s_list, int* s_n, in
 __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv, const TriBox* __restrict__ box,
                                                   const int* __restrict__ large_flag, int* count, const int* __restrict__ off,
                                                   int2* raw, int4* ent, const int* __restrict__ tsrc,
-                                                  const int* __restrict__ tlayer, int ranked, int phase) {
+                                                  const int* __restrict__ tlayer, const int* __restrict__ order, int ranked,
+                                                  int phase) {
     __shared__ int s_bucket[kAggSlots];
     __shared__ int s_count[kAggSlots];
     __shared__ int s_base[kAggSlots];
@@ -164,9 +165,11 @@ __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const int tl = static_cast<int>(threadIdx.x) / kTriLanes;
-    const int t = static_cast<int>(blockIdx.x) * kTriPerBlock + tl;
+    const int nt = dv->nt;
+    const int at_k = static_cast<int>(blockIdx.x) * kTriPerBlock + tl;   // the block's k-th triangle in visiting order
+    const int t = (order && at_k < nt) ? order[at_k] : at_k;
     const int lane = static_cast<int>(threadIdx.x) % kTriLanes;
-    const bool live = t < dv->nt && !large_flag[t];   // a span of at most 64 cells
+    const bool live = t < nt && !large_flag[t];   // a span of at most 64 cells
     CellSpan sp;
     sp.init(make_int4(0, 0, 0, 0), make_int4(-1, -1, -1, 0));
     if (live) {
@@ -439,7 +442,7 @@ __global__ __launch_bounds__(256) void k_gather_particles(const GatherSource* __
 }
 
 __global__ __launch_bounds__(256) void k_gather_triangles(const GatherSource* __restrict__ src, const int4* __restrict__ segs,
-                                                          float4* tv, int* tsrc, int* tlayer, int* extent_bits) {
+                                                          float4* tv, int* tsrc, int* tlayer, int* order, int* extent_bits) {
     const int4 sg = segs[blockIdx.x];
     const int t = sg.y + static_cast<int>(threadIdx.x);
     const bool active = t < sg.z;
@@ -456,6 +459,7 @@ __global__ __launch_bounds__(256) void k_gather_triangles(const GatherSource* __
         tv[3 * i + 2] = with_w(c, 0.0f);
         tsrc[i] = S.source_id;
         tlayer[i] = S.layer;
+        order[i] = S.tbase + (S.order ? S.order[t] : t);
     }
     const float ext = fmaxf(fmaxf(a.x, fmaxf(b.x, c.x)) - fminf(a.x, fminf(b.x, c.x)),
                             fmaxf(fmaxf(a.y, fmaxf(b.y, c.y)) - fminf(a.y, fminf(b.y, c.y)),
@@ -507,6 +511,7 @@ bool DynamicColliders::reserve_triangles(int n) {
     tv_ = static_cast<float4*>(alloc(0, sizeof(float4) * 3 * m));
     tsrc_ = static_cast<int*>(alloc(0, sizeof(int) * m));
     tlayer_ = static_cast<int*>(alloc(0, sizeof(int) * m));
+    tri_order_ = static_cast<int*>(alloc(0, sizeof(int) * m));
     box_ = static_cast<TriBox*>(alloc(0, sizeof(TriBox) * m));
     pctr_ = static_cast<float4*>(alloc(0, sizeof(float4) * m));
     bounds_valid_ = false;
@@ -529,7 +534,7 @@ bool DynamicColliders::reserve_triangles(int n) {
         scan_tmp_ = static_cast<int*>(alloc(2, sizeof(int) * scan_tmp_ints(need_scan)));
         scan_cap_ = scan_tmp_ ? need_scan : 0;
     }
-    if (!tv_ || !tsrc_ || !tlayer_ || !box_ || !pctr_ || !large_flag_ || !large_off_ || !large_ || !tri_count_ || !tri_off_ ||
+    if (!tv_ || !tsrc_ || !tlayer_ || !tri_order_ || !box_ || !pctr_ || !large_flag_ || !large_off_ || !large_ || !tri_count_ || !tri_off_ ||
         !tri_raw_ || !tri_keys_ || !tri_ksl_ || !tri_ent_ || !tri_bin_ || !occupancy_ || !extent_bits_ || !scan_tmp_)
         return false;
     tri_cap_ = n;
@@ -633,6 +638,7 @@ bool DynamicColliders::set_triangles(hipStream_t st, const float* soup, int n, c
     if (!sync_stream()) return false;
     signed_ = false;
     tri_motion_ = false;
+    tri_ordered_ = false;
     if (n == 0) { nt_ = 0; bounds_valid_ = false; return d_view_ ? upload_view(st) : true; }
     if (!reserve_triangles(n)) return false;
     float max_extent = 0.0f;
@@ -677,6 +683,7 @@ bool DynamicColliders::set_indexed(hipStream_t st, const float* verts, int nvert
     nverts = std::max(nverts, 0);
     ntri = std::max(ntri, 0);
     if (!sync_stream()) return false;
+    tri_ordered_ = false;
     if (nverts == 0 || ntri == 0) {
         nt_ = 0;
         signed_ = tri_motion_ = bounds_valid_ = false;
@@ -796,7 +803,7 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
     // frame): the device tables already hold these sources, so no upload and
     // no wait for the kernels still reading them.
     auto same_source = [](const GatherSource& x, const GatherSource& y) {
-        return x.pos == y.pos && x.tris == y.tris && x.nv == y.nv && x.nt == y.nt && x.pbase == y.pbase && x.tbase == y.tbase &&
+        return x.pos == y.pos && x.tris == y.tris && x.order == y.order && x.nv == y.nv && x.nt == y.nt && x.pbase == y.pbase && x.tbase == y.tbase &&
                std::memcmp(&x.radius, &y.radius, sizeof(float)) == 0 && x.source_id == y.source_id && x.layer == y.layer;
     };
     bool same = np == np_ && nt == nt_ && sources.size() == gathered_.size() && d_sources_ && d_segments_;
@@ -847,7 +854,10 @@ bool DynamicColliders::launch_gather(hipStream_t st, int npseg, int ntseg, bool 
                  extent_bits_);
     else
         launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, extent_bits_, 0, 1);
-    if (ntseg > 0) launch_k(k_gather_triangles, dim3(ntseg), dim3(256), 0, st, d_sources_, d_segments_ + npseg, tv_, tsrc_, tlayer_, extent_bits_);
+    if (ntseg > 0)
+        launch_k(k_gather_triangles, dim3(ntseg), dim3(256), 0, st, d_sources_, d_segments_ + npseg, tv_, tsrc_, tlayer_, tri_order_,
+                 extent_bits_);
+    tri_ordered_ = ntseg > 0;
     launch_k(k_scene_cell, dim3(1), dim3(1), 0, st, d_view_, extent_bits_, margin, thickness);
     return hip_ok(hipGetLastError(), "scene gather");
 }
@@ -874,14 +884,16 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
                 TimedLaunch tc("tri_hash_count", st);
                 launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
                          static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, tri_ent_,
-                         fsrc, static_cast<const int*>(tlayer_), ranked, 0);
+                         fsrc, static_cast<const int*>(tlayer_), static_cast<const int*>(tri_ordered_ ? tri_order_ : nullptr),
+                         ranked, 0);
             }
             exclusive_scan(st, tri_count_, buckets, tri_off_, scan_tmp_);
             {
                 TimedLaunch ts("tri_hash_scatter", st);
                 launch_k(k_tri_hash, gw, blk, 0, st, static_cast<const DynView*>(d_view_), static_cast<const TriBox*>(box_),
                          static_cast<const int*>(large_flag_), tri_count_, static_cast<const int*>(tri_off_), tri_raw_, tri_ent_,
-                         fsrc, static_cast<const int*>(tlayer_), ranked, 1);
+                         fsrc, static_cast<const int*>(tlayer_), static_cast<const int*>(tri_ordered_ ? tri_order_ : nullptr),
+                         ranked, 1);
             }
             if (ranked) {
                 const dim3 gr(std::min<unsigned>(grid_for(static_cast<long long>(nt_) * 16, 256), 16384u));

@@ -123,6 +123,7 @@ struct DynView {
 struct GatherSource {
     const float4* pos;       // current positions
     const int4* tris;        // its triangles
+    const int* order;        // the order the hash build visits them (a permutation of 0 .. nt - 1), or null
     int nv, nt;
     int pbase, tbase;        // first particle / triangle in the scene
     float radius;
@@ -188,6 +189,8 @@ private:
     float4* tv_ = nullptr;
     int* tsrc_ = nullptr;
     int* tlayer_ = nullptr;
+    int* tri_order_ = nullptr;   // the hash build's triangle order (gathered scenes; others in index order)
+    bool tri_ordered_ = false;
     bool tri_filter_ = false;
     TriBox* box_ = nullptr;
     int *large_flag_ = nullptr, *large_off_ = nullptr, *large_ = nullptr;

@@ -381,6 +381,55 @@ constexpr int PoolTuner::kCandidates[PoolTuner::kCount];
 
 struct HostPin { int index; float target[3]; float weight; };
 
+// The triangles of a mesh in Morton order of their centroids (10 bits per
+// axis over the mesh's bounds), ties by index.  A scene's hash build takes a
+// gathered cloth's triangles in this order, so the triangles of one
+// workgroup lie side by side and share most of their hash cells: fewer
+// distinct buckets per workgroup, so fewer device atomics, and each bucket's
+// entries from one workgroup are written together.  The order changes no
+// bucket's contents.
+static std::vector<int> spatial_order(const std::vector<float4>& pos, const std::vector<int>& tri, int count) {
+    std::vector<int> order(std::max(count, 0));
+    for (int t = 0; t < count; ++t) order[t] = t;
+    const int n = static_cast<int>(pos.size());
+    if (count < 2 || n == 0) return order;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    std::vector<float> ctr(3 * static_cast<size_t>(count), 0.0f);
+    for (int t = 0; t < count; ++t) {
+        const int a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+        if (a >= n || b >= n || c >= n) continue;
+        const float x[3] = {(pos[a].x + pos[b].x + pos[c].x) / 3.0f, (pos[a].y + pos[b].y + pos[c].y) / 3.0f,
+                            (pos[a].z + pos[b].z + pos[c].z) / 3.0f};
+        for (int k = 0; k < 3; ++k) {
+            if (!std::isfinite(x[k])) continue;
+            ctr[3 * t + k] = x[k];
+            lo[k] = std::min(lo[k], x[k]);
+            hi[k] = std::max(hi[k], x[k]);
+        }
+    }
+    auto spread = [](unsigned v) {   // 10 bits to every third bit
+        v &= 0x3ffu;
+        v = (v | (v << 16)) & 0x030000ffu;
+        v = (v | (v << 8)) & 0x0300f00fu;
+        v = (v | (v << 4)) & 0x030c30c3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    };
+    std::vector<std::pair<unsigned, int>> keys(count);
+    for (int t = 0; t < count; ++t) {
+        unsigned q[3];
+        for (int k = 0; k < 3; ++k) {
+            const float w = hi[k] - lo[k];
+            const float u = (std::isfinite(w) && w > 0.0f) ? (ctr[3 * t + k] - lo[k]) / w : 0.0f;
+            q[k] = static_cast<unsigned>(std::min(std::max(u, 0.0f), 1.0f) * 1023.0f);
+        }
+        keys[t] = {spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2), t};
+    }
+    std::sort(keys.begin(), keys.end());
+    for (int t = 0; t < count; ++t) order[t] = keys[t].second;
+    return order;
+}
+
 // A global dynamic scene (A41:309-340): the colliders every attached cloth
 // sees, built by an update from host arrays or gathered on the device.
 struct GlobalScene {
@@ -419,6 +468,7 @@ struct Solver {
     // device state
     Dev<float4> pos0, pos1, prev, vel, rest;
     Dev<int> adj_off, lra_off, tri_off, tri_inc, pin_off, pin_ids, bend_off;
+    Dev<int> tri_order;   // the order a gathered scene's hash build visits this cloth's triangles
     Dev<int2> adj, bends, ell;
     Dev<short> ell16;
     Dev<float> ell_rest;
@@ -829,6 +879,7 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     }
     std::vector<float> h_share, h_scale;
     pressure_setup(*this, P, tri, edges, edge_rest, h_share, h_scale);
+    const std::vector<int> h_tri_order = spatial_order(h_pos, tri, cfg.triangle_count);
 
     ntiles = (n + kTile - 1) / kTile;
     std::vector<Tile> h_tiles(ntiles);
@@ -842,7 +893,7 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
               adj.upload(h_adj, "edge incidence allocation") && ell.upload(h_ell, "edge incidence allocation") &&
               ell16.upload(h_ell16, "edge incidence allocation") && ell_rest.upload(h_ell_rest, "edge incidence allocation") && lra_off.upload(h_lra_off, "LRA incidence allocation") &&
               lra_adj.upload(h_lra, "LRA incidence allocation") && tri_off.upload(h_tri_off, "triangle incidence allocation") &&
-              tri_inc.upload(h_tri_inc, "triangle incidence allocation") && tris.upload(h_tris, "triangle allocation") &&
+              tri_inc.upload(h_tri_inc, "triangle incidence allocation") && tris.upload(h_tris, "triangle allocation") && tri_order.upload(h_tri_order, "triangle allocation") &&
               bends.upload(h_bends, "bend allocation") && bend_rest.upload(h_bend_rest, "bend rest allocation") &&
               bend_off.upload(h_bend_off, "bend incidence allocation") && bend_adj.upload(h_bend_adj, "bend incidence allocation") &&
               bend_segs.upload(h_bend_segs, "bend segment allocation") && area_share.upload(h_share, "pressure share allocation") &&
@@ -2018,6 +2069,7 @@ SSBL_API int ssbl_ext_update_global_scene_from_handles(void* scene, void* const*
         GatherSource g{};
         g.pos = s->pos0.p;
         g.tris = s->tris.p;
+        g.order = s->tri_order.p;
         g.nv = s->nv;
         g.nt = s->cfg.triangle_count;
         g.radius = particle_radii[k];
