@@ -127,7 +127,7 @@ struct CellSpan {
     }
 };
 
-constexpr int kTriLanes = 8;      // lanes per triangle in the hash builds: lane l takes cells l, l + 8, ...
+constexpr int kTriLanes = 4;      // lanes per triangle in the hash builds: lane l takes cells l, l + 4, ...
 constexpr int kTriPerBlock = 256 / kTriLanes;
 constexpr int kAggSlots = 1024;   // LDS table of the distinct buckets a block's entries fall in
 constexpr int kAggProbes = 32;    // a bucket with no slot within this many probes counts globally, entry by entry

@@ -283,8 +283,9 @@ __device__ __forceinline__ bool dyn_bucket_rejected(const ClothView& c, int4 bin
 // One dynamic triangle against point p (A41:3999-4323 per candidate): counts
 // the candidate, the AABB and bounding-sphere rejections, and returns the
 // contact depth (0 when out of reach) with the push along the contact normal.
-__device__ __forceinline__ float dyn_triangle_depth(const SSBL_CONST DynView& D, int t, const TriBox& bx, F3 p, float cm, unsigned& cand,
-                                                    unsigned& rej, F3& delta) {
+template <typename Corners>
+__device__ __forceinline__ float dyn_triangle_depth_with(const TriBox& bx, Corners corners, F3 p, float cm, unsigned& cand, unsigned& rej,
+                                                         F3& delta) {
     ++cand;
     const float4 mn = box_lo(bx), mx = box_hi(bx);
     const F3 q0 = f3(fminf(fmaxf(p.x, mn.x), mx.x), fminf(fmaxf(p.y, mn.y), mx.y), fminf(fmaxf(p.z, mn.z), mx.z));
@@ -294,7 +295,8 @@ __device__ __forceinline__ float dyn_triangle_depth(const SSBL_CONST DynView& D,
     const float rr = ce.w + cm;
     const F3 dc = p - f3(ce);
     if (dot(dc, dc) > rr * rr) { ++rej; return 0.0f; }
-    const F3 a = f3(gload(D.tv + 3 * t)), b = f3(gload(D.tv + 3 * t + 1)), cc = f3(gload(D.tv + 3 * t + 2));
+    F3 a, b, cc;
+    corners(a, b, cc);
     const F3 q = closest_point_on_triangle(p, a, b, cc);
     const F3 d = p - q;
     const float dsq = dot(d, d);
@@ -307,6 +309,19 @@ __device__ __forceinline__ float dyn_triangle_depth(const SSBL_CONST DynView& D,
     const float pen = cm - dist;
     delta = mul(n, pen);
     return pen;
+}
+
+// The corners are read only once the box and sphere tests pass.
+__device__ __forceinline__ float dyn_triangle_depth(const SSBL_CONST DynView& D, int t, const TriBox& bx, F3 p, float cm, unsigned& cand,
+                                                    unsigned& rej, F3& delta) {
+    return dyn_triangle_depth_with(
+        bx,
+        [&](F3& a, F3& b, F3& c) {
+            a = f3(gload(D.tv + 3 * t));
+            b = f3(gload(D.tv + 3 * t + 1));
+            c = f3(gload(D.tv + 3 * t + 2));
+        },
+        p, cm, cand, rej, delta);
 }
 
 // One dynamic particle (position and radius pj) against point p
@@ -427,44 +442,62 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
                     return w < total ? gload(D.tri_ent + k) : make_int4(-1, 0, 0, 0);
                 };
                 // Items flow through three stages: a round's entry records are
-                // requested two rounds ahead and its triangles' bounds one round
-                // ahead, so a round's own work waits only for its corners.  The
-                // source filter (in the entry) runs before the bounds are read;
-                // it and the canonical-cell test (A41:3934-3982) are both
-                // side-effect free.
+                // requested two rounds ahead and its triangles' bounds and
+                // corners one round ahead, so a round issues no load it waits
+                // for itself.  The source filter (in the entry) runs before the
+                // bounds are read; it and the canonical-cell test
+                // (A41:3934-3982) are both side-effect free.
                 auto accepted = [&](int4 e) {
                     return e.x >= 0 && e.x / kDynMaxHashCells < nt && (!D.tsrc || dyn_enabled(c, e.y, e.z));
                 };
-                auto bounds_of = [&](int4 e) {
-                    TriBox bx{};
-                    if (accepted(e)) bx = gload(D.box + e.x / kDynMaxHashCells);
-                    return bx;
+                // a triangle's bounds and corners, requested together one
+                // round ahead (the corners whether or not the bounds pass)
+                struct TriRec {
+                    TriBox bx;
+                    float4 a, b, c;
+                };
+                auto record_of = [&](int4 e) {
+                    TriRec r{};
+                    if (accepted(e)) {
+                        const int t = e.x / kDynMaxHashCells;
+                        r.bx = gload(D.box + t);
+                        r.a = gload(D.tv + 3 * t);
+                        r.b = gload(D.tv + 3 * t + 1);
+                        r.c = gload(D.tv + 3 * t + 2);
+                    }
+                    return r;
                 };
                 int own = owner_of(lane);
                 int4 en = entry_of(lane, own);
                 int own1 = owner_of(lane + 64);
                 int4 en1 = entry_of(lane + 64, own1);
-                TriBox bx = bounds_of(en);
+                TriRec rc = record_of(en);
                 for (int base = 0; base < total; base += 64) {
                     const int w = base + lane;
                     const int ox = __shfl(cx, own, 64), oy = __shfl(cy, own, 64), oz = __shfl(cz, own, 64);
                     const F3 op = shfl3(s.p, own);
                     const int own2 = owner_of(w + 128);
                     const int4 en2 = entry_of(w + 128, own2);
-                    const TriBox bx1 = bounds_of(en1);
+                    const TriRec rc1 = record_of(en1);
                     if (w < total && accepted(en)) {
-                        const int t = en.x / kDynMaxHashCells;
-                        const int4 lo = box_cmin(bx), hi = box_cmax(bx);
+                        const int4 lo = box_cmin(rc.bx), hi = box_cmax(rc.bx);
                         if (ox >= lo.x && ox <= hi.x && oy >= lo.y && oy <= hi.y && oz >= lo.z && oz <= hi.z) {
                             F3 d;
-                            const unsigned long long kk =
-                                depth_key(dyn_triangle_depth(D, t, bx, op, cm, cand, rej, d), static_cast<unsigned>(en.x));
+                            const float pen = dyn_triangle_depth_with(
+                                rc.bx,
+                                [&](F3& a, F3& b, F3& c) {
+                                    a = f3(rc.a);
+                                    b = f3(rc.b);
+                                    c = f3(rc.c);
+                                },
+                                op, cm, cand, rej, d);
+                            const unsigned long long kk = depth_key(pen, static_cast<unsigned>(en.x));
                             if (kk != 0ull) atomicMax(best + own, kk);
                         }
                     }
                     own = own1;
                     en = en1;
-                    bx = bx1;
+                    rc = rc1;
                     own1 = own2;
                     en1 = en2;
                 }
