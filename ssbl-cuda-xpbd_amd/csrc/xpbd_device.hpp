@@ -210,6 +210,12 @@ struct LaunchCtx {
     const ClothView* views;     // device table
     const Tile* tiles;          // device tile table
     int ntiles;
+    // The dynamic contact pass adds each tile's probe work to tile_work and
+    // takes its tiles in tile_order (most work first), which
+    // launch_order_tiles rebuilds from the work of the step before (both
+    // null: tiles in table order)
+    int* tile_work;
+    int* tile_order;
 };
 
 void launch_integrate(const LaunchCtx& c, int parity, unsigned ops, bool full);
@@ -224,6 +230,7 @@ void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned 
                          hipEvent_t begin = nullptr, hipEvent_t end = nullptr);
 // Signed contacts: the pass after self collision (inside = false) and the
 // inside-only passes of a step's last iteration (A41:9322-9359).
+void launch_order_tiles(const LaunchCtx& c);
 void launch_signed_pass(const LaunchCtx& c, int parity, bool inside, unsigned ops);
 void launch_bend_color(const LaunchCtx& c, const int4* segments, int nseg_blocks, int parity, unsigned ops);
 // One colour of the coloured stretch mode; segments {cloth, begin, end}.

@@ -391,11 +391,12 @@ __device__ __forceinline__ F3 shfl3(F3 a, int lane) { return f3(__shfl(a.x, lane
 // per-lane walks are already balanced.  The launch takes the limit
 // (kDefaultPoolMax unless tuned); every value gives the same contacts.
 
-__device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState& s, bool active, float dt, bool count) {
+__device__ inline int dynamic_triangle_contacts(const ClothView& c, VertexState& s, bool active, float dt, bool count) {
     const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
     const int lane = static_cast<int>(__lane_id());
     const float cm = fmaxf(c.collision_margin, 0.0f);
     const int nt = D.nt;
+    int work = 0;   // the wave's pooled items
     if (nt > 0 && cm > 0.0f) {
         float best_pen = 0.0f;
         F3 best_delta = f3(0.0f, 0.0f, 0.0f);
@@ -421,6 +422,7 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
             }
             const int start = incl - n_i;
             const int total = __shfl(incl, 63, 64);
+            work = total;
             if (total > 0) {   // uniform over the wave
                 __shared__ unsigned long long s_best[kTile];
                 unsigned long long* best = s_best + (threadIdx.x & ~63u);
@@ -551,12 +553,14 @@ __device__ inline void dynamic_triangle_contacts(const ClothView& c, VertexState
             }
         }
     }
+    return work;
 }
 
-__device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState& s, bool active, bool count, int pool_max) {
+__device__ inline int dynamic_particle_contacts(const ClothView& c, VertexState& s, bool active, bool count, int pool_max) {
     const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
     const int lane = static_cast<int>(__lane_id());
     const int np_count = D.np;
+    int work = 0;   // the wave's cell searches
     if (np_count > 0) {
         const float margin = fmaxf(c.collision_margin, 0.0f);
         const float reach = D.max_radius + margin;   // no particle reaches further
@@ -580,6 +584,7 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
             // their bucket order k, so (depth, ~(16 c + k)) orders contacts as
             // the serial search meets them.
             const unsigned long long probes = __ballot(probe);
+            work = 27 * __popcll(probes);
             if (__popcll(probes) > pool_max) {
                 // many probes in this wave: each lane searches its own 27 cells
                 if (probe) {
@@ -692,6 +697,7 @@ __device__ inline void dynamic_particle_contacts(const ClothView& c, VertexState
             }
         }
     }
+    return work;
 }
 
 // A41:5846-5884
@@ -995,8 +1001,13 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
 // exactly as the in-place vertex pass would apply them, for every vertex of
 // every cloth in the launch.
 __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
-                                                        int ntiles, int parity, unsigned ops, unsigned tail_ops) {
-    const Tile tile = tiles[xcd_tile(ntiles)];
+                                                        int ntiles, int parity, unsigned ops, unsigned tail_ops,
+                                                        const int* __restrict__ tile_order, int* tile_work) {
+    // with an order, the tiles that searched most in the step before come
+    // first: the launch lasts as long as its slowest waves, so those start at
+    // once instead of in a later round of waves
+    const int ti = tile_order ? tile_order[blockIdx.x] : xcd_tile(ntiles);
+    const Tile tile = tiles[ti];
     const ClothView& c = views[tile.cloth];
     const bool has_dyn = c.dyn != nullptr;
     const bool tail = (tail_ops & kOpVelocity) != 0u;
@@ -1010,8 +1021,9 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     const bool active = in_tile && p4.w != 0.0f;
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
     if (has_dyn) {
-        dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
-        dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u, static_cast<int>((ops >> kOpPoolShift) & 0x7fu));
+        int work = dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
+        work += dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u, static_cast<int>((ops >> kOpPoolShift) & 0x7fu));
+        if (tile_work && (threadIdx.x & 63u) == 0u && work > 0) atomicAdd(&tile_work[ti], work);
     }
     if (!in_tile) return;
     if (tail) {
@@ -1038,6 +1050,32 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     if (s.p.x != p4.x || s.p.y != p4.y || s.p.z != p4.z) gstore(pos + v, with_w(s.p, p4.w));
     if (s.q_dirty) gstore(c.prev + v, with_w(s.q_, 0.0f));
     if (s.v_dirty) gstore(c.vel + v, with_w(s.vel_, 0.0f));
+}
+
+// The dynamic contact pass's tile order for the next step: tiles by the
+// work their waves did in the step before (tile_work, then cleared), most
+// first, counting-sorted on the bit length of the work.  One workgroup.
+__global__ __launch_bounds__(1024) void k_order_tiles(int ntiles, int* work, int* order) {
+    __shared__ int hist[33];
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid < 33) hist[tid] = 0;
+    __syncthreads();
+    auto key = [](int w) { return 32 - (w > 0 ? 32 - __clz(w) : 0); };   // more work, smaller key
+    for (int t = tid; t < ntiles; t += 1024) atomicAdd(&hist[key(work[t])], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int k = 0; k < 33; ++k) {
+            const int n = hist[k];
+            hist[k] = run;
+            run += n;
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < ntiles; t += 1024) {
+        order[atomicAdd(&hist[key(work[t])], 1)] = t;
+        work[t] = 0;
+    }
 }
 
 // ------------------------------------------------- signed dynamic contacts
@@ -1818,12 +1856,19 @@ void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsi
         launch_k(k_vertex_pass<kGatherNone>, grid, block, 0, c.stream, c.views, c.tiles, c.ntiles, src_parity, dst_parity, ops);
 }
 
+void launch_order_tiles(const LaunchCtx& c) {
+    if (!c.tile_order || !c.tile_work || c.ntiles <= 0) return;
+    TimedLaunch t("dynamic_order", c.stream);
+    launch_k(k_order_tiles, dim3(1), dim3(1024), 0, c.stream, c.ntiles, c.tile_work, c.tile_order);
+}
+
 void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops, int pool_max, hipEvent_t begin,
                          hipEvent_t end) {
     TimedLaunch t("dynamic_contacts", c.stream);
     const unsigned pops = ops | (static_cast<unsigned>(std::min(std::max(pool_max, 0), 64)) << kOpPoolShift);
     if (begin) hipEventRecord(begin, c.stream);
-    launch_k(k_dynamic_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, pops, tail_ops);
+    launch_k(k_dynamic_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, pops, tail_ops,
+             static_cast<const int*>(c.tile_order), c.tile_work);
     if (end) hipEventRecord(end, c.stream);
 }
 

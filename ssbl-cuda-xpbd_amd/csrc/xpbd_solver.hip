@@ -481,6 +481,7 @@ struct Solver {
     Dev<float> max_smoothed;
     Dev<ClothView> d_view;
     Dev<Tile> d_tiles;
+    Dev<int> tile_work, tile_order;   // the dynamic contact pass's tile order (LaunchCtx)
     std::vector<int> bend_seg_first, bend_seg_count;   // per colour, into bend_segs
     std::vector<int4> h_bend_segs;
     // coloured stretch mode: the edges as uploaded, whether their colouring
@@ -899,9 +900,11 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
               bend_segs.upload(h_bend_segs, "bend segment allocation") && area_share.upload(h_share, "pressure share allocation") &&
               rest_scale.upload(h_scale, "pressure scale allocation") && pin_off.upload(h_pin_off, "pin allocation") &&
               counts.alloc(kCountSlots, "diagnostic allocation") && max_smoothed.alloc(1, "diagnostic allocation") &&
-              d_view.alloc(1, "view allocation") && d_tiles.upload(h_tiles, "tile allocation");
+              d_view.alloc(1, "view allocation") && d_tiles.upload(h_tiles, "tile allocation") &&
+              tile_work.alloc(std::max(ntiles, 1), "tile allocation") && tile_order.alloc(std::max(ntiles, 1), "tile allocation");
     if (!ok || !staging.reserve(sizeof(ClothView))) return false;
     if (!hip_ok(hipMemset(counts.p, 0, sizeof(unsigned long long) * kCountSlots), "diagnostic reset")) return false;
+    if (!hip_ok(hipMemset(tile_work.p, 0, sizeof(int) * tile_work.n), "tile reset")) return false;
     if (!hip_ok(hipMemset(max_smoothed.p, 0, sizeof(float)), "diagnostic reset")) return false;
     if (cfg.self_collision && !self.prepare(cfg, edges, tri, n, counts.p)) return false;
     if (cfg.stretch_optimization_enabled && !pcg.prepare(cfg, edges, edge_rest, edge_colors, n, h_adj_off[n], ntiles, ell_width, ell_pitch,
@@ -1255,6 +1258,9 @@ static bool run_schedule(std::vector<Solver*>& members, const LaunchCtx& ctx, co
         launch_vertex_pass(ctx, parity, parity ^ 1, stretch | ops);
         parity ^= 1;
     };
+    // the dynamic contact pass takes the tiles that searched most in the step
+    // before first
+    if (plan.any_dyn) launch_order_tiles(ctx);
     // the hard-stretch scalars of the step start at zero (A41:7586-7624)
     if (plan.any_pcg)
         for (const PcgGroup* g : pcg_groups) g->reset(st, dviews);
@@ -1465,7 +1471,7 @@ static bool step_one(Solver& s, int substeps, int iterations, int fetch, int for
     if (!prepare_member(s, substeps, iterations, count)) return false;
     if (!s.staging.upload(s.d_view.p, &s.view, sizeof(ClothView), lib_stream())) return false;
     if (s.nv > 0) {
-        LaunchCtx ctx{lib_stream(), s.d_view.p, s.d_tiles.p, s.ntiles};
+        LaunchCtx ctx{lib_stream(), s.d_view.p, s.d_tiles.p, s.ntiles, s.tile_work.p, s.tile_order.p};
         std::vector<Solver*> one{&s};
         SelfBatch* sb = s.cfg.self_collision ? s.self.single_batch(s.view, s.d_view.p) : nullptr;
         const StepPlan plan = plan_of(one);
@@ -1505,6 +1511,7 @@ struct Batch {
     int stretch_mode = SSBL_EXT_STRETCH_JACOBI;   // shared by every member
     Dev<ClothView> d_views;
     Dev<Tile> d_tiles;
+    Dev<int> tile_work, tile_order;   // the dynamic contact pass's tile order (LaunchCtx)
     Dev<int4> d_bend_segs, d_stretch_segs;
     std::vector<int4> h_bend_segs, h_stretch_segs;
     std::vector<int> seg_first, seg_count, sseg_first, sseg_count;
@@ -1570,6 +1577,10 @@ static bool build_batch(Batch& b) {
     for (auto& kv : by_cadence)
         if (!b.pcg_groups[g++].build(kv.second, kv.first)) return false;
     // grow-only: a batch given new members keeps its device tables and staging
+    const size_t nt = std::max<size_t>(tiles.size(), 1);
+    if (!b.tile_work.reserve(nt, "batch tile allocation") || !b.tile_order.reserve(nt, "batch tile allocation") ||
+        !hip_ok(hipMemset(b.tile_work.p, 0, sizeof(int) * b.tile_work.n), "batch tile reset"))
+        return false;
     return b.d_views.reserve(b.members.size(), "batch view allocation") && b.d_tiles.upload_grow(tiles, "batch tile allocation") &&
            b.d_bend_segs.upload_grow(b.h_bend_segs, "batch bend allocation") &&
            b.d_stretch_segs.upload_grow(b.h_stretch_segs, "batch stretch allocation") &&
@@ -1601,7 +1612,7 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, b
     }
     if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
     b.stepped_substeps = substeps;
-    LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles};
+    LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles, b.tile_work.p, b.tile_order.p};
     SelfBatch* sb = b.self.get(b.members, b.h_views, b.d_views.p);
     const StepPlan plan = plan_of(b.members);
     if (plan.any_dyn && !b.tuner.poll(b.tuner.tuning())) return false;
