@@ -117,6 +117,19 @@ SSBL_API int ssbl_ext_set_graph_mode(int mode);
  * batch's (handle NULL). */
 SSBL_API int ssbl_ext_graph_stats(void* handle, void* batch, long long* captures, long long* replays);
 
+/* Concurrent batch parts (on by default: parts 2).  A batch of two or more
+ * cloths and at least 64 tiles of 256 vertices is split, when created or
+ * given new members, into two parts of about equal vertex counts (largest
+ * cloths dealt first to the part with fewer vertices), and each step runs the
+ * two parts' launch sequences on two streams, forked from and joined back to
+ * the library stream: the parts' cloths share nothing within a step, so one
+ * part's launches use the GPU while the other's wait for memory or finish on
+ * a few slow waves.  Results are the same as one table's.  parts 1 keeps
+ * batches created afterwards as one table; returns the previous setting.
+ * Steps with launch tuning on run as one table.  No counterpart in the
+ * reference, which steps its cloths one by one. */
+SSBL_API int ssbl_ext_set_batch_parts(int parts);
+
 /* Launch tuning (off by default).  A handle or batch whose steps run the
  * dynamic collider contacts times the first of those steps and every 16th
  * after it, with the particle pass's pooling limit (the most probing

@@ -3402,6 +3402,7 @@ SSBL_API int ssbl_ext_synchronize(void) { return 1; }
 // The oracle launches nothing, so it has no graphs to replay or launches to tune.
 SSBL_API int ssbl_ext_set_graph_mode(int) { return 0; }
 SSBL_API int ssbl_ext_set_autotune(int) { return 0; }
+SSBL_API int ssbl_ext_set_batch_parts(int) { return 1; }
 SSBL_API int ssbl_ext_tuned_pool_limit(void*, void*) { return -1; }
 SSBL_API int ssbl_ext_graph_stats(void*, void*, long long* captures, long long* replays) {
     if (!captures || !replays) return fail("invalid graph stats request") ? 1 : 0;

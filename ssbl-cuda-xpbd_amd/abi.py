@@ -136,6 +136,7 @@ EXT_SIGNATURES = {
     "ssbl_ext_synchronize": (_i, []),
     "ssbl_ext_set_graph_mode": (_i, [_i]),
     "ssbl_ext_set_autotune": (_i, [_i]),
+    "ssbl_ext_set_batch_parts": (_i, [_i]),
     "ssbl_ext_tuned_pool_limit": (_i, [_vp, _vp]),
     "ssbl_ext_graph_stats": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "ssbl_ext_kernel_timing": (_i, [_i]),

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <memory>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -49,6 +50,17 @@ bool hip_ok(hipError_t err, const char* what) {
 }
 
 hipStream_t lib_stream() {
+    static hipStream_t stream = [] {
+        hipStream_t s = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+        return s;
+    }();
+    return stream;
+}
+
+// The stream a batch's second part runs on (forked from and joined back to
+// the library stream within each step; default priority like the first).
+static hipStream_t part_stream() {
     static hipStream_t stream = [] {
         hipStream_t s = nullptr;
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
@@ -1523,7 +1535,23 @@ struct Batch {
     std::vector<PcgGroup> pcg_groups;   // members on the hard-stretch solve, one group per cadence
     StepGraph graph;
     PoolTuner tuner;
+    // Two halves of the members (by vertex count), stepped concurrently on
+    // two streams: their cloths share nothing within a step, so one half's
+    // launch fills the GPU while the other's waits for memory or ends on a
+    // few slow waves.  Empty: the batch steps as one table.
+    std::vector<std::unique_ptr<Batch>> parts;
+    hipEvent_t fork = nullptr, join = nullptr;
+    Batch() = default;
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+    ~Batch() {
+        if (fork) hipEventDestroy(fork);
+        if (join) hipEventDestroy(join);
+    }
 };
+
+std::atomic<int> g_batch_parts{2};   // ssbl_ext_set_batch_parts: 1 or 2
+constexpr int kMinSplitTiles = 64;   // a batch with fewer tiles steps as one table
 
 // Colour c of the batch: colour c's segments of every member, in member order,
 // with the member index in the segment's cloth field.
@@ -1550,7 +1578,7 @@ static void merge_color_segs(const std::vector<Solver*>& members, std::vector<in
     }
 }
 
-static bool build_batch(Batch& b) {
+static bool build_tables(Batch& b) {
     std::vector<Tile> tiles;
     b.stretch_mode = b.members[0]->stretch_mode;
     for (size_t m = 0; m < b.members.size(); ++m) {
@@ -1587,6 +1615,37 @@ static bool build_batch(Batch& b) {
            b.staging.reserve(sizeof(ClothView) * b.members.size());
 }
 
+// The batch's own tables, and its two parts when it is large enough: the
+// members are dealt largest first to the part with fewer vertices so far,
+// each part keeping the members' relative order.
+static bool build_batch(Batch& b) {
+    if (!build_tables(b)) return false;
+    b.parts.clear();
+    if (g_batch_parts.load(std::memory_order_relaxed) < 2 || b.members.size() < 2 || b.ntiles < kMinSplitTiles || !part_stream())
+        return true;
+    std::vector<size_t> idx(b.members.size());
+    for (size_t m = 0; m < idx.size(); ++m) idx[m] = m;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return b.members[x]->nv > b.members[y]->nv; });
+    std::vector<char> side(b.members.size(), 0);
+    long long load[2] = {0, 0};
+    for (size_t m : idx) {
+        const int k = load[1] < load[0] ? 1 : 0;
+        side[m] = static_cast<char>(k);
+        load[k] += b.members[m]->nv;
+    }
+    for (int k = 0; k < 2; ++k) {
+        auto part = std::make_unique<Batch>();
+        for (size_t m = 0; m < b.members.size(); ++m)
+            if (side[m] == k) part->members.push_back(b.members[m]);
+        if (part->members.empty()) { b.parts.clear(); return true; }
+        if (!build_tables(*part)) return false;
+        b.parts.push_back(std::move(part));
+    }
+    if (!b.fork && !hip_ok(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming), "batch event")) return false;
+    if (!b.join && !hip_ok(hipEventCreateWithFlags(&b.join, hipEventDisableTiming), "batch event")) return false;
+    return true;
+}
+
 // New members for an existing batch: its tables are rebuilt in place (no
 // allocation unless they grow) and its graphs dropped.  Waits for the
 // library's queued work, which may still read the old tables.
@@ -1612,23 +1671,60 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, b
     }
     if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
     b.stepped_substeps = substeps;
-    LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles, b.tile_work.p, b.tile_order.p};
-    SelfBatch* sb = b.self.get(b.members, b.h_views, b.d_views.p);
     const StepPlan plan = plan_of(b.members);
-    if (plan.any_dyn && !b.tuner.poll(b.tuner.tuning())) return false;
-    const bool tune = plan.any_dyn && b.tuner.tuning();
-    if (tune && !b.tuner.begin_step(substeps)) return false;
-    const ColorSegs bends{b.d_bend_segs.p, &b.seg_first, &b.seg_count};
-    const ColorSegs stretch_colors{b.d_stretch_segs.p, &b.sseg_first, &b.sseg_count};
-    std::vector<const PcgGroup*> groups;
-    for (const PcgGroup& g : b.pcg_groups) groups.push_back(&g);
-    auto record = [&] {
-        return run_schedule(b.members, ctx, b.d_views.p, bends, stretch_colors, groups, sb, substeps, iterations, fetch, b.tuner, tune);
-    };
+    // the launch-candidate timing of the tuner runs on the whole table only
+    const bool split = !b.parts.empty() && !b.tuner.tuning();
     std::vector<long long> key;
     const bool graphed = graph_key(plan, substeps, iterations, fetch, b.tuner, &key);
-    if (!(graphed ? b.graph.run(lib_stream(), key, record) : record())) return false;
-    if (tune) b.tuner.end_step();
+    if (split) {
+        // each part's views, then its schedule on its own stream between a
+        // fork from and a join back to the library stream
+        struct PartRun {
+            LaunchCtx ctx;
+            SelfBatch* sb;
+            std::vector<const PcgGroup*> groups;
+        };
+        std::vector<PartRun> runs;
+        for (size_t k = 0; k < b.parts.size(); ++k) {
+            Batch& p = *b.parts[k];
+            for (size_t m = 0; m < p.members.size(); ++m) p.h_views[m] = p.members[m]->view;
+            if (!p.staging.upload(p.d_views.p, p.h_views.data(), sizeof(ClothView) * p.h_views.size(), lib_stream())) return false;
+            p.stepped_substeps = substeps;
+            PartRun r{LaunchCtx{k == 0 ? lib_stream() : part_stream(), p.d_views.p, p.d_tiles.p, p.ntiles, p.tile_work.p, p.tile_order.p},
+                      p.self.get(p.members, p.h_views, p.d_views.p), {}};
+            for (const PcgGroup& g : p.pcg_groups) r.groups.push_back(&g);
+            runs.push_back(std::move(r));
+        }
+        auto record = [&] {
+            if (!hip_ok(hipEventRecord(b.fork, lib_stream()), "batch fork") || !hip_ok(hipStreamWaitEvent(part_stream(), b.fork, 0), "batch fork"))
+                return false;
+            for (size_t k = 0; k < b.parts.size(); ++k) {
+                Batch& p = *b.parts[k];
+                const ColorSegs bends{p.d_bend_segs.p, &p.seg_first, &p.seg_count};
+                const ColorSegs stretch_colors{p.d_stretch_segs.p, &p.sseg_first, &p.sseg_count};
+                if (!run_schedule(p.members, runs[k].ctx, p.d_views.p, bends, stretch_colors, runs[k].groups, runs[k].sb, substeps, iterations,
+                                  fetch, p.tuner, false))
+                    return false;
+            }
+            return hip_ok(hipEventRecord(b.join, part_stream()), "batch join") && hip_ok(hipStreamWaitEvent(lib_stream(), b.join, 0), "batch join");
+        };
+        if (!(graphed ? b.graph.run(lib_stream(), key, record) : record())) return false;
+    } else {
+        LaunchCtx ctx{lib_stream(), b.d_views.p, b.d_tiles.p, b.ntiles, b.tile_work.p, b.tile_order.p};
+        SelfBatch* sb = b.self.get(b.members, b.h_views, b.d_views.p);
+        if (plan.any_dyn && !b.tuner.poll(b.tuner.tuning())) return false;
+        const bool tune = plan.any_dyn && b.tuner.tuning();
+        if (tune && !b.tuner.begin_step(substeps)) return false;
+        const ColorSegs bends{b.d_bend_segs.p, &b.seg_first, &b.seg_count};
+        const ColorSegs stretch_colors{b.d_stretch_segs.p, &b.sseg_first, &b.sseg_count};
+        std::vector<const PcgGroup*> groups;
+        for (const PcgGroup& g : b.pcg_groups) groups.push_back(&g);
+        auto record = [&] {
+            return run_schedule(b.members, ctx, b.d_views.p, bends, stretch_colors, groups, sb, substeps, iterations, fetch, b.tuner, tune);
+        };
+        if (!(graphed ? b.graph.run(lib_stream(), key, record) : record())) return false;
+        if (tune) b.tuner.end_step();
+    }
     if (plan.any_dyn) ++b.tuner.step;
     for (Solver* s : b.members)
         if (!s->dyn.clear_motion(lib_stream())) return false;
@@ -2127,6 +2223,10 @@ SSBL_API int ssbl_ext_synchronize(void) { g_error.clear(); return sync_stream() 
 SSBL_API int ssbl_ext_set_graph_mode(int mode) {
     g_error.clear();
     return g_graph_mode.exchange(mode != 0 ? 1 : 0);
+}
+SSBL_API int ssbl_ext_set_batch_parts(int parts) {
+    g_error.clear();
+    return g_batch_parts.exchange(parts >= 2 ? 2 : 1);
 }
 SSBL_API int ssbl_ext_set_autotune(int mode) {
     g_error.clear();

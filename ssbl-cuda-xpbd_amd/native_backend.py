@@ -532,6 +532,11 @@ def graph_stats(lib, handle=None, batch=None) -> tuple[int, int]:
     return int(captures.value), int(replays.value)
 
 
+def set_batch_parts(lib, parts: int) -> int:
+    """1 or 2 concurrent parts for batches created afterwards; returns the previous setting."""
+    return int(lib.ssbl_ext_set_batch_parts(int(parts)))
+
+
 def set_autotune(lib, enabled: bool) -> bool:
     """Turns launch tuning on or off; returns the previous mode."""
     return bool(lib.ssbl_ext_set_autotune(int(bool(enabled))))
