@@ -390,6 +390,7 @@ def main():
     ap.add_argument("--serving-small-budget-gb", type=float, default=16.0,
                     help="HBM budget of the second served run, small enough to bind")
     ap.add_argument("--graph", type=int, default=1, help="graph replay of the substep loop (0 = direct launches)")
+    ap.add_argument("--batch-parts", type=int, default=2, help="concurrent parts of a large batch (1 = one launch table)")
     ap.add_argument("--only", choices=("batched", "selfcol", "stress", "serving"), default=None,
                     help="only step one workload, untimed, and exit (for profiler and counter passes): the batched "
                          "64-cloth form, the 10k-vertex self-collision cloth or the stress case for warmup + steps "
@@ -416,6 +417,7 @@ def main():
     steps = args.steps if args.steps > 0 else scene.frames
     lib = nb.load_library()
     nb.set_graph_mode(lib, bool(args.graph))
+    nb.set_batch_parts(lib, args.batch_parts)
     log(f"workload {scene.name}: {len(scene.cloths)} cloths, {scene.vertex_count} vertices, "
         f"{scene.substeps}x{scene.iterations}, {steps} frames; device {lib.ssbl_ext_backend_name().decode()}")
     if args.batched_only:
@@ -453,11 +455,12 @@ def main():
     restart(members, scene)
     counters = {"dynamic_triangle_candidate_count": 0, "dynamic_particle_candidate_count": 0, "fast_triangle_pair_contacts": 0,
                 "dynamic_particle_contacts": 0, "static_sdf_contact_count": 0, "scene_entries": 0}
-    probes, done = [], 0
+    probes, probes_parts, done = [], [], 0
     for at in sorted({0, steps // 4, steps // 2, (3 * steps) // 4, steps}):
         run_batched(batch, scene, at - done, cross_b, diagnostics=True, counters=counters)
         done = at
         probes.append(batch.time_pass(32))
+        probes_parts.append(batch.time_iteration(32))
     avg_ms = float(np.mean(probes))
     kernels = {}
     for name, rec in stats.items():
@@ -524,6 +527,13 @@ def main():
             "avg_launch_ms": avg_ms,
             "avg_launch_ms_at": {f"frame_{a}": p for a, p in zip(sorted({0, steps // 4, steps // 2, (3 * steps) // 4, steps}), probes)},
             "avg_launch_ms_dispatch_stamped": vp["total_ms"] / vp["launches"] if vp else None,
+            # the batch steps as two concurrent parts: one iteration over all
+            # cloths is two launches on two streams; its time, same frames
+            "two_part_iteration": {
+                "avg_iteration_ms": float(np.mean(probes_parts)),
+                "achieved": bytes_per_launch / (float(np.mean(probes_parts)) * 1e-3) / 1e9,
+                "frac": bytes_per_launch / (float(np.mean(probes_parts)) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            },
             "other_kernels": roof_more,
         },
     }

@@ -78,6 +78,11 @@ SSBL_API int ssbl_ext_step_batch_ex(void* batch, int substeps, int iterations, i
  * for a batch not stepped since it was created or given new members (the
  * pass needs a prepared step's member views). */
 SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms);
+/* The same for `iterations` iterations of a batch stepped as two parts
+ * (ssbl_ext_set_batch_parts): each iteration is its two parts' passes on
+ * their two streams, as in its steps, and the mean is the time of one
+ * iteration over all members (one table's pass otherwise). */
+SSBL_API int ssbl_ext_time_batch_iteration(void* batch, int iterations, float* avg_ms);
 
 /* Global dynamic scene from cloth handles, gathered on the device: particle k
  * of handle h is that handle's vertex k at its current position with radius

@@ -3398,6 +3398,7 @@ SSBL_API int ssbl_ext_set_static_sdf(void* handle, const int* dims, const float*
 }
 SSBL_API long long ssbl_ext_device_bytes(void*) { fail("oracle: no device memory"); return -1; }
 SSBL_API int ssbl_ext_time_batch_pass(void*, int, float*) { return fail("oracle: no device timing") ? 1 : 0; }
+SSBL_API int ssbl_ext_time_batch_iteration(void*, int, float*) { return fail("oracle: no device timing") ? 1 : 0; }
 SSBL_API int ssbl_ext_synchronize(void) { return 1; }
 // The oracle launches nothing, so it has no graphs to replay or launches to tune.
 SSBL_API int ssbl_ext_set_graph_mode(int) { return 0; }

@@ -130,6 +130,7 @@ EXT_SIGNATURES = {
     "ssbl_ext_step_batch": (_i, [_vp, _i, _i, _i]),
     "ssbl_ext_step_batch_ex": (_i, [_vp, _i, _i, _i, _i]),
     "ssbl_ext_time_batch_pass": (_i, [_vp, _i, _fp]),
+    "ssbl_ext_time_batch_iteration": (_i, [_vp, _i, _fp]),
     "ssbl_ext_device_bytes": (ctypes.c_longlong, [_vp]),
     "ssbl_ext_global_scene_entries": (ctypes.c_longlong, [_vp]),
     "ssbl_ext_update_global_scene_from_handles": (_i, [_vp, ctypes.POINTER(_vp), _ip, _ip, _fp, _i, ctypes.c_float, ctypes.c_float]),

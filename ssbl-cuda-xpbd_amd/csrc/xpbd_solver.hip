@@ -1540,6 +1540,7 @@ struct Batch {
     // launch fills the GPU while the other's waits for memory or ends on a
     // few slow waves.  Empty: the batch steps as one table.
     std::vector<std::unique_ptr<Batch>> parts;
+    bool own_tables = false;   // the one-table form's tables below are built (always, unless split)
     hipEvent_t fork = nullptr, join = nullptr;
     Batch() = default;
     Batch(const Batch&) = delete;
@@ -1619,10 +1620,22 @@ static bool build_tables(Batch& b) {
 // members are dealt largest first to the part with fewer vertices so far,
 // each part keeping the members' relative order.
 static bool build_batch(Batch& b) {
-    if (!build_tables(b)) return false;
-    b.parts.clear();
-    if (g_batch_parts.load(std::memory_order_relaxed) < 2 || b.members.size() < 2 || b.ntiles < kMinSplitTiles || !part_stream())
+    b.own_tables = false;
+    int tiles = 0;
+    for (const Solver* s : b.members) tiles += s->ntiles;
+    if (g_batch_parts.load(std::memory_order_relaxed) < 2 || b.members.size() < 2 || tiles < kMinSplitTiles || !part_stream()) {
+        b.parts.clear();
+        if (!build_tables(b)) return false;
+        b.own_tables = true;
         return true;
+    }
+    // split: the batch's own one-table form is built only when asked for
+    // (launch tuning, one-table pass timing)
+    b.stretch_mode = b.members[0]->stretch_mode;
+    for (const Solver* s : b.members)
+        if (s->stretch_mode != b.stretch_mode) return fail("batch members must share the stretch mode");
+    b.ntiles = tiles;
+    b.stepped_substeps = 0;
     std::vector<size_t> idx(b.members.size());
     for (size_t m = 0; m < idx.size(); ++m) idx[m] = m;
     std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return b.members[x]->nv > b.members[y]->nv; });
@@ -1633,16 +1646,36 @@ static bool build_batch(Batch& b) {
         side[m] = static_cast<char>(k);
         load[k] += b.members[m]->nv;
     }
+    // a batch given new members keeps its parts' (grow-only) tables
+    b.parts.resize(2);
+    bool degenerate = false;
     for (int k = 0; k < 2; ++k) {
-        auto part = std::make_unique<Batch>();
+        if (!b.parts[k]) b.parts[k] = std::make_unique<Batch>();
+        Batch& part = *b.parts[k];
+        part.members.clear();
         for (size_t m = 0; m < b.members.size(); ++m)
-            if (side[m] == k) part->members.push_back(b.members[m]);
-        if (part->members.empty()) { b.parts.clear(); return true; }
-        if (!build_tables(*part)) return false;
-        b.parts.push_back(std::move(part));
+            if (side[m] == k) part.members.push_back(b.members[m]);
+        if (part.members.empty()) { degenerate = true; break; }
+        if (!build_tables(part)) return false;
+    }
+    if (degenerate) {
+        b.parts.clear();
+        if (!build_tables(b)) return false;
+        b.own_tables = true;
+        return true;
     }
     if (!b.fork && !hip_ok(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming), "batch event")) return false;
     if (!b.join && !hip_ok(hipEventCreateWithFlags(&b.join, hipEventDisableTiming), "batch event")) return false;
+    return true;
+}
+
+// The one-table form of a split batch, built on first use.
+static bool ensure_own_tables(Batch& b) {
+    if (b.own_tables) return true;
+    const int stepped = b.stepped_substeps;
+    if (!build_tables(b)) return false;
+    b.stepped_substeps = stepped;
+    b.own_tables = true;
     return true;
 }
 
@@ -1667,16 +1700,20 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, b
         Solver& s = *b.members[m];
         reset_diag(s);
         if (!prepare_member(s, substeps, iterations, fetch)) return false;
-        b.h_views[m] = s.view;
     }
-    if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
-    b.stepped_substeps = substeps;
-    const StepPlan plan = plan_of(b.members);
     // the launch-candidate timing of the tuner runs on the whole table only
-    const bool split = !b.parts.empty() && !b.tuner.tuning();
+    const StepPlan plan = plan_of(b.members);
+    const bool split = !b.parts.empty() && !(plan.any_dyn && b.tuner.tuning());
+    if (!split) {
+        if (!ensure_own_tables(b)) return false;
+        for (size_t m = 0; m < b.members.size(); ++m) b.h_views[m] = b.members[m]->view;
+        if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
+    }
+    b.stepped_substeps = substeps;
     std::vector<long long> key;
-    const bool graphed = graph_key(plan, substeps, iterations, fetch, b.tuner, &key);
     if (split) {
+        const bool graphed = graph_key(plan, substeps, iterations, fetch, b.tuner, &key);
+        key.push_back(1);   // the two-part launch sequence
         // each part's views, then its schedule on its own stream between a
         // fork from and a join back to the library stream
         struct PartRun {
@@ -1702,8 +1739,9 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, b
                 Batch& p = *b.parts[k];
                 const ColorSegs bends{p.d_bend_segs.p, &p.seg_first, &p.seg_count};
                 const ColorSegs stretch_colors{p.d_stretch_segs.p, &p.sseg_first, &p.sseg_count};
+                // the batch's tuned pooling limit (never timed here)
                 if (!run_schedule(p.members, runs[k].ctx, p.d_views.p, bends, stretch_colors, runs[k].groups, runs[k].sb, substeps, iterations,
-                                  fetch, p.tuner, false))
+                                  fetch, b.tuner, false))
                     return false;
             }
             return hip_ok(hipEventRecord(b.join, part_stream()), "batch join") && hip_ok(hipStreamWaitEvent(lib_stream(), b.join, 0), "batch join");
@@ -1715,6 +1753,8 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, b
         if (plan.any_dyn && !b.tuner.poll(b.tuner.tuning())) return false;
         const bool tune = plan.any_dyn && b.tuner.tuning();
         if (tune && !b.tuner.begin_step(substeps)) return false;
+        const bool graphed = graph_key(plan, substeps, iterations, fetch, b.tuner, &key);
+        key.push_back(0);
         const ColorSegs bends{b.d_bend_segs.p, &b.seg_first, &b.seg_count};
         const ColorSegs stretch_colors{b.d_stretch_segs.p, &b.sseg_first, &b.sseg_count};
         std::vector<const PcgGroup*> groups;
@@ -1750,13 +1790,25 @@ static bool step_batch(Batch& b, int substeps, int iterations, int force_sync, b
 // The batch must have been stepped since its tables were built: the members'
 // views (refilled here, with that step's substep count) describe a prepared
 // step only then.
-static bool time_batch_pass(Batch& b, int launches, float* avg_ms) {
+// With `parts`, a batch stepped as two parts runs each iteration as its two
+// parts' launches on their two streams (as its steps do), and the mean is
+// the time of one iteration over all members.
+static bool time_batch_pass(Batch& b, int launches, float* avg_ms, bool parts = false) {
     if (b.stepped_substeps <= 0 || b.ntiles <= 0) return fail("time a batch's pass after stepping it once");
+    if (!ensure_own_tables(b)) return false;
     for (size_t m = 0; m < b.members.size(); ++m) {
         b.members[m]->fill_view(b.stepped_substeps);
         b.h_views[m] = b.members[m]->view;
     }
     if (!b.staging.upload(b.d_views.p, b.h_views.data(), sizeof(ClothView) * b.h_views.size(), lib_stream())) return false;
+    parts = parts && !b.parts.empty();
+    if (parts) {
+        for (auto& part : b.parts) {
+            for (size_t m = 0; m < part->members.size(); ++m) part->h_views[m] = part->members[m]->view;
+            if (!part->staging.upload(part->d_views.p, part->h_views.data(), sizeof(ClothView) * part->h_views.size(), lib_stream()))
+                return false;
+        }
+    }
     launches = std::max(2, launches + (launches & 1));
     size_t total = 0;
     for (Solver* s : b.members) total += 4 * static_cast<size_t>(s->nv);
@@ -1787,7 +1839,18 @@ static bool time_batch_pass(Batch& b, int launches, float* avg_ms) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!hip_ok(hipEventCreate(&e0), "timing event") || !hip_ok(hipEventCreate(&e1), "timing event")) return false;
     bool ok = hip_ok(hipEventRecord(e0, ctx.stream), "timing event");
-    for (int k = 0; ok && k < launches; ++k) launch_vertex_pass(ctx, k & 1, (k & 1) ^ 1, ops);
+    if (parts) {
+        ok = ok && hip_ok(hipStreamWaitEvent(part_stream(), e0, 0), "timing event");
+        for (int k = 0; ok && k < launches; ++k)
+            for (size_t q = 0; q < b.parts.size(); ++q) {
+                const Batch& p = *b.parts[q];
+                launch_vertex_pass(LaunchCtx{q == 0 ? lib_stream() : part_stream(), p.d_views.p, p.d_tiles.p, p.ntiles}, k & 1, (k & 1) ^ 1,
+                                   ops);
+            }
+        ok = ok && hip_ok(hipEventRecord(b.join, part_stream()), "timing event") && hip_ok(hipStreamWaitEvent(lib_stream(), b.join, 0), "timing event");
+    } else {
+        for (int k = 0; ok && k < launches; ++k) launch_vertex_pass(ctx, k & 1, (k & 1) ^ 1, ops);
+    }
     ok = ok && hip_ok(hipEventRecord(e1, ctx.stream), "timing event") && hip_ok(hipEventSynchronize(e1), "timing event");
     float ms = 0.0f;
     ok = ok && hip_ok(hipEventElapsedTime(&ms, e0, e1), "timing event");
@@ -2218,6 +2281,11 @@ SSBL_API int ssbl_ext_time_batch_pass(void* batch, int launches, float* avg_ms) 
     g_error.clear();
     if (!batch || !avg_ms) return fail("invalid batch timing request") ? 1 : 0;
     return time_batch_pass(*static_cast<Batch*>(batch), launches, avg_ms) ? 1 : 0;
+}
+SSBL_API int ssbl_ext_time_batch_iteration(void* batch, int iterations, float* avg_ms) {
+    g_error.clear();
+    if (!batch || !avg_ms) return fail("invalid batch timing request") ? 1 : 0;
+    return time_batch_pass(*static_cast<Batch*>(batch), iterations, avg_ms, true) ? 1 : 0;
 }
 SSBL_API int ssbl_ext_synchronize(void) { g_error.clear(); return sync_stream() ? 1 : 0; }
 SSBL_API int ssbl_ext_set_graph_mode(int mode) {

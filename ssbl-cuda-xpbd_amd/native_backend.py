@@ -501,6 +501,14 @@ class SolverBatch:
             raise NativeSolverError(last_error(self._lib))
         return float(out[0])
 
+    def time_iteration(self, iterations: int = 64) -> float:
+        """Mean ms of one iteration over all members, a batch stepped as two
+        parts running its parts' passes concurrently as its steps do."""
+        out = np.zeros(1, np.float32)
+        if not self._lib.ssbl_ext_time_batch_iteration(self._handle, int(iterations), _fptr(out)):
+            raise NativeSolverError(last_error(self._lib))
+        return float(out[0])
+
     def close(self) -> None:
         if getattr(self, "_handle", None):
             self._lib.ssbl_ext_destroy_batch(self._handle)
