@@ -194,6 +194,10 @@ struct ClothView {
     const DynView* sdyn;
     float collider_dt;
     int pad5;
+    // Per vertex, the deepest triangle contact key of the pooled searches of
+    // the substep's dynamic contacts (0 between launches: the contact pass
+    // clears what it reads).
+    unsigned long long* dyn_best;
     // Stretch edges {i, j, rest length bits, 0} in input (colour-sorted)
     // order, for the coloured Gauss-Seidel stretch mode (null otherwise).
     const int4* sedges;
@@ -210,21 +214,33 @@ struct LaunchCtx {
     const ClothView* views;     // device table
     const Tile* tiles;          // device tile table
     int ntiles;
-    // The dynamic contact pass adds each tile's probe work to tile_work and
-    // takes its tiles in tile_order (most work first), which
-    // launch_order_tiles rebuilds from the work of the step before (both
-    // null: tiles in table order)
+    // The dynamic contacts' launch plan (required when a launch has dynamic
+    // contacts): their launches add each tile's work to tile_work
+    // (dynamic_work_ints), and launch_order_tiles turns the work of the step
+    // before into tile_order (dynamic_order_ints): the contact pass's tile
+    // order and the triangle searches' units (k_order_tiles).
     int* tile_work;
     int* tile_order;
 };
+
+// Triangle search units: a tile's pooled lists split over up to
+// kDynMaxSlices workgroups, one more per kDynSliceItems items of its longest
+// wave list in the step before.
+constexpr int kDynSliceItems = 512;
+constexpr int kDynMaxSlices = 8;   // at most 16 (4 bits of a unit)
+constexpr int kUnitSliceShift = 23, kUnitSlicesShift = 27, kUnitTileMask = (1 << kUnitSliceShift) - 1;
+inline int dynamic_work_ints(int ntiles) { return 2 * ntiles; }
+inline int dynamic_order_ints(int ntiles) { return 4 * ntiles + 1; }
 
 void launch_integrate(const LaunchCtx& c, int parity, unsigned ops, bool full);
 void launch_vertex_pass(const LaunchCtx& c, int src_parity, int dst_parity, unsigned ops);
 // tail_ops: the closing pin polish and velocity update, fused into the
 // particle launch when nothing else runs between (0: none)
-// The dynamic contact pass (triangles, then particles) pools the particle
-// searches of a wave with at most `pool_max` probing vertices (0..64); the
-// events, when given, bracket the launch.
+// The dynamic contacts: the triangle search units, then the contact pass
+// (triangles from the searches' keys, then particles), which pools the
+// particle searches of a wave with at most `pool_max` probing vertices
+// (0..64); the events, when given, bracket both launches.  The context's
+// launch plan must have been built by launch_order_tiles in the same step.
 constexpr int kDefaultPoolMax = 16;
 void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops, int pool_max = kDefaultPoolMax,
                          hipEvent_t begin = nullptr, hipEvent_t end = nullptr);

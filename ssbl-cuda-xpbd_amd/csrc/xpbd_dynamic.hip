@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "xpbd_math.hpp"
 #include "xpbd_timing.hpp"
@@ -177,11 +178,19 @@ __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv
         sp.init(box_cmin(bx), box_cmax(bx));
     }
     const int buckets = dv->tri_buckets;
+    // the triangle's layer range (its layer's place among the scene's layers)
+    const int classes = dv->tri_classes;
+    int cls = 0;
+    if (live && classes > 1 && tlayer) {
+        const int layer = tlayer[t];
+        for (int k = 1; k < classes; ++k) cls += layer >= dv->tri_class_layer[k] ? 1 : 0;
+    }
+    const int clo = dv->tri_class_base[cls], chi = dv->tri_class_base[cls + 1];
     short* my_slot = s_slot + tl * kDynMaxHashCells;
     for (int idx = lane; idx < sp.n; idx += kTriLanes) {
         int x, y, z;
         sp.cell(idx, &x, &y, &z);
-        const int b = static_cast<int>(hash_cell(x, y, z, buckets));
+        const int b = tri_bucket(classes, buckets, clo, chi, x, y, z);
         const int h = agg_slot(s_bucket, s_list, &s_n, b);
         if (h >= 0) atomicAdd(&s_count[h], 1);
         else if (phase == 0) atomicAdd(&count[b], 1);
@@ -211,12 +220,17 @@ __global__ __launch_bounds__(256) void k_tri_hash(const DynView* __restrict__ dv
         } else {
             int x, y, z;
             sp.cell(idx, &x, &y, &z);
-            b = static_cast<int>(hash_cell(x, y, z, buckets));
+            b = tri_bucket(classes, buckets, clo, chi, x, y, z);
             slot = atomicSub(&count[b], 1) - 1;
         }
         const int at = off[b] + slot, key = t * kDynMaxHashCells + idx;
-        if (ranked) raw[at] = make_int2(key, b);
-        else ent[at] = make_int4(key, sl.x, sl.y, 0);
+        if (ranked) {
+            raw[at] = make_int2(key, b);
+        } else {
+            int x, y, z;
+            sp.cell(idx, &x, &y, &z);
+            ent[at] = make_int4(key, sl.x, sl.y, cell_tag(x, y, z));
+        }
     }
 }
 
@@ -265,31 +279,41 @@ __global__ __launch_bounds__(256) void k_tri_rank(const int* __restrict__ off, i
 __device__ __forceinline__ int2 entry_source_layer(int2 e) { return e; }
 __device__ __forceinline__ int2 entry_source_layer(const PartEntry& e) { return make_int2(e.sl.x, e.sl.y); }
 __device__ __forceinline__ int2 entry_source_layer(int4 e) { return make_int2(e.y, e.z); }
+__device__ __forceinline__ int entry_tag(const PartEntry&) { return 0; }
+__device__ __forceinline__ int entry_tag(int4 e) { return e.w; }
+__device__ __forceinline__ int entry_tag(int2) { return 0; }
+// tagged: bins of a triangle hash with layer ranges, whose third field is
+// the cell tag all entries share (0: several) instead of the minimum layer
+// (one layer per range)
 template <typename R>
 __global__ __launch_bounds__(256) void k_bucket_bins(const int* __restrict__ off, int buckets, int cap, const R* __restrict__ rec,
-                                                     int4* bins) {
+                                                     int4* bins, int tagged) {
     for (int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); b < buckets; b += static_cast<int>(gridDim.x * blockDim.x)) {
         const int base = off[b], n = off[b + 1] - base;
-        int min_layer = rec ? 0x7fffffff : static_cast<int>(0x80000000u), src = -2;
+        int min_layer = rec ? 0x7fffffff : static_cast<int>(0x80000000u), src = -2, tag = -1;
         if (rec) {
             // four entries in flight per round (the merge is order-free: a
-            // minimum, and whether all sources agree)
+            // minimum, and whether all sources and cells agree)
             const int m = min(n, cap);
             for (int k = 0; k < m; k += 4) {
-                int2 e[4];
+                R r[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) e[u] = k + u < m ? entry_source_layer(rec[base + k + u]) : make_int2(-2, 0x7fffffff);
+                for (int u = 0; u < 4; ++u)
+                    if (k + u < m) r[u] = rec[base + k + u];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     if (k + u >= m) break;
-                    min_layer = min(min_layer, e[u].y);
-                    src = (src == -2 || src == e[u].x) ? e[u].x : -1;
+                    const int2 e = entry_source_layer(r[u]);
+                    const int g = entry_tag(r[u]);
+                    min_layer = min(min_layer, e.y);
+                    src = (src == -2 || src == e.x) ? e.x : -1;
+                    tag = (tag == -1 || tag == g) ? g : 0;
                 }
             }
         } else if (n > 0) {
             src = -1;
         }
-        bins[b] = make_int4(base, n, min_layer, src);
+        bins[b] = make_int4(base, n, tagged ? (tag == -1 ? 0 : tag) : min_layer, src);
     }
 }
 
@@ -619,6 +643,9 @@ bool DynamicColliders::upload_view(hipStream_t st) {
     v.part_motion = part_motion_ ? 1 : 0;
     v.pctr = pctr_;
     v.pprev = pprev_;
+    v.tri_classes = (v.tri_hash && tri_classes_ > 1 && !signed_ && tri_filter_) ? tri_classes_ : 1;
+    for (int k = 0; k <= kMaxLayerClasses; ++k) v.tri_class_base[k] = v.tri_classes > 1 ? class_base_[k] : 0;
+    for (int k = 0; k < kMaxLayerClasses; ++k) v.tri_class_layer[k] = v.tri_classes > 1 ? class_layer_[k] : 0;
     if (v.tri_hash && v.tri_buckets > tri_buckets_) return fail("dynamic triangle hash capacity");
     if (v.part_hash && v.part_buckets > part_buckets_) return fail("dynamic particle hash capacity");
     // the device copy already holds this view (a scene gathered again from
@@ -639,6 +666,7 @@ bool DynamicColliders::set_triangles(hipStream_t st, const float* soup, int n, c
     signed_ = false;
     tri_motion_ = false;
     tri_ordered_ = false;
+    tri_classes_ = 1;
     if (n == 0) { nt_ = 0; bounds_valid_ = false; return d_view_ ? upload_view(st) : true; }
     if (!reserve_triangles(n)) return false;
     float max_extent = 0.0f;
@@ -684,6 +712,7 @@ bool DynamicColliders::set_indexed(hipStream_t st, const float* verts, int nvert
     ntri = std::max(ntri, 0);
     if (!sync_stream()) return false;
     tri_ordered_ = false;
+    tri_classes_ = 1;
     if (nverts == 0 || ntri == 0) {
         nt_ = 0;
         signed_ = tri_motion_ = bounds_valid_ = false;
@@ -792,6 +821,28 @@ bool DynamicColliders::gather(hipStream_t st, const std::vector<GatherSource>& s
     margin = std::max(margin, 0.0f);
     thickness = std::max(thickness, 0.0f);
     const bool keep_prev = np == np_ && np > 0;
+    // one bucket range per distinct triangle layer, each sized by its share
+    // of the triangles (one range when there are more layers than ranges)
+    tri_classes_ = 1;
+    {
+        std::map<int, long long> per_layer;
+        for (const GatherSource& s : sources)
+            if (s.nt > 0) per_layer[s.layer] += s.nt;
+        const long long buckets = next_pow2(std::max(kDynMinBuckets, nt * 4));
+        if (nt >= kDynTriangleHashMin && per_layer.size() >= 2 && per_layer.size() <= static_cast<size_t>(kMaxLayerClasses)) {
+            int k = 0;
+            bool ok = true;
+            class_base_[0] = 0;
+            for (const auto& kv : per_layer) {
+                const long long size = buckets * kv.second / nt;
+                ok = ok && size >= 1;
+                class_layer_[k] = kv.first;
+                class_base_[k + 1] = class_base_[k] + static_cast<int>(size);
+                ++k;
+            }
+            if (ok) tri_classes_ = k;
+        }
+    }
     // segment tables: {source, begin, end} per 256 items
     std::vector<int4> segs;
     for (size_t k = 0; k < sources.size(); ++k)
@@ -904,10 +955,10 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
             TimedLaunch tb("tri_bins", st);
             if (ranked)
                 launch_k(k_bucket_bins<int2>, gb, blk, 0, st, static_cast<const int*>(tri_off_), buckets, 0x7fffffff,
-                         static_cast<const int2*>(tri_filter_ ? tri_ksl_ : nullptr), tri_bin_);
+                         static_cast<const int2*>(tri_filter_ ? tri_ksl_ : nullptr), tri_bin_, 0);
             else
                 launch_k(k_bucket_bins<int4>, gb, blk, 0, st, static_cast<const int*>(tri_off_), buckets, 0x7fffffff,
-                         static_cast<const int4*>(tri_filter_ ? tri_ent_ : nullptr), tri_bin_);
+                         static_cast<const int4*>(tri_filter_ ? tri_ent_ : nullptr), tri_bin_, view_.tri_classes > 1 ? 1 : 0);
             if (count_diag && occupancy) {
                 launch_k(k_fill_i32, dim3(1), dim3(1), 0, st, occupancy_, 0, 1);
                 launch_k(k_bucket_max, dim3(std::min<unsigned>(grid_for(buckets, 256), 4096u)), blk, 0, st,
@@ -928,7 +979,7 @@ bool DynamicColliders::build(hipStream_t st, unsigned long long* counts, bool co
         }
         launch_k(k_bucket_bins<PartEntry>, dim3(std::min<unsigned>(grid_for(buckets, 256), 16384u)), blk, 0, st,
                  static_cast<const int*>(poff_), buckets, kDynParticleSlots, static_cast<const PartEntry*>(part_filter_ ? pent_ : nullptr),
-                 pbin_);
+                 pbin_, 0);
         launch_k(k_coarse_fill, dim3(grid_for(coarse_buckets_, 256)), blk, 0, st, coarse_, coarse_buckets_, coarse_off_);
         launch_k(k_coarse_insert, g, blk, 0, st, static_cast<const DynView*>(d_view_), coarse_, coarse_off_);
     }

@@ -28,6 +28,7 @@ constexpr int kDynMaxHashCells = 64;        // A41:79
 constexpr int kDynParticleHashMin = 128;    // A41:25
 constexpr int kDynTriangleHashMin = 64;     // A41:39
 constexpr int kDynMinBuckets = 1024;        // A41:27, 40
+constexpr int kMaxLayerClasses = 8;         // layer ranges of a gathered scene's triangle hash
 
 enum DynCountSlot {
     kCountDynParticleCandidates = 4, kCountDynParticleContacts = 5, kCountDynParticleOverflow = 6, kCountTrianglePairs = 7,
@@ -87,8 +88,8 @@ struct DynView {
     const TriBox* box;       // bounds of every triangle (cell span only in hashed sets)
     const int* tri_keys;     // triangle * 64 + cell index, ascending per bucket
     const int2* tri_ksl;     // {source id, layer} of each key's triangle (filtered sets)
-    const int4* tri_bin;     // per bucket: {offset, count, min layer, the one source of all entries | -1 mixed | -2 empty}
-    const int4* tri_ent;     // unranked sets: {key, source, layer, 0} per entry, in no particular order
+    const int4* tri_bin;     // per bucket: {offset, count, min layer (with layer ranges: the entries' one cell tag | 0), the one source of all entries | -1 mixed | -2 empty}
+    const int4* tri_ent;     // unranked sets: {key, source, layer, cell tag} per entry, in no particular order
     const int* large;        // triangles spanning more than 64 cells, ascending
     const int* nlarge;       // their count (device)
     int np;
@@ -117,7 +118,37 @@ struct DynView {
     int signed_set, tri_motion, part_motion, pad5;
     const float4* pctr;      // previous triangle centres
     const float4* pprev;     // previous particle positions (.w = radius)
+    // A gathered scene whose triangles carry 2 .. kMaxLayerClasses distinct
+    // layers keeps one bucket range per layer, ascending: a triangle of
+    // layer tri_class_layer[k] is entered in range k, buckets
+    // [tri_class_base[k], tri_class_base[k + 1]) (sized by the layer's share
+    // of the triangles), so a query that accepts only lower layers probes
+    // only the ranges below its own and walks no entry its filter rejects by
+    // layer.  tri_classes 1: one range, hash_cell over tri_buckets.
+    int tri_classes;
+    int tri_class_base[kMaxLayerClasses + 1];
+    int tri_class_layer[kMaxLayerClasses];
 };
+
+// Nonzero tag of a cell.  The entries of a triangle hash with layer ranges
+// carry their cell's tag, and a bucket bin there holds the tag its entries
+// share (0: several cells): a query from a cell of another tag cannot pass
+// the canonical-cell test on any of them (a triangle whose span holds the
+// query's cell is listed under that cell, in the query's own bucket), so it
+// skips the bucket with the bin load alone.
+__device__ __forceinline__ int cell_tag(int x, int y, int z) {
+    const unsigned h = static_cast<unsigned>(x) * 0x8da6b343u ^ static_cast<unsigned>(y) * 0xd8163841u ^ static_cast<unsigned>(z) * 0xcb1ab31fu;
+    return static_cast<int>(h | 1u);
+}
+
+// Bucket of cell (x, y, z) in the layer range [lo, hi) of a triangle hash
+// with `classes` ranges (multiply-shift into the range for two or more).
+__device__ __forceinline__ int tri_bucket(int classes, int buckets, int lo, int hi, int x, int y, int z) {
+    if (classes <= 1) return static_cast<int>(hash_cell(x, y, z, buckets));
+    const unsigned h = (static_cast<unsigned>(x) * 73856093u ^ static_cast<unsigned>(y) * 19349663u ^ static_cast<unsigned>(z) * 83492791u) *
+                       2654435761u;
+    return lo + static_cast<int>((static_cast<unsigned long long>(h) * static_cast<unsigned>(hi - lo)) >> 32);
+}
 
 // One contributing cloth of a device-gathered scene.
 struct GatherSource {
@@ -216,6 +247,10 @@ private:
     int coarse_buckets_ = 0;
     float max_radius_ = 0.0f;
     int* scan_tmp_ = nullptr;
+    // layer ranges of the triangle hash (DynView::tri_classes)
+    int tri_classes_ = 1;
+    int class_base_[kMaxLayerClasses + 1] = {};
+    int class_layer_[kMaxLayerClasses] = {};
     std::vector<GatherSource> gathered_;   // the sources the device tables hold (empty: none valid)
     GatherSource* d_sources_ = nullptr;
     int sources_cap_ = 0;

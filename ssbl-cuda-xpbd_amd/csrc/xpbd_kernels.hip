@@ -391,169 +391,244 @@ __device__ __forceinline__ F3 shfl3(F3 a, int lane) { return f3(__shfl(a.x, lane
 // per-lane walks are already balanced.  The launch takes the limit
 // (kDefaultPoolMax unless tuned); every value gives the same contacts.
 
-__device__ inline int dynamic_triangle_contacts(const ClothView& c, VertexState& s, bool active, float dt, bool count) {
+// The diagnostic weight of one listing of a triangle in a layer range's
+// bucket, for counters equal to those of one table over the whole scene:
+// that table (hash_cell over `buckets`) lists the triangle once per cell of
+// its span [lo, hi] hashed to the query cell (x, y, z)'s bucket, and
+// A41:3934-3982 takes each listing as a candidate.  The range's bucket
+// (range [rlo, rhi)) may list it a different number of times; the first of
+// those listings (span cell `idx` in key order) carries the one-table count
+// and the others none.  Spans of hashed triangles hold at most 64 cells.
+__device__ inline int one_table_listings(int4 lo, int4 hi, int x, int y, int z, int buckets, int classes, int rlo, int rhi, int idx) {
+    const unsigned h = hash_cell(x, y, z, buckets);
+    const int rb = tri_bucket(classes, buckets, rlo, rhi, x, y, z);
+    int m = 0, first = -1, k = 0;
+    for (int qz = lo.z; qz <= hi.z; ++qz)
+        for (int qy = lo.y; qy <= hi.y; ++qy)
+            for (int qx = lo.x; qx <= hi.x; ++qx, ++k) {
+                m += hash_cell(qx, qy, qz, buckets) == h ? 1 : 0;
+                if (first < 0 && tri_bucket(classes, buckets, rlo, rhi, qx, qy, qz) == rb) first = k;
+            }
+    return idx == first ? m : 0;
+}
+
+// The pooled triangle search of one wave (vertices at p; lanes without a
+// vertex pass active false): the wave's (vertex, bucket entry) items, the
+// rounds of 64 items `slice`, slice + nslices, slice + 2 nslices, ... of them
+// (all rounds with one slice), each item's (depth, ~key) folded into its
+// vertex's maximum in LDS.  Returns the lane's vertex's maximum over those
+// rounds (0: no contact among them); `total` gets the wave's item count.
+// The maximum over all slices names the same contact as one pass over all
+// rounds: the key orders contacts exactly.  A hash with layer ranges is
+// searched range by range (the rounds numbered on across the ranges), and
+// in the lower-layers mode only the ranges below the query's layer.
+__device__ inline unsigned long long dynamic_triangle_pool(const ClothView& c, F3 p, bool active, int slice, int nslices, bool count,
+                                                           int& total_out) {
     const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
     const int lane = static_cast<int>(__lane_id());
     const float cm = fmaxf(c.collision_margin, 0.0f);
     const int nt = D.nt;
-    int work = 0;   // the wave's pooled items
-    if (nt > 0 && cm > 0.0f) {
-        float best_pen = 0.0f;
-        F3 best_delta = f3(0.0f, 0.0f, 0.0f);
-        unsigned cand = 0, rej = 0;
-        if (D.tri_hash) {   // own cell only: the unsigned query radius is 0 (A41:3623-3633)
-            int cx = 0, cy = 0, cz = 0;
-            int4 bin = make_int4(0, 0, 0, -2);
-            if (active) {
-                cx = cell_coord(s.p.x, D.tri_cell); cy = cell_coord(s.p.y, D.tri_cell); cz = cell_coord(s.p.z, D.tri_cell);
-                bin = gload(D.tri_bin + hash_cell(cx, cy, cz, D.tri_buckets));
-                if (dyn_bucket_rejected(c, bin)) bin.y = 0;
-            }
-            // The wave's probes as one list of (vertex, entry) items, 64 at a
-            // time: lane i owns items [start_i, start_i + n_i); item w belongs
-            // to the last lane whose start is at most w.  Each item's
-            // (depth, ~key) goes into its vertex's LDS maximum; the winner's
-            // push is recomputed by the vertex afterwards.
-            const int n_i = bin.y;
-            int incl = n_i;
-            for (int d = 1; d < 64; d <<= 1) {
-                const int o = __shfl_up(incl, d, 64);
-                if (lane >= d) incl += o;
-            }
-            const int start = incl - n_i;
-            const int total = __shfl(incl, 63, 64);
-            work = total;
-            if (total > 0) {   // uniform over the wave
-                __shared__ unsigned long long s_best[kTile];
-                unsigned long long* best = s_best + (threadIdx.x & ~63u);
-                best[lane] = 0ull;
-                wave_lds_sync();
-                const int first = bin.x - start;   // entry of item w is first + w in the owner's bucket
-                // owner lane of item w: the last lane whose start is at most w
-                auto owner_of = [&](int w) {
-                    int own = 0;
-                    for (int step = 32; step > 0; step >>= 1) {
-                        const int probe = own + step;
-                        const int sp = __shfl(start, min(probe, 63), 64);
-                        if (probe < 64 && sp <= w) own = probe;
-                    }
-                    return own;
-                };
-                auto entry_of = [&](int w, int own) {
-                    const int k = __shfl(first, own, 64) + w;
-                    return w < total ? gload(D.tri_ent + k) : make_int4(-1, 0, 0, 0);
-                };
-                // Items flow through three stages: a round's entry records are
-                // requested two rounds ahead and its triangles' bounds and
-                // corners one round ahead, so a round issues no load it waits
-                // for itself.  The source filter (in the entry) runs before the
-                // bounds are read; it and the canonical-cell test
-                // (A41:3934-3982) are both side-effect free.
-                auto accepted = [&](int4 e) {
-                    return e.x >= 0 && e.x / kDynMaxHashCells < nt && (!D.tsrc || dyn_enabled(c, e.y, e.z));
-                };
-                // a triangle's bounds and corners, requested together one
-                // round ahead (the corners whether or not the bounds pass)
-                struct TriRec {
-                    TriBox bx;
-                    float4 a, b, c;
-                };
-                auto record_of = [&](int4 e) {
-                    TriRec r{};
-                    if (accepted(e)) {
-                        const int t = e.x / kDynMaxHashCells;
-                        r.bx = gload(D.box + t);
-                        r.a = gload(D.tv + 3 * t);
-                        r.b = gload(D.tv + 3 * t + 1);
-                        r.c = gload(D.tv + 3 * t + 2);
-                    }
-                    return r;
-                };
-                int own = owner_of(lane);
-                int4 en = entry_of(lane, own);
-                int own1 = owner_of(lane + 64);
-                int4 en1 = entry_of(lane + 64, own1);
-                TriRec rc = record_of(en);
-                for (int base = 0; base < total; base += 64) {
-                    const int w = base + lane;
-                    const int ox = __shfl(cx, own, 64), oy = __shfl(cy, own, 64), oz = __shfl(cz, own, 64);
-                    const F3 op = shfl3(s.p, own);
-                    const int own2 = owner_of(w + 128);
-                    const int4 en2 = entry_of(w + 128, own2);
-                    const TriRec rc1 = record_of(en1);
-                    if (w < total && accepted(en)) {
-                        const int4 lo = box_cmin(rc.bx), hi = box_cmax(rc.bx);
-                        if (ox >= lo.x && ox <= hi.x && oy >= lo.y && oy <= hi.y && oz >= lo.z && oz <= hi.z) {
-                            F3 d;
-                            const float pen = dyn_triangle_depth_with(
-                                rc.bx,
-                                [&](F3& a, F3& b, F3& c) {
-                                    a = f3(rc.a);
-                                    b = f3(rc.b);
-                                    c = f3(rc.c);
-                                },
-                                op, cm, cand, rej, d);
-                            const unsigned long long kk = depth_key(pen, static_cast<unsigned>(en.x));
-                            if (kk != 0ull) atomicMax(best + own, kk);
-                        }
-                    }
-                    own = own1;
-                    en = en1;
-                    rc = rc1;
-                    own1 = own2;
-                    en1 = en2;
-                }
-                wave_lds_sync();
-                const unsigned long long top = best[lane];
-                if (top != 0ull) {
-                    // the winning entry again, for its push (same inputs, same result)
-                    const int t = static_cast<int>(0xffffffffu - static_cast<unsigned>(top)) / kDynMaxHashCells;
-                    unsigned dc = 0, dr = 0;
-                    F3 d;
-                    best_pen = dyn_triangle_depth(D, t, gload(D.box + t), s.p, cm, dc, dr, d);
-                    best_delta = d;
-                }
-            }
-            if (active) {
-                const int nl = gload(D.nlarge);
-                for (int k = 0; k < nl; ++k) {
-                    const int t = gload(D.large + k);
-                    if (t < 0 || t >= nt) continue;
-                    if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) continue;
-                    F3 d;
-                    const float pen = dyn_triangle_depth(D, t, D.box[t], s.p, cm, cand, rej, d);
-                    if (pen > best_pen) { best_pen = pen; best_delta = d; }
-                }
-            }
-        } else if (active) {
-            for (int t = 0; t < nt; ++t) {
-                if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) continue;
-                F3 d;
-                const float pen = dyn_triangle_depth(D, t, D.box[t], s.p, cm, cand, rej, d);
-                if (pen > best_pen) { best_pen = pen; best_delta = d; }
+    const int nsl = max(nslices, 1);
+    total_out = 0;
+    // own cell only: the unsigned query radius is 0 (A41:3623-3633)
+    int cx = 0, cy = 0, cz = 0;
+    if (active) { cx = cell_coord(p.x, D.tri_cell); cy = cell_coord(p.y, D.tri_cell); cz = cell_coord(p.z, D.tri_cell); }
+    unsigned cand = 0, rej = 0;
+    __shared__ unsigned long long s_best[kTile];
+    unsigned long long* best = s_best + (threadIdx.x & ~63u);
+    best[lane] = 0ull;
+    wave_lds_sync();
+    const int classes = max(D.tri_classes, 1);
+    int rounds_before = 0;   // rounds of the ranges searched so far (uniform)
+    for (int cls = 0; cls < classes; ++cls) {
+        // a range of layers at or above the query's holds nothing the
+        // lower-layers filter accepts (uniform: one cloth per wave)
+        if (classes > 1 && c.dyn_mode == 2 && D.tri_class_layer[cls] >= c.dyn_target_layer) continue;
+        int4 bin = make_int4(0, 0, 0, -2);
+        if (active) {
+            bin = gload(D.tri_bin + tri_bucket(classes, D.tri_buckets, D.tri_class_base[cls], D.tri_class_base[cls + 1], cx, cy, cz));
+            if (classes > 1) {
+                // one layer per range (its filter is the range skip above);
+                // the bin's third field is its entries' one cell tag
+                if (bin.w == -2 || (c.dyn_mode != 0 && c.dyn_target_src != 0 && bin.w == c.dyn_target_src) ||
+                    (bin.z != 0 && bin.z != cell_tag(cx, cy, cz)))
+                    bin.y = 0;
+            } else if (dyn_bucket_rejected(c, bin)) {
+                bin.y = 0;
             }
         }
-        if (count && cand) atomicAdd(&c.counts[kCountDynTriCandidates], static_cast<unsigned long long>(cand));
-        if (count && rej) atomicAdd(&c.counts[kCountDynTriAabbRejects], static_cast<unsigned long long>(rej));
-        if (active && best_pen > 0.0f) {
-            const F3 applied = mul(best_delta, 0.5f);
-            const F3 np = s.p + applied;
-            F3 v = s.vel();
-            const float vn = dot(v, applied);
-            if (vn < 0.0f) {
-                const float lsq = fmaxf(dot(applied, applied), kEps);
-                v = v - mul(applied, 0.5f * vn / lsq);
-                s.set_vel(v);
+        // The wave's probes as one list of (vertex, entry) items, 64 at a
+        // time: lane i owns items [start_i, start_i + n_i); item w belongs to
+        // the last lane whose start is at most w.
+        const int n_i = bin.y;
+        int incl = n_i;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        const int start = incl - n_i;
+        const int total = __shfl(incl, 63, 64);
+        total_out += total;
+        // this slice's first round of the range: rounds are dealt to the
+        // slices by their number counted across the ranges
+        const int r0 = ((slice - rounds_before) % nsl + nsl) % nsl;
+        rounds_before += (total + 63) / 64;
+        if (64 * r0 >= total) continue;   // uniform over the wave
+        const int first = bin.x - start;   // entry of item w is first + w in the owner's bucket
+        // owner lane of item w: the last lane whose start is at most w
+        auto owner_of = [&](int w) {
+            int own = 0;
+            for (int step = 32; step > 0; step >>= 1) {
+                const int probe = own + step;
+                const int sp = __shfl(start, min(probe, 63), 64);
+                if (probe < 64 && sp <= w) own = probe;
             }
-            s.p = np;
-            s.set_q(np - mul(v, fmaxf(dt, kEps)));
-            if (count) {
-                atomicAdd(&c.counts[kCountTrianglePairs], 1ull);
-                atomicAdd(&c.counts[kCountExactImpulse], 1ull);
+            return own;
+        };
+        auto entry_of = [&](int w, int own) {
+            const int k = __shfl(first, own, 64) + w;
+            return w < total ? gload(D.tri_ent + k) : make_int4(-1, 0, 0, 0);
+        };
+        // Items flow through three stages: a round's entry records are
+        // requested two rounds ahead and its triangles' bounds and corners
+        // one round ahead, so a round issues no load it waits for itself.
+        // The source filter (in the entry) runs before the bounds are read;
+        // it and the canonical-cell test (A41:3934-3982) are both side-effect
+        // free.
+        auto accepted = [&](int4 e) {
+            return e.x >= 0 && e.x / kDynMaxHashCells < nt && (!D.tsrc || dyn_enabled(c, e.y, e.z));
+        };
+        // a triangle's bounds and corners, requested together one round
+        // ahead (the corners whether or not the bounds pass)
+        struct TriRec {
+            TriBox bx;
+            float4 a, b, c;
+        };
+        auto record_of = [&](int4 e) {
+            TriRec r{};
+            if (accepted(e)) {
+                const int t = e.x / kDynMaxHashCells;
+                r.bx = gload(D.box + t);
+                r.a = gload(D.tv + 3 * t);
+                r.b = gload(D.tv + 3 * t + 1);
+                r.c = gload(D.tv + 3 * t + 2);
             }
+            return r;
+        };
+        const int stride = 64 * nsl;
+        const int b0 = 64 * r0;
+        int own = owner_of(b0 + lane);
+        int4 en = entry_of(b0 + lane, own);
+        int own1 = owner_of(b0 + stride + lane);
+        int4 en1 = entry_of(b0 + stride + lane, own1);
+        TriRec rc = record_of(en);
+        for (int base = b0; base < total; base += stride) {
+            const int w = base + lane;
+            const int ox = __shfl(cx, own, 64), oy = __shfl(cy, own, 64), oz = __shfl(cz, own, 64);
+            const F3 op = shfl3(p, own);
+            const int own2 = owner_of(w + 2 * stride);
+            const int4 en2 = entry_of(w + 2 * stride, own2);
+            const TriRec rc1 = record_of(en1);
+            if (w < total && accepted(en)) {
+                const int4 lo = box_cmin(rc.bx), hi = box_cmax(rc.bx);
+                if (ox >= lo.x && ox <= hi.x && oy >= lo.y && oy <= hi.y && oz >= lo.z && oz <= hi.z) {
+                    F3 d;
+                    const unsigned rej0 = rej;
+                    const float pen = dyn_triangle_depth_with(
+                        rc.bx,
+                        [&](F3& a, F3& b, F3& c) {
+                            a = f3(rc.a);
+                            b = f3(rc.b);
+                            c = f3(rc.c);
+                        },
+                        op, cm, cand, rej, d);
+                    if (count && classes > 1) {
+                        // counters of one table over the whole scene (the
+                        // contact is the same: every listing names the same
+                        // triangle)
+                        const unsigned m = static_cast<unsigned>(
+                            one_table_listings(lo, hi, ox, oy, oz, D.tri_buckets, classes, D.tri_class_base[cls], D.tri_class_base[cls + 1],
+                                               en.x % kDynMaxHashCells));
+                        cand += m - 1u;
+                        if (rej != rej0) rej += m - 1u;
+                    }
+                    const unsigned long long kk = depth_key(pen, static_cast<unsigned>(en.x));
+                    if (kk != 0ull) atomicMax(best + own, kk);
+                }
+            }
+            own = own1;
+            en = en1;
+            rc = rc1;
+            own1 = own2;
+            en1 = en2;
         }
     }
-    return work;
+    wave_lds_sync();
+    const unsigned long long top = best[lane];
+    if (count && cand) atomicAdd(&c.counts[kCountDynTriCandidates], static_cast<unsigned long long>(cand));
+    if (count && rej) atomicAdd(&c.counts[kCountDynTriAabbRejects], static_cast<unsigned long long>(rej));
+    return top;
+}
+
+// The triangle contact of a vertex (A41:3999-4323) from the maximum key of
+// its pooled search (`top`, 0: none): the winning entry evaluated again for
+// its push, then the large triangles (walked by every vertex), then the
+// response.  Without a triangle hash (small sets) the vertex walks every
+// triangle here instead.
+__device__ inline void dynamic_triangle_contact(const ClothView& c, VertexState& s, bool active, float dt, bool count,
+                                                unsigned long long top) {
+    const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
+    const float cm = fmaxf(c.collision_margin, 0.0f);
+    const int nt = D.nt;
+    if (!active || nt <= 0 || !(cm > 0.0f)) return;
+    float best_pen = 0.0f;
+    F3 best_delta = f3(0.0f, 0.0f, 0.0f);
+    unsigned cand = 0, rej = 0;
+    if (D.tri_hash) {
+        if (top != 0ull) {
+            // the winning entry again, for its push (same inputs, same result)
+            const int t = static_cast<int>(0xffffffffu - static_cast<unsigned>(top)) / kDynMaxHashCells;
+            unsigned dc = 0, dr = 0;
+            F3 d;
+            best_pen = dyn_triangle_depth(D, t, gload(D.box + t), s.p, cm, dc, dr, d);
+            best_delta = d;
+        }
+        const int nl = gload(D.nlarge);
+        for (int k = 0; k < nl; ++k) {
+            const int t = gload(D.large + k);
+            if (t < 0 || t >= nt) continue;
+            if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) continue;
+            F3 d;
+            const float pen = dyn_triangle_depth(D, t, D.box[t], s.p, cm, cand, rej, d);
+            if (pen > best_pen) { best_pen = pen; best_delta = d; }
+        }
+    } else {
+        for (int t = 0; t < nt; ++t) {
+            if (D.tsrc && !dyn_enabled(c, D.tsrc[t], D.tlayer ? D.tlayer[t] : 0)) continue;
+            F3 d;
+            const float pen = dyn_triangle_depth(D, t, D.box[t], s.p, cm, cand, rej, d);
+            if (pen > best_pen) { best_pen = pen; best_delta = d; }
+        }
+    }
+    if (count && cand) atomicAdd(&c.counts[kCountDynTriCandidates], static_cast<unsigned long long>(cand));
+    if (count && rej) atomicAdd(&c.counts[kCountDynTriAabbRejects], static_cast<unsigned long long>(rej));
+    if (best_pen > 0.0f) {
+        const F3 applied = mul(best_delta, 0.5f);
+        const F3 np = s.p + applied;
+        F3 v = s.vel();
+        const float vn = dot(v, applied);
+        if (vn < 0.0f) {
+            const float lsq = fmaxf(dot(applied, applied), kEps);
+            v = v - mul(applied, 0.5f * vn / lsq);
+            s.set_vel(v);
+        }
+        s.p = np;
+        s.set_q(np - mul(v, fmaxf(dt, kEps)));
+        if (count) {
+            atomicAdd(&c.counts[kCountTrianglePairs], 1ull);
+            atomicAdd(&c.counts[kCountExactImpulse], 1ull);
+        }
+    }
 }
 
 __device__ inline int dynamic_particle_contacts(const ClothView& c, VertexState& s, bool active, bool count, int pool_max) {
@@ -991,21 +1066,56 @@ __global__ __launch_bounds__(kTile) void k_integrate(const ClothView* __restrict
     c.vel[v] = with_w(s.vel_, 0.0f);
 }
 
-// Dynamic collider contacts of the last iteration (A41:9230-9252) in a pass
+// Dynamic collider contacts of the last iteration (A41:9230-9252), in two
+// launches: the pooled triangle searches (k_dynamic_triangles), then a pass
 // of their own (the hash probes want registers the constraint passes do
-// not): the triangle contacts, then the particle contacts on the updated
-// position, as the reference's two vertex-parallel launches do; only the
-// vertex's own state is read and written, so one launch equals two.
-// `tail_ops` are the iteration's closing vertex-local steps when nothing
-// else runs before them: the second pin projection and the velocity update,
-// exactly as the in-place vertex pass would apply them, for every vertex of
-// every cloth in the launch.
+// not): each vertex's triangle contact from the searches' result, then its
+// particle contacts on the updated position, as the reference's two
+// vertex-parallel launches do; only the vertex's own state is read and
+// written.  `tail_ops` are the iteration's closing vertex-local steps when
+// nothing else runs before them: the second pin projection and the velocity
+// update, exactly as the in-place vertex pass would apply them, for every
+// vertex of every cloth in the launch.
+//
+// The triangle searches of a wave can take tens of rounds where the cloths
+// pile up, and one launch lasts as long as its slowest wave.  So they run as
+// units of (tile, slice, slices): the unit's waves take every slices-th
+// round of their lists, starting at round `slice`, and fold their vertices'
+// maxima into dyn_best (a device atomic maximum per vertex with a contact).
+// Heavy tiles get up to kDynMaxSlices units, so their rounds spread over
+// that many workgroups.  The order kernel below deals the units out from the
+// step before's work.  The contact pass reads each vertex's key and clears it.
+__global__ __launch_bounds__(kTile) void k_dynamic_triangles(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
+                                                             int parity, unsigned ops, const int* __restrict__ units,
+                                                             const int* __restrict__ nunits, int* tri_work) {
+    if (static_cast<int>(blockIdx.x) >= *nunits) return;   // uniform over the block
+    const int u = units[blockIdx.x];
+    const int ti = u & kUnitTileMask, slice = (u >> kUnitSliceShift) & 15, nsl = ((u >> kUnitSlicesShift) & 15) + 1;
+    const Tile tile = tiles[ti];
+    const ClothView& c = views[tile.cloth];
+    if (c.dyn == nullptr || !c.dyn_best) return;
+    {
+        const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
+        if (!D.tri_hash || D.nt <= 0 || !(fmaxf(c.collision_margin, 0.0f) > 0.0f)) return;
+    }
+    const int v = tile.begin + static_cast<int>(threadIdx.x);
+    const float4* pos = parity ? c.pos1 : c.pos0;
+    const bool in_tile = v < tile.end;
+    const float4 p4 = in_tile ? gload(pos + v) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool active = in_tile && p4.w != 0.0f;
+    int total = 0;
+    const unsigned long long top = dynamic_triangle_pool(c, f3(p4), active, slice, nsl, (ops & kOpCount) != 0u, total);
+    if (active && top != 0ull) atomicMax(c.dyn_best + v, top);
+    // the tile's heaviest wave list, for the next step's units
+    if (slice == 0 && tri_work && (threadIdx.x & 63u) == 0u && total > 0) atomicMax(&tri_work[ti], total);
+}
+
 __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restrict__ views, const Tile* __restrict__ tiles,
                                                         int ntiles, int parity, unsigned ops, unsigned tail_ops,
                                                         const int* __restrict__ tile_order, int* tile_work) {
-    // with an order, the tiles that searched most in the step before come
-    // first: the launch lasts as long as its slowest waves, so those start at
-    // once instead of in a later round of waves
+    // with an order, the tiles whose particle searches took longest in the
+    // step before come first: the launch lasts as long as its slowest waves,
+    // so those start at once instead of in a later round of waves
     const int ti = tile_order ? tile_order[blockIdx.x] : xcd_tile(ntiles);
     const Tile tile = tiles[ti];
     const ClothView& c = views[tile.cloth];
@@ -1018,11 +1128,15 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     float4* pos = parity ? c.pos1 : c.pos0;
     const bool in_tile = v < tile.end;
     const float4 p4 = in_tile ? gload(pos + v) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    unsigned long long top = (has_dyn && in_tile && c.dyn_best) ? gload(c.dyn_best + v) : 0ull;
     const bool active = in_tile && p4.w != 0.0f;
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
     if (has_dyn) {
-        int work = dynamic_triangle_contacts(c, s, active, c.dt_sub, (ops & kOpCount) != 0u);
-        work += dynamic_particle_contacts(c, s, active, (ops & kOpCount) != 0u, static_cast<int>((ops >> kOpPoolShift) & 0x7fu));
+        if (top != 0ull) gstore(c.dyn_best + v, 0ull);   // ready for the next searches
+        if (!active) top = 0ull;
+        const bool count = (ops & kOpCount) != 0u;
+        dynamic_triangle_contact(c, s, active, c.dt_sub, count, top);
+        const int work = dynamic_particle_contacts(c, s, active, count, static_cast<int>((ops >> kOpPoolShift) & 0x7fu));
         if (tile_work && (threadIdx.x & 63u) == 0u && work > 0) atomicAdd(&tile_work[ti], work);
     }
     if (!in_tile) return;
@@ -1052,30 +1166,74 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     if (s.v_dirty) gstore(c.vel + v, with_w(s.vel_, 0.0f));
 }
 
-// The dynamic contact pass's tile order for the next step: tiles by the
-// work their waves did in the step before (tile_work, then cleared), most
-// first, counting-sorted on the bit length of the work.  One workgroup.
+// The next step's launch plan of the dynamic contacts, from the work of the
+// step before (then cleared), in one workgroup:
+//  - the contact pass's tile order: tiles by their particle-search work
+//    (work[0, n)), most first, counting-sorted on the bit length of the work;
+//  - the triangle search units: tiles by the longest item list of one of
+//    their waves (work[n, 2n)), most first, each tile with
+//    ceil(list / kDynSliceItems) slices (1 to kDynMaxSlices), extra slices
+//    granted down that order while the launch's n spare units last.
+// Layout of `order` (dynamic_order_ints): [0, n) the pass's order, [n, 2n)
+// the tiles sorted by triangle work, [2n] the unit count, [2n + 1, 4n + 1)
+// the units, tile | slice << kUnitSliceShift | (slices - 1) << kUnitSlicesShift.
 __global__ __launch_bounds__(1024) void k_order_tiles(int ntiles, int* work, int* order) {
     __shared__ int hist[33];
+    __shared__ int sums[1024];
     const int tid = static_cast<int>(threadIdx.x);
-    if (tid < 33) hist[tid] = 0;
-    __syncthreads();
     auto key = [](int w) { return 32 - (w > 0 ? 32 - __clz(w) : 0); };   // more work, smaller key
-    for (int t = tid; t < ntiles; t += 1024) atomicAdd(&hist[key(work[t])], 1);
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int k = 0; k < 33; ++k) {
-            const int n = hist[k];
-            hist[k] = run;
-            run += n;
+    auto sort = [&](const int* w, int* out) {
+        if (tid < 33) hist[tid] = 0;
+        __syncthreads();
+        for (int t = tid; t < ntiles; t += 1024) atomicAdd(&hist[key(w[t])], 1);
+        __syncthreads();
+        if (tid == 0) {
+            int run = 0;
+            for (int k = 0; k < 33; ++k) {
+                const int n = hist[k];
+                hist[k] = run;
+                run += n;
+            }
         }
-    }
+        __syncthreads();
+        for (int t = tid; t < ntiles; t += 1024) out[atomicAdd(&hist[key(w[t])], 1)] = t;
+        __syncthreads();
+    };
+    sort(work, order);
+    int* sorted = order + ntiles;
+    sort(work + ntiles, sorted);
+    // wanted extra slices of sorted position q: d_q; granted down the order
+    // while the spare units (ntiles) last, so position q's units start at
+    // q + min(E_q, ntiles) with E_q = d_0 + ... + d_{q-1}
+    const int chunk = (ntiles + 1023) / 1024;
+    const int lo = min(tid * chunk, ntiles), hi = min(lo + chunk, ntiles);
+    auto extra = [&](int q) {
+        const int w = work[ntiles + sorted[q]];
+        return w <= kDynSliceItems ? 0 : min((w - 1) / kDynSliceItems + 1, kDynMaxSlices) - 1;
+    };
+    int mine = 0;
+    for (int q = lo; q < hi; ++q) mine += extra(q);
+    sums[tid] = mine;
     __syncthreads();
-    for (int t = tid; t < ntiles; t += 1024) {
-        order[atomicAdd(&hist[key(work[t])], 1)] = t;
-        work[t] = 0;
+    for (int d = 1; d < 1024; d <<= 1) {   // inclusive scan of the per-thread sums
+        const int o = tid >= d ? sums[tid - d] : 0;
+        __syncthreads();
+        sums[tid] += o;
+        __syncthreads();
     }
+    int e = sums[tid] - mine;   // E at position lo
+    int* units = order + 2 * ntiles + 1;
+    for (int q = lo; q < hi; ++q) {
+        const int dq = extra(q);
+        const int first = q + min(e, ntiles);
+        const int slices = 1 + min(e + dq, ntiles) - min(e, ntiles);
+        for (int k = 0; k < slices; ++k)
+            units[first + k] = sorted[q] | (k << kUnitSliceShift) | ((slices - 1) << kUnitSlicesShift);
+        e += dq;
+    }
+    if (tid == 1023) order[2 * ntiles] = ntiles + min(sums[1023], ntiles);
+    __syncthreads();
+    for (int t = tid; t < 2 * ntiles; t += 1024) work[t] = 0;
 }
 
 // ------------------------------------------------- signed dynamic contacts
@@ -1864,11 +2022,20 @@ void launch_order_tiles(const LaunchCtx& c) {
 
 void launch_dynamic_pass(const LaunchCtx& c, int parity, unsigned ops, unsigned tail_ops, int pool_max, hipEvent_t begin,
                          hipEvent_t end) {
-    TimedLaunch t("dynamic_contacts", c.stream);
     const unsigned pops = ops | (static_cast<unsigned>(std::min(std::max(pool_max, 0), 64)) << kOpPoolShift);
     if (begin) hipEventRecord(begin, c.stream);
-    launch_k(k_dynamic_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, pops, tail_ops,
-             static_cast<const int*>(c.tile_order), c.tile_work);
+    {
+        // the units (launch_order_tiles) fit 2 * ntiles workgroups
+        TimedLaunch t("dynamic_triangles", c.stream);
+        const int* units = c.tile_order + 2 * c.ntiles + 1;
+        launch_k(k_dynamic_triangles, dim3(2 * c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, parity, ops, units,
+                 static_cast<const int*>(c.tile_order + 2 * c.ntiles), c.tile_work + c.ntiles);
+    }
+    {
+        TimedLaunch t("dynamic_contacts", c.stream);
+        launch_k(k_dynamic_pass, dim3(c.ntiles), dim3(kTile), 0, c.stream, c.views, c.tiles, c.ntiles, parity, pops, tail_ops,
+                 static_cast<const int*>(c.tile_order), c.tile_work);
+    }
     if (end) hipEventRecord(end, c.stream);
 }
 

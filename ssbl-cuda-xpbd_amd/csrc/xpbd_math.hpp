@@ -107,6 +107,8 @@ __device__ __forceinline__ int4 gload(const int4* p) {
     const gv4i x = *(const SSBL_GLOBAL gv4i*)(p);
     return make_int4(x.x, x.y, x.z, x.w);
 }
+__device__ __forceinline__ unsigned long long gload(const unsigned long long* p) { return *(const SSBL_GLOBAL unsigned long long*)(p); }
+__device__ __forceinline__ void gstore(unsigned long long* p, unsigned long long v) { *(SSBL_GLOBAL unsigned long long*)(p) = v; }
 __device__ __forceinline__ void gstore(float4* p, float4 v) {
     gv4f x;
     x.x = v.x; x.y = v.y; x.z = v.z; x.w = v.w;

@@ -481,6 +481,7 @@ struct Solver {
     Dev<float4> pos0, pos1, prev, vel, rest;
     Dev<int> adj_off, lra_off, tri_off, tri_inc, pin_off, pin_ids, bend_off;
     Dev<int> tri_order;   // the order a gathered scene's hash build visits this cloth's triangles
+    Dev<unsigned long long> dyn_best;   // per vertex: the triangle searches' deepest contact key (zero between launches)
     Dev<int2> adj, bends, ell;
     Dev<short> ell16;
     Dev<float> ell_rest;
@@ -913,10 +914,12 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
               rest_scale.upload(h_scale, "pressure scale allocation") && pin_off.upload(h_pin_off, "pin allocation") &&
               counts.alloc(kCountSlots, "diagnostic allocation") && max_smoothed.alloc(1, "diagnostic allocation") &&
               d_view.alloc(1, "view allocation") && d_tiles.upload(h_tiles, "tile allocation") &&
-              tile_work.alloc(std::max(ntiles, 1), "tile allocation") && tile_order.alloc(std::max(ntiles, 1), "tile allocation");
+              tile_work.alloc(dynamic_work_ints(std::max(ntiles, 1)), "tile allocation") &&
+              tile_order.alloc(dynamic_order_ints(std::max(ntiles, 1)), "tile allocation") && dyn_best.alloc(n, "dynamic contact allocation");
     if (!ok || !staging.reserve(sizeof(ClothView))) return false;
     if (!hip_ok(hipMemset(counts.p, 0, sizeof(unsigned long long) * kCountSlots), "diagnostic reset")) return false;
     if (!hip_ok(hipMemset(tile_work.p, 0, sizeof(int) * tile_work.n), "tile reset")) return false;
+    if (n > 0 && !hip_ok(hipMemset(dyn_best.p, 0, sizeof(unsigned long long) * dyn_best.n), "dynamic contact reset")) return false;
     if (!hip_ok(hipMemset(max_smoothed.p, 0, sizeof(float)), "diagnostic reset")) return false;
     if (cfg.self_collision && !self.prepare(cfg, edges, tri, n, counts.p)) return false;
     if (cfg.stretch_optimization_enabled && !pcg.prepare(cfg, edges, edge_rest, edge_colors, n, h_adj_off[n], ntiles, ell_width, ell_pitch,
@@ -1189,6 +1192,7 @@ void Solver::fill_view(int substeps) {
     const bool own_signed = dyn.is_signed() && (dyn.triangles() > 0 || dyn.particles() > 0);
     v.dyn = (dyn_on() && (use_scene() || !dyn.is_signed())) ? dyn_set().device_view() : nullptr;
     v.sdyn = own_signed ? dyn.device_view() : nullptr;
+    v.dyn_best = dyn_best.p;
     v.collider_dt = (std::isfinite(cfg.dt) && cfg.dt > 1.0e-7f) ? cfg.dt : std::max(v.dt_sub, 1.0e-7f);   // A41:4513-4518
     v.dyn_mode = use_scene() ? cross_mode : 0;
     v.dyn_target_src = target_src;
@@ -1606,8 +1610,9 @@ static bool build_tables(Batch& b) {
     for (auto& kv : by_cadence)
         if (!b.pcg_groups[g++].build(kv.second, kv.first)) return false;
     // grow-only: a batch given new members keeps its device tables and staging
-    const size_t nt = std::max<size_t>(tiles.size(), 1);
-    if (!b.tile_work.reserve(nt, "batch tile allocation") || !b.tile_order.reserve(nt, "batch tile allocation") ||
+    const int nt = std::max(static_cast<int>(tiles.size()), 1);
+    if (!b.tile_work.reserve(dynamic_work_ints(nt), "batch tile allocation") ||
+        !b.tile_order.reserve(dynamic_order_ints(nt), "batch tile allocation") ||
         !hip_ok(hipMemset(b.tile_work.p, 0, sizeof(int) * b.tile_work.n), "batch tile reset"))
         return false;
     return b.d_views.reserve(b.members.size(), "batch view allocation") && b.d_tiles.upload_grow(tiles, "batch tile allocation") &&

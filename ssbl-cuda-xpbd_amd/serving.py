@@ -94,8 +94,9 @@ def _next_pow2(v: int) -> int:
 def estimate_device_bytes(cloth: ClothBuildData, options: SolverOptions) -> int:
     """Upper bound of the device memory one handle holds, for admission.
 
-    Sums the library's allocations per element (float4 state x 5, edge
-    incidences at both endpoints plus the 8-slot table, triangles and their
+    Sums the library's allocations per element (float4 state x 5, the
+    dynamic contacts' 8-byte key, edge incidences at both endpoints plus the
+    8-slot table, triangles and their
     incidences, bends, pins, per-vertex pressure terms, the static SDF at its
     maximum resolution, the three self-collision hashes with their tables,
     and the PCG vectors), plus a per-allocation allowance. The admission
@@ -107,7 +108,7 @@ def estimate_device_bytes(cloth: ClothBuildData, options: SolverOptions) -> int:
     b = len(cloth.bends)
     lra = len(cloth.lra_edges)
     p = len(cloth.pin_indices)
-    core = (5 * 16 + 6 * 4 + 8 * 8 + 2 * 4) * v + 16 * e + 8 * e + (16 + 12) * t + (8 + 4 + 32 + 16) * b + 32 * lra + 24 * p
+    core = (5 * 16 + 6 * 4 + 8 * 8 + 2 * 4 + 8) * v + 16 * e + 8 * e + (16 + 12) * t + (8 + 4 + 32 + 16) * b + 32 * lra + 24 * p
     res = max(int(options.static_sdf_max_resolution), 2)
     sdf = 4 * res ** 3
     self_bytes = 0
