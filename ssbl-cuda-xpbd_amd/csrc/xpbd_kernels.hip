@@ -631,6 +631,30 @@ __device__ inline void dynamic_triangle_contact(const ClothView& c, VertexState&
     }
 }
 
+// Calls f(entry, k) for the first `lim` entries of a particle bucket at
+// `base`, in bucket order, loading them four at a time (an entry's cell,
+// source and layer, and position: the loads of a batch are independent, so
+// a batch costs one round trip instead of four).
+template <class F>
+__device__ __forceinline__ void for_bucket_entries(const SSBL_CONST DynView& D, int base, int lim, F f) {
+    for (int k0 = 0; k0 < lim; k0 += 4) {
+        int4 ck[4];
+        int2 sl[4];
+        float4 pp[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool in = k0 + u < lim;
+            const PartEntry* e = D.pent + base + min(k0 + u, lim - 1);
+            ck[u] = in ? gload(&e->ck) : make_int4(0, 0, 0, -1);
+            sl[u] = in ? gload(reinterpret_cast<const int2*>(&e->sl)) : make_int2(0, 0);
+            pp[u] = in ? gload(&e->p) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k0 + u < lim) f(ck[u], sl[u], pp[u], k0 + u);
+    }
+}
+
 __device__ inline int dynamic_particle_contacts(const ClothView& c, VertexState& s, bool active, bool count, int pool_max) {
     const SSBL_CONST DynView& D = *(const SSBL_CONST DynView*)(c.dyn);
     const int lane = static_cast<int>(__lane_id());
@@ -678,15 +702,14 @@ __device__ inline int dynamic_particle_contacts(const ClothView& c, VertexState&
                             if (dyn_bucket_rejected(c, bins[k9])) continue;
                             const int qx = cx + k9 % 3 - 1, qy = cy + k9 / 3 - 1, qz = cz + dz;
                             const int base = bins[k9].x, lim = min(bins[k9].y, kDynParticleSlots);
-                            for (int k = 0; k < lim; ++k) {
-                                const PartEntry e = gload(D.pent + base + k);
-                                if (e.ck.x != qx || e.ck.y != qy || e.ck.z != qz) continue;
-                                if (D.psrc && !dyn_enabled(c, e.sl.x, e.sl.y)) continue;
-                                if (e.ck.w < 0 || e.ck.w >= np_count) continue;
+                            for_bucket_entries(D, base, lim, [&](int4 ck, int2 sl, float4 pp, int) {
+                                if (ck.x != qx || ck.y != qy || ck.z != qz) return;
+                                if (D.psrc && !dyn_enabled(c, sl.x, sl.y)) return;
+                                if (ck.w < 0 || ck.w >= np_count) return;
                                 F3 d;
-                                const float pen = dyn_particle_depth_at(e.p, s.p, margin, cand, d);
+                                const float pen = dyn_particle_depth_at(pp, s.p, margin, cand, d);
                                 if (pen > best_pen) { best_pen = pen; best_delta = d; }
-                            }
+                            });
                         }
                     }
                 }
@@ -726,16 +749,15 @@ __device__ inline int dynamic_particle_contacts(const ClothView& c, VertexState&
                             if (!dyn_bucket_rejected(c, bin)) {
                                 unsigned long long key = 0ull;
                                 const int lim = min(bin.y, kDynParticleSlots);
-                                for (int k = 0; k < lim; ++k) {
-                                    const PartEntry e = gload(D.pent + bin.x + k);
-                                    if (e.ck.x != qx || e.ck.y != qy || e.ck.z != qz) continue;
-                                    if (D.psrc && !dyn_enabled(c, e.sl.x, e.sl.y)) continue;
-                                    if (e.ck.w < 0 || e.ck.w >= np_count) continue;
+                                for_bucket_entries(D, bin.x, lim, [&](int4 ck, int2 sl, float4 pp, int k) {
+                                    if (ck.x != qx || ck.y != qy || ck.z != qz) return;
+                                    if (D.psrc && !dyn_enabled(c, sl.x, sl.y)) return;
+                                    if (ck.w < 0 || ck.w >= np_count) return;
                                     F3 d;
-                                    const unsigned long long kk = depth_key(dyn_particle_depth_at(e.p, op, margin, cand, d),
+                                    const unsigned long long kk = depth_key(dyn_particle_depth_at(pp, op, margin, cand, d),
                                                                             static_cast<unsigned>(cell * kDynParticleSlots + k));
                                     key = kk > key ? kk : key;
-                                }
+                                });
                                 if (key != 0ull) atomicMax(best + own, key);
                             }
                         }
