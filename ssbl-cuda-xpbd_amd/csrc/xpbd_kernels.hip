@@ -688,28 +688,53 @@ __device__ inline int dynamic_particle_contacts(const ClothView& c, VertexState&
                 // many probes in this wave: each lane searches its own 27 cells
                 if (probe) {
                     const int cx = cell_coord(s.p.x, D.part_cell), cy = cell_coord(s.p.y, D.part_cell), cz = cell_coord(s.p.z, D.part_cell);
-                    // the bins of a z plane are loaded together, then visited in
-                    // the reference's order (z, then y, then x innermost)
+                    // the bins of a z plane are loaded together; their
+                    // entries are then taken as one list in the reference's
+                    // order (z, then y, then x innermost; bucket order
+                    // within a cell), loaded four at a time
                     for (int dz = -1; dz <= 1; ++dz) {
-                        int4 bins[9];
+                        int skip[9], pre[10];   // entry j of bin k is pent[j + skip[k]] for pre[k] <= j < pre[k + 1]
+                        pre[0] = 0;
 #pragma unroll
-                        for (int k = 0; k < 9; ++k)
-                            bins[k] = cell_in_reach(s.p, cx, cy, cz, k % 3 - 1, k / 3 - 1, dz, D.part_cell, reach)
-                                          ? gload(D.pbin + hash_cell(cx + k % 3 - 1, cy + k / 3 - 1, cz + dz, D.part_buckets))
-                                          : make_int4(0, 0, 0, -2);
+                        for (int k = 0; k < 9; ++k) {
+                            const int4 bin = cell_in_reach(s.p, cx, cy, cz, k % 3 - 1, k / 3 - 1, dz, D.part_cell, reach)
+                                                 ? gload(D.pbin + hash_cell(cx + k % 3 - 1, cy + k / 3 - 1, cz + dz, D.part_buckets))
+                                                 : make_int4(0, 0, 0, -2);
+                            const int n = dyn_bucket_rejected(c, bin) ? 0 : min(bin.y, kDynParticleSlots);
+                            skip[k] = bin.x - pre[k];
+                            pre[k + 1] = pre[k] + n;
+                        }
+                        const int n_plane = pre[9];
+                        for (int j0 = 0; j0 < n_plane; j0 += 4) {
+                            int4 ck[4];
+                            int2 sl[4];
+                            float4 pp[4];
+                            int kb[4];
 #pragma unroll
-                        for (int k9 = 0; k9 < 9; ++k9) {
-                            if (dyn_bucket_rejected(c, bins[k9])) continue;
-                            const int qx = cx + k9 % 3 - 1, qy = cy + k9 / 3 - 1, qz = cz + dz;
-                            const int base = bins[k9].x, lim = min(bins[k9].y, kDynParticleSlots);
-                            for_bucket_entries(D, base, lim, [&](int4 ck, int2 sl, float4 pp, int) {
-                                if (ck.x != qx || ck.y != qy || ck.z != qz) return;
-                                if (D.psrc && !dyn_enabled(c, sl.x, sl.y)) return;
-                                if (ck.w < 0 || ck.w >= np_count) return;
+                            for (int u = 0; u < 4; ++u) {
+                                const int j = min(j0 + u, n_plane - 1);
+                                int k = 0, at = j + skip[0];
+#pragma unroll
+                                for (int kk = 1; kk < 9; ++kk)
+                                    if (pre[kk] <= j) { k = kk; at = j + skip[kk]; }
+                                kb[u] = k;
+                                const PartEntry* e = D.pent + at;
+                                ck[u] = gload(&e->ck);
+                                sl[u] = gload(reinterpret_cast<const int2*>(&e->sl));
+                                pp[u] = gload(&e->p);
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                if (j0 + u >= n_plane) break;
+                                const int k9 = kb[u];
+                                const int qx = cx + k9 % 3 - 1, qy = cy + k9 / 3 - 1, qz = cz + dz;
+                                if (ck[u].x != qx || ck[u].y != qy || ck[u].z != qz) continue;
+                                if (D.psrc && !dyn_enabled(c, sl[u].x, sl[u].y)) continue;
+                                if (ck[u].w < 0 || ck[u].w >= np_count) continue;
                                 F3 d;
-                                const float pen = dyn_particle_depth_at(pp, s.p, margin, cand, d);
+                                const float pen = dyn_particle_depth_at(pp[u], s.p, margin, cand, d);
                                 if (pen > best_pen) { best_pen = pen; best_delta = d; }
-                            });
+                            }
                         }
                     }
                 }
