@@ -78,8 +78,9 @@ def kernel_bytes(name: str, scene, frames: int, entries: float, counters: dict) 
     """Algorithmic bytes of all launches of one timed scope over `frames`
     frames of the cross-cloth workload (scene rebuilt once per frame; the
     dynamic contact pass runs once per substep):
-      tri_hash_count   per triangle its 64-byte span record and 4-byte large
-                       flag; per bucket a 4-byte count written and read back;
+      tri_hash_count   per triangle its 64-byte span record, 4-byte large
+                       flag and 4-byte layer (its bucket range); per bucket a
+                       4-byte count written and read back;
       tri_hash_scatter per triangle the record, flag and 8-byte source/layer;
                        per entry (triangle, cell) a 16-byte record written;
                        per bucket its offset and count (12 bytes);
@@ -87,24 +88,30 @@ def kernel_bytes(name: str, scene, frames: int, entries: float, counters: dict) 
                        every entry's 16-byte record read;
       part_hash        per particle its 16-byte position, 4-byte source, layer,
                        cell (16) and 48-byte packed entry; per bucket 12 bytes;
+      dynamic_triangles per vertex and launch its position (16) and
+                       16-byte triangle bin; per candidate triangle entry its
+                       16-byte entry, 64-byte bounds record and 48-byte corners
       dynamic_contacts per vertex and launch position, previous position and
-                       velocity read and written (72), inverse mass (4) and
-                       its 16-byte triangle bin; per probed triangle entry 16
-                       bytes plus the 64-byte triangle record, per probed
+                       velocity read and written (72), its two contact keys
+                       (16) and coarse summary entry (8); per triangle contact
+                       the winner's record and corners again (112), per probed
                        particle entry 48 bytes (counts from the diagnostics of
                        the same frames)."""
     nt, npart, tb, pb = scene_hash_sizes(scene)
     if name == "tri_hash_count":
-        return frames * (nt * 68.0 + tb * 8.0)
+        return frames * (nt * 72.0 + tb * 8.0)
     if name == "tri_hash_scatter":
         return frames * (nt * 76.0 + entries * 16.0 + tb * 12.0)
     if name == "tri_bins":
         return frames * (tb * 20.0 + entries * 16.0)
     if name == "part_hash":
         return frames * (npart * 88.0 + pb * 12.0)
+    if name == "dynamic_triangles":
+        launches = frames * scene.substeps
+        return launches * scene.vertex_count * 32.0 + counters["dynamic_triangle_candidate_count"] * 128.0
     if name == "dynamic_contacts":
         launches = frames * scene.substeps
-        return (launches * scene.vertex_count * 92.0 + counters["dynamic_triangle_candidate_count"] * 80.0
+        return (launches * scene.vertex_count * 96.0 + counters["fast_triangle_pair_contacts"] * 112.0
                 + counters["dynamic_particle_candidate_count"] * 48.0)
     return None
 
@@ -474,7 +481,7 @@ def main():
         per = rec["total_ms"] / max(rec["launches"], 1)
         kernels[name] = {"launches": rec["launches"], "total_ms": rec["total_ms"], "avg_us": per * 1e3}
     roof_more = []
-    for name in ("tri_hash_count", "tri_hash_scatter", "tri_bins", "part_hash", "dynamic_contacts"):
+    for name in ("tri_hash_count", "tri_hash_scatter", "tri_bins", "part_hash", "dynamic_triangles", "dynamic_contacts"):
         if name not in kernels:
             continue
         b = kernel_bytes(name, scene, steps, counters["scene_entries"] / steps, counters)
