@@ -92,8 +92,8 @@ def kernel_bytes(name: str, scene, frames: int, entries: float, counters: dict) 
                        16-byte triangle bin; per candidate triangle entry its
                        16-byte entry, 64-byte bounds record and 48-byte corners
       dynamic_contacts per vertex and launch position, previous position and
-                       velocity read and written (72), its two contact keys
-                       (16) and coarse summary entry (8); per triangle contact
+                       velocity read and written (72), its triangle contact
+                       key (8) and coarse summary entry (8); per triangle contact
                        the winner's record and corners again (112), per probed
                        particle entry 48 bytes (counts from the diagnostics of
                        the same frames)."""
@@ -111,7 +111,7 @@ def kernel_bytes(name: str, scene, frames: int, entries: float, counters: dict) 
         return launches * scene.vertex_count * 32.0 + counters["dynamic_triangle_candidate_count"] * 128.0
     if name == "dynamic_contacts":
         launches = frames * scene.substeps
-        return (launches * scene.vertex_count * 96.0 + counters["fast_triangle_pair_contacts"] * 112.0
+        return (launches * scene.vertex_count * 88.0 + counters["fast_triangle_pair_contacts"] * 112.0
                 + counters["dynamic_particle_candidate_count"] * 48.0)
     return None
 
