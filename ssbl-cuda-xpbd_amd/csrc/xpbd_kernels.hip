@@ -206,21 +206,63 @@ __device__ __forceinline__ void sdf_clamped_corners(const ClothView& c, F3 p, Sd
     cell.corners(c, gx, gy, gz);
 }
 
-// Central difference of the clamped samples at p + d and p - d, both cells'
-// loads in flight together.
-__device__ __forceinline__ float sdf_difference(const ClothView& c, F3 p, F3 d, F3 md) {
-    SdfCell hi, lo;
-    sdf_clamped_corners(c, p + d, hi);
-    sdf_clamped_corners(c, p + md, lo);
-    return hi.blend() - lo.blend();
+// The corner loads of the clamped sample at p (A41:1693-1723).
+__device__ __forceinline__ void sdf_clamped_loads(const ClothView& c, F3 p, float (&k)[8]) {
+    SdfCell cell;
+    sdf_clamped_corners(c, p, cell);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) k[j] = cell.k[j];
+}
+
+// The clamped sample at p from its corners k: the cell and weights are
+// computed again, exactly as for the loads.
+__device__ __forceinline__ float sdf_clamped_blend(const ClothView& c, F3 p, const float (&k)[8]) {
+    const float inv = c.sdf_inv_clamped;
+    const float gx = fminf(fmaxf((p.x - c.sdf_origin[0]) * inv, 0.0f), c.sdf_top[0]);
+    const float gy = fminf(fmaxf((p.y - c.sdf_origin[1]) * inv, 0.0f), c.sdf_top[1]);
+    const float gz = fminf(fmaxf((p.z - c.sdf_origin[2]) * inv, 0.0f), c.sdf_top[2]);
+    SdfCell cell;
+    cell.tx = gx - static_cast<float>(min(max(static_cast<int>(floorf(gx)), 0), c.sdf_dim[0] - 1));
+    cell.ty = gy - static_cast<float>(min(max(static_cast<int>(floorf(gy)), 0), c.sdf_dim[1] - 1));
+    cell.tz = gz - static_cast<float>(min(max(static_cast<int>(floorf(gz)), 0), c.sdf_dim[2] - 1));
+#pragma unroll
+    for (int j = 0; j < 8; ++j) cell.k[j] = k[j];
+    return cell.blend();
+}
+
+// Hides x's value from the compiler, so that arithmetic on it is done again
+// rather than kept in registers from an earlier identical computation.
+__device__ __forceinline__ F3 opaque(F3 x) {
+    asm volatile("" : "+v"(x.x), "+v"(x.y), "+v"(x.z));
+    return x;
 }
 
 // A41:1771-1785, falling back to the nearest static triangle (A41:1749-1769).
-__device__ inline F3 sdf_normal(const ClothView& c, F3 p) {
+// The six clamped samples of the central differences load in two rounds:
+// the x and y pairs' 32 corners together, then the z pair's with the
+// x, y, z of the caller's `extra` element (returned in *extra_out).  The
+// weights are computed
+// after the loads return, so that only the corner values are held while
+// they are in flight.
+__device__ inline F3 sdf_normal(const ClothView& c, F3 p, const float4* extra, F3* extra_out) {
     const float h = fmaxf(c.sdf_voxel, 1.0e-5f);
-    const float dx = sdf_difference(c, p, f3(h, 0.0f, 0.0f), f3(-h, 0.0f, 0.0f));
-    const float dy = sdf_difference(c, p, f3(0.0f, h, 0.0f), f3(0.0f, -h, 0.0f));
-    const float dz = sdf_difference(c, p, f3(0.0f, 0.0f, h), f3(0.0f, 0.0f, -h));
+    float kxh[8], kxl[8], kyh[8], kyl[8];
+    sdf_clamped_loads(c, p + f3(h, 0.0f, 0.0f), kxh);
+    sdf_clamped_loads(c, p + f3(-h, 0.0f, 0.0f), kxl);
+    sdf_clamped_loads(c, p + f3(0.0f, h, 0.0f), kyh);
+    sdf_clamped_loads(c, p + f3(0.0f, -h, 0.0f), kyl);
+    __builtin_amdgcn_sched_barrier(0);
+    const F3 pa = opaque(p);
+    const float dx = sdf_clamped_blend(c, pa + f3(h, 0.0f, 0.0f), kxh) - sdf_clamped_blend(c, pa + f3(-h, 0.0f, 0.0f), kxl);
+    const float dy = sdf_clamped_blend(c, pa + f3(0.0f, h, 0.0f), kyh) - sdf_clamped_blend(c, pa + f3(0.0f, -h, 0.0f), kyl);
+    __builtin_amdgcn_sched_barrier(0);
+    float kzh[8], kzl[8];
+    sdf_clamped_loads(c, pa + f3(0.0f, 0.0f, h), kzh);
+    sdf_clamped_loads(c, pa + f3(0.0f, 0.0f, -h), kzl);
+    *extra_out = gload_xyz(extra);
+    __builtin_amdgcn_sched_barrier(0);
+    const F3 pb = opaque(pa);
+    const float dz = sdf_clamped_blend(c, pb + f3(0.0f, 0.0f, h), kzh) - sdf_clamped_blend(c, pb + f3(0.0f, 0.0f, -h), kzl);
     const F3 n = f3(dx, dy, dz);
     const float lsq = dot(n, n);
     if (isfinite(lsq) && lsq > 1.0e-12f) return mul(n, rsqrt_exact(lsq));
@@ -245,12 +287,14 @@ __device__ inline void sdf_contact(const ClothView& c, VertexState& s, float dt,
     if (margin <= 0.0f) return;
     float d = 0.0f;
     if (!sdf_sample(c, s.p, &d) || d >= margin) return;
-    const F3 n = sdf_normal(c, s.p);
+    // the velocity load goes out with the normal's last round
+    F3 vraw;
+    const F3 n = sdf_normal(c, s.p, c.vel + s.v, &vraw);
+    F3 v = s.v_have ? s.vel_ : vraw;
     if (!finite3(n)) return;
     const float pen = margin - d;
     if (!isfinite(pen) || pen <= 0.0f) return;
     const F3 corrected = s.p + mul(n, pen);
-    F3 v = s.vel();
     const float inward = fmaxf(-dot(v, n), 0.0f);
     contact_response(c, n, inward + pen / fmaxf(dt, kEps), v, count);
     s.p = corrected;
