@@ -75,9 +75,9 @@ struct VertexState {
     __device__ void set_vel(F3 x) { vel_ = x; v_have = true; v_dirty = true; }
 };
 
-// A41:3466-3509: every pin of vertex v in ascending pin order.
-__device__ inline void apply_pins(const ClothView& c, VertexState& s, bool pinned, float dt) {
-    const int b = c.pin_off[s.v], e = c.pin_off[s.v + 1];
+// A41:3466-3509: every pin of vertex v in ascending pin order, pins b .. e
+// of the vertex-ordered pin list (pin_off[v], pin_off[v + 1]).
+__device__ inline void apply_pins(const ClothView& c, VertexState& s, bool pinned, float dt, int b, int e) {
     for (int k = b; k < e; ++k) {
         const float4 pin = c.pins[c.pin_ids[k]];
         const float relax = pin.w;
@@ -96,6 +96,17 @@ __device__ inline void apply_pins(const ClothView& c, VertexState& s, bool pinne
         s.set_q(corrected - mul(damped, fmaxf(dt, kEps)));
         s.set_vel(damped);
     }
+}
+__device__ inline void apply_pins(const ClothView& c, VertexState& s, bool pinned, float dt) {
+    apply_pins(c, s, pinned, dt, c.pin_off[s.v], c.pin_off[s.v + 1]);
+}
+
+// The vertex's pin range, requested with its other loads at the start of a
+// pass whose `ops` apply pins ({0, 0} otherwise).
+__device__ __forceinline__ int2 pin_range(const ClothView& c, int v, unsigned ops) {
+    if (!(ops & (kOpPin | kOpPinPolish | kOpIntegratePin))) return make_int2(0, 0);
+    const int* off = c.pin_off;
+    return make_int2(gload(off + v), gload(off + v + 1));
 }
 
 // A41:3511-3590
@@ -1222,6 +1233,15 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     unsigned long long top = (has_dyn && in_tile && c.dyn_best) ? gload(c.dyn_best + v) : 0ull;
     const bool active = in_tile && p4.w != 0.0f;
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
+    // the closing steps' reads (pin range, previous position, velocity) go
+    // out with the vertex's first loads instead of after the searches
+    int2 pins = make_int2(0, 0);
+    if (tail && in_tile) {
+        pins = pin_range(c, v, tail_ops);
+        s.q_ = gload_xyz(c.prev + v);
+        s.vel_ = gload_xyz(c.vel + v);
+        s.q_have = s.v_have = true;
+    }
     if (has_dyn) {
         if (top != 0ull) gstore(c.dyn_best + v, 0ull);   // ready for the next searches
         if (!active) top = 0ull;
@@ -1234,7 +1254,7 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     if (tail) {
         const float dt = c.dt_sub;
         const bool pinned = p4.w == 0.0f;
-        if ((tail_ops & kOpPinPolish) && ((tail_ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt);
+        if ((tail_ops & kOpPinPolish) && ((tail_ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt, pins.x, pins.y);
         if (pinned) s.set_vel(f3(0.0f, 0.0f, 0.0f));
         else velocity_update(c, s, dt);
         if (tail_ops & kOpIntegrate) {
@@ -1243,7 +1263,7 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
             // buffer, the integrated one goes to the other
             gstore(pos + v, with_w(s.p, p4.w));
             integrate_plain(c, s, pinned, s.vel_, dt);
-            if (tail_ops & kOpIntegratePin) apply_pins(c, s, pinned, dt);
+            if (tail_ops & kOpIntegratePin) apply_pins(c, s, pinned, dt, pins.x, pins.y);
             gstore((parity ? c.pos0 : c.pos1) + v, with_w(s.p, p4.w));
             gstore(c.prev + v, with_w(s.q_, 0.0f));
             gstore(c.vel + v, with_w(s.vel_, 0.0f));
@@ -1816,13 +1836,13 @@ enum GatherKind { kGatherNone = 0, kGatherStretch = 1, kGatherLra = 2, kGatherBe
 // The vertex-local steps of a pass after its constraint gather, then the
 // vertex's writes.
 __device__ __forceinline__ void vertex_tail(const ClothView& c, VertexState& s, float w, bool pinned, float dt, float4* dst,
-                                            unsigned ops, bool count) {
-    if (ops & kOpPin) apply_pins(c, s, pinned, dt);
+                                            unsigned ops, bool count, int2 pins) {
+    if (ops & kOpPin) apply_pins(c, s, pinned, dt, pins.x, pins.y);
     if ((ops & kOpAnalytic) && !pinned) analytic_contacts(c, s, dt, count);
     if ((ops & kOpSdf) && c.sdf_ready && !pinned) sdf_contact(c, s, dt, count);
     // A41:9360-9369: the second pin projection runs after any contact work,
     // and on every last iteration (dynamic collision is scheduled there).
-    if ((ops & kOpPinPolish) && ((ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt);
+    if ((ops & kOpPinPolish) && ((ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt, pins.x, pins.y);
     if (ops & kOpVelocity) {
         if (pinned) s.set_vel(f3(0.0f, 0.0f, 0.0f));
         else velocity_update(c, s, dt);
@@ -1847,6 +1867,7 @@ __device__ inline void vertex_pass_staged(const ClothView& c, const Tile& tile, 
     stage_positions(src, lo, hi - lo, stage);
     const int vl = live ? v : tile.begin;
     const F3 q = gload_xyz(c.prev + vl);
+    const int2 pins = pin_range(c, vl, ops);
     int2 ell[kEllMax];
     ell_load(c, vl, ell);
     __syncthreads();
@@ -1858,7 +1879,7 @@ __device__ inline void vertex_pass_staged(const ClothView& c, const Tile& tile, 
     const bool pinned = w == 0.0f;
     VertexState s{&c, v, f3(p4), q, f3(0.0f, 0.0f, 0.0f), true, false, false, false};
     stretch_gather(c, StagedSource{stage, lo}, src, s, w, ell, count);
-    vertex_tail(c, s, w, pinned, dt, dst, ops, count);
+    vertex_tail(c, s, w, pinned, dt, dst, ops, count, pins);
 }
 
 // One solver iteration for every vertex of the tile: the constraint gather
@@ -1889,6 +1910,7 @@ __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __res
     int2 ell[kEllMax];
     F3 q = f3(0.0f, 0.0f, 0.0f);
     const float4 p4 = gload(src + v);
+    const int2 pins = pin_range(c, v, ops);
     if constexpr (kGather == kGatherStretch) {
         gather = c.direct_stretch != 0;
         if (gather) {
@@ -1917,7 +1939,7 @@ __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __res
     } else if constexpr (kGather == kGatherBend) {
         bend_gather(c, src, s, w, count);
     }
-    vertex_tail(c, s, w, pinned, dt, dst, ops, count);
+    vertex_tail(c, s, w, pinned, dt, dst, ops, count, pins);
 }
 
 // ------------------------------------------------------ coloured stretch
