@@ -121,8 +121,8 @@ struct ClothView {
     const int* bend_off;       // V+1, uncoloured bends only
     const int4* bend_adj;      // {other | first bit, rest bits, bend index, 0}
     const int* pin_off;        // V+1 -> pins of that vertex (ascending pin index)
-    const int* pin_ids;
-    const float4* pins;        // {target xyz, relaxation}; relaxation 2 = hard, <=0 ignored
+    const int* pin_ids;        // pin index of each vertex-ordered record
+    const float4* pins;        // vertex-ordered {target xyz, relaxation}; relaxation 2 = hard, <=0 ignored
     const float* area_share;
     const float* rest_scale;
     const SsblXpbdForceField* fields;

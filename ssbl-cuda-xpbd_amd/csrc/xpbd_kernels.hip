@@ -75,11 +75,12 @@ struct VertexState {
     __device__ void set_vel(F3 x) { vel_ = x; v_have = true; v_dirty = true; }
 };
 
-// A41:3466-3509: every pin of vertex v in ascending pin order, pins b .. e
-// of the vertex-ordered pin list (pin_off[v], pin_off[v + 1]).
-__device__ inline void apply_pins(const ClothView& c, VertexState& s, bool pinned, float dt, int b, int e) {
+// A41:3466-3509: every pin of vertex v in ascending pin order, records
+// b .. e of the vertex-ordered pin records (pin_off[v], pin_off[v + 1]);
+// `first` is record b, already loaded.
+__device__ inline void apply_pins(const ClothView& c, VertexState& s, bool pinned, float dt, int b, int e, float4 first) {
     for (int k = b; k < e; ++k) {
-        const float4 pin = c.pins[c.pin_ids[k]];
+        const float4 pin = k == b ? first : gload(c.pins + k);
         const float relax = pin.w;
         if (relax < 0.0f) continue;                       // weight < 0.05 or not finite
         const F3 target = f3(pin.x, pin.y, pin.z);
@@ -98,16 +99,52 @@ __device__ inline void apply_pins(const ClothView& c, VertexState& s, bool pinne
     }
 }
 __device__ inline void apply_pins(const ClothView& c, VertexState& s, bool pinned, float dt) {
-    apply_pins(c, s, pinned, dt, c.pin_off[s.v], c.pin_off[s.v + 1]);
+    const int b = c.pin_off[s.v], e = c.pin_off[s.v + 1];
+    apply_pins(c, s, pinned, dt, b, e, b < e ? gload(c.pins + b) : make_float4(0.0f, 0.0f, 0.0f, -1.0f));
 }
 
-// The vertex's pin range, requested with its other loads at the start of a
-// pass whose `ops` apply pins ({0, 0} otherwise).
-__device__ __forceinline__ int2 pin_range(const ClothView& c, int v, unsigned ops) {
-    if (!(ops & (kOpPin | kOpPinPolish | kOpIntegratePin))) return make_int2(0, 0);
-    const int* off = c.pin_off;
-    return make_int2(gload(off + v), gload(off + v + 1));
-}
+// A vertex's pins as a pass applies them: the range, requested with the
+// vertex's other loads at the start of a pass whose `ops` apply pins
+// ({0, 0} otherwise).  The contact pass also requests the first record with
+// them (request_first); the vertex passes load it where it is applied (held
+// through their gather and contact work, it would not fit their register
+// budget).
+struct VertexPins {
+    int b, e;
+    float4 first;
+    bool have_first;
+    __device__ __forceinline__ void range(const ClothView& c, int v, unsigned ops) {
+        have_first = false;
+        if (!(ops & (kOpPin | kOpPinPolish | kOpIntegratePin))) { b = e = 0; return; }
+        const int* off = c.pin_off;
+        b = gload(off + v);
+        e = gload(off + v + 1);
+    }
+    __device__ __forceinline__ void request_first(const ClothView& c) {
+        first = b < e ? gload(c.pins + b) : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        have_first = true;
+    }
+    __device__ __forceinline__ void apply(const ClothView& c, VertexState& s, bool pinned, float dt) const {
+        if (b >= e) return;
+        apply_pins(c, s, pinned, dt, b, e, have_first ? first : gload(c.pins + b));
+    }
+    // The pin projection after a pass's contact work, for a vertex whose
+    // range was not empty (`had`): the range is read again rather than held
+    // through the contact work, whose registers it would need.  After a
+    // first application in the same pass (`again`) a vertex the reference
+    // flags as pinned is skipped: every pin of such a vertex sets position,
+    // previous position and velocity from its record alone, and the contact
+    // steps between the two applications skip pinned vertices, so the
+    // second application would write what the first wrote.
+    __device__ __forceinline__ void apply_after_contacts(const ClothView& c, VertexState& s, bool pinned, float dt, bool had,
+                                                         bool again) const {
+        if (!had || (again && pinned)) return;
+        int v = s.v;
+        asm volatile("" : "+v"(v));   // a load made again, not the first one kept
+        const int rb = gload(c.pin_off + v), re = gload(c.pin_off + v + 1);
+        apply_pins(c, s, pinned, dt, rb, re, gload(c.pins + rb));
+    }
+};
 
 // A41:3511-3590
 __device__ inline void analytic_contacts(const ClothView& c, VertexState& s, float dt, bool count) {
@@ -298,12 +335,15 @@ __device__ inline void sdf_contact(const ClothView& c, VertexState& s, float dt,
     if (margin <= 0.0f) return;
     float d = 0.0f;
     if (!sdf_sample(c, s.p, &d) || d >= margin) return;
+    // (the depth is formed before the normal, and kept from being moved
+    // after it: one register held through its loads instead of two)
+    float pen = margin - d;
+    asm volatile("" : "+v"(pen));
     // the velocity load goes out with the normal's last round
     F3 vraw;
     const F3 n = sdf_normal(c, s.p, c.vel + s.v, &vraw);
     F3 v = s.v_have ? s.vel_ : vraw;
     if (!finite3(n)) return;
-    const float pen = margin - d;
     if (!isfinite(pen) || pen <= 0.0f) return;
     const F3 corrected = s.p + mul(n, pen);
     const float inward = fmaxf(-dot(v, n), 0.0f);
@@ -1235,12 +1275,14 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     VertexState s{&c, v, f3(p4), f3(0.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 0.0f), false, false, false, false};
     // the closing steps' reads (pin range, previous position, velocity) go
     // out with the vertex's first loads instead of after the searches
-    int2 pins = make_int2(0, 0);
+    VertexPins pins;
+    pins.range(c, v, 0u);
     if (tail && in_tile) {
-        pins = pin_range(c, v, tail_ops);
+        pins.range(c, v, tail_ops);
         s.q_ = gload_xyz(c.prev + v);
         s.vel_ = gload_xyz(c.vel + v);
         s.q_have = s.v_have = true;
+        pins.request_first(c);
     }
     if (has_dyn) {
         if (top != 0ull) gstore(c.dyn_best + v, 0ull);   // ready for the next searches
@@ -1254,7 +1296,7 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
     if (tail) {
         const float dt = c.dt_sub;
         const bool pinned = p4.w == 0.0f;
-        if ((tail_ops & kOpPinPolish) && ((tail_ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt, pins.x, pins.y);
+        if ((tail_ops & kOpPinPolish) && ((tail_ops & kOpLastIter) || c.contacts_on)) pins.apply(c, s, pinned, dt);
         if (pinned) s.set_vel(f3(0.0f, 0.0f, 0.0f));
         else velocity_update(c, s, dt);
         if (tail_ops & kOpIntegrate) {
@@ -1263,7 +1305,7 @@ __global__ __launch_bounds__(kTile) void k_dynamic_pass(const ClothView* __restr
             // buffer, the integrated one goes to the other
             gstore(pos + v, with_w(s.p, p4.w));
             integrate_plain(c, s, pinned, s.vel_, dt);
-            if (tail_ops & kOpIntegratePin) apply_pins(c, s, pinned, dt, pins.x, pins.y);
+            if (tail_ops & kOpIntegratePin) pins.apply(c, s, pinned, dt);
             gstore((parity ? c.pos0 : c.pos1) + v, with_w(s.p, p4.w));
             gstore(c.prev + v, with_w(s.q_, 0.0f));
             gstore(c.vel + v, with_w(s.vel_, 0.0f));
@@ -1836,13 +1878,15 @@ enum GatherKind { kGatherNone = 0, kGatherStretch = 1, kGatherLra = 2, kGatherBe
 // The vertex-local steps of a pass after its constraint gather, then the
 // vertex's writes.
 __device__ __forceinline__ void vertex_tail(const ClothView& c, VertexState& s, float w, bool pinned, float dt, float4* dst,
-                                            unsigned ops, bool count, int2 pins) {
-    if (ops & kOpPin) apply_pins(c, s, pinned, dt, pins.x, pins.y);
+                                            unsigned ops, bool count, const VertexPins& pins) {
+    const bool had_pins = pins.b < pins.e;
+    if (ops & kOpPin) pins.apply(c, s, pinned, dt);
     if ((ops & kOpAnalytic) && !pinned) analytic_contacts(c, s, dt, count);
     if ((ops & kOpSdf) && c.sdf_ready && !pinned) sdf_contact(c, s, dt, count);
     // A41:9360-9369: the second pin projection runs after any contact work,
     // and on every last iteration (dynamic collision is scheduled there).
-    if ((ops & kOpPinPolish) && ((ops & kOpLastIter) || c.contacts_on)) apply_pins(c, s, pinned, dt, pins.x, pins.y);
+    if ((ops & kOpPinPolish) && ((ops & kOpLastIter) || c.contacts_on))
+        pins.apply_after_contacts(c, s, pinned, dt, had_pins, (ops & kOpPin) != 0u);
     if (ops & kOpVelocity) {
         if (pinned) s.set_vel(f3(0.0f, 0.0f, 0.0f));
         else velocity_update(c, s, dt);
@@ -1867,7 +1911,8 @@ __device__ inline void vertex_pass_staged(const ClothView& c, const Tile& tile, 
     stage_positions(src, lo, hi - lo, stage);
     const int vl = live ? v : tile.begin;
     const F3 q = gload_xyz(c.prev + vl);
-    const int2 pins = pin_range(c, vl, ops);
+    VertexPins pins;
+    pins.range(c, vl, ops);
     int2 ell[kEllMax];
     ell_load(c, vl, ell);
     __syncthreads();
@@ -1910,7 +1955,8 @@ __global__ __launch_bounds__(kTile, 8) void k_vertex_pass(const ClothView* __res
     int2 ell[kEllMax];
     F3 q = f3(0.0f, 0.0f, 0.0f);
     const float4 p4 = gload(src + v);
-    const int2 pins = pin_range(c, v, ops);
+    VertexPins pins;
+    pins.range(c, v, ops);
     if constexpr (kGather == kGatherStretch) {
         gather = c.direct_stretch != 0;
         if (gather) {

@@ -1070,10 +1070,10 @@ bool Solver::upload_pins(const int* idx, const float* p, const float* w, int cou
     // iteration count changes the relaxation) allocates nothing
     size_t in_range = 0;
     for (const HostPin& pin : pins) in_range += (pin.index >= 0 && pin.index < nv) ? 1 : 0;
-    if (pin_off.n == static_cast<size_t>(nv) + 1 && pin_ids.n == in_range && d_pins.n == pins.size()) return true;
+    if (pin_off.n == static_cast<size_t>(nv) + 1 && pin_ids.n == in_range && d_pins.n == in_range) return true;
     if (!sync_stream()) return false;
     return pin_off.alloc(static_cast<size_t>(nv) + 1, "pin allocation") && pin_ids.alloc(in_range, "pin allocation") &&
-           d_pins.alloc(pins.size(), "pin allocation");
+           d_pins.alloc(in_range, "pin allocation");
 }
 
 bool Solver::refresh_pins(int iterations) {
@@ -1104,8 +1104,12 @@ bool Solver::refresh_pins(int iterations) {
     std::vector<int> cur(off.begin(), off.end() - 1);
     for (size_t k = 0; k < pins.size(); ++k)
         if (pins[k].index >= 0 && pins[k].index < nv) ids[cur[pins[k].index]++] = static_cast<int>(k);
+    // the device records in the same vertex order as ids, so a vertex's pins
+    // are records pin_off[v] .. pin_off[v + 1] with no index between
+    std::vector<float4> by_vertex(ids.size());
+    for (size_t j = 0; j < ids.size(); ++j) by_vertex[j] = h[static_cast<size_t>(ids[j])];
     if (!sync_stream()) return false;
-    if (!pin_off.upload(off, "pin upload") || !pin_ids.upload(ids, "pin upload") || !d_pins.upload(h, "pin upload")) return false;
+    if (!pin_off.upload(off, "pin upload") || !pin_ids.upload(ids, "pin upload") || !d_pins.upload(by_vertex, "pin upload")) return false;
     pins_iterations = iterations;
     return true;
 }
