@@ -64,12 +64,12 @@ struct VertexState {
     F3 q_, vel_;
     bool q_have, q_dirty, v_have, v_dirty;
     __device__ F3 q() {
-        if (!q_have) { q_ = f3(gload(c->prev + v)); q_have = true; }
+        if (!q_have) { q_ = gload_xyz(c->prev + v); q_have = true; }
         return q_;
     }
     __device__ void set_q(F3 x) { q_ = x; q_have = true; q_dirty = true; }
     __device__ F3 vel() {
-        if (!v_have) { vel_ = f3(gload(c->vel + v)); v_have = true; }
+        if (!v_have) { vel_ = gload_xyz(c->vel + v); v_have = true; }
         return vel_;
     }
     __device__ void set_vel(F3 x) { vel_ = x; v_have = true; v_dirty = true; }
@@ -486,25 +486,20 @@ __device__ __forceinline__ F3 shfl3(F3 a, int lane) { return f3(__shfl(a.x, lane
 // per-lane walks are already balanced.  The launch takes the limit
 // (kDefaultPoolMax unless tuned); every value gives the same contacts.
 
-// The diagnostic weight of one listing of a triangle in a layer range's
-// bucket, for counters equal to those of one table over the whole scene:
-// that table (hash_cell over `buckets`) lists the triangle once per cell of
-// its span [lo, hi] hashed to the query cell (x, y, z)'s bucket, and
-// A41:3934-3982 takes each listing as a candidate.  The range's bucket
-// (range [rlo, rhi)) may list it a different number of times; the first of
-// those listings (span cell `idx` in key order) carries the one-table count
-// and the others none.  Spans of hashed triangles hold at most 64 cells.
-__device__ inline int one_table_listings(int4 lo, int4 hi, int x, int y, int z, int buckets, int classes, int rlo, int rhi, int idx) {
+// The diagnostic weight of a triangle found by a query, for counters equal
+// to those of one table over the whole scene: that table (hash_cell over
+// `buckets`) lists the triangle once per cell of its span [lo, hi] hashed to
+// the query cell (x, y, z)'s bucket, and A41:3934-3982 takes each listing as
+// a candidate.  The search evaluates only the listing under the query's own
+// cell, which carries this count.  Spans of hashed triangles hold at most 64
+// cells.
+__device__ inline int one_table_listings(int4 lo, int4 hi, int x, int y, int z, int buckets) {
     const unsigned h = hash_cell(x, y, z, buckets);
-    const int rb = tri_bucket(classes, buckets, rlo, rhi, x, y, z);
-    int m = 0, first = -1, k = 0;
+    int m = 0;
     for (int qz = lo.z; qz <= hi.z; ++qz)
         for (int qy = lo.y; qy <= hi.y; ++qy)
-            for (int qx = lo.x; qx <= hi.x; ++qx, ++k) {
-                m += hash_cell(qx, qy, qz, buckets) == h ? 1 : 0;
-                if (first < 0 && tri_bucket(classes, buckets, rlo, rhi, qx, qy, qz) == rb) first = k;
-            }
-    return idx == first ? m : 0;
+            for (int qx = lo.x; qx <= hi.x; ++qx) m += hash_cell(qx, qy, qz, buckets) == h ? 1 : 0;
+    return m;
 }
 
 // The pooled triangle search of one wave (vertices at p; lanes without a
@@ -529,6 +524,7 @@ __device__ inline unsigned long long dynamic_triangle_pool(const ClothView& c, F
     int cx = 0, cy = 0, cz = 0;
     if (active) { cx = cell_coord(p.x, D.tri_cell); cy = cell_coord(p.y, D.tri_cell); cz = cell_coord(p.z, D.tri_cell); }
     unsigned cand = 0, rej = 0;
+    const int qtag = active ? cell_tag(cx, cy, cz) : 0;
     __shared__ unsigned long long s_best[kTile];
     unsigned long long* best = s_best + (threadIdx.x & ~63u);
     best[lane] = 0ull;
@@ -599,9 +595,12 @@ __device__ inline unsigned long long dynamic_triangle_pool(const ClothView& c, F
             TriBox bx;
             float4 a, b, c;
         };
-        auto record_of = [&](int4 e) {
+        // only an entry listed under the query's own cell (its tag first) can
+        // be the query's listing of its triangle: the others are skipped
+        // before their records are requested
+        auto record_of = [&](int4 e, int tag) {
             TriRec r{};
-            if (accepted(e)) {
+            if (accepted(e) && e.w == tag) {
                 const int t = e.x / kDynMaxHashCells;
                 r.bx = gload(D.box + t);
                 r.a = gload(D.tv + 3 * t);
@@ -616,17 +615,23 @@ __device__ inline unsigned long long dynamic_triangle_pool(const ClothView& c, F
         int4 en = entry_of(b0 + lane, own);
         int own1 = owner_of(b0 + stride + lane);
         int4 en1 = entry_of(b0 + stride + lane, own1);
-        TriRec rc = record_of(en);
+        TriRec rc = record_of(en, __shfl(qtag, own, 64));
         for (int base = b0; base < total; base += stride) {
             const int w = base + lane;
             const int ox = __shfl(cx, own, 64), oy = __shfl(cy, own, 64), oz = __shfl(cz, own, 64);
             const F3 op = shfl3(p, own);
             const int own2 = owner_of(w + 2 * stride);
             const int4 en2 = entry_of(w + 2 * stride, own2);
-            const TriRec rc1 = record_of(en1);
-            if (w < total && accepted(en)) {
+            const TriRec rc1 = record_of(en1, __shfl(qtag, own1, 64));
+            if (w < total && accepted(en) && en.w == cell_tag(ox, oy, oz)) {
                 const int4 lo = box_cmin(rc.bx), hi = box_cmax(rc.bx);
-                if (ox >= lo.x && ox <= hi.x && oy >= lo.y && oy <= hi.y && oz >= lo.z && oz <= hi.z) {
+                // A triangle whose span holds the query's cell is listed under
+                // that cell, in this bucket: that listing alone is evaluated
+                // (its other listings here name the same triangle, so the
+                // deepest contact and its triangle are the same)
+                const int sx = hi.x - lo.x + 1, sy = hi.y - lo.y + 1;
+                const bool own_listing = ((oz - lo.z) * sy + (oy - lo.y)) * sx + (ox - lo.x) == en.x % kDynMaxHashCells;
+                if (ox >= lo.x && ox <= hi.x && oy >= lo.y && oy <= hi.y && oz >= lo.z && oz <= hi.z && own_listing) {
                     F3 d;
                     const unsigned rej0 = rej;
                     const float pen = dyn_triangle_depth_with(
@@ -637,13 +642,12 @@ __device__ inline unsigned long long dynamic_triangle_pool(const ClothView& c, F
                             c = f3(rc.c);
                         },
                         op, cm, cand, rej, d);
-                    if (count && classes > 1) {
-                        // counters of one table over the whole scene (the
-                        // contact is the same: every listing names the same
-                        // triangle)
-                        const unsigned m = static_cast<unsigned>(
-                            one_table_listings(lo, hi, ox, oy, oz, D.tri_buckets, classes, D.tri_class_base[cls], D.tri_class_base[cls + 1],
-                                               en.x % kDynMaxHashCells));
+                    if (count) {
+                        // counters of one table over the whole scene, which
+                        // lists the triangle once for each of its span cells
+                        // hashed to the query's bucket (the contact is the
+                        // same: every listing names the same triangle)
+                        const unsigned m = static_cast<unsigned>(one_table_listings(lo, hi, ox, oy, oz, D.tri_buckets));
                         cand += m - 1u;
                         if (rej != rej0) rej += m - 1u;
                     }
