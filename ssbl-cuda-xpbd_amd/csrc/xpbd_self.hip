@@ -153,8 +153,9 @@ __device__ __forceinline__ int bucket_rank(const int* __restrict__ raw, int base
 // A41:4550-4575: vertex cell and bucket.  phase 0 counts, 1 scatters (the
 // rank pass then keeps the first 32 of a bucket; A41:4570-4574 drops the
 // others).
-__device__ __forceinline__ void vertex_hash_item(const ClothView& c, const SelfView& sv, int parity, int i, int phase) {
-    if (i >= sv.nv) return;
+__device__ __forceinline__ void vertex_hash_item(const ClothView& c, const SelfView& sv, int parity, int at, int phase) {
+    if (at >= sv.nv) return;
+    const int i = sv.vorder ? sv.vorder[at] : at;
     if (phase == 0) {
         const float4 p = cur_pos(c, parity)[i];
         const int cx = cell_coord(p.x, sv.cell), cy = cell_coord(p.y, sv.cell), cz = cell_coord(p.z, sv.cell);
@@ -238,9 +239,12 @@ constexpr int kSelfAggProbes = 32;
 __device__ __forceinline__ void prim_hash_block(const ClothView& c, const SelfView& sv, int parity, int kind, int blk, int phase,
                                                 int count_diag) {
     const int t = blk * 256 + static_cast<int>(threadIdx.x);
-    const int k = t / kPrimLanes;
     const int lane = t % kPrimLanes;
     const int n = kind == 0 ? sv.nt : sv.ne;
+    // the block's primitives in the visiting order
+    const int* order = kind == 0 ? sv.torder : sv.eorder;
+    const int at = t / kPrimLanes;
+    const int k = (order && at < n) ? order[at] : at;
     int b0[3] = {0, 0, 0}, b1[3] = {-1, -1, -1};
     bool guarded = false;
     bool ok = false;
@@ -1221,7 +1225,8 @@ void* SelfCollision::alloc_bytes(size_t bytes) {
 
 // A41:6784-6994 sizing: cell = max(1.8 r, 1 mm); buckets = next_pow2(max(1024, 4n)).
 bool SelfCollision::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, const std::vector<int>& tris, int nv,
-                            unsigned long long* counts) {
+                            unsigned long long* counts, const std::vector<int>& vorder, const std::vector<int>& torder,
+                            const std::vector<int>& eorder) {
     ready_ = false;
     const float req = std::max(cfg.self_collision_distance, 0.0f);
     const float r = req > 0.0f ? req : std::max(std::max(cfg.cloth_thickness, cfg.collision_margin), 0.0f);
@@ -1304,6 +1309,18 @@ bool SelfCollision::prepare(const SsblXpbdConfig& cfg, const std::vector<int>& e
     if (s.ne > 0 && !hip_ok(hipMemcpy(d_edges, he.data(), sizeof(int2) * s.ne, hipMemcpyHostToDevice), "self edge upload")) return false;
     s.tris = d_tris;
     s.edges = d_edges;
+    // visiting orders (a missing or mis-sized one: mesh order)
+    auto upload_order = [&](const std::vector<int>& order, int n, const int** out) {
+        *out = nullptr;
+        if (n <= 0 || order.size() != static_cast<size_t>(n)) return true;
+        int* d = static_cast<int*>(alloc_bytes(sizeof(int) * static_cast<size_t>(n)));
+        if (!d || !hip_ok(hipMemcpy(d, order.data(), sizeof(int) * static_cast<size_t>(n), hipMemcpyHostToDevice), "self order upload"))
+            return false;
+        *out = d;
+        return true;
+    };
+    if (!upload_order(vorder, nv, &s.vorder) || !upload_order(torder, s.nt, &s.torder) || !upload_order(eorder, s.ne, &s.eorder))
+        return false;
     if (!hip_ok(hipMemset(s.acc, 0, sizeof(unsigned long long) * 3 * static_cast<size_t>(nv)), "self reset") ||
         !hip_ok(hipMemset(s.accw, 0, sizeof(int) * nv), "self reset") ||
         !hip_ok(hipMemset(s.nb_count, 0, sizeof(unsigned) * nv), "self reset") ||

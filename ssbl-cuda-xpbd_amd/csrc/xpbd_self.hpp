@@ -63,6 +63,12 @@ struct SelfView {
     int* accw;                   // V
     const int4* tris;
     const int2* edges;           // mesh order, endpoints clamped >= 0
+    // The order in which a hash build visits the vertices, triangles and
+    // edges (permutations, by Morton code of their rest positions, so that a
+    // block's items share buckets); the items and their keys are unchanged.
+    const int* vorder;
+    const int* torder;
+    const int* eorder;
     int* scan_tmp;               // block sums for the bucket scans
     unsigned long long* counts;  // the cloth's diagnostic counters
 };
@@ -81,7 +87,8 @@ public:
     SelfCollision& operator=(const SelfCollision&) = delete;
     ~SelfCollision() override;
     bool prepare(const SsblXpbdConfig& cfg, const std::vector<int>& edges, const std::vector<int>& tris, int nv,
-                 unsigned long long* counts);
+                 unsigned long long* counts, const std::vector<int>& vorder, const std::vector<int>& torder,
+                 const std::vector<int>& eorder);
     // The driver for a lone cloth: `view` is the cloth's host view mirror and
     // `dview` its device copy.
     SelfBatch* single_batch(const ClothView& view, const ClothView* dview);

@@ -921,7 +921,22 @@ bool Solver::create(const SsblXpbdConfig* config, const SsblXpbdMesh* mesh) {
     if (!hip_ok(hipMemset(tile_work.p, 0, sizeof(int) * tile_work.n), "tile reset")) return false;
     if (n > 0 && !hip_ok(hipMemset(dyn_best.p, 0, sizeof(unsigned long long) * dyn_best.n), "dynamic contact reset")) return false;
     if (!hip_ok(hipMemset(max_smoothed.p, 0, sizeof(float)), "diagnostic reset")) return false;
-    if (cfg.self_collision && !self.prepare(cfg, edges, tri, n, counts.p)) return false;
+    if (cfg.self_collision) {
+        // the self-collision hashes visit vertices, triangles and edges in
+        // Morton order of their rest positions (a vertex as a degenerate
+        // triangle, an edge as the triangle of its ends)
+        std::vector<int> as_tri(3 * static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) as_tri[3 * i] = as_tri[3 * i + 1] = as_tri[3 * i + 2] = i;
+        const std::vector<int> v_order = spatial_order(h_pos, as_tri, n);
+        const int ne = static_cast<int>(edges.size() / 2);
+        as_tri.assign(3 * static_cast<size_t>(ne), 0);
+        for (int e = 0; e < ne; ++e) {
+            as_tri[3 * e] = edges[2 * e];
+            as_tri[3 * e + 1] = as_tri[3 * e + 2] = edges[2 * e + 1];
+        }
+        const std::vector<int> e_order = spatial_order(h_pos, as_tri, ne);
+        if (!self.prepare(cfg, edges, tri, n, counts.p, v_order, h_tri_order, e_order)) return false;
+    }
     if (cfg.stretch_optimization_enabled && !pcg.prepare(cfg, edges, edge_rest, edge_colors, n, h_adj_off[n], ntiles, ell_width, ell_pitch,
                                                            ell_overflow, counts.p))
         return false;
